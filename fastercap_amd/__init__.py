@@ -1,0 +1,265 @@
+"""MI355X (gfx950) FasterCap BEM solve path.
+
+Host side (mesh builder, include/fastercap_host.h) + device C-ABI
+(include/fastercap_amd.h), wrapped with ctypes.  The hot path (link fill,
+hierarchical product, GMRES) only runs on the GPU: `Device` raises if the HIP
+library is missing or no GPU is visible; there is no CPU fallback.
+
+Python mirror of the reference interfaces:
+  Mesh.build(path, ...)       CAutoRefine ReadFastCapFile/BuildSuperHierarchy/
+                              AutoRefinePanels/links (Solver/Autorefine.cpp)
+  Device.upload/fill/mvp      AutoRefineLinks fill, CMultHier::MultiplyMatByVec_fast
+  Device.solve                CSolveCap::Solve (Solver/SolveCapacitance.cpp:1138-1425)
+  run(argv)                   FasterCapConsole::main + CSolveCap::Run (manual mode)
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _ffi
+from ._ffi import (
+    FCG_PREC_FP64,
+    FCG_PREC_MIXED,
+    FCG_PRECOND_JACOBI,
+    FCG_PRECOND_NONE,
+    HostOptions,
+    HostStats,
+    JobDesc,
+    NodeGeom,
+    SolveOpts,
+    SolveStats,
+)
+
+__all__ = [
+    "Mesh",
+    "Device",
+    "SolveResult",
+    "FasterCapError",
+    "run",
+    "FCG_PREC_FP64",
+    "FCG_PREC_MIXED",
+    "FCG_PRECOND_JACOBI",
+    "FCG_PRECOND_NONE",
+    "TWO_PI_TIMES_E0",
+]
+
+TWO_PI_TIMES_E0 = 5.563250279878e-11  # Solver/Potential.h:42
+
+
+class FasterCapError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"FC error {code}: {msg}")
+        self.code = code
+
+
+def _np(ptr, n, dtype):
+    if n == 0:
+        return np.zeros(0, dtype=dtype)
+    return np.ctypeslib.as_array(ptr, shape=(n,)).view(dtype)
+
+
+class Mesh:
+    """Refined geometry + link structure produced by the host builder."""
+
+    def __init__(self, handle: int):
+        self._h = handle
+        self.lib = _ffi.host_lib()
+        self.desc = JobDesc()
+        rc = self.lib.fch_get_desc(C.c_void_p(handle), C.byref(self.desc))
+        if rc != 0:
+            raise FasterCapError(rc, "fch_get_desc failed")
+        st = HostStats()
+        self.lib.fch_get_stats(C.c_void_p(handle), C.byref(st))
+        self.stats = st
+        d = self.desc
+        self.dim = d.dim
+        self.N = d.n_leaves
+        self.M = d.n_nodes
+        self.L = d.n_links
+        self.G = d.n_groups
+        self.row_ptr = _np(d.row_ptr, self.M + 1, np.int64)
+        self.col = _np(d.col, self.L, np.int32)
+        self.left = _np(d.left, self.M, np.int32)
+        self.right = _np(d.right, self.M, np.int32)
+        self.parent = _np(d.parent, self.M, np.int32)
+        self.leaf_lo = _np(d.leaf_lo, self.M, np.int32)
+        self.leaf_hi = _np(d.leaf_hi, self.M, np.int32)
+        self.leaf_node = _np(d.leaf_node, self.N, np.int32)
+        self.group_is_diel = _np(d.group_is_diel, self.G, np.int32)
+        self.group_leaf_start = _np(d.group_leaf_start, self.G + 1, np.int32)
+        self.group_outperm = _np(d.group_outperm, self.G, np.float64)
+        self.group_inperm = _np(d.group_inperm, self.G, np.float64)
+        self.leaf_outperm = _np(d.leaf_outperm, self.N, np.float64)
+        raw = np.ctypeslib.as_array(C.cast(d.nodes, C.POINTER(C.c_uint8)), shape=(self.M * C.sizeof(NodeGeom),))
+        self.nodes_raw = raw
+        self.group_names = [self.lib.fch_group_name(C.c_void_p(handle), g).decode() for g in range(self.G)]
+
+    @classmethod
+    def build(cls, path: str, mesh_eps: float = 0.1, eps_ratio: float = 1.0, curv_coeff: float = 3.0) -> "Mesh":
+        lib = _ffi.host_lib()
+        o = HostOptions()
+        lib.fch_default_options(C.byref(o))
+        o.mesh_eps, o.eps_ratio, o.curv_coeff = mesh_eps, eps_ratio, curv_coeff
+        h = C.c_void_p()
+        rc = lib.fch_build(str(path).encode(), C.byref(o), C.byref(h))
+        if rc != 0:
+            raise FasterCapError(rc, lib.fch_last_error().decode())
+        return cls(h.value)
+
+    def node_geom(self) -> np.ndarray:
+        dt = np.dtype(
+            [
+                ("v", np.float32, 9),
+                ("c", np.float32, 3),
+                ("flags", np.uint32),
+                ("cond", np.int32),
+                ("maxside", np.float64),
+                ("area", np.float64),
+                ("dn", np.float64, 3),
+            ]
+        )
+        assert dt.itemsize == 96
+        return self.nodes_raw.view(dt)
+
+    @property
+    def conductor_names(self):
+        return [n for n, d in zip(self.group_names, self.group_is_diel) if not d]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self.lib.fch_free(C.c_void_p(h))
+            self._h = None
+
+
+@dataclass
+class SolveResult:
+    cap: np.ndarray  # rows x n_cond, unscaled (2D: multiply by 2*pi*e0 for F/m)
+    iterations: list = field(default_factory=list)
+    residual: float = 0.0
+    t_solve_s: float = 0.0
+    t_mvp_s: float = 0.0
+    n_mvp: int = 0
+    charges: np.ndarray | None = None
+
+
+def _dptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Device:
+    """One fcg context on a HIP device."""
+
+    def __init__(self, device: int = 0):
+        self.lib = _ffi.device_lib()
+        h = C.c_void_p()
+        rc = self.lib.fcg_ctx_create(device, C.byref(h))
+        if rc != 0:
+            raise FasterCapError(rc, f"cannot create a context on HIP device {device}")
+        self._h = h
+
+    def _check(self, rc, what):
+        if rc != 0:
+            msg = self.lib.fcg_last_error(self._h)
+            raise FasterCapError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def upload(self, mesh: Mesh) -> int:
+        j = C.c_int()
+        self._check(self.lib.fcg_job_upload(self._h, C.byref(mesh.desc), C.byref(j)), "fcg_job_upload")
+        return j.value
+
+    def release(self, job: int):
+        self._check(self.lib.fcg_job_release(self._h, job), "fcg_job_release")
+
+    def fill(self, job: int, precision: int = FCG_PREC_FP64):
+        self._check(self.lib.fcg_fill(self._h, job, precision), "fcg_fill")
+
+    def coeffs(self, job: int, mesh: Mesh):
+        coef = np.empty(mesh.L, dtype=np.float64)
+        self_c = np.empty(mesh.N, dtype=np.float64)
+        self._check(self.lib.fcg_download_coeffs(self._h, job, _dptr(coef), _dptr(self_c)), "fcg_download_coeffs")
+        return coef, self_c
+
+    def mvp(self, job: int, q: np.ndarray) -> np.ndarray:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        k = 1 if q.ndim == 1 else q.shape[0]
+        v = np.empty_like(q)
+        self._check(self.lib.fcg_mvp(self._h, job, _dptr(q), _dptr(v), k), "fcg_mvp")
+        return v
+
+    def precond(self, job: int, q: np.ndarray) -> np.ndarray:
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        k = 1 if q.ndim == 1 else q.shape[0]
+        v = np.empty_like(q)
+        self._check(self.lib.fcg_precond_apply(self._h, job, _dptr(q), _dptr(v), k), "fcg_precond_apply")
+        return v
+
+    def solve(
+        self,
+        job: int,
+        mesh: Mesh,
+        tol: float = 0.01,
+        max_iter: int = 1000,
+        precond: int = FCG_PRECOND_JACOBI,
+        precision: int = FCG_PREC_FP64,
+        want_charges: bool = False,
+    ) -> SolveResult:
+        ncond = int((mesh.group_is_diel == 0).sum())
+        rows_max = ncond
+        cap = np.zeros(rows_max * ncond, dtype=np.float64)
+        charges = np.zeros(rows_max * mesh.N, dtype=np.float64) if want_charges else None
+        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=8)
+        st = SolveStats()
+        rows, cols = C.c_int(), C.c_int()
+        self._check(
+            self.lib.fcg_solve(
+                self._h,
+                job,
+                C.byref(o),
+                _dptr(cap),
+                C.byref(rows),
+                C.byref(cols),
+                _dptr(charges) if charges is not None else None,
+                C.byref(st),
+            ),
+            "fcg_solve",
+        )
+        r, c = rows.value, cols.value
+        return SolveResult(
+            cap=cap[: r * c].reshape(r, c),
+            iterations=[st.iterations[i] for i in range(min(r, 64))],
+            residual=st.final_residual,
+            t_solve_s=st.t_solve_s,
+            t_mvp_s=st.t_mvp_s,
+            n_mvp=st.n_mvp,
+            charges=None if charges is None else charges[: r * mesh.N].reshape(r, mesh.N),
+        )
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.fcg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run(argv):
+    """Manual-mode solve with the reference's switches (-m -d -mc -t -pj/-pn);
+    returns (mesh, SolveResult).  FasterCapConsole::ParseCmdLine
+    (FasterCapConsole.cpp:136-357) + CSolveCap::Run (Solver/SolveCapacitance.cpp:86-256)."""
+    from .cli import parse_args
+
+    a = parse_args(argv)
+    mesh = Mesh.build(a.file, mesh_eps=a.mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff)
+    dev = Device(0)
+    job = dev.upload(mesh)
+    dev.fill(job, FCG_PREC_FP64)
+    res = dev.solve(job, mesh, tol=a.tol, precond=a.precond)
+    return mesh, res

@@ -1,0 +1,208 @@
+"""ctypes declarations of the C-ABI structs of include/fastercap_amd.h and
+include/fastercap_host.h, and loaders for the two in-tree libraries."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+
+
+class NodeGeom(C.Structure):
+    _fields_ = [
+        ("v", C.c_float * 9),
+        ("c", C.c_float * 3),
+        ("flags", C.c_uint32),
+        ("cond", C.c_int32),
+        ("maxside", C.c_double),
+        ("area", C.c_double),
+        ("dn", C.c_double * 3),
+    ]
+
+
+assert C.sizeof(NodeGeom) == 96
+
+
+class JobDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("n_leaves", C.c_int32),
+        ("n_nodes", C.c_int32),
+        ("n_groups", C.c_int32),
+        ("n_links", C.c_int64),
+        ("nodes", C.POINTER(NodeGeom)),
+        ("left", C.POINTER(C.c_int32)),
+        ("right", C.POINTER(C.c_int32)),
+        ("parent", C.POINTER(C.c_int32)),
+        ("leaf_lo", C.POINTER(C.c_int32)),
+        ("leaf_hi", C.POINTER(C.c_int32)),
+        ("leaf_node", C.POINTER(C.c_int32)),
+        ("row_ptr", C.POINTER(C.c_int64)),
+        ("col", C.POINTER(C.c_int32)),
+        ("group_is_diel", C.POINTER(C.c_int32)),
+        ("group_leaf_start", C.POINTER(C.c_int32)),
+        ("group_outperm", C.POINTER(C.c_double)),
+        ("group_inperm", C.POINTER(C.c_double)),
+        ("leaf_outperm", C.POINTER(C.c_double)),
+    ]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [
+        ("tol", C.c_double),
+        ("max_iter", C.c_int32),
+        ("precond", C.c_int32),
+        ("precision", C.c_int32),
+        ("max_refine", C.c_int32),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int32 * 64),
+        ("n_rhs", C.c_int32),
+        ("max_iterations", C.c_int32),
+        ("final_residual", C.c_double),
+        ("t_fill_s", C.c_double),
+        ("t_solve_s", C.c_double),
+        ("t_mvp_s", C.c_double),
+        ("n_mvp", C.c_int64),
+    ]
+
+
+class HostOptions(C.Structure):
+    _fields_ = [
+        ("mesh_eps", C.c_double),
+        ("eps_ratio", C.c_double),
+        ("curv_coeff", C.c_double),
+        ("verbose", C.c_int32),
+    ]
+
+
+class HostStats(C.Structure):
+    _fields_ = [
+        ("input_raw_cond_panels", C.c_int64),
+        ("input_raw_diel_panels", C.c_int64),
+        ("input_panels", C.c_int64),
+        ("panels", C.c_int64),
+        ("nodes", C.c_int64),
+        ("links", C.c_int64),
+        ("n_conductors", C.c_int32),
+        ("n_dielectrics", C.c_int32),
+        ("dim", C.c_int32),
+        ("max_level", C.c_int32),
+        ("max_mesh_eps", C.c_double),
+        ("t_read_s", C.c_double),
+        ("t_superh_s", C.c_double),
+        ("t_discretize_s", C.c_double),
+        ("t_links_s", C.c_double),
+    ]
+
+
+# FC_* codes (Solver/SolverGlobal.h:39-49) plus this library's own
+FCG_OK = 0
+FCG_PREC_FP64 = 0
+FCG_PREC_MIXED = 1
+FCG_PRECOND_NONE = 0
+FCG_PRECOND_JACOBI = 1
+FCG_NODE_LEAF = 1
+FCG_NODE_DIEL = 2
+FCG_NODE_SUPER = 128
+
+# every symbol the device header declares
+DEVICE_SYMBOLS = (
+    "fcg_ctx_create",
+    "fcg_ctx_destroy",
+    "fcg_job_upload",
+    "fcg_job_release",
+    "fcg_fill",
+    "fcg_download_coeffs",
+    "fcg_mvp",
+    "fcg_mvp_device",
+    "fcg_precond_apply",
+    "fcg_solve",
+    "fcg_job_conductors",
+    "fcg_last_error",
+)
+HOST_SYMBOLS = (
+    "fch_default_options",
+    "fch_build",
+    "fch_get_desc",
+    "fch_get_stats",
+    "fch_group_name",
+    "fch_last_error",
+    "fch_free",
+)
+
+
+def _lib_path(name: str) -> str:
+    return os.path.join(LIB_DIR, name)
+
+
+_host = None
+_dev = None
+
+
+def host_lib() -> C.CDLL:
+    global _host
+    if _host is None:
+        path = _lib_path("libfastercap_host.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
+        lib = C.CDLL(path)
+        lib.fch_default_options.argtypes = [C.POINTER(HostOptions)]
+        lib.fch_default_options.restype = None
+        lib.fch_build.argtypes = [C.c_char_p, C.POINTER(HostOptions), C.POINTER(C.c_void_p)]
+        lib.fch_build.restype = C.c_int
+        lib.fch_get_desc.argtypes = [C.c_void_p, C.POINTER(JobDesc)]
+        lib.fch_get_desc.restype = C.c_int
+        lib.fch_get_stats.argtypes = [C.c_void_p, C.POINTER(HostStats)]
+        lib.fch_get_stats.restype = C.c_int
+        lib.fch_group_name.argtypes = [C.c_void_p, C.c_int]
+        lib.fch_group_name.restype = C.c_char_p
+        lib.fch_last_error.argtypes = []
+        lib.fch_last_error.restype = C.c_char_p
+        lib.fch_free.argtypes = [C.c_void_p]
+        lib.fch_free.restype = None
+        _host = lib
+    return _host
+
+
+def device_lib() -> C.CDLL:
+    """Load the gfx950 library; raises if it is missing (no CPU fallback exists)."""
+    global _dev
+    if _dev is None:
+        path = _lib_path("libfcg.so")
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built; the HIP path is required (run __graft_entry__.build())")
+        lib = C.CDLL(path)
+        vp = C.c_void_p
+        dp = C.POINTER(C.c_double)
+        lib.fcg_ctx_create.argtypes = [C.c_int, C.POINTER(vp)]
+        lib.fcg_ctx_destroy.argtypes = [vp]
+        lib.fcg_job_upload.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(C.c_int)]
+        lib.fcg_job_release.argtypes = [vp, C.c_int]
+        lib.fcg_fill.argtypes = [vp, C.c_int, C.c_int]
+        lib.fcg_download_coeffs.argtypes = [vp, C.c_int, dp, dp]
+        lib.fcg_mvp.argtypes = [vp, C.c_int, dp, dp, C.c_int]
+        lib.fcg_mvp_device.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        lib.fcg_precond_apply.argtypes = [vp, C.c_int, dp, dp, C.c_int]
+        lib.fcg_solve.argtypes = [
+            vp,
+            C.c_int,
+            C.POINTER(SolveOpts),
+            dp,
+            C.POINTER(C.c_int),
+            C.POINTER(C.c_int),
+            dp,
+            C.POINTER(SolveStats),
+        ]
+        lib.fcg_job_conductors.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+        lib.fcg_last_error.argtypes = [vp]
+        lib.fcg_last_error.restype = C.c_char_p
+        for name in DEVICE_SYMBOLS:
+            if name != "fcg_last_error":
+                getattr(lib, name).restype = C.c_int
+        _dev = lib
+    return _dev

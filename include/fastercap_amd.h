@@ -1,0 +1,166 @@
+/*
+ * fastercap_amd.h - C-ABI of the MI355X (gfx950) BEM solve path of FasterCap.
+ *
+ * The library owns device memory for "jobs": one refined panel hierarchy with its
+ * destination-sorted link list.  A host solver (the mesh builder of this package,
+ * or FasterCap's own CSolveCap) uploads the hierarchy once, then calls the fill,
+ * the hierarchical matrix-vector product and the preconditioned GMRES through the
+ * entry points below.  All sizes and pointers are plain C; no exceptions cross the
+ * boundary; every function returns an FC_* code (0 = success) as FasterCap does
+ * (Solver/SolverGlobal.h:39-49).
+ *
+ * Reference interfaces each entry point replaces are cited per function.
+ */
+#ifndef FASTERCAP_AMD_H
+#define FASTERCAP_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Return codes: identical values to FasterCap's FC_* (Solver/SolverGlobal.h:39-49). */
+#define FCG_OK 0
+#define FCG_GENERIC_ERROR 1
+#define FCG_COMMAND_LINE_ERROR 64
+#define FCG_CANNOT_OPEN_FILE 66
+#define FCG_OUT_OF_MEMORY 71
+#define FCG_FILE_ERROR 74
+#define FCG_EXCEPTION_ERROR 98
+#define FCG_USER_BREAK 125
+/* Codes of this library that have no FasterCap counterpart. */
+#define FCG_INVALID_ARGUMENT 65
+#define FCG_DEVICE_ERROR 96
+
+/* Node flag bits: same values as CAutoElement::m_ucType (Solver/AutoElement.h:44-48). */
+#define FCG_NODE_LEAF 1u
+#define FCG_NODE_DIEL 2u
+#define FCG_NODE_OUTPERM_NORMAL_DIR 4u
+#define FCG_NODE_OUTPERM_ELEMENT_LEVEL 8u
+#define FCG_NODE_SUPER 128u
+
+/* Precision modes of the link coefficients and of the product. */
+#define FCG_PREC_FP64 0  /* every coefficient and accumulation in FP64 */
+#define FCG_PREC_MIXED 1 /* FP32 far-field coefficients/products, FP64 near field + refinement */
+
+/* Preconditioner bit flags: values of AUTOREFINE_PRECOND_* (Solver/SolverGlobal.h:59-68). */
+#define FCG_PRECOND_NONE 0
+#define FCG_PRECOND_JACOBI 1
+
+/*
+ * Geometry of one tree node (leaf panel, refined parent or super node), 96 bytes.
+ * 3D: v = triangle vertices (x,y,z) x 3; 2D: v[0..2] and v[3..5] are the segment
+ * end points (z = 0).  Values are the reference's stored float geometry
+ * (CAutoPanel::m_clsVertex / m_clsCentroid, Solver/AutoPanel.h:196-199).
+ */
+typedef struct fcg_node_geom {
+    float v[9];       /* vertices */
+    float c[3];       /* centroid */
+    uint32_t flags;   /* FCG_NODE_* */
+    int32_t cond;     /* index of the owning conductor / dielectric group */
+    double maxside;   /* GetMaxSideLen() (3D) or GetLength() (2D) */
+    double area;      /* m_dDimension: area (3D) or length (2D) */
+    double dn[3];     /* GetDielNormal(): unit normal towards the outer permittivity */
+} fcg_node_geom;
+
+/*
+ * One job = one refined geometry.  Nodes are numbered in the post-order of the
+ * reference's RecurseIndex (Solver/Autorefine.cpp:1173-1220); leaves are numbered
+ * in left-first DFS order across the groups, so every node owns the contiguous
+ * leaf range [leaf_lo, leaf_hi).  Row r of the link CSR holds the links of node r
+ * in the order the reference's RefineMutual recursion recorded them; col holds
+ * the source node ids.
+ */
+typedef struct fcg_job_desc {
+    int32_t dim;               /* 3 or 2 */
+    int32_t n_leaves;          /* N */
+    int32_t n_nodes;           /* M */
+    int32_t n_groups;          /* conductors + dielectric interfaces, deque order */
+    int64_t n_links;           /* L (mutual links, self terms excluded) */
+    const fcg_node_geom* nodes; /* [M] */
+    const int32_t* left;       /* [M] left child node id, -1 for leaves */
+    const int32_t* right;      /* [M] right child node id, -1 for leaves */
+    const int32_t* parent;     /* [M] parent node id, -1 for group roots */
+    const int32_t* leaf_lo;    /* [M] */
+    const int32_t* leaf_hi;    /* [M] */
+    const int32_t* leaf_node;  /* [N] node id of leaf k */
+    const int64_t* row_ptr;    /* [M+1] */
+    const int32_t* col;        /* [L] */
+    const int32_t* group_is_diel;     /* [n_groups] */
+    const int32_t* group_leaf_start;  /* [n_groups+1] leaf ranges of the groups */
+    const double* group_outperm;      /* [n_groups] outer permittivity (real part) */
+    const double* group_inperm;       /* [n_groups] inner permittivity (dielectrics) */
+    const double* leaf_outperm;       /* [N] permittivity multiplying the leaf charge in C */
+} fcg_job_desc;
+
+typedef struct fcg_solve_opts {
+    double tol;        /* GMRES relative tolerance (-t), reference stop rule */
+    int32_t max_iter;  /* cap on Arnoldi steps (reference: 1000) */
+    int32_t precond;   /* FCG_PRECOND_* */
+    int32_t precision; /* FCG_PREC_FP64 or FCG_PREC_MIXED */
+    int32_t max_refine;/* outer FP64 refinement sweeps in FCG_PREC_MIXED */
+} fcg_solve_opts;
+
+typedef struct fcg_solve_stats {
+    int32_t iterations[64]; /* GMRES iterations per right-hand side (first 64) */
+    int32_t n_rhs;
+    int32_t max_iterations;
+    double final_residual;  /* largest relative residual estimate at exit */
+    double t_fill_s, t_solve_s, t_mvp_s;
+    int64_t n_mvp;          /* number of hierarchical products executed */
+} fcg_solve_stats;
+
+typedef struct fcg_ctx fcg_ctx;
+
+/* Create a context on HIP device `device`. No reference counterpart (the reference
+ * state lives in CSolveCap / CMultHier objects, Solver/SolveCapacitance.h:53-191). */
+int fcg_ctx_create(int device, fcg_ctx** out);
+int fcg_ctx_destroy(fcg_ctx* ctx);
+
+/* Upload a job's hierarchy and link structure into HBM.
+ * Replaces the in-memory link chunk allocation of CAutoRefine::AutoRefineLinks
+ * (Solver/Autorefine.cpp:481-706) and the tree the MVP walks. */
+int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* desc, int* job);
+int fcg_job_release(fcg_ctx* ctx, int job);
+
+/* Compute the self coefficients and every link coefficient on the device.
+ * Replaces the fill loop of CAutoRefine::AutoRefineLinks (Solver/Autorefine.cpp:760-840)
+ * calling PotEstimateOpt (Solver/Autorefine.cpp:3450-3733, 2D :4156-4325), and the self
+ * terms of RefineSelf/SelfPotential (Solver/Autorefine.cpp:2115-2229, 2358-2506). */
+int fcg_fill(fcg_ctx* ctx, int job, int precision);
+
+/* Copy the filled coefficients to host memory (coef [L] and self [N], either may be NULL). */
+int fcg_download_coeffs(fcg_ctx* ctx, int job, double* coef, double* self_coef);
+
+/* v = P q for k right-hand sides, host buffers laid out vector after vector ([k][N]).
+ * Replaces CMultHier::MultiplyMatByVec_fast (Solver/MultiplyHierarchical.cpp:285-458). */
+int fcg_mvp(fcg_ctx* ctx, int job, const double* q, double* v, int k);
+
+/* Same product on device-resident buffers, enqueued on `stream` (a hipStream_t, NULL =
+ * the context stream).  No host synchronisation. */
+int fcg_mvp_device(fcg_ctx* ctx, int job, const double* d_q, double* d_v, int k, void* stream);
+
+/* Pq = P^-1 q (Jacobi) for k vectors, host buffers.
+ * Replaces CSolveCap::ComputePrecondVectFast (Solver/SolveCapacitance.cpp:3458-3526). */
+int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k);
+
+/* Solve every conductor right-hand side of the job with right-preconditioned GMRES
+ * and assemble the Maxwell capacitance matrix cap[rows][n_cond] (row-major; rows =
+ * n_cond in 3D, n_cond-1 in 2D; 2D values unscaled, as the reference keeps them
+ * before OutputCapMtx).  Replaces CSolveCap::Solve + gmresPrecondSFastAll
+ * (Solver/SolveCapacitance.cpp:1138-1425, 2288-2574). */
+int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, int* rows, int* cols,
+              double* charges /* [rows][N] or NULL */, fcg_solve_stats* stats);
+
+/* Number of conductors (non-dielectric groups) of a job. */
+int fcg_job_conductors(fcg_ctx* ctx, int job, int* n_cond);
+
+/* Last error message of the context (static storage, valid until the next call). */
+const char* fcg_last_error(fcg_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTERCAP_AMD_H */

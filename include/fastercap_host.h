@@ -1,0 +1,64 @@
+/*
+ * fastercap_host.h - host side of the MI355X FasterCap solve path.
+ *
+ * Reads a FastCap-format geometry file, builds the super hierarchy, adaptively
+ * discretizes the panels and discovers the hierarchical links, exactly as the
+ * reference's CAutoRefine does before its link-fill loop, then flattens the result
+ * into the plain arrays of fcg_job_desc (include/fastercap_amd.h).  This is the
+ * caller of the device C-ABI; it performs no coefficient fill and no solve.
+ *
+ * Reference interfaces mirrored:
+ *   fch_build   ReadFastCapFile + BuildSuperHierarchy + AutoRefinePanels + the link
+ *               record pass of AutoRefineLinks (Solver/Autorefine.cpp:4568-4674,
+ *               6410-6544, 293-479, 481-756; RecurseIndex :1173-1220)
+ *   fch_options CAutoRefGlobalVars mesh fields and defaults (Solver/SolverGlobal.cpp:238-279)
+ */
+#ifndef FASTERCAP_HOST_H
+#define FASTERCAP_HOST_H
+
+#include "fastercap_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fch_options {
+    double mesh_eps;   /* -m  m_dMeshEps (default 0.1) */
+    double eps_ratio;  /* -d  m_dEpsRatio (default 1.0); link eps = eps_ratio * mesh_eps */
+    double curv_coeff; /* -mc m_dMeshCurvCoeff (default 3.0, clamped to >= 1) */
+    int32_t verbose;   /* -v */
+} fch_options;
+
+typedef struct fch_stats {
+    int64_t input_raw_cond_panels, input_raw_diel_panels, input_panels;
+    int64_t panels, nodes, links;     /* N, M, L (links exclude self terms) */
+    int32_t n_conductors, n_dielectrics, dim;
+    int32_t max_level;                /* deepest recursion level reached */
+    double max_mesh_eps;              /* Max Mesh relative refinement value */
+    double t_read_s, t_superh_s, t_discretize_s, t_links_s;
+} fch_stats;
+
+typedef struct fch_mesh fch_mesh;
+
+void fch_default_options(fch_options* opts);
+
+/* Build a refined mesh with its link structure. Returns an FC_* code. */
+int fch_build(const char* path, const fch_options* opts, fch_mesh** out);
+
+/* Point desc at the mesh's arrays (valid while the mesh lives). */
+int fch_get_desc(const fch_mesh* mesh, fcg_job_desc* desc);
+
+int fch_get_stats(const fch_mesh* mesh, fch_stats* stats);
+
+/* Names of the groups in deque order (dielectrics first, as the reference orders them). */
+const char* fch_group_name(const fch_mesh* mesh, int group);
+
+/* Last error text of fch_build (thread-local static storage). */
+const char* fch_last_error(void);
+
+void fch_free(fch_mesh* mesh);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FASTERCAP_HOST_H */

@@ -1,0 +1,525 @@
+// CPU restatement of FasterCap 6.0.7's BEM solve path: the checker of the gfx950
+// path and the reported CPU baseline.  TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// Written independently of the device kernels: plain arrays, scalar loops, the
+// reference's sequential summation orders, OpenMP exactly where the reference has
+// it (fill over nodes, MVP gather over nodes).
+#include "oracle.h"
+
+#include <omp.h>
+
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace {
+
+const double kFourPiE0 = 1.1126500559756e-10;  // Potential.h:43
+const double kTwoE0 = 1.7708375634e-11;        // Potential.h:44
+const double kPi = 3.1415926535897932384626433832795;
+const double kPiHalf = 1.5707963267948966192313216916398;
+const double kGeoEps = 1E-6;    // POTENTIAL_GEO_EPS
+const double kTolEps = 1E-12;   // POTENTIAL_TOL_EPS
+const double kLog4p2 = 3.3862943611198906188344642429164;  // Potential.cpp:39
+
+void vsub(const double* a, const double* b, double* o) {
+    o[0] = a[0] - b[0];
+    o[1] = a[1] - b[1];
+    o[2] = a[2] - b[2];
+}
+double vdot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+void vcross(const double* a, const double* b, double* o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+double vnorm(const double* a) { return std::sqrt(vdot(a, a)); }
+
+// Quadrature tables: InitNumerical rules 2, 3, 5 (Potential.cpp:112-128, 133-149, 181-206).
+struct Rule {
+    int n;
+    double px[6], py[6], w[6];
+};
+Rule make_rule(int id) {
+    Rule r;
+    std::memset(&r, 0, sizeof r);
+    if (id == 2) {
+        r.n = 3;
+        r.px[0] = 4.0 / 6.0; r.py[0] = 1.0 / 6.0;
+        r.px[1] = 1.0 / 6.0; r.py[1] = 4.0 / 6.0;
+        r.px[2] = 1.0 / 6.0; r.py[2] = 1.0 / 6.0;
+        for (int i = 0; i < 3; ++i) r.w[i] = 1.0 / 3.0;
+    } else if (id == 3) {
+        r.n = 3;
+        r.px[0] = 0.0; r.py[0] = 0.5;
+        r.px[1] = 0.5; r.py[1] = 0.0;
+        r.px[2] = 0.5; r.py[2] = 0.5;
+        for (int i = 0; i < 3; ++i) r.w[i] = 1.0 / 3.0;
+    } else {
+        const double a = 0.659027622374092, b = 0.231933368553031, c = 0.109039009072877;
+        const double xs[6] = {a, a, b, b, c, c}, ys[6] = {b, c, a, c, a, b};
+        r.n = 6;
+        for (int i = 0; i < 6; ++i) {
+            r.px[i] = xs[i];
+            r.py[i] = ys[i];
+            r.w[i] = 1.0 / 6.0;
+        }
+    }
+    return r;
+}
+const Rule kRule2 = make_rule(2), kRule3 = make_rule(3), kRule5 = make_rule(5);
+const Rule& rule_of(int id) { return id == 2 ? kRule2 : (id == 3 ? kRule3 : kRule5); }
+
+// Quadrature of a kernel over the triangle in the reference's local frame.
+template <class F>
+double tri_quadrature(const float* v, int rule, F kernel) {
+    double V[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) V[i][k] = v[3 * i + k];
+    double s1[3], s2[3], s3[3], nrm[3], z[3], x[3], y[3];
+    vsub(V[1], V[0], s1);
+    vsub(V[2], V[1], s2);
+    vsub(V[0], V[2], s3);
+    nrm[0] = s1[1] * s2[2] - s1[2] * s2[1];
+    nrm[1] = -s1[0] * s2[2] + s1[2] * s2[0];
+    nrm[2] = s1[0] * s2[1] - s1[1] * s2[0];
+    const double nm = vnorm(nrm);
+    for (int k = 0; k < 3; ++k) z[k] = nrm[k] / nm;
+    const double m1 = vnorm(s1);
+    for (int k = 0; k < 3; ++k) x[k] = s1[k] / m1;
+    vcross(x, z, y);
+    const double d01 = m1, d02 = -vdot(s3, x), d12 = -vdot(s3, y);
+    const double jac = (d02 - d01) * d12 - d12 * d02;
+    const Rule& R = rule_of(rule);
+    double acc = 0;
+    for (int i = 0; i < R.n; ++i) {
+        const double a0 = (d01 - d02) * R.px[i] + d02 * (1.0 - R.py[i]);
+        const double a1 = d12 * (1.0 - R.px[i] - R.py[i]);
+        double p[3];
+        for (int k = 0; k < 3; ++k) p[k] = V[0][k] + x[k] * a0 + y[k] * a1;
+        acc += R.w[i] * kernel(p);
+    }
+    acc *= jac / 2.0;
+    return acc / (nm / 2.0);
+}
+
+struct Job {
+    const fcg_job_desc* d;
+    const fcg_node_geom& node(int i) const { return d->nodes[i]; }
+};
+
+void centroid(const fcg_node_geom& g, double* c) {
+    c[0] = g.c[0];
+    c[1] = g.c[1];
+    c[2] = g.c[2];
+}
+
+}  // namespace
+
+extern "C" {
+
+double orc_potential_opt_3d(const double* r, const float* v) {
+    double V[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) V[i][k] = v[3 * i + k];
+    double side[3][3];
+    for (int i = 0; i < 3; ++i) vsub(V[(i + 1) % 3], V[i], side[i]);
+    double n[3];
+    vcross(side[0], side[1], n);
+    const double nmod = vnorm(n);
+    for (int k = 0; k < 3; ++k) n[k] /= nmod;
+    const double nr = vdot(n, r);
+    double rho[3];
+    for (int k = 0; k < 3; ++k) rho[k] = r[k] - n[k] * nr;
+    double P[3][3], Rv[3][3], Rlen[3];
+    for (int i = 0; i < 3; ++i) {
+        const double nv = vdot(n, V[i]);
+        for (int k = 0; k < 3; ++k) {
+            P[i][k] = (V[i][k] - n[k] * nv) - rho[k];
+            Rv[i][k] = V[i][k] - r[k];
+        }
+        Rlen[i] = vnorm(Rv[i]);
+    }
+    const double d = -vdot(n, Rv[0]);
+    const double ad = std::fabs(d);
+    double total = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        const int j = (i + 1) % 3;
+        const double len = vnorm(side[i]);
+        double l[3], u[3];
+        for (int k = 0; k < 3; ++k) l[k] = side[i][k] / len;
+        vcross(l, n, u);
+        const double p0s = vdot(P[j], u);
+        const double p0 = std::fabs(p0s);
+        const double lp = vdot(P[j], l);
+        const double lm = vdot(P[i], l);
+        const double r0 = std::sqrt(p0 * p0 + d * d);
+        const double lmax = lp > lm ? std::fabs(lp) : std::fabs(lm);
+        if (!(p0 / lmax > kGeoEps)) continue;
+        if (!(std::fabs((Rlen[j] + lp) / lmax) > kGeoEps)) continue;
+        if (!(std::fabs((Rlen[i] + lm) / lmax) > kGeoEps)) continue;
+        const double t1 = std::atan(p0 * lp / (r0 * r0 + ad * Rlen[j]));
+        const double t2 = std::atan(p0 * lm / (r0 * r0 + ad * Rlen[i]));
+        const double lg = std::log((Rlen[j] + lp) / (Rlen[i] + lm));
+        const double s = (p0s > 0) ? 1.0 : ((p0s < 0) ? -1.0 : 0.0);
+        total += s * (p0 * lg - ad * (t1 - t2));
+    }
+    return total * 2.0 / nmod;
+}
+
+double orc_potential_num_3d(const double* r, const float* v, int rule) {
+    return tri_quadrature(v, rule, [&](const double* p) {
+        double dd[3];
+        vsub(p, r, dd);
+        return 1.0 / vnorm(dd);
+    });
+}
+
+double orc_enfield_num_3d(const double* r, const float* v, const double* n, int rule) {
+    return tri_quadrature(v, rule, [&](const double* p) {
+        double dd[3];
+        vsub(r, p, dd);
+        const double rd = vnorm(dd);
+        return vdot(dd, n) / (rd * rd * rd);
+    });
+}
+
+// Local segment frame shared by the 2D routines (Potential.cpp:1866-1899).
+static void seg_frame(const double* r, const float* v, double* a, double* xo, double* yo, double* len, double* ex,
+                      double* ey) {
+    const double sx = (double)v[3] - (double)v[0], sy = (double)v[4] - (double)v[1];
+    *len = std::sqrt(sx * sx + sy * sy);
+    ex[0] = sx / *len;
+    ex[1] = sy / *len;
+    ey[0] = -ex[1];
+    ey[1] = ex[0];
+    const double ox = (double)v[0] + sx / 2.0, oy = (double)v[1] + sy / 2.0;
+    *a = *len / 2.0;
+    const double px = r[0] - ox, py = r[1] - oy;
+    *xo = px * ex[0] + py * ex[1];
+    *yo = px * ey[0] + py * ey[1];
+}
+
+double orc_potential_opt_2d(const double* r, const float* v) {
+    double a, x, y, len, ex[2], ey[2];
+    seg_frame(r, v, &a, &x, &y, &len, ex, ey);
+    const double xp = x + a, xm = x - a;
+    double res;
+    if (std::fabs(y) < kTolEps && (std::fabs(xp) < kTolEps || std::fabs(xm) < kTolEps)) {
+        res = -a * (std::log(a * a) + kLog4p2);
+    } else {
+        const double y2 = y * y;
+        res = -xp * (std::log(y2 + xp * xp) / 2.0 - 1.0) + xm * (std::log(y2 + xm * xm) / 2.0 - 1.0);
+        res += (std::fabs(xp) < kTolEps) ? std::fabs(y) * kPiHalf : y * std::atan2(y, xp);
+        res += (std::fabs(xm) < kTolEps) ? -std::fabs(y) * kPiHalf : -y * std::atan2(y, xm);
+    }
+    return res / len;
+}
+
+double orc_enfield_2d(const double* r, const float* v, const double* n) {
+    double a, x, y, len, ex[2], ey[2];
+    seg_frame(r, v, &a, &x, &y, &len, ex, ey);
+    double nx = n[0] * ex[0] + n[1] * ex[1];
+    double ny = n[0] * ey[0] + n[1] * ey[1];
+    const double nn = std::sqrt(nx * nx + ny * ny);
+    if (nn > 0) {
+        nx /= nn;
+        ny /= nn;
+    }
+    const double xp = x + a, xm = x - a;
+    double res;
+    if (std::fabs(y) < kTolEps && (std::fabs(xp) < kTolEps || std::fabs(xm) < kTolEps)) {
+        res = -a * (std::log(a * a) + kLog4p2);
+    } else {
+        const double y2 = y * y;
+        const double fx = (std::log(y2 + xp * xp) - std::log(y2 + xm * xm)) / 2.0;
+        const double fy = std::atan2(y, xm) - std::atan2(y, xp);
+        res = fx * nx + fy * ny;
+    }
+    return res / len;
+}
+
+double orc_pot_estimate(const fcg_job_desc* d, int i, int j) {
+    const fcg_node_geom& p1 = d->nodes[i];
+    const fcg_node_geom& p2 = d->nodes[j];
+    double c1[3], c2[3], dist[3];
+    centroid(p1, c1);
+    centroid(p2, c2);
+    vsub(c1, c2, dist);
+    const bool diel1 = (p1.flags & FCG_NODE_DIEL) != 0;
+    if (d->dim == 2) {
+        dist[2] = 0;
+        const double rd = std::sqrt(dist[0] * dist[0] + dist[1] * dist[1]);
+        const double rmax = p1.maxside > p2.maxside ? p1.maxside : p2.maxside;
+        if (!diel1) {
+            if (rd / rmax < 4.0) return orc_potential_opt_2d(c1, p2.v);
+            return -std::log(rd);
+        }
+        if (rd / rmax < 4.0) return orc_enfield_2d(c1, p2.v, p1.dn);
+        return (dist[0] * p1.dn[0] + dist[1] * p1.dn[1]) / (rd * rd);
+    }
+    const double rd = vnorm(dist);
+    const double rmax = p1.maxside > p2.maxside ? p1.maxside : p2.maxside;
+    const double ratio = rd / rmax;
+    if (!diel1) {
+        if (ratio < 1.0) return orc_potential_opt_3d(c1, p2.v) / kFourPiE0;
+        if (ratio < 2.0) return orc_potential_num_3d(c1, p2.v, 5) / kFourPiE0;
+        if (ratio < 4.0) return orc_potential_num_3d(c1, p2.v, 3) / kFourPiE0;
+        if (ratio < 10.0) return orc_potential_num_3d(c1, p2.v, 2) / kFourPiE0;
+        return 1 / (kFourPiE0 * rd);
+    }
+    const double* n1 = p1.dn;
+    if (ratio < 1.0) {
+        const double h = rd / 20;
+        double pp[3], pm[3];
+        for (int k = 0; k < 3; ++k) {
+            pp[k] = c1[k] + n1[k] * h;
+            pm[k] = c1[k] - n1[k] * h;
+        }
+        const double vp = orc_potential_opt_3d(pp, p2.v);
+        const double vm = orc_potential_opt_3d(pm, p2.v);
+        return (vm - vp) / (2 * h * kFourPiE0);
+    }
+    if (ratio < 2.0) return orc_enfield_num_3d(c1, p2.v, n1, 5) / kFourPiE0;
+    if (ratio < 4.0) return orc_enfield_num_3d(c1, p2.v, n1, 3) / kFourPiE0;
+    if (ratio < 10.0) return orc_enfield_num_3d(c1, p2.v, n1, 2) / kFourPiE0;
+    return vdot(dist, n1) / (kFourPiE0 * rd * rd * rd);
+}
+
+double orc_self(const fcg_job_desc* d, int leaf) {
+    const int node = d->leaf_node[leaf];
+    const fcg_node_geom& g = d->nodes[node];
+    if (g.flags & FCG_NODE_DIEL) {
+        const double eb = d->group_outperm[g.cond], ea = d->group_inperm[g.cond];
+        if (d->dim == 3) return (ea + eb) / (kTwoE0 * (eb - ea) * g.area);
+        return kPi * (ea + eb) / ((eb - ea) * g.area);
+    }
+    double c[3];
+    centroid(g, c);
+    if (d->dim == 3) return orc_potential_opt_3d(c, g.v) / kFourPiE0;
+    return orc_potential_opt_2d(c, g.v);
+}
+
+int orc_fill(const fcg_job_desc* d, double* coef, double* self_coef, int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const long M = d->n_nodes;
+#pragma omp parallel for
+    for (long i = 0; i < M; ++i) {
+        for (int64_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; ++k) coef[k] = orc_pot_estimate(d, (int)i, d->col[k]);
+    }
+    if (self_coef)
+        for (int l = 0; l < d->n_leaves; ++l) self_coef[l] = orc_self(d, l);
+    return 0;
+}
+
+int orc_mvp(const fcg_job_desc* d, const double* coef, const double* self_coef, const double* q, double* v,
+            int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+    const int M = d->n_nodes, N = d->n_leaves;
+    std::vector<double> chg(M), pot(M);
+    // up-pass (MultiplyHierarchical.cpp:72-127): post-order numbering means children precede parents
+    for (int i = 0; i < M; ++i) {
+        if (d->left[i] < 0) {
+            const int k = d->leaf_lo[i];
+            chg[i] = q[k];
+            pot[i] = self_coef[k] * chg[i];
+        } else {
+            chg[i] = chg[d->left[i]] + chg[d->right[i]];
+            pot[i] = 0.0;
+        }
+    }
+    // gather (MultiplyHierarchical.cpp:141-226), sequential per node, OpenMP over nodes
+#pragma omp parallel for
+    for (long i = 0; i < M; ++i) {
+        double acc = pot[i];
+        for (int64_t k = d->row_ptr[i]; k < d->row_ptr[i + 1]; ++k) acc += chg[d->col[k]] * coef[k];
+        pot[i] = acc;
+    }
+    // down-pass (MultiplyHierarchical.cpp:231-283): parents after children in post-order,
+    // so walk the numbering backwards (pre-order visits parents first)
+    for (int i = M - 1; i >= 0; --i) {
+        if (d->left[i] >= 0) {
+            pot[d->left[i]] += pot[i];
+            pot[d->right[i]] += pot[i];
+        }
+    }
+    for (int k = 0; k < N; ++k) v[k] = pot[d->leaf_node[k]];
+    if (d->dim == 2) {
+        // zero-total-charge row (MultiplyHierarchical.cpp:305-330)
+        int first_cond = N;
+        for (int g = 0; g < d->n_groups; ++g)
+            if (!d->group_is_diel[g]) {
+                first_cond = d->group_leaf_start[g];
+                break;
+            }
+        const double last = v[N - 1];
+        for (int i = first_cond; i < N - 1; ++i) v[i] -= last;
+        double s = 0.0;
+        for (int i = 0; i < N; ++i) s += q[i];
+        v[N - 1] = 1.0 * s;
+    }
+    return 0;
+}
+
+// gmresPrecondSFastAll (SolveCapacitance.cpp:2288-2574), Jacobi or no preconditioner.
+static int gmres(const fcg_job_desc* d, const double* coef, const double* self_coef, const std::vector<double>& b,
+                 std::vector<double>& x, double tol, int precond, int nthreads) {
+    const long n = d->n_leaves;
+    auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
+        for (long i = 0; i < n; ++i) out[i] = precond ? in[i] / self_coef[i] : in[i];
+    };
+    auto nrm = [&](const std::vector<double>& a) {
+        double s = 0;
+        for (long i = 0; i < n; ++i) s += a[i] * a[i];
+        return std::sqrt(s);
+    };
+    std::vector<double> x0(n, 0.0), r(n), v(n), w(n), pq(n);
+    double normr, normb;
+    if (precond) {
+        prec(b, x0);
+        orc_mvp(d, coef, self_coef, x0.data(), v.data(), nthreads);
+        for (long k = 0; k < n; ++k) r[k] = b[k] - v[k];
+        normr = nrm(r);
+        normb = nrm(b);
+    } else {
+        r = b;
+        normr = nrm(r);
+        normb = normr;
+    }
+    const int maxit = (int)(n < 1000 ? n : 1000);
+    std::vector<std::vector<double>> Q;
+    std::vector<std::vector<double>> H;
+    std::vector<double> c(maxit + 1), s(maxit + 1), g(maxit + 2, 0.0);
+    Q.emplace_back(n);
+    for (long k = 0; k < n; ++k) Q[0][k] = r[k] / normr;
+    g[0] = normr;
+    if (normr / normb < tol) {
+        x = x0;
+        return 0;
+    }
+    int i;
+    double rel = normr / normb;
+    for (i = 0; i < maxit; ++i) {
+        if (precond) {
+            prec(Q[i], pq);
+            orc_mvp(d, coef, self_coef, pq.data(), v.data(), nthreads);
+        } else {
+            orc_mvp(d, coef, self_coef, Q[i].data(), v.data(), nthreads);
+        }
+        w = v;
+        H.emplace_back(i + 2, 0.0);
+        std::vector<double>& h = H[i];
+        for (int j = 0; j <= i; ++j) {
+            double sdot = 0;
+            for (long k = 0; k < n; ++k) sdot += Q[j][k] * w[k];
+            h[j] = sdot;
+            for (long k = 0; k < n; ++k) v[k] = v[k] - h[j] * Q[j][k];
+        }
+        h[i + 1] = nrm(v);
+        Q.emplace_back(n);
+        for (long k = 0; k < n; ++k) Q[i + 1][k] = v[k] / h[i + 1];
+        for (int j = 0; j < i; ++j) {
+            const double t1 = h[j], t2 = h[j + 1];
+            h[j] = c[j] * t1 - s[j] * t2;
+            h[j + 1] = c[j] * t2 + s[j] * t1;
+        }
+        const double t1 = h[i], t2 = h[i + 1];
+        const double len = std::sqrt(t1 * t1 + t2 * t2);
+        c[i] = t1 / len;
+        s[i] = -t2 / len;
+        h[i] = c[i] * t1 - s[i] * t2;
+        h[i + 1] = 0;
+        const double gi = g[i];
+        g[i] = c[i] * gi;
+        g[i + 1] = s[i] * gi;
+        rel = std::fabs(g[i + 1]) / normb;
+        if (rel < tol) break;
+    }
+    int iters = i;
+    if (i >= maxit) iters = i - 1;
+    std::vector<double> z(g.begin(), g.begin() + iters + 1);
+    for (int ii = iters; ii >= 0; --ii) {
+        z[ii] = z[ii] / H[ii][ii];
+        for (int j = ii - 1; j >= 0; --j) z[j] = z[j] - H[ii][j] * z[ii];
+    }
+    std::vector<double> y(n, 0.0);
+    for (long k = 0; k < n; ++k) {
+        double t = 0;
+        for (int j = 0; j <= iters; ++j) t += Q[j][k] * z[j];
+        y[k] = t;
+    }
+    if (precond) {
+        prec(y, x);
+        for (long k = 0; k < n; ++k) x[k] = x0[k] + x[k];
+    } else {
+        x = y;
+    }
+    return iters + 1;
+}
+
+int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
+              double* cap, int* rows, int* cols, int* iters, int nthreads) {
+    const long n = d->n_leaves;
+    int ncond = 0;
+    for (int g = 0; g < d->n_groups; ++g)
+        if (!d->group_is_diel[g]) ++ncond;
+    const int nrow = (d->dim == 2 && ncond > 1) ? ncond - 1 : ncond;
+    *rows = nrow;
+    *cols = ncond;
+    std::vector<double> b(n, 0.0), x(n, 0.0);
+    int row = 0;
+    for (int g = 0; g < d->n_groups; ++g) {
+        if (d->group_is_diel[g]) continue;
+        const int lo = d->group_leaf_start[g], hi = d->group_leaf_start[g + 1];
+        if (d->dim == 2 && hi == n) continue;  // last conductor row is the balance row (SolveCapacitance.cpp:1233-1240)
+        for (int k = lo; k < hi; ++k) b[k] = 1.0;
+        const int it = gmres(d, coef, self_coef, b, x, tol, precond, nthreads);
+        if (iters) iters[row] = it;
+        int col = 0;
+        for (int g2 = 0; g2 < d->n_groups; ++g2) {
+            if (d->group_is_diel[g2]) continue;
+            double sum = 0.0;
+            for (int k = d->group_leaf_start[g2]; k < d->group_leaf_start[g2 + 1]; ++k)
+                sum += d->leaf_outperm[k] * x[k];
+            cap[(size_t)row * ncond + col] = sum;
+            ++col;
+        }
+        for (int k = lo; k < hi; ++k) b[k] = 0.0;
+        ++row;
+    }
+    return 0;
+}
+
+int orc_compressed_dense(const fcg_job_desc* d, const double* coef, const double* self_coef, double* P) {
+    const long N = d->n_leaves;
+    std::memset(P, 0, sizeof(double) * N * N);
+    for (long i = 0; i < N; ++i) P[i * N + i] = self_coef[i];
+    for (int a = 0; a < d->n_nodes; ++a)
+        for (int64_t k = d->row_ptr[a]; k < d->row_ptr[a + 1]; ++k) {
+            const int b = d->col[k];
+            for (int i = d->leaf_lo[a]; i < d->leaf_hi[a]; ++i)
+                for (int j = d->leaf_lo[b]; j < d->leaf_hi[b]; ++j) P[(long)i * N + j] += coef[k];
+        }
+    return 0;
+}
+
+long long orc_link_coverage(const fcg_job_desc* d) {
+    const long N = d->n_leaves;
+    std::vector<unsigned char> cnt((size_t)N * N, 0);
+    for (int a = 0; a < d->n_nodes; ++a)
+        for (int64_t k = d->row_ptr[a]; k < d->row_ptr[a + 1]; ++k) {
+            const int b = d->col[k];
+            for (int i = d->leaf_lo[a]; i < d->leaf_hi[a]; ++i)
+                for (int j = d->leaf_lo[b]; j < d->leaf_hi[b]; ++j) {
+                    unsigned char& c = cnt[(size_t)i * N + j];
+                    if (c < 255) ++c;
+                }
+        }
+    long long bad = 0;
+    for (long i = 0; i < N; ++i)
+        for (long j = 0; j < N; ++j) bad += (cnt[(size_t)i * N + j] != (i == j ? 0 : 1));
+    return bad;
+}
+
+}  // extern "C"

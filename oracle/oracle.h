@@ -1,0 +1,66 @@
+/*
+ * oracle.h - CPU restatement of FasterCap 6.0.7's BEM solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY: this library is the checker for the gfx950 path and the
+ * reported CPU baseline.  Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it; the product never calls it.
+ *
+ * It consumes the flat job arrays of include/fastercap_amd.h (built by the host
+ * mesh builder) and restates, independently of the device code:
+ *   orc_potential_opt_3d   Solver/Potential.cpp:1561-1711
+ *   orc_potential_num_3d   Solver/Potential.cpp:2077-2153 (rules :112-206)
+ *   orc_enfield_num_3d     Solver/Potential.cpp:2172-2254
+ *   orc_potential_opt_2d   Solver/Potential.cpp:1866-1949
+ *   orc_enfield_2d         Solver/Potential.cpp:1967-2060
+ *   orc_pot_estimate       CAutoRefine::PotEstimateOpt one-sided 3D/2D (Solver/Autorefine.cpp:3474-3733, 4156-4325)
+ *   orc_self               CAutoRefine::SelfPotential (Solver/Autorefine.cpp:2358-2506)
+ *   orc_fill               fill loop of AutoRefineLinks (Solver/Autorefine.cpp:760-840), OpenMP over nodes
+ *   orc_mvp                CMultHier::MultiplyMatByVec_fast (Solver/MultiplyHierarchical.cpp:72-330)
+ *   orc_solve              CSolveCap::Solve + gmresPrecondSFastAll + Jacobi
+ *                          (Solver/SolveCapacitance.cpp:1138-1425, 2288-2574, 3489-3526)
+ *   orc_compressed_dense   the compressed matrix column by column (DebugDumpPotMtx pattern,
+ *                          Solver/SolveCapacitance.cpp:3734-3795), built directly from the links
+ *   orc_link_coverage      link-partition invariant: every ordered leaf pair i != j is covered
+ *                          by exactly one link
+ */
+#ifndef FASTERCAP_ORACLE_H
+#define FASTERCAP_ORACLE_H
+
+#include "../include/fastercap_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+double orc_potential_opt_3d(const double* r, const float* v);
+double orc_potential_num_3d(const double* r, const float* v, int rule);
+double orc_enfield_num_3d(const double* r, const float* v, const double* n, int rule);
+double orc_potential_opt_2d(const double* r, const float* v);
+double orc_enfield_2d(const double* r, const float* v, const double* n);
+
+double orc_pot_estimate(const fcg_job_desc* d, int node_i, int node_j);
+double orc_self(const fcg_job_desc* d, int leaf);
+
+/* coef [L], self [N]; nthreads <= 0 uses the OpenMP default. Returns 0. */
+int orc_fill(const fcg_job_desc* d, double* coef, double* self_coef, int nthreads);
+
+/* v = P q for one vector (N), 2D fix-ups included. Returns 0. */
+int orc_mvp(const fcg_job_desc* d, const double* coef, const double* self_coef, const double* q, double* v,
+            int nthreads);
+
+/* C matrix (rows x n_cond, row-major), 3D: rows = n_cond, 2D: n_cond - 1, unscaled.
+ * iters[rows] receives GMRES iteration counts. precond: 0 none, 1 Jacobi. */
+int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
+              double* cap, int* rows, int* cols, int* iters, int nthreads);
+
+/* Dense compressed matrix P[N][N] (row-major) assembled from the links. */
+int orc_compressed_dense(const fcg_job_desc* d, const double* coef, const double* self_coef, double* P);
+
+/* Returns the number of ordered leaf pairs whose link cover count differs from the
+ * expected value (1 for i != j, 0 for i == j); 0 means the links partition the matrix. */
+long long orc_link_coverage(const fcg_job_desc* d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

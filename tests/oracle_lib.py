@@ -1,0 +1,127 @@
+"""ctypes loader of the oracle (oracle/_build/liboracle.so): the CPU restatement
+of FasterCap's solve path used as the checker.  Only tests, smoke() and the
+bench's cpu_baseline leg import this module."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from fastercap_amd._ffi import JobDesc
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
+        L = C.CDLL(LIB)
+        dp = C.POINTER(C.c_double)
+        fp = C.POINTER(C.c_float)
+        jd = C.POINTER(JobDesc)
+        L.orc_potential_opt_3d.argtypes = [dp, fp]
+        L.orc_potential_num_3d.argtypes = [dp, fp, C.c_int]
+        L.orc_enfield_num_3d.argtypes = [dp, fp, dp, C.c_int]
+        L.orc_potential_opt_2d.argtypes = [dp, fp]
+        L.orc_enfield_2d.argtypes = [dp, fp, dp]
+        for f in ("orc_potential_opt_3d", "orc_potential_num_3d", "orc_enfield_num_3d", "orc_potential_opt_2d",
+                  "orc_enfield_2d", "orc_pot_estimate", "orc_self"):
+            getattr(L, f).restype = C.c_double
+        L.orc_pot_estimate.argtypes = [jd, C.c_int, C.c_int]
+        L.orc_self.argtypes = [jd, C.c_int]
+        L.orc_fill.argtypes = [jd, dp, dp, C.c_int]
+        L.orc_mvp.argtypes = [jd, dp, dp, dp, dp, C.c_int]
+        L.orc_solve.argtypes = [jd, dp, dp, C.c_double, C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                C.POINTER(C.c_int), C.c_int]
+        L.orc_compressed_dense.argtypes = [jd, dp, dp, dp]
+        L.orc_link_coverage.argtypes = [jd]
+        L.orc_link_coverage.restype = C.c_longlong
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def potential_opt_3d(r, v9):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    v = np.ascontiguousarray(v9, dtype=np.float32)
+    return lib().orc_potential_opt_3d(_d(r), _f(v))
+
+
+def potential_num_3d(r, v9, rule):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    v = np.ascontiguousarray(v9, dtype=np.float32)
+    return lib().orc_potential_num_3d(_d(r), _f(v), rule)
+
+
+def enfield_num_3d(r, v9, n, rule):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    v = np.ascontiguousarray(v9, dtype=np.float32)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    return lib().orc_enfield_num_3d(_d(r), _f(v), _d(n), rule)
+
+
+def potential_opt_2d(r, v6):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    v = np.ascontiguousarray(v6, dtype=np.float32)
+    return lib().orc_potential_opt_2d(_d(r), _f(v))
+
+
+def enfield_2d(r, v6, n):
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    v = np.ascontiguousarray(v6, dtype=np.float32)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    return lib().orc_enfield_2d(_d(r), _f(v), _d(n))
+
+
+def fill(mesh, nthreads=0):
+    coef = np.empty(mesh.L, dtype=np.float64)
+    selfc = np.empty(mesh.N, dtype=np.float64)
+    lib().orc_fill(C.byref(mesh.desc), _d(coef), _d(selfc), nthreads)
+    return coef, selfc
+
+
+def mvp(mesh, coef, selfc, q, nthreads=0):
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.empty_like(q)
+    lib().orc_mvp(C.byref(mesh.desc), _d(coef), _d(selfc), _d(q), _d(v), nthreads)
+    return v
+
+
+def solve(mesh, coef, selfc, tol=1e-8, precond=1, nthreads=0):
+    ncond = int((mesh.group_is_diel == 0).sum())
+    cap = np.zeros(ncond * ncond, dtype=np.float64)
+    iters = np.zeros(ncond, dtype=np.int32)
+    rows, cols = C.c_int(), C.c_int()
+    lib().orc_solve(C.byref(mesh.desc), _d(coef), _d(selfc), tol, precond, _d(cap), C.byref(rows), C.byref(cols),
+                    iters.ctypes.data_as(C.POINTER(C.c_int)), nthreads)
+    r, c = rows.value, cols.value
+    return cap[: r * c].reshape(r, c), iters[:r]
+
+
+def compressed_dense(mesh, coef, selfc):
+    P = np.empty((mesh.N, mesh.N), dtype=np.float64)
+    lib().orc_compressed_dense(C.byref(mesh.desc), _d(coef), _d(selfc), _d(P))
+    return P
+
+
+def link_coverage_errors(mesh):
+    return int(lib().orc_link_coverage(C.byref(mesh.desc)))
+
+
+def pot_estimate(mesh, i, j):
+    return lib().orc_pot_estimate(C.byref(mesh.desc), i, j)
