@@ -393,6 +393,7 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     DevBuf qptrs;
     HIPCHECK(ctx, qptrs.alloc(sizeof(double*) * (size_t)(cap_iter + 2)));
     std::vector<double*> hptrs;
+    hptrs.reserve((size_t)cap_iter + 2);
     auto push_basis = [&]() -> int {
         Q.emplace_back(new DevBuf());
         HIPCHECK(ctx, Q.back()->alloc(sizeof(double) * NK));
@@ -421,9 +422,10 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     // norms of k vectors: partial dots of the vector with itself
     DevBuf selfptr;
     HIPCHECK(ctx, selfptr.alloc(sizeof(double*)));
+    double* norm_src = nullptr;  // outlives every copy enqueued from it
     auto norms = [&](const double* vec, std::vector<double>& out) -> int {
-        double* p = const_cast<double*>(vec);
-        HIPCHECK(ctx, hipMemcpyAsync(selfptr.p, &p, sizeof(double*), hipMemcpyHostToDevice, s));
+        norm_src = const_cast<double*>(vec);
+        HIPCHECK(ctx, hipMemcpyAsync(selfptr.p, &norm_src, sizeof(double*), hipMemcpyHostToDevice, s));
         HIPCHECK(ctx, fcg::launch_block_dots(selfptr.as<double* const>(), 1, vec, N, k, part.as<double>(), nchunks, s));
         HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, k, nrm.as<double>(), s));
         out.resize((size_t)k);

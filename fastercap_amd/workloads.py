@@ -1,0 +1,147 @@
+"""Seeded FastCap-format geometry generators for the benchmark / test workloads.
+
+Each generator writes plain FastCap input files (the format of
+Solver/Autorefine.cpp:4746-5988) into a directory and returns the path of the
+top-level file, so the host builder reads exactly what the reference would.
+"""
+from __future__ import annotations
+
+import math
+import os
+
+import numpy as np
+
+BASE_SEED = 20261203
+
+# Face vertex order of a unit box, as in the reference's own cube example
+# (hlp/InputFile/3D/InputFile_3D_ch02.htm).
+_UNIT_BOX_FACES = (
+    ((1, 1, 0), (1, 0, 0), (1, 0, 1), (1, 1, 1)),
+    ((0, 1, 0), (1, 1, 0), (1, 1, 1), (0, 1, 1)),
+    ((1, 0, 0), (0, 0, 0), (0, 0, 1), (1, 0, 1)),
+    ((0, 0, 0), (0, 1, 0), (0, 1, 1), (0, 0, 1)),
+    ((0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0)),
+    ((0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)),
+)
+
+
+def _fmt(x: float) -> str:
+    return repr(float(x))
+
+
+def box_quads(name: str, lo, hi, scale: float = 1.0):
+    """Six 'Q' statements of an axis-aligned box [lo, hi]."""
+    lines = []
+    for face in _UNIT_BOX_FACES:
+        pts = []
+        for corner in face:
+            p = [(lo[a] if corner[a] == 0 else hi[a]) * scale for a in range(3)]
+            pts.append(" ".join(_fmt(c) for c in p))
+        lines.append(f"Q {name}  " + "  ".join(pts))
+    return lines
+
+
+def two_plates(outdir: str, gap: float = 0.1, pitch: float | None = None) -> str:
+    """Two 1 m x 1 m zero-thickness plates, one quad each.  Without `pitch` they are
+    stacked at vertical distance `gap`; with `pitch` they are coplanar at that
+    lateral centre pitch."""
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, "plates.txt")
+    if pitch is None:
+        a = "Q A 0 0 0  1 0 0  1 1 0  0 1 0"
+        b = f"Q B 0 0 {gap!r}  1 0 {gap!r}  1 1 {gap!r}  0 1 {gap!r}"
+    else:
+        a = "Q A 0 0 0  1 0 0  1 1 0  0 1 0"
+        b = f"Q B {pitch!r} 0 0  {pitch + 1.0!r} 0 0  {pitch + 1.0!r} 1 0  {pitch!r} 1 0"
+    with open(path, "w") as f:
+        f.write("* two parallel plates (FastCap format)\n" + a + "\n" + b + "\n")
+    return path
+
+
+def crossing_bus(outdir: str, unit: float = 1e-6) -> str:
+    """8 bars of section 1x1 and length 9: 4 along x on z in [0,1] at y = 0,2,4,6,
+    4 along y on z in [2,3] at x = 0,2,4,6; each face one quad."""
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, "bus.txt")
+    lines = ["* 8-conductor crossing bus, unit 1e-6 m"]
+    for i, y in enumerate((0, 2, 4, 6)):
+        lines += box_quads(f"x{i}", (-1, y, 0), (8, y + 1, 1), unit)
+    for i, x in enumerate((0, 2, 4, 6)):
+        lines += box_quads(f"y{i}", (x, -1, 2), (x + 1, 8, 3), unit)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def random_boxes(outdir: str, n: int = 32, seed: int = 0, with_interface: bool = True, span: float = 20.0,
+                 min_gap: float = 0.05) -> str:
+    """`n` axis-aligned boxes, sides log-uniform in [0.2, 2], centres uniform in a
+    span^3 cube above z = 0, overlaps (gap < min_gap) rejected; optionally a planar
+    dielectric interface at z = 0 (eps 3.9 below, 1 above) meshed as one quad."""
+    rng = np.random.default_rng(BASE_SEED + seed)
+    boxes = []
+    tries = 0
+    while len(boxes) < n:
+        tries += 1
+        if tries > 100000:
+            raise RuntimeError("cannot place boxes")
+        side = np.exp(rng.uniform(math.log(0.2), math.log(2.0), size=3))
+        c = rng.uniform(0.0, span, size=3)
+        lo, hi = c - side / 2, c + side / 2
+        if lo[2] < min_gap:
+            continue
+        ok = True
+        for (blo, bhi) in boxes:
+            gap = np.maximum(np.maximum(blo - hi, lo - bhi), 0.0)
+            if np.all(gap < min_gap):
+                ok = False
+                break
+        if ok:
+            boxes.append((lo, hi))
+    os.makedirs(outdir, exist_ok=True)
+    main = os.path.join(outdir, "boxes.lst")
+    lines = ["* random box cluster (FastCap format)"]
+    if with_interface:
+        lines.append("D plane 1.0 3.9 0 0 0 10 10 30")
+    for i, (lo, hi) in enumerate(boxes):
+        lines += box_quads(f"b{i}", lo, hi)
+    if with_interface:
+        lines.append("End")
+        lines.append("File plane")
+        lines.append(f"Q iface {-0.5 * span!r} {-0.5 * span!r} 0  {1.5 * span!r} {-0.5 * span!r} 0  "
+                     f"{1.5 * span!r} {1.5 * span!r} 0  {-0.5 * span!r} {1.5 * span!r} 0")
+        lines.append("End")
+    with open(main, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return main
+
+
+def coplanar_lines_2d(outdir: str, seed: int = 0, height: float = 1.0, ground_t: float = 1.0) -> str:
+    """2D cross-section: 3 rectangular lines (width, spacing, thickness ~ U[0.5,3])
+    at `height` over a ground line of width 30 and thickness `ground_t`, each
+    rectangle as 4 'S' segments."""
+    rng = np.random.default_rng(BASE_SEED + seed)
+    w = rng.uniform(0.5, 3.0, size=3)
+    s = rng.uniform(0.5, 3.0, size=2)
+    t = rng.uniform(0.5, 3.0, size=3)
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, "lines2d.txt")
+    lines = ["0 2D coplanar lines over ground (FastCap 2D format)"]
+
+    def rect(name, x0, y0, x1, y1):
+        return [
+            f"S {name} {x0!r} {y0!r} {x1!r} {y0!r}",
+            f"S {name} {x1!r} {y0!r} {x1!r} {y1!r}",
+            f"S {name} {x1!r} {y1!r} {x0!r} {y1!r}",
+            f"S {name} {x0!r} {y1!r} {x0!r} {y0!r}",
+        ]
+
+    total = w.sum() + s.sum()
+    x = -total / 2
+    for i in range(3):
+        lines += rect(f"l{i}", x, ground_t + height, x + w[i], ground_t + height + t[i])
+        x += w[i] + (s[i] if i < 2 else 0.0)
+    lines += rect("gnd", -15.0, 0.0, 15.0, ground_t)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
