@@ -238,6 +238,11 @@ class Device:
             charges=None if charges is None else charges[: r * mesh.N].reshape(r, mesh.N),
         )
 
+    def mvp_timing(self, job: int, k: int, reps: int = 20) -> dict:
+        t = _ffi.MvpTiming()
+        self._check(self.lib.fcg_mvp_timing_run(self._h, job, k, reps, C.byref(t)), "fcg_mvp_timing_run")
+        return {f: getattr(t, f) for f, _ in _ffi.MvpTiming._fields_}
+
     def close(self):
         if getattr(self, "_h", None):
             self.lib.fcg_ctx_destroy(self._h)

@@ -71,6 +71,18 @@ class SolveStats(C.Structure):
     ]
 
 
+class MvpTiming(C.Structure):
+    _fields_ = [
+        ("ms_scan", C.c_double),
+        ("ms_charge", C.c_double),
+        ("ms_gather", C.c_double),
+        ("ms_down", C.c_double),
+        ("ms_total", C.c_double),
+        ("reps", C.c_int32),
+        ("k", C.c_int32),
+    ]
+
+
 class HostOptions(C.Structure):
     _fields_ = [
         ("mesh_eps", C.c_double),
@@ -124,6 +136,7 @@ DEVICE_SYMBOLS = (
     "fcg_solve",
     "fcg_job_conductors",
     "fcg_last_error",
+    "fcg_mvp_timing_run",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -199,6 +212,7 @@ def device_lib() -> C.CDLL:
             C.POINTER(SolveStats),
         ]
         lib.fcg_job_conductors.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+        lib.fcg_mvp_timing_run.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(MvpTiming)]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p
         for name in DEVICE_SYMBOLS:

@@ -322,6 +322,52 @@ int fcg_mvp(fcg_ctx* ctx, int job, const double* q, double* v, int k) {
     return FCG_OK;
 }
 
+int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* out) {
+    Job* j = get_job(ctx, job);
+    if (!j || !out || k < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_scratch(ctx, *j, k);
+    if (rc != FCG_OK) return rc;
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)j->N * k;
+    DevBuf dq, dv;
+    HIPCHECK(ctx, dq.alloc(sizeof(double) * n));
+    HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
+    HIPCHECK(ctx, fcg::launch_random(dq.as<double>(), n, 12345, s));
+    struct Ev {
+        hipEvent_t e[5] = {};
+        ~Ev() {
+            for (auto x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
+    // one untimed product to warm caches and code objects
+    HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s));
+    double acc[4] = {0, 0, 0, 0}, tot = 0;
+    for (int r = 0; r < reps; ++r) {
+        HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
+        HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
+        for (int st = 0; st < 4; ++st) {
+            float ms = 0;
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[st], ev.e[st + 1]));
+            acc[st] += ms;
+        }
+        float ms = 0;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[0], ev.e[4]));
+        tot += ms;
+    }
+    out->ms_scan = acc[0] / reps;
+    out->ms_charge = acc[1] / reps;
+    out->ms_gather = acc[2] / reps;
+    out->ms_down = acc[3] / reps;
+    out->ms_total = tot / reps;
+    out->reps = reps;
+    out->k = k;
+    return FCG_OK;
+}
+
 int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k) {
     Job* j = get_job(ctx, job);
     if (!j || !q || !pq || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");

@@ -55,8 +55,12 @@ hipError_t launch_self(const JobView& j, hipStream_t s);
 
 // Product v = P q on k vectors ([k][N] layout). If jacobi != 0 the input is
 // first divided by the self coefficients (right preconditioning fused in).
+// `ev`, when given, holds 5 events recorded before the scan and after the scan,
+// the charges, the gather and the down-pass (stage timing).
 hipError_t launch_mvp(const JobView& j, MvpScratch& sc, const double* q, double* v, int k, int jacobi,
-                      hipStream_t s);
+                      hipStream_t s, hipEvent_t* ev = nullptr);
+// Deterministic pseudo-random fill of n doubles in [-1, 1).
+hipError_t launch_random(double* out, size_t n, uint64_t seed, hipStream_t s);
 
 // Elementwise / Krylov helpers on [k][N] blocks.
 hipError_t launch_jacobi(const JobView& j, const double* q, double* pq, int k, hipStream_t s);

@@ -470,15 +470,18 @@ static void gather_dispatch(const JobView& j, const MvpScratch& sc, int k, hipSt
 }
 
 hipError_t launch_mvp(const JobView& j, MvpScratch& sc, const double* q, double* v, int k, int jacobi,
-                      hipStream_t s) {
+                      hipStream_t s, hipEvent_t* ev) {
     const int N = j.N, M = j.M;
     const int nblk = (N + kScanTile - 1) / kScanTile;
     if (k > sc.kmax || nblk > sc.nblk) return hipErrorInvalidValue;
+    if (ev) (void)hipEventRecord(ev[0], s);
     k_scan_reduce<<<dim3(nblk, k), kScanThreads, 0, s>>>(q, j.self, N, jacobi, sc.block_sums, nblk);
     k_scan_top<<<k, 1024, 0, s>>>(sc.block_sums, nblk);
     k_scan_final<<<dim3(nblk, k), kScanThreads, 0, s>>>(q, j.self, N, jacobi, sc.block_sums, nblk, sc.prefix);
+    if (ev) (void)hipEventRecord(ev[1], s);
     k_node_charge<<<blocks_for(M, 256), 256, 0, s>>>(j.left, j.leaf_lo, j.leaf_hi, sc.prefix, q, j.self, N, M, k,
                                                      jacobi, sc.charge);
+    if (ev) (void)hipEventRecord(ev[2], s);
     const double avg = M > 0 ? (double)j.L / M : 0.0;
     if (avg >= 24)
         gather_dispatch<32>(j, sc, k, s);
@@ -486,6 +489,7 @@ hipError_t launch_mvp(const JobView& j, MvpScratch& sc, const double* q, double*
         gather_dispatch<8>(j, sc, k, s);
     else
         gather_dispatch<4>(j, sc, k, s);
+    if (ev) (void)hipEventRecord(ev[3], s);
     for (int r0 = 0; r0 < k; r0 += 8) {
         if (k - r0 <= 4)
             k_down<4><<<blocks_for(N, 256), 256, 0, s>>>(j.leaf_node, j.parent, sc.pot, N, k, r0, v);
@@ -498,6 +502,24 @@ hipError_t launch_mvp(const JobView& j, MvpScratch& sc, const double* q, double*
         k_2d_last<<<blocks_for(k, 64), 64, 0, s>>>(v, sc.prefix, N, k, lt);
         k_2d_fix<<<dim3(blocks_for(N, 256), k), 256, 0, s>>>(v, lt, N, j.first_cond_leaf);
     }
+    if (ev) (void)hipEventRecord(ev[4], s);
+    return hipGetLastError();
+}
+
+__global__ void k_random(double* __restrict__ out, size_t n, uint64_t seed) {
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    // splitmix64
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (t + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z = z ^ (z >> 31);
+    out[t] = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+hipError_t launch_random(double* out, size_t n, uint64_t seed, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    k_random<<<blocks_for(n, 256), 256, 0, s>>>(out, n, seed);
     return hipGetLastError();
 }
 

@@ -154,6 +154,20 @@ int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k)
 int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, int* rows, int* cols,
               double* charges /* [rows][N] or NULL */, fcg_solve_stats* stats);
 
+/* Average per-stage device time of the hierarchical product, measured with HIP
+ * events on the context stream over `reps` products of k pseudo-random device
+ * vectors.  No reference counterpart: plays the role of the reference's phase
+ * timers (Solver/SolveCapacitance.cpp:704-720) for the product alone. */
+typedef struct fcg_mvp_timing {
+    double ms_scan;    /* prefix scan of the input (3 kernels) */
+    double ms_charge;  /* node charges */
+    double ms_gather;  /* link gather (the dominant kernel) */
+    double ms_down;    /* down-pass (+ 2D fix-up) */
+    double ms_total;   /* whole product */
+    int32_t reps, k;
+} fcg_mvp_timing;
+int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* out);
+
 /* Number of conductors (non-dielectric groups) of a job. */
 int fcg_job_conductors(fcg_ctx* ctx, int job, int* n_cond);
 

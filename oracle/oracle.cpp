@@ -460,6 +460,11 @@ static int gmres(const fcg_job_desc* d, const double* coef, const double* self_c
 
 int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
               double* cap, int* rows, int* cols, int* iters, int nthreads) {
+    return orc_solve_rows(d, coef, self_coef, tol, precond, -1, cap, rows, cols, iters, nthreads);
+}
+
+int orc_solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
+                   int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads) {
     const long n = d->n_leaves;
     int ncond = 0;
     for (int g = 0; g < d->n_groups; ++g)
@@ -473,6 +478,7 @@ int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef
         if (d->group_is_diel[g]) continue;
         const int lo = d->group_leaf_start[g], hi = d->group_leaf_start[g + 1];
         if (d->dim == 2 && hi == n) continue;  // last conductor row is the balance row (SolveCapacitance.cpp:1233-1240)
+        if (max_rows >= 0 && row >= max_rows) break;
         for (int k = lo; k < hi; ++k) b[k] = 1.0;
         const int it = gmres(d, coef, self_coef, b, x, tol, precond, nthreads);
         if (iters) iters[row] = it;

@@ -53,6 +53,10 @@ int orc_mvp(const fcg_job_desc* d, const double* coef, const double* self_coef, 
 int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
               double* cap, int* rows, int* cols, int* iters, int nthreads);
 
+/* Same, stopping after the first `max_rows` right-hand sides (bounded CPU baseline samples). */
+int orc_solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
+                   int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads);
+
 /* Dense compressed matrix P[N][N] (row-major) assembled from the links. */
 int orc_compressed_dense(const fcg_job_desc* d, const double* coef, const double* self_coef, double* P);
 

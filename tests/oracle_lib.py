@@ -41,6 +41,8 @@ def lib():
         L.orc_mvp.argtypes = [jd, dp, dp, dp, dp, C.c_int]
         L.orc_solve.argtypes = [jd, dp, dp, C.c_double, C.c_int, dp, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                 C.POINTER(C.c_int), C.c_int]
+        L.orc_solve_rows.argtypes = [jd, dp, dp, C.c_double, C.c_int, C.c_int, dp, C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]
         L.orc_compressed_dense.argtypes = [jd, dp, dp, dp]
         L.orc_link_coverage.argtypes = [jd]
         L.orc_link_coverage.restype = C.c_longlong
@@ -111,6 +113,16 @@ def solve(mesh, coef, selfc, tol=1e-8, precond=1, nthreads=0):
                     iters.ctypes.data_as(C.POINTER(C.c_int)), nthreads)
     r, c = rows.value, cols.value
     return cap[: r * c].reshape(r, c), iters[:r]
+
+
+def solve_rows(mesh, coef, selfc, max_rows, tol=1e-8, precond=1, nthreads=0):
+    ncond = int((mesh.group_is_diel == 0).sum())
+    cap = np.zeros(ncond * ncond, dtype=np.float64)
+    iters = np.zeros(ncond, dtype=np.int32)
+    rows, cols = C.c_int(), C.c_int()
+    lib().orc_solve_rows(C.byref(mesh.desc), _d(coef), _d(selfc), tol, precond, max_rows, _d(cap), C.byref(rows),
+                         C.byref(cols), iters.ctypes.data_as(C.POINTER(C.c_int)), nthreads)
+    return cap.reshape(ncond, ncond)[:max_rows], iters[:max_rows]
 
 
 def compressed_dense(mesh, coef, selfc):
