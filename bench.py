@@ -186,6 +186,7 @@ def main():
             "ms": tm["ms_total"],
             "gbs": mvp_gbs,
             "stage_ms": {s: tm["ms_" + s] for s in ("scan", "charge", "gather", "down")},
+            "gather_variant": tm["variant"],
         },
         "c_rel_err_vs_fp64_converged": {"max_per_entry": c_err_entry, "max_vs_diagonal": c_err_diag},
         "roofline": {

@@ -243,6 +243,16 @@ class Device:
         self._check(self.lib.fcg_mvp_timing_run(self._h, job, k, reps, C.byref(t)), "fcg_mvp_timing_run")
         return {f: getattr(t, f) for f, _ in _ffi.MvpTiming._fields_}
 
+    def set_gather_variant(self, job: int, variant: int) -> int:
+        n = C.c_int()
+        self._check(self.lib.fcg_set_gather_variant(self._h, job, variant, C.byref(n)), "fcg_set_gather_variant")
+        return n.value
+
+    def autotune(self, job: int, k: int, reps: int = 3):
+        v, ms = C.c_int(), C.c_double()
+        self._check(self.lib.fcg_autotune(self._h, job, k, reps, C.byref(v), C.byref(ms)), "fcg_autotune")
+        return v.value, ms.value
+
     def close(self):
         if getattr(self, "_h", None):
             self.lib.fcg_ctx_destroy(self._h)

@@ -80,6 +80,7 @@ class MvpTiming(C.Structure):
         ("ms_total", C.c_double),
         ("reps", C.c_int32),
         ("k", C.c_int32),
+        ("variant", C.c_int32),
     ]
 
 
@@ -137,6 +138,8 @@ DEVICE_SYMBOLS = (
     "fcg_job_conductors",
     "fcg_last_error",
     "fcg_mvp_timing_run",
+    "fcg_autotune",
+    "fcg_set_gather_variant",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -213,6 +216,8 @@ def device_lib() -> C.CDLL:
         ]
         lib.fcg_job_conductors.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         lib.fcg_mvp_timing_run.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(MvpTiming)]
+        lib.fcg_set_gather_variant.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        lib.fcg_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p
         for name in DEVICE_SYMBOLS:

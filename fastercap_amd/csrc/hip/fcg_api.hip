@@ -40,16 +40,30 @@ struct DevBuf {
     }
 };
 
+struct PinnedBuf {
+    void* p = nullptr;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    hipError_t alloc(size_t n) { return hipHostMalloc(&p, n, hipHostMallocDefault); }
+    template <typename T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
 struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
-    DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, gout, gin, coef, self;
+    DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, gout, gin, coef, self, diag;
     DevBuf leaf_w, col_range;
     // host copies of the group table
     std::vector<int32_t> group_is_diel, group_leaf_start;
     std::vector<double> leaf_outperm;
     int first_cond_leaf = 0;
     bool filled = false;
+    int gather_variant = 0;
+    int tuned_k = 0;  // rhs count the gather variant was tuned for (0 = not tuned)
     // product scratch
     DevBuf prefix, block_sums, charge, pot;
     MvpScratch sc{};
@@ -73,7 +87,9 @@ struct Job {
         v.group_inperm = gin.as<double>();
         v.coef = coef.as<double>();
         v.self = self.as<double>();
+        v.diag = diag.as<double>();
         v.first_cond_leaf = first_cond_leaf;
+        v.gather_variant = gather_variant;
         return v;
     }
 };
@@ -225,6 +241,8 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     HIPCHECK(ctx, upload(j->leaf_w, d->leaf_outperm, (size_t)j->N, s));
     HIPCHECK(ctx, j->coef.alloc(sizeof(double) * (size_t)(j->L > 0 ? j->L : 1)));
     HIPCHECK(ctx, j->self.alloc(sizeof(double) * (size_t)j->N));
+    HIPCHECK(ctx, j->diag.alloc(sizeof(double) * (size_t)j->M));
+    HIPCHECK(ctx, hipMemsetAsync(j->diag.p, 0, sizeof(double) * (size_t)j->M, s));
     j->group_is_diel.assign(d->group_is_diel, d->group_is_diel + j->G);
     j->group_leaf_start.assign(d->group_leaf_start, d->group_leaf_start + j->G + 1);
     j->leaf_outperm.assign(d->leaf_outperm, d->leaf_outperm + j->N);
@@ -365,6 +383,65 @@ int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* o
     out->ms_total = tot / reps;
     out->reps = reps;
     out->k = k;
+    out->variant = j->gather_variant;
+    return FCG_OK;
+}
+
+int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants) {
+    Job* j = get_job(ctx, job);
+    if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+    if (n_variants) *n_variants = fcg::gather_variant_count();
+    if (variant < 0) return FCG_OK;  // query only
+    if (variant >= fcg::gather_variant_count()) return fail(ctx, FCG_INVALID_ARGUMENT, "bad gather variant");
+    j->gather_variant = variant;
+    j->tuned_k = -1;  // pinned by the caller: fcg_solve will not re-tune
+    return FCG_OK;
+}
+
+int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* best_ms) {
+    Job* j = get_job(ctx, job);
+    if (!j || k < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    int rc = ensure_scratch(ctx, *j, k);
+    if (rc != FCG_OK) return rc;
+    hipStream_t s = ctx->stream;
+    const size_t n = (size_t)j->N * k;
+    DevBuf dq, dv;
+    HIPCHECK(ctx, dq.alloc(sizeof(double) * n));
+    HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
+    HIPCHECK(ctx, fcg::launch_random(dq.as<double>(), n, 777, s));
+    struct Ev {
+        hipEvent_t e[5] = {};
+        ~Ev() {
+            for (auto x : e)
+                if (x) (void)hipEventDestroy(x);
+        }
+    } ev;
+    for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
+    int best = 0;
+    double best_t = 1e300;
+    for (int var = 0; var < fcg::gather_variant_count(); ++var) {
+        j->gather_variant = var;
+        HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s));
+        double t = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
+            HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
+            float ms = 0;
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
+            t += ms;
+        }
+        t /= reps;
+        if (t < best_t) {
+            best_t = t;
+            best = var;
+        }
+    }
+    j->gather_variant = best;
+    j->tuned_k = k;
+    if (variant) *variant = best;
+    if (best_ms) *best_ms = best_t;
     return FCG_OK;
 }
 
@@ -418,6 +495,8 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     if (cap_iter > N) cap_iter = N;
     int rc = ensure_scratch(ctx, *j, k);
     if (rc != FCG_OK) return rc;
+    if (j->tuned_k != k && j->tuned_k >= 0 && (rc = fcg_autotune(ctx, job, k, 3, nullptr, nullptr)) != FCG_OK)
+        return rc;
     JobView v = j->view();
     const size_t NK = (size_t)N * k;
 
@@ -434,24 +513,32 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
             for (int n = rhs[(size_t)q].first; n < rhs[(size_t)q].second; ++n) hb[(size_t)q * N + n] = 1.0;
         HIPCHECK(ctx, hipMemcpyAsync(b.p, hb.data(), sizeof(double) * NK, hipMemcpyHostToDevice, s));
     }
-    // Krylov basis, grown one vector ([k][N]) at a time
-    std::vector<std::unique_ptr<DevBuf>> Q;
-    DevBuf qptrs;
-    HIPCHECK(ctx, qptrs.alloc(sizeof(double*) * (size_t)(cap_iter + 2)));
+    // Krylov basis: [k][N] vectors allocated in chunks; device table of their addresses
+    constexpr int kChunk = 64;
+    std::vector<std::unique_ptr<DevBuf>> chunks;
     std::vector<double*> hptrs;
-    hptrs.reserve((size_t)cap_iter + 2);
-    auto push_basis = [&]() -> int {
-        Q.emplace_back(new DevBuf());
-        HIPCHECK(ctx, Q.back()->alloc(sizeof(double) * NK));
-        hptrs.push_back(Q.back()->as<double>());
-        HIPCHECK(ctx, hipMemcpyAsync(qptrs.as<double*>() + (hptrs.size() - 1), &hptrs.back(), sizeof(double*),
-                                     hipMemcpyHostToDevice, s));
+    hptrs.reserve((size_t)cap_iter + 2 + kChunk);
+    DevBuf qptrs;
+    HIPCHECK(ctx, qptrs.alloc(sizeof(double*) * (size_t)(cap_iter + 2 + kChunk)));
+    auto ensure_basis = [&](size_t count) -> int {
+        while (hptrs.size() < count) {
+            chunks.emplace_back(new DevBuf());
+            HIPCHECK(ctx, chunks.back()->alloc(sizeof(double) * NK * kChunk));
+            const size_t base = hptrs.size();
+            for (int c = 0; c < kChunk; ++c) hptrs.push_back(chunks.back()->as<double>() + (size_t)c * NK);
+            // new table entries are not read by any enqueued kernel yet
+            HIPCHECK(ctx, hipMemcpy(qptrs.as<double*>() + base, hptrs.data() + base, sizeof(double*) * kChunk,
+                                    hipMemcpyHostToDevice));
+        }
         return FCG_OK;
     };
     const int maxvals = (cap_iter + 2) * k;
+    // h1 | h2 | |w|^2 of one Arnoldi step, contiguous so that one copy brings them back
     HIPCHECK(ctx, part.alloc(sizeof(double) * (size_t)maxvals * nchunks));
-    HIPCHECK(ctx, hbuf.alloc(sizeof(double) * (size_t)maxvals * 2));
+    HIPCHECK(ctx, hbuf.alloc(sizeof(double) * ((size_t)maxvals * 2 + k)));
     HIPCHECK(ctx, nrm.alloc(sizeof(double) * (size_t)k * 2));
+    PinnedBuf hstage;
+    HIPCHECK(ctx, hstage.alloc(sizeof(double) * ((size_t)maxvals * 2 + 2 * k)));
     int64_t n_mvp = 0;
     double t_mvp = 0.0;
     struct Events {
@@ -465,19 +552,29 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     HIPCHECK(ctx, hipEventCreate(&evs.b));
     hipEvent_t ev0 = evs.a, ev1 = evs.b;
 
-    // norms of k vectors: partial dots of the vector with itself
+    // squared norms of the k vectors of `vec` into out (synchronises the stream)
     DevBuf selfptr;
     HIPCHECK(ctx, selfptr.alloc(sizeof(double*)));
-    double* norm_src = nullptr;  // outlives every copy enqueued from it
+    double** norm_src = reinterpret_cast<double**>(hstage.as<double>() + (size_t)maxvals * 2 + k);
     auto norms = [&](const double* vec, std::vector<double>& out) -> int {
-        norm_src = const_cast<double*>(vec);
-        HIPCHECK(ctx, hipMemcpyAsync(selfptr.p, &norm_src, sizeof(double*), hipMemcpyHostToDevice, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        *norm_src = const_cast<double*>(vec);
+        HIPCHECK(ctx, hipMemcpyAsync(selfptr.p, norm_src, sizeof(double*), hipMemcpyHostToDevice, s));
         HIPCHECK(ctx, fcg::launch_block_dots(selfptr.as<double* const>(), 1, vec, N, k, part.as<double>(), nchunks, s));
         HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, k, nrm.as<double>(), s));
-        out.resize((size_t)k);
-        HIPCHECK(ctx, hipMemcpyAsync(out.data(), nrm.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+        HIPCHECK(ctx, hipMemcpyAsync(hstage.p, nrm.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
         HIPCHECK(ctx, hipStreamSynchronize(s));
+        out.assign(hstage.as<double>(), hstage.as<double>() + k);
         for (auto& x : out) x = std::sqrt(x);
+        return FCG_OK;
+    };
+    // divide the k vectors of `in` by the values `d` (host) into `out`
+    auto scale_by = [&](const double* in, const std::vector<double>& d, double* out) -> int {
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        double* st = hstage.as<double>() + (size_t)maxvals * 2;
+        for (int q = 0; q < k; ++q) st[q] = d[(size_t)q];
+        HIPCHECK(ctx, hipMemcpyAsync(nrm.p, st, sizeof(double) * k, hipMemcpyHostToDevice, s));
+        HIPCHECK(ctx, fcg::launch_scale(in, nrm.as<double>(), out, N, k, s));
         return FCG_OK;
     };
 
@@ -495,12 +592,11 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
         if ((rc = norms(r.as<double>(), normr)) != FCG_OK) return rc;
         normb = normr;
     }
-    if ((rc = push_basis()) != FCG_OK) return rc;
-    HIPCHECK(ctx, hipMemcpyAsync(nrm.p, normr.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-    HIPCHECK(ctx, fcg::launch_scale(r.as<double>(), nrm.as<double>(), hptrs[0], N, k, s));
+    if ((rc = ensure_basis(1)) != FCG_OK) return rc;
+    if ((rc = scale_by(r.as<double>(), normr, hptrs[0])) != FCG_OK) return rc;
 
     // per-rhs Givens state
-    std::vector<std::vector<double>> H((size_t)k);     // column-major R: H[r][col*(cap+1)+row]
+    std::vector<std::vector<double>> H((size_t)k);  // column i at H[r][i*ld .. i*ld+i]
     std::vector<std::vector<double>> cs((size_t)k), sn((size_t)k), g((size_t)k);
     std::vector<int> done((size_t)k, 0), iters((size_t)k, -1);
     std::vector<double> rel((size_t)k, 0.0);
@@ -509,64 +605,67 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
         g[(size_t)q].assign((size_t)cap_iter + 2, 0.0);
         g[(size_t)q][0] = normr[(size_t)q];
         rel[(size_t)q] = normr[(size_t)q] / normb[(size_t)q];
-        if (rel[(size_t)q] < opts->tol) {
-            done[(size_t)q] = 1;  // initial guess already good enough
-            iters[(size_t)q] = -1;
-        }
+        if (rel[(size_t)q] < opts->tol) done[(size_t)q] = 1;  // the initial guess is good enough
     }
     auto all_done = [&]() {
         for (int d : done)
             if (!d) return false;
         return true;
     };
-    std::vector<double> h1((size_t)maxvals), h2((size_t)maxvals), hn;
+    std::vector<double> hn((size_t)k);
     int i = 0;
     for (i = 0; i < cap_iter && !all_done(); ++i) {
+        const int nv = i + 1;
+        if ((rc = ensure_basis((size_t)nv + 1)) != FCG_OK) return rc;
+        double* h1 = hbuf.as<double>();
+        double* h2 = h1 + (size_t)nv * k;
+        double* w2 = h2 + (size_t)nv * k;
         // w = A P q_i
         HIPCHECK(ctx, hipEventRecord(ev0, s));
         HIPCHECK(ctx, fcg::launch_mvp(v, j->sc, hptrs[(size_t)i], w.as<double>(), k, jac ? 1 : 0, s));
         HIPCHECK(ctx, hipEventRecord(ev1, s));
         ++n_mvp;
-        const int nv = i + 1;
-        // CGS2: two projection passes
-        for (int pass = 0; pass < 2; ++pass) {
-            HIPCHECK(ctx, fcg::launch_block_dots(qptrs.as<double* const>(), nv, w.as<double>(), N, k,
-                                                 part.as<double>(), nchunks, s));
-            HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, nv * k, hbuf.as<double>(), s));
-            HIPCHECK(ctx, fcg::launch_basis_update(qptrs.as<double* const>(), nv, hbuf.as<double>(), w.as<double>(),
-                                                   N, k, s));
-            std::vector<double>& dst = pass == 0 ? h1 : h2;
-            HIPCHECK(ctx, hipMemcpyAsync(dst.data(), hbuf.p, sizeof(double) * nv * k, hipMemcpyDeviceToHost, s));
-        }
-        if ((rc = norms(w.as<double>(), hn)) != FCG_OK) return rc;  // synchronises the stream
-        if ((rc = push_basis()) != FCG_OK) return rc;
-        HIPCHECK(ctx, hipMemcpyAsync(nrm.p, hn.data(), sizeof(double) * k, hipMemcpyHostToDevice, s));
-        HIPCHECK(ctx, fcg::launch_scale(w.as<double>(), nrm.as<double>(), hptrs[(size_t)i + 1], N, k, s));
+        // classical Gram-Schmidt, two passes; the second pass also returns |w|^2
+        HIPCHECK(ctx, fcg::launch_block_dots(qptrs.as<double* const>(), nv, w.as<double>(), N, k, part.as<double>(),
+                                             nchunks, s));
+        HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, nv * k, h1, s));
+        HIPCHECK(ctx, fcg::launch_basis_update(qptrs.as<double* const>(), nv, h1, w.as<double>(), N, k, s));
+        HIPCHECK(ctx, fcg::launch_block_dots(qptrs.as<double* const>(), nv, w.as<double>(), N, k, part.as<double>(),
+                                             nchunks, s));
+        HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, nv * k, h2, s));
+        HIPCHECK(ctx, fcg::launch_basis_update_norm(qptrs.as<double* const>(), nv, h2, w.as<double>(), N, k,
+                                                    part.as<double>(), nchunks, s));
+        HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, k, w2, s));
+        HIPCHECK(ctx, hipMemcpyAsync(hstage.p, h1, sizeof(double) * ((size_t)2 * nv * k + k), hipMemcpyDeviceToHost,
+                                     s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
         {
             float ms = 0.0f;
             HIPCHECK(ctx, hipEventElapsedTime(&ms, ev0, ev1));
             t_mvp += 1e-3 * ms;
         }
+        const double* hh1 = hstage.as<double>();
+        const double* hh2 = hh1 + (size_t)nv * k;
+        const double* hw2 = hh2 + (size_t)nv * k;
+        for (int q = 0; q < k; ++q) hn[(size_t)q] = std::sqrt(hw2[q]);
         for (int q = 0; q < k; ++q) {
             if (done[(size_t)q]) continue;
             std::vector<double>& Hq = H[(size_t)q];
             if (Hq.size() < (size_t)(i + 1) * ld) Hq.resize((size_t)(i + 1) * ld, 0.0);
-            double* col = Hq.data() + (size_t)i * ld;
-            for (int jj = 0; jj <= i; ++jj) col[jj] = h1[(size_t)jj * k + q] + h2[(size_t)jj * k + q];
-            col[i + 1 < ld ? i + 1 : ld - 1] = 0.0;
-            double hsub = hn[(size_t)q];
+            double* colv = Hq.data() + (size_t)i * ld;
+            for (int jj = 0; jj <= i; ++jj) colv[jj] = hh1[(size_t)jj * k + q] + hh2[(size_t)jj * k + q];
             auto& c = cs[(size_t)q];
             auto& sv = sn[(size_t)q];
             for (int jj = 0; jj < i; ++jj) {
-                const double t1 = col[jj], t2 = col[jj + 1];
-                col[jj] = c[(size_t)jj] * t1 - sv[(size_t)jj] * t2;
-                col[jj + 1] = c[(size_t)jj] * t2 + sv[(size_t)jj] * t1;
+                const double t1 = colv[jj], t2 = colv[jj + 1];
+                colv[jj] = c[(size_t)jj] * t1 - sv[(size_t)jj] * t2;
+                colv[jj + 1] = c[(size_t)jj] * t2 + sv[(size_t)jj] * t1;
             }
-            const double t1 = col[i], t2 = hsub;
+            const double t1 = colv[i], t2 = hn[(size_t)q];
             const double len = std::sqrt(t1 * t1 + t2 * t2);
             c.push_back(t1 / len);
             sv.push_back(-t2 / len);
-            col[i] = c[(size_t)i] * t1 - sv[(size_t)i] * t2;
+            colv[i] = c[(size_t)i] * t1 - sv[(size_t)i] * t2;
             const double gi = g[(size_t)q][(size_t)i];
             g[(size_t)q][(size_t)i] = c[(size_t)i] * gi;
             g[(size_t)q][(size_t)i + 1] = sv[(size_t)i] * gi;
@@ -574,9 +673,11 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
             iters[(size_t)q] = i;
             if (rel[(size_t)q] < opts->tol) done[(size_t)q] = 1;
         }
+        if (all_done()) break;
+        if ((rc = scale_by(w.as<double>(), hn, hptrs[(size_t)i + 1])) != FCG_OK) return rc;
     }
     // back substitution per rhs, z[jj][r] (zero beyond each rhs's own count)
-    const int nvec_total = (int)Q.size();
+    const int nvec_total = (int)hptrs.size();
     std::vector<double> z((size_t)nvec_total * k, 0.0);
     int max_it = 0;
     for (int q = 0; q < k; ++q) {

@@ -33,7 +33,9 @@ struct JobView {
     const double* group_inperm;
     double* coef;    // [L]
     double* self;    // [N]
+    double* diag;    // [M] self coefficient of leaf nodes, 0 elsewhere
     int first_cond_leaf;  // 2D zero-total-charge fix-up start
+    int gather_variant;   // index into the gather configuration table
 };
 
 // Scratch of one hierarchical product on k right-hand sides.
@@ -71,7 +73,10 @@ hipError_t launch_reduce_partials(const double* partial, int nblocks, int nvals,
 // w[r][n] -= sum_jj h[jj][r] * Q_jj[r][n]
 hipError_t launch_basis_update(double* const* basis, int nvec, const double* h, double* w, int N, int k,
                                hipStream_t s);
-// out[r][n] = a[r][n] * scale[r]
+// Same update, plus partial[r][chunk] = sum over the chunk of the updated w^2.
+hipError_t launch_basis_update_norm(double* const* basis, int nvec, const double* h, double* w, int N, int k,
+                                    double* partial, int nchunks, hipStream_t s);
+// out[r][n] = a[r][n] / d[r] (0 where d[r] == 0)
 hipError_t launch_scale(const double* a, const double* scale, double* out, int N, int k, hipStream_t s);
 // out[r][n] = x[r][n] + y[r][n]
 hipError_t launch_add(const double* x, const double* y, double* out, int N, int k, hipStream_t s);
@@ -86,5 +91,6 @@ hipError_t launch_charge_sums(const double* x, const double* leaf_weight, const 
                               int ncol, double* out, hipStream_t s);
 
 int dots_blocks(int N);
+int gather_variant_count();
 
 }  // namespace fcg

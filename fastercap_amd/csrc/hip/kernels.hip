@@ -107,7 +107,7 @@ __global__ void __launch_bounds__(256) k_fill(const fcg_node_geom* __restrict__ 
 
 __global__ void k_self(const fcg_node_geom* __restrict__ nodes, const int32_t* __restrict__ leaf_node,
                        const double* __restrict__ gout, const double* __restrict__ gin, int N, int dim,
-                       double* __restrict__ self) {
+                       double* __restrict__ self, double* __restrict__ diag) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= N) return;
     const fcg_node_geom& g = nodes[leaf_node[k]];
@@ -125,6 +125,7 @@ __global__ void k_self(const fcg_node_geom* __restrict__ nodes, const int32_t* _
         s = potential_opt_2d<double>((double)g.c[0], (double)g.c[1], g.v);
     }
     self[k] = s;
+    diag[leaf_node[k]] = s;  // non-leaf entries stay zero
 }
 
 // ------------------------------------------------------------------ up-pass scan
@@ -240,42 +241,98 @@ __global__ void k_node_charge(const int32_t* __restrict__ left, const int32_t* _
 }
 
 // ------------------------------------------------------------------ gather
-// G lanes cooperate on one destination node; KC right-hand sides per pass.
-template <int G, int KC>
+// One wavefront per destination node.  Lane = (slot, rhs): KR lanes share one link
+// slot and each reads its own right-hand side, so a link's KR source charges are
+// one contiguous [k]-row of the node-major charge array (one or two cache lines
+// per slot instead of one per lane); the 64/KR slots walk the row's links in
+// stride, and a shuffle tree over the slots reduces each right-hand side.
+// Blocks are remapped so that each of the 8 XCDs owns one contiguous range of
+// destination rows: post-order rows are spatially coherent, so the charges one
+// XCD gathers stay resident in its own L2.
+constexpr int kNumXcd = 8;
+constexpr int kGatherWaves = 4;  // rows per 256-thread block
+
+// KR lanes per link slot, VR right-hand sides per lane (KR*VR >= the rhs chunk),
+// SL = 64/KR slots, U links per slot in flight per iteration.
+template <int KR, int VR, int U>
 __global__ void __launch_bounds__(256) k_gather(const int64_t* __restrict__ row_ptr, const int32_t* __restrict__ col,
-                                                const double* __restrict__ coef, const int32_t* __restrict__ left,
-                                                const int32_t* __restrict__ leaf_lo,
-                                                const double* __restrict__ self, const double* __restrict__ charge,
-                                                int M, int k, int r0, double* __restrict__ pot) {
-    const int gid = (int)((blockIdx.x * (size_t)blockDim.x + threadIdx.x) / G);
-    const int lane = threadIdx.x & (G - 1);
-    if (gid >= M) return;
-    const int kc = (k - r0) < KC ? (k - r0) : KC;
-    double acc[KC];
+                                                const double* __restrict__ coef, const double* __restrict__ diag,
+                                                const double* __restrict__ charge, int M, int k, int r0,
+                                                int blocks_per_xcd, double* __restrict__ pot) {
+    constexpr int SL = 64 / KR;
+    const int b = (int)(blockIdx.x % kNumXcd) * blocks_per_xcd + (int)(blockIdx.x / kNumXcd);
+    const int row = b * kGatherWaves + (int)(threadIdx.x >> 6);
+    if (row >= M) return;
+    const int lane = threadIdx.x & 63;
+    const int slot = lane / KR;
+    const int rb = (lane % KR) * VR;  // first rhs (relative to r0) of this lane
+    const int kc = min(k - r0, KR * VR);
+    // every load that does not depend on the link list is issued up front
+    const int64_t beg = row_ptr[row], end = row_ptr[row + 1];
+    const double sf = diag[row];
+    double own[VR];
 #pragma unroll
-    for (int r = 0; r < KC; ++r) acc[r] = 0.0;
-    const int64_t b = row_ptr[gid], e = row_ptr[gid + 1];
-    for (int64_t l = b + lane; l < e; l += G) {
-        const int c = col[l];
-        const double w = coef[l];
-        const double* src = charge + (size_t)c * k + r0;
+    for (int v = 0; v < VR; ++v) own[v] = (slot == 0 && rb + v < kc) ? charge[(size_t)row * k + r0 + rb + v] : 0.0;
+    const double* src = charge + r0 + rb;
+    double acc[VR];
 #pragma unroll
-        for (int r = 0; r < KC; ++r)
-            if (r < kc) acc[r] += w * src[r];
+    for (int v = 0; v < VR; ++v) acc[v] = 0.0;
+    for (int64_t l = beg + slot; l < end; l += U * SL) {
+        int c[U];
+        double w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int64_t ll = l + (int64_t)u * SL;
+            const bool ok = ll < end;
+            c[u] = ok ? col[ll] : 0;
+            w[u] = ok ? coef[ll] : 0.0;
+        }
+        double ch[U][VR];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < VR; ++v) ch[u][v] = (rb + v < kc) ? src[(size_t)c[u] * k + v] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int v = 0; v < VR; ++v) acc[v] += w[u] * ch[u][v];
     }
 #pragma unroll
-    for (int r = 0; r < KC; ++r) {
+    for (int v = 0; v < VR; ++v)
 #pragma unroll
-        for (int off = G / 2; off > 0; off >>= 1) acc[r] += __shfl_xor(acc[r], off, G);
-    }
-    if (lane == 0) {
-        const bool leaf = left[gid] < 0;
-        const double s = leaf ? self[leaf_lo[gid]] : 0.0;
-        const double* own = charge + (size_t)gid * k + r0;
+        for (int off = KR; off < 64; off <<= 1) acc[v] += __shfl_xor(acc[v], off, 64);
+    if (slot == 0) {
 #pragma unroll
-        for (int r = 0; r < KC; ++r)
-            if (r < kc) pot[(size_t)gid * k + r0 + r] = (leaf ? s * own[r] : 0.0) + acc[r];
+        for (int v = 0; v < VR; ++v)
+            if (rb + v < kc) pot[(size_t)row * k + r0 + rb + v] = sf * own[v] + acc[v];
     }
+}
+
+// Gather variants: (KR, VR, U) triples per rhs-chunk width; gather_variant picks one.
+struct GatherCfg {
+    int kr, vr, u;
+};
+static const GatherCfg kGatherCfgs[] = {
+    {8, 1, 2}, {4, 2, 2}, {4, 2, 4}, {8, 1, 4}, {2, 4, 2}, {16, 1, 2}, {1, 8, 2}, {2, 4, 4},
+};
+constexpr int kNumGatherCfgs = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
+
+int gather_variant_count() { return kNumGatherCfgs; }
+
+template <int KR, int VR, int U>
+static void launch_gather_t(const JobView& j, const MvpScratch& sc, int k, int r0, unsigned grid, int per_xcd,
+                            hipStream_t s) {
+    k_gather<KR, VR, U><<<grid, 256, 0, s>>>(j.row_ptr, j.col, j.coef, j.diag, sc.charge, j.M, k, r0, per_xcd,
+                                             sc.pot);
+}
+
+// width = KR*VR rhs per pass
+static int gather_width(int variant, int k) {
+    const GatherCfg& c = kGatherCfgs[variant];
+    int w = c.kr * c.vr;
+    // narrow rhs counts: shrink the lane group so padding lanes stay few
+    while (w / 2 >= k && w > c.vr) w /= 2;
+    return w;
 }
 
 // ------------------------------------------------------------------ down-pass
@@ -370,6 +427,28 @@ __global__ void k_reduce_partials(const double* __restrict__ partial, int nchunk
     if (threadIdx.x == 0) out[v] = s;
 }
 
+// w[r][n] -= sum_jj h[jj][r] Q_jj[r][n] on one chunk of one rhs per block, and the
+// chunk's partial sum of the updated w^2 (grid (nchunks, k)).
+__global__ void __launch_bounds__(kDotThreads) k_basis_update_norm(double* const* __restrict__ basis, int nvec,
+                                                                   const double* __restrict__ h,
+                                                                   double* __restrict__ w, int N, int k,
+                                                                   double* __restrict__ partial, int nchunks) {
+    __shared__ double sh[kDotThreads / 64];
+    const int r = blockIdx.y;
+    const int n0 = blockIdx.x * kDotChunk;
+    const int n1 = min(N, n0 + kDotChunk);
+    double ss = 0.0;
+    for (int n = n0 + threadIdx.x; n < n1; n += kDotThreads) {
+        const size_t t = (size_t)r * N + n;
+        double acc = w[t];
+        for (int jj = 0; jj < nvec; ++jj) acc -= h[(size_t)jj * k + r] * basis[jj][t];
+        w[t] = acc;
+        ss += acc * acc;
+    }
+    ss = block_sum(ss, sh);
+    if (threadIdx.x == 0) partial[(size_t)r * nchunks + blockIdx.x] = ss;
+}
+
 // h layout [nvec][k]
 __global__ void k_basis_update(double* const* __restrict__ basis, int nvec, const double* __restrict__ h,
                                double* __restrict__ w, int N, int k) {
@@ -445,28 +524,36 @@ hipError_t launch_fill(const JobView& j, const int32_t* link_row, hipStream_t s)
 
 hipError_t launch_self(const JobView& j, hipStream_t s) {
     k_self<<<blocks_for(j.N, 256), 256, 0, s>>>(j.nodes, j.leaf_node, j.group_outperm, j.group_inperm, j.N, j.dim,
-                                              j.self);
+                                              j.self, j.diag);
     return hipGetLastError();
 }
 
-template <int G>
-static void gather_dispatch(const JobView& j, const MvpScratch& sc, int k, hipStream_t s) {
-    const unsigned grid = blocks_for((size_t)j.M * G, 256);
-    for (int r0 = 0; r0 < k; r0 += 8) {
-        const int kc = k - r0;
-        if (kc == 1)
-            k_gather<G, 1><<<grid, 256, 0, s>>>(j.row_ptr, j.col, j.coef, j.left, j.leaf_lo, j.self, sc.charge,
-                                                j.M, k, r0, sc.pot);
-        else if (kc == 2)
-            k_gather<G, 2><<<grid, 256, 0, s>>>(j.row_ptr, j.col, j.coef, j.left, j.leaf_lo, j.self, sc.charge,
-                                                j.M, k, r0, sc.pot);
-        else if (kc <= 4)
-            k_gather<G, 4><<<grid, 256, 0, s>>>(j.row_ptr, j.col, j.coef, j.left, j.leaf_lo, j.self, sc.charge,
-                                                j.M, k, r0, sc.pot);
-        else
-            k_gather<G, 8><<<grid, 256, 0, s>>>(j.row_ptr, j.col, j.coef, j.left, j.leaf_lo, j.self, sc.charge,
-                                                j.M, k, r0, sc.pot);
+static hipError_t gather_dispatch(const JobView& j, const MvpScratch& sc, int k, hipStream_t s) {
+    const int rows_blocks = (j.M + kGatherWaves - 1) / kGatherWaves;
+    const int per_xcd = (rows_blocks + kNumXcd - 1) / kNumXcd;
+    const unsigned grid = (unsigned)(per_xcd * kNumXcd);
+    const int variant = (j.gather_variant >= 0 && j.gather_variant < kNumGatherCfgs) ? j.gather_variant : 0;
+    const GatherCfg& c = kGatherCfgs[variant];
+    const int width = gather_width(variant, k);
+    const int kr = width / c.vr;
+    for (int r0 = 0; r0 < k; r0 += width) {
+        // dispatch on (kr, vr, u)
+#define FCG_G(KR, VR, U)                                          \
+    if (kr == KR && c.vr == VR && c.u == U) {                      \
+        launch_gather_t<KR, VR, U>(j, sc, k, r0, grid, per_xcd, s); \
+        continue;                                                   \
     }
+        FCG_G(1, 1, 2) FCG_G(2, 1, 2) FCG_G(4, 1, 2) FCG_G(8, 1, 2) FCG_G(16, 1, 2) FCG_G(32, 1, 2) FCG_G(64, 1, 2)
+        FCG_G(1, 1, 4) FCG_G(2, 1, 4) FCG_G(4, 1, 4) FCG_G(8, 1, 4) FCG_G(16, 1, 4) FCG_G(32, 1, 4) FCG_G(64, 1, 4)
+        FCG_G(1, 2, 2) FCG_G(2, 2, 2) FCG_G(4, 2, 2) FCG_G(8, 2, 2) FCG_G(16, 2, 2) FCG_G(32, 2, 2) FCG_G(64, 2, 2)
+        FCG_G(1, 2, 4) FCG_G(2, 2, 4) FCG_G(4, 2, 4) FCG_G(8, 2, 4) FCG_G(16, 2, 4) FCG_G(32, 2, 4) FCG_G(64, 2, 4)
+        FCG_G(1, 4, 2) FCG_G(2, 4, 2) FCG_G(4, 4, 2) FCG_G(8, 4, 2) FCG_G(16, 4, 2) FCG_G(32, 4, 2) FCG_G(64, 4, 2)
+        FCG_G(1, 4, 4) FCG_G(2, 4, 4) FCG_G(4, 4, 4) FCG_G(8, 4, 4) FCG_G(16, 4, 4) FCG_G(32, 4, 4) FCG_G(64, 4, 4)
+        FCG_G(1, 8, 2) FCG_G(2, 8, 2) FCG_G(4, 8, 2) FCG_G(8, 8, 2) FCG_G(16, 8, 2) FCG_G(32, 8, 2) FCG_G(64, 8, 2)
+#undef FCG_G
+        return hipErrorInvalidConfiguration;  // no instantiation for this (kr, vr, u)
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_mvp(const JobView& j, MvpScratch& sc, const double* q, double* v, int k, int jacobi,
@@ -482,13 +569,10 @@ hipError_t launch_mvp(const JobView& j, MvpScratch& sc, const double* q, double*
     k_node_charge<<<blocks_for(M, 256), 256, 0, s>>>(j.left, j.leaf_lo, j.leaf_hi, sc.prefix, q, j.self, N, M, k,
                                                      jacobi, sc.charge);
     if (ev) (void)hipEventRecord(ev[2], s);
-    const double avg = M > 0 ? (double)j.L / M : 0.0;
-    if (avg >= 24)
-        gather_dispatch<32>(j, sc, k, s);
-    else if (avg >= 6)
-        gather_dispatch<8>(j, sc, k, s);
-    else
-        gather_dispatch<4>(j, sc, k, s);
+    {
+        const hipError_t ge = gather_dispatch(j, sc, k, s);
+        if (ge != hipSuccess) return ge;
+    }
     if (ev) (void)hipEventRecord(ev[3], s);
     for (int r0 = 0; r0 < k; r0 += 8) {
         if (k - r0 <= 4)
@@ -544,6 +628,12 @@ hipError_t launch_reduce_partials(const double* partial, int nchunks, int nvals,
 hipError_t launch_basis_update(double* const* basis, int nvec, const double* h, double* w, int N, int k,
                                hipStream_t s) {
     k_basis_update<<<blocks_for((size_t)N * k, 256), 256, 0, s>>>(basis, nvec, h, w, N, k);
+    return hipGetLastError();
+}
+
+hipError_t launch_basis_update_norm(double* const* basis, int nvec, const double* h, double* w, int N, int k,
+                                   double* partial, int nchunks, hipStream_t s) {
+    k_basis_update_norm<<<dim3(nchunks, k), kDotThreads, 0, s>>>(basis, nvec, h, w, N, k, partial, nchunks);
     return hipGetLastError();
 }
 

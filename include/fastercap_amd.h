@@ -165,8 +165,18 @@ typedef struct fcg_mvp_timing {
     double ms_down;    /* down-pass (+ 2D fix-up) */
     double ms_total;   /* whole product */
     int32_t reps, k;
+    int32_t variant;   /* gather configuration in use */
 } fcg_mvp_timing;
 int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* out);
+
+/* Time every gather configuration (lanes per link, right-hand sides per lane,
+ * links in flight) on this job with k right-hand sides and keep the fastest for
+ * the job's later products.  fcg_solve calls it once per job and rhs count.  No
+ * reference counterpart (the reference has a single CPU loop). */
+int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* best_ms);
+
+/* Pin the gather configuration of a job (variant < 0 only queries the count). */
+int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants);
 
 /* Number of conductors (non-dielectric groups) of a job. */
 int fcg_job_conductors(fcg_ctx* ctx, int job, int* n_cond);

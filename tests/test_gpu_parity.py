@@ -60,6 +60,23 @@ def test_mvp_matches_oracle(dev, plates, k):
     dev.release(job)
 
 
+@pytest.mark.parametrize("k", [1, 2, 5, 8, 17])
+def test_every_gather_variant_matches_oracle(dev, plates, k):
+    job = _filled(dev, plates)
+    ocoef, oself = orc.fill(plates)
+    q = np.random.default_rng(100 + k).standard_normal((k, plates.N))
+    vo = np.stack([orc.mvp(plates, ocoef, oself, q[r]) for r in range(k)])
+    nvar = dev.set_gather_variant(job, -1)
+    for var in range(nvar):
+        print(f"gather variant {var} k={k}", flush=True)
+        dev.set_gather_variant(job, var)
+        v = dev.mvp(job, q)
+        assert np.max(np.abs(v - vo)) / np.max(np.abs(vo)) < 1e-12, var
+    best, ms = dev.autotune(job, k)
+    assert 0 <= best < nvar and ms > 0
+    dev.release(job)
+
+
 def test_mvp_unit_columns_match_compressed_matrix(dev, geo_dir):
     mesh = fc.Mesh.build(two_cubes(geo_dir), mesh_eps=0.02)
     print(f"mvp columns: cubes N={mesh.N}", flush=True)
