@@ -10,6 +10,7 @@ Python mirror of the reference interfaces:
                               AutoRefinePanels/links (Solver/Autorefine.cpp)
   Device.upload/fill/mvp      AutoRefineLinks fill, CMultHier::MultiplyMatByVec_fast
   Device.solve                CSolveCap::Solve (Solver/SolveCapacitance.cpp:1138-1425)
+  Device.solve_batch, Engine  several independent solves advanced together
   run(argv)                   FasterCapConsole::main + CSolveCap::Run (manual mode)
 """
 from __future__ import annotations
@@ -21,6 +22,9 @@ import numpy as np
 
 from . import _ffi
 from ._ffi import (
+    FCG_MVP_FP32,
+    FCG_MVP_FP32_ACC64,
+    FCG_MVP_FP64,
     FCG_PREC_FP64,
     FCG_PREC_MIXED,
     FCG_PRECOND_JACOBI,
@@ -36,9 +40,13 @@ from ._ffi import (
 __all__ = [
     "Mesh",
     "Device",
+    "Engine",
     "SolveResult",
     "FasterCapError",
     "run",
+    "FCG_MVP_FP64",
+    "FCG_MVP_FP32",
+    "FCG_MVP_FP32_ACC64",
     "FCG_PREC_FP64",
     "FCG_PREC_MIXED",
     "FCG_PRECOND_JACOBI",
@@ -190,6 +198,15 @@ class Device:
         self._check(self.lib.fcg_mvp(self._h, job, _dptr(q), _dptr(v), k), "fcg_mvp")
         return v
 
+    def mvp_prec(self, job: int, q: np.ndarray, mvp_prec: int) -> np.ndarray:
+        """Product in one of the three precisions (FCG_MVP_*); the mixed ones need
+        a mixed fill of the job."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        k = 1 if q.ndim == 1 else q.shape[0]
+        v = np.empty_like(q)
+        self._check(self.lib.fcg_mvp_prec(self._h, job, mvp_prec, _dptr(q), _dptr(v), k), "fcg_mvp_prec")
+        return v
+
     def precond(self, job: int, q: np.ndarray) -> np.ndarray:
         q = np.ascontiguousarray(q, dtype=np.float64)
         k = 1 if q.ndim == 1 else q.shape[0]
@@ -206,12 +223,13 @@ class Device:
         precond: int = FCG_PRECOND_JACOBI,
         precision: int = FCG_PREC_FP64,
         want_charges: bool = False,
+        max_refine: int = 8,
     ) -> SolveResult:
         ncond = int((mesh.group_is_diel == 0).sum())
         rows_max = ncond
         cap = np.zeros(rows_max * ncond, dtype=np.float64)
         charges = np.zeros(rows_max * mesh.N, dtype=np.float64) if want_charges else None
-        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=8)
+        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine)
         st = SolveStats()
         rows, cols = C.c_int(), C.c_int()
         self._check(
@@ -238,6 +256,65 @@ class Device:
             charges=None if charges is None else charges[: r * mesh.N].reshape(r, mesh.N),
         )
 
+    def solve_batch(
+        self,
+        jobs,
+        meshes,
+        tol: float = 0.01,
+        max_iter: int = 1000,
+        precond: int = FCG_PRECOND_JACOBI,
+        precision: int = FCG_PREC_FP64,
+        max_refine: int = 8,
+    ):
+        """Solve several filled jobs together (one launch per stage for all of
+        them); returns one SolveResult per job, in order."""
+        jobs = list(jobs)
+        ncs = [int((m.group_is_diel == 0).sum()) for m in meshes]
+        caps = np.zeros(sum(n * n for n in ncs), dtype=np.float64)
+        ja = (C.c_int * len(jobs))(*jobs)
+        rows = (C.c_int * len(jobs))()
+        cols = (C.c_int * len(jobs))()
+        sts = (SolveStats * len(jobs))()
+        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine)
+        self._check(
+            self.lib.fcg_solve_batch(self._h, ja, len(jobs), C.byref(o), _dptr(caps), rows, cols, sts),
+            "fcg_solve_batch",
+        )
+        out, off = [], 0
+        for i, (m, n) in enumerate(zip(meshes, ncs)):
+            r, c = rows[i], cols[i]
+            st = sts[i]
+            out.append(
+                SolveResult(
+                    cap=caps[off : off + r * c].reshape(r, c).copy(),
+                    iterations=[st.iterations[q] for q in range(min(r, 64))],
+                    residual=st.final_residual,
+                    t_solve_s=st.t_solve_s,
+                    n_mvp=st.n_mvp,
+                )
+            )
+            off += r * c
+        return out
+
+    def footprint(self, job: int, precision: int = FCG_PREC_FP64, iters: int = 64):
+        """(resident bytes of the job, bytes one solve of it allocates with a
+        Krylov basis of `iters` vectors)."""
+        a, b = C.c_int64(), C.c_int64()
+        self._check(
+            self.lib.fcg_job_footprint(self._h, job, precision, iters, C.byref(a), C.byref(b)), "fcg_job_footprint"
+        )
+        return a.value, b.value
+
+    def conductors(self, job: int) -> int:
+        n = C.c_int()
+        self._check(self.lib.fcg_job_conductors(self._h, job, C.byref(n)), "fcg_job_conductors")
+        return n.value
+
+    def engine(self, tol: float = 0.01, max_iter: int = 1000, precond: int = FCG_PRECOND_JACOBI,
+               precision: int = FCG_PREC_FP64, max_refine: int = 8) -> "Engine":
+        return Engine(self, SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision,
+                                      max_refine=max_refine))
+
     def mvp_timing(self, job: int, k: int, reps: int = 20) -> dict:
         t = _ffi.MvpTiming()
         self._check(self.lib.fcg_mvp_timing_run(self._h, job, k, reps, C.byref(t)), "fcg_mvp_timing_run")
@@ -256,6 +333,64 @@ class Device:
     def close(self):
         if getattr(self, "_h", None):
             self.lib.fcg_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Engine:
+    """Continuous-batching solver: submit filled jobs at any time, step, poll
+    finished results.  Every step advances every active solve by one GMRES
+    stage with one launch per kernel for all of them."""
+
+    def __init__(self, dev: Device, opts: SolveOpts):
+        self.dev = dev
+        self.lib = dev.lib
+        self.opts = opts
+        h = C.c_void_p()
+        dev._check(self.lib.fcg_engine_create(dev._h, C.byref(opts), C.byref(h)), "fcg_engine_create")
+        self._h = h
+
+    def submit(self, job: int, tag: int):
+        self.dev._check(self.lib.fcg_engine_submit(self._h, job, tag), "fcg_engine_submit")
+
+    def step(self):
+        """Advance every active solve one stage; returns (active, ready)."""
+        a, r = C.c_int(), C.c_int()
+        self.dev._check(self.lib.fcg_engine_step(self._h, C.byref(a), C.byref(r)), "fcg_engine_step")
+        return a.value, r.value
+
+    def poll(self, max_cols: int = 256):
+        """Next finished solve as (tag, job, SolveResult), or None."""
+        cap = np.zeros(max_cols * max_cols, dtype=np.float64)
+        tag, job, rows, cols = C.c_int64(), C.c_int(), C.c_int(), C.c_int()
+        st = SolveStats()
+        rc = self.lib.fcg_engine_poll(self._h, C.byref(tag), C.byref(job), _dptr(cap), cap.size, C.byref(rows),
+                                      C.byref(cols), C.byref(st))
+        if rc != 0:
+            return None
+        r, c = rows.value, cols.value
+        res = SolveResult(
+            cap=cap[: r * c].reshape(r, c).copy(),
+            iterations=[st.iterations[q] for q in range(min(r, 64))],
+            residual=st.final_residual,
+            t_solve_s=st.t_solve_s,
+            n_mvp=st.n_mvp,
+        )
+        return tag.value, job.value, res
+
+    def bytes(self) -> int:
+        b = C.c_int64()
+        self.dev._check(self.lib.fcg_engine_bytes(self._h, C.byref(b)), "fcg_engine_bytes")
+        return b.value
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.lib.fcg_engine_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -119,6 +119,9 @@ FCG_PREC_FP64 = 0
 FCG_PREC_MIXED = 1
 FCG_PRECOND_NONE = 0
 FCG_PRECOND_JACOBI = 1
+FCG_MVP_FP64 = 0
+FCG_MVP_FP32 = 1
+FCG_MVP_FP32_ACC64 = 2
 FCG_NODE_LEAF = 1
 FCG_NODE_DIEL = 2
 FCG_NODE_SUPER = 128
@@ -140,6 +143,15 @@ DEVICE_SYMBOLS = (
     "fcg_mvp_timing_run",
     "fcg_autotune",
     "fcg_set_gather_variant",
+    "fcg_mvp_prec",
+    "fcg_solve_batch",
+    "fcg_engine_create",
+    "fcg_engine_submit",
+    "fcg_engine_step",
+    "fcg_engine_poll",
+    "fcg_engine_bytes",
+    "fcg_engine_destroy",
+    "fcg_job_footprint",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -218,6 +230,17 @@ def device_lib() -> C.CDLL:
         lib.fcg_mvp_timing_run.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(MvpTiming)]
         lib.fcg_set_gather_variant.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
         lib.fcg_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        ip = C.POINTER(C.c_int)
+        i64p = C.POINTER(C.c_int64)
+        lib.fcg_mvp_prec.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]
+        lib.fcg_solve_batch.argtypes = [vp, ip, C.c_int, C.POINTER(SolveOpts), dp, ip, ip, C.POINTER(SolveStats)]
+        lib.fcg_engine_create.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(vp)]
+        lib.fcg_engine_submit.argtypes = [vp, C.c_int, C.c_int64]
+        lib.fcg_engine_step.argtypes = [vp, ip, ip]
+        lib.fcg_engine_poll.argtypes = [vp, i64p, ip, dp, C.c_int, ip, ip, C.POINTER(SolveStats)]
+        lib.fcg_engine_bytes.argtypes = [vp, i64p]
+        lib.fcg_engine_destroy.argtypes = [vp]
+        lib.fcg_job_footprint.argtypes = [vp, C.c_int, C.c_int, C.c_int, i64p, i64p]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p
         for name in DEVICE_SYMBOLS:

@@ -1,35 +1,47 @@
 // C-ABI of the gfx950 FasterCap solve path (declarations and reference citations
-// in include/fastercap_amd.h).  Owns the HBM-resident job state and drives the
-// kernels of kernels.hip on one stream per context (default priority).
+// in include/fastercap_amd.h).  Owns the HBM-resident jobs and a batched GMRES
+// engine that advances any set of solves together, one launch per stage per step.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
 
 #include "kernels.h"
 
+using fcg::BatchView;
+using fcg::BJob;
+using fcg::Chunk;
 using fcg::DD;
-using fcg::JobView;
-using fcg::MvpScratch;
+using fcg::KJob;
+using fcg::KrylovView;
+using fcg::Seg;
 
 namespace {
+
+using clk = std::chrono::steady_clock;
 
 struct DevBuf {
     void* p = nullptr;
     size_t bytes = 0;
-    ~DevBuf() {
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
     }
     hipError_t alloc(size_t n) {
-        if (p) {
-            (void)hipFree(p);
-            p = nullptr;
-        }
+        release();
         bytes = n;
         if (n == 0) return hipSuccess;
         return hipMalloc(&p, n);
@@ -42,55 +54,206 @@ struct DevBuf {
 
 struct PinnedBuf {
     void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
     ~PinnedBuf() {
         if (p) (void)hipHostFree(p);
     }
-    hipError_t alloc(size_t n) { return hipHostMalloc(&p, n, hipHostMallocDefault); }
+    hipError_t alloc(size_t n) {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        bytes = n;
+        return hipHostMalloc(&p, n, hipHostMallocDefault);
+    }
     template <typename T>
     T* as() const {
         return reinterpret_cast<T*>(p);
     }
 };
 
+// Caching device allocator for solve state: freed blocks are kept per rounded size
+// and reused by later solves (serving keeps allocating and releasing solves).
+struct Pool {
+    std::map<size_t, std::vector<void*>> free_;
+    int64_t held = 0, in_use = 0;
+    static size_t round(size_t n) {
+        if (n <= 4096) return 4096;
+        size_t p2 = 1;
+        while (p2 < n) p2 <<= 1;
+        const size_t step = std::max<size_t>(p2 / 8, 4096);
+        return (n + step - 1) / step * step;
+    }
+    hipError_t get(size_t n, void** out, size_t* rounded) {
+        const size_t r = round(n);
+        *rounded = r;
+        auto it = free_.find(r);
+        if (it != free_.end() && !it->second.empty()) {
+            *out = it->second.back();
+            it->second.pop_back();
+            in_use += (int64_t)r;
+            return hipSuccess;
+        }
+        hipError_t e = hipMalloc(out, r);
+        if (e == hipSuccess) {
+            held += (int64_t)r;
+            in_use += (int64_t)r;
+        }
+        return e;
+    }
+    void put(void* p, size_t r) {
+        if (!p) return;
+        free_[r].push_back(p);
+        in_use -= (int64_t)r;
+    }
+    ~Pool() {
+        for (auto& kv : free_)
+            for (void* p : kv.second) (void)hipFree(p);
+    }
+};
+
+// Pool-backed device block; released to the pool (not to HIP) when dropped.
+struct PBuf {
+    void* p = nullptr;
+    size_t r = 0;
+    Pool* pool = nullptr;
+    PBuf() = default;
+    PBuf(const PBuf&) = delete;
+    PBuf& operator=(const PBuf&) = delete;
+    ~PBuf() { release(); }
+    void release() {
+        if (p && pool) pool->put(p, r);
+        p = nullptr;
+        r = 0;
+    }
+    hipError_t get(Pool* pl, size_t n) {
+        release();
+        pool = pl;
+        return pl->get(n, &p, &r);
+    }
+    template <typename T>
+    T* as() const {
+        return reinterpret_cast<T*>(p);
+    }
+};
+
+// Per-step tables: slabs of device memory with paired pinned staging,
+// bump-allocated and reset once the step's work has completed.
+struct Tables {
+    struct Slab {
+        DevBuf dev;
+        PinnedBuf host;
+        size_t used = 0;
+    };
+    std::vector<std::unique_ptr<Slab>> slabs;
+    size_t cur = 0;
+    hipError_t take(size_t bytes, char** h, char** d) {
+        const size_t need = (bytes + 255) & ~(size_t)255;
+        while (cur < slabs.size() && slabs[cur]->used + need > slabs[cur]->dev.bytes) ++cur;
+        if (cur == slabs.size()) {
+            auto sl = std::make_unique<Slab>();
+            const size_t sz = std::max<size_t>(need, (size_t)8 << 20);
+            hipError_t e = sl->dev.alloc(sz);
+            if (e != hipSuccess) return e;
+            e = sl->host.alloc(sz);
+            if (e != hipSuccess) return e;
+            slabs.push_back(std::move(sl));
+        }
+        Slab& sl = *slabs[cur];
+        *h = sl.host.as<char>() + sl.used;
+        *d = sl.dev.as<char>() + sl.used;
+        sl.used += need;
+        return hipSuccess;
+    }
+    // copy host data into a fresh device region (enqueued on s)
+    hipError_t push(const void* data, size_t bytes, hipStream_t s, void** dptr) {
+        char *h, *d;
+        hipError_t e = take(bytes, &h, &d);
+        if (e != hipSuccess) return e;
+        *dptr = d;
+        if (!bytes) return hipSuccess;
+        std::memcpy(h, data, bytes);
+        return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+    }
+    // pinned host area for a device-to-host copy (valid until reset)
+    hipError_t host_area(size_t bytes, void** hptr) {
+        char *h, *d;
+        hipError_t e = take(bytes, &h, &d);
+        *hptr = h;
+        return e;
+    }
+    void reset() {
+        for (auto& s : slabs) s->used = 0;
+        cur = 0;
+    }
+};
+
+// Scratch of one hierarchical product on k right-hand sides: leaf prefix sums,
+// node charges and node potentials.  Every concurrently running solve owns one.
+struct Scratch {
+    int k = 0, nblk = 0;
+    PBuf prefix, block_sums, charge64, charge32, pot64, pot32, lt;
+    size_t bytes() const {
+        return prefix.r + block_sums.r + charge64.r + charge32.r + pot64.r + pot32.r + lt.r;
+    }
+};
+
 struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
-    DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, gout, gin, coef, self, diag;
-    DevBuf leaf_w, col_range;
-    // host copies of the group table
+    DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
+        col_range, self, diag, coef64, coef32;
+    bool filled64 = false, filled32 = false;
     std::vector<int32_t> group_is_diel, group_leaf_start;
-    std::vector<double> leaf_outperm;
+    std::vector<std::pair<int, int>> rhs;  // leaf range raised to 1 V per right-hand side
+    int ncol = 0;
     int first_cond_leaf = 0;
-    bool filled = false;
+    Scratch scr;  // product scratch of the stand-alone entry points
     int gather_variant = 0;
-    int tuned_k = 0;  // rhs count the gather variant was tuned for (0 = not tuned)
-    // product scratch
-    DevBuf prefix, block_sums, charge, pot;
-    MvpScratch sc{};
-    JobView view() const {
-        JobView v{};
-        v.dim = dim;
-        v.N = N;
-        v.M = M;
-        v.G = G;
-        v.L = L;
-        v.nodes = nodes.as<fcg_node_geom>();
-        v.left = left.as<int32_t>();
-        v.right = right.as<int32_t>();
-        v.parent = parent.as<int32_t>();
-        v.leaf_lo = leaf_lo.as<int32_t>();
-        v.leaf_hi = leaf_hi.as<int32_t>();
-        v.leaf_node = leaf_node.as<int32_t>();
-        v.row_ptr = row_ptr.as<int64_t>();
-        v.col = col.as<int32_t>();
-        v.group_outperm = gout.as<double>();
-        v.group_inperm = gin.as<double>();
-        v.coef = coef.as<double>();
-        v.self = self.as<double>();
-        v.diag = diag.as<double>();
-        v.first_cond_leaf = first_cond_leaf;
-        v.gather_variant = gather_variant;
-        return v;
+    int tuned_k = 0;  // 0 = not tuned, -1 = pinned by the caller
+
+    int64_t resident_bytes() const {
+        return (int64_t)(nodes.bytes + left.bytes + right.bytes + parent.bytes + leaf_lo.bytes + leaf_hi.bytes +
+                         leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
+                         leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
+                         scr.bytes());
+    }
+
+    BJob rec(const Scratch& sc, const double* q, double* v) const {
+        BJob b{};
+        b.N = N;
+        b.M = M;
+        b.k = sc.k;
+        b.dim = dim;
+        b.first_cond_leaf = first_cond_leaf;
+        b.nblk = sc.nblk;
+        b.L = L;
+        b.nodes = nodes.as<fcg_node_geom>();
+        b.row_ptr = row_ptr.as<int64_t>();
+        b.col = col.as<int32_t>();
+        b.link_row = link_row.as<int32_t>();
+        b.coef64 = coef64.as<double>();
+        b.coef32 = coef32.as<float>();
+        b.diag = diag.as<double>();
+        b.self = self.as<double>();
+        b.left = left.as<int32_t>();
+        b.leaf_lo = leaf_lo.as<int32_t>();
+        b.leaf_hi = leaf_hi.as<int32_t>();
+        b.parent = parent.as<int32_t>();
+        b.leaf_node = leaf_node.as<int32_t>();
+        b.gout = gout.as<double>();
+        b.gin = gin.as<double>();
+        b.prefix = sc.prefix.as<DD>();
+        b.block_sums = sc.block_sums.as<DD>();
+        b.charge64 = sc.charge64.as<double>();
+        b.charge32 = sc.charge32.as<float>();
+        b.pot64 = sc.pot64.as<double>();
+        b.pot32 = sc.pot32.as<float>();
+        b.lt = sc.lt.as<double>();
+        b.q = q;
+        b.v = v;
+        return b;
     }
 };
 
@@ -99,8 +262,10 @@ struct Job {
 struct fcg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    Pool pool;  // declared before everything that returns blocks to it
     std::vector<std::unique_ptr<Job>> jobs;
     std::string err;
+    Tables tabs;  // tables of stand-alone calls (reset per call)
 };
 
 namespace {
@@ -110,13 +275,19 @@ int fail(fcg_ctx* c, int code, const std::string& msg) {
     return code;
 }
 
-#define HIPCHECK(ctx, call)                                                                              \
-    do {                                                                                                 \
-        hipError_t e_ = (call);                                                                          \
-        if (e_ != hipSuccess) {                                                                          \
-            return fail(ctx, e_ == hipErrorOutOfMemory ? FCG_OUT_OF_MEMORY : FCG_DEVICE_ERROR,           \
-                        std::string(#call) + ": " + hipGetErrorString(e_));                              \
-        }                                                                                                \
+#define HIPCHECK(ctx, call)                                                                    \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? FCG_OUT_OF_MEMORY : FCG_DEVICE_ERROR, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));                    \
+        }                                                                                      \
+    } while (0)
+
+#define RCCHECK(call)                  \
+    do {                               \
+        int rc_ = (call);              \
+        if (rc_ != FCG_OK) return rc_; \
     } while (0)
 
 Job* get_job(fcg_ctx* ctx, int job) {
@@ -131,20 +302,83 @@ hipError_t upload(DevBuf& b, const T* src, size_t n, hipStream_t s) {
     return hipMemcpyAsync(b.p, src, n * sizeof(T), hipMemcpyHostToDevice, s);
 }
 
-int ensure_scratch(fcg_ctx* ctx, Job& j, int k) {
-    const int nblk = (j.N + fcg::kScanTile - 1) / fcg::kScanTile;
-    if (j.sc.kmax >= k && j.sc.nblk >= nblk) return FCG_OK;
+// product scratch for k right-hand sides of an (N leaves, M nodes) job
+int ensure_scratch(fcg_ctx* ctx, Scratch& sc, int N, int M, int k, bool need32) {
+    const int nblk = (N + fcg::kScanTile - 1) / fcg::kScanTile;
+    if (sc.k == k && (!need32 || sc.charge32.p)) return FCG_OK;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-    HIPCHECK(ctx, j.prefix.alloc(sizeof(DD) * (size_t)k * (j.N + 1)));
-    HIPCHECK(ctx, j.block_sums.alloc(sizeof(DD) * (size_t)k * (nblk > 1 ? nblk : 1) + 16 * (size_t)k));
-    HIPCHECK(ctx, j.charge.alloc(sizeof(double) * (size_t)k * j.M));
-    HIPCHECK(ctx, j.pot.alloc(sizeof(double) * (size_t)k * j.M));
-    j.sc.prefix = j.prefix.as<DD>();
-    j.sc.block_sums = j.block_sums.as<DD>();
-    j.sc.charge = j.charge.as<double>();
-    j.sc.pot = j.pot.as<double>();
-    j.sc.nblk = nblk;
-    j.sc.kmax = k;
+    Pool* pl = &ctx->pool;
+    if (sc.k != k) {
+        HIPCHECK(ctx, sc.prefix.get(pl, sizeof(DD) * (size_t)k * (N + 1)));
+        HIPCHECK(ctx, sc.block_sums.get(pl, sizeof(DD) * (size_t)k * nblk));
+        HIPCHECK(ctx, sc.charge64.get(pl, sizeof(double) * (size_t)k * M));
+        HIPCHECK(ctx, sc.pot64.get(pl, sizeof(double) * (size_t)k * M));
+        HIPCHECK(ctx, sc.lt.get(pl, sizeof(double) * 2 * (size_t)k));
+        sc.charge32.release();
+        sc.pot32.release();
+    }
+    if (need32 && !sc.charge32.p) {
+        HIPCHECK(ctx, sc.charge32.get(pl, sizeof(float) * (size_t)k * M));
+        HIPCHECK(ctx, sc.pot32.get(pl, sizeof(float) * (size_t)k * M));
+    }
+    sc.k = k;
+    sc.nblk = nblk;
+    return FCG_OK;
+}
+
+// Upload a batch table for the given (job, q, v) triples into `tabs`.
+struct BatchEntry {
+    Job* j;
+    const Scratch* sc;
+    const double* q;
+    double* v;
+};
+int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, BatchView* out) {
+    std::memset(out, 0, sizeof *out);
+    const int n = (int)ents.size();
+    if (n == 0) return FCG_OK;
+    std::vector<BJob> recs((size_t)n);
+    std::vector<int64_t> off((size_t)6 * (n + 1), 0);
+    int64_t* lo = off.data();
+    int64_t* to = lo + (n + 1);
+    int64_t* no = to + (n + 1);
+    int64_t* ro = no + (n + 1);
+    int64_t* fo = ro + (n + 1);
+    int64_t* ko = fo + (n + 1);
+    int kmax = 0, any2d = 0;
+    for (int i = 0; i < n; ++i) {
+        const Job& J = *ents[(size_t)i].j;
+        const Scratch& sc = *ents[(size_t)i].sc;
+        recs[(size_t)i] = J.rec(sc, ents[(size_t)i].q, ents[(size_t)i].v);
+        lo[i + 1] = lo[i] + J.L;
+        to[i + 1] = to[i] + (int64_t)sc.k * sc.nblk;
+        no[i + 1] = no[i] + J.M;
+        ro[i + 1] = ro[i] + (J.M + 3) / 4;
+        fo[i + 1] = fo[i] + J.N;
+        ko[i + 1] = ko[i] + sc.k;
+        kmax = std::max(kmax, sc.k);
+        any2d |= (J.dim == 2);
+    }
+    void *dr, *doff;
+    HIPCHECK(ctx, tabs.push(recs.data(), sizeof(BJob) * n, ctx->stream, &dr));
+    HIPCHECK(ctx, tabs.push(off.data(), sizeof(int64_t) * off.size(), ctx->stream, &doff));
+    int64_t* d = reinterpret_cast<int64_t*>(doff);
+    out->jobs = reinterpret_cast<const BJob*>(dr);
+    out->njobs = n;
+    out->link_off = d;
+    out->tile_off = d + (n + 1);
+    out->node_off = d + 2 * (n + 1);
+    out->rowb_off = d + 3 * (n + 1);
+    out->leaf_off = d + 4 * (n + 1);
+    out->rhs_off = d + 5 * (n + 1);
+    out->links = lo[n];
+    out->tiles = to[n];
+    out->nodes = no[n];
+    out->rowb = ro[n];
+    out->leaves = fo[n];
+    out->rhs = ko[n];
+    out->kmax = kmax;
+    out->any2d = any2d;
     return FCG_OK;
 }
 
@@ -170,9 +404,12 @@ int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
             return fail(ctx, FCG_INVALID_ARGUMENT, "parent index out of order");
         if (d->leaf_lo[i] < 0 || d->leaf_hi[i] > N || d->leaf_lo[i] >= d->leaf_hi[i])
             return fail(ctx, FCG_INVALID_ARGUMENT, "bad leaf range");
+        if (!leaf && (d->leaf_lo[i] != d->leaf_lo[d->left[i]] || d->leaf_hi[i] != d->leaf_hi[d->right[i]] ||
+                      d->leaf_hi[d->left[i]] != d->leaf_lo[d->right[i]]))
+            return fail(ctx, FCG_INVALID_ARGUMENT, "leaf ranges do not nest");
         if (d->nodes[i].cond < 0 || d->nodes[i].cond >= d->n_groups)
             return fail(ctx, FCG_INVALID_ARGUMENT, "bad node group");
-        if (d->nodes[i].maxside <= 0.0) return fail(ctx, FCG_INVALID_ARGUMENT, "non-positive node size");
+        if (!(d->nodes[i].maxside > 0.0)) return fail(ctx, FCG_INVALID_ARGUMENT, "non-positive node size");
     }
     for (int k = 0; k < N; ++k) {
         const int nd = d->leaf_node[k];
@@ -183,6 +420,544 @@ int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
         if (d->col[l] < 0 || d->col[l] >= M) return fail(ctx, FCG_INVALID_ARGUMENT, "link source out of range");
     if (d->group_leaf_start[0] != 0 || d->group_leaf_start[d->n_groups] != N)
         return fail(ctx, FCG_INVALID_ARGUMENT, "group leaf ranges do not span the leaves");
+    for (int g = 0; g < d->n_groups; ++g)
+        if (d->group_leaf_start[g + 1] <= d->group_leaf_start[g])
+            return fail(ctx, FCG_INVALID_ARGUMENT, "empty group");
+    return FCG_OK;
+}
+
+// One stand-alone product on device buffers of job j (batch of one).
+int run_mvp(fcg_ctx* ctx, Job& j, int prec, const double* dq, double* dv, int k, int jacobi, hipStream_t s,
+            hipEvent_t* ev = nullptr) {
+    if (prec == fcg::kMvp64 && !j.filled64) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in FP64");
+    if (prec != fcg::kMvp64 && !j.filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in mixed mode");
+    RCCHECK(ensure_scratch(ctx, j.scr, j.N, j.M, k, prec == fcg::kMvp32));
+    HIPCHECK(ctx, hipStreamSynchronize(s));  // the table arena is reused below
+    ctx->tabs.reset();
+    BatchView bv;
+    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, dq, dv}}, &bv));
+    HIPCHECK(ctx, fcg::launch_mvp(bv, prec, jacobi, j.gather_variant, s, ev));
+    return FCG_OK;
+}
+
+}  // namespace
+
+// ======================================================================== engine
+namespace {
+
+enum Phase { kInit = 0, kResid = 1, kArn = 2, kUpdate = 3, kFinish = 4, kDone = 5 };
+
+struct Solve {
+    int job_id = -1;
+    Job* J = nullptr;
+    int64_t tag = 0;
+    int k = 0, N = 0, ncol = 0;
+    bool mixed = false, jac = true, want_x = false;
+    PBuf b, x, r, w;
+    Scratch scr;
+    // Krylov basis: blocks of vectors, host list and device address table
+    std::vector<std::unique_ptr<PBuf>> qblocks;
+    std::vector<double*> hq;
+    PBuf qtab;
+    int qcap = 0;
+    int qsynced = 0;  // entries of the device table already written
+    int phase = kInit;
+    int i = 0;  // Arnoldi step index within the cycle (the next product uses Q[i])
+    int cycles = 0;
+    bool need_scale = false;    // Q[i] = src / d pending
+    bool scale_from_r = false;  // src is r (cycle start) or w (continuation)
+    std::vector<double> d;      // scale divisors per rhs
+    std::vector<double> normb, beta, thr;
+    std::vector<std::vector<double>> H, cs, sn, g;
+    std::vector<int> done, last_i, total_steps;
+    std::vector<double> z;  // pending solution update [nz][k]
+    int nz = 0;
+    int cap_iter = 0;
+    int64_t n_mvp = 0;
+    double rel = 0.0;
+    clk::time_point t0;
+    int hoff = 0;
+    int64_t bytes() const {
+        int64_t s = (int64_t)(b.r + x.r + r.r + w.r + qtab.r + scr.bytes());
+        for (auto& q : qblocks) s += (int64_t)q->r;
+        return s;
+    }
+};
+
+struct Ready {
+    int64_t tag;
+    int job;
+    std::vector<double> cap, x;
+    int rows, cols;
+    fcg_solve_stats stats;
+};
+
+}  // namespace
+
+struct fcg_engine {
+    fcg_ctx* ctx = nullptr;
+    fcg_solve_opts opts{};
+    std::vector<std::unique_ptr<Solve>> active;
+    std::deque<Ready> ready;
+    Tables tabs;
+    PBuf partial_a, partial_b, hbuf;
+    bool want_x = false;
+};
+
+namespace {
+
+int ensure_pbuf(fcg_ctx* ctx, PBuf& b, size_t bytes) {
+    if (b.p && b.r >= bytes) return FCG_OK;
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHECK(ctx, b.get(&ctx->pool, bytes));
+    return FCG_OK;
+}
+
+// make sure basis vector index `idx` exists and its address is in the device table
+int ensure_basis(fcg_ctx* ctx, Solve& S, int idx) {
+    const size_t vec = (size_t)S.k * S.N;
+    while ((int)S.hq.size() <= idx) {
+        // blocks of about 64 MiB, 4 to 64 vectors
+        const size_t per = std::max<size_t>(4, std::min<size_t>(64, ((size_t)64 << 20) / (vec * 8 + 1)));
+        S.qblocks.emplace_back(new PBuf());
+        HIPCHECK(ctx, S.qblocks.back()->get(&ctx->pool, sizeof(double) * vec * per));
+        for (size_t c = 0; c < per; ++c) S.hq.push_back(S.qblocks.back()->as<double>() + c * vec);
+    }
+    if ((int)S.hq.size() > S.qcap) {
+        // a larger table; kernels already enqueued keep reading the old one, so the
+        // stream is drained before the old block returns to the pool
+        HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+        const int cap = (int)S.hq.size() + 64;
+        S.qtab.release();
+        HIPCHECK(ctx, S.qtab.get(&ctx->pool, sizeof(double*) * (size_t)cap));
+        S.qcap = cap;
+        S.qsynced = 0;
+    }
+    if (S.qsynced < (int)S.hq.size()) {
+        // new entries are not read by any enqueued kernel yet
+        HIPCHECK(ctx, hipMemcpy(S.qtab.as<double*>() + S.qsynced, S.hq.data() + S.qsynced,
+                                sizeof(double*) * (S.hq.size() - S.qsynced), hipMemcpyHostToDevice));
+        S.qsynced = (int)S.hq.size();
+    }
+    return FCG_OK;
+}
+
+KJob kjob_of(Solve& S, double* w, int nv, int zoff) {
+    KJob J{};
+    J.Q = S.qtab.as<double*>();
+    J.w = w;
+    J.x = S.x.as<double>();
+    J.r = S.r.as<double>();
+    J.b = S.b.as<double>();
+    J.self = S.J->self.as<double>();
+    J.N = S.N;
+    J.k = S.k;
+    J.nv = nv;
+    J.hoff = S.hoff;
+    J.zoff = zoff;
+    J.nz = S.nz;
+    J.jacobi = S.jac ? 1 : 0;
+    return J;
+}
+
+// Krylov tables for a list of solves
+struct KEnt {
+    Solve* S;
+    double* w;
+    int nv;
+    int zoff;
+};
+int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, KrylovView* kv) {
+    std::memset(kv, 0, sizeof *kv);
+    std::vector<KJob> kj;
+    std::vector<Seg> segs;
+    std::vector<Chunk> chunks;
+    int maxnv = 0;
+    for (size_t e = 0; e < ents.size(); ++e) {
+        Solve& S = *ents[e].S;
+        kj.push_back(kjob_of(S, ents[e].w, ents[e].nv, ents[e].zoff));
+        maxnv = std::max(maxnv, ents[e].nv);
+        for (int r = 0; r < S.k; ++r) {
+            Seg sg;
+            sg.job = (int)e;
+            sg.r = r;
+            sg.c_lo = (int)chunks.size();
+            for (int n0 = 0; n0 < S.N; n0 += fcg::kChunk)
+                chunks.push_back(Chunk{(int)segs.size(), n0, std::min(S.N, n0 + fcg::kChunk)});
+            sg.c_hi = (int)chunks.size();
+            segs.push_back(sg);
+        }
+    }
+    void *a, *b2, *c;
+    HIPCHECK(ctx, tabs.push(kj.data(), sizeof(KJob) * kj.size(), ctx->stream, &a));
+    HIPCHECK(ctx, tabs.push(segs.data(), sizeof(Seg) * segs.size(), ctx->stream, &b2));
+    HIPCHECK(ctx, tabs.push(chunks.data(), sizeof(Chunk) * chunks.size(), ctx->stream, &c));
+    kv->kjobs = reinterpret_cast<const KJob*>(a);
+    kv->segs = reinterpret_cast<const Seg*>(b2);
+    kv->chunks = reinterpret_cast<const Chunk*>(c);
+    kv->nsegs = (int)segs.size();
+    kv->nchunks = (int)chunks.size();
+    kv->maxnv = maxnv;
+    return FCG_OK;
+}
+
+void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
+    const double tol = E->opts.tol;
+    S.beta.assign(beta, beta + S.k);
+    S.thr.assign((size_t)S.k, 0.0);
+    S.done.assign((size_t)S.k, 0);
+    S.last_i.assign((size_t)S.k, -1);
+    for (int q = 0; q < S.k; ++q) {
+        S.H[(size_t)q].clear();
+        S.cs[(size_t)q].clear();
+        S.sn[(size_t)q].clear();
+        S.g[(size_t)q].assign((size_t)S.cap_iter + 2, 0.0);
+        S.g[(size_t)q][0] = beta[q];
+        if (!S.mixed) {
+            // reference stop rule |g_{i+1}| / |b| < tol (SolveCapacitance.cpp:2493-2500)
+            S.thr[(size_t)q] = tol * S.normb[(size_t)q];
+        } else {
+            // inner FP32 solve: reduce the residual to half the outer target, at most
+            // to the FP32 product's own accuracy
+            const double rel_in = std::max(0.5 * tol * S.normb[(size_t)q] / beta[q], 2e-6);
+            S.thr[(size_t)q] = rel_in * beta[q];
+        }
+        if (beta[q] / S.normb[(size_t)q] < tol) S.done[(size_t)q] = 1;
+    }
+    S.i = 0;
+    S.cycles++;
+    S.need_scale = true;
+    S.scale_from_r = true;
+    S.d.assign(beta, beta + S.k);
+    S.phase = kArn;
+    bool any = false;
+    for (int d : S.done) any |= !d;
+    if (!any) S.phase = kFinish;  // the current solution already meets the tolerance
+}
+
+// back substitution of every rhs; z[jj][r], zero beyond each rhs's own count
+void prepare_update(Solve& S) {
+    const int ld = S.cap_iter + 1;
+    int nz = 0;
+    for (int q = 0; q < S.k; ++q) nz = std::max(nz, S.last_i[(size_t)q] + 1);
+    S.nz = nz;
+    S.z.assign((size_t)nz * S.k, 0.0);
+    for (int q = 0; q < S.k; ++q) {
+        const int it = S.last_i[(size_t)q];
+        if (it < 0) continue;
+        std::vector<double> zz(S.g[(size_t)q].begin(), S.g[(size_t)q].begin() + it + 1);
+        const std::vector<double>& Hq = S.H[(size_t)q];
+        for (int ii = it; ii >= 0; --ii) {
+            zz[(size_t)ii] = zz[(size_t)ii] / Hq[(size_t)ii * ld + ii];
+            for (int jj = ii - 1; jj >= 0; --jj) zz[(size_t)jj] -= Hq[(size_t)ii * ld + jj] * zz[(size_t)ii];
+        }
+        for (int jj = 0; jj <= it; ++jj) S.z[(size_t)jj * S.k + q] = zz[(size_t)jj];
+    }
+    S.phase = kUpdate;
+}
+
+// Givens update of one Arnoldi step of every rhs (h1 + h2 = column, |w| = subdiagonal)
+void givens_step(Solve& S, const double* h1, const double* h2, const double* w2) {
+    const int i = S.i;
+    const int ld = S.cap_iter + 1;
+    const int k = S.k;
+    S.d.assign((size_t)k, 0.0);
+    for (int q = 0; q < k; ++q) {
+        const double hn = std::sqrt(w2[q]);
+        S.d[(size_t)q] = hn;
+        if (S.done[(size_t)q]) continue;
+        std::vector<double>& Hq = S.H[(size_t)q];
+        if (Hq.size() < (size_t)(i + 1) * ld) Hq.resize(std::max(Hq.size() * 2, (size_t)(i + 1) * ld), 0.0);
+        double* colv = Hq.data() + (size_t)i * ld;
+        for (int jj = 0; jj <= i; ++jj) colv[jj] = h1[(size_t)jj * k + q] + h2[(size_t)jj * k + q];
+        auto& c = S.cs[(size_t)q];
+        auto& sv = S.sn[(size_t)q];
+        for (int jj = 0; jj < i; ++jj) {
+            const double t1 = colv[jj], t2 = colv[jj + 1];
+            colv[jj] = c[(size_t)jj] * t1 - sv[(size_t)jj] * t2;
+            colv[jj + 1] = c[(size_t)jj] * t2 + sv[(size_t)jj] * t1;
+        }
+        const double t1 = colv[i], t2 = hn;
+        const double len = std::sqrt(t1 * t1 + t2 * t2);
+        c.push_back(t1 / len);
+        sv.push_back(-t2 / len);
+        colv[i] = c[(size_t)i] * t1 - sv[(size_t)i] * t2;
+        const double gi = S.g[(size_t)q][(size_t)i];
+        S.g[(size_t)q][(size_t)i] = c[(size_t)i] * gi;
+        S.g[(size_t)q][(size_t)i + 1] = sv[(size_t)i] * gi;
+        S.last_i[(size_t)q] = i;
+        S.total_steps[(size_t)q]++;
+        if (std::fabs(S.g[(size_t)q][(size_t)i + 1]) < S.thr[(size_t)q]) S.done[(size_t)q] = 1;
+    }
+}
+
+int engine_submit(fcg_engine* E, int job, int64_t tag) {
+    fcg_ctx* ctx = E->ctx;
+    Job* J = get_job(ctx, job);
+    if (!J) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+    const bool mixed = E->opts.precision == FCG_PREC_MIXED;
+    if (mixed ? !J->filled32 : !J->filled64) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in this precision");
+    if (J->rhs.empty()) return fail(ctx, FCG_INVALID_ARGUMENT, "job has no conductor right-hand side");
+    auto S = std::make_unique<Solve>();
+    S->job_id = job;
+    S->J = J;
+    S->tag = tag;
+    S->k = (int)J->rhs.size();
+    S->N = J->N;
+    S->ncol = J->ncol;
+    S->mixed = mixed;
+    S->jac = E->opts.precond == FCG_PRECOND_JACOBI;
+    S->want_x = E->want_x;
+    S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
+    RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, S->k, mixed));
+    const size_t vec = (size_t)S->k * S->N;
+    HIPCHECK(ctx, S->b.get(&ctx->pool, sizeof(double) * vec));
+    HIPCHECK(ctx, S->x.get(&ctx->pool, sizeof(double) * vec));
+    HIPCHECK(ctx, S->r.get(&ctx->pool, sizeof(double) * vec));
+    HIPCHECK(ctx, S->w.get(&ctx->pool, sizeof(double) * vec));
+    {
+        // unit potential on each conductor's panels (SolveCapacitance.cpp:1209-1211)
+        std::vector<double> hb(vec, 0.0);
+        S->normb.assign((size_t)S->k, 0.0);
+        for (int q = 0; q < S->k; ++q) {
+            const auto rg = J->rhs[(size_t)q];
+            for (int n = rg.first; n < rg.second; ++n) hb[(size_t)q * S->N + n] = 1.0;
+            S->normb[(size_t)q] = std::sqrt((double)(rg.second - rg.first));
+        }
+        HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+        HIPCHECK(ctx, hipMemcpy(S->b.p, hb.data(), sizeof(double) * vec, hipMemcpyHostToDevice));
+    }
+    RCCHECK(ensure_basis(ctx, *S, 1));
+    S->H.assign((size_t)S->k, {});
+    S->cs.assign((size_t)S->k, {});
+    S->sn.assign((size_t)S->k, {});
+    S->g.assign((size_t)S->k, {});
+    S->total_steps.assign((size_t)S->k, 0);
+    S->t0 = clk::now();
+    S->phase = kInit;
+    E->active.push_back(std::move(S));
+    return FCG_OK;
+}
+
+int engine_step(fcg_engine* E) {
+    fcg_ctx* ctx = E->ctx;
+    hipStream_t s = ctx->stream;
+    E->tabs.reset();  // the previous step ended with a synchronisation
+    const bool mixed = E->opts.precision == FCG_PREC_MIXED;
+
+    // ---- A: pending basis scalings, solution updates and starts
+    {
+        std::vector<KEnt> sc;
+        std::vector<double> dvals;
+        for (auto& up : E->active) {
+            Solve& S = *up;
+            if (S.phase != kArn || !S.need_scale) continue;
+            RCCHECK(ensure_basis(ctx, S, S.i + 1));
+            sc.push_back(KEnt{&S, S.scale_from_r ? S.r.as<double>() : S.w.as<double>(), S.i, 0});
+            for (int q = 0; q < S.k; ++q) dvals.push_back(S.d[(size_t)q]);
+        }
+        if (!sc.empty()) {
+            KrylovView kv;
+            RCCHECK(make_krylov(ctx, E->tabs, sc, &kv));
+            void* dd;
+            HIPCHECK(ctx, E->tabs.push(dvals.data(), sizeof(double) * dvals.size(), s, &dd));
+            HIPCHECK(ctx, fcg::launch_scale_new(kv, reinterpret_cast<double*>(dd), s));
+            for (auto& e : sc) e.S->need_scale = false;
+        }
+        std::vector<KEnt> up;
+        std::vector<double> zall;
+        for (auto& u : E->active) {
+            Solve& S = *u;
+            if (S.phase != kUpdate) continue;
+            up.push_back(KEnt{&S, S.w.as<double>(), 0, (int)zall.size()});
+            zall.insert(zall.end(), S.z.begin(), S.z.end());
+        }
+        if (!up.empty()) {
+            KrylovView kv;
+            RCCHECK(make_krylov(ctx, E->tabs, up, &kv));
+            void* dz;
+            HIPCHECK(ctx, E->tabs.push(zall.data(), sizeof(double) * zall.size(), s, &dz));
+            HIPCHECK(ctx, fcg::launch_xupdate(kv, reinterpret_cast<double*>(dz), s));
+            for (auto& e : up) {
+                e.S->nz = 0;
+                e.S->phase = e.S->mixed ? kResid : kFinish;
+            }
+        }
+        std::vector<KEnt> init;
+        for (auto& u : E->active)
+            if (u->phase == kInit) init.push_back(KEnt{u.get(), u->w.as<double>(), 0, 0});
+        if (!init.empty()) {
+            KrylovView kv;
+            RCCHECK(make_krylov(ctx, E->tabs, init, &kv));
+            HIPCHECK(ctx, fcg::launch_init_x(kv, s));
+            for (auto& e : init) e.S->phase = kResid;
+        }
+    }
+    // finishing solves: conductor charge sums (SolveCapacitance.cpp:1305-1366)
+    std::vector<std::pair<Solve*, double*>> fin;
+    std::vector<double*> fin_x;
+    std::vector<std::unique_ptr<PBuf>> fin_tmp;
+    for (auto& u : E->active) {
+        Solve& S = *u;
+        if (S.phase != kFinish) continue;
+        const size_t nval = (size_t)S.k * S.ncol;
+        void* h;
+        HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * nval, &h));
+        fin_tmp.emplace_back(new PBuf());
+        HIPCHECK(ctx, fin_tmp.back()->get(&ctx->pool, sizeof(double) * nval));
+        HIPCHECK(ctx, fcg::launch_charge_sums(S.x.as<double>(), S.J->leaf_w.as<double>(), S.J->col_range.as<int32_t>(),
+                                              S.N, S.k, S.ncol, fin_tmp.back()->as<double>(), s));
+        HIPCHECK(ctx, hipMemcpyAsync(h, fin_tmp.back()->p, sizeof(double) * nval, hipMemcpyDeviceToHost, s));
+        fin.emplace_back(&S, reinterpret_cast<double*>(h));
+        double* hx = nullptr;
+        if (S.want_x) {
+            void* hxv;
+            HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * (size_t)S.k * S.N, &hxv));
+            hx = reinterpret_cast<double*>(hxv);
+            HIPCHECK(ctx, hipMemcpyAsync(hx, S.x.p, sizeof(double) * (size_t)S.k * S.N, hipMemcpyDeviceToHost, s));
+        }
+        fin_x.push_back(hx);
+    }
+
+    // ---- B: refinement residuals r = b - A x
+    std::vector<Solve*> res, arn;
+    for (auto& u : E->active) {
+        if (u->phase == kResid) res.push_back(u.get());
+        if (u->phase == kArn) arn.push_back(u.get());
+    }
+    int hoff = 0;
+    for (Solve* S : res) {
+        S->hoff = hoff;
+        hoff += S->k;
+    }
+    for (Solve* S : arn) {
+        S->hoff = hoff;
+        hoff += (2 * (S->i + 1) + 1) * S->k;
+    }
+    RCCHECK(ensure_pbuf(ctx, E->hbuf, sizeof(double) * (size_t)std::max(hoff, 1)));
+    if (!res.empty()) {
+        std::vector<BatchEntry> be;
+        std::vector<KEnt> ke;
+        for (Solve* S : res) {
+            be.push_back(BatchEntry{S->J, &S->scr, S->x.as<double>(), S->w.as<double>()});
+            ke.push_back(KEnt{S, S->w.as<double>(), 0, 0});
+            S->n_mvp++;
+        }
+        BatchView bv;
+        RCCHECK(make_batch(ctx, E->tabs, be, &bv));
+        const int var = res.size() == 1 ? res[0]->J->gather_variant : 0;
+        HIPCHECK(ctx, fcg::launch_mvp(bv, mixed ? fcg::kMvp32Acc : fcg::kMvp64, 0, var, s));
+        KrylovView kv;
+        RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
+        RCCHECK(ensure_pbuf(ctx, E->partial_a, sizeof(double) * (size_t)kv.nchunks));
+        HIPCHECK(ctx, fcg::launch_resid(kv, E->partial_a.as<double>(), s));
+        HIPCHECK(ctx, fcg::launch_reduce_norm(kv, E->partial_a.as<double>(), E->hbuf.as<double>(), s));
+    }
+
+    // ---- C: one Arnoldi step (product + two Gram-Schmidt passes) of every active cycle
+    if (!arn.empty()) {
+        std::vector<BatchEntry> be;
+        std::vector<KEnt> ke;
+        for (Solve* S : arn) {
+            RCCHECK(ensure_basis(ctx, *S, S->i + 1));
+            be.push_back(BatchEntry{S->J, &S->scr, S->hq[(size_t)S->i], S->w.as<double>()});
+            ke.push_back(KEnt{S, S->w.as<double>(), S->i + 1, 0});
+            S->n_mvp++;
+        }
+        BatchView bv;
+        RCCHECK(make_batch(ctx, E->tabs, be, &bv));
+        const int var = arn.size() == 1 ? arn[0]->J->gather_variant : 0;
+        HIPCHECK(ctx, fcg::launch_mvp(bv, mixed ? fcg::kMvp32 : fcg::kMvp64, arn[0]->jac ? 1 : 0, var, s));
+        KrylovView kv;
+        RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
+        RCCHECK(ensure_pbuf(ctx, E->partial_b, sizeof(double) * (size_t)kv.nchunks * std::max(kv.maxnv, 1)));
+        double* pb = E->partial_b.as<double>();
+        double* hb = E->hbuf.as<double>();
+        HIPCHECK(ctx, fcg::launch_dots(kv, pb, s));
+        HIPCHECK(ctx, fcg::launch_reduce(kv, pb, hb, 0, s));
+        HIPCHECK(ctx, fcg::launch_update(kv, hb, 0, 0, pb, s));
+        HIPCHECK(ctx, fcg::launch_dots(kv, pb, s));
+        HIPCHECK(ctx, fcg::launch_reduce(kv, pb, hb, 1, s));
+        HIPCHECK(ctx, fcg::launch_update(kv, hb, 1, 1, pb, s));
+        HIPCHECK(ctx, fcg::launch_reduce_norm(kv, pb, hb, s));
+    }
+
+    // ---- D: one copy back, one synchronisation
+    double* hh = nullptr;
+    if (hoff > 0) {
+        void* h;
+        HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * (size_t)hoff, &h));
+        hh = reinterpret_cast<double*>(h);
+        HIPCHECK(ctx, hipMemcpyAsync(hh, E->hbuf.p, sizeof(double) * (size_t)hoff, hipMemcpyDeviceToHost, s));
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+
+    // ---- host bookkeeping
+    for (size_t f = 0; f < fin.size(); ++f) {
+        Solve& S = *fin[f].first;
+        Ready rd;
+        rd.tag = S.tag;
+        rd.job = S.job_id;
+        rd.rows = S.k;
+        rd.cols = S.ncol;
+        rd.cap.assign(fin[f].second, fin[f].second + (size_t)S.k * S.ncol);
+        if (fin_x[f]) rd.x.assign(fin_x[f], fin_x[f] + (size_t)S.k * S.N);
+        std::memset(&rd.stats, 0, sizeof rd.stats);
+        rd.stats.n_rhs = S.k;
+        int mx = 0;
+        for (int q = 0; q < S.k; ++q) {
+            if (q < 64) rd.stats.iterations[q] = S.total_steps[(size_t)q];
+            mx = std::max(mx, S.total_steps[(size_t)q]);
+        }
+        rd.stats.max_iterations = mx;
+        rd.stats.final_residual = S.rel;
+        rd.stats.n_mvp = S.n_mvp;
+        rd.stats.t_solve_s = std::chrono::duration<double>(clk::now() - S.t0).count();
+        E->ready.push_back(std::move(rd));
+        S.phase = kDone;
+    }
+    for (Solve* S : res) {
+        std::vector<double> beta((size_t)S->k);
+        double worst = 0.0;
+        bool all = true;
+        for (int q = 0; q < S->k; ++q) {
+            beta[(size_t)q] = std::sqrt(hh[S->hoff + q]);
+            const double rel = beta[(size_t)q] / S->normb[(size_t)q];
+            worst = std::max(worst, rel);
+            if (!(rel < E->opts.tol)) all = false;
+        }
+        if (S->mixed) S->rel = worst;
+        if (S->mixed && (all || S->cycles >= std::max(1, E->opts.max_refine)))
+            S->phase = kFinish;
+        else
+            start_cycle(E, *S, beta.data());
+    }
+    for (Solve* S : arn) {
+        const int nv = S->i + 1;
+        const double* h1 = hh + S->hoff;
+        const double* h2 = h1 + (size_t)nv * S->k;
+        const double* w2 = h2 + (size_t)nv * S->k;
+        givens_step(*S, h1, h2, w2);
+        bool all = true;
+        double worst = 0.0;
+        for (int q = 0; q < S->k; ++q) {
+            if (!S->done[(size_t)q]) all = false;
+            const int it = S->last_i[(size_t)q];
+            if (it >= 0) worst = std::max(worst, std::fabs(S->g[(size_t)q][(size_t)it + 1]) / S->normb[(size_t)q]);
+        }
+        if (!S->mixed) S->rel = worst;
+        if (all || S->i + 1 >= S->cap_iter) {
+            prepare_update(*S);
+        } else {
+            S->i += 1;
+            S->need_scale = true;
+            S->scale_from_r = false;
+        }
+    }
+    // drop finished solves (their device memory returns to the pool)
+    E->active.erase(std::remove_if(E->active.begin(), E->active.end(),
+                                   [](const std::unique_ptr<Solve>& u) { return u->phase == kDone; }),
+                    E->active.end());
     return FCG_OK;
 }
 
@@ -207,9 +982,9 @@ int fcg_ctx_destroy(fcg_ctx* ctx) {
     if (!ctx) return FCG_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    ctx->jobs.clear();
-    (void)hipStreamDestroy(ctx->stream);
+    hipStream_t s = ctx->stream;
     delete ctx;
+    (void)hipStreamDestroy(s);
     return FCG_OK;
 }
 
@@ -217,8 +992,7 @@ const char* fcg_last_error(fcg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null
 
 int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     if (!ctx || !job) return FCG_INVALID_ARGUMENT;
-    int rc = validate_desc(ctx, d);
-    if (rc != FCG_OK) return rc;
+    RCCHECK(validate_desc(ctx, d));
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     auto j = std::make_unique<Job>();
     j->dim = d->dim;
@@ -239,23 +1013,32 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     HIPCHECK(ctx, upload(j->gout, d->group_outperm, (size_t)j->G, s));
     HIPCHECK(ctx, upload(j->gin, d->group_inperm, (size_t)j->G, s));
     HIPCHECK(ctx, upload(j->leaf_w, d->leaf_outperm, (size_t)j->N, s));
-    HIPCHECK(ctx, j->coef.alloc(sizeof(double) * (size_t)(j->L > 0 ? j->L : 1)));
+    HIPCHECK(ctx, j->link_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(j->L, 1)));
     HIPCHECK(ctx, j->self.alloc(sizeof(double) * (size_t)j->N));
     HIPCHECK(ctx, j->diag.alloc(sizeof(double) * (size_t)j->M));
     HIPCHECK(ctx, hipMemsetAsync(j->diag.p, 0, sizeof(double) * (size_t)j->M, s));
     j->group_is_diel.assign(d->group_is_diel, d->group_is_diel + j->G);
     j->group_leaf_start.assign(d->group_leaf_start, d->group_leaf_start + j->G + 1);
-    j->leaf_outperm.assign(d->leaf_outperm, d->leaf_outperm + j->N);
     j->first_cond_leaf = j->N;
     std::vector<int32_t> ranges;
     for (int g = 0; g < j->G; ++g) {
         if (j->group_is_diel[(size_t)g]) continue;
-        if (j->first_cond_leaf == j->N) j->first_cond_leaf = j->group_leaf_start[(size_t)g];
-        ranges.push_back(j->group_leaf_start[(size_t)g]);
-        ranges.push_back(j->group_leaf_start[(size_t)g + 1]);
+        const int lo = j->group_leaf_start[(size_t)g], hi = j->group_leaf_start[(size_t)g + 1];
+        if (j->first_cond_leaf == j->N) j->first_cond_leaf = lo;
+        ranges.push_back(lo);
+        ranges.push_back(hi);
+        ++j->ncol;
+        // 2D: the conductor that ends the leaf list is the balance row (SolveCapacitance.cpp:1233-1240)
+        if (!(j->dim == 2 && hi == j->N)) j->rhs.emplace_back(lo, hi);
     }
     if (ranges.empty()) return fail(ctx, FCG_INVALID_ARGUMENT, "job has no conductor");
     HIPCHECK(ctx, upload(j->col_range, ranges.data(), ranges.size(), s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    // destination node of every link (used by the fill)
+    ctx->tabs.reset();
+    BatchView bv;
+    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{j.get(), &j->scr, nullptr, nullptr}}, &bv));
+    HIPCHECK(ctx, fcg::launch_link_rows(bv, s));
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->jobs.push_back(std::move(j));
     *job = (int)ctx->jobs.size() - 1;
@@ -274,9 +1057,21 @@ int fcg_job_release(fcg_ctx* ctx, int job) {
 int fcg_job_conductors(fcg_ctx* ctx, int job, int* n_cond) {
     Job* j = get_job(ctx, job);
     if (!j || !n_cond) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
-    int n = 0;
-    for (int32_t d : j->group_is_diel) n += d ? 0 : 1;
-    *n_cond = n;
+    *n_cond = j->ncol;
+    return FCG_OK;
+}
+
+int fcg_job_footprint(fcg_ctx* ctx, int job, int precision, int iters, int64_t* resident, int64_t* solve) {
+    Job* j = get_job(ctx, job);
+    if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+    const int64_t k = (int64_t)j->rhs.size();
+    int64_t res = j->resident_bytes();
+    if (precision == FCG_PREC_MIXED && !j->coef32.p) res += 4 * j->L;
+    if (precision != FCG_PREC_MIXED && !j->coef64.p) res += 8 * j->L;
+    if (resident) *resident = res;
+    // vectors b, x, r, w and the basis, plus the solve's own product scratch
+    const int64_t scratch = k * ((int64_t)(j->N + 1) * 16 + (int64_t)j->M * (precision == FCG_PREC_MIXED ? 24 : 16));
+    if (solve) *solve = (int64_t)8 * k * j->N * (4 + std::max(iters, 1) + 1) + scratch;
     return FCG_OK;
 }
 
@@ -287,85 +1082,97 @@ int fcg_fill(fcg_ctx* ctx, int job, int precision) {
         return fail(ctx, FCG_INVALID_ARGUMENT, "unknown precision mode");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    JobView v = j->view();
-    DevBuf link_row;
-    HIPCHECK(ctx, link_row.alloc(sizeof(int32_t) * (size_t)(j->L > 0 ? j->L : 1)));
-    HIPCHECK(ctx, fcg::launch_link_rows(v, link_row.as<int32_t>(), s));
-    HIPCHECK(ctx, fcg::launch_fill(v, link_row.as<int32_t>(), s));
-    HIPCHECK(ctx, fcg::launch_self(v, s));
+    if (precision == FCG_PREC_MIXED && !j->coef32.p)
+        HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)std::max<int64_t>(j->L, 1)));
+    if (precision == FCG_PREC_FP64 && !j->coef64.p)
+        HIPCHECK(ctx, j->coef64.alloc(sizeof(double) * (size_t)std::max<int64_t>(j->L, 1)));
     HIPCHECK(ctx, hipStreamSynchronize(s));
-    j->filled = true;
+    ctx->tabs.reset();
+    BatchView bv;
+    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{j, &j->scr, nullptr, nullptr}}, &bv));
+    HIPCHECK(ctx, fcg::launch_fill(bv, precision, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    if (precision == FCG_PREC_MIXED)
+        j->filled32 = true;
+    else
+        j->filled64 = true;
     return FCG_OK;
 }
 
 int fcg_download_coeffs(fcg_ctx* ctx, int job, double* coef, double* self_coef) {
     Job* j = get_job(ctx, job);
     if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
-    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    if (coef && j->L > 0)
-        HIPCHECK(ctx, hipMemcpyAsync(coef, j->coef.p, sizeof(double) * (size_t)j->L, hipMemcpyDeviceToHost,
-                                     ctx->stream));
-    if (self_coef)
-        HIPCHECK(ctx, hipMemcpyAsync(self_coef, j->self.p, sizeof(double) * (size_t)j->N, hipMemcpyDeviceToHost,
-                                     ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    if (coef && j->L > 0) {
+        if (j->filled64) {
+            HIPCHECK(ctx, hipMemcpy(coef, j->coef64.p, sizeof(double) * (size_t)j->L, hipMemcpyDeviceToHost));
+        } else {
+            std::vector<float> tmp((size_t)j->L);
+            HIPCHECK(ctx, hipMemcpy(tmp.data(), j->coef32.p, sizeof(float) * (size_t)j->L, hipMemcpyDeviceToHost));
+            for (int64_t l = 0; l < j->L; ++l) coef[l] = tmp[(size_t)l];
+        }
+    }
+    if (self_coef) HIPCHECK(ctx, hipMemcpy(self_coef, j->self.p, sizeof(double) * (size_t)j->N, hipMemcpyDeviceToHost));
     return FCG_OK;
 }
 
 int fcg_mvp_device(fcg_ctx* ctx, int job, const double* d_q, double* d_v, int k, void* stream) {
     Job* j = get_job(ctx, job);
     if (!j || !d_q || !d_v || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    int rc = ensure_scratch(ctx, *j, k);
-    if (rc != FCG_OK) return rc;
-    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
-    HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, d_q, d_v, k, 0, s));
-    return FCG_OK;
+    return run_mvp(ctx, *j, fcg::kMvp64, d_q, d_v, k, 0, stream ? (hipStream_t)stream : ctx->stream);
 }
 
-int fcg_mvp(fcg_ctx* ctx, int job, const double* q, double* v, int k) {
+int fcg_mvp_prec(fcg_ctx* ctx, int job, int mvp_prec, const double* q, double* v, int k) {
     Job* j = get_job(ctx, job);
     if (!j || !q || !v || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     DevBuf dq, dv;
     const size_t n = (size_t)j->N * k;
     HIPCHECK(ctx, upload(dq, q, n, ctx->stream));
     HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
-    int rc = fcg_mvp_device(ctx, job, dq.as<double>(), dv.as<double>(), k, nullptr);
-    if (rc != FCG_OK) return rc;
+    RCCHECK(run_mvp(ctx, *j, mvp_prec, dq.as<double>(), dv.as<double>(), k, 0, ctx->stream));
     HIPCHECK(ctx, hipMemcpyAsync(v, dv.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     return FCG_OK;
 }
 
+int fcg_mvp(fcg_ctx* ctx, int job, const double* q, double* v, int k) {
+    return fcg_mvp_prec(ctx, job, FCG_MVP_FP64, q, v, k);
+}
+
+namespace {
+struct Events5 {
+    hipEvent_t e[5] = {};
+    ~Events5() {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+};
+}  // namespace
+
 int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* out) {
     Job* j = get_job(ctx, job);
     if (!j || !out || k < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    int rc = ensure_scratch(ctx, *j, k);
-    if (rc != FCG_OK) return rc;
+    const int prec = j->filled64 ? fcg::kMvp64 : fcg::kMvp32;
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)j->N * k;
     DevBuf dq, dv;
     HIPCHECK(ctx, dq.alloc(sizeof(double) * n));
     HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
     HIPCHECK(ctx, fcg::launch_random(dq.as<double>(), n, 12345, s));
-    struct Ev {
-        hipEvent_t e[5] = {};
-        ~Ev() {
-            for (auto x : e)
-                if (x) (void)hipEventDestroy(x);
-        }
-    } ev;
+    Events5 ev;
     for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
-    // one untimed product to warm caches and code objects
-    HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s));
+    RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s));
     double acc[4] = {0, 0, 0, 0}, tot = 0;
     for (int r = 0; r < reps; ++r) {
-        HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
+        RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
         HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
         for (int st = 0; st < 4; ++st) {
             float ms = 0;
@@ -394,39 +1201,32 @@ int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants) 
     if (variant < 0) return FCG_OK;  // query only
     if (variant >= fcg::gather_variant_count()) return fail(ctx, FCG_INVALID_ARGUMENT, "bad gather variant");
     j->gather_variant = variant;
-    j->tuned_k = -1;  // pinned by the caller: fcg_solve will not re-tune
+    j->tuned_k = -1;  // pinned by the caller
     return FCG_OK;
 }
 
 int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* best_ms) {
     Job* j = get_job(ctx, job);
     if (!j || k < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    int rc = ensure_scratch(ctx, *j, k);
-    if (rc != FCG_OK) return rc;
+    const int prec = j->filled64 ? fcg::kMvp64 : fcg::kMvp32;
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)j->N * k;
     DevBuf dq, dv;
     HIPCHECK(ctx, dq.alloc(sizeof(double) * n));
     HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
     HIPCHECK(ctx, fcg::launch_random(dq.as<double>(), n, 777, s));
-    struct Ev {
-        hipEvent_t e[5] = {};
-        ~Ev() {
-            for (auto x : e)
-                if (x) (void)hipEventDestroy(x);
-        }
-    } ev;
+    Events5 ev;
     for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
     int best = 0;
     double best_t = 1e300;
     for (int var = 0; var < fcg::gather_variant_count(); ++var) {
         j->gather_variant = var;
-        HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s));
+        RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s));
         double t = 0.0;
         for (int r = 0; r < reps; ++r) {
-            HIPCHECK(ctx, fcg::launch_mvp(j->view(), j->sc, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
+            RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
             HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
             float ms = 0;
             HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
@@ -448,291 +1248,131 @@ int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* b
 int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k) {
     Job* j = get_job(ctx, job);
     if (!j || !q || !pq || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     DevBuf dq, dp;
     const size_t n = (size_t)j->N * k;
     HIPCHECK(ctx, upload(dq, q, n, ctx->stream));
     HIPCHECK(ctx, dp.alloc(sizeof(double) * n));
-    HIPCHECK(ctx, fcg::launch_jacobi(j->view(), dq.as<double>(), dp.as<double>(), k, ctx->stream));
+    HIPCHECK(ctx, fcg::launch_jacobi(dq.as<double>(), j->self.as<double>(), j->N, k, dp.as<double>(), ctx->stream));
     HIPCHECK(ctx, hipMemcpyAsync(pq, dp.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     return FCG_OK;
 }
 
-// Right-preconditioned GMRES on all conductor right-hand sides at once
-// (gmresPrecondSFastAll, Solver/SolveCapacitance.cpp:2288-2574): x0 = P b,
-// r0 = b - A x0, Arnoldi with classical Gram-Schmidt and one re-orthogonalisation
-// pass, Givens rotations per right-hand side on the host, the reference stop rule
-// |g_{i+1}| / |b| < tol per right-hand side, x = x0 + P (Q z).
-int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, int* rows_out, int* cols_out,
-              double* charges, fcg_solve_stats* stats) {
-    using clk = std::chrono::steady_clock;
-    Job* j = get_job(ctx, job);
-    if (!j || !opts || !cap) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (!j->filled) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+int fcg_engine_create(fcg_ctx* ctx, const fcg_solve_opts* opts, fcg_engine** out) {
+    if (!ctx || !opts || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     if (opts->precond != FCG_PRECOND_NONE && opts->precond != FCG_PRECOND_JACOBI)
         return fail(ctx, FCG_INVALID_ARGUMENT, "unsupported preconditioner");
-    HIPCHECK(ctx, hipSetDevice(ctx->device));
-    hipStream_t s = ctx->stream;
-    const int N = j->N;
-    const bool jac = opts->precond == FCG_PRECOND_JACOBI;
-    auto t_start = clk::now();
+    if (opts->precision != FCG_PREC_FP64 && opts->precision != FCG_PREC_MIXED)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "unknown precision mode");
+    if (!(opts->tol > 0.0)) return fail(ctx, FCG_INVALID_ARGUMENT, "tolerance must be positive");
+    auto e = std::make_unique<fcg_engine>();
+    e->ctx = ctx;
+    e->opts = *opts;
+    *out = e.release();
+    return FCG_OK;
+}
 
-    // right-hand sides: one per conductor; 2D skips the conductor that ends the leaf list
-    std::vector<std::pair<int, int>> rhs;  // leaf range raised to 1 V
-    int ncol = 0;
-    for (int g = 0; g < j->G; ++g) {
-        if (j->group_is_diel[(size_t)g]) continue;
-        ++ncol;
-        const int lo = j->group_leaf_start[(size_t)g], hi = j->group_leaf_start[(size_t)g + 1];
-        if (j->dim == 2 && hi == N) continue;
-        rhs.emplace_back(lo, hi);
-    }
-    const int k = (int)rhs.size();
-    if (k == 0) return fail(ctx, FCG_INVALID_ARGUMENT, "no conductor right-hand side");
-    int cap_iter = opts->max_iter > 0 ? opts->max_iter : 1000;
-    if (cap_iter > N) cap_iter = N;
-    int rc = ensure_scratch(ctx, *j, k);
-    if (rc != FCG_OK) return rc;
-    if (j->tuned_k != k && j->tuned_k >= 0 && (rc = fcg_autotune(ctx, job, k, 3, nullptr, nullptr)) != FCG_OK)
-        return rc;
-    JobView v = j->view();
-    const size_t NK = (size_t)N * k;
+int fcg_engine_submit(fcg_engine* eng, int job, int64_t tag) {
+    if (!eng) return FCG_INVALID_ARGUMENT;
+    HIPCHECK(eng->ctx, hipSetDevice(eng->ctx->device));
+    return engine_submit(eng, job, tag);
+}
 
-    DevBuf b, x0, r, w, tmp, part, hbuf, nrm, dz, dcap;
-    HIPCHECK(ctx, b.alloc(sizeof(double) * NK));
-    HIPCHECK(ctx, x0.alloc(sizeof(double) * NK));
-    HIPCHECK(ctx, r.alloc(sizeof(double) * NK));
-    HIPCHECK(ctx, w.alloc(sizeof(double) * NK));
-    HIPCHECK(ctx, tmp.alloc(sizeof(double) * NK));
-    const int nchunks = fcg::dots_blocks(N);
-    {
-        std::vector<double> hb(NK, 0.0);
-        for (int q = 0; q < k; ++q)
-            for (int n = rhs[(size_t)q].first; n < rhs[(size_t)q].second; ++n) hb[(size_t)q * N + n] = 1.0;
-        HIPCHECK(ctx, hipMemcpyAsync(b.p, hb.data(), sizeof(double) * NK, hipMemcpyHostToDevice, s));
-    }
-    // Krylov basis: [k][N] vectors allocated in chunks; device table of their addresses
-    constexpr int kChunk = 64;
-    std::vector<std::unique_ptr<DevBuf>> chunks;
-    std::vector<double*> hptrs;
-    hptrs.reserve((size_t)cap_iter + 2 + kChunk);
-    DevBuf qptrs;
-    HIPCHECK(ctx, qptrs.alloc(sizeof(double*) * (size_t)(cap_iter + 2 + kChunk)));
-    auto ensure_basis = [&](size_t count) -> int {
-        while (hptrs.size() < count) {
-            chunks.emplace_back(new DevBuf());
-            HIPCHECK(ctx, chunks.back()->alloc(sizeof(double) * NK * kChunk));
-            const size_t base = hptrs.size();
-            for (int c = 0; c < kChunk; ++c) hptrs.push_back(chunks.back()->as<double>() + (size_t)c * NK);
-            // new table entries are not read by any enqueued kernel yet
-            HIPCHECK(ctx, hipMemcpy(qptrs.as<double*>() + base, hptrs.data() + base, sizeof(double*) * kChunk,
-                                    hipMemcpyHostToDevice));
-        }
-        return FCG_OK;
-    };
-    const int maxvals = (cap_iter + 2) * k;
-    // h1 | h2 | |w|^2 of one Arnoldi step, contiguous so that one copy brings them back
-    HIPCHECK(ctx, part.alloc(sizeof(double) * (size_t)maxvals * nchunks));
-    HIPCHECK(ctx, hbuf.alloc(sizeof(double) * ((size_t)maxvals * 2 + k)));
-    HIPCHECK(ctx, nrm.alloc(sizeof(double) * (size_t)k * 2));
-    PinnedBuf hstage;
-    HIPCHECK(ctx, hstage.alloc(sizeof(double) * ((size_t)maxvals * 2 + 2 * k)));
-    int64_t n_mvp = 0;
-    double t_mvp = 0.0;
-    struct Events {
-        hipEvent_t a = nullptr, b = nullptr;
-        ~Events() {
-            if (a) (void)hipEventDestroy(a);
-            if (b) (void)hipEventDestroy(b);
-        }
-    } evs;
-    HIPCHECK(ctx, hipEventCreate(&evs.a));
-    HIPCHECK(ctx, hipEventCreate(&evs.b));
-    hipEvent_t ev0 = evs.a, ev1 = evs.b;
+int fcg_engine_step(fcg_engine* eng, int* active, int* ready) {
+    if (!eng) return FCG_INVALID_ARGUMENT;
+    HIPCHECK(eng->ctx, hipSetDevice(eng->ctx->device));
+    if (!eng->active.empty()) RCCHECK(engine_step(eng));
+    if (active) *active = (int)eng->active.size();
+    if (ready) *ready = (int)eng->ready.size();
+    return FCG_OK;
+}
 
-    // squared norms of the k vectors of `vec` into out (synchronises the stream)
-    DevBuf selfptr;
-    HIPCHECK(ctx, selfptr.alloc(sizeof(double*)));
-    double** norm_src = reinterpret_cast<double**>(hstage.as<double>() + (size_t)maxvals * 2 + k);
-    auto norms = [&](const double* vec, std::vector<double>& out) -> int {
-        HIPCHECK(ctx, hipStreamSynchronize(s));
-        *norm_src = const_cast<double*>(vec);
-        HIPCHECK(ctx, hipMemcpyAsync(selfptr.p, norm_src, sizeof(double*), hipMemcpyHostToDevice, s));
-        HIPCHECK(ctx, fcg::launch_block_dots(selfptr.as<double* const>(), 1, vec, N, k, part.as<double>(), nchunks, s));
-        HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, k, nrm.as<double>(), s));
-        HIPCHECK(ctx, hipMemcpyAsync(hstage.p, nrm.p, sizeof(double) * k, hipMemcpyDeviceToHost, s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));
-        out.assign(hstage.as<double>(), hstage.as<double>() + k);
-        for (auto& x : out) x = std::sqrt(x);
-        return FCG_OK;
-    };
-    // divide the k vectors of `in` by the values `d` (host) into `out`
-    auto scale_by = [&](const double* in, const std::vector<double>& d, double* out) -> int {
-        HIPCHECK(ctx, hipStreamSynchronize(s));
-        double* st = hstage.as<double>() + (size_t)maxvals * 2;
-        for (int q = 0; q < k; ++q) st[q] = d[(size_t)q];
-        HIPCHECK(ctx, hipMemcpyAsync(nrm.p, st, sizeof(double) * k, hipMemcpyHostToDevice, s));
-        HIPCHECK(ctx, fcg::launch_scale(in, nrm.as<double>(), out, N, k, s));
-        return FCG_OK;
-    };
+int fcg_engine_poll(fcg_engine* eng, int64_t* tag, int* job, double* cap, int cap_len, int* rows, int* cols,
+                    fcg_solve_stats* stats) {
+    if (!eng) return FCG_INVALID_ARGUMENT;
+    if (eng->ready.empty()) return FCG_INVALID_ARGUMENT;
+    Ready& r = eng->ready.front();
+    if (cap && cap_len < (int)r.cap.size()) return fail(eng->ctx, FCG_INVALID_ARGUMENT, "result buffer too small");
+    if (tag) *tag = r.tag;
+    if (job) *job = r.job;
+    if (cap) std::memcpy(cap, r.cap.data(), sizeof(double) * r.cap.size());
+    if (rows) *rows = r.rows;
+    if (cols) *cols = r.cols;
+    if (stats) *stats = r.stats;
+    eng->ready.pop_front();
+    return FCG_OK;
+}
 
-    std::vector<double> normb, normr;
-    if (jac) {
-        HIPCHECK(ctx, fcg::launch_jacobi(v, b.as<double>(), x0.as<double>(), k, s));
-        HIPCHECK(ctx, fcg::launch_mvp(v, j->sc, x0.as<double>(), w.as<double>(), k, 0, s));
-        ++n_mvp;
-        HIPCHECK(ctx, fcg::launch_sub(b.as<double>(), w.as<double>(), r.as<double>(), N, k, s));
-        if ((rc = norms(r.as<double>(), normr)) != FCG_OK) return rc;
-        if ((rc = norms(b.as<double>(), normb)) != FCG_OK) return rc;
-    } else {
-        HIPCHECK(ctx, hipMemsetAsync(x0.p, 0, sizeof(double) * NK, s));
-        HIPCHECK(ctx, hipMemcpyAsync(r.p, b.p, sizeof(double) * NK, hipMemcpyDeviceToDevice, s));
-        if ((rc = norms(r.as<double>(), normr)) != FCG_OK) return rc;
-        normb = normr;
-    }
-    if ((rc = ensure_basis(1)) != FCG_OK) return rc;
-    if ((rc = scale_by(r.as<double>(), normr, hptrs[0])) != FCG_OK) return rc;
+int fcg_engine_bytes(fcg_engine* eng, int64_t* bytes) {
+    if (!eng || !bytes) return FCG_INVALID_ARGUMENT;
+    int64_t b = 0;
+    for (auto& s : eng->active) b += s->bytes();
+    *bytes = b;
+    return FCG_OK;
+}
 
-    // per-rhs Givens state
-    std::vector<std::vector<double>> H((size_t)k);  // column i at H[r][i*ld .. i*ld+i]
-    std::vector<std::vector<double>> cs((size_t)k), sn((size_t)k), g((size_t)k);
-    std::vector<int> done((size_t)k, 0), iters((size_t)k, -1);
-    std::vector<double> rel((size_t)k, 0.0);
-    const int ld = cap_iter + 1;
-    for (int q = 0; q < k; ++q) {
-        g[(size_t)q].assign((size_t)cap_iter + 2, 0.0);
-        g[(size_t)q][0] = normr[(size_t)q];
-        rel[(size_t)q] = normr[(size_t)q] / normb[(size_t)q];
-        if (rel[(size_t)q] < opts->tol) done[(size_t)q] = 1;  // the initial guess is good enough
+int fcg_engine_destroy(fcg_engine* eng) {
+    if (!eng) return FCG_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(eng->ctx->stream);
+    delete eng;
+    return FCG_OK;
+}
+
+namespace {
+// run an engine over `jobs` to completion; results in job order
+int run_batch(fcg_ctx* ctx, const int* jobs, int njobs, const fcg_solve_opts* opts, bool want_x, double* caps,
+              int* rows, int* cols, fcg_solve_stats* stats, double* charges) {
+    fcg_engine* e = nullptr;
+    RCCHECK(fcg_engine_create(ctx, opts, &e));
+    std::unique_ptr<fcg_engine> guard(e);
+    e->want_x = want_x;
+    std::vector<size_t> off((size_t)njobs + 1, 0);
+    for (int i = 0; i < njobs; ++i) {
+        Job* j = get_job(ctx, jobs[i]);
+        if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+        off[(size_t)i + 1] = off[(size_t)i] + j->rhs.size() * (size_t)j->ncol;
     }
-    auto all_done = [&]() {
-        for (int d : done)
-            if (!d) return false;
-        return true;
-    };
-    std::vector<double> hn((size_t)k);
-    int i = 0;
-    for (i = 0; i < cap_iter && !all_done(); ++i) {
-        const int nv = i + 1;
-        if ((rc = ensure_basis((size_t)nv + 1)) != FCG_OK) return rc;
-        double* h1 = hbuf.as<double>();
-        double* h2 = h1 + (size_t)nv * k;
-        double* w2 = h2 + (size_t)nv * k;
-        // w = A P q_i
-        HIPCHECK(ctx, hipEventRecord(ev0, s));
-        HIPCHECK(ctx, fcg::launch_mvp(v, j->sc, hptrs[(size_t)i], w.as<double>(), k, jac ? 1 : 0, s));
-        HIPCHECK(ctx, hipEventRecord(ev1, s));
-        ++n_mvp;
-        // classical Gram-Schmidt, two passes; the second pass also returns |w|^2
-        HIPCHECK(ctx, fcg::launch_block_dots(qptrs.as<double* const>(), nv, w.as<double>(), N, k, part.as<double>(),
-                                             nchunks, s));
-        HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, nv * k, h1, s));
-        HIPCHECK(ctx, fcg::launch_basis_update(qptrs.as<double* const>(), nv, h1, w.as<double>(), N, k, s));
-        HIPCHECK(ctx, fcg::launch_block_dots(qptrs.as<double* const>(), nv, w.as<double>(), N, k, part.as<double>(),
-                                             nchunks, s));
-        HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, nv * k, h2, s));
-        HIPCHECK(ctx, fcg::launch_basis_update_norm(qptrs.as<double* const>(), nv, h2, w.as<double>(), N, k,
-                                                    part.as<double>(), nchunks, s));
-        HIPCHECK(ctx, fcg::launch_reduce_partials(part.as<double>(), nchunks, k, w2, s));
-        HIPCHECK(ctx, hipMemcpyAsync(hstage.p, h1, sizeof(double) * ((size_t)2 * nv * k + k), hipMemcpyDeviceToHost,
-                                     s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));
-        {
-            float ms = 0.0f;
-            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev0, ev1));
-            t_mvp += 1e-3 * ms;
+    for (int i = 0; i < njobs; ++i) RCCHECK(fcg_engine_submit(e, jobs[i], i));
+    int got = 0;
+    while (got < njobs) {
+        int active = 0, ready = 0;
+        RCCHECK(fcg_engine_step(e, &active, &ready));
+        while (!e->ready.empty()) {
+            Ready& r = e->ready.front();
+            const int i = (int)r.tag;
+            std::memcpy(caps + off[(size_t)i], r.cap.data(), sizeof(double) * r.cap.size());
+            if (rows) rows[i] = r.rows;
+            if (cols) cols[i] = r.cols;
+            if (stats) stats[i] = r.stats;
+            if (charges && !r.x.empty()) std::memcpy(charges, r.x.data(), sizeof(double) * r.x.size());
+            e->ready.pop_front();
+            ++got;
         }
-        const double* hh1 = hstage.as<double>();
-        const double* hh2 = hh1 + (size_t)nv * k;
-        const double* hw2 = hh2 + (size_t)nv * k;
-        for (int q = 0; q < k; ++q) hn[(size_t)q] = std::sqrt(hw2[q]);
-        for (int q = 0; q < k; ++q) {
-            if (done[(size_t)q]) continue;
-            std::vector<double>& Hq = H[(size_t)q];
-            if (Hq.size() < (size_t)(i + 1) * ld) Hq.resize((size_t)(i + 1) * ld, 0.0);
-            double* colv = Hq.data() + (size_t)i * ld;
-            for (int jj = 0; jj <= i; ++jj) colv[jj] = hh1[(size_t)jj * k + q] + hh2[(size_t)jj * k + q];
-            auto& c = cs[(size_t)q];
-            auto& sv = sn[(size_t)q];
-            for (int jj = 0; jj < i; ++jj) {
-                const double t1 = colv[jj], t2 = colv[jj + 1];
-                colv[jj] = c[(size_t)jj] * t1 - sv[(size_t)jj] * t2;
-                colv[jj + 1] = c[(size_t)jj] * t2 + sv[(size_t)jj] * t1;
-            }
-            const double t1 = colv[i], t2 = hn[(size_t)q];
-            const double len = std::sqrt(t1 * t1 + t2 * t2);
-            c.push_back(t1 / len);
-            sv.push_back(-t2 / len);
-            colv[i] = c[(size_t)i] * t1 - sv[(size_t)i] * t2;
-            const double gi = g[(size_t)q][(size_t)i];
-            g[(size_t)q][(size_t)i] = c[(size_t)i] * gi;
-            g[(size_t)q][(size_t)i + 1] = sv[(size_t)i] * gi;
-            rel[(size_t)q] = std::fabs(g[(size_t)q][(size_t)i + 1]) / normb[(size_t)q];
-            iters[(size_t)q] = i;
-            if (rel[(size_t)q] < opts->tol) done[(size_t)q] = 1;
-        }
-        if (all_done()) break;
-        if ((rc = scale_by(w.as<double>(), hn, hptrs[(size_t)i + 1])) != FCG_OK) return rc;
-    }
-    // back substitution per rhs, z[jj][r] (zero beyond each rhs's own count)
-    const int nvec_total = (int)hptrs.size();
-    std::vector<double> z((size_t)nvec_total * k, 0.0);
-    int max_it = 0;
-    for (int q = 0; q < k; ++q) {
-        const int it = iters[(size_t)q];
-        if (it < 0) continue;
-        if (it + 1 > max_it) max_it = it + 1;
-        std::vector<double> zz(g[(size_t)q].begin(), g[(size_t)q].begin() + it + 1);
-        const std::vector<double>& Hq = H[(size_t)q];
-        for (int ii = it; ii >= 0; --ii) {
-            zz[(size_t)ii] = zz[(size_t)ii] / Hq[(size_t)ii * ld + ii];
-            for (int jj = ii - 1; jj >= 0; --jj) zz[(size_t)jj] -= Hq[(size_t)ii * ld + jj] * zz[(size_t)ii];
-        }
-        for (int jj = 0; jj <= it; ++jj) z[(size_t)jj * k + q] = zz[(size_t)jj];
-    }
-    HIPCHECK(ctx, dz.alloc(sizeof(double) * z.size()));
-    HIPCHECK(ctx, hipMemcpyAsync(dz.p, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice, s));
-    HIPCHECK(ctx, hipMemsetAsync(tmp.p, 0, sizeof(double) * NK, s));
-    if (max_it > 0)
-        HIPCHECK(ctx, fcg::launch_basis_combine(qptrs.as<double* const>(), max_it, dz.as<double>(), tmp.as<double>(),
-                                                N, k, s));
-    // x = x0 + P y
-    double* xout = r.as<double>();
-    if (jac) {
-        HIPCHECK(ctx, fcg::launch_jacobi(v, tmp.as<double>(), xout, k, s));
-    } else {
-        HIPCHECK(ctx, hipMemcpyAsync(xout, tmp.p, sizeof(double) * NK, hipMemcpyDeviceToDevice, s));
-    }
-    HIPCHECK(ctx, fcg::launch_add(x0.as<double>(), xout, xout, N, k, s));
-    HIPCHECK(ctx, dcap.alloc(sizeof(double) * (size_t)k * ncol));
-    HIPCHECK(ctx, fcg::launch_charge_sums(xout, j->leaf_w.as<double>(), j->col_range.as<int32_t>(), N, k, ncol,
-                                          dcap.as<double>(), s));
-    HIPCHECK(ctx, hipMemcpyAsync(cap, dcap.p, sizeof(double) * (size_t)k * ncol, hipMemcpyDeviceToHost, s));
-    if (charges)
-        HIPCHECK(ctx, hipMemcpyAsync(charges, xout, sizeof(double) * NK, hipMemcpyDeviceToHost, s));
-    HIPCHECK(ctx, hipStreamSynchronize(s));
-    if (rows_out) *rows_out = k;
-    if (cols_out) *cols_out = ncol;
-    if (stats) {
-        std::memset(stats, 0, sizeof *stats);
-        stats->n_rhs = k;
-        double worst = 0.0;
-        int mx = 0;
-        for (int q = 0; q < k; ++q) {
-            const int it = iters[(size_t)q] + 1;
-            if (q < 64) stats->iterations[q] = it;
-            if (it > mx) mx = it;
-            if (rel[(size_t)q] > worst) worst = rel[(size_t)q];
-        }
-        stats->max_iterations = mx;
-        stats->final_residual = worst;
-        stats->n_mvp = n_mvp;
-        stats->t_mvp_s = t_mvp;
-        stats->t_solve_s = std::chrono::duration<double>(clk::now() - t_start).count();
+        if (active == 0 && got < njobs) return fail(ctx, FCG_GENERIC_ERROR, "batch ended with missing results");
     }
     return FCG_OK;
+}
+}  // namespace
+
+int fcg_solve_batch(fcg_ctx* ctx, const int* jobs, int njobs, const fcg_solve_opts* opts, double* caps, int* rows,
+                    int* cols, fcg_solve_stats* stats) {
+    if (!ctx || !jobs || njobs < 1 || !opts || !caps) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    return run_batch(ctx, jobs, njobs, opts, false, caps, rows, cols, stats, nullptr);
+}
+
+int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, int* rows_out, int* cols_out,
+              double* charges, fcg_solve_stats* stats) {
+    Job* j = get_job(ctx, job);
+    if (!j || !opts || !cap) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    const bool mixed = opts->precision == FCG_PREC_MIXED;
+    const int k = (int)j->rhs.size();
+    if (k > 0 && j->tuned_k == 0 && (mixed ? j->filled32 : j->filled64))
+        RCCHECK(fcg_autotune(ctx, job, k, 3, nullptr, nullptr));
+    return run_batch(ctx, &job, 1, opts, charges != nullptr, cap, rows_out, cols_out, stats, charges);
 }
 
 }  // extern "C"

@@ -1,6 +1,10 @@
-// gfx950 kernels of the FasterCap solve path.  Host-side launch wrappers are
-// declared here and implemented in kernels.hip; the C-ABI (fcg_api.hip) owns the
-// memory and calls them on the context stream.
+// gfx950 kernels of the FasterCap solve path, batched over jobs.
+//
+// Every kernel works on a batch of jobs described by a device table of BJob
+// records and per-kernel work offsets (cumulative work units per job, searched
+// with a short binary search).  A batch of one job is the single-solve path.
+// Host-side launch wrappers are declared here and implemented in kernels.hip;
+// the C-ABI (fcg_api.hip) owns the memory and the tables.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -15,82 +19,132 @@ struct DD {
     double hi, lo;
 };
 
-// Device view of one job (all pointers are device memory).
-struct JobView {
-    int dim, N, M, G;
+constexpr int kScanTile = 2048;  // elements per scan block (256 threads x 8)
+constexpr int kChunk = 4096;     // Krylov vector chunk per block
+constexpr int kNumXcd = 8;       // MI355X accelerator complex dies
+
+// Precision of one hierarchical product.
+enum MvpPrec : int {
+    kMvp64 = 0,     // FP64 coefficients, FP64 charges and accumulation
+    kMvp32 = 1,     // FP32 coefficients, FP32 charges and accumulation (inner solves)
+    kMvp32Acc = 2,  // FP32 coefficients, FP64 charges and accumulation (refinement residual)
+};
+
+// Device record of one job in a batch.
+struct BJob {
+    int N, M, k, dim;
+    int first_cond_leaf;  // 2D zero-total-charge fix-up start
+    int nblk;             // scan tiles per right-hand side
     int64_t L;
     const fcg_node_geom* nodes;
-    const int32_t* left;
-    const int32_t* right;
-    const int32_t* parent;
-    const int32_t* leaf_lo;
-    const int32_t* leaf_hi;
-    const int32_t* leaf_node;
-    const int32_t* node_depth;
     const int64_t* row_ptr;
     const int32_t* col;
-    const double* group_outperm;
-    const double* group_inperm;
-    double* coef;    // [L]
-    double* self;    // [N]
-    double* diag;    // [M] self coefficient of leaf nodes, 0 elsewhere
-    int first_cond_leaf;  // 2D zero-total-charge fix-up start
-    int gather_variant;   // index into the gather configuration table
+    const int32_t* link_row;  // [L] destination node of each link (fill only)
+    double* coef64;           // [L] or null
+    float* coef32;            // [L] or null
+    double* diag;             // [M] self coefficient of leaves, 0 elsewhere
+    double* self;             // [N]
+    const int32_t* left;
+    const int32_t* leaf_lo;
+    const int32_t* leaf_hi;
+    const int32_t* parent;
+    const int32_t* leaf_node;
+    const double* gout;
+    const double* gin;
+    // product scratch sized for k right-hand sides
+    DD* prefix;      // [k][N+1]
+    DD* block_sums;  // [k][nblk]
+    double* charge64;
+    float* charge32;  // [M][k]
+    double* pot64;
+    float* pot32;  // [M][k]
+    double* lt;    // [2k] 2D fix-up scratch
+    // current product operands ([k][N] each)
+    const double* q;
+    double* v;
 };
 
-// Scratch of one hierarchical product on k right-hand sides.
-struct MvpScratch {
-    DD* prefix;       // [k][N+1]
-    DD* block_sums;   // [k][nblk]
-    double* charge;   // [M][k]
-    double* pot;      // [M][k]
-    int nblk;
-    int kmax;
+// Work offsets of a batch: cumulative work units per job (length njobs + 1).
+struct BatchView {
+    const BJob* jobs;
+    int njobs;
+    const int64_t* link_off;  // links
+    const int64_t* tile_off;  // k * nblk scan tiles
+    const int64_t* node_off;  // nodes
+    const int64_t* rowb_off;  // gather blocks (4 rows each)
+    const int64_t* leaf_off;  // leaves
+    const int64_t* rhs_off;   // right-hand sides
+    int64_t links, tiles, nodes, rowb, leaves, rhs;
+    int kmax;  // largest k in the batch
+    int any2d;
 };
 
-constexpr int kScanTile = 2048;  // elements per scan block (256 threads x 8)
+// Krylov bookkeeping of one job for the batched GMRES kernels.
+struct KJob {
+    double* const* Q;    // device array of basis vector addresses ([k][N] each)
+    double* w;           // work vector [k][N]
+    double* x;           // current solution [k][N]
+    double* r;           // residual [k][N]
+    const double* b;     // right-hand sides [k][N]
+    const double* self;  // self coefficients [N] (Jacobi)
+    int N, k, nv;        // nv = basis vectors to project against this step
+    int hoff;            // offset of this job's block in the step buffer
+    int zoff, nz;        // solution-update coefficients z[zoff + jj*k + r], jj < nz
+    int jacobi;
+};
 
-// Fill
-hipError_t launch_link_rows(const JobView& j, int32_t* link_row, hipStream_t s);
-hipError_t launch_fill(const JobView& j, const int32_t* link_row, hipStream_t s);
-hipError_t launch_self(const JobView& j, hipStream_t s);
+// Segment (job, rhs) and chunk tables for the Krylov kernels.
+struct Seg {
+    int job, r, c_lo, c_hi;  // chunk range [c_lo, c_hi)
+};
+struct Chunk {
+    int seg, n0, n1;
+};
+struct KrylovView {
+    const KJob* kjobs;
+    const Seg* segs;
+    const Chunk* chunks;
+    int nsegs, nchunks, maxnv;
+};
 
-// Product v = P q on k vectors ([k][N] layout). If jacobi != 0 the input is
-// first divided by the self coefficients (right preconditioning fused in).
-// `ev`, when given, holds 5 events recorded before the scan and after the scan,
-// the charges, the gather and the down-pass (stage timing).
-hipError_t launch_mvp(const JobView& j, MvpScratch& sc, const double* q, double* v, int k, int jacobi,
-                      hipStream_t s, hipEvent_t* ev = nullptr);
-// Deterministic pseudo-random fill of n doubles in [-1, 1).
-hipError_t launch_random(double* out, size_t n, uint64_t seed, hipStream_t s);
+// fill of every link and self term of the batch; mixed stores FP32 coefficients
+hipError_t launch_link_rows(const BatchView& b, hipStream_t s);
+hipError_t launch_fill(const BatchView& b, int precision, hipStream_t s);
 
-// Elementwise / Krylov helpers on [k][N] blocks.
-hipError_t launch_jacobi(const JobView& j, const double* q, double* pq, int k, hipStream_t s);
-// partial[b][jj][r] = sum over block b of Q_jj[r][n] * w[r][n]; Q_jj = basis[jj] ([k][N])
-hipError_t launch_block_dots(double* const* basis, int nvec, const double* w, int N, int k, double* partial,
-                             int nblocks, hipStream_t s);
-hipError_t launch_reduce_partials(const double* partial, int nblocks, int nvals, double* out, hipStream_t s);
-// w[r][n] -= sum_jj h[jj][r] * Q_jj[r][n]
-hipError_t launch_basis_update(double* const* basis, int nvec, const double* h, double* w, int N, int k,
-                               hipStream_t s);
-// Same update, plus partial[r][chunk] = sum over the chunk of the updated w^2.
-hipError_t launch_basis_update_norm(double* const* basis, int nvec, const double* h, double* w, int N, int k,
-                                    double* partial, int nchunks, hipStream_t s);
-// out[r][n] = a[r][n] / d[r] (0 where d[r] == 0)
-hipError_t launch_scale(const double* a, const double* scale, double* out, int N, int k, hipStream_t s);
-// out[r][n] = x[r][n] + y[r][n]
-hipError_t launch_add(const double* x, const double* y, double* out, int N, int k, hipStream_t s);
-// out[r][n] = x[r][n] - y[r][n]
-hipError_t launch_sub(const double* x, const double* y, double* out, int N, int k, hipStream_t s);
-// out[r][n] += sum_jj z[jj][r] * Q_jj[r][n]
-hipError_t launch_basis_combine(double* const* basis, int nvec, const double* z, double* out, int N, int k,
-                                hipStream_t s);
-// out[r][c] = sum over leaves n of conductor column c of leaf_weight[n] * x[r][n];
-// col_range[2c], col_range[2c+1] bound the leaves of column c
-hipError_t launch_charge_sums(const double* x, const double* leaf_weight, const int32_t* col_range, int N, int k,
-                              int ncol, double* out, hipStream_t s);
-
-int dots_blocks(int N);
+// v = P q on every job of the batch (operands in BJob::q / BJob::v).  jacobi
+// divides the input by the self terms first.  `gather_variant` picks the gather
+// lane layout; `ev` (5 events or null) times the stages.
+hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, int gather_variant, hipStream_t s,
+                      hipEvent_t* ev = nullptr);
 int gather_variant_count();
+
+// Krylov helpers (batched over segments).  Step buffer layout per job at hoff:
+// h1 [nv][k], h2 [nv][k], |w|^2 [k].
+hipError_t launch_dots(const KrylovView& kv, double* partial, hipStream_t s);
+hipError_t launch_reduce(const KrylovView& kv, const double* partial, double* hbuf, int which, hipStream_t s);
+hipError_t launch_update(const KrylovView& kv, const double* hbuf, int which, int norm, double* partial,
+                         hipStream_t s);
+hipError_t launch_reduce_norm(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s);
+// Q_nv[seg] = w[seg] / d[seg]   (d per segment, 0 -> zero vector)
+hipError_t launch_scale_new(const KrylovView& kv, const double* d, hipStream_t s);
+// x = P b (Jacobi) or b
+hipError_t launch_init_x(const KrylovView& kv, hipStream_t s);
+// r = b - w with |r|^2 chunk partials (the product A x is in w)
+hipError_t launch_resid(const KrylovView& kv, double* partial, hipStream_t s);
+// x += P (sum_jj z[jj] Q_jj)
+hipError_t launch_xupdate(const KrylovView& kv, const double* z, hipStream_t s);
+
+// elementwise / small helpers
+hipError_t launch_random(double* out, size_t n, uint64_t seed, hipStream_t s);
+hipError_t launch_add(const double* x, const double* y, double* out, size_t n, hipStream_t s);
+hipError_t launch_sub(const double* x, const double* y, double* out, size_t n, hipStream_t s);
+hipError_t launch_jacobi(const double* q, const double* self, int N, int k, double* pq, hipStream_t s);
+// y[r][n] += sum_jj z[jj][r] Q_jj[r][n] for jj < nvec
+hipError_t launch_combine(double* const* Q, int nvec, const double* z, double* y, int N, int k, hipStream_t s);
+// out[r][c] = sum over leaves of conductor column c of w[n] x[r][n]
+hipError_t launch_charge_sums(const double* x, const double* w, const int32_t* col_range, int N, int k, int ncol,
+                              double* out, hipStream_t s);
+// squared norms of the k segments of a [k][N] vector into out[k]
+hipError_t launch_sqnorms(const double* x, int N, int k, double* out, hipStream_t s);
 
 }  // namespace fcg

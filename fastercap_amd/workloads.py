@@ -129,6 +129,7 @@ def coplanar_lines_2d(outdir: str, seed: int = 0, height: float = 1.0, ground_t:
     lines = ["0 2D coplanar lines over ground (FastCap 2D format)"]
 
     def rect(name, x0, y0, x1, y1):
+        x0, y0, x1, y1 = (float(v) for v in (x0, y0, x1, y1))
         return [
             f"S {name} {x0!r} {y0!r} {x1!r} {y0!r}",
             f"S {name} {x1!r} {y0!r} {x1!r} {y1!r}",
@@ -142,6 +143,22 @@ def coplanar_lines_2d(outdir: str, seed: int = 0, height: float = 1.0, ground_t:
         lines += rect(f"l{i}", x, ground_t + height, x + w[i], ground_t + height + t[i])
         x += w[i] + (s[i] if i < 2 else 0.0)
     lines += rect("gnd", -15.0, 0.0, 15.0, ground_t)
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    return path
+
+
+def coax_2d(outdir: str, a: float = 1.0, b: float = 2.0, n: int = 96) -> str:
+    """2D cross-section of a coaxial line: inner circle of radius `a` and outer
+    circle of radius `b`, each an `n`-gon of 'S' segments (closed-form
+    capacitance 2 pi e0 / ln(b/a) per metre, up to the polygon approximation)."""
+    os.makedirs(outdir, exist_ok=True)
+    path = os.path.join(outdir, "coax2d.txt")
+    lines = ["0 2D coaxial line (FastCap 2D format)"]
+    for name, r in (("inner", a), ("outer", b)):
+        for i in range(n):
+            t0, t1 = 2 * math.pi * i / n, 2 * math.pi * (i + 1) / n
+            lines.append(f"S {name} {r * math.cos(t0)!r} {r * math.sin(t0)!r} {r * math.cos(t1)!r} {r * math.sin(t1)!r}")
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     return path

@@ -142,6 +142,15 @@ int fcg_mvp(fcg_ctx* ctx, int job, const double* q, double* v, int k);
  * the context stream).  No host synchronisation. */
 int fcg_mvp_device(fcg_ctx* ctx, int job, const double* d_q, double* d_v, int k, void* stream);
 
+/* Product precisions of fcg_mvp_prec. */
+#define FCG_MVP_FP64 0       /* FP64 coefficients, charges and accumulation */
+#define FCG_MVP_FP32 1       /* FP32 coefficients, charges and accumulation (inner solves, mixed path) */
+#define FCG_MVP_FP32_ACC64 2 /* FP32 coefficients, FP64 charges and accumulation (refinement residual) */
+
+/* v = P q in a given precision (the job must be filled in FCG_PREC_FP64 for
+ * FCG_MVP_FP64 and in FCG_PREC_MIXED for the two FP32-coefficient modes). */
+int fcg_mvp_prec(fcg_ctx* ctx, int job, int mvp_prec, const double* q, double* v, int k);
+
 /* Pq = P^-1 q (Jacobi) for k vectors, host buffers.
  * Replaces CSolveCap::ComputePrecondVectFast (Solver/SolveCapacitance.cpp:3458-3526). */
 int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k);
@@ -177,6 +186,38 @@ int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* b
 
 /* Pin the gather configuration of a job (variant < 0 only queries the count). */
 int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants);
+
+/* Solve several jobs together: every GMRES step of every job is advanced by one
+ * batched launch per stage.  caps receives the per-job matrices back to back
+ * (rows[i] x cols[i] each, in job order); stats may be NULL or hold njobs entries.
+ * Replaces the per-conductor sequential loop of CSolveCap::Solve
+ * (Solver/SolveCapacitance.cpp:1198-1419) extended across jobs. */
+int fcg_solve_batch(fcg_ctx* ctx, const int* jobs, int njobs, const fcg_solve_opts* opts, double* caps, int* rows,
+                    int* cols, fcg_solve_stats* stats);
+
+/*
+ * Serving engine: continuous batching of solves.  Jobs submitted while others are
+ * in flight join at the next step; each step advances every active solve by one
+ * Arnoldi step (or one refinement residual) with one launch per stage for all of
+ * them.  Finished results are popped with fcg_engine_poll.  No reference
+ * counterpart (the reference solves one geometry per process run).
+ */
+typedef struct fcg_engine fcg_engine;
+int fcg_engine_create(fcg_ctx* ctx, const fcg_solve_opts* opts, fcg_engine** out);
+int fcg_engine_submit(fcg_engine* eng, int job, int64_t tag);
+/* Advance all active solves by one step; returns the number still active and the
+ * number of finished results waiting to be polled. */
+int fcg_engine_step(fcg_engine* eng, int* active, int* ready);
+/* Pop one finished result; returns FCG_OK, or FCG_INVALID_ARGUMENT when none is ready. */
+int fcg_engine_poll(fcg_engine* eng, int64_t* tag, int* job, double* cap, int cap_len, int* rows, int* cols,
+                    fcg_solve_stats* stats);
+/* Device bytes held by the engine's active solves (admission control). */
+int fcg_engine_bytes(fcg_engine* eng, int64_t* bytes);
+int fcg_engine_destroy(fcg_engine* eng);
+
+/* Device bytes a job occupies (structure + coefficients of `precision`) and an
+ * estimate of the solve working set for `iters` Arnoldi steps. */
+int fcg_job_footprint(fcg_ctx* ctx, int job, int precision, int iters, int64_t* resident, int64_t* solve);
 
 /* Number of conductors (non-dielectric groups) of a job. */
 int fcg_job_conductors(fcg_ctx* ctx, int job, int* n_cond);
