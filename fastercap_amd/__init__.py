@@ -185,6 +185,11 @@ class Device:
     def fill(self, job: int, precision: int = FCG_PREC_FP64):
         self._check(self.lib.fcg_fill(self._h, job, precision), "fcg_fill")
 
+    def fill_batch(self, jobs, precision: int = FCG_PREC_FP64):
+        jobs = list(jobs)
+        ja = (C.c_int * len(jobs))(*jobs)
+        self._check(self.lib.fcg_fill_batch(self._h, ja, len(jobs), precision), "fcg_fill_batch")
+
     def coeffs(self, job: int, mesh: Mesh):
         coef = np.empty(mesh.L, dtype=np.float64)
         self_c = np.empty(mesh.N, dtype=np.float64)
@@ -318,6 +323,14 @@ class Device:
     def mvp_timing(self, job: int, k: int, reps: int = 20) -> dict:
         t = _ffi.MvpTiming()
         self._check(self.lib.fcg_mvp_timing_run(self._h, job, k, reps, C.byref(t)), "fcg_mvp_timing_run")
+        return {f: getattr(t, f) for f, _ in _ffi.MvpTiming._fields_}
+
+    def mvp_timing_batch(self, jobs, mvp_prec: int = FCG_MVP_FP64, reps: int = 10) -> dict:
+        jobs = list(jobs)
+        ja = (C.c_int * len(jobs))(*jobs)
+        t = _ffi.MvpTiming()
+        self._check(self.lib.fcg_mvp_timing_batch(self._h, ja, len(jobs), mvp_prec, reps, C.byref(t)),
+                    "fcg_mvp_timing_batch")
         return {f: getattr(t, f) for f, _ in _ffi.MvpTiming._fields_}
 
     def set_gather_variant(self, job: int, variant: int) -> int:

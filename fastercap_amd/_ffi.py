@@ -152,6 +152,8 @@ DEVICE_SYMBOLS = (
     "fcg_engine_bytes",
     "fcg_engine_destroy",
     "fcg_job_footprint",
+    "fcg_fill_batch",
+    "fcg_mvp_timing_batch",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -232,6 +234,8 @@ def device_lib() -> C.CDLL:
         lib.fcg_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         ip = C.POINTER(C.c_int)
         i64p = C.POINTER(C.c_int64)
+        lib.fcg_mvp_timing_batch.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.POINTER(MvpTiming)]
+        lib.fcg_fill_batch.argtypes = [vp, ip, C.c_int, C.c_int]
         lib.fcg_mvp_prec.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]
         lib.fcg_solve_batch.argtypes = [vp, ip, C.c_int, C.POINTER(SolveOpts), dp, ip, ip, C.POINTER(SolveStats)]
         lib.fcg_engine_create.argtypes = [vp, C.POINTER(SolveOpts), C.POINTER(vp)]

@@ -1075,29 +1075,40 @@ int fcg_job_footprint(fcg_ctx* ctx, int job, int precision, int iters, int64_t* 
     return FCG_OK;
 }
 
-int fcg_fill(fcg_ctx* ctx, int job, int precision) {
-    Job* j = get_job(ctx, job);
-    if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
+    if (!ctx || !jobs || njobs < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     if (precision != FCG_PREC_FP64 && precision != FCG_PREC_MIXED)
         return fail(ctx, FCG_INVALID_ARGUMENT, "unknown precision mode");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    if (precision == FCG_PREC_MIXED && !j->coef32.p)
-        HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)std::max<int64_t>(j->L, 1)));
-    if (precision == FCG_PREC_FP64 && !j->coef64.p)
-        HIPCHECK(ctx, j->coef64.alloc(sizeof(double) * (size_t)std::max<int64_t>(j->L, 1)));
+    std::vector<BatchEntry> ents;
+    for (int i = 0; i < njobs; ++i) {
+        Job* j = get_job(ctx, jobs[i]);
+        if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+        for (const auto& e : ents)
+            if (e.j == j) return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
+        if (precision == FCG_PREC_MIXED && !j->coef32.p)
+            HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)std::max<int64_t>(j->L, 1)));
+        if (precision == FCG_PREC_FP64 && !j->coef64.p)
+            HIPCHECK(ctx, j->coef64.alloc(sizeof(double) * (size_t)std::max<int64_t>(j->L, 1)));
+        ents.push_back(BatchEntry{j, &j->scr, nullptr, nullptr});
+    }
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->tabs.reset();
     BatchView bv;
-    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{j, &j->scr, nullptr, nullptr}}, &bv));
+    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv));
     HIPCHECK(ctx, fcg::launch_fill(bv, precision, s));
     HIPCHECK(ctx, hipStreamSynchronize(s));
-    if (precision == FCG_PREC_MIXED)
-        j->filled32 = true;
-    else
-        j->filled64 = true;
+    for (auto& e : ents) {
+        if (precision == FCG_PREC_MIXED)
+            e.j->filled32 = true;
+        else
+            e.j->filled64 = true;
+    }
     return FCG_OK;
 }
+
+int fcg_fill(fcg_ctx* ctx, int job, int precision) { return fcg_fill_batch(ctx, &job, 1, precision); }
 
 int fcg_download_coeffs(fcg_ctx* ctx, int job, double* coef, double* self_coef) {
     Job* j = get_job(ctx, job);
@@ -1191,6 +1202,68 @@ int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* o
     out->reps = reps;
     out->k = k;
     out->variant = j->gather_variant;
+    return FCG_OK;
+}
+
+int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, fcg_mvp_timing* out) {
+    if (!ctx || !jobs || njobs < 1 || reps < 1 || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    std::vector<std::unique_ptr<DevBuf>> qs, vs;
+    std::vector<Job*> js;
+    int ktot = 0;
+    for (int i = 0; i < njobs; ++i) {
+        Job* j = get_job(ctx, jobs[i]);
+        if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+        for (Job* o : js)
+            if (o == j) return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
+        if (mvp_prec == FCG_MVP_FP64 ? !j->filled64 : !j->filled32)
+            return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in this precision");
+        const int k = std::max(1, (int)j->rhs.size());
+        RCCHECK(ensure_scratch(ctx, j->scr, j->N, j->M, k, mvp_prec == FCG_MVP_FP32));
+        const size_t n = (size_t)j->N * k;
+        qs.emplace_back(new DevBuf());
+        vs.emplace_back(new DevBuf());
+        HIPCHECK(ctx, qs.back()->alloc(sizeof(double) * n));
+        HIPCHECK(ctx, vs.back()->alloc(sizeof(double) * n));
+        HIPCHECK(ctx, fcg::launch_random(qs.back()->as<double>(), n, 4242 + (uint64_t)i, s));
+        js.push_back(j);
+        ktot += k;
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    ctx->tabs.reset();
+    std::vector<BatchEntry> ents;
+    for (size_t i = 0; i < js.size(); ++i) ents.push_back(BatchEntry{js[i], &js[i]->scr, qs[i]->as<double>(), vs[i]->as<double>()});
+    BatchView bv;
+    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv));
+    const int var = njobs == 1 ? js[0]->gather_variant : 0;
+    Events5 ev;
+    for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
+    HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s));
+    double acc[4] = {0, 0, 0, 0}, tot = 0;
+    for (int r = 0; r < reps; ++r) {
+        HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s, ev.e));
+        HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
+        for (int st = 0; st < 4; ++st) {
+            float ms = 0;
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[st], ev.e[st + 1]));
+            acc[st] += ms;
+        }
+        float ms = 0;
+        HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[0], ev.e[4]));
+        tot += ms;
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    out->ms_scan = acc[0] / reps;
+    out->ms_charge = acc[1] / reps;
+    out->ms_gather = acc[2] / reps;
+    out->ms_down = acc[3] / reps;
+    out->ms_total = tot / reps;
+    out->reps = reps;
+    out->k = ktot;
+    out->variant = var;
     return FCG_OK;
 }
 

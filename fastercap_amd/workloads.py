@@ -162,3 +162,35 @@ def coax_2d(outdir: str, a: float = 1.0, b: float = 2.0, n: int = 96) -> str:
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
     return path
+
+
+def batch_uneven(outdir: str, n2d: int = 1024, n3d: int = 256, seed: int = 0, m2d: float = 0.02,
+                 m3d_range=(0.0025, 0.01)):
+    """Batch of uneven extraction jobs: `n2d` seeded 2D three-line cross-sections
+    (coplanar_lines_2d, mesh_eps m2d: about 330-380 segments each) and `n3d`
+    seeded coplanar plate pairs at a lateral pitch uniform in [1.5, 3] with mesh_eps
+    log-uniform in m3d_range (about 2k-33k panels).  Returns a list of
+    (path, mesh_eps, kind) tuples, 2D jobs first."""
+    rng = np.random.default_rng(BASE_SEED + 7919 * seed + 1)
+    jobs = []
+    for i in range(n2d):
+        d = os.path.join(outdir, f"l2d_{i}")
+        jobs.append((coplanar_lines_2d(d, seed=seed * 100003 + i), m2d, "2d"))
+    lo, hi = math.log(m3d_range[0]), math.log(m3d_range[1])
+    for i in range(n3d):
+        pitch = float(rng.uniform(1.5, 3.0))
+        m = float(math.exp(rng.uniform(lo, hi)))
+        d = os.path.join(outdir, f"p3d_{i}")
+        jobs.append((two_plates(d, pitch=pitch), m, "3d"))
+    return jobs
+
+
+def build_meshes(jobs, threads: int = 16):
+    """Build the meshes of (path, mesh_eps, kind) jobs on a thread pool (the host
+    builder releases the interpreter lock and keeps no shared state)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from . import Mesh
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(lambda j: Mesh.build(j[0], mesh_eps=j[1]), jobs))

@@ -130,6 +130,8 @@ int fcg_job_release(fcg_ctx* ctx, int job);
  * calling PotEstimateOpt (Solver/Autorefine.cpp:3450-3733, 2D :4156-4325), and the self
  * terms of RefineSelf/SelfPotential (Solver/Autorefine.cpp:2115-2229, 2358-2506). */
 int fcg_fill(fcg_ctx* ctx, int job, int precision);
+/* fcg_fill of several jobs in one launch (same arithmetic per link). */
+int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision);
 
 /* Copy the filled coefficients to host memory (coef [L] and self [N], either may be NULL). */
 int fcg_download_coeffs(fcg_ctx* ctx, int job, double* coef, double* self_coef);
@@ -177,6 +179,10 @@ typedef struct fcg_mvp_timing {
     int32_t variant;   /* gather configuration in use */
 } fcg_mvp_timing;
 int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* out);
+/* The same for one batched product over several filled jobs, each on as many
+ * random right-hand sides as it has conductor solves, in precision mvp_prec
+ * (FCG_MVP_*); out->k is the total right-hand side count. */
+int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, fcg_mvp_timing* out);
 
 /* Time every gather configuration (lanes per link, right-hand sides per lane,
  * links in flight) on this job with k right-hand sides and keep the fastest for
