@@ -194,3 +194,88 @@ def build_meshes(jobs, threads: int = 16):
 
     with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
         return list(ex.map(lambda j: Mesh.build(j[0], mesh_eps=j[1]), jobs))
+
+
+def _circle_segments(name: str, r: float, n: int):
+    lines = []
+    for i in range(n):
+        t0, t1 = 2 * math.pi * i / n, 2 * math.pi * (i + 1) / n
+        lines.append(f"S {name} {r * math.cos(t0)!r} {r * math.sin(t0)!r} {r * math.cos(t1)!r} {r * math.sin(t1)!r}")
+    return lines
+
+
+def coated_coax_2d(outdir: str, a: float = 0.1, b: float = 0.15, c: float = 0.2, eps_in: float = 2.0,
+                   eps_out: float = 1.0, n: int = 40) -> str:
+    """2D coated coaxial cable (the example of the reference's 2D input-file help):
+    inner conductor radius a, dielectric interface at radius b (eps_in inside,
+    eps_out outside), outer conductor radius c; each circle an n-gon.  Closed form
+    C = C1 C2 / (C1 + C2), C1 = 2 pi e0 eps_in / ln(b/a), C2 = 2 pi e0 eps_out / ln(c/b)."""
+    os.makedirs(outdir, exist_ok=True)
+    files = {}
+    for tag, r in (("inner", a), ("iface", b), ("outer", c)):
+        p = os.path.join(outdir, f"{tag}.txt")
+        with open(p, "w") as f:
+            f.write(f"0 circle radius {r!r}\n" + "\n".join(_circle_segments(tag, r, n)) + "\n")
+        files[tag] = os.path.basename(p)
+    main = os.path.join(outdir, "coated_coax.lst")
+    with open(main, "w") as f:
+        f.write("* 2D coated coaxial cable\n")
+        f.write(f"C {files['inner']} {eps_in!r}  0.0 0.0\n")
+        # the reference point (centre) lies in the region of the first permittivity
+        f.write(f"D {files['iface']} {eps_in!r} {eps_out!r}  0.0 0.0  0.0 0.0\n")
+        f.write(f"C {files['outer']} {eps_out!r}  0.0 0.0\n")
+    return main
+
+
+def _icosphere(subdiv: int):
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [tuple(c / math.sqrt(1 + t * t) for c in v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(i, j):
+            key = (min(i, j), max(i, j))
+            if key not in cache:
+                p = [(verts[i][k] + verts[j][k]) / 2 for k in range(3)]
+                n = math.sqrt(sum(c * c for c in p))
+                verts.append(tuple(c / n for c in p))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    return verts, faces
+
+
+def sphere_in_shell(outdir: str, a: float = 1.0, b: float = 2.0, eps_shell: float = 3.0, subdiv: int = 2) -> str:
+    """Conductor sphere of radius a inside a concentric dielectric sphere of radius b
+    (eps_shell between them, vacuum outside), both icospheres of 20*4^subdiv 'T'
+    panels.  Closed form C = 4 pi e0 / ((1/a - 1/b)/eps_shell + 1/b)."""
+    os.makedirs(outdir, exist_ok=True)
+    verts, faces = _icosphere(subdiv)
+
+    def write(name, r):
+        p = os.path.join(outdir, f"{name}.txt")
+        with open(p, "w") as f:
+            f.write(f"0 sphere radius {r!r}\n")
+            for tri in faces:
+                pts = "  ".join(" ".join(repr(r * c) for c in verts[i]) for i in tri)
+                f.write(f"T {name} {pts}\n")
+        return os.path.basename(p)
+
+    fc = write("ball", a)
+    fd = write("shell", b)
+    main = os.path.join(outdir, "sphere_shell.lst")
+    with open(main, "w") as f:
+        f.write("* conductor sphere in a dielectric shell\n")
+        f.write(f"C {fc} {eps_shell!r}  0 0 0\n")
+        f.write(f"D {fd} {eps_shell!r} 1.0  0 0 0  0 0 0\n")
+    return main

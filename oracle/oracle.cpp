@@ -510,6 +510,16 @@ int orc_compressed_dense(const fcg_job_desc* d, const double* coef, const double
     return 0;
 }
 
+int orc_leaf_dense(const fcg_job_desc* d, double* P, int nthreads) {
+    const long N = d->n_leaves;
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < N; ++i)
+        for (long j = 0; j < N; ++j)
+            P[i * N + j] = (i == j) ? orc_self(d, (int)i) : orc_pot_estimate(d, d->leaf_node[i], d->leaf_node[j]);
+    return 0;
+}
+
 long long orc_link_coverage(const fcg_job_desc* d) {
     const long N = d->n_leaves;
     std::vector<unsigned char> cnt((size_t)N * N, 0);

@@ -20,6 +20,8 @@
  *                          (Solver/SolveCapacitance.cpp:1138-1425, 2288-2574, 3489-3526)
  *   orc_compressed_dense   the compressed matrix column by column (DebugDumpPotMtx pattern,
  *                          Solver/SolveCapacitance.cpp:3734-3795), built directly from the links
+ *   orc_leaf_dense         the same matrix with every leaf pair evaluated directly (no
+ *                          hierarchical approximation), for closed-form checks
  *   orc_link_coverage      link-partition invariant: every ordered leaf pair i != j is covered
  *                          by exactly one link
  */
@@ -59,6 +61,11 @@ int orc_solve_rows(const fcg_job_desc* d, const double* coef, const double* self
 
 /* Dense compressed matrix P[N][N] (row-major) assembled from the links. */
 int orc_compressed_dense(const fcg_job_desc* d, const double* coef, const double* self_coef, double* P);
+
+/* Leaf-level dense matrix P[N][N]: every leaf pair evaluated directly with the
+ * same estimate as a leaf-to-leaf link (no hierarchical approximation), self
+ * terms on the diagonal. */
+int orc_leaf_dense(const fcg_job_desc* d, double* P, int nthreads);
 
 /* Returns the number of ordered leaf pairs whose link cover count differs from the
  * expected value (1 for i != j, 0 for i == j); 0 means the links partition the matrix. */

@@ -44,6 +44,7 @@ def lib():
         L.orc_solve_rows.argtypes = [jd, dp, dp, C.c_double, C.c_int, C.c_int, dp, C.POINTER(C.c_int),
                                      C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_int]
         L.orc_compressed_dense.argtypes = [jd, dp, dp, dp]
+        L.orc_leaf_dense.argtypes = [jd, dp, C.c_int]
         L.orc_link_coverage.argtypes = [jd]
         L.orc_link_coverage.restype = C.c_longlong
         _lib = L
@@ -128,6 +129,12 @@ def solve_rows(mesh, coef, selfc, max_rows, tol=1e-8, precond=1, nthreads=0):
 def compressed_dense(mesh, coef, selfc):
     P = np.empty((mesh.N, mesh.N), dtype=np.float64)
     lib().orc_compressed_dense(C.byref(mesh.desc), _d(coef), _d(selfc), _d(P))
+    return P
+
+
+def leaf_dense(mesh, nthreads=0):
+    P = np.empty((mesh.N, mesh.N), dtype=np.float64)
+    lib().orc_leaf_dense(C.byref(mesh.desc), _d(P), nthreads)
     return P
 
 

@@ -203,3 +203,23 @@ def test_footprint_reports_bytes(dev, meshes):
     assert res64 > 8 * m.L and res32 > 4 * m.L and res64 > res32
     assert s64 > 8 * m.N * 32
     dev.release(job)
+
+
+@pytest.mark.parametrize("which", ["coated_coax_2d", "sphere_in_shell"])
+def test_curved_dielectrics_match_oracle(dev, geo_dir, which):
+    d = os.path.join(geo_dir, "gpu_" + which)
+    if which == "coated_coax_2d":
+        m = fc.Mesh.build(workloads.coated_coax_2d(d), mesh_eps=0.05)
+    else:
+        m = fc.Mesh.build(workloads.sphere_in_shell(d, b=3.0, subdiv=2), mesh_eps=0.05)
+    job = dev.upload(m)
+    dev.fill(job)
+    ocoef, oself = orc.fill(m)
+    coef, selfc = dev.coeffs(job, m)
+    assert np.max(np.abs(coef - ocoef) / np.abs(ocoef)) < 1e-12
+    res = dev.solve(job, m, tol=1e-10, max_iter=2000)
+    cap, iters = orc.solve(m, ocoef, oself, tol=1e-10)
+    print(f"{which}: N={m.N} gpu {res.cap.ravel()} oracle {cap.ravel()} iters {res.iterations} {list(iters)}",
+          flush=True)
+    assert np.max(np.abs(res.cap - cap) / np.abs(np.diag(cap))[:, None]) < 1e-6
+    dev.release(job)
