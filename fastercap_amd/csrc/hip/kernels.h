@@ -33,12 +33,15 @@ enum MvpPrec : int {
 // Device record of one job in a batch.
 struct BJob {
     int N, M, k, dim;
+    int kp;               // node-major row stride: k rounded up to a power of two
     int first_cond_leaf;  // 2D zero-total-charge fix-up start
     int nblk;             // scan tiles per right-hand side
     int64_t L;
     const fcg_node_geom* nodes;
     const int64_t* row_ptr;
-    const int32_t* col;
+    const int32_t* col;       // [L] source slot (permuted node index), ascending within a row
+    const int32_t* perm;      // [M] node -> slot of its charge / potential row
+    const int32_t* iperm;     // [M] slot -> node
     const int32_t* link_row;  // [L] destination node of each link (fill only)
     double* coef64;           // [L] or null
     float* coef32;            // [L] or null
@@ -55,9 +58,9 @@ struct BJob {
     DD* prefix;      // [k][N+1]
     DD* block_sums;  // [k][nblk]
     double* charge64;
-    float* charge32;  // [M][k]
+    float* charge32;  // [M][kp], row perm[node]
     double* pot64;
-    float* pot32;  // [M][k]
+    float* pot32;  // [M][kp], row perm[node]
     double* lt;    // [2k] 2D fix-up scratch
     // current product operands ([k][N] each)
     const double* q;
@@ -75,7 +78,8 @@ struct BatchView {
     const int64_t* leaf_off;  // leaves
     const int64_t* rhs_off;   // right-hand sides
     int64_t links, tiles, nodes, rowb, leaves, rhs;
-    int kmax;  // largest k in the batch
+    int kmax;   // largest k in the batch
+    int kpmin;  // smallest row stride in the batch
     int any2d;
 };
 
@@ -117,6 +121,12 @@ hipError_t launch_fill(const BatchView& b, int precision, hipStream_t s);
 hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, int gather_variant, hipStream_t s,
                       hipEvent_t* ev = nullptr);
 int gather_variant_count();
+// power-of-two row stride of the node-major arrays for k right-hand sides
+inline int padded_k(int k) {
+    int p = 1;
+    while (p < k) p <<= 1;
+    return p;
+}
 
 // Krylov helpers (batched over segments).  Step buffer layout per job at hoff:
 // h1 [nv][k], h2 [nv][k], |w|^2 [k].

@@ -143,7 +143,7 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, int mixed) {
     const BJob& J = bv.jobs[jb];
     const int64_t l = g - bv.link_off[jb];
     const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
-    const fcg_node_geom& p2 = J.nodes[J.col[l]];
+    const fcg_node_geom& p2 = J.nodes[J.iperm[J.col[l]]];
     if (mixed)
         J.coef32[l] = pot_estimate_mixed(p1, p2, J.dim);
     else
@@ -271,7 +271,8 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_final(BatchView bv, int j
     }
 }
 
-// node charges, node-major [M][k], in FP64 or FP32
+// node charges, node-major [M][kp] at row perm[node], in FP64 or FP32; the
+// padding columns k..kp-1 are zero so that vector loads of a row stay finite
 template <typename TQ>
 __global__ void k_node_charge(BatchView bv, int jacobi) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -281,7 +282,8 @@ __global__ void k_node_charge(BatchView bv, int jacobi) {
     const int i = (int)(g - bv.node_off[jb]);
     const int lo = J.leaf_lo[i], hi = J.leaf_hi[i];
     const bool leaf = J.left[i] < 0;
-    TQ* out = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.charge64) : reinterpret_cast<TQ*>(J.charge32);
+    TQ* out = ((sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.charge64) : reinterpret_cast<TQ*>(J.charge32)) +
+              (size_t)J.perm[i] * J.kp;
     for (int r = 0; r < J.k; ++r) {
         double c;
         if (leaf) {
@@ -290,25 +292,35 @@ __global__ void k_node_charge(BatchView bv, int jacobi) {
             const DD* P = J.prefix + (size_t)r * (J.N + 1);
             c = dd_diff(P[hi], P[lo]);
         }
-        out[(size_t)i * J.k + r] = (TQ)c;
+        out[r] = (TQ)c;
     }
+    for (int r = J.k; r < J.kp; ++r) out[r] = (TQ)0;
 }
 
 // ------------------------------------------------------------------ gather
-// One wavefront per destination node.  Lane = (slot, rhs): KR lanes share one
-// link slot and each reads VR consecutive right-hand sides of the source node, so
-// one link's charges are one contiguous [k]-row of the node-major charge array;
-// the 64/KR slots walk the row's links in stride, U links per slot in flight, and
-// a shuffle tree over the slots reduces each right-hand side.  Blocks are remapped
-// so that each of the 8 XCDs owns one contiguous range of destination rows:
-// post-order rows are spatially coherent, so the charges one XCD gathers stay in
-// its own L2.
+// One wavefront per destination node.  Lane = (slot, rhs group): KR lanes share
+// one link slot and each loads VR consecutive right-hand sides of the source row
+// with one vector load, so a link's charges are one contiguous kp-row of the
+// node-major charge array; the 64/KR slots walk the row's links in stride, U
+// links per slot in flight, and a shuffle tree over the slots reduces each
+// right-hand side.  Charge rows are laid out depth-major (nodes of one tree depth
+// together, post-order within a depth) and each row's links are sorted by source
+// row, so neighbouring links of one instruction often share a cache line.  Blocks
+// are remapped so that each of the 8 XCDs owns one contiguous range of
+// destination rows: post-order rows are spatially coherent, so the charges one
+// XCD gathers stay in its own L2.
 constexpr int kGatherWaves = 4;  // rows per 256-thread block
+
+template <typename T, int VR>
+struct alignas(sizeof(T) * VR) VecT {
+    T v[VR];
+};
 
 template <typename TC, typename TQ, typename TA, int KR, int VR, int U>
 __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd) {
     constexpr int SL = 64 / KR;
     constexpr int W = KR * VR;
+    using V = VecT<TQ, VR>;
     const int64_t gb = (int64_t)(blockIdx.x % kNumXcd) * blocks_per_xcd + (blockIdx.x / kNumXcd);
     if (gb >= bv.rowb) return;
     const int jb = find_job(bv.rowb_off, bv.njobs, gb);
@@ -318,16 +330,16 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
     const int lane = threadIdx.x & 63;
     const int slot = lane / KR;
     const int rl = (lane % KR) * VR;
-    const int k = J.k;
+    const int k = J.k, kp = J.kp;
     const TC* coef = (sizeof(TC) == 8) ? reinterpret_cast<const TC*>(J.coef64) : reinterpret_cast<const TC*>(J.coef32);
     const TQ* charge =
         (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.charge64) : reinterpret_cast<const TQ*>(J.charge32);
     TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.pot64) : reinterpret_cast<TQ*>(J.pot32);
     const int64_t beg = J.row_ptr[row], end = J.row_ptr[row + 1];
     const TA sf = (TA)J.diag[row];
+    const size_t orow = (size_t)J.perm[row] * kp;
     for (int r0 = 0; r0 < k; r0 += W) {
-        const int kc = min(k - r0, W);
-        const TQ* src = charge + r0 + rl;
+        const bool active = r0 + rl < k;  // lanes past the last rhs load nothing
         TA acc[VR];
 #pragma unroll
         for (int v = 0; v < VR; ++v) acc[v] = (TA)0;
@@ -341,15 +353,20 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
                 c[u] = ok ? J.col[ll] : 0;
                 w[u] = ok ? (TA)coef[ll] : (TA)0;
             }
-            TA ch[U][VR];
+            V ch[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (active) {
+                    ch[u] = *reinterpret_cast<const V*>(charge + (size_t)c[u] * kp + r0 + rl);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < VR; ++v) ch[u].v[v] = (TQ)0;
+                }
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u)
 #pragma unroll
-                for (int v = 0; v < VR; ++v) ch[u][v] = (rl + v < kc) ? (TA)src[(size_t)c[u] * k + v] : (TA)0;
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int v = 0; v < VR; ++v) acc[v] += w[u] * ch[u][v];
+                for (int v = 0; v < VR; ++v) acc[v] += w[u] * (TA)ch[u].v[v];
         }
 #pragma unroll
         for (int v = 0; v < VR; ++v)
@@ -358,8 +375,8 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
         if (slot == 0) {
 #pragma unroll
             for (int v = 0; v < VR; ++v)
-                if (rl + v < kc) {
-                    const size_t o = (size_t)row * k + r0 + rl + v;
+                if (r0 + rl + v < k) {
+                    const size_t o = orow + r0 + rl + v;
                     pot[o] = (TQ)(sf * (TA)charge[o] + acc[v]);
                 }
         }
@@ -369,22 +386,29 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
 struct GatherCfg {
     int kr, vr, u;
 };
+// requested lane shapes (lanes per link, rhs per lane, links in flight per slot);
+// gather_shape fits them to the batch's row strides
 static const GatherCfg kGatherCfgs[] = {
-    {4, 2, 4}, {8, 1, 2}, {4, 2, 2}, {8, 1, 4}, {2, 4, 2}, {16, 1, 2}, {2, 4, 4}, {1, 2, 4},
+    {2, 4, 4}, {1, 8, 4}, {4, 2, 4}, {1, 8, 2}, {2, 4, 2}, {8, 1, 4}, {4, 2, 2}, {1, 4, 4}, {16, 1, 2}, {2, 2, 4},
 };
 constexpr int kNumGatherCfgs = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 int gather_variant_count() { return kNumGatherCfgs; }
 
-// lane group width for a batch whose largest rhs count is kmax
-static GatherCfg gather_shape(int variant, int kmax) {
+// VR must divide every job's row stride (vector loads stay inside one row), and
+// lanes beyond the largest rhs count are dropped
+static GatherCfg gather_shape(int variant, int kpmin, int kmax) {
     GatherCfg c = kGatherCfgs[(variant >= 0 && variant < kNumGatherCfgs) ? variant : 0];
+    while (c.vr > kpmin) {
+        c.vr /= 2;
+        c.kr = c.kr * 2 > 16 ? 16 : c.kr * 2;
+    }
     while (c.kr > 1 && (c.kr / 2) * c.vr >= kmax) c.kr /= 2;
     return c;
 }
 
 template <typename TC, typename TQ, typename TA>
 static hipError_t gather_dispatch(const BatchView& bv, int variant, hipStream_t s) {
-    const GatherCfg c = gather_shape(variant, bv.kmax);
+    const GatherCfg c = gather_shape(variant, bv.kpmin, bv.kmax);
     const int64_t per_xcd = (bv.rowb + kNumXcd - 1) / kNumXcd;
     const unsigned grid = (unsigned)(per_xcd * kNumXcd);
     if (grid == 0) return hipSuccess;
@@ -393,12 +417,12 @@ static hipError_t gather_dispatch(const BatchView& bv, int variant, hipStream_t 
         k_gather<TC, TQ, TA, KR, VR, U><<<grid, 256, 0, s>>>(bv, (int)per_xcd); \
         return hipGetLastError();                                               \
     }
-    FCG_G(1, 1, 2) FCG_G(2, 1, 2) FCG_G(4, 1, 2) FCG_G(8, 1, 2) FCG_G(16, 1, 2)
-    FCG_G(1, 1, 4) FCG_G(2, 1, 4) FCG_G(4, 1, 4) FCG_G(8, 1, 4)
-    FCG_G(1, 2, 2) FCG_G(2, 2, 2) FCG_G(4, 2, 2)
-    FCG_G(1, 2, 4) FCG_G(2, 2, 4) FCG_G(4, 2, 4)
-    FCG_G(1, 4, 2) FCG_G(2, 4, 2)
-    FCG_G(1, 4, 4) FCG_G(2, 4, 4)
+#define FCG_GU(KR, VR) FCG_G(KR, VR, 2) FCG_G(KR, VR, 4)
+    FCG_GU(1, 1) FCG_GU(2, 1) FCG_GU(4, 1) FCG_GU(8, 1) FCG_GU(16, 1)
+    FCG_GU(1, 2) FCG_GU(2, 2) FCG_GU(4, 2) FCG_GU(8, 2)
+    FCG_GU(1, 4) FCG_GU(2, 4) FCG_GU(4, 4)
+    FCG_GU(1, 8) FCG_GU(2, 8)
+#undef FCG_GU
 #undef FCG_G
     return hipErrorInvalidConfiguration;
 }
@@ -420,7 +444,7 @@ __global__ void k_down(BatchView bv) {
         for (int r = 0; r < KC; ++r) acc[r] = 0.0;
         int a = J.leaf_node[leaf];
         while (a >= 0) {
-            const TQ* p = pot + (size_t)a * k + r0;
+            const TQ* p = pot + (size_t)J.perm[a] * J.kp + r0;
 #pragma unroll
             for (int r = 0; r < KC; ++r)
                 if (r < kc) acc[r] += (double)p[r];

@@ -216,7 +216,9 @@ def test_curved_dielectrics_match_oracle(dev, geo_dir, which):
     dev.fill(job)
     ocoef, oself = orc.fill(m)
     coef, selfc = dev.coeffs(job, m)
-    assert np.max(np.abs(coef - ocoef) / np.abs(ocoef)) < 1e-12
+    # field coefficients of nearly tangential pairs cancel to ~1e-7 of the largest,
+    # so the bound mixes a relative and an absolute term
+    assert np.all(np.abs(coef - ocoef) <= 1e-12 * np.abs(ocoef) + 1e-14 * np.abs(ocoef).max())
     res = dev.solve(job, m, tol=1e-10, max_iter=2000)
     cap, iters = orc.solve(m, ocoef, oself, tol=1e-10)
     print(f"{which}: N={m.N} gpu {res.cap.ravel()} oracle {cap.ravel()} iters {res.iterations} {list(iters)}",
