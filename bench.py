@@ -37,15 +37,6 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def gather_bytes(mesh, k, b_coef=8, b_node=8):
-    """Algorithmic HBM bytes of one gather launch over a job with k right-hand
-    sides: coefficient (b_coef) and source index (4 B) of every link, the row
-    pointers (8 B, M+1) and self terms (8 B, M) of every destination node, every
-    node's k charges read once and its k potentials written once (b_node each)."""
-    L, M = mesh.L, mesh.M
-    return L * (b_coef + 4) + (M + 1) * 8 + M * 8 + 2 * M * k * b_node
-
-
 def pct(xs, p):
     s = sorted(xs)
     return s[min(len(s) - 1, int(round(p / 100.0 * (len(s) - 1))))]
@@ -102,6 +93,7 @@ def bench_batch(args, fc, dev, tmp):
         dev.solve_batch(jobs, meshes, tol=tol, precision=MIX)
     log("warmup done")
     lat = []
+    dev.profile(enable=1, reset=True)
     t_start = time.perf_counter()
     for s in range(args.steps):
         a = time.perf_counter()
@@ -112,9 +104,13 @@ def bench_batch(args, fc, dev, tmp):
         lat = [r.t_solve_s for r in res]
         log(f"step {s}: fill {1e3 * (b - a):.1f} ms, batched mixed solve {1e3 * (c - b):.1f} ms")
     t_mixed = time.perf_counter() - t_start
+    prof = dev.profile(enable=0, reset=True)
     value = args.steps * nj / t_mixed
     mixed_caps = [r.cap for r in res]
     iters_mixed = [max(r.iterations) for r in res]
+    if args.only_mixed:  # profiling aid: the timed mixed solves alone
+        log(f"timed-region products: {prof}")
+        return None
 
     # plain FP64 path: one job at a time (fill + solve), as the reference runs
     dev.fill_batch(jobs, F64)
@@ -139,11 +135,11 @@ def bench_batch(args, fc, dev, tmp):
     log(f"mixed vs FP64 tol 1e-8: per-entry {e_conv[0]:.3e} vs-diag {e_conv[1]:.3e}; "
         f"plain FP64 at tol {tol} vs tol 1e-8: {e_plain_conv[0]:.3e}")
 
-    # dominant kernel: the FP32 gather of one batched product over every job
-    tm = dev.mvp_timing_batch(jobs, fc.FCG_MVP_FP32, reps=args.reps)
-    gb = sum(gather_bytes(m, max(1, ncond(m) - (1 if m.dim == 2 else 0)), 4, 4) for m in meshes)
-    achieved = gb / (tm["ms_gather"] * 1e-3) / 1e9
-    log(f"batched FP32 product stages ms: {tm}")
+    # dominant kernel: the FP32 gather of the inner products of the timed solves
+    # (HIP events around every launch; bytes per launch from the active jobs)
+    g = prof["fp32"]
+    achieved = g["gather_bytes"] / (g["gather_ms"] * 1e-3) / 1e9
+    log(f"timed-region products: {prof}")
 
     cpu = None
     if not args.no_cpu:
@@ -184,16 +180,23 @@ def bench_batch(args, fc, dev, tmp):
         "gmres_iterations_max_per_job": {"p50": pct(iters_mixed, 50), "max": max(iters_mixed)},
         "c_rel_err_mixed_vs_fp64_tol1e-8": {"max_per_entry": e_conv[0], "max_vs_diagonal": e_conv[1]},
         "c_rel_err_mixed_vs_fp64_plain_same_tol": {"max_per_entry": e_plain[0], "max_vs_diagonal": e_plain[1]},
-        "mvp": {"precision": "fp32", "k_total": tm["k"], "ms": tm["ms_total"],
-                "stage_ms": {s: tm["ms_" + s] for s in ("scan", "charge", "gather", "down")}},
+        "mvp": {
+            "inner_fp32": {"launches": g["mvp_launches"], "avg_ms": g["mvp_ms"] / max(1, g["mvp_launches"])},
+            "residual_fp32_coef_fp64_acc": {
+                "launches": prof["fp32_acc64"]["mvp_launches"],
+                "avg_ms": prof["fp32_acc64"]["mvp_ms"] / max(1, prof["fp32_acc64"]["mvp_launches"])},
+        },
         "roofline": {
-            "kernel": "k_gather (FP32 coefficients and charges), one launch over the whole batch",
+            "kernel": "k_gather<float,float,float,*> (inner FP32 products of the timed batched solves)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": None,
+            "launches": g["gather_launches"],
+            "avg_launch_ms": g["gather_ms"] / max(1, g["gather_launches"]),
+            "alg_bytes_per_launch": g["gather_bytes"] / max(1, g["gather_launches"]),
         },
         "cpu_baseline": cpu,
     }
@@ -229,6 +232,7 @@ def bench_single(args, fc, dev, tmp):
         dev.solve(job, mesh, tol=args.tol, precision=prec)
     log("warmup done")
     lat, fills, solves = [], [], []
+    dev.profile(enable=1, reset=True)
     t_start = time.perf_counter()
     for s in range(args.steps):
         a = time.perf_counter()
@@ -241,20 +245,17 @@ def bench_single(args, fc, dev, tmp):
         solves.append(c - b)
         log(f"step {s}: fill {1e3 * (b - a):.1f} ms solve {1e3 * (c - b):.1f} ms iters {res.iterations}")
     t_total = time.perf_counter() - t_start
+    prof = dev.profile(enable=0, reset=True)
     value = args.steps / t_total
     cap = res.cap
     dev.fill(job, fc.FCG_PREC_FP64)
     ref = dev.solve(job, mesh, tol=1e-8).cap
     e_pe, e_pd = c_errors([cap], [ref])
     log(f"C error vs tol 1e-8 FP64 solve: per-entry {e_pe:.3e}, relative to diagonal {e_pd:.3e}")
-    if prec == fc.FCG_PREC_MIXED:
-        tm = dev.mvp_timing_batch([job], fc.FCG_MVP_FP32, reps=args.reps)
-        gb = gather_bytes(mesh, k, 4, 4)
-    else:
-        tm = dev.mvp_timing(job, k, reps=args.reps)
-        gb = gather_bytes(mesh, k)
-    achieved = gb / (tm["ms_gather"] * 1e-3) / 1e9
-    log(f"mvp stages ms: {tm}")
+    g = prof["fp32" if prec == fc.FCG_PREC_MIXED else "fp64"]
+    achieved = g["gather_bytes"] / (g["gather_ms"] * 1e-3) / 1e9
+    log(f"timed-region products: {prof}")
+    tm = dev.mvp_timing(job, k, reps=args.reps) if prec == fc.FCG_PREC_FP64 else None
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -289,11 +290,15 @@ def bench_single(args, fc, dev, tmp):
         "fill_ms": 1e3 * statistics.median(fills),
         "solve_ms": 1e3 * statistics.median(solves),
         "gmres_iterations": res.iterations,
-        "mvp": {"k": k, "ms": tm["ms_total"], "stage_ms": {s: tm["ms_" + s] for s in ("scan", "charge", "gather", "down")},
-                "gather_variant": tm["variant"]},
+        "mvp": {"k": k, "launches": g["mvp_launches"], "avg_ms": g["mvp_ms"] / max(1, g["mvp_launches"]),
+                "stage_ms": None if tm is None else {s: tm["ms_" + s] for s in ("scan", "charge", "gather", "down")},
+                "gather_variant": None if tm is None else tm["variant"]},
         "c_rel_err_vs_fp64_tol1e-8": {"max_per_entry": e_pe, "max_vs_diagonal": e_pd},
-        "roofline": {"kernel": "k_gather", "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None},
+        "roofline": {"kernel": "k_gather (products of the timed solves)", "bound": "hbm", "achieved": achieved,
+                     "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "launches": g["gather_launches"],
+                     "avg_launch_ms": g["gather_ms"] / max(1, g["gather_launches"]),
+                     "alg_bytes_per_launch": g["gather_bytes"] / max(1, g["gather_launches"])},
         "cpu_baseline": cpu,
         "host_build_s": t_build,
     }
@@ -312,6 +317,7 @@ def main():
     ap.add_argument("--n3d", type=int, default=256)
     ap.add_argument("--reps", type=int, default=10, help="timed repetitions of the product for the roofline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only-mixed", action="store_true", help="batch: run only the timed mixed solves (profiling)")
     ap.add_argument("--cpu-rows", type=int, default=1, help="single-job CPU sample: right-hand sides")
     ap.add_argument("--cpu-jobs", type=int, default=1280, help="batch CPU sample: jobs (evenly strided)")
     args = ap.parse_args()
@@ -328,6 +334,9 @@ def main():
     dev = fc.Device(0)
     if args.workload == "batch":
         out = bench_batch(args, fc, dev, tmp)
+        if out is None:
+            dev.close()
+            return
     else:
         out = bench_single(args, fc, dev, tmp)
     line = {

@@ -333,6 +333,25 @@ class Device:
                     "fcg_mvp_timing_batch")
         return {f: getattr(t, f) for f, _ in _ffi.MvpTiming._fields_}
 
+    def profile(self, enable: int = -1, reset: bool = False) -> dict:
+        """Gather/product device time and algorithmic bytes of engine solves, per
+        product precision ("fp64", "fp32", "fp32_acc64"); enable 1/0 switches the
+        HIP-event accounting on/off, -1 leaves it."""
+        p = _ffi.Profile()
+        self._check(self.lib.fcg_ctx_profile(self._h, enable, 1 if reset else 0, C.byref(p)), "fcg_ctx_profile")
+        out = {}
+        for i, name in enumerate(("fp64", "fp32", "fp32_acc64")):
+            out[name] = {
+                "gather_ms": p.gather_ms[i],
+                "gather_bytes": p.gather_bytes[i],
+                "gather_launches": p.gather_launches[i],
+                "mvp_ms": p.mvp_ms[i],
+                "mvp_launches": p.mvp_launches[i],
+            }
+        out["engine"] = {"steps": p.steps, "host_plan_ms": p.host_plan_ms, "host_wait_ms": p.host_wait_ms,
+                         "host_post_ms": p.host_post_ms}
+        return out
+
     def set_gather_variant(self, job: int, variant: int) -> int:
         n = C.c_int()
         self._check(self.lib.fcg_set_gather_variant(self._h, job, variant, C.byref(n)), "fcg_set_gather_variant")

@@ -84,6 +84,20 @@ class MvpTiming(C.Structure):
     ]
 
 
+class Profile(C.Structure):
+    _fields_ = [
+        ("gather_ms", C.c_double * 3),
+        ("gather_bytes", C.c_double * 3),
+        ("gather_launches", C.c_int64 * 3),
+        ("mvp_ms", C.c_double * 3),
+        ("mvp_launches", C.c_int64 * 3),
+        ("steps", C.c_int64),
+        ("host_plan_ms", C.c_double),
+        ("host_wait_ms", C.c_double),
+        ("host_post_ms", C.c_double),
+    ]
+
+
 class HostOptions(C.Structure):
     _fields_ = [
         ("mesh_eps", C.c_double),
@@ -154,6 +168,7 @@ DEVICE_SYMBOLS = (
     "fcg_job_footprint",
     "fcg_fill_batch",
     "fcg_mvp_timing_batch",
+    "fcg_ctx_profile",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -234,6 +249,7 @@ def device_lib() -> C.CDLL:
         lib.fcg_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         ip = C.POINTER(C.c_int)
         i64p = C.POINTER(C.c_int64)
+        lib.fcg_ctx_profile.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Profile)]
         lib.fcg_mvp_timing_batch.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.POINTER(MvpTiming)]
         lib.fcg_fill_batch.argtypes = [vp, ip, C.c_int, C.c_int]
         lib.fcg_mvp_prec.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <string>
@@ -167,7 +168,9 @@ struct Tables {
         sl.used += need;
         return hipSuccess;
     }
-    // copy host data into a fresh device region (enqueued on s)
+    // copy host data into a fresh device region: enqueued on s at once, or, in
+    // deferred mode, by the next flush (one copy per slab for a whole step)
+    bool defer = false;
     hipError_t push(const void* data, size_t bytes, hipStream_t s, void** dptr) {
         char *h, *d;
         hipError_t e = take(bytes, &h, &d);
@@ -175,7 +178,16 @@ struct Tables {
         *dptr = d;
         if (!bytes) return hipSuccess;
         std::memcpy(h, data, bytes);
+        if (defer) return hipSuccess;
         return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
+    }
+    hipError_t flush(hipStream_t s) {
+        for (auto& sl : slabs) {
+            if (!sl->used) continue;
+            hipError_t e = hipMemcpyAsync(sl->dev.p, sl->host.p, sl->used, hipMemcpyHostToDevice, s);
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     }
     // pinned host area for a device-to-host copy (valid until reset)
     hipError_t host_area(size_t bytes, void** hptr) {
@@ -204,7 +216,7 @@ struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
-        col_range, self, diag, coef64, coef32, perm, iperm;
+        col_range, self, diag, coef64, coef32, perm, iperm, rhs_range;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -222,7 +234,7 @@ struct Job {
         return (int64_t)(nodes.bytes + left.bytes + right.bytes + parent.bytes + leaf_lo.bytes + leaf_hi.bytes +
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
-                         perm.bytes + iperm.bytes + scr.bytes());
+                         perm.bytes + iperm.bytes + rhs_range.bytes + scr.bytes());
     }
 
     BJob rec(const Scratch& sc, const double* q, double* v) const {
@@ -270,6 +282,8 @@ struct Job {
 struct fcg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    bool profiling = false;
+    fcg_profile prof{};
     Pool pool;  // declared before everything that returns blocks to it
     std::vector<std::unique_ptr<Job>> jobs;
     std::string err;
@@ -315,7 +329,8 @@ int ensure_scratch(fcg_ctx* ctx, Scratch& sc, int N, int M, int k, bool need32) 
     const int nblk = (N + fcg::kScanTile - 1) / fcg::kScanTile;
     const int kp = fcg::padded_k(k);
     if (sc.k == k && (!need32 || sc.charge32.p)) return FCG_OK;
-    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    // buffers already in use may be read by queued kernels: drain before replacing
+    if (sc.k != 0) HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     Pool* pl = &ctx->pool;
     if (sc.k != k) {
         HIPCHECK(ctx, sc.prefix.get(pl, sizeof(DD) * (size_t)k * (N + 1)));
@@ -393,6 +408,54 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     out->any2d = any2d;
     return FCG_OK;
 }
+
+// Algorithmic bytes of one gather launch over a batch (see fcg_profile).
+double gather_alg_bytes(const std::vector<BatchEntry>& ents, int prec) {
+    const double bc = prec == fcg::kMvp64 ? 8.0 : 4.0;
+    const double bn = prec == fcg::kMvp32 ? 4.0 : 8.0;
+    double b = 0.0;
+    for (const auto& e : ents)
+        b += (double)e.j->L * (bc + 4.0) + 16.0 * e.j->M + 8.0 + 2.0 * e.j->M * (double)e.sc->k * bn;
+    return b;
+}
+
+// Events around one product of the engine when the context is profiling.
+struct ProfEvents {
+    hipEvent_t e[5] = {};
+    bool on = false;
+    int prec = 0;
+    double bytes = 0.0;
+    ~ProfEvents() {
+        for (auto x : e)
+            if (x) (void)hipEventDestroy(x);
+    }
+    hipError_t arm(bool enable, int p, double b) {
+        on = enable;
+        prec = p;
+        bytes = b;
+        if (!on) return hipSuccess;
+        for (auto& x : e)
+            if (!x) {
+                hipError_t r = hipEventCreate(&x);
+                if (r != hipSuccess) return r;
+            }
+        return hipSuccess;
+    }
+    hipEvent_t* events() { return on ? e : nullptr; }
+    // after the stream has been synchronised
+    void collect(fcg_profile& pr) {
+        if (!on) return;
+        float g = 0.f, t = 0.f;
+        if (hipEventElapsedTime(&g, e[2], e[3]) != hipSuccess) return;
+        if (hipEventElapsedTime(&t, e[0], e[4]) != hipSuccess) return;
+        pr.gather_ms[prec] += g;
+        pr.gather_bytes[prec] += bytes;
+        pr.gather_launches[prec] += 1;
+        pr.mvp_ms[prec] += t;
+        pr.mvp_launches[prec] += 1;
+        on = false;
+    }
+};
 
 // Validate the structural invariants the kernels rely on (bounds of every index).
 int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
@@ -518,14 +581,11 @@ struct Solve {
     int64_t tag = 0;
     int k = 0, N = 0, ncol = 0;
     bool mixed = false, jac = true, want_x = false;
-    PBuf b, x, r, w;
+    PBuf x, r, w;
     Scratch scr;
-    // Krylov basis: blocks of vectors, host list and device address table
+    // Krylov basis: blocks of vectors and their addresses (sent with each step's tables)
     std::vector<std::unique_ptr<PBuf>> qblocks;
     std::vector<double*> hq;
-    PBuf qtab;
-    int qcap = 0;
-    int qsynced = 0;  // entries of the device table already written
     int phase = kInit;
     int i = 0;  // Arnoldi step index within the cycle (the next product uses Q[i])
     int cycles = 0;
@@ -543,7 +603,7 @@ struct Solve {
     clk::time_point t0;
     int hoff = 0;
     int64_t bytes() const {
-        int64_t s = (int64_t)(b.r + x.r + r.r + w.r + qtab.r + scr.bytes());
+        int64_t s = (int64_t)(x.r + r.r + w.r + scr.bytes());
         for (auto& q : qblocks) s += (int64_t)q->r;
         return s;
     }
@@ -561,6 +621,7 @@ struct Ready {
 
 struct fcg_engine {
     fcg_ctx* ctx = nullptr;
+    ProfEvents pe_res, pe_arn;
     fcg_solve_opts opts{};
     std::vector<std::unique_ptr<Solve>> active;
     std::deque<Ready> ready;
@@ -571,14 +632,15 @@ struct fcg_engine {
 
 namespace {
 
+// step buffers grow geometrically: a step's size creeps up with the Arnoldi index
 int ensure_pbuf(fcg_ctx* ctx, PBuf& b, size_t bytes) {
     if (b.p && b.r >= bytes) return FCG_OK;
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-    HIPCHECK(ctx, b.get(&ctx->pool, bytes));
+    HIPCHECK(ctx, b.get(&ctx->pool, std::max(bytes, 2 * b.r)));
     return FCG_OK;
 }
 
-// make sure basis vector index `idx` exists and its address is in the device table
+// make sure basis vector index `idx` exists (device memory from the pool only)
 int ensure_basis(fcg_ctx* ctx, Solve& S, int idx) {
     const size_t vec = (size_t)S.k * S.N;
     while ((int)S.hq.size() <= idx) {
@@ -588,32 +650,16 @@ int ensure_basis(fcg_ctx* ctx, Solve& S, int idx) {
         HIPCHECK(ctx, S.qblocks.back()->get(&ctx->pool, sizeof(double) * vec * per));
         for (size_t c = 0; c < per; ++c) S.hq.push_back(S.qblocks.back()->as<double>() + c * vec);
     }
-    if ((int)S.hq.size() > S.qcap) {
-        // a larger table; kernels already enqueued keep reading the old one, so the
-        // stream is drained before the old block returns to the pool
-        HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-        const int cap = (int)S.hq.size() + 64;
-        S.qtab.release();
-        HIPCHECK(ctx, S.qtab.get(&ctx->pool, sizeof(double*) * (size_t)cap));
-        S.qcap = cap;
-        S.qsynced = 0;
-    }
-    if (S.qsynced < (int)S.hq.size()) {
-        // new entries are not read by any enqueued kernel yet
-        HIPCHECK(ctx, hipMemcpy(S.qtab.as<double*>() + S.qsynced, S.hq.data() + S.qsynced,
-                                sizeof(double*) * (S.hq.size() - S.qsynced), hipMemcpyHostToDevice));
-        S.qsynced = (int)S.hq.size();
-    }
     return FCG_OK;
 }
 
-KJob kjob_of(Solve& S, double* w, int nv, int zoff) {
+KJob kjob_of(Solve& S, double* const* qtab, double* w, int nv, int zoff) {
     KJob J{};
-    J.Q = S.qtab.as<double*>();
+    J.Q = qtab;
     J.w = w;
     J.x = S.x.as<double>();
     J.r = S.r.as<double>();
-    J.b = S.b.as<double>();
+    J.brange = S.J->rhs_range.as<int32_t>();
     J.self = S.J->self.as<double>();
     J.N = S.N;
     J.k = S.k;
@@ -638,9 +684,21 @@ int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, Krylo
     std::vector<Seg> segs;
     std::vector<Chunk> chunks;
     int maxnv = 0;
+    // basis address tables of every entry, back to back
+    std::vector<double*> qaddr;
+    std::vector<size_t> qoff;
     for (size_t e = 0; e < ents.size(); ++e) {
         Solve& S = *ents[e].S;
-        kj.push_back(kjob_of(S, ents[e].w, ents[e].nv, ents[e].zoff));
+        const size_t need = std::min(S.hq.size(), (size_t)std::max(ents[e].nv + 1, S.nz));
+        qoff.push_back(qaddr.size());
+        qaddr.insert(qaddr.end(), S.hq.begin(), S.hq.begin() + need);
+    }
+    void* qd = nullptr;
+    HIPCHECK(ctx, tabs.push(qaddr.data(), sizeof(double*) * qaddr.size(), ctx->stream, &qd));
+    double* const* qbase = reinterpret_cast<double* const*>(qd);
+    for (size_t e = 0; e < ents.size(); ++e) {
+        Solve& S = *ents[e].S;
+        kj.push_back(kjob_of(S, qbase + qoff[e], ents[e].w, ents[e].nv, ents[e].zoff));
         maxnv = std::max(maxnv, ents[e].nv);
         for (int r = 0; r < S.k; ++r) {
             Seg sg;
@@ -704,7 +762,6 @@ void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
 
 // back substitution of every rhs; z[jj][r], zero beyond each rhs's own count
 void prepare_update(Solve& S) {
-    const int ld = S.cap_iter + 1;
     int nz = 0;
     for (int q = 0; q < S.k; ++q) nz = std::max(nz, S.last_i[(size_t)q] + 1);
     S.nz = nz;
@@ -715,8 +772,9 @@ void prepare_update(Solve& S) {
         std::vector<double> zz(S.g[(size_t)q].begin(), S.g[(size_t)q].begin() + it + 1);
         const std::vector<double>& Hq = S.H[(size_t)q];
         for (int ii = it; ii >= 0; --ii) {
-            zz[(size_t)ii] = zz[(size_t)ii] / Hq[(size_t)ii * ld + ii];
-            for (int jj = ii - 1; jj >= 0; --jj) zz[(size_t)jj] -= Hq[(size_t)ii * ld + jj] * zz[(size_t)ii];
+            const double* col = Hq.data() + (size_t)ii * (ii + 1) / 2;
+            zz[(size_t)ii] = zz[(size_t)ii] / col[ii];
+            for (int jj = ii - 1; jj >= 0; --jj) zz[(size_t)jj] -= col[jj] * zz[(size_t)ii];
         }
         for (int jj = 0; jj <= it; ++jj) S.z[(size_t)jj * S.k + q] = zz[(size_t)jj];
     }
@@ -726,7 +784,6 @@ void prepare_update(Solve& S) {
 // Givens update of one Arnoldi step of every rhs (h1 + h2 = column, |w| = subdiagonal)
 void givens_step(Solve& S, const double* h1, const double* h2, const double* w2) {
     const int i = S.i;
-    const int ld = S.cap_iter + 1;
     const int k = S.k;
     S.d.assign((size_t)k, 0.0);
     for (int q = 0; q < k; ++q) {
@@ -734,8 +791,9 @@ void givens_step(Solve& S, const double* h1, const double* h2, const double* w2)
         S.d[(size_t)q] = hn;
         if (S.done[(size_t)q]) continue;
         std::vector<double>& Hq = S.H[(size_t)q];
-        if (Hq.size() < (size_t)(i + 1) * ld) Hq.resize(std::max(Hq.size() * 2, (size_t)(i + 1) * ld), 0.0);
-        double* colv = Hq.data() + (size_t)i * ld;
+        // rotated columns packed: column i (rows 0..i) at i(i+1)/2
+        Hq.resize((size_t)(i + 1) * (i + 2) / 2);
+        double* colv = Hq.data() + (size_t)i * (i + 1) / 2;
         for (int jj = 0; jj <= i; ++jj) colv[jj] = h1[(size_t)jj * k + q] + h2[(size_t)jj * k + q];
         auto& c = S.cs[(size_t)q];
         auto& sv = S.sn[(size_t)q];
@@ -778,21 +836,15 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
     RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, S->k, mixed));
     const size_t vec = (size_t)S->k * S->N;
-    HIPCHECK(ctx, S->b.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->x.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->r.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->w.get(&ctx->pool, sizeof(double) * vec));
-    {
-        // unit potential on each conductor's panels (SolveCapacitance.cpp:1209-1211)
-        std::vector<double> hb(vec, 0.0);
-        S->normb.assign((size_t)S->k, 0.0);
-        for (int q = 0; q < S->k; ++q) {
-            const auto rg = J->rhs[(size_t)q];
-            for (int n = rg.first; n < rg.second; ++n) hb[(size_t)q * S->N + n] = 1.0;
-            S->normb[(size_t)q] = std::sqrt((double)(rg.second - rg.first));
-        }
-        HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
-        HIPCHECK(ctx, hipMemcpy(S->b.p, hb.data(), sizeof(double) * vec, hipMemcpyHostToDevice));
+    // unit potential on each conductor's panels (SolveCapacitance.cpp:1209-1211); the
+    // kernels form b from the job's leaf ranges, so only its norm is kept here
+    S->normb.assign((size_t)S->k, 0.0);
+    for (int q = 0; q < S->k; ++q) {
+        const auto rg = J->rhs[(size_t)q];
+        S->normb[(size_t)q] = std::sqrt((double)(rg.second - rg.first));
     }
     RCCHECK(ensure_basis(ctx, *S, 1));
     S->H.assign((size_t)S->k, {});
@@ -809,8 +861,13 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
 int engine_step(fcg_engine* E) {
     fcg_ctx* ctx = E->ctx;
     hipStream_t s = ctx->stream;
+    const auto t_plan = clk::now();
     E->tabs.reset();  // the previous step ended with a synchronisation
+    E->tabs.defer = true;
     const bool mixed = E->opts.precision == FCG_PREC_MIXED;
+    // Every table of the step is written to pinned staging first and reaches the
+    // device in one copy; the launches are queued here and issued after that copy.
+    std::vector<std::function<hipError_t()>> q;
 
     // ---- A: pending basis scalings, solution updates and starts
     {
@@ -828,7 +885,7 @@ int engine_step(fcg_engine* E) {
             RCCHECK(make_krylov(ctx, E->tabs, sc, &kv));
             void* dd;
             HIPCHECK(ctx, E->tabs.push(dvals.data(), sizeof(double) * dvals.size(), s, &dd));
-            HIPCHECK(ctx, fcg::launch_scale_new(kv, reinterpret_cast<double*>(dd), s));
+            q.push_back([=] { return fcg::launch_scale_new(kv, reinterpret_cast<double*>(dd), s); });
             for (auto& e : sc) e.S->need_scale = false;
         }
         std::vector<KEnt> up;
@@ -844,7 +901,7 @@ int engine_step(fcg_engine* E) {
             RCCHECK(make_krylov(ctx, E->tabs, up, &kv));
             void* dz;
             HIPCHECK(ctx, E->tabs.push(zall.data(), sizeof(double) * zall.size(), s, &dz));
-            HIPCHECK(ctx, fcg::launch_xupdate(kv, reinterpret_cast<double*>(dz), s));
+            q.push_back([=] { return fcg::launch_xupdate(kv, reinterpret_cast<double*>(dz), s); });
             for (auto& e : up) {
                 e.S->nz = 0;
                 e.S->phase = e.S->mixed ? kResid : kFinish;
@@ -856,7 +913,7 @@ int engine_step(fcg_engine* E) {
         if (!init.empty()) {
             KrylovView kv;
             RCCHECK(make_krylov(ctx, E->tabs, init, &kv));
-            HIPCHECK(ctx, fcg::launch_init_x(kv, s));
+            q.push_back([=] { return fcg::launch_init_x(kv, s); });
             for (auto& e : init) e.S->phase = kResid;
         }
     }
@@ -872,16 +929,24 @@ int engine_step(fcg_engine* E) {
         HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * nval, &h));
         fin_tmp.emplace_back(new PBuf());
         HIPCHECK(ctx, fin_tmp.back()->get(&ctx->pool, sizeof(double) * nval));
-        HIPCHECK(ctx, fcg::launch_charge_sums(S.x.as<double>(), S.J->leaf_w.as<double>(), S.J->col_range.as<int32_t>(),
-                                              S.N, S.k, S.ncol, fin_tmp.back()->as<double>(), s));
-        HIPCHECK(ctx, hipMemcpyAsync(h, fin_tmp.back()->p, sizeof(double) * nval, hipMemcpyDeviceToHost, s));
+        {
+            const double* x = S.x.as<double>();
+            const double* w = S.J->leaf_w.as<double>();
+            const int32_t* cr = S.J->col_range.as<int32_t>();
+            double* out = fin_tmp.back()->as<double>();
+            const int N = S.N, k = S.k, nc = S.ncol;
+            q.push_back([=] { return fcg::launch_charge_sums(x, w, cr, N, k, nc, out, s); });
+            q.push_back([=] { return hipMemcpyAsync(h, out, sizeof(double) * nval, hipMemcpyDeviceToHost, s); });
+        }
         fin.emplace_back(&S, reinterpret_cast<double*>(h));
         double* hx = nullptr;
         if (S.want_x) {
             void* hxv;
             HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * (size_t)S.k * S.N, &hxv));
             hx = reinterpret_cast<double*>(hxv);
-            HIPCHECK(ctx, hipMemcpyAsync(hx, S.x.p, sizeof(double) * (size_t)S.k * S.N, hipMemcpyDeviceToHost, s));
+            const void* xs = S.x.p;
+            const size_t nx = sizeof(double) * (size_t)S.k * S.N;
+            q.push_back([=] { return hipMemcpyAsync(hx, xs, nx, hipMemcpyDeviceToHost, s); });
         }
         fin_x.push_back(hx);
     }
@@ -913,12 +978,17 @@ int engine_step(fcg_engine* E) {
         BatchView bv;
         RCCHECK(make_batch(ctx, E->tabs, be, &bv));
         const int var = res.size() == 1 ? res[0]->J->gather_variant : 0;
-        HIPCHECK(ctx, fcg::launch_mvp(bv, mixed ? fcg::kMvp32Acc : fcg::kMvp64, 0, var, s));
+        const int prec = mixed ? fcg::kMvp32Acc : fcg::kMvp64;
+        HIPCHECK(ctx, E->pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
+        hipEvent_t* ev = E->pe_res.events();
+        q.push_back([=] { return fcg::launch_mvp(bv, prec, 0, var, s, ev); });
         KrylovView kv;
         RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
         RCCHECK(ensure_pbuf(ctx, E->partial_a, sizeof(double) * (size_t)kv.nchunks));
-        HIPCHECK(ctx, fcg::launch_resid(kv, E->partial_a.as<double>(), s));
-        HIPCHECK(ctx, fcg::launch_reduce_norm(kv, E->partial_a.as<double>(), E->hbuf.as<double>(), s));
+        double* pa = E->partial_a.as<double>();
+        double* hb = E->hbuf.as<double>();
+        q.push_back([=] { return fcg::launch_resid(kv, pa, s); });
+        q.push_back([=] { return fcg::launch_reduce_norm(kv, pa, hb, s); });
     }
 
     // ---- C: one Arnoldi step (product + two Gram-Schmidt passes) of every active cycle
@@ -934,30 +1004,42 @@ int engine_step(fcg_engine* E) {
         BatchView bv;
         RCCHECK(make_batch(ctx, E->tabs, be, &bv));
         const int var = arn.size() == 1 ? arn[0]->J->gather_variant : 0;
-        HIPCHECK(ctx, fcg::launch_mvp(bv, mixed ? fcg::kMvp32 : fcg::kMvp64, arn[0]->jac ? 1 : 0, var, s));
+        const int prec = mixed ? fcg::kMvp32 : fcg::kMvp64;
+        HIPCHECK(ctx, E->pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
+        hipEvent_t* ev = E->pe_arn.events();
+        const int jac = arn[0]->jac ? 1 : 0;
+        q.push_back([=] { return fcg::launch_mvp(bv, prec, jac, var, s, ev); });
         KrylovView kv;
         RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
         RCCHECK(ensure_pbuf(ctx, E->partial_b, sizeof(double) * (size_t)kv.nchunks * std::max(kv.maxnv, 1)));
         double* pb = E->partial_b.as<double>();
         double* hb = E->hbuf.as<double>();
-        HIPCHECK(ctx, fcg::launch_dots(kv, pb, s));
-        HIPCHECK(ctx, fcg::launch_reduce(kv, pb, hb, 0, s));
-        HIPCHECK(ctx, fcg::launch_update(kv, hb, 0, 0, pb, s));
-        HIPCHECK(ctx, fcg::launch_dots(kv, pb, s));
-        HIPCHECK(ctx, fcg::launch_reduce(kv, pb, hb, 1, s));
-        HIPCHECK(ctx, fcg::launch_update(kv, hb, 1, 1, pb, s));
-        HIPCHECK(ctx, fcg::launch_reduce_norm(kv, pb, hb, s));
+        q.push_back([=] { return fcg::launch_dots(kv, pb, s); });
+        q.push_back([=] { return fcg::launch_reduce(kv, pb, hb, 0, s); });
+        q.push_back([=] { return fcg::launch_update(kv, hb, 0, 0, pb, s); });
+        q.push_back([=] { return fcg::launch_dots(kv, pb, s); });
+        q.push_back([=] { return fcg::launch_reduce(kv, pb, hb, 1, s); });
+        q.push_back([=] { return fcg::launch_update(kv, hb, 1, 1, pb, s); });
+        q.push_back([=] { return fcg::launch_reduce_norm(kv, pb, hb, s); });
     }
 
-    // ---- D: one copy back, one synchronisation
+    // ---- D: one table copy, the queued launches, one copy back, one synchronisation
     double* hh = nullptr;
     if (hoff > 0) {
         void* h;
         HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * (size_t)hoff, &h));
         hh = reinterpret_cast<double*>(h);
-        HIPCHECK(ctx, hipMemcpyAsync(hh, E->hbuf.p, sizeof(double) * (size_t)hoff, hipMemcpyDeviceToHost, s));
+        const void* src = E->hbuf.p;
+        const size_t nb = sizeof(double) * (size_t)hoff;
+        q.push_back([=] { return hipMemcpyAsync(hh, src, nb, hipMemcpyDeviceToHost, s); });
     }
+    HIPCHECK(ctx, E->tabs.flush(s));
+    for (auto& f : q) HIPCHECK(ctx, f());
+    const auto t_wait = clk::now();
     HIPCHECK(ctx, hipStreamSynchronize(s));
+    const auto t_post = clk::now();
+    E->pe_res.collect(ctx->prof);
+    E->pe_arn.collect(ctx->prof);
 
     // ---- host bookkeeping
     for (size_t f = 0; f < fin.size(); ++f) {
@@ -1025,6 +1107,13 @@ int engine_step(fcg_engine* E) {
     E->active.erase(std::remove_if(E->active.begin(), E->active.end(),
                                    [](const std::unique_ptr<Solve>& u) { return u->phase == kDone; }),
                     E->active.end());
+    if (ctx->profiling) {
+        const auto t_end = clk::now();
+        ctx->prof.steps += 1;
+        ctx->prof.host_plan_ms += std::chrono::duration<double, std::milli>(t_wait - t_plan).count();
+        ctx->prof.host_wait_ms += std::chrono::duration<double, std::milli>(t_post - t_wait).count();
+        ctx->prof.host_post_ms += std::chrono::duration<double, std::milli>(t_end - t_post).count();
+    }
     return FCG_OK;
 }
 
@@ -1108,6 +1197,16 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     }
     if (ranges.empty()) return fail(ctx, FCG_INVALID_ARGUMENT, "job has no conductor");
     HIPCHECK(ctx, upload(j->col_range, ranges.data(), ranges.size(), s));
+    {
+        std::vector<int32_t> rr;
+        for (const auto& rg : j->rhs) {
+            rr.push_back(rg.first);
+            rr.push_back(rg.second);
+        }
+        if (rr.empty()) rr.assign(2, 0);
+        HIPCHECK(ctx, upload(j->rhs_range, rr.data(), rr.size(), s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));  // rr is released at the end of this block
+    }
     HIPCHECK(ctx, hipStreamSynchronize(s));
     // destination node of every link (used by the fill)
     ctx->tabs.reset();
@@ -1344,6 +1443,14 @@ int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec,
     out->reps = reps;
     out->k = ktot;
     out->variant = var;
+    return FCG_OK;
+}
+
+int fcg_ctx_profile(fcg_ctx* ctx, int enable, int reset, fcg_profile* out) {
+    if (!ctx) return FCG_INVALID_ARGUMENT;
+    if (out) *out = ctx->prof;
+    if (reset) ctx->prof = fcg_profile{};
+    if (enable >= 0) ctx->profiling = enable != 0;
     return FCG_OK;
 }
 

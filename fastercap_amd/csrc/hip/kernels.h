@@ -85,11 +85,11 @@ struct BatchView {
 
 // Krylov bookkeeping of one job for the batched GMRES kernels.
 struct KJob {
-    double* const* Q;    // device array of basis vector addresses ([k][N] each)
-    double* w;           // work vector [k][N]
-    double* x;           // current solution [k][N]
-    double* r;           // residual [k][N]
-    const double* b;     // right-hand sides [k][N]
+    double* const* Q;       // device array of basis vector addresses ([k][N] each)
+    double* w;              // work vector [k][N]
+    double* x;              // current solution [k][N]
+    double* r;              // residual [k][N]
+    const int32_t* brange;  // [2k] right-hand side r is 1 on leaves [brange[2r], brange[2r+1]), 0 elsewhere
     const double* self;  // self coefficients [N] (Jacobi)
     int N, k, nv;        // nv = basis vectors to project against this step
     int hoff;            // offset of this job's block in the step buffer

@@ -613,8 +613,11 @@ __global__ void k_init_x(KrylovView kv) {
     const Seg sg = kv.segs[ch.seg];
     const KJob kj = kv.kjobs[sg.job];
     const size_t o = (size_t)sg.r * kj.N;
-    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += blockDim.x)
-        kj.x[o + n] = kj.jacobi ? kj.b[o + n] / kj.self[n] : kj.b[o + n];
+    const int lo = kj.brange[2 * sg.r], hi = kj.brange[2 * sg.r + 1];
+    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += blockDim.x) {
+        const double b = (n >= lo && n < hi) ? 1.0 : 0.0;
+        kj.x[o + n] = kj.jacobi ? b / kj.self[n] : b;
+    }
 }
 
 __global__ void __launch_bounds__(kKThreads) k_resid(KrylovView kv, double* __restrict__ partial) {
@@ -623,9 +626,10 @@ __global__ void __launch_bounds__(kKThreads) k_resid(KrylovView kv, double* __re
     const Seg sg = kv.segs[ch.seg];
     const KJob kj = kv.kjobs[sg.job];
     const size_t o = (size_t)sg.r * kj.N;
+    const int lo = kj.brange[2 * sg.r], hi = kj.brange[2 * sg.r + 1];
     double ss = 0.0;
     for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += kKThreads) {
-        const double rr = kj.b[o + n] - kj.w[o + n];
+        const double rr = ((n >= lo && n < hi) ? 1.0 : 0.0) - kj.w[o + n];
         kj.r[o + n] = rr;
         ss += rr * rr;
     }
