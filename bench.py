@@ -88,6 +88,10 @@ def bench_batch(args, fc, dev, tmp):
     MIX, F64 = fc.FCG_PREC_MIXED, fc.FCG_PREC_FP64
     tol = args.tol
 
+    dev.fill_batch(jobs, MIX)
+    tuned = {"fp32": dev.autotune_batch(jobs, fc.FCG_MVP_FP32),
+             "fp32_acc64": dev.autotune_batch(jobs, fc.FCG_MVP_FP32_ACC64)}
+    log(f"batch gather shapes (variant, gather ms): {tuned}")
     for _ in range(max(1, args.warmup)):
         dev.fill_batch(jobs, MIX)
         dev.solve_batch(jobs, meshes, tol=tol, precision=MIX)
@@ -123,6 +127,8 @@ def bench_batch(args, fc, dev, tmp):
         plain.append(dev.solve(j, m, tol=tol))
     t_plain = time.perf_counter() - a
     log(f"plain FP64, one job at a time: {t_plain:.2f}s ({nj / t_plain:.1f} solves/s)")
+    tuned["fp64"] = dev.autotune_batch(jobs, fc.FCG_MVP_FP64)
+    dev.solve_batch(jobs, meshes, tol=tol, precision=F64)  # warm
     a = time.perf_counter()
     dev.fill_batch(jobs, F64)
     bres = dev.solve_batch(jobs, meshes, tol=tol, precision=F64)
@@ -174,6 +180,7 @@ def bench_batch(args, fc, dev, tmp):
             "precision": "mixed: FP32 far-field links and inner products, FP64 near field, Arnoldi, refinement",
         },
         "latency_ms": {"p50": 1e3 * pct(lat, 50), "p99": 1e3 * pct(lat, 99)},
+        "batch_gather_shape": {k: v[0] for k, v in tuned.items()},
         "fp64_plain_solves_per_s": nj / t_plain,
         "fp64_batched_solves_per_s": nj / t_b64,
         "speedup_vs_fp64_plain": value / (nj / t_plain),

@@ -352,6 +352,18 @@ class Device:
                          "host_post_ms": p.host_post_ms}
         return out
 
+    def autotune_batch(self, jobs, mvp_prec: int = FCG_MVP_FP64, reps: int = 3):
+        """Pick the gather shape of products over several jobs (engine steps)."""
+        jobs = list(jobs)
+        ja = (C.c_int * len(jobs))(*jobs)
+        v, ms = C.c_int(), C.c_double()
+        self._check(self.lib.fcg_autotune_batch(self._h, ja, len(jobs), mvp_prec, reps, C.byref(v), C.byref(ms)),
+                    "fcg_autotune_batch")
+        return v.value, ms.value
+
+    def set_batch_variant(self, mvp_prec: int, variant: int):
+        self._check(self.lib.fcg_set_batch_variant(self._h, mvp_prec, variant), "fcg_set_batch_variant")
+
     def set_gather_variant(self, job: int, variant: int) -> int:
         n = C.c_int()
         self._check(self.lib.fcg_set_gather_variant(self._h, job, variant, C.byref(n)), "fcg_set_gather_variant")

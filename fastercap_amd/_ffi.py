@@ -169,6 +169,8 @@ DEVICE_SYMBOLS = (
     "fcg_fill_batch",
     "fcg_mvp_timing_batch",
     "fcg_ctx_profile",
+    "fcg_autotune_batch",
+    "fcg_set_batch_variant",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -249,6 +251,9 @@ def device_lib() -> C.CDLL:
         lib.fcg_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         ip = C.POINTER(C.c_int)
         i64p = C.POINTER(C.c_int64)
+        lib.fcg_autotune_batch.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double)]
+        lib.fcg_set_batch_variant.argtypes = [vp, C.c_int, C.c_int]
         lib.fcg_ctx_profile.argtypes = [vp, C.c_int, C.c_int, C.POINTER(Profile)]
         lib.fcg_mvp_timing_batch.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.POINTER(MvpTiming)]
         lib.fcg_fill_batch.argtypes = [vp, ip, C.c_int, C.c_int]

@@ -227,8 +227,10 @@ struct Job {
     int ncol = 0;
     int first_cond_leaf = 0;
     Scratch scr;  // product scratch of the stand-alone entry points
-    int gather_variant = 0;
-    int tuned_k = 0;  // 0 = not tuned, -1 = pinned by the caller
+    // gather lane shape per product precision (index fcg::MvpPrec) and whether it
+    // was chosen by measurement (1), pinned by the caller (-1) or is the default (0)
+    int gather_variant[3] = {0, 0, 0};
+    int tuned[3] = {0, 0, 0};
 
     int64_t resident_bytes() const {
         return (int64_t)(nodes.bytes + left.bytes + right.bytes + parent.bytes + leaf_lo.bytes + leaf_hi.bytes +
@@ -284,6 +286,7 @@ struct fcg_ctx {
     hipStream_t stream = nullptr;
     bool profiling = false;
     fcg_profile prof{};
+    int batch_variant[3] = {0, 0, 0};  // gather lane shape of products over several jobs
     Pool pool;  // declared before everything that returns blocks to it
     std::vector<std::unique_ptr<Job>> jobs;
     std::string err;
@@ -564,7 +567,7 @@ int run_mvp(fcg_ctx* ctx, Job& j, int prec, const double* dq, double* dv, int k,
     ctx->tabs.reset();
     BatchView bv;
     RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, dq, dv}}, &bv));
-    HIPCHECK(ctx, fcg::launch_mvp(bv, prec, jacobi, j.gather_variant, s, ev));
+    HIPCHECK(ctx, fcg::launch_mvp(bv, prec, jacobi, j.gather_variant[prec], s, ev));
     return FCG_OK;
 }
 
@@ -977,8 +980,8 @@ int engine_step(fcg_engine* E) {
         }
         BatchView bv;
         RCCHECK(make_batch(ctx, E->tabs, be, &bv));
-        const int var = res.size() == 1 ? res[0]->J->gather_variant : 0;
         const int prec = mixed ? fcg::kMvp32Acc : fcg::kMvp64;
+        const int var = res.size() == 1 ? res[0]->J->gather_variant[prec] : ctx->batch_variant[prec];
         HIPCHECK(ctx, E->pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
         hipEvent_t* ev = E->pe_res.events();
         q.push_back([=] { return fcg::launch_mvp(bv, prec, 0, var, s, ev); });
@@ -1003,8 +1006,8 @@ int engine_step(fcg_engine* E) {
         }
         BatchView bv;
         RCCHECK(make_batch(ctx, E->tabs, be, &bv));
-        const int var = arn.size() == 1 ? arn[0]->J->gather_variant : 0;
         const int prec = mixed ? fcg::kMvp32 : fcg::kMvp64;
+        const int var = arn.size() == 1 ? arn[0]->J->gather_variant[prec] : ctx->batch_variant[prec];
         HIPCHECK(ctx, E->pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
         hipEvent_t* ev = E->pe_arn.events();
         const int jac = arn[0]->jac ? 1 : 0;
@@ -1380,12 +1383,16 @@ int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* o
     out->ms_total = tot / reps;
     out->reps = reps;
     out->k = k;
-    out->variant = j->gather_variant;
+    out->variant = j->gather_variant[prec];
     return FCG_OK;
 }
 
-int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, fcg_mvp_timing* out) {
-    if (!ctx || !jobs || njobs < 1 || reps < 1 || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+namespace {
+// Time batched products over `jobs` (each on as many random right-hand sides as it
+// has conductor solves) for every gather shape in `vars`.
+int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, const std::vector<int>& vars,
+               std::vector<fcg_mvp_timing>& outs) {
+    if (!ctx || !jobs || njobs < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
         return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
@@ -1414,35 +1421,78 @@ int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec,
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->tabs.reset();
     std::vector<BatchEntry> ents;
-    for (size_t i = 0; i < js.size(); ++i) ents.push_back(BatchEntry{js[i], &js[i]->scr, qs[i]->as<double>(), vs[i]->as<double>()});
+    for (size_t i = 0; i < js.size(); ++i)
+        ents.push_back(BatchEntry{js[i], &js[i]->scr, qs[i]->as<double>(), vs[i]->as<double>()});
     BatchView bv;
     RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv));
-    const int var = njobs == 1 ? js[0]->gather_variant : 0;
     Events5 ev;
     for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
-    HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s));
-    double acc[4] = {0, 0, 0, 0}, tot = 0;
-    for (int r = 0; r < reps; ++r) {
-        HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s, ev.e));
-        HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
-        for (int st = 0; st < 4; ++st) {
+    outs.clear();
+    for (int var : vars) {
+        HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s));
+        double acc[4] = {0, 0, 0, 0}, tot = 0;
+        for (int r = 0; r < reps; ++r) {
+            HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s, ev.e));
+            HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
+            for (int st = 0; st < 4; ++st) {
+                float ms = 0;
+                HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[st], ev.e[st + 1]));
+                acc[st] += ms;
+            }
             float ms = 0;
-            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[st], ev.e[st + 1]));
-            acc[st] += ms;
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[0], ev.e[4]));
+            tot += ms;
         }
-        float ms = 0;
-        HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[0], ev.e[4]));
-        tot += ms;
+        fcg_mvp_timing o{};
+        o.ms_scan = acc[0] / reps;
+        o.ms_charge = acc[1] / reps;
+        o.ms_gather = acc[2] / reps;
+        o.ms_down = acc[3] / reps;
+        o.ms_total = tot / reps;
+        o.reps = reps;
+        o.k = ktot;
+        o.variant = var;
+        outs.push_back(o);
     }
     HIPCHECK(ctx, hipStreamSynchronize(s));
-    out->ms_scan = acc[0] / reps;
-    out->ms_charge = acc[1] / reps;
-    out->ms_gather = acc[2] / reps;
-    out->ms_down = acc[3] / reps;
-    out->ms_total = tot / reps;
-    out->reps = reps;
-    out->k = ktot;
-    out->variant = var;
+    return FCG_OK;
+}
+}  // namespace
+
+int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, fcg_mvp_timing* out) {
+    if (!out || !jobs || njobs < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
+    Job* j0 = get_job(ctx, jobs[0]);
+    if (!j0) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+    const int var = njobs == 1 ? j0->gather_variant[mvp_prec] : ctx->batch_variant[mvp_prec];
+    std::vector<fcg_mvp_timing> outs;
+    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, reps, {var}, outs));
+    *out = outs[0];
+    return FCG_OK;
+}
+
+int fcg_autotune_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, int* variant,
+                       double* best_ms) {
+    std::vector<int> vars;
+    for (int v = 0; v < fcg::gather_variant_count(); ++v) vars.push_back(v);
+    std::vector<fcg_mvp_timing> outs;
+    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, reps, vars, outs));
+    int best = 0;
+    for (size_t i = 1; i < outs.size(); ++i)
+        if (outs[i].ms_gather < outs[(size_t)best].ms_gather) best = (int)i;
+    ctx->batch_variant[mvp_prec] = outs[(size_t)best].variant;
+    if (variant) *variant = outs[(size_t)best].variant;
+    if (best_ms) *best_ms = outs[(size_t)best].ms_gather;
+    return FCG_OK;
+}
+
+int fcg_set_batch_variant(fcg_ctx* ctx, int mvp_prec, int variant) {
+    if (!ctx) return FCG_INVALID_ARGUMENT;
+    if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
+    if (variant < 0 || variant >= fcg::gather_variant_count()) return fail(ctx, FCG_INVALID_ARGUMENT, "bad gather variant");
+    ctx->batch_variant[mvp_prec] = variant;
     return FCG_OK;
 }
 
@@ -1460,8 +1510,10 @@ int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants) 
     if (n_variants) *n_variants = fcg::gather_variant_count();
     if (variant < 0) return FCG_OK;  // query only
     if (variant >= fcg::gather_variant_count()) return fail(ctx, FCG_INVALID_ARGUMENT, "bad gather variant");
-    j->gather_variant = variant;
-    j->tuned_k = -1;  // pinned by the caller
+    for (int p = 0; p < 3; ++p) {
+        j->gather_variant[p] = variant;
+        j->tuned[p] = -1;  // pinned by the caller
+    }
     return FCG_OK;
 }
 
@@ -1470,7 +1522,6 @@ int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* b
     if (!j || k < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    const int prec = j->filled64 ? fcg::kMvp64 : fcg::kMvp32;
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)j->N * k;
     DevBuf dq, dv;
@@ -1479,29 +1530,39 @@ int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* b
     HIPCHECK(ctx, fcg::launch_random(dq.as<double>(), n, 777, s));
     Events5 ev;
     for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
-    int best = 0;
-    double best_t = 1e300;
-    for (int var = 0; var < fcg::gather_variant_count(); ++var) {
-        j->gather_variant = var;
-        RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s));
-        double t = 0.0;
-        for (int r = 0; r < reps; ++r) {
-            RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
-            HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
-            float ms = 0;
-            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
-            t += ms;
+    int first_best = -1;
+    double first_t = 0.0;
+    // every filled precision; the values reported are those of the first one tuned
+    for (int prec = 0; prec < 3; ++prec) {
+        if (prec == fcg::kMvp64 ? !j->filled64 : !j->filled32) continue;
+        int best = 0;
+        double best_t = 1e300;
+        for (int var = 0; var < fcg::gather_variant_count(); ++var) {
+            j->gather_variant[prec] = var;
+            RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s));
+            double t = 0.0;
+            for (int r = 0; r < reps; ++r) {
+                RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
+                HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
+                float ms = 0;
+                HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
+                t += ms;
+            }
+            t /= reps;
+            if (t < best_t) {
+                best_t = t;
+                best = var;
+            }
         }
-        t /= reps;
-        if (t < best_t) {
-            best_t = t;
-            best = var;
+        j->gather_variant[prec] = best;
+        j->tuned[prec] = 1;
+        if (first_best < 0) {
+            first_best = best;
+            first_t = best_t;
         }
     }
-    j->gather_variant = best;
-    j->tuned_k = k;
-    if (variant) *variant = best;
-    if (best_ms) *best_ms = best_t;
+    if (variant) *variant = first_best;
+    if (best_ms) *best_ms = first_t;
     return FCG_OK;
 }
 
@@ -1630,7 +1691,8 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     const bool mixed = opts->precision == FCG_PREC_MIXED;
     const int k = (int)j->rhs.size();
-    if (k > 0 && j->tuned_k == 0 && (mixed ? j->filled32 : j->filled64))
+    const int p = mixed ? fcg::kMvp32 : fcg::kMvp64;
+    if (k > 0 && j->tuned[p] == 0 && (mixed ? j->filled32 : j->filled64))
         RCCHECK(fcg_autotune(ctx, job, k, 3, nullptr, nullptr));
     return run_batch(ctx, &job, 1, opts, charges != nullptr, cap, rows_out, cols_out, stats, charges);
 }

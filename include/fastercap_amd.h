@@ -213,6 +213,14 @@ int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* b
 /* Pin the gather configuration of a job (variant < 0 only queries the count). */
 int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants);
 
+/* Time every gather configuration on one product over several filled jobs in
+ * precision mvp_prec (FCG_MVP_*) and keep the fastest for the context's products
+ * over several jobs (the solve engine's steps).  No reference counterpart. */
+int fcg_autotune_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, int* variant,
+                       double* best_ms);
+/* Pin the gather configuration of the context's products over several jobs. */
+int fcg_set_batch_variant(fcg_ctx* ctx, int mvp_prec, int variant);
+
 /* Solve several jobs together: every GMRES step of every job is advanced by one
  * batched launch per stage.  caps receives the per-job matrices back to back
  * (rows[i] x cols[i] each, in job order); stats may be NULL or hold njobs entries.
