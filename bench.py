@@ -209,6 +209,87 @@ def bench_batch(args, fc, dev, tmp):
     }
 
 
+def bench_serve(args, fc, dev, tmp):
+    """Serving stream: Poisson arrivals over a pool of resident jobs."""
+    import numpy as np
+
+    from fastercap_amd import serving, workloads
+
+    t0 = time.perf_counter()
+    pool = workloads.serve_pool(tmp, n_jobs=args.serve_jobs, n_max=args.serve_max_panels, threads=16)
+    meshes = [m for m, _, _ in pool]
+    kinds = [k for _, k, _ in pool]
+    log(f"built {len(meshes)} serving jobs in {time.perf_counter() - t0:.1f}s: panels "
+        f"{min(m.N for m in meshes)}..{max(m.N for m in meshes)}, links {sum(m.L for m in meshes)}")
+    jobs = [dev.upload(m) for m in meshes]
+    MIX, F64 = fc.FCG_PREC_MIXED, fc.FCG_PREC_FP64
+    dev.fill_batch(jobs, F64)
+    dev.autotune_batch(jobs, fc.FCG_MVP_FP64)
+    refs = {j: r.cap for j, r in zip(jobs, dev.solve_batch(jobs, meshes, tol=1e-8, precision=F64))}
+    dev.fill_batch(jobs, MIX)
+    dev.autotune_batch(jobs, fc.FCG_MVP_FP32)
+    dev.autotune_batch(jobs, fc.FCG_MVP_FP32_ACC64)
+    # closed-batch capacity: every job once, all together
+    dev.solve_batch(jobs, meshes, tol=args.tol, precision=MIX)
+    a = time.perf_counter()
+    dev.solve_batch(jobs, meshes, tol=args.tol, precision=MIX)
+    cap_rate = len(jobs) / (time.perf_counter() - a)
+    log(f"closed-batch capacity {cap_rate:.1f} solves/s")
+
+    def serve(rate, window, seed):
+        srv = serving.Server(dev, tol=args.tol, precision=MIX)
+        for j in jobs:
+            srv.add_job(j)
+        arr = serving.poisson_arrivals(rate, max(8, int(rate * window)), jobs, seed)
+        rep = srv.run(arr)
+        srv.close()
+        errs = [float(np.max(np.abs(r.result.cap - refs[r.job]) / np.abs(np.diag(refs[r.job]))[:, None]))
+                for r in rep.requests]
+        return rep, max(errs) if errs else float("nan")
+
+    serve(cap_rate, 2.0, 1)  # warm
+    dev.profile(enable=1, reset=True)
+    rate = args.serve_load * cap_rate
+    rep, err = serve(rate, args.serve_window, 7)
+    prof = dev.profile(enable=0, reset=True)
+    log(f"load {args.serve_load:.2f}x capacity ({rate:.1f}/s): {rep.completed} solves in {rep.elapsed_s:.1f}s = "
+        f"{rep.solves_per_s:.1f}/s, latency {rep.latency_ms}, max active {rep.max_active}, C err {err:.2e}")
+    half, err_h = serve(0.5 * cap_rate, args.serve_window / 2, 11)
+    log(f"half load: {half.solves_per_s:.1f}/s, latency {half.latency_ms}")
+    g = prof["fp32"]
+    achieved = g["gather_bytes"] / (g["gather_ms"] * 1e-3) / 1e9 if g["gather_ms"] > 0 else 0.0
+    return {
+        "metric": "served solves/s (Poisson arrivals over resident jobs, mixed precision, <=1e-4 rel C error)",
+        "value": rep.solves_per_s,
+        "unit": "solves/s",
+        "ms_per_step": 1e3 * rep.elapsed_s / max(1, rep.steps),
+        "dtype": "f32+f64",
+        "config": {
+            "workload": f"{len(jobs)} resident jobs from the plate / crossing-bus / random-box generators, half over "
+                        f"the eps 3.9 interface, panels log-uniform in [1000, {args.serve_max_panels}]",
+            "jobs": len(jobs),
+            "kinds": {k: kinds.count(k) for k in set(kinds)},
+            "panels_total": int(sum(m.N for m in meshes)),
+            "links_total": int(sum(m.L for m in meshes)),
+            "arrival_rate_per_s": rate,
+            "load_vs_closed_batch_capacity": args.serve_load,
+            "requests": rep.completed,
+            "gmres_tol": args.tol,
+        },
+        "latency_ms": rep.latency_ms,
+        "queue_ms": rep.queue_ms,
+        "half_load": {"rate_per_s": 0.5 * cap_rate, "solves_per_s": half.solves_per_s, "latency_ms": half.latency_ms},
+        "closed_batch_capacity_solves_per_s": cap_rate,
+        "max_active": rep.max_active,
+        "c_rel_err_vs_fp64_tol1e-8_max_vs_diagonal": max(err, err_h),
+        "roofline": {"kernel": "k_gather<float,float,float,*> (inner FP32 products while serving)", "bound": "hbm",
+                     "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
+                     "traffic": None, "launches": g["gather_launches"],
+                     "avg_launch_ms": g["gather_ms"] / max(1, g["gather_launches"])},
+        "cpu_baseline": None,
+    }
+
+
 def single_workload(name, d):
     from fastercap_amd import workloads
 
@@ -217,7 +298,8 @@ def single_workload(name, d):
     if name == "plates":
         return workloads.two_plates(d), 0.01, "two 1x1 m plates 0.1 m gap, -m0.01"
     if name == "boxes":
-        return workloads.random_boxes(d, n=32, seed=0), 0.02, "32 random boxes over eps 3.9 interface, -m0.02"
+        return (workloads.random_boxes(d, n=32, seed=0), 3.4e-6,
+                "32 random boxes (sides log-uniform [0.2,2] in a 20^3 cube) over an eps 3.9 interface, -m3.4e-6")
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -316,7 +398,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="batch", choices=["batch", "bus", "plates", "boxes"])
+    ap.add_argument("--workload", default="batch", choices=["batch", "serve", "bus", "plates", "boxes"])
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-max-panels", type=int, default=200_000)
+    ap.add_argument("--serve-load", type=float, default=1.5, help="arrival rate / closed-batch capacity")
+    ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="single-job workloads")
     ap.add_argument("--mesh-eps", type=float, default=None)
     ap.add_argument("--tol", type=float, default=None)
@@ -333,7 +419,7 @@ def main():
                                    "the design notes, not run", "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     if args.tol is None:
-        args.tol = 1e-5 if args.workload == "batch" else 1e-4
+        args.tol = 1e-5 if args.workload in ("batch", "serve") else 1e-4
 
     import fastercap_amd as fc
 
@@ -344,6 +430,8 @@ def main():
         if out is None:
             dev.close()
             return
+    elif args.workload == "serve":
+        out = bench_serve(args, fc, dev, tmp)
     else:
         out = bench_single(args, fc, dev, tmp)
     line = {

@@ -279,3 +279,56 @@ def sphere_in_shell(outdir: str, a: float = 1.0, b: float = 2.0, eps_shell: floa
         f.write(f"C {fc} {eps_shell!r}  0 0 0\n")
         f.write(f"D {fd} {eps_shell!r} 1.0  0 0 0  0 0 0\n")
     return main
+
+
+def _sized_mesh(make_path, target, m0, exponent=1.6, tries=3):
+    """Build a mesh whose panel count is near `target` by adjusting mesh_eps from a
+    first guess m0 with the power law N ~ m^-exponent (a few builds)."""
+    from . import Mesh
+
+    m = m0
+    mesh = Mesh.build(make_path(), mesh_eps=m)
+    for _ in range(tries):
+        ratio = mesh.N / float(target)
+        if 0.7 < ratio < 1.4:
+            break
+        m = m * ratio ** (1.0 / exponent)
+        mesh = Mesh.build(make_path(), mesh_eps=m)
+    return mesh, m
+
+
+def serve_pool(outdir: str, n_jobs: int = 32, seed: int = 0, n_min: int = 1000, n_max: int = 200_000,
+               threads: int = 16):
+    """Jobs for the serving stream, drawn from the plate, crossing-bus and
+    random-box generators: half in vacuum, half over the eps 3.9 interface, with
+    target panel counts log-uniform in [n_min, n_max].  Returns a list of
+    (mesh, kind, mesh_eps)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    rng = np.random.default_rng(BASE_SEED + 104729 * seed + 5)
+    specs = []
+    for i in range(n_jobs):
+        target = float(np.exp(rng.uniform(np.log(n_min), np.log(n_max))))
+        if i % 2 == 1:
+            kind = "boxes_interface"
+        else:
+            kind = ("plates", "bus", "boxes_vacuum")[int(rng.integers(0, 3))]
+        specs.append((i, kind, target, int(rng.integers(0, 1 << 30)), float(rng.uniform(0.05, 0.2)),
+                      int(rng.choice([8, 16, 32]))))
+
+    def build(spec):
+        i, kind, target, sd, gap, nbox = spec
+        d = os.path.join(outdir, f"serve_{i}")
+        if kind == "plates":
+            mk, m0 = (lambda: two_plates(d, gap=gap)), 0.02
+        elif kind == "bus":
+            mk, m0 = (lambda: crossing_bus(d)), 2000.0
+        elif kind == "boxes_vacuum":
+            mk, m0 = (lambda: random_boxes(d, n=nbox, seed=sd, with_interface=False)), 0.05
+        else:
+            mk, m0 = (lambda: random_boxes(d, n=nbox, seed=sd)), 1e-4
+        mesh, m = _sized_mesh(mk, target, m0)
+        return mesh, kind, m
+
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(build, specs))
