@@ -411,6 +411,8 @@ def main():
     ap.add_argument("--reps", type=int, default=10, help="timed repetitions of the product for the roofline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-mixed", action="store_true", help="batch: run only the timed mixed solves (profiling)")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="only copy this many MiB device-to-device (reference kernel for memory counters) and exit")
     ap.add_argument("--cpu-rows", type=int, default=1, help="single-job CPU sample: right-hand sides")
     ap.add_argument("--cpu-jobs", type=int, default=1280, help="batch CPU sample: jobs (evenly strided)")
     args = ap.parse_args()
@@ -419,7 +421,19 @@ def main():
                                    "the design notes, not run", "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     if args.tol is None:
-        args.tol = 1e-5 if args.workload in ("batch", "serve") else 1e-4
+        args.tol = 1e-5 if args.workload in ("batch", "serve", "boxes") else 1e-4
+
+    if args.calibrate_copy_mib > 0:
+        import torch
+
+        n = args.calibrate_copy_mib * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calibration_copy_bytes": 4 * n, "copies": 3}), flush=True)
+        return
 
     import fastercap_amd as fc
 

@@ -390,6 +390,7 @@ struct GatherCfg {
 // gather_shape fits them to the batch's row strides
 static const GatherCfg kGatherCfgs[] = {
     {2, 4, 4}, {1, 8, 4}, {4, 2, 4}, {1, 8, 2}, {2, 4, 2}, {8, 1, 4}, {4, 2, 2}, {1, 4, 4}, {16, 1, 2}, {2, 2, 4},
+    {4, 8, 2}, {8, 4, 2}, {16, 2, 2},  // wide shapes for many right-hand sides (k up to 32 per pass)
 };
 constexpr int kNumGatherCfgs = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 int gather_variant_count() { return kNumGatherCfgs; }
@@ -421,7 +422,8 @@ static hipError_t gather_dispatch(const BatchView& bv, int variant, hipStream_t 
     FCG_GU(1, 1) FCG_GU(2, 1) FCG_GU(4, 1) FCG_GU(8, 1) FCG_GU(16, 1)
     FCG_GU(1, 2) FCG_GU(2, 2) FCG_GU(4, 2) FCG_GU(8, 2)
     FCG_GU(1, 4) FCG_GU(2, 4) FCG_GU(4, 4)
-    FCG_GU(1, 8) FCG_GU(2, 8)
+    FCG_GU(1, 8) FCG_GU(2, 8) FCG_GU(4, 8)
+    FCG_GU(8, 4) FCG_GU(16, 2)
 #undef FCG_GU
 #undef FCG_G
     return hipErrorInvalidConfiguration;
