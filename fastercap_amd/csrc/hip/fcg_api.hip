@@ -1517,11 +1517,9 @@ int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants) 
     return FCG_OK;
 }
 
-int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* best_ms) {
-    Job* j = get_job(ctx, job);
-    if (!j || k < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
-    HIPCHECK(ctx, hipSetDevice(ctx->device));
+namespace {
+// measure every gather shape of job j in product precision prec and keep the fastest
+int autotune_prec(fcg_ctx* ctx, Job* j, int prec, int k, int reps, int* variant, double* best_ms) {
     hipStream_t s = ctx->stream;
     const size_t n = (size_t)j->N * k;
     DevBuf dq, dv;
@@ -1530,39 +1528,51 @@ int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* b
     HIPCHECK(ctx, fcg::launch_random(dq.as<double>(), n, 777, s));
     Events5 ev;
     for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
-    int first_best = -1;
-    double first_t = 0.0;
-    // every filled precision; the values reported are those of the first one tuned
-    for (int prec = 0; prec < 3; ++prec) {
-        if (prec == fcg::kMvp64 ? !j->filled64 : !j->filled32) continue;
-        int best = 0;
-        double best_t = 1e300;
-        for (int var = 0; var < fcg::gather_variant_count(); ++var) {
-            j->gather_variant[prec] = var;
-            RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s));
-            double t = 0.0;
-            for (int r = 0; r < reps; ++r) {
-                RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
-                HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
-                float ms = 0;
-                HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
-                t += ms;
-            }
-            t /= reps;
-            if (t < best_t) {
-                best_t = t;
-                best = var;
-            }
+    int best = 0;
+    double best_t = 1e300;
+    for (int var = 0; var < fcg::gather_variant_count(); ++var) {
+        j->gather_variant[prec] = var;
+        RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s));
+        double t = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s, ev.e));
+            HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
+            float ms = 0;
+            HIPCHECK(ctx, hipEventElapsedTime(&ms, ev.e[2], ev.e[3]));
+            t += ms;
         }
-        j->gather_variant[prec] = best;
-        j->tuned[prec] = 1;
-        if (first_best < 0) {
-            first_best = best;
-            first_t = best_t;
+        t /= reps;
+        if (t < best_t) {
+            best_t = t;
+            best = var;
         }
     }
-    if (variant) *variant = first_best;
-    if (best_ms) *best_ms = first_t;
+    j->gather_variant[prec] = best;
+    j->tuned[prec] = 1;
+    if (variant) *variant = best;
+    if (best_ms) *best_ms = best_t;
+    return FCG_OK;
+}
+}  // namespace
+
+int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* best_ms) {
+    Job* j = get_job(ctx, job);
+    if (!j || k < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    // every filled precision; the values reported are those of the first one tuned
+    bool first = true;
+    for (int prec = 0; prec < 3; ++prec) {
+        if (prec == fcg::kMvp64 ? !j->filled64 : !j->filled32) continue;
+        int v = 0;
+        double ms = 0.0;
+        RCCHECK(autotune_prec(ctx, j, prec, k, reps, &v, &ms));
+        if (first) {
+            if (variant) *variant = v;
+            if (best_ms) *best_ms = ms;
+            first = false;
+        }
+    }
     return FCG_OK;
 }
 
@@ -1691,9 +1701,14 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     const bool mixed = opts->precision == FCG_PREC_MIXED;
     const int k = (int)j->rhs.size();
+    // first solve of a job in this precision: pick its gather shapes (inner and,
+    // when mixed, refinement products)
     const int p = mixed ? fcg::kMvp32 : fcg::kMvp64;
-    if (k > 0 && j->tuned[p] == 0 && (mixed ? j->filled32 : j->filled64))
-        RCCHECK(fcg_autotune(ctx, job, k, 3, nullptr, nullptr));
+    if (k > 0 && j->tuned[p] == 0 && (mixed ? j->filled32 : j->filled64)) {
+        RCCHECK(autotune_prec(ctx, j, p, k, 3, nullptr, nullptr));
+        if (mixed && j->tuned[fcg::kMvp32Acc] == 0)
+            RCCHECK(autotune_prec(ctx, j, fcg::kMvp32Acc, k, 3, nullptr, nullptr));
+    }
     return run_batch(ctx, &job, 1, opts, charges != nullptr, cap, rows_out, cols_out, stats, charges);
 }
 
