@@ -445,15 +445,9 @@ class Engine:
 
 
 def run(argv):
-    """Manual-mode solve with the reference's switches (-m -d -mc -t -pj/-pn);
-    returns (mesh, SolveResult).  FasterCapConsole::ParseCmdLine
+    """Manual- or -a-mode solve with the reference's switches; prints the matrix as
+    FasterCap does and returns (mesh, SolveResult).  FasterCapConsole::ParseCmdLine
     (FasterCapConsole.cpp:136-357) + CSolveCap::Run (Solver/SolveCapacitance.cpp:86-256)."""
-    from .cli import parse_args
+    from . import cli
 
-    a = parse_args(argv)
-    mesh = Mesh.build(a.file, mesh_eps=a.mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff)
-    dev = Device(0)
-    job = dev.upload(mesh)
-    dev.fill(job, FCG_PREC_FP64)
-    res = dev.solve(job, mesh, tol=a.tol, precond=a.precond)
-    return mesh, res
+    return cli.run(cli.parse_args(argv))

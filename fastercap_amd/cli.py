@@ -1,0 +1,177 @@
+"""Python mirror of the `fastercap` command line (fastercap_amd/csrc/host/cli.cpp).
+
+Options and defaults follow FasterCapConsole::ParseCmdLine (FasterCapConsole.cpp:136-357)
+and CAutoRefGlobalVars::Reset (Solver/SolverGlobal.cpp:53-90); the -a loop follows
+CSolveCap::SolveMain (Solver/SolveCapacitance.cpp:294-477) and the matrix output
+CSolveCap::OutputCapMtx (:724-809).
+
+    python -m fastercap_amd.cli cubes.txt -a0.01
+"""
+from __future__ import annotations
+
+import math
+import sys
+import time
+from dataclasses import dataclass
+
+import numpy as np
+
+COMMAND_LINE_ERROR = 64
+MAX_AUTO_ITERATIONS = 128  # SOLVE_MAX_AUTO_ITERATIONS
+AUTO_INCREMENT = 1.1       # SOLVE_AUTO_INCREMENT_FACTOR
+
+
+class CommandLineError(ValueError):
+    pass
+
+
+@dataclass
+class Args:
+    file: str = ""
+    mesh_eps: float = 0.1
+    eps_ratio: float = 1.0
+    curv_coeff: float = 3.0
+    tol: float = 0.01
+    auto: bool = False
+    auto_err: float = 0.01
+    auto_precond: bool = False
+    precond: int = 1  # FCG_PRECOND_JACOBI
+    precision: int = 0  # FCG_PREC_FP64
+    verbose: bool = False
+    device: int = 0
+    stop: bool = False  # -b? / -bv
+
+
+def _num(s: str, opt: str) -> float:
+    try:
+        return float(s)
+    except ValueError:
+        raise CommandLineError(f"bad value in parameter '{opt}'") from None
+
+
+def parse_args(argv) -> Args:
+    a = Args()
+    for opt in argv:
+        if not opt.startswith("-"):
+            if a.file:
+                raise CommandLineError(f"more than one input file ('{a.file}', '{opt}')")
+            a.file = opt
+        elif opt in ("-b?", "-bv"):
+            a.stop = True
+        elif opt == "-b":
+            pass
+        elif opt == "-v":
+            a.verbose = True
+        elif opt == "-ap":
+            a.auto_precond = True
+        elif opt == "-pj":
+            a.precond = 1
+        elif opt == "-pn":
+            a.precond = 0
+        elif opt == "-gmixed":
+            a.precision = 1
+        elif opt.startswith("-gdev"):
+            a.device = int(_num(opt[5:], opt))
+        elif opt.startswith("-mc"):
+            a.curv_coeff = _num(opt[3:], opt)
+        elif opt.startswith("-m"):
+            a.mesh_eps = _num(opt[2:], opt)
+        elif opt.startswith("-d"):
+            a.eps_ratio = _num(opt[2:], opt)
+        elif opt.startswith("-t"):
+            a.tol = _num(opt[2:], opt)
+        elif opt.startswith("-a"):
+            a.auto = True
+            if len(opt) > 2:
+                a.auto_err = _num(opt[2:], opt)
+        else:
+            raise CommandLineError(f"unsupported parameter '{opt}' (this build covers the solve path only)")
+    if not a.file and not a.stop:
+        raise CommandLineError("no input file")
+    if a.curv_coeff < 1.0:
+        a.curv_coeff = 1.0
+    return a
+
+
+def format_cap(cap: np.ndarray, names, dim: int, verbose: bool = False) -> str:
+    """The reference's matrix printout: 2D values in F/m with the balance
+    conductor's column dropped unless verbose."""
+    from . import TWO_PI_TIMES_E0
+
+    M, N = cap.shape
+    if dim == 2 and not verbose and N > 1:
+        N -= 1
+    scale = TWO_PI_TIMES_E0 if dim == 2 else 1.0
+    lines = ["Capacitance matrix is:", f"Dimension {M} x {N}"]
+    for i in range(M):
+        lines.append(f"{names[i]}  " + "".join(f"{cap[i, j] * scale:g} " for j in range(N)))
+    return "\n".join(lines) + "\n"
+
+
+def solve_file(dev, a: Args, mesh_eps: float, tol: float):
+    from . import Mesh
+
+    mesh = Mesh.build(a.file, mesh_eps=mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff)
+    job = dev.upload(mesh)
+    try:
+        dev.fill(job, a.precision)
+        res = dev.solve(job, mesh, tol=tol, precond=a.precond, precision=a.precision)
+    finally:
+        dev.release(job)
+    return mesh, res
+
+
+def run(a: Args, out=sys.stdout):
+    """Manual or -a run; returns (mesh, SolveResult) of the final iteration."""
+    from . import Device
+
+    dev = Device(a.device)
+    try:
+        if not a.auto:
+            mesh, res = solve_file(dev, a, a.mesh_eps, a.tol)
+            out.write(format_cap(res.cap, mesh.conductor_names, mesh.dim, a.verbose))
+            return mesh, res
+        tol = a.auto_err / 2.0
+        out.write(f"Auto calculation with max error: {a.auto_err:g}\n\nIteration number #0\n")
+        mesh, res = solve_file(dev, a, 1e32, tol)
+        out.write(format_cap(res.cap, mesh.conductor_names, mesh.dim, a.verbose))
+        old_p, old_l = mesh.N, mesh.L
+        for i in range(1, MAX_AUTO_ITERATIONS):
+            out.write(f"\nIteration number #{i}\n")
+            m = mesh.stats.max_mesh_eps
+            for _ in range(MAX_AUTO_ITERATIONS):
+                m = m / math.sqrt(2.0)
+                nmesh, nres = solve_file(dev, a, m, tol)
+                if nmesh.L > old_l * AUTO_INCREMENT and nmesh.N > old_p * AUTO_INCREMENT:
+                    break
+                m = nmesh.stats.max_mesh_eps
+            diff = np.linalg.norm(nres.cap - res.cap) / np.linalg.norm(nres.cap)
+            out.write(format_cap(nres.cap, nmesh.conductor_names, nmesh.dim, a.verbose))
+            out.write(f"Weighted Frobenius norm of the difference between capacitance (auto option): {diff:g}\n")
+            mesh, res = nmesh, nres
+            old_p, old_l = mesh.N, mesh.L
+            if diff <= a.auto_err:
+                break
+        return mesh, res
+    finally:
+        dev.close()
+
+
+def main(argv=None) -> int:
+    argv = sys.argv[1:] if argv is None else argv
+    try:
+        a = parse_args(argv)
+    except CommandLineError as e:
+        print(f"Error: {e}", file=sys.stderr)
+        return COMMAND_LINE_ERROR
+    if a.stop:
+        print("fastercap (gfx950 solve path), FasterCap 6.0.7 compatible input and output")
+        return 0
+    t0 = time.perf_counter()
+    run(a)
+    print(f"Total time: {time.perf_counter() - t0:g} s")
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

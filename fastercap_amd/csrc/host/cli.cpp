@@ -1,0 +1,287 @@
+// fastercap: command-line drop-in of FasterCap's console solve on the gfx950 path.
+//
+// Options and defaults follow FasterCapConsole::ParseCmdLine
+// (FasterCapConsole.cpp:136-357) and CAutoRefGlobalVars::Reset
+// (Solver/SolverGlobal.cpp:53-90); the run follows CSolveCap::Run / SolveMain
+// (Solver/SolveCapacitance.cpp:86-578), including the -a loop, and the matrix is
+// printed as CSolveCap::OutputCapMtx does (Solver/SolveCapacitance.cpp:724-809).
+// The mesh and links come from the host builder (include/fastercap_host.h), the
+// fill and the solve from the device library (include/fastercap_amd.h).
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fastercap_amd.h"
+#include "fastercap_host.h"
+
+namespace {
+
+constexpr double kTwoPiE0 = 5.563250279878e-11;  // Solver/Potential.h:42
+constexpr int kMaxAutoIterations = 128;          // SOLVE_MAX_AUTO_ITERATIONS
+constexpr double kAutoIncrement = 1.1;           // SOLVE_AUTO_INCREMENT_FACTOR
+
+struct Options {
+    std::string file;
+    double mesh_eps = 0.1, eps_ratio = 1.0, curv = 3.0, tol = 0.01, auto_err = 0.01;
+    bool autom = false, verbose = false, auto_precond = false;
+    int precond = FCG_PRECOND_JACOBI;
+    int precision = FCG_PREC_FP64;
+    int device = 0;
+};
+
+void usage() {
+    std::printf(
+        "Usage: fastercap <input file> [options]\n"
+        "  -a<err>  Automatically refine, stop when the relative C-matrix change is below err (default 0.01)\n"
+        "  -ap      Automatic preconditioner choice (only Jacobi is available here)\n"
+        "  -m<eps>  Mesh relative refinement value (default 0.1)\n"
+        "  -mc<c>   Mesh curvature coefficient (default 3)\n"
+        "  -t<tol>  GMRES iteration tolerance (default 0.01)\n"
+        "  -d<r>    Direct potential interaction coefficient to mesh refinement ratio (default 1)\n"
+        "  -pj      Jacobi preconditioner (default)\n"
+        "  -pn      No preconditioner\n"
+        "  -v       Verbose output\n"
+        "  -b       Console mode (always)\n"
+        "  -b?      This text\n"
+        "  -bv      Print only the version\n"
+        "  -gmixed  Mixed precision (FP32 far field and products, FP64 refinement)\n"
+        "  -gdev<n> HIP device (default 0)\n");
+}
+
+// returns 0 or FC_COMMAND_LINE_ERROR
+int parse(int argc, char** argv, Options& o, bool& stop) {
+    stop = false;
+    for (int i = 1; i < argc; ++i) {
+        const char* a = argv[i];
+        if (a[0] != '-') {
+            if (!o.file.empty()) {
+                std::fprintf(stderr, "Error: more than one input file ('%s', '%s')\n", o.file.c_str(), a);
+                return FCG_COMMAND_LINE_ERROR;
+            }
+            o.file = a;
+            continue;
+        }
+        auto num = [&](const char* s, double* v) {
+            char* end = nullptr;
+            *v = std::strtod(s, &end);
+            return end != s && *end == '\0';
+        };
+        bool ok = true;
+        if (!std::strcmp(a, "-b?")) {
+            usage();
+            stop = true;
+            return 0;
+        } else if (!std::strcmp(a, "-bv")) {
+            std::printf("fastercap (gfx950 solve path), FasterCap 6.0.7 compatible input and output\n");
+            stop = true;
+            return 0;
+        } else if (!std::strcmp(a, "-b")) {
+        } else if (!std::strcmp(a, "-v")) {
+            o.verbose = true;
+        } else if (!std::strcmp(a, "-ap")) {
+            o.auto_precond = true;
+        } else if (!std::strcmp(a, "-pj")) {
+            o.precond = FCG_PRECOND_JACOBI;
+        } else if (!std::strcmp(a, "-pn")) {
+            o.precond = FCG_PRECOND_NONE;
+        } else if (!std::strcmp(a, "-gmixed")) {
+            o.precision = FCG_PREC_MIXED;
+        } else if (!std::strncmp(a, "-gdev", 5)) {
+            double v;
+            ok = num(a + 5, &v);
+            o.device = (int)v;
+        } else if (!std::strncmp(a, "-mc", 3)) {
+            ok = num(a + 3, &o.curv);
+        } else if (a[1] == 'm') {
+            ok = num(a + 2, &o.mesh_eps);
+        } else if (a[1] == 'd') {
+            ok = num(a + 2, &o.eps_ratio);
+        } else if (a[1] == 't') {
+            ok = num(a + 2, &o.tol);
+        } else if (a[1] == 'a') {
+            o.autom = true;
+            ok = a[2] == '\0' || num(a + 2, &o.auto_err);
+        } else {
+            std::fprintf(stderr, "Error: unsupported parameter '%s' (this build covers the solve path only)\n", a);
+            return FCG_COMMAND_LINE_ERROR;
+        }
+        if (!ok) {
+            std::fprintf(stderr, "Error: bad value in parameter '%s'\n", a);
+            return FCG_COMMAND_LINE_ERROR;
+        }
+    }
+    if (o.file.empty() && !stop) {
+        std::fprintf(stderr, "Error: no input file\n");
+        usage();
+        return FCG_COMMAND_LINE_ERROR;
+    }
+    return 0;
+}
+
+struct Result {
+    std::vector<double> cap;
+    int rows = 0, cols = 0, dim = 3;
+    std::vector<std::string> names;
+    fch_stats hs{};
+    fcg_solve_stats ss{};
+};
+
+// one refinement + fill + solve at the given mesh eps
+int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Result& r) {
+    fch_options ho;
+    fch_default_options(&ho);
+    ho.mesh_eps = mesh_eps;
+    ho.eps_ratio = o.eps_ratio;
+    ho.curv_coeff = o.curv;
+    ho.verbose = o.verbose ? 1 : 0;
+    fch_mesh* mesh = nullptr;
+    int rc = fch_build(o.file.c_str(), &ho, &mesh);
+    if (rc != 0) {
+        std::fprintf(stderr, "Error: %s\n", fch_last_error());
+        return rc;
+    }
+    fcg_job_desc d;
+    fch_get_desc(mesh, &d);
+    fch_get_stats(mesh, &r.hs);
+    r.dim = d.dim;
+    r.names.clear();
+    for (int g = 0; g < d.n_groups; ++g)
+        if (!d.group_is_diel[g]) r.names.push_back(fch_group_name(mesh, g));
+    int job = -1;
+    rc = fcg_job_upload(ctx, &d, &job);
+    if (rc == FCG_OK) rc = fcg_fill(ctx, job, o.precision);
+    if (rc == FCG_OK) {
+        fcg_solve_opts so{tol, 1000, o.precond, o.precision, 8};
+        const int nc = (int)r.names.size();
+        r.cap.assign((size_t)nc * nc, 0.0);
+        rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, nullptr, &r.ss);
+    }
+    if (rc != FCG_OK) std::fprintf(stderr, "Error: %s\n", fcg_last_error(ctx));
+    if (job >= 0) fcg_job_release(ctx, job);
+    fch_free(mesh);
+    return rc;
+}
+
+void print_cap(const Options& o, const Result& r) {
+    std::printf("Capacitance matrix is:\n");
+    int M = r.rows, N = r.cols;
+    if (r.dim == 2 && !o.verbose) {
+        if (N > 1)
+            --N;  // the balance conductor's column
+        else
+            std::fprintf(stderr, "Error: only one input conductor defined for a 2D problem. Result is meaningless.");
+    }
+    std::printf("Dimension %d x %d\n", M, N);
+    const double scale = r.dim == 2 ? kTwoPiE0 : 1.0;
+    for (int i = 0; i < M; ++i) {
+        std::printf("%s  ", i < (int)r.names.size() ? r.names[(size_t)i].c_str() : "?");
+        for (int j = 0; j < N; ++j) std::printf("%g ", r.cap[(size_t)i * r.cols + j] * scale);
+        std::printf("\n");
+    }
+    for (int i = 0; i < M; ++i)
+        for (int j = 0; j < N; ++j)
+            if (i != j && r.cap[(size_t)i * r.cols + j] >= 0.0)
+                std::fprintf(stderr, "Warning: capacitance matrix has a non-negative off-diagonal element at row %d col %d\n",
+                             i + 1, j + 1);
+    for (int i = 0; i < M; ++i) {
+        double sum = 0.0;
+        for (int j = 0; j < N; ++j)
+            if (i != j) sum += std::fabs(r.cap[(size_t)i * r.cols + j]);
+        if (std::fabs(r.cap[(size_t)i * r.cols + i]) < sum)
+            std::fprintf(stderr, "Warning: capacitance matrix is not diagonally dominant due to row %d\n", i + 1);
+    }
+    std::printf("\n");
+}
+
+void print_stats(const Result& r) {
+    std::printf("Number of input panels: %lld of which %lld conductors and %lld dielectric\n",
+                (long long)r.hs.input_panels, (long long)r.hs.input_raw_cond_panels,
+                (long long)r.hs.input_raw_diel_panels);
+    std::printf("Number of panels after refinement: %lld\n", (long long)r.hs.panels);
+    std::printf("Number of links: %lld\n", (long long)(r.hs.links + r.hs.panels));
+    std::printf("Max Mesh relative refinement value: %g\n", r.hs.max_mesh_eps);
+    std::printf("Max GMRES iterations: %d\n", r.ss.max_iterations);
+}
+
+double frob(const std::vector<double>& a) {
+    double s = 0.0;
+    for (double v : a) s += v * v;
+    return std::sqrt(s);
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    Options o;
+    bool stop = false;
+    int rc = parse(argc, argv, o, stop);
+    if (rc != 0 || stop) return rc;
+    if (o.curv < 1.0) {
+        std::fprintf(stderr, "Warning: curvature coefficient %f is below the minimum value of 1.0, setting to 1.0\n",
+                     o.curv);
+        o.curv = 1.0;
+    }
+    if (o.auto_precond)
+        std::fprintf(stderr, "Warning: automatic preconditioner requested; only Jacobi is available, using Jacobi\n");
+    std::printf("Running fastercap (gfx950 solve path)\nInput file: %s\n", o.file.c_str());
+    fcg_ctx* ctx = nullptr;
+    if (fcg_ctx_create(o.device, &ctx) != FCG_OK) {
+        std::fprintf(stderr, "Error: cannot open HIP device %d\n", o.device);
+        return FCG_DEVICE_ERROR;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    Result r;
+    if (!o.autom) {
+        rc = solve_once(ctx, o, o.mesh_eps, o.tol, r);
+        if (rc == FCG_OK) {
+            print_stats(r);
+            print_cap(o, r);
+        }
+    } else {
+        // Solver/SolveCapacitance.cpp:294-477
+        std::printf("Auto calculation with max error: %g\n", o.auto_err);
+        const double tol = o.auto_err / 2.0;
+        std::printf("\nIteration number #0 ***************************\n\n");
+        rc = solve_once(ctx, o, 1e32, tol, r);
+        if (rc == FCG_OK) {
+            print_stats(r);
+            print_cap(o, r);
+        }
+        // CAutoRefine::GetLinksNum: mutual links only
+        long long old_panels = r.hs.panels, old_links = r.hs.links;
+        for (int i = 1; rc == FCG_OK && i < kMaxAutoIterations; ++i) {
+            std::printf("\nIteration number #%d ***************************\n\n", i);
+            Result nr;
+            double m = r.hs.max_mesh_eps;
+            for (int j = 0; j < kMaxAutoIterations; ++j) {
+                m = m / std::sqrt(2.0);
+                rc = solve_once(ctx, o, m, tol, nr);
+                if (rc != FCG_OK) break;
+                const long long np = nr.hs.panels, nl = nr.hs.links;
+                if (nl > old_links * kAutoIncrement && np > old_panels * kAutoIncrement) break;
+                std::printf("Delta in refined panel and link count w.r.t. previous iteration less than %d%%\n",
+                            (int)((kAutoIncrement - 1.0) * 100.0));
+                m = nr.hs.max_mesh_eps;
+            }
+            if (rc != FCG_OK) break;
+            std::vector<double> diff(nr.cap.size());
+            for (size_t q = 0; q < diff.size(); ++q) diff[q] = nr.cap[q] - r.cap[q];
+            const double rel = frob(diff) / frob(nr.cap);
+            print_stats(nr);
+            print_cap(o, nr);
+            std::printf("Weighted Frobenius norm of the difference between capacitance (auto option): %g\n", rel);
+            old_panels = nr.hs.panels;
+            old_links = nr.hs.links;
+            r = nr;
+            if (rel <= o.auto_err) break;
+        }
+    }
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::printf("Total time: %g s\n", secs);
+    fcg_ctx_destroy(ctx);
+    return rc;
+}

@@ -1,0 +1,36 @@
+"""Command-line parsing and matrix printout of the drop-in CLI (no GPU)."""
+import numpy as np
+import pytest
+
+from fastercap_amd import cli
+
+
+def test_defaults_match_reference():
+    a = cli.parse_args(["in.lst"])
+    # CAutoRefGlobalVars::Reset (Solver/SolverGlobal.cpp:53-90)
+    assert (a.mesh_eps, a.eps_ratio, a.tol, a.auto, a.auto_err, a.precond) == (0.1, 1.0, 0.01, False, 0.01, 1)
+    assert a.curv_coeff == 3.0
+
+
+def test_switches():
+    a = cli.parse_args(["-a0.005", "-m0.02", "-mc2", "-d0.5", "-t1e-4", "-pn", "-v", "-b", "x.txt", "-gmixed"])
+    assert a.auto and a.auto_err == 0.005 and a.mesh_eps == 0.02 and a.curv_coeff == 2.0
+    assert a.eps_ratio == 0.5 and a.tol == 1e-4 and a.precond == 0 and a.verbose and a.precision == 1
+    assert cli.parse_args(["x", "-mc0.5"]).curv_coeff == 1.0  # clamped as the reference does
+
+
+@pytest.mark.parametrize("bad", [["x", "-g"], ["x", "-pb32"], ["x", "-mabc"], [], ["a", "b"]])
+def test_errors(bad):
+    with pytest.raises(cli.CommandLineError):
+        cli.parse_args(bad)
+    assert cli.main(bad) == cli.COMMAND_LINE_ERROR
+
+
+def test_matrix_printout():
+    cap = np.array([[1.5e-10, -2.5e-11, -1e-11], [-2.5e-11, 1.4e-10, -1.1e-11]])
+    txt = cli.format_cap(cap[:, :2], ["g1_A", "g2_B"], 3)
+    assert txt.splitlines()[:3] == ["Capacitance matrix is:", "Dimension 2 x 2", "g1_A  1.5e-10 -2.5e-11 "]
+    # 2D: scaled by 2 pi e0, balance conductor column dropped
+    txt = cli.format_cap(np.array([[1.0, -1.0]]), ["inner", "outer"], 2)
+    assert txt.splitlines()[1] == "Dimension 1 x 1"
+    assert txt.splitlines()[2] == "inner  5.56325e-11 "

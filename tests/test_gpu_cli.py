@@ -1,0 +1,66 @@
+"""The command-line drop-ins on the GPU: the reference help's documented
+`FasterCap cubes.txt -a0.01` run (hlp/Output/Output_ch01.htm) through the
+`fastercap` binary and the Python mirror, and a 2D run through the binary."""
+import io
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from fastercap_amd import cli, workloads
+from test_golden_cpu import two_cubes
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+BIN = os.path.join(os.path.dirname(HERE), "fastercap_amd", "bin", "fastercap")
+
+
+def _last_matrix(text):
+    lines = text.splitlines()
+    start = max(i for i, l in enumerate(lines) if l.startswith("Capacitance matrix is:"))
+    m, n = [int(x) for x in lines[start + 1].split()[1::2]]
+    rows = [lines[start + 2 + i].split() for i in range(m)]
+    return [r[0] for r in rows], np.array([[float(v) for v in r[1:1 + n]] for r in rows])
+
+
+def test_binary_auto_two_cubes(geo_dir):
+    d = os.path.join(geo_dir, "cli_cubes")
+    os.makedirs(d, exist_ok=True)
+    path = two_cubes(d)
+    gold = json.load(open(os.path.join(HERE, "golden", "two_cubes_doc.json")))
+    print(f"running {BIN} {path} -a0.01", flush=True)
+    out = subprocess.run([BIN, path, "-a0.01"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    names, cap = _last_matrix(out.stdout)
+    assert names == ["g1_cube.txt", "g2_cube.txt"] or len(names) == 2
+    assert out.stdout.count("Iteration number #") == len(gold["iterations"])
+    np.testing.assert_allclose(cap, np.array(gold["iterations"][-1]["cap"]), rtol=1e-5)
+
+
+def test_python_cli_matches_binary(geo_dir):
+    d = os.path.join(geo_dir, "cli_cubes2")
+    os.makedirs(d, exist_ok=True)
+    path = two_cubes(d)
+    buf = io.StringIO()
+    mesh, res = cli.run(cli.parse_args([path, "-m0.05", "-t1e-6"]), out=buf)
+    out = subprocess.run([BIN, path, "-m0.05", "-t1e-6"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    _, cap_bin = _last_matrix(out.stdout)
+    _, cap_py = _last_matrix(buf.getvalue())
+    np.testing.assert_allclose(cap_py, cap_bin, rtol=1e-5)
+
+
+def test_binary_2d_coated_coax(geo_dir):
+    path = workloads.coated_coax_2d(os.path.join(geo_dir, "cli_coax"))
+    out = subprocess.run([BIN, path, "-m0.1", "-t1e-8"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    names, cap = _last_matrix(out.stdout)
+    assert cap.shape == (1, 1)  # balance conductor's column dropped, as the reference prints 2D results
+    assert abs(cap[0, 0] / 113.4e-12 - 1.0) < 0.025
+
+
+def test_binary_rejects_unsupported_switch(geo_dir):
+    out = subprocess.run([BIN, "x.txt", "-g"], capture_output=True, text=True, timeout=60)
+    assert out.returncode == 64
