@@ -353,7 +353,8 @@ class Device:
         return out
 
     def autotune_batch(self, jobs, mvp_prec: int = FCG_MVP_FP64, reps: int = 3):
-        """Pick the gather shape of products over several jobs (engine steps)."""
+        """Pick the gather shape of each rhs class of products over several jobs
+        (engine steps); returns (shape of the largest class, gather ms)."""
         jobs = list(jobs)
         ja = (C.c_int * len(jobs))(*jobs)
         v, ms = C.c_int(), C.c_double()
@@ -362,6 +363,7 @@ class Device:
         return v.value, ms.value
 
     def set_batch_variant(self, mvp_prec: int, variant: int):
+        """Pin one gather shape for every rhs class (variant < 0: class defaults)."""
         self._check(self.lib.fcg_set_batch_variant(self._h, mvp_prec, variant), "fcg_set_batch_variant")
 
     def set_gather_variant(self, job: int, variant: int) -> int:

@@ -286,11 +286,18 @@ struct fcg_ctx {
     hipStream_t stream = nullptr;
     bool profiling = false;
     fcg_profile prof{};
-    int batch_variant[3] = {0, 0, 0};  // gather lane shape of products over several jobs
+    // gather lane shape of products over several jobs, per product precision and
+    // rhs class (-1: the class's default shape)
+    int batch_variant[3][fcg::kGatherClasses];
     Pool pool;  // declared before everything that returns blocks to it
     std::vector<std::unique_ptr<Job>> jobs;
     std::string err;
     Tables tabs;  // tables of stand-alone calls (reset per call)
+
+    fcg_ctx() {
+        for (auto& row : batch_variant)
+            for (int& v : row) v = -1;
+    }
 };
 
 namespace {
@@ -361,32 +368,54 @@ struct BatchEntry {
     const double* q;
     double* v;
 };
-int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, BatchView* out) {
+// Jobs are listed by row stride (stable), so every rhs class owns one contiguous
+// range of batch rows; `plan` (optional) receives those ranges with no lane shape
+// chosen yet (see choose_variants).
+int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, BatchView* out,
+               fcg::GatherPlan* plan = nullptr) {
     std::memset(out, 0, sizeof *out);
+    if (plan) plan->n = 0;
     const int n = (int)ents.size();
     if (n == 0) return FCG_OK;
+    std::vector<int> order((size_t)n);
+    for (int i = 0; i < n; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int a, int b) { return ents[(size_t)a].sc->kp < ents[(size_t)b].sc->kp; });
     std::vector<BJob> recs((size_t)n);
-    std::vector<int64_t> off((size_t)6 * (n + 1), 0);
+    std::vector<int64_t> off((size_t)5 * (n + 1), 0);
     int64_t* lo = off.data();
     int64_t* to = lo + (n + 1);
     int64_t* no = to + (n + 1);
-    int64_t* ro = no + (n + 1);
-    int64_t* fo = ro + (n + 1);
+    int64_t* fo = no + (n + 1);
     int64_t* ko = fo + (n + 1);
     int kmax = 0, any2d = 0, kpmin = 1 << 30;
     for (int i = 0; i < n; ++i) {
-        const Job& J = *ents[(size_t)i].j;
-        const Scratch& sc = *ents[(size_t)i].sc;
-        recs[(size_t)i] = J.rec(sc, ents[(size_t)i].q, ents[(size_t)i].v);
+        const BatchEntry& e = ents[(size_t)order[(size_t)i]];
+        const Job& J = *e.j;
+        const Scratch& sc = *e.sc;
+        recs[(size_t)i] = J.rec(sc, e.q, e.v);
         lo[i + 1] = lo[i] + J.L;
         to[i + 1] = to[i] + (int64_t)sc.k * sc.nblk;
         no[i + 1] = no[i] + J.M;
-        ro[i + 1] = ro[i] + (J.M + 3) / 4;
         fo[i + 1] = fo[i] + J.N;
         ko[i + 1] = ko[i] + sc.k;
         kmax = std::max(kmax, sc.k);
         kpmin = std::min(kpmin, std::max(sc.kp, 1));
         any2d |= (J.dim == 2);
+        if (plan) {
+            const int cls = fcg::gather_class(std::max(sc.kp, 1));
+            if (plan->n == 0 || plan->seg[plan->n - 1].cls != cls) {
+                fcg::GatherSeg& g = plan->seg[plan->n++];
+                g.row_lo = no[i];
+                g.kpmin = std::max(sc.kp, 1);
+                g.kmax = 0;
+                g.cls = cls;
+                g.variant = -1;
+            }
+            fcg::GatherSeg& g = plan->seg[plan->n - 1];
+            g.row_hi = no[i + 1];
+            g.kmax = std::max(g.kmax, sc.k);
+        }
     }
     void *dr, *doff;
     HIPCHECK(ctx, tabs.push(recs.data(), sizeof(BJob) * n, ctx->stream, &dr));
@@ -397,19 +426,31 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     out->link_off = d;
     out->tile_off = d + (n + 1);
     out->node_off = d + 2 * (n + 1);
-    out->rowb_off = d + 3 * (n + 1);
-    out->leaf_off = d + 4 * (n + 1);
-    out->rhs_off = d + 5 * (n + 1);
+    out->leaf_off = d + 3 * (n + 1);
+    out->rhs_off = d + 4 * (n + 1);
     out->links = lo[n];
     out->tiles = to[n];
     out->nodes = no[n];
-    out->rowb = ro[n];
     out->leaves = fo[n];
     out->rhs = ko[n];
     out->kmax = kmax;
     out->kpmin = kpmin;
     out->any2d = any2d;
     return FCG_OK;
+}
+
+// Lane shape of every class range of a product plan: a batch of one job uses the
+// job's own shape, a batch of several the context's shape for each class.
+void choose_variants(const fcg_ctx* ctx, int prec, const std::vector<BatchEntry>& ents, fcg::GatherPlan& plan) {
+    for (int i = 0; i < plan.n; ++i) {
+        fcg::GatherSeg& g = plan.seg[i];
+        if (ents.size() == 1) {
+            g.variant = ents[0].j->gather_variant[prec];
+        } else {
+            const int v = ctx->batch_variant[prec][g.cls];
+            g.variant = v >= 0 ? v : fcg::default_gather_variant(prec, g.cls);
+        }
+    }
 }
 
 // Algorithmic bytes of one gather launch over a batch (see fcg_profile).
@@ -566,8 +607,11 @@ int run_mvp(fcg_ctx* ctx, Job& j, int prec, const double* dq, double* dv, int k,
     HIPCHECK(ctx, hipStreamSynchronize(s));  // the table arena is reused below
     ctx->tabs.reset();
     BatchView bv;
-    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, dq, dv}}, &bv));
-    HIPCHECK(ctx, fcg::launch_mvp(bv, prec, jacobi, j.gather_variant[prec], s, ev));
+    fcg::GatherPlan gp;
+    const std::vector<BatchEntry> ents{BatchEntry{&j, &j.scr, dq, dv}};
+    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv, &gp));
+    choose_variants(ctx, prec, ents, gp);
+    HIPCHECK(ctx, fcg::launch_mvp(bv, prec, jacobi, gp, s, ev));
     return FCG_OK;
 }
 
@@ -979,12 +1023,13 @@ int engine_step(fcg_engine* E) {
             S->n_mvp++;
         }
         BatchView bv;
-        RCCHECK(make_batch(ctx, E->tabs, be, &bv));
+        fcg::GatherPlan gp;
+        RCCHECK(make_batch(ctx, E->tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32Acc : fcg::kMvp64;
-        const int var = res.size() == 1 ? res[0]->J->gather_variant[prec] : ctx->batch_variant[prec];
+        choose_variants(ctx, prec, be, gp);
         HIPCHECK(ctx, E->pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
         hipEvent_t* ev = E->pe_res.events();
-        q.push_back([=] { return fcg::launch_mvp(bv, prec, 0, var, s, ev); });
+        q.push_back([=] { return fcg::launch_mvp(bv, prec, 0, gp, s, ev); });
         KrylovView kv;
         RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
         RCCHECK(ensure_pbuf(ctx, E->partial_a, sizeof(double) * (size_t)kv.nchunks));
@@ -1005,13 +1050,14 @@ int engine_step(fcg_engine* E) {
             S->n_mvp++;
         }
         BatchView bv;
-        RCCHECK(make_batch(ctx, E->tabs, be, &bv));
+        fcg::GatherPlan gp;
+        RCCHECK(make_batch(ctx, E->tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32 : fcg::kMvp64;
-        const int var = arn.size() == 1 ? arn[0]->J->gather_variant[prec] : ctx->batch_variant[prec];
+        choose_variants(ctx, prec, be, gp);
         HIPCHECK(ctx, E->pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
         hipEvent_t* ev = E->pe_arn.events();
         const int jac = arn[0]->jac ? 1 : 0;
-        q.push_back([=] { return fcg::launch_mvp(bv, prec, jac, var, s, ev); });
+        q.push_back([=] { return fcg::launch_mvp(bv, prec, jac, gp, s, ev); });
         KrylovView kv;
         RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
         RCCHECK(ensure_pbuf(ctx, E->partial_b, sizeof(double) * (size_t)kv.nchunks * std::max(kv.maxnv, 1)));
@@ -1199,6 +1245,10 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         if (!(j->dim == 2 && hi == j->N)) j->rhs.emplace_back(lo, hi);
     }
     if (ranges.empty()) return fail(ctx, FCG_INVALID_ARGUMENT, "job has no conductor");
+    {
+        const int cls = fcg::gather_class(fcg::padded_k(std::max(1, (int)j->rhs.size())));
+        for (int p = 0; p < 3; ++p) j->gather_variant[p] = fcg::default_gather_variant(p, cls);
+    }
     HIPCHECK(ctx, upload(j->col_range, ranges.data(), ranges.size(), s));
     {
         std::vector<int32_t> rr;
@@ -1388,10 +1438,18 @@ int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* o
 }
 
 namespace {
+// One timed product of time_batch: seg >= 0 gives class range `seg` the shape
+// `var` (the other ranges keep the context's), seg == kAllRanges gives every range
+// the shape `var`, seg == kAsPlanned keeps the context's shapes.
+struct Trial {
+    int seg, var;
+};
+constexpr int kAsPlanned = -1, kAllRanges = -2;
+
 // Time batched products over `jobs` (each on as many random right-hand sides as it
-// has conductor solves) for every gather shape in `vars`.
-int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, const std::vector<int>& vars,
-               std::vector<fcg_mvp_timing>& outs) {
+// has conductor solves) for every trial; `plan_out` receives the class ranges.
+int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, const std::vector<Trial>& trials,
+               std::vector<fcg_mvp_timing>& outs, fcg::GatherPlan* plan_out = nullptr) {
     if (!ctx || !jobs || njobs < 1 || reps < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
         return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
@@ -1424,15 +1482,25 @@ int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps,
     for (size_t i = 0; i < js.size(); ++i)
         ents.push_back(BatchEntry{js[i], &js[i]->scr, qs[i]->as<double>(), vs[i]->as<double>()});
     BatchView bv;
-    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv));
+    fcg::GatherPlan base;
+    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv, &base));
+    choose_variants(ctx, mvp_prec, ents, base);
+    if (plan_out) *plan_out = base;
     Events5 ev;
     for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
     outs.clear();
-    for (int var : vars) {
-        HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s));
+    for (const Trial& t : trials) {
+        fcg::GatherPlan gp = base;
+        if (t.seg == kAllRanges) {
+            for (int i = 0; i < gp.n; ++i) gp.seg[i].variant = t.var;
+        } else if (t.seg >= 0) {
+            if (t.seg >= gp.n) return fail(ctx, FCG_INVALID_ARGUMENT, "bad class range");
+            gp.seg[t.seg].variant = t.var;
+        }
+        HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, gp, s));
         double acc[4] = {0, 0, 0, 0}, tot = 0;
         for (int r = 0; r < reps; ++r) {
-            HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, var, s, ev.e));
+            HIPCHECK(ctx, fcg::launch_mvp(bv, mvp_prec, 0, gp, s, ev.e));
             HIPCHECK(ctx, hipEventSynchronize(ev.e[4]));
             for (int st = 0; st < 4; ++st) {
                 float ms = 0;
@@ -1451,7 +1519,7 @@ int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps,
         o.ms_total = tot / reps;
         o.reps = reps;
         o.k = ktot;
-        o.variant = var;
+        o.variant = gp.n > 0 ? gp.seg[t.seg >= 0 ? t.seg : 0].variant : 0;
         outs.push_back(o);
     }
     HIPCHECK(ctx, hipStreamSynchronize(s));
@@ -1461,29 +1529,38 @@ int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps,
 
 int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, fcg_mvp_timing* out) {
     if (!out || !jobs || njobs < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
-        return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
-    Job* j0 = get_job(ctx, jobs[0]);
-    if (!j0) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
-    const int var = njobs == 1 ? j0->gather_variant[mvp_prec] : ctx->batch_variant[mvp_prec];
     std::vector<fcg_mvp_timing> outs;
-    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, reps, {var}, outs));
+    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, reps, {Trial{kAsPlanned, 0}}, outs));
     *out = outs[0];
     return FCG_OK;
 }
 
 int fcg_autotune_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, int* variant,
                        double* best_ms) {
-    std::vector<int> vars;
-    for (int v = 0; v < fcg::gather_variant_count(); ++v) vars.push_back(v);
+    if (!jobs || njobs < 2) return fail(ctx, FCG_INVALID_ARGUMENT, "a batch needs at least two jobs");
+    // the class ranges are separate launches: tune each with the others as they are
+    fcg::GatherPlan gp;
     std::vector<fcg_mvp_timing> outs;
-    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, reps, vars, outs));
-    int best = 0;
-    for (size_t i = 1; i < outs.size(); ++i)
-        if (outs[i].ms_gather < outs[(size_t)best].ms_gather) best = (int)i;
-    ctx->batch_variant[mvp_prec] = outs[(size_t)best].variant;
-    if (variant) *variant = outs[(size_t)best].variant;
-    if (best_ms) *best_ms = outs[(size_t)best].ms_gather;
+    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, 1, {Trial{kAsPlanned, 0}}, outs, &gp));
+    std::vector<Trial> trials;
+    const int nv = fcg::gather_variant_count();
+    for (int i = 0; i < gp.n; ++i)
+        for (int v = 0; v < nv; ++v) trials.push_back(Trial{i, v});
+    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, reps, trials, outs));
+    int64_t most = -1;
+    for (int i = 0; i < gp.n; ++i) {
+        int best = 0;
+        for (int v = 1; v < nv; ++v)
+            if (outs[(size_t)(i * nv + v)].ms_gather < outs[(size_t)(i * nv + best)].ms_gather) best = v;
+        ctx->batch_variant[mvp_prec][gp.seg[i].cls] = best;
+        const int64_t rows = gp.seg[i].row_hi - gp.seg[i].row_lo;
+        if (rows > most) {
+            most = rows;
+            if (variant) *variant = best;
+        }
+    }
+    RCCHECK(time_batch(ctx, jobs, njobs, mvp_prec, reps, {Trial{kAsPlanned, 0}}, outs));
+    if (best_ms) *best_ms = outs[0].ms_gather;
     return FCG_OK;
 }
 
@@ -1491,8 +1568,8 @@ int fcg_set_batch_variant(fcg_ctx* ctx, int mvp_prec, int variant) {
     if (!ctx) return FCG_INVALID_ARGUMENT;
     if (mvp_prec < FCG_MVP_FP64 || mvp_prec > FCG_MVP_FP32_ACC64)
         return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
-    if (variant < 0 || variant >= fcg::gather_variant_count()) return fail(ctx, FCG_INVALID_ARGUMENT, "bad gather variant");
-    ctx->batch_variant[mvp_prec] = variant;
+    if (variant >= fcg::gather_variant_count()) return fail(ctx, FCG_INVALID_ARGUMENT, "bad gather variant");
+    for (int& v : ctx->batch_variant[mvp_prec]) v = variant < 0 ? -1 : variant;
     return FCG_OK;
 }
 

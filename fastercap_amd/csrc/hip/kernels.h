@@ -74,10 +74,9 @@ struct BatchView {
     const int64_t* link_off;  // links
     const int64_t* tile_off;  // k * nblk scan tiles
     const int64_t* node_off;  // nodes
-    const int64_t* rowb_off;  // gather blocks (4 rows each)
     const int64_t* leaf_off;  // leaves
     const int64_t* rhs_off;   // right-hand sides
-    int64_t links, tiles, nodes, rowb, leaves, rhs;
+    int64_t links, tiles, nodes, leaves, rhs;
     int kmax;   // largest k in the batch
     int kpmin;  // smallest row stride in the batch
     int any2d;
@@ -115,12 +114,36 @@ struct KrylovView {
 hipError_t launch_link_rows(const BatchView& b, hipStream_t s);
 hipError_t launch_fill(const BatchView& b, int precision, hipStream_t s);
 
+// Rhs classes of the gather: jobs whose node-major row stride is kp = 2^c share
+// class c (strides beyond the last class share it).  A batch lists its jobs
+// ordered by stride, so each class owns one contiguous range of batch rows and
+// the gather runs one launch per class, each with its own lane shape.
+constexpr int kGatherClasses = 12;
+inline int gather_class(int kp) {
+    int c = 0;
+    while ((1 << c) < kp && c < kGatherClasses - 1) ++c;
+    return c;
+}
+struct GatherSeg {
+    int64_t row_lo, row_hi;  // batch rows [row_lo, row_hi)
+    int kpmin, kmax;         // smallest row stride and largest rhs count in the range
+    int cls, variant;        // rhs class and gather lane shape
+};
+struct GatherPlan {
+    int n = 0;
+    GatherSeg seg[kGatherClasses];
+};
+
 // v = P q on every job of the batch (operands in BJob::q / BJob::v).  jacobi
-// divides the input by the self terms first.  `gather_variant` picks the gather
-// lane layout; `ev` (5 events or null) times the stages.
-hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, int gather_variant, hipStream_t s,
+// divides the input by the self terms first.  The plan's segments cover every
+// batch row once and pick each range's gather lane layout; `ev` (5 events or
+// null) times the stages.
+hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan& plan, hipStream_t s,
                       hipEvent_t* ev = nullptr);
 int gather_variant_count();
+// gather lane shape used for rhs class `cls` in product precision `prec` unless
+// a job or context pins another one (measured best shapes of short-row batches)
+int default_gather_variant(int prec, int cls);
 // power-of-two row stride of the node-major arrays for k right-hand sides
 inline int padded_k(int k) {
     int p = 1;

@@ -316,20 +316,23 @@ struct alignas(sizeof(T) * VR) VecT {
     T v[VR];
 };
 
-template <typename TC, typename TQ, typename TA, int KR, int VR, int U>
-__global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd) {
-    constexpr int SL = 64 / KR;
+template <typename TC, typename TQ, typename TA, int KR, int VR, int U, int RPW>
+__global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd, int64_t row_lo, int64_t row_hi) {
+    constexpr int G = 64 / RPW;  // lanes per destination row
+    constexpr int SL = G / KR;
     constexpr int W = KR * VR;
     using V = VecT<TQ, VR>;
     const int64_t gb = (int64_t)(blockIdx.x % kNumXcd) * blocks_per_xcd + (blockIdx.x / kNumXcd);
-    if (gb >= bv.rowb) return;
-    const int jb = find_job(bv.rowb_off, bv.njobs, gb);
-    const BJob& J = bv.jobs[jb];
-    const int row = (int)(gb - bv.rowb_off[jb]) * kGatherWaves + (int)(threadIdx.x >> 6);
-    if (row >= J.M) return;
     const int lane = threadIdx.x & 63;
-    const int slot = lane / KR;
-    const int rl = (lane % KR) * VR;
+    // global row index over the batch; a lane group never straddles two rows
+    const int64_t grow = row_lo + (gb * kGatherWaves + (threadIdx.x >> 6)) * RPW + lane / G;
+    if (grow >= row_hi) return;
+    const int jb = find_job(bv.node_off, bv.njobs, grow);
+    const BJob& J = bv.jobs[jb];
+    const int row = (int)(grow - bv.node_off[jb]);
+    const int gl = lane % G;
+    const int slot = gl / KR;
+    const int rl = (gl % KR) * VR;
     const int k = J.k, kp = J.kp;
     const TC* coef = (sizeof(TC) == 8) ? reinterpret_cast<const TC*>(J.coef64) : reinterpret_cast<const TC*>(J.coef32);
     const TQ* charge =
@@ -368,10 +371,11 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
 #pragma unroll
                 for (int v = 0; v < VR; ++v) acc[v] += w[u] * (TA)ch[u].v[v];
         }
+        // reduce over the slots of this row's lane group
 #pragma unroll
         for (int v = 0; v < VR; ++v)
 #pragma unroll
-            for (int off = KR; off < 64; off <<= 1) acc[v] += __shfl_xor(acc[v], off, 64);
+            for (int off = KR; off < G; off <<= 1) acc[v] += __shfl_xor(acc[v], off, 64);
         if (slot == 0) {
 #pragma unroll
             for (int v = 0; v < VR; ++v)
@@ -384,13 +388,17 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
 }
 
 struct GatherCfg {
-    int kr, vr, u;
+    int kr, vr, u, rpw;
 };
-// requested lane shapes (lanes per link, rhs per lane, links in flight per slot);
-// gather_shape fits them to the batch's row strides
+// requested lane shapes (lanes per link, rhs per lane, links in flight per slot,
+// rows per wavefront); gather_shape fits them to the batch's row strides
 static const GatherCfg kGatherCfgs[] = {
-    {2, 4, 4}, {1, 8, 4}, {4, 2, 4}, {1, 8, 2}, {2, 4, 2}, {8, 1, 4}, {4, 2, 2}, {1, 4, 4}, {16, 1, 2}, {2, 2, 4},
-    {4, 8, 2}, {8, 4, 2}, {16, 2, 2},  // wide shapes for many right-hand sides (k up to 32 per pass)
+    {2, 4, 4, 1}, {1, 8, 4, 1}, {4, 2, 4, 1}, {1, 8, 2, 1}, {2, 4, 2, 1}, {8, 1, 4, 1}, {4, 2, 2, 1},
+    {1, 4, 4, 1}, {16, 1, 2, 1}, {2, 2, 4, 1},
+    {4, 8, 2, 1}, {8, 4, 2, 1}, {16, 2, 2, 1},  // wide shapes for many right-hand sides (k up to 32 per pass)
+    {2, 2, 4, 2}, {1, 4, 4, 2}, {2, 4, 4, 2}, {4, 2, 4, 2}, {1, 2, 4, 4}, {2, 2, 4, 4},  // short rows
+    {1, 2, 2, 4}, {1, 2, 4, 8}, {2, 2, 4, 8}, {1, 1, 4, 8}, {1, 4, 4, 8}, {1, 2, 2, 8}, {1, 2, 4, 16},
+    {1, 1, 4, 16}, {1, 2, 2, 16},
 };
 constexpr int kNumGatherCfgs = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 int gather_variant_count() { return kNumGatherCfgs; }
@@ -399,31 +407,67 @@ int gather_variant_count() { return kNumGatherCfgs; }
 // lanes beyond the largest rhs count are dropped
 static GatherCfg gather_shape(int variant, int kpmin, int kmax) {
     GatherCfg c = kGatherCfgs[(variant >= 0 && variant < kNumGatherCfgs) ? variant : 0];
+    const int G = 64 / c.rpw;
     while (c.vr > kpmin) {
         c.vr /= 2;
-        c.kr = c.kr * 2 > 16 ? 16 : c.kr * 2;
+        c.kr = c.kr * 2 > G / 4 ? c.kr : c.kr * 2;
+        if (c.kr > 16) c.kr = 16;
     }
     while (c.kr > 1 && (c.kr / 2) * c.vr >= kmax) c.kr /= 2;
     return c;
 }
 
+static int find_cfg(int kr, int vr, int u, int rpw) {
+    for (int i = 0; i < kNumGatherCfgs; ++i)
+        if (kGatherCfgs[i].kr == kr && kGatherCfgs[i].vr == vr && kGatherCfgs[i].u == u && kGatherCfgs[i].rpw == rpw)
+            return i;
+    return 0;
+}
+
+int default_gather_variant(int prec, int cls) {
+    // fastest shapes measured per rhs class: few rhs want several rows per wavefront
+    // (rows average 70-160 links there), many rhs want one row per wavefront with
+    // the lanes spread over the rhs
+    const bool f32 = prec == kMvp32;
+    switch (cls) {
+        case 0:
+        case 1: return find_cfg(1, 2, 4, 4);
+        case 2: return f32 ? find_cfg(1, 4, 4, 8) : find_cfg(2, 2, 4, 8);
+        case 3: return f32 ? find_cfg(2, 4, 4, 2) : find_cfg(2, 4, 2, 1);
+        case 4: return find_cfg(8, 4, 2, 1);
+        default: return f32 ? find_cfg(4, 8, 2, 1) : find_cfg(8, 4, 2, 1);
+    }
+}
+
 template <typename TC, typename TQ, typename TA>
-static hipError_t gather_dispatch(const BatchView& bv, int variant, hipStream_t s) {
-    const GatherCfg c = gather_shape(variant, bv.kpmin, bv.kmax);
-    const int64_t per_xcd = (bv.rowb + kNumXcd - 1) / kNumXcd;
+static hipError_t gather_dispatch(const BatchView& bv, const GatherSeg& sg, hipStream_t s) {
+    const GatherCfg c = gather_shape(sg.variant, sg.kpmin, sg.kmax);
+    const int64_t rows = sg.row_hi - sg.row_lo;
+    const int64_t rows_per_block = (int64_t)kGatherWaves * c.rpw;
+    const int64_t blocks = (rows + rows_per_block - 1) / rows_per_block;
+    const int64_t per_xcd = (blocks + kNumXcd - 1) / kNumXcd;
     const unsigned grid = (unsigned)(per_xcd * kNumXcd);
     if (grid == 0) return hipSuccess;
-#define FCG_G(KR, VR, U)                                                       \
-    if (c.kr == KR && c.vr == VR && c.u == U) {                                 \
-        k_gather<TC, TQ, TA, KR, VR, U><<<grid, 256, 0, s>>>(bv, (int)per_xcd); \
-        return hipGetLastError();                                               \
+#define FCG_G(KR, VR, U, R)                                                           \
+    if (c.kr == KR && c.vr == VR && c.u == U && c.rpw == R) {                         \
+        k_gather<TC, TQ, TA, KR, VR, U, R><<<grid, 256, 0, s>>>(bv, (int)per_xcd, sg.row_lo, sg.row_hi); \
+        return hipGetLastError();                                                     \
     }
-#define FCG_GU(KR, VR) FCG_G(KR, VR, 2) FCG_G(KR, VR, 4)
+#define FCG_GU(KR, VR) FCG_G(KR, VR, 2, 1) FCG_G(KR, VR, 4, 1)
     FCG_GU(1, 1) FCG_GU(2, 1) FCG_GU(4, 1) FCG_GU(8, 1) FCG_GU(16, 1)
     FCG_GU(1, 2) FCG_GU(2, 2) FCG_GU(4, 2) FCG_GU(8, 2)
     FCG_GU(1, 4) FCG_GU(2, 4) FCG_GU(4, 4)
     FCG_GU(1, 8) FCG_GU(2, 8) FCG_GU(4, 8)
     FCG_GU(8, 4) FCG_GU(16, 2)
+    FCG_G(1, 1, 4, 2) FCG_G(2, 1, 4, 2) FCG_G(4, 1, 4, 2) FCG_G(8, 1, 4, 2)
+    FCG_G(1, 2, 4, 2) FCG_G(2, 2, 4, 2) FCG_G(4, 2, 4, 2)
+    FCG_G(1, 4, 4, 2) FCG_G(2, 4, 4, 2)
+    FCG_G(1, 1, 4, 4) FCG_G(2, 1, 4, 4) FCG_G(4, 1, 4, 4)
+    FCG_G(1, 2, 4, 4) FCG_G(2, 2, 4, 4)
+    FCG_G(1, 1, 2, 4) FCG_G(2, 1, 2, 4) FCG_G(1, 2, 2, 4) FCG_G(2, 2, 2, 4)
+    FCG_G(1, 1, 4, 8) FCG_G(2, 1, 4, 8) FCG_G(1, 2, 4, 8) FCG_G(2, 2, 4, 8) FCG_G(1, 4, 4, 8) FCG_G(2, 4, 4, 8)
+    FCG_G(1, 1, 2, 8) FCG_G(2, 1, 2, 8) FCG_G(1, 2, 2, 8)
+    FCG_G(1, 1, 4, 16) FCG_G(1, 2, 4, 16) FCG_G(1, 1, 2, 16) FCG_G(1, 2, 2, 16)
 #undef FCG_GU
 #undef FCG_G
     return hipErrorInvalidConfiguration;
@@ -497,7 +541,7 @@ hipError_t launch_fill(const BatchView& b, int precision, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, int variant, hipStream_t s, hipEvent_t* ev) {
+hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan& plan, hipStream_t s, hipEvent_t* ev) {
     if (b.njobs == 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], s);
     k_scan_reduce<<<(unsigned)b.tiles, kScanThreads, 0, s>>>(b, jacobi);
@@ -510,14 +554,16 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, int variant, hip
     else
         k_node_charge<double><<<nb, 256, 0, s>>>(b, jacobi);
     if (ev) (void)hipEventRecord(ev[2], s);
-    hipError_t ge;
-    if (prec == kMvp64)
-        ge = gather_dispatch<double, double, double>(b, variant, s);
-    else if (prec == kMvp32)
-        ge = gather_dispatch<float, float, float>(b, variant, s);
-    else
-        ge = gather_dispatch<float, double, double>(b, variant, s);
-    if (ge != hipSuccess) return ge;
+    for (int i = 0; i < plan.n; ++i) {
+        hipError_t ge;
+        if (prec == kMvp64)
+            ge = gather_dispatch<double, double, double>(b, plan.seg[i], s);
+        else if (prec == kMvp32)
+            ge = gather_dispatch<float, float, float>(b, plan.seg[i], s);
+        else
+            ge = gather_dispatch<float, double, double>(b, plan.seg[i], s);
+        if (ge != hipSuccess) return ge;
+    }
     if (ev) (void)hipEventRecord(ev[3], s);
     const unsigned lb = blocks_for((size_t)b.leaves, 256);
     if (prec == kMvp32)

@@ -213,12 +213,17 @@ int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* b
 /* Pin the gather configuration of a job (variant < 0 only queries the count). */
 int fcg_set_gather_variant(fcg_ctx* ctx, int job, int variant, int* n_variants);
 
-/* Time every gather configuration on one product over several filled jobs in
- * precision mvp_prec (FCG_MVP_*) and keep the fastest for the context's products
- * over several jobs (the solve engine's steps).  No reference counterpart. */
+/* Products over several jobs gather each rhs class (jobs whose padded conductor
+ * count is the same power of two) in a launch of its own.  Time every gather
+ * configuration on each class present in a product over the filled jobs (at
+ * least two) in precision mvp_prec (FCG_MVP_*) and keep each class's fastest for
+ * the context's products over several jobs (the solve engine's steps).  *variant
+ * receives the choice of the class with the most rows, *best_ms the gather time
+ * with every class's choice.  No reference counterpart. */
 int fcg_autotune_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps, int* variant,
                        double* best_ms);
-/* Pin the gather configuration of the context's products over several jobs. */
+/* Pin the gather configuration of every rhs class of the context's products over
+ * several jobs; variant < 0 restores each class's default configuration. */
 int fcg_set_batch_variant(fcg_ctx* ctx, int mvp_prec, int variant);
 
 /* Solve several jobs together: every GMRES step of every job is advanced by one
