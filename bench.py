@@ -93,12 +93,14 @@ def bench_batch(args, fc, dev, tmp):
     # no extra tuning launches (the rocprof averages of the same command then cover
     # the solve launches only)
     tuned = {}
+    if args.only_mixed:  # profiling aid: account for every product of the process
+        dev.profile(enable=1, reset=True)
     for _ in range(max(1, args.warmup)):
         dev.fill_batch(jobs, MIX)
         dev.solve_batch(jobs, meshes, tol=tol, precision=MIX)
     log("warmup done")
     lat = []
-    dev.profile(enable=1, reset=True)
+    dev.profile(enable=1, reset=not args.only_mixed)
     t_start = time.perf_counter()
     for s in range(args.steps):
         a = time.perf_counter()
@@ -113,8 +115,8 @@ def bench_batch(args, fc, dev, tmp):
     value = args.steps * nj / t_mixed
     mixed_caps = [r.cap for r in res]
     iters_mixed = [max(r.iterations) for r in res]
-    if args.only_mixed:  # profiling aid: the timed mixed solves alone
-        log(f"timed-region products: {prof}")
+    if args.only_mixed:  # profiling aid: the mixed solves alone, warm-up included
+        log(f"all products of the process: {prof}")
         return None
 
     # plain FP64 path: one job at a time (fill + solve), as the reference runs
@@ -143,16 +145,20 @@ def bench_batch(args, fc, dev, tmp):
         f"plain FP64 at tol {tol} vs tol 1e-8: {e_plain_conv[0]:.3e}")
 
     # dominant kernel: the FP32 gather of the inner products of the timed solves
-    # (HIP events around every launch; bytes per launch from the active jobs)
+    # (HIP events around the gather of every product, which is one launch per rhs
+    # class of the product's jobs; bytes per product from the active jobs)
     g = prof["fp32"]
     achieved = g["gather_bytes"] / (g["gather_ms"] * 1e-3) / 1e9
-    # HBM bytes per launch measured by rocprofv3 FETCH_SIZE / WRITE_SIZE passes on
-    # this workload, scaled by a plain copy's counter readings (profiles/)
+    # HBM bytes per product: the ratio of HBM bytes (rocprofv3 FETCH_SIZE and
+    # WRITE_SIZE passes on this workload, scaled by a plain copy's readings) to
+    # algorithmic bytes, stored under profiles/, times this run's algorithmic bytes
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "gather_traffic_batch.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f)["bytes_per_launch"]
+            ratio = json.load(f).get("traffic_over_alg")
+        if ratio:
+            traffic = ratio * g["gather_bytes"] / max(1, g["gather_launches"])
     log(f"timed-region products: {prof}")
 
     cpu = None
@@ -188,7 +194,8 @@ def bench_batch(args, fc, dev, tmp):
             "precision": "mixed: FP32 far-field links and inner products, FP64 near field, Arnoldi, refinement",
         },
         "latency_ms": {"p50": 1e3 * pct(lat, 50), "p99": 1e3 * pct(lat, 99)},
-        "batch_gather_shape": {"fp32": 0, "fp32_acc64": 0, "fp64": tuned["fp64"][0]},
+        "batch_gather": "one launch per rhs class (2D lines k=3 -> stride 4, plates k=2 -> stride 2), "
+                        "default per-class shapes for the mixed solves, tuned per class for batched FP64",
         "fp64_plain_solves_per_s": nj / t_plain,
         "fp64_batched_solves_per_s": nj / t_b64,
         "speedup_vs_fp64_plain": value / (nj / t_plain),
@@ -210,9 +217,10 @@ def bench_batch(args, fc, dev, tmp):
             "frac": achieved / PEAK_HBM_GBS,
             "traffic": traffic,
             "traffic_source": "profiles/gather_traffic_batch.md" if traffic else None,
-            "launches": g["gather_launches"],
-            "avg_launch_ms": g["gather_ms"] / max(1, g["gather_launches"]),
-            "alg_bytes_per_launch": g["gather_bytes"] / max(1, g["gather_launches"]),
+            "products": g["gather_launches"],
+            "kernel_launches": g["gather_kernels"],
+            "avg_gather_ms_per_product": g["gather_ms"] / max(1, g["gather_launches"]),
+            "alg_bytes_per_product": g["gather_bytes"] / max(1, g["gather_launches"]),
         },
         "cpu_baseline": cpu,
     }
@@ -293,8 +301,8 @@ def bench_serve(args, fc, dev, tmp):
         "c_rel_err_vs_fp64_tol1e-8_max_vs_diagonal": max(err, err_h),
         "roofline": {"kernel": "k_gather<float,float,float,*> (inner FP32 products while serving)", "bound": "hbm",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
-                     "traffic": None, "launches": g["gather_launches"],
-                     "avg_launch_ms": g["gather_ms"] / max(1, g["gather_launches"])},
+                     "traffic": None, "products": g["gather_launches"], "kernel_launches": g["gather_kernels"],
+                     "avg_gather_ms_per_product": g["gather_ms"] / max(1, g["gather_launches"])},
         "cpu_baseline": None,
     }
 
@@ -394,9 +402,9 @@ def bench_single(args, fc, dev, tmp):
         "c_rel_err_vs_fp64_tol1e-8": {"max_per_entry": e_pe, "max_vs_diagonal": e_pd},
         "roofline": {"kernel": "k_gather (products of the timed solves)", "bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                     "launches": g["gather_launches"],
-                     "avg_launch_ms": g["gather_ms"] / max(1, g["gather_launches"]),
-                     "alg_bytes_per_launch": g["gather_bytes"] / max(1, g["gather_launches"])},
+                     "products": g["gather_launches"], "kernel_launches": g["gather_kernels"],
+                     "avg_gather_ms_per_product": g["gather_ms"] / max(1, g["gather_launches"]),
+                     "alg_bytes_per_product": g["gather_bytes"] / max(1, g["gather_launches"])},
         "cpu_baseline": cpu,
         "host_build_s": t_build,
     }

@@ -345,6 +345,7 @@ class Device:
                 "gather_ms": p.gather_ms[i],
                 "gather_bytes": p.gather_bytes[i],
                 "gather_launches": p.gather_launches[i],
+                "gather_kernels": p.gather_kernels[i],
                 "mvp_ms": p.mvp_ms[i],
                 "mvp_launches": p.mvp_launches[i],
             }

@@ -95,6 +95,7 @@ class Profile(C.Structure):
         ("host_plan_ms", C.c_double),
         ("host_wait_ms", C.c_double),
         ("host_post_ms", C.c_double),
+        ("gather_kernels", C.c_int64 * 3),
     ]
 
 

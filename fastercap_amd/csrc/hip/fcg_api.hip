@@ -453,7 +453,7 @@ void choose_variants(const fcg_ctx* ctx, int prec, const std::vector<BatchEntry>
     }
 }
 
-// Algorithmic bytes of one gather launch over a batch (see fcg_profile).
+// Algorithmic bytes of the gather of one product over a batch (see fcg_profile).
 double gather_alg_bytes(const std::vector<BatchEntry>& ents, int prec) {
     const double bc = prec == fcg::kMvp64 ? 8.0 : 4.0;
     const double bn = prec == fcg::kMvp32 ? 4.0 : 8.0;
@@ -467,16 +467,17 @@ double gather_alg_bytes(const std::vector<BatchEntry>& ents, int prec) {
 struct ProfEvents {
     hipEvent_t e[5] = {};
     bool on = false;
-    int prec = 0;
+    int prec = 0, kernels = 0;
     double bytes = 0.0;
     ~ProfEvents() {
         for (auto x : e)
             if (x) (void)hipEventDestroy(x);
     }
-    hipError_t arm(bool enable, int p, double b) {
+    hipError_t arm(bool enable, int p, double b, int nkernels) {
         on = enable;
         prec = p;
         bytes = b;
+        kernels = nkernels;
         if (!on) return hipSuccess;
         for (auto& x : e)
             if (!x) {
@@ -495,6 +496,7 @@ struct ProfEvents {
         pr.gather_ms[prec] += g;
         pr.gather_bytes[prec] += bytes;
         pr.gather_launches[prec] += 1;
+        pr.gather_kernels[prec] += kernels;
         pr.mvp_ms[prec] += t;
         pr.mvp_launches[prec] += 1;
         on = false;
@@ -1027,7 +1029,7 @@ int engine_step(fcg_engine* E) {
         RCCHECK(make_batch(ctx, E->tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32Acc : fcg::kMvp64;
         choose_variants(ctx, prec, be, gp);
-        HIPCHECK(ctx, E->pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
+        HIPCHECK(ctx, E->pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
         hipEvent_t* ev = E->pe_res.events();
         q.push_back([=] { return fcg::launch_mvp(bv, prec, 0, gp, s, ev); });
         KrylovView kv;
@@ -1054,7 +1056,7 @@ int engine_step(fcg_engine* E) {
         RCCHECK(make_batch(ctx, E->tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32 : fcg::kMvp64;
         choose_variants(ctx, prec, be, gp);
-        HIPCHECK(ctx, E->pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0));
+        HIPCHECK(ctx, E->pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
         hipEvent_t* ev = E->pe_arn.events();
         const int jac = arn[0]->jac ? 1 : 0;
         q.push_back([=] { return fcg::launch_mvp(bv, prec, jac, gp, s, ev); });

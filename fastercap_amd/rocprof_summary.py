@@ -4,8 +4,8 @@
         per-kernel totals (time, calls, average) and, for every PREFIX, the sum over
         all kernels whose name starts with it (template instantiations of one kernel)
     python -m fastercap_amd.rocprof_summary pmc COUNTER_COLLECTION.csv COUNTER PREFIX [--scale F]
-        mean / min / max of COUNTER per dispatch of the kernels starting with PREFIX,
-        optionally multiplied by a calibration factor F
+        mean / min / max / total of COUNTER per dispatch of the kernels starting with
+        PREFIX, optionally multiplied by a calibration factor F
 """
 from __future__ import annotations
 
@@ -45,7 +45,7 @@ def pmc(path: str, counter: str, prefix: str, scale: float = 1.0):
     if not vals:
         return {"counter": counter, "prefix": prefix, "dispatches": 0}
     return {"counter": counter, "prefix": prefix, "dispatches": len(vals), "mean": sum(vals) / len(vals),
-            "min": min(vals), "max": max(vals), "scale": scale}
+            "min": min(vals), "max": max(vals), "total": sum(vals), "scale": scale}
 
 
 def main(argv=None):

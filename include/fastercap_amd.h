@@ -180,12 +180,14 @@ typedef struct fcg_mvp_timing {
 } fcg_mvp_timing;
 int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* out);
 
-/* Device time and algorithmic bytes of the gather launches made by solves on this
+/* Device time and algorithmic bytes of the gathers made by solves on this
  * context, accumulated per product precision (index FCG_MVP_*) while profiling is
- * on (HIP events around every gather launch of the solve engine).  Algorithmic
- * bytes of one launch = sum over its jobs of L (b_coef + 4) + 16 M + 8 + 2 M k b_node
- * (b_coef, b_node = 8 for FP64 and 4 for FP32 coefficients / node values).
- * No reference counterpart (a measurement hook). */
+ * on (HIP events around the gather of every product of the solve engine; a
+ * product's gather is one kernel launch per rhs class of its jobs, counted in
+ * gather_kernels).  Algorithmic bytes of one product's gather = sum over its jobs
+ * of L (b_coef + 4) + 16 M + 8 + 2 M k b_node (b_coef, b_node = 8 for FP64 and 4
+ * for FP32 coefficients / node values).  No reference counterpart (a measurement
+ * hook). */
 typedef struct fcg_profile {
     double gather_ms[3];
     double gather_bytes[3];
@@ -196,6 +198,7 @@ typedef struct fcg_profile {
     double host_plan_ms;     /* host time building tables and queueing launches */
     double host_wait_ms;     /* host time blocked on the step's synchronisation */
     double host_post_ms;     /* host time in Givens updates and result bookkeeping */
+    int64_t gather_kernels[3]; /* gather kernel launches (gather_launches counts products) */
 } fcg_profile;
 /* enable: 1 on, 0 off, -1 unchanged; reset clears the totals after copying them to out (may be NULL). */
 int fcg_ctx_profile(fcg_ctx* ctx, int enable, int reset, fcg_profile* out);
