@@ -216,7 +216,7 @@ struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
-        col_range, self, diag, coef64, coef32, perm, iperm, rhs_range;
+        col_range, self, diag, coef64, coef32, perm, iperm, rorder, rhs_range;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -236,7 +236,7 @@ struct Job {
         return (int64_t)(nodes.bytes + left.bytes + right.bytes + parent.bytes + leaf_lo.bytes + leaf_hi.bytes +
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
-                         perm.bytes + iperm.bytes + rhs_range.bytes + scr.bytes());
+                         perm.bytes + iperm.bytes + rorder.bytes + rhs_range.bytes + scr.bytes());
     }
 
     BJob rec(const Scratch& sc, const double* q, double* v) const {
@@ -254,6 +254,7 @@ struct Job {
         b.col = col.as<int32_t>();
         b.perm = perm.as<int32_t>();
         b.iperm = iperm.as<int32_t>();
+        b.rorder = rorder.as<int32_t>();
         b.link_row = link_row.as<int32_t>();
         b.coef64 = coef64.as<double>();
         b.coef32 = coef32.as<float>();
@@ -547,13 +548,16 @@ int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
     return FCG_OK;
 }
 
+constexpr int64_t kGatherRowChunk = 4096;
+
 // Charge-row layout of a job: nodes grouped by tree depth (root first), post-order
 // within a depth, so the sources one row links to (mostly nodes of similar size
 // near the row) sit in neighbouring rows; every row's links are then sorted by
-// source row.  Returns perm (node -> row), iperm, the sorted source rows and each
-// stored link's offset within its row in the descriptor's order.
+// source row.  Returns perm (node -> row), iperm, the gather's row order, the
+// sorted source rows and each stored link's offset within its row in the
+// descriptor's order.
 void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector<int32_t>& iperm,
-                  std::vector<int32_t>& col_sorted, std::vector<uint32_t>& link_pos) {
+                  std::vector<int32_t>& rorder, std::vector<int32_t>& col_sorted, std::vector<uint32_t>& link_pos) {
     const int M = d->n_nodes;
     std::vector<int32_t> depth((size_t)M, 0);
     int maxd = 0;
@@ -571,6 +575,22 @@ void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector
         const int64_t slot = start[(size_t)depth[(size_t)i]]++;
         perm[(size_t)i] = (int32_t)slot;
         iperm[(size_t)slot] = i;
+    }
+    {
+        // gather row order: post-order chunks of kGatherRowChunk nodes (each about
+        // a subtree, so the rows one XCD walks stay spatially close and their
+        // sources stay in its L2), charge-row order inside a chunk (neighbouring
+        // rows of a wavefront are same-depth neighbours with overlapping sources)
+        const int64_t C = kGatherRowChunk;
+        const int64_t nc = (M + C - 1) / C;
+        std::vector<int64_t> cs((size_t)nc + 1, 0);
+        for (int i = 0; i < M; ++i) cs[(size_t)(i / C) + 1]++;
+        for (int64_t c = 0; c < nc; ++c) cs[(size_t)c + 1] += cs[(size_t)c];
+        rorder.assign((size_t)M, 0);
+        for (int sl = 0; sl < M; ++sl) {
+            const int node = iperm[(size_t)sl];
+            rorder[(size_t)cs[(size_t)(node / C)]++] = node;
+        }
     }
     const int64_t L = d->n_links;
     col_sorted.assign((size_t)L, 0);
@@ -1217,10 +1237,11 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     HIPCHECK(ctx, upload(j->leaf_node, d->leaf_node, (size_t)j->N, s));
     HIPCHECK(ctx, upload(j->row_ptr, d->row_ptr, (size_t)j->M + 1, s));
     {
-        std::vector<int32_t> perm, iperm, cols;
-        build_layout(d, perm, iperm, cols, j->link_pos);
+        std::vector<int32_t> perm, iperm, rorder, cols;
+        build_layout(d, perm, iperm, rorder, cols, j->link_pos);
         HIPCHECK(ctx, upload(j->perm, perm.data(), perm.size(), s));
         HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
+        HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
         HIPCHECK(ctx, upload(j->col, cols.data(), cols.size(), s));
         j->h_row_ptr.assign(d->row_ptr, d->row_ptr + j->M + 1);
         HIPCHECK(ctx, hipStreamSynchronize(s));  // the host vectors above are released here

@@ -42,6 +42,7 @@ struct BJob {
     const int32_t* col;       // [L] source slot (permuted node index), ascending within a row
     const int32_t* perm;      // [M] node -> slot of its charge / potential row
     const int32_t* iperm;     // [M] slot -> node
+    const int32_t* rorder;    // [M] gather row order -> node
     const int32_t* link_row;  // [L] destination node of each link (fill only)
     double* coef64;           // [L] or null
     float* coef32;            // [L] or null

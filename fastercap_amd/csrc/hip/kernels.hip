@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
     if (grow >= row_hi) return;
     const int jb = find_job(bv.node_off, bv.njobs, grow);
     const BJob& J = bv.jobs[jb];
-    const int row = (int)(grow - bv.node_off[jb]);
+    const int row = J.rorder[grow - bv.node_off[jb]];
     const int gl = lane % G;
     const int slot = gl / KR;
     const int rl = (gl % KR) * VR;
@@ -431,9 +431,9 @@ int default_gather_variant(int prec, int cls) {
     const bool f32 = prec == kMvp32;
     switch (cls) {
         case 0:
-        case 1: return find_cfg(1, 2, 4, 4);
+        case 1: return find_cfg(1, 2, 4, 8);
         case 2: return f32 ? find_cfg(1, 4, 4, 8) : find_cfg(2, 2, 4, 8);
-        case 3: return f32 ? find_cfg(2, 4, 4, 2) : find_cfg(2, 4, 2, 1);
+        case 3: return find_cfg(2, 4, 4, 2);
         case 4: return find_cfg(8, 4, 2, 1);
         default: return f32 ? find_cfg(4, 8, 2, 1) : find_cfg(8, 4, 2, 1);
     }
