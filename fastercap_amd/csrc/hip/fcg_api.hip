@@ -216,7 +216,7 @@ struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
-        col_range, self, diag, coef64, coef32, perm, iperm, rorder, rhs_range;
+        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, rhs_range;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -236,7 +236,8 @@ struct Job {
         return (int64_t)(nodes.bytes + left.bytes + right.bytes + parent.bytes + leaf_lo.bytes + leaf_hi.bytes +
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
-                         perm.bytes + iperm.bytes + rorder.bytes + rhs_range.bytes + scr.bytes());
+                         perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + rhs_range.bytes +
+                         scr.bytes());
     }
 
     BJob rec(const Scratch& sc, const double* q, double* v) const {
@@ -255,6 +256,7 @@ struct Job {
         b.perm = perm.as<int32_t>();
         b.iperm = iperm.as<int32_t>();
         b.rorder = rorder.as<int32_t>();
+        b.fill_order = fill_order.as<uint32_t>();
         b.link_row = link_row.as<int32_t>();
         b.coef64 = coef64.as<double>();
         b.coef32 = coef32.as<float>();
@@ -514,6 +516,8 @@ int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
     const int N = d->n_leaves, M = d->n_nodes;
     if (N < 1 || M < N || d->n_groups < 1 || d->n_links < 0)
         return fail(ctx, FCG_INVALID_ARGUMENT, "bad sizes in job descriptor");
+    if (d->n_links > (int64_t)UINT32_MAX)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "more than 2^32 links in one job (link indices are 32-bit)");
     if (d->row_ptr[0] != 0 || d->row_ptr[M] != d->n_links)
         return fail(ctx, FCG_INVALID_ARGUMENT, "row_ptr does not span the links");
     for (int i = 0; i < M; ++i) {
@@ -615,6 +619,73 @@ void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector
                     link_pos[(size_t)l] = tmp[(size_t)(l - b)].second;
                 }
             }
+        });
+    }
+    for (auto& x : th) x.join();
+}
+
+// Branch of the link integral (PotEstimateOpt, Solver/Potential.cpp) a link
+// takes: the bracket of centroid distance over the larger side, dielectric rows
+// apart.  The fill evaluates the branch itself; this only schedules it.
+constexpr int kFillClasses = 10;
+int fill_class(const fcg_node_geom& a, const fcg_node_geom& b, int dim) {
+    const double dx = (double)a.c[0] - (double)b.c[0], dy = (double)a.c[1] - (double)b.c[1];
+    const double dz = dim == 3 ? (double)a.c[2] - (double)b.c[2] : 0.0;
+    const double ratio = std::sqrt(dx * dx + dy * dy + dz * dz) / std::max(a.maxside, b.maxside);
+    int c;
+    if (dim == 3)
+        c = ratio < 1.0 ? 0 : ratio < 2.0 ? 1 : ratio < 4.0 ? 2 : ratio < 10.0 ? 3 : 4;
+    else
+        c = ratio < 4.0 ? 0 : 1;
+    return (a.flags & FCG_NODE_DIEL) ? c + 5 : c;
+}
+
+// Fill order of a job's stored links: grouped by fill_class (stable, so links of
+// one class keep their row order), so the lanes of a wavefront mostly take the
+// same branch instead of every wavefront running each branch its lanes need.
+void build_fill_order(const fcg_job_desc* d, const std::vector<int32_t>& iperm,
+                      const std::vector<int32_t>& col_sorted, std::vector<uint32_t>& order) {
+    const int M = d->n_nodes;
+    const int64_t L = d->n_links;
+    order.assign((size_t)L, 0);
+    if (L == 0) return;
+    const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<uint8_t> cls((size_t)L);
+    std::vector<int64_t> cnt((size_t)nth * kFillClasses, 0);
+    auto row_range = [&](int t, int& r0, int& r1) {  // rows split by link count
+        r0 = (int)(std::lower_bound(d->row_ptr, d->row_ptr + M + 1, L * t / nth) - d->row_ptr);
+        r1 = (int)(std::lower_bound(d->row_ptr, d->row_ptr + M + 1, L * (t + 1) / nth) - d->row_ptr);
+        if (t == nth - 1) r1 = M;
+    };
+    std::vector<std::thread> th;
+    for (int t = 0; t < nth; ++t) {
+        th.emplace_back([&, t] {
+            int r0, r1;
+            row_range(t, r0, r1);
+            for (int r = r0; r < r1; ++r)
+                for (int64_t l = d->row_ptr[r]; l < d->row_ptr[r + 1]; ++l) {
+                    const int c = fill_class(d->nodes[r], d->nodes[iperm[(size_t)col_sorted[(size_t)l]]], d->dim);
+                    cls[(size_t)l] = (uint8_t)c;
+                    cnt[(size_t)t * kFillClasses + c]++;
+                }
+        });
+    }
+    for (auto& x : th) x.join();
+    th.clear();
+    // class-major, thread-minor offsets keep the partition stable
+    std::vector<int64_t> off((size_t)nth * kFillClasses, 0);
+    int64_t run = 0;
+    for (int c = 0; c < kFillClasses; ++c)
+        for (int t = 0; t < nth; ++t) {
+            off[(size_t)t * kFillClasses + c] = run;
+            run += cnt[(size_t)t * kFillClasses + c];
+        }
+    for (int t = 0; t < nth; ++t) {
+        th.emplace_back([&, t] {
+            int r0, r1;
+            row_range(t, r0, r1);
+            int64_t* o = &off[(size_t)t * kFillClasses];
+            for (int64_t l = d->row_ptr[r0]; l < d->row_ptr[r1]; ++l) order[(size_t)o[cls[(size_t)l]]++] = (uint32_t)l;
         });
     }
     for (auto& x : th) x.join();
@@ -1242,6 +1313,9 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         HIPCHECK(ctx, upload(j->perm, perm.data(), perm.size(), s));
         HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
         HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
+        std::vector<uint32_t> forder;
+        build_fill_order(d, iperm, cols, forder);
+        if (!forder.empty()) HIPCHECK(ctx, upload(j->fill_order, forder.data(), forder.size(), s));
         HIPCHECK(ctx, upload(j->col, cols.data(), cols.size(), s));
         j->h_row_ptr.assign(d->row_ptr, d->row_ptr + j->M + 1);
         HIPCHECK(ctx, hipStreamSynchronize(s));  // the host vectors above are released here

@@ -43,6 +43,7 @@ struct BJob {
     const int32_t* perm;      // [M] node -> slot of its charge / potential row
     const int32_t* iperm;     // [M] slot -> node
     const int32_t* rorder;    // [M] gather row order -> node
+    const uint32_t* fill_order;  // [L] links grouped by integral branch
     const int32_t* link_row;  // [L] destination node of each link (fill only)
     double* coef64;           // [L] or null
     float* coef32;            // [L] or null

@@ -141,7 +141,7 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, int mixed) {
     if (g >= bv.links) return;
     const int jb = find_job(bv.link_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
-    const int64_t l = g - bv.link_off[jb];
+    const int64_t l = J.fill_order[g - bv.link_off[jb]];
     const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
     const fcg_node_geom& p2 = J.nodes[J.iperm[J.col[l]]];
     if (mixed)
