@@ -20,8 +20,10 @@
 
 #if defined(__HIPCC__)
 #define FCG_HD __host__ __device__ inline
+#define FCG_UNROLL _Pragma("unroll")
 #else
 #define FCG_HD inline
+#define FCG_UNROLL
 #endif
 
 #define FCG_FOUR_PI_TIMES_E0 1.1126500559756e-10
@@ -212,6 +214,51 @@ FCG_HD T enfield_num_3d(Vec3<T> r, const float* v, Vec3<T> tn, int rule) {
     }
     result *= f.jac / (T)2.0;
     result /= f.area;
+    return result;
+}
+
+// Device forms of the two quadratures above.  Point (px, py) of the local frame
+// is the barycentric point py V0 + px V1 + (1 - px - py) V2, and the frame's
+// Jacobian equals twice the area, so each rule reduces to a weighted sum over
+// points formed from the vertices relative to r: no frame, no normalisations.
+// Mathematically identical; results differ from the frame forms by rounding only.
+template <typename T>
+FCG_HD T inv_sqrt(T x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrt(x);
+#else
+    return (T)1 / sqrt(x);
+#endif
+}
+
+template <typename T, int RULE>
+FCG_HD T potential_quad_3d(Vec3<T> r, const float* v) {
+    const Vec3<T> a = sub3(vtx<T>(v, 0), r), b = sub3(vtx<T>(v, 1), r), c = sub3(vtx<T>(v, 2), r);
+    const QuadRule q = quad_rule(RULE);
+    T result = 0;
+FCG_UNROLL
+    for (int i = 0; i < q.n; ++i) {
+        const T w0 = (T)q.y[i], w1 = (T)q.x[i], w2 = (T)1 - w0 - w1;
+        const Vec3<T> d{w0 * a.x + w1 * b.x + w2 * c.x, w0 * a.y + w1 * b.y + w2 * c.y,
+                        w0 * a.z + w1 * b.z + w2 * c.z};
+        result += (T)q.w[i] * inv_sqrt(dot3(d, d));
+    }
+    return result;
+}
+
+template <typename T, int RULE>
+FCG_HD T enfield_quad_3d(Vec3<T> r, const float* v, Vec3<T> tn) {
+    const Vec3<T> a = sub3(r, vtx<T>(v, 0)), b = sub3(r, vtx<T>(v, 1)), c = sub3(r, vtx<T>(v, 2));
+    const QuadRule q = quad_rule(RULE);
+    T result = 0;
+FCG_UNROLL
+    for (int i = 0; i < q.n; ++i) {
+        const T w0 = (T)q.y[i], w1 = (T)q.x[i], w2 = (T)1 - w0 - w1;
+        const Vec3<T> d{w0 * a.x + w1 * b.x + w2 * c.x, w0 * a.y + w1 * b.y + w2 * c.y,
+                        w0 * a.z + w1 * b.z + w2 * c.z};
+        const T ir = inv_sqrt(dot3(d, d));
+        result += (T)q.w[i] * dot3(d, tn) * (ir * ir * ir);
+    }
     return result;
 }
 

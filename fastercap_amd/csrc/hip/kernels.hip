@@ -95,9 +95,9 @@ __device__ T pot_estimate(const fcg_node_geom& p1, const fcg_node_geom& p2, int 
     const T k4pe0 = (T)FCG_FOUR_PI_TIMES_E0;
     if (!diel1) {
         if (ratio < (T)1.0) return potential_opt_3d<T>(c1, p2.v) / k4pe0;
-        if (ratio < (T)2.0) return potential_num_3d<T>(c1, p2.v, 5) / k4pe0;
-        if (ratio < (T)4.0) return potential_num_3d<T>(c1, p2.v, 3) / k4pe0;
-        if (ratio < (T)10.0) return potential_num_3d<T>(c1, p2.v, 2) / k4pe0;
+        if (ratio < (T)2.0) return potential_quad_3d<T, 5>(c1, p2.v) / k4pe0;
+        if (ratio < (T)4.0) return potential_quad_3d<T, 3>(c1, p2.v) / k4pe0;
+        if (ratio < (T)10.0) return potential_quad_3d<T, 2>(c1, p2.v) / k4pe0;
         return (T)1 / (k4pe0 * rd);
     }
     const Vec3<T> n1{(T)p1.dn[0], (T)p1.dn[1], (T)p1.dn[2]};
@@ -109,9 +109,9 @@ __device__ T pot_estimate(const fcg_node_geom& p1, const fcg_node_geom& p2, int 
         const T pn = potential_opt_3d<T>(em, p2.v);
         return (pn - pp) / ((T)2 * h * k4pe0);
     }
-    if (ratio < (T)2.0) return enfield_num_3d<T>(c1, p2.v, n1, 5) / k4pe0;
-    if (ratio < (T)4.0) return enfield_num_3d<T>(c1, p2.v, n1, 3) / k4pe0;
-    if (ratio < (T)10.0) return enfield_num_3d<T>(c1, p2.v, n1, 2) / k4pe0;
+    if (ratio < (T)2.0) return enfield_quad_3d<T, 5>(c1, p2.v, n1) / k4pe0;
+    if (ratio < (T)4.0) return enfield_quad_3d<T, 3>(c1, p2.v, n1) / k4pe0;
+    if (ratio < (T)10.0) return enfield_quad_3d<T, 2>(c1, p2.v, n1) / k4pe0;
     return dot3(dist, n1) / (k4pe0 * rd * rd * rd);
 }
 
