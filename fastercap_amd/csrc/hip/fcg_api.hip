@@ -759,15 +759,48 @@ struct Ready {
 
 }  // namespace
 
-struct fcg_engine {
-    fcg_ctx* ctx = nullptr;
-    ProfEvents pe_res, pe_arn;
-    fcg_solve_opts opts{};
+namespace {
+// One part of an engine's solves.  The engine alternates between its cohorts, so
+// the host's bookkeeping and planning of one cohort overlap the device work of
+// the other; each cohort's own steps stay strictly ordered (plan, launch, wait,
+// bookkeeping), so results do not depend on the split.
+struct Cohort {
     std::vector<std::unique_ptr<Solve>> active;
-    std::deque<Ready> ready;
     Tables tabs;
     PBuf partial_a, partial_b, hbuf;
+    ProfEvents pe_res, pe_arn;
+    hipEvent_t ev = nullptr;  // end of the step in flight
+    bool inflight = false;
+    int64_t load = 0;  // links x rhs of the active solves (balances submissions)
+    // the step in flight, as its bookkeeping needs it
+    std::vector<Solve*> res, arn;
+    std::vector<std::pair<Solve*, int64_t>> fin;
+    std::vector<double*> fin_x;
+    double* fin_host = nullptr;
+    double* hh = nullptr;
+    PBuf fin_dev;
+    clk::time_point t_plan, t_wait;
+    Cohort() = default;
+    Cohort(const Cohort&) = delete;
+    Cohort& operator=(const Cohort&) = delete;
+    ~Cohort() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+};
+constexpr int kCohorts = 2;
+}  // namespace
+
+struct fcg_engine {
+    fcg_ctx* ctx = nullptr;
+    fcg_solve_opts opts{};
+    Cohort co[kCohorts];
+    std::deque<Ready> ready;
     bool want_x = false;
+    size_t n_active() const {
+        size_t n = 0;
+        for (const Cohort& c : co) n += c.active.size();
+        return n;
+    }
 };
 
 namespace {
@@ -994,16 +1027,21 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->total_steps.assign((size_t)S->k, 0);
     S->t0 = clk::now();
     S->phase = kInit;
-    E->active.push_back(std::move(S));
+    Cohort* C = &E->co[0];
+    for (Cohort& c : E->co)
+        if (c.load < C->load) C = &c;
+    C->load += J->L * (int64_t)S->k;
+    C->active.push_back(std::move(S));
     return FCG_OK;
 }
 
-int engine_step(fcg_engine* E) {
+// Plan the next step of cohort C and queue it; its bookkeeping is engine_finish.
+int engine_launch(fcg_engine* E, Cohort& C) {
     fcg_ctx* ctx = E->ctx;
     hipStream_t s = ctx->stream;
-    const auto t_plan = clk::now();
-    E->tabs.reset();  // the previous step ended with a synchronisation
-    E->tabs.defer = true;
+    C.t_plan = clk::now();
+    C.tabs.reset();  // the cohort's previous step has completed (engine_finish)
+    C.tabs.defer = true;
     const bool mixed = E->opts.precision == FCG_PREC_MIXED;
     // Every table of the step is written to pinned staging first and reaches the
     // device in one copy; the launches are queued here and issued after that copy.
@@ -1013,7 +1051,7 @@ int engine_step(fcg_engine* E) {
     {
         std::vector<KEnt> sc;
         std::vector<double> dvals;
-        for (auto& up : E->active) {
+        for (auto& up : C.active) {
             Solve& S = *up;
             if (S.phase != kArn || !S.need_scale) continue;
             RCCHECK(ensure_basis(ctx, S, S.i + 1));
@@ -1022,15 +1060,15 @@ int engine_step(fcg_engine* E) {
         }
         if (!sc.empty()) {
             KrylovView kv;
-            RCCHECK(make_krylov(ctx, E->tabs, sc, &kv));
+            RCCHECK(make_krylov(ctx, C.tabs, sc, &kv));
             void* dd;
-            HIPCHECK(ctx, E->tabs.push(dvals.data(), sizeof(double) * dvals.size(), s, &dd));
+            HIPCHECK(ctx, C.tabs.push(dvals.data(), sizeof(double) * dvals.size(), s, &dd));
             q.push_back([=] { return fcg::launch_scale_new(kv, reinterpret_cast<double*>(dd), s); });
             for (auto& e : sc) e.S->need_scale = false;
         }
         std::vector<KEnt> up;
         std::vector<double> zall;
-        for (auto& u : E->active) {
+        for (auto& u : C.active) {
             Solve& S = *u;
             if (S.phase != kUpdate) continue;
             up.push_back(KEnt{&S, S.w.as<double>(), 0, (int)zall.size()});
@@ -1038,9 +1076,9 @@ int engine_step(fcg_engine* E) {
         }
         if (!up.empty()) {
             KrylovView kv;
-            RCCHECK(make_krylov(ctx, E->tabs, up, &kv));
+            RCCHECK(make_krylov(ctx, C.tabs, up, &kv));
             void* dz;
-            HIPCHECK(ctx, E->tabs.push(zall.data(), sizeof(double) * zall.size(), s, &dz));
+            HIPCHECK(ctx, C.tabs.push(zall.data(), sizeof(double) * zall.size(), s, &dz));
             q.push_back([=] { return fcg::launch_xupdate(kv, reinterpret_cast<double*>(dz), s); });
             for (auto& e : up) {
                 e.S->nz = 0;
@@ -1048,41 +1086,33 @@ int engine_step(fcg_engine* E) {
             }
         }
         std::vector<KEnt> init;
-        for (auto& u : E->active)
+        for (auto& u : C.active)
             if (u->phase == kInit) init.push_back(KEnt{u.get(), u->w.as<double>(), 0, 0});
         if (!init.empty()) {
             KrylovView kv;
-            RCCHECK(make_krylov(ctx, E->tabs, init, &kv));
+            RCCHECK(make_krylov(ctx, C.tabs, init, &kv));
             q.push_back([=] { return fcg::launch_init_x(kv, s); });
             for (auto& e : init) e.S->phase = kResid;
         }
     }
-    // finishing solves: conductor charge sums (SolveCapacitance.cpp:1305-1366)
-    std::vector<std::pair<Solve*, double*>> fin;
+    // finishing solves: conductor charge sums (SolveCapacitance.cpp:1305-1366), one
+    // launch and one copy for all of them
+    std::vector<std::pair<Solve*, int64_t>> fin;
     std::vector<double*> fin_x;
-    std::vector<std::unique_ptr<PBuf>> fin_tmp;
-    for (auto& u : E->active) {
+    std::vector<fcg::ChargeSumJob> csj;
+    int64_t fin_vals = 0;
+    for (auto& u : C.active) {
         Solve& S = *u;
         if (S.phase != kFinish) continue;
-        const size_t nval = (size_t)S.k * S.ncol;
-        void* h;
-        HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * nval, &h));
-        fin_tmp.emplace_back(new PBuf());
-        HIPCHECK(ctx, fin_tmp.back()->get(&ctx->pool, sizeof(double) * nval));
-        {
-            const double* x = S.x.as<double>();
-            const double* w = S.J->leaf_w.as<double>();
-            const int32_t* cr = S.J->col_range.as<int32_t>();
-            double* out = fin_tmp.back()->as<double>();
-            const int N = S.N, k = S.k, nc = S.ncol;
-            q.push_back([=] { return fcg::launch_charge_sums(x, w, cr, N, k, nc, out, s); });
-            q.push_back([=] { return hipMemcpyAsync(h, out, sizeof(double) * nval, hipMemcpyDeviceToHost, s); });
-        }
-        fin.emplace_back(&S, reinterpret_cast<double*>(h));
+        const int64_t nval = (int64_t)S.k * S.ncol;
+        csj.push_back(fcg::ChargeSumJob{S.x.as<double>(), S.J->leaf_w.as<double>(), S.J->col_range.as<int32_t>(),
+                                        S.N, S.k, S.ncol, fin_vals, fin_vals});
+        fin.emplace_back(&S, fin_vals);
+        fin_vals += nval;
         double* hx = nullptr;
         if (S.want_x) {
             void* hxv;
-            HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * (size_t)S.k * S.N, &hxv));
+            HIPCHECK(ctx, C.tabs.host_area(sizeof(double) * (size_t)S.k * S.N, &hxv));
             hx = reinterpret_cast<double*>(hxv);
             const void* xs = S.x.p;
             const size_t nx = sizeof(double) * (size_t)S.k * S.N;
@@ -1090,10 +1120,26 @@ int engine_step(fcg_engine* E) {
         }
         fin_x.push_back(hx);
     }
+    double* fin_host = nullptr;
+    PBuf& fin_dev = C.fin_dev;
+    if (!csj.empty()) {
+        void *dt, *h;
+        HIPCHECK(ctx, C.tabs.push(csj.data(), sizeof(fcg::ChargeSumJob) * csj.size(), s, &dt));
+        HIPCHECK(ctx, C.tabs.host_area(sizeof(double) * (size_t)fin_vals, &h));
+        HIPCHECK(ctx, fin_dev.get(&ctx->pool, sizeof(double) * (size_t)fin_vals));
+        fin_host = reinterpret_cast<double*>(h);
+        const auto* jt = reinterpret_cast<const fcg::ChargeSumJob*>(dt);
+        const int nj = (int)csj.size();
+        const int64_t nb = fin_vals;
+        double* out = fin_dev.as<double>();
+        const size_t nbytes = sizeof(double) * (size_t)fin_vals;
+        q.push_back([=] { return fcg::launch_charge_sums(jt, nj, nb, out, s); });
+        q.push_back([=] { return hipMemcpyAsync(h, out, nbytes, hipMemcpyDeviceToHost, s); });
+    }
 
     // ---- B: refinement residuals r = b - A x
     std::vector<Solve*> res, arn;
-    for (auto& u : E->active) {
+    for (auto& u : C.active) {
         if (u->phase == kResid) res.push_back(u.get());
         if (u->phase == kArn) arn.push_back(u.get());
     }
@@ -1106,7 +1152,7 @@ int engine_step(fcg_engine* E) {
         S->hoff = hoff;
         hoff += (2 * (S->i + 1) + 1) * S->k;
     }
-    RCCHECK(ensure_pbuf(ctx, E->hbuf, sizeof(double) * (size_t)std::max(hoff, 1)));
+    RCCHECK(ensure_pbuf(ctx, C.hbuf, sizeof(double) * (size_t)std::max(hoff, 1)));
     if (!res.empty()) {
         std::vector<BatchEntry> be;
         std::vector<KEnt> ke;
@@ -1117,17 +1163,17 @@ int engine_step(fcg_engine* E) {
         }
         BatchView bv;
         fcg::GatherPlan gp;
-        RCCHECK(make_batch(ctx, E->tabs, be, &bv, &gp));
+        RCCHECK(make_batch(ctx, C.tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32Acc : fcg::kMvp64;
         choose_variants(ctx, prec, be, gp);
-        HIPCHECK(ctx, E->pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
-        hipEvent_t* ev = E->pe_res.events();
+        HIPCHECK(ctx, C.pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
+        hipEvent_t* ev = C.pe_res.events();
         q.push_back([=] { return fcg::launch_mvp(bv, prec, 0, gp, s, ev); });
         KrylovView kv;
-        RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
-        RCCHECK(ensure_pbuf(ctx, E->partial_a, sizeof(double) * (size_t)kv.nchunks));
-        double* pa = E->partial_a.as<double>();
-        double* hb = E->hbuf.as<double>();
+        RCCHECK(make_krylov(ctx, C.tabs, ke, &kv));
+        RCCHECK(ensure_pbuf(ctx, C.partial_a, sizeof(double) * (size_t)kv.nchunks));
+        double* pa = C.partial_a.as<double>();
+        double* hb = C.hbuf.as<double>();
         q.push_back([=] { return fcg::launch_resid(kv, pa, s); });
         q.push_back([=] { return fcg::launch_reduce_norm(kv, pa, hb, s); });
     }
@@ -1144,18 +1190,18 @@ int engine_step(fcg_engine* E) {
         }
         BatchView bv;
         fcg::GatherPlan gp;
-        RCCHECK(make_batch(ctx, E->tabs, be, &bv, &gp));
+        RCCHECK(make_batch(ctx, C.tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32 : fcg::kMvp64;
         choose_variants(ctx, prec, be, gp);
-        HIPCHECK(ctx, E->pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
-        hipEvent_t* ev = E->pe_arn.events();
+        HIPCHECK(ctx, C.pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
+        hipEvent_t* ev = C.pe_arn.events();
         const int jac = arn[0]->jac ? 1 : 0;
         q.push_back([=] { return fcg::launch_mvp(bv, prec, jac, gp, s, ev); });
         KrylovView kv;
-        RCCHECK(make_krylov(ctx, E->tabs, ke, &kv));
-        RCCHECK(ensure_pbuf(ctx, E->partial_b, sizeof(double) * (size_t)kv.nchunks * std::max(kv.maxnv, 1)));
-        double* pb = E->partial_b.as<double>();
-        double* hb = E->hbuf.as<double>();
+        RCCHECK(make_krylov(ctx, C.tabs, ke, &kv));
+        RCCHECK(ensure_pbuf(ctx, C.partial_b, sizeof(double) * (size_t)kv.nchunks * std::max(kv.maxnv, 1)));
+        double* pb = C.partial_b.as<double>();
+        double* hb = C.hbuf.as<double>();
         q.push_back([=] { return fcg::launch_dots(kv, pb, s); });
         q.push_back([=] { return fcg::launch_reduce(kv, pb, hb, 0, s); });
         q.push_back([=] { return fcg::launch_update(kv, hb, 0, 0, pb, s); });
@@ -1169,19 +1215,42 @@ int engine_step(fcg_engine* E) {
     double* hh = nullptr;
     if (hoff > 0) {
         void* h;
-        HIPCHECK(ctx, E->tabs.host_area(sizeof(double) * (size_t)hoff, &h));
+        HIPCHECK(ctx, C.tabs.host_area(sizeof(double) * (size_t)hoff, &h));
         hh = reinterpret_cast<double*>(h);
-        const void* src = E->hbuf.p;
+        const void* src = C.hbuf.p;
         const size_t nb = sizeof(double) * (size_t)hoff;
         q.push_back([=] { return hipMemcpyAsync(hh, src, nb, hipMemcpyDeviceToHost, s); });
     }
-    HIPCHECK(ctx, E->tabs.flush(s));
+    HIPCHECK(ctx, C.tabs.flush(s));
     for (auto& f : q) HIPCHECK(ctx, f());
+    if (!C.ev) HIPCHECK(ctx, hipEventCreateWithFlags(&C.ev, hipEventDisableTiming));
+    HIPCHECK(ctx, hipEventRecord(C.ev, s));
+    C.inflight = true;
+    C.res = std::move(res);
+    C.arn = std::move(arn);
+    C.fin = std::move(fin);
+    C.fin_x = std::move(fin_x);
+    C.fin_host = fin_host;
+    C.hh = hh;
+    C.t_wait = clk::now();
+    return FCG_OK;
+}
+
+// Wait for cohort C's step in flight and do its bookkeeping.
+int engine_finish(fcg_engine* E, Cohort& C) {
+    fcg_ctx* ctx = E->ctx;
     const auto t_wait = clk::now();
-    HIPCHECK(ctx, hipStreamSynchronize(s));
+    HIPCHECK(ctx, hipEventSynchronize(C.ev));
+    C.inflight = false;
     const auto t_post = clk::now();
-    E->pe_res.collect(ctx->prof);
-    E->pe_arn.collect(ctx->prof);
+    C.pe_res.collect(ctx->prof);
+    C.pe_arn.collect(ctx->prof);
+    const auto& res = C.res;
+    const auto& arn = C.arn;
+    const auto& fin = C.fin;
+    const auto& fin_x = C.fin_x;
+    const double* fin_host = C.fin_host;
+    const double* hh = C.hh;
 
     // ---- host bookkeeping
     for (size_t f = 0; f < fin.size(); ++f) {
@@ -1191,7 +1260,7 @@ int engine_step(fcg_engine* E) {
         rd.job = S.job_id;
         rd.rows = S.k;
         rd.cols = S.ncol;
-        rd.cap.assign(fin[f].second, fin[f].second + (size_t)S.k * S.ncol);
+        rd.cap.assign(fin_host + fin[f].second, fin_host + fin[f].second + (size_t)S.k * S.ncol);
         if (fin_x[f]) rd.x.assign(fin_x[f], fin_x[f] + (size_t)S.k * S.N);
         std::memset(&rd.stats, 0, sizeof rd.stats);
         rd.stats.n_rhs = S.k;
@@ -1246,16 +1315,33 @@ int engine_step(fcg_engine* E) {
         }
     }
     // drop finished solves (their device memory returns to the pool)
-    E->active.erase(std::remove_if(E->active.begin(), E->active.end(),
-                                   [](const std::unique_ptr<Solve>& u) { return u->phase == kDone; }),
-                    E->active.end());
+    for (auto& u : C.active)
+        if (u->phase == kDone) C.load -= u->J->L * (int64_t)u->k;
+    C.active.erase(std::remove_if(C.active.begin(), C.active.end(),
+                                  [](const std::unique_ptr<Solve>& u) { return u->phase == kDone; }),
+                   C.active.end());
+    C.res.clear();
+    C.arn.clear();
+    C.fin.clear();
+    C.fin_x.clear();
     if (ctx->profiling) {
         const auto t_end = clk::now();
-        ctx->prof.steps += 1;
-        ctx->prof.host_plan_ms += std::chrono::duration<double, std::milli>(t_wait - t_plan).count();
+        ctx->prof.host_plan_ms += std::chrono::duration<double, std::milli>(C.t_wait - C.t_plan).count();
         ctx->prof.host_wait_ms += std::chrono::duration<double, std::milli>(t_post - t_wait).count();
         ctx->prof.host_post_ms += std::chrono::duration<double, std::milli>(t_end - t_post).count();
     }
+    return FCG_OK;
+}
+
+// One engine step: every cohort's step in flight is finished and its next one
+// queued, cohort after cohort, so the device runs one cohort while the host
+// serves the other.
+int engine_step(fcg_engine* E) {
+    for (Cohort& C : E->co) {
+        if (C.inflight) RCCHECK(engine_finish(E, C));
+        if (!C.active.empty()) RCCHECK(engine_launch(E, C));
+    }
+    if (E->ctx->profiling) E->ctx->prof.steps += 1;
     return FCG_OK;
 }
 
@@ -1788,8 +1874,10 @@ int fcg_engine_submit(fcg_engine* eng, int job, int64_t tag) {
 int fcg_engine_step(fcg_engine* eng, int* active, int* ready) {
     if (!eng) return FCG_INVALID_ARGUMENT;
     HIPCHECK(eng->ctx, hipSetDevice(eng->ctx->device));
-    if (!eng->active.empty()) RCCHECK(engine_step(eng));
-    if (active) *active = (int)eng->active.size();
+    bool any = false;
+    for (const Cohort& c : eng->co) any |= !c.active.empty() || c.inflight;
+    if (any) RCCHECK(engine_step(eng));
+    if (active) *active = (int)eng->n_active();
     if (ready) *ready = (int)eng->ready.size();
     return FCG_OK;
 }
@@ -1813,7 +1901,8 @@ int fcg_engine_poll(fcg_engine* eng, int64_t* tag, int* job, double* cap, int ca
 int fcg_engine_bytes(fcg_engine* eng, int64_t* bytes) {
     if (!eng || !bytes) return FCG_INVALID_ARGUMENT;
     int64_t b = 0;
-    for (auto& s : eng->active) b += s->bytes();
+    for (const Cohort& c : eng->co)
+        for (auto& s : c.active) b += s->bytes();
     *bytes = b;
     return FCG_OK;
 }

@@ -176,9 +176,17 @@ hipError_t launch_sub(const double* x, const double* y, double* out, size_t n, h
 hipError_t launch_jacobi(const double* q, const double* self, int N, int k, double* pq, hipStream_t s);
 // y[r][n] += sum_jj z[jj][r] Q_jj[r][n] for jj < nvec
 hipError_t launch_combine(double* const* Q, int nvec, const double* z, double* y, int N, int k, hipStream_t s);
-// out[r][c] = sum over leaves of conductor column c of w[n] x[r][n]
-hipError_t launch_charge_sums(const double* x, const double* w, const int32_t* col_range, int N, int k, int ncol,
-                              double* out, hipStream_t s);
+// Conductor charge sums of several finished solves in one launch: for each job,
+// out[off + r ncol + c] = sum over the leaves of conductor column c of w[n] x[r][n].
+struct ChargeSumJob {
+    const double* x;         // [k][N] solution
+    const double* w;         // [N] leaf weights
+    const int32_t* col_range;  // [2 ncol] leaf range of every conductor column
+    int N, k, ncol;
+    int64_t blk;  // first block of this job (ascending, first entry 0)
+    int64_t off;  // first output value of this job
+};
+hipError_t launch_charge_sums(const ChargeSumJob* jobs, int njobs, int64_t nblocks, double* out, hipStream_t s);
 // squared norms of the k segments of a [k][N] vector into out[k]
 hipError_t launch_sqnorms(const double* x, int N, int k, double* out, hipStream_t s);
 

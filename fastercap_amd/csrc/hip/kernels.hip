@@ -810,19 +810,27 @@ hipError_t launch_combine(double* const* Q, int nvec, const double* z, double* y
 }
 
 // grid (ncol, k)
-__global__ void k_charge_sums(const double* __restrict__ x, const double* __restrict__ wgt,
-                              const int32_t* __restrict__ col_range, int N, int k, int ncol, double* __restrict__ out) {
+// one block per (job, rhs, conductor column)
+__global__ void k_charge_sums(const ChargeSumJob* __restrict__ jobs, int njobs, double* __restrict__ out) {
     __shared__ double sh[4];
-    const int c = blockIdx.x, r = blockIdx.y;
+    const int64_t b = blockIdx.x;
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].blk <= b) lo = mid; else hi = mid - 1;
+    }
+    const ChargeSumJob& J = jobs[lo];
+    const int64_t loc = b - J.blk;
+    const int r = (int)(loc / J.ncol), c = (int)(loc % J.ncol);
     double s = 0.0;
-    for (int n = col_range[2 * c] + threadIdx.x; n < col_range[2 * c + 1]; n += blockDim.x)
-        s += wgt[n] * x[(size_t)r * N + n];
+    for (int n = J.col_range[2 * c] + threadIdx.x; n < J.col_range[2 * c + 1]; n += blockDim.x)
+        s += J.w[n] * J.x[(size_t)r * J.N + n];
     s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[(size_t)r * ncol + c] = s;
+    if (threadIdx.x == 0) out[J.off + loc] = s;
 }
-hipError_t launch_charge_sums(const double* x, const double* w, const int32_t* col_range, int N, int k, int ncol,
-                              double* out, hipStream_t s) {
-    k_charge_sums<<<dim3(ncol, k), 256, 0, s>>>(x, w, col_range, N, k, ncol, out);
+hipError_t launch_charge_sums(const ChargeSumJob* jobs, int njobs, int64_t nblocks, double* out, hipStream_t s) {
+    if (njobs == 0 || nblocks == 0) return hipSuccess;
+    k_charge_sums<<<(unsigned)nblocks, 256, 0, s>>>(jobs, njobs, out);
     return hipGetLastError();
 }
 
