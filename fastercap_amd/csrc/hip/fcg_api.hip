@@ -553,6 +553,9 @@ int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
 }
 
 constexpr int64_t kGatherRowChunk = 4096;
+// spare entries after a job's link index and coefficient arrays (room for
+// vector loads of the link streams; nothing reads them at present)
+constexpr int64_t kLinkSpare = 4;
 
 // Charge-row layout of a job: nodes grouped by tree depth (root first), post-order
 // within a depth, so the sources one row links to (mostly nodes of similar size
@@ -1402,6 +1405,7 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         std::vector<uint32_t> forder;
         build_fill_order(d, iperm, cols, forder);
         if (!forder.empty()) HIPCHECK(ctx, upload(j->fill_order, forder.data(), forder.size(), s));
+        cols.resize(cols.size() + kLinkSpare, 0);
         HIPCHECK(ctx, upload(j->col, cols.data(), cols.size(), s));
         j->h_row_ptr.assign(d->row_ptr, d->row_ptr + j->M + 1);
         HIPCHECK(ctx, hipStreamSynchronize(s));  // the host vectors above are released here
@@ -1498,9 +1502,9 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
         for (const auto& e : ents)
             if (e.j == j) return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
         if (precision == FCG_PREC_MIXED && !j->coef32.p)
-            HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)std::max<int64_t>(j->L, 1)));
+            HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)(j->L + kLinkSpare)));
         if (precision == FCG_PREC_FP64 && !j->coef64.p)
-            HIPCHECK(ctx, j->coef64.alloc(sizeof(double) * (size_t)std::max<int64_t>(j->L, 1)));
+            HIPCHECK(ctx, j->coef64.alloc(sizeof(double) * (size_t)(j->L + kLinkSpare)));
         ents.push_back(BatchEntry{j, &j->scr, nullptr, nullptr});
     }
     HIPCHECK(ctx, hipStreamSynchronize(s));

@@ -347,14 +347,21 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
 #pragma unroll
         for (int v = 0; v < VR; ++v) acc[v] = (TA)0;
         for (int64_t l = beg + slot; l < end; l += U * SL) {
+            // lanes past the row's last link skip the index and coefficient loads
+            // (a branch, measured faster than a select) and gather charge row 0
+            // with weight 0
             int c[U];
             TA w[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int64_t ll = l + (int64_t)u * SL;
-                const bool ok = ll < end;
-                c[u] = ok ? J.col[ll] : 0;
-                w[u] = ok ? (TA)coef[ll] : (TA)0;
+                if (ll < end) {
+                    c[u] = J.col[ll];
+                    w[u] = (TA)coef[ll];
+                } else {
+                    c[u] = 0;
+                    w[u] = (TA)0;
+                }
             }
             V ch[U];
 #pragma unroll
@@ -390,15 +397,13 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
 struct GatherCfg {
     int kr, vr, u, rpw;
 };
-// requested lane shapes (lanes per link, rhs per lane, links in flight per slot,
-// rows per wavefront); gather_shape fits them to the batch's row strides
+// requested lane shapes: lanes per link, rhs per lane, links in flight per slot,
+// rows per wavefront; gather_shape fits them to the batch's row strides
 static const GatherCfg kGatherCfgs[] = {
     {2, 4, 4, 1}, {1, 8, 4, 1}, {4, 2, 4, 1}, {1, 8, 2, 1}, {2, 4, 2, 1}, {8, 1, 4, 1}, {4, 2, 2, 1},
-    {1, 4, 4, 1}, {16, 1, 2, 1}, {2, 2, 4, 1},
-    {4, 8, 2, 1}, {8, 4, 2, 1}, {16, 2, 2, 1},  // wide shapes for many right-hand sides (k up to 32 per pass)
-    {2, 2, 4, 2}, {1, 4, 4, 2}, {2, 4, 4, 2}, {4, 2, 4, 2}, {1, 2, 4, 4}, {2, 2, 4, 4},  // short rows
-    {1, 2, 2, 4}, {1, 2, 4, 8}, {2, 2, 4, 8}, {1, 1, 4, 8}, {1, 4, 4, 8}, {1, 2, 2, 8}, {1, 2, 4, 16},
-    {1, 1, 4, 16}, {1, 2, 2, 16},
+    {1, 4, 4, 1}, {16, 1, 2, 1}, {2, 2, 4, 1}, {4, 8, 2, 1}, {8, 4, 2, 1}, {16, 2, 2, 1}, {2, 2, 4, 2},
+    {1, 4, 4, 2}, {2, 4, 4, 2}, {4, 2, 4, 2}, {1, 2, 4, 4}, {2, 2, 4, 4}, {1, 2, 2, 4}, {1, 2, 4, 8},
+    {2, 2, 4, 8}, {1, 1, 4, 8}, {1, 4, 4, 8}, {1, 2, 2, 8}, {1, 2, 4, 16}, {1, 1, 4, 16}, {1, 2, 2, 16},
 };
 constexpr int kNumGatherCfgs = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
 int gather_variant_count() { return kNumGatherCfgs; }
@@ -418,9 +423,10 @@ static GatherCfg gather_shape(int variant, int kpmin, int kmax) {
 }
 
 static int find_cfg(int kr, int vr, int u, int rpw) {
-    for (int i = 0; i < kNumGatherCfgs; ++i)
-        if (kGatherCfgs[i].kr == kr && kGatherCfgs[i].vr == vr && kGatherCfgs[i].u == u && kGatherCfgs[i].rpw == rpw)
-            return i;
+    for (int i = 0; i < kNumGatherCfgs; ++i) {
+        const GatherCfg& g = kGatherCfgs[i];
+        if (g.kr == kr && g.vr == vr && g.u == u && g.rpw == rpw) return i;
+    }
     return 0;
 }
 
@@ -448,27 +454,24 @@ static hipError_t gather_dispatch(const BatchView& bv, const GatherSeg& sg, hipS
     const int64_t per_xcd = (blocks + kNumXcd - 1) / kNumXcd;
     const unsigned grid = (unsigned)(per_xcd * kNumXcd);
     if (grid == 0) return hipSuccess;
-#define FCG_G(KR, VR, U, R)                                                           \
-    if (c.kr == KR && c.vr == VR && c.u == U && c.rpw == R) {                         \
+#define FCG_G(KR, VR, U, R)                                                                    \
+    if (c.kr == KR && c.vr == VR && c.u == U && c.rpw == R) {                                  \
         k_gather<TC, TQ, TA, KR, VR, U, R><<<grid, 256, 0, s>>>(bv, (int)per_xcd, sg.row_lo, sg.row_hi); \
-        return hipGetLastError();                                                     \
+        return hipGetLastError();                                                              \
     }
-#define FCG_GU(KR, VR) FCG_G(KR, VR, 2, 1) FCG_G(KR, VR, 4, 1)
-    FCG_GU(1, 1) FCG_GU(2, 1) FCG_GU(4, 1) FCG_GU(8, 1) FCG_GU(16, 1)
-    FCG_GU(1, 2) FCG_GU(2, 2) FCG_GU(4, 2) FCG_GU(8, 2)
-    FCG_GU(1, 4) FCG_GU(2, 4) FCG_GU(4, 4)
-    FCG_GU(1, 8) FCG_GU(2, 8) FCG_GU(4, 8)
-    FCG_GU(8, 4) FCG_GU(16, 2)
-    FCG_G(1, 1, 4, 2) FCG_G(2, 1, 4, 2) FCG_G(4, 1, 4, 2) FCG_G(8, 1, 4, 2)
-    FCG_G(1, 2, 4, 2) FCG_G(2, 2, 4, 2) FCG_G(4, 2, 4, 2)
-    FCG_G(1, 4, 4, 2) FCG_G(2, 4, 4, 2)
-    FCG_G(1, 1, 4, 4) FCG_G(2, 1, 4, 4) FCG_G(4, 1, 4, 4)
-    FCG_G(1, 2, 4, 4) FCG_G(2, 2, 4, 4)
-    FCG_G(1, 1, 2, 4) FCG_G(2, 1, 2, 4) FCG_G(1, 2, 2, 4) FCG_G(2, 2, 2, 4)
-    FCG_G(1, 1, 4, 8) FCG_G(2, 1, 4, 8) FCG_G(1, 2, 4, 8) FCG_G(2, 2, 4, 8) FCG_G(1, 4, 4, 8) FCG_G(2, 4, 4, 8)
-    FCG_G(1, 1, 2, 8) FCG_G(2, 1, 2, 8) FCG_G(1, 2, 2, 8)
-    FCG_G(1, 1, 4, 16) FCG_G(1, 2, 4, 16) FCG_G(1, 1, 2, 16) FCG_G(1, 2, 2, 16)
-#undef FCG_GU
+    // every shape gather_shape can return for the table above
+    FCG_G(1, 1, 2, 1) FCG_G(1, 1, 4, 1) FCG_G(1, 2, 2, 1) FCG_G(1, 2, 4, 1) FCG_G(1, 4, 2, 1)
+    FCG_G(1, 4, 4, 1) FCG_G(1, 8, 2, 1) FCG_G(1, 8, 4, 1) FCG_G(2, 1, 2, 1) FCG_G(2, 1, 4, 1)
+    FCG_G(2, 2, 2, 1) FCG_G(2, 2, 4, 1) FCG_G(2, 4, 2, 1) FCG_G(2, 4, 4, 1) FCG_G(2, 8, 2, 1)
+    FCG_G(4, 1, 2, 1) FCG_G(4, 1, 4, 1) FCG_G(4, 2, 2, 1) FCG_G(4, 2, 4, 1) FCG_G(4, 4, 2, 1)
+    FCG_G(4, 8, 2, 1) FCG_G(8, 1, 2, 1) FCG_G(8, 1, 4, 1) FCG_G(8, 2, 2, 1) FCG_G(8, 4, 2, 1)
+    FCG_G(16, 1, 2, 1) FCG_G(16, 2, 2, 1) FCG_G(1, 1, 4, 2) FCG_G(1, 2, 4, 2) FCG_G(1, 4, 4, 2)
+    FCG_G(2, 1, 4, 2) FCG_G(2, 2, 4, 2) FCG_G(2, 4, 4, 2) FCG_G(4, 1, 4, 2) FCG_G(4, 2, 4, 2)
+    FCG_G(8, 1, 4, 2) FCG_G(1, 1, 2, 4) FCG_G(1, 1, 4, 4) FCG_G(1, 2, 2, 4) FCG_G(1, 2, 4, 4)
+    FCG_G(2, 1, 2, 4) FCG_G(2, 1, 4, 4) FCG_G(2, 2, 4, 4) FCG_G(4, 1, 4, 4) FCG_G(1, 1, 2, 8)
+    FCG_G(1, 1, 4, 8) FCG_G(1, 2, 2, 8) FCG_G(1, 2, 4, 8) FCG_G(1, 4, 4, 8) FCG_G(2, 1, 2, 8)
+    FCG_G(2, 1, 4, 8) FCG_G(2, 2, 4, 8) FCG_G(1, 1, 2, 16) FCG_G(1, 1, 4, 16) FCG_G(1, 2, 2, 16)
+    FCG_G(1, 2, 4, 16)
 #undef FCG_G
     return hipErrorInvalidConfiguration;
 }
