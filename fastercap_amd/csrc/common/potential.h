@@ -114,6 +114,8 @@ FCG_HD T potential_opt_3d(Vec3<T> r, const float* v) {
     const Vec3<T> rho{r.x - n.x * nr, r.y - n.y * nr, r.z - n.z * nr};
     Vec3<T> Pmp[3], Rmp[3];
     T Rn[3];
+    // both edge loops are unrolled so the small arrays live in registers
+FCG_UNROLL
     for (int i = 0; i < 3; ++i) {
         const T nv = dot3(n, V[i]);
         const Vec3<T> rhomp{V[i].x - n.x * nv, V[i].y - n.y * nv, V[i].z - n.z * nv};
@@ -124,6 +126,7 @@ FCG_HD T potential_opt_3d(Vec3<T> r, const float* v) {
     const T d = -dot3(n, Rmp[0]);
     const T dnorm = fabs(d);
     T result = 0;
+FCG_UNROLL
     for (int i = 0; i < 3; ++i) {
         const int j = (i == 2) ? 0 : i + 1;
         const T sl = norm3(side[i]);

@@ -216,10 +216,11 @@ struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
-        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, rhs_range;
+        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
+    std::vector<int64_t> fill_cnt;  // links per fill class
     std::vector<uint32_t> link_pos;
     bool filled64 = false, filled32 = false;
     std::vector<int32_t> group_is_diel, group_leaf_start;
@@ -236,7 +237,7 @@ struct Job {
         return (int64_t)(nodes.bytes + left.bytes + right.bytes + parent.bytes + leaf_lo.bytes + leaf_hi.bytes +
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
-                         perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + rhs_range.bytes +
+                         perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
                          scr.bytes());
     }
 
@@ -257,6 +258,7 @@ struct Job {
         b.iperm = iperm.as<int32_t>();
         b.rorder = rorder.as<int32_t>();
         b.fill_order = fill_order.as<uint32_t>();
+        b.fill_start = fill_start.as<int64_t>();
         b.link_row = link_row.as<int32_t>();
         b.coef64 = coef64.as<double>();
         b.coef32 = coef32.as<float>();
@@ -375,7 +377,7 @@ struct BatchEntry {
 // range of batch rows; `plan` (optional) receives those ranges with no lane shape
 // chosen yet (see choose_variants).
 int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, BatchView* out,
-               fcg::GatherPlan* plan = nullptr) {
+               fcg::GatherPlan* plan = nullptr, std::vector<int>* order_out = nullptr) {
     std::memset(out, 0, sizeof *out);
     if (plan) plan->n = 0;
     const int n = (int)ents.size();
@@ -384,6 +386,7 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     for (int i = 0; i < n; ++i) order[(size_t)i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int a, int b) { return ents[(size_t)a].sc->kp < ents[(size_t)b].sc->kp; });
+    if (order_out) *order_out = order;
     std::vector<BJob> recs((size_t)n);
     std::vector<int64_t> off((size_t)5 * (n + 1), 0);
     int64_t* lo = off.data();
@@ -627,30 +630,32 @@ void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector
     for (auto& x : th) x.join();
 }
 
-// Branch of the link integral (PotEstimateOpt, Solver/Potential.cpp) a link
-// takes: the bracket of centroid distance over the larger side, dielectric rows
-// apart.  The fill evaluates the branch itself; this only schedules it.
-constexpr int kFillClasses = 10;
+// Fill class of a link (kernels.h): the branch of PotEstimateOpt
+// (Solver/Autorefine.cpp:3474-3733, 2D :4156-4325) from the centroid distance over
+// the larger max side, computed in FP64 in the reference's order of operations,
+// then whether the row is a dielectric panel, then the dimension.  The fill
+// kernels evaluate exactly this branch.
+using fcg::kFillClasses;
 int fill_class(const fcg_node_geom& a, const fcg_node_geom& b, int dim) {
     const double dx = (double)a.c[0] - (double)b.c[0], dy = (double)a.c[1] - (double)b.c[1];
-    const double dz = dim == 3 ? (double)a.c[2] - (double)b.c[2] : 0.0;
-    const double ratio = std::sqrt(dx * dx + dy * dy + dz * dz) / std::max(a.maxside, b.maxside);
-    int c;
-    if (dim == 3)
-        c = ratio < 1.0 ? 0 : ratio < 2.0 ? 1 : ratio < 4.0 ? 2 : ratio < 10.0 ? 3 : 4;
-    else
-        c = ratio < 4.0 ? 0 : 1;
-    return (a.flags & FCG_NODE_DIEL) ? c + 5 : c;
+    const double rmax = std::max(a.maxside, b.maxside);
+    const int diel = (a.flags & FCG_NODE_DIEL) ? 1 : 0;
+    if (dim == 2) return 10 + 2 * diel + (std::sqrt(dx * dx + dy * dy) / rmax < 4.0 ? 0 : 1);
+    const double dz = (double)a.c[2] - (double)b.c[2];
+    const double ratio = std::sqrt(dx * dx + dy * dy + dz * dz) / rmax;
+    const int br = ratio < 1.0 ? 0 : ratio < 2.0 ? 1 : ratio < 4.0 ? 2 : ratio < 10.0 ? 3 : 4;
+    return 5 * diel + br;
 }
 
 // Fill order of a job's stored links: grouped by fill_class (stable, so links of
-// one class keep their row order), so the lanes of a wavefront mostly take the
-// same branch instead of every wavefront running each branch its lanes need.
+// one class keep their row order); cnt_out receives the links per class.
 void build_fill_order(const fcg_job_desc* d, const std::vector<int32_t>& iperm,
-                      const std::vector<int32_t>& col_sorted, std::vector<uint32_t>& order) {
+                      const std::vector<int32_t>& col_sorted, std::vector<uint32_t>& order,
+                      std::vector<int64_t>& cnt_out) {
     const int M = d->n_nodes;
     const int64_t L = d->n_links;
     order.assign((size_t)L, 0);
+    cnt_out.assign(kFillClasses, 0);
     if (L == 0) return;
     const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     std::vector<uint8_t> cls((size_t)L);
@@ -682,6 +687,7 @@ void build_fill_order(const fcg_job_desc* d, const std::vector<int32_t>& iperm,
         for (int t = 0; t < nth; ++t) {
             off[(size_t)t * kFillClasses + c] = run;
             run += cnt[(size_t)t * kFillClasses + c];
+            cnt_out[(size_t)c] += cnt[(size_t)t * kFillClasses + c];
         }
     for (int t = 0; t < nth; ++t) {
         th.emplace_back([&, t] {
@@ -1403,8 +1409,11 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
         HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
         std::vector<uint32_t> forder;
-        build_fill_order(d, iperm, cols, forder);
+        build_fill_order(d, iperm, cols, forder, j->fill_cnt);
         if (!forder.empty()) HIPCHECK(ctx, upload(j->fill_order, forder.data(), forder.size(), s));
+        std::vector<int64_t> fstart((size_t)kFillClasses + 1, 0);
+        for (int c = 0; c < kFillClasses; ++c) fstart[(size_t)c + 1] = fstart[(size_t)c] + j->fill_cnt[(size_t)c];
+        HIPCHECK(ctx, upload(j->fill_start, fstart.data(), fstart.size(), s));
         cols.resize(cols.size() + kLinkSpare, 0);
         HIPCHECK(ctx, upload(j->col, cols.data(), cols.size(), s));
         j->h_row_ptr.assign(d->row_ptr, d->row_ptr + j->M + 1);
@@ -1510,8 +1519,21 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->tabs.reset();
     BatchView bv;
-    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv));
-    HIPCHECK(ctx, fcg::launch_fill(bv, precision, s));
+    std::vector<int> order;
+    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv, nullptr, &order));
+    // per class, the prefix of the jobs' link counts in the batch's job order
+    const int64_t* offp[kFillClasses];
+    int64_t total[kFillClasses];
+    for (int c = 0; c < kFillClasses; ++c) {
+        std::vector<int64_t> off(order.size() + 1, 0);
+        for (size_t i = 0; i < order.size(); ++i)
+            off[i + 1] = off[i] + ents[(size_t)order[i]].j->fill_cnt[(size_t)c];
+        total[c] = off.back();
+        void* d;
+        HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
+        offp[c] = reinterpret_cast<const int64_t*>(d);
+    }
+    HIPCHECK(ctx, fcg::launch_fill(bv, offp, total, precision, s));
     HIPCHECK(ctx, hipStreamSynchronize(s));
     for (auto& e : ents) {
         if (precision == FCG_PREC_MIXED)

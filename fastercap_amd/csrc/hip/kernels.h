@@ -43,7 +43,8 @@ struct BJob {
     const int32_t* perm;      // [M] node -> slot of its charge / potential row
     const int32_t* iperm;     // [M] slot -> node
     const int32_t* rorder;    // [M] gather row order -> node
-    const uint32_t* fill_order;  // [L] links grouped by integral branch
+    const uint32_t* fill_order;  // [L] links grouped by fill class
+    const int64_t* fill_start;   // [kFillClasses + 1] first fill_order entry of each class
     const int32_t* link_row;  // [L] destination node of each link (fill only)
     double* coef64;           // [L] or null
     float* coef32;            // [L] or null
@@ -112,9 +113,19 @@ struct KrylovView {
     int nsegs, nchunks, maxnv;
 };
 
-// fill of every link and self term of the batch; mixed stores FP32 coefficients
+// Fill classes: which integral branch of PotEstimateOpt a link takes (brackets of
+// centroid distance over the larger max side), on a dielectric row or not, 3D or
+// 2D.  3D: 5 diel + branch (0 analytic, 1 rule 5, 2 rule 3, 3 rule 2, 4 point);
+// 2D: 10 + 2 diel + (0 analytic, 1 point).  Each class is filled by a kernel of
+// its own.
+constexpr int kFillClasses = 14;
+
+// fill of every link and self term of the batch; mixed stores FP32 coefficients.
+// off[c] (device, njobs + 1 entries) is the prefix over the batch's jobs of their
+// class-c link counts, total[c] its last entry.
 hipError_t launch_link_rows(const BatchView& b, hipStream_t s);
-hipError_t launch_fill(const BatchView& b, int precision, hipStream_t s);
+hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int64_t* total, int precision,
+                       hipStream_t s);
 
 // Rhs classes of the gather: jobs whose node-major row stride is kp = 2^c share
 // class c (strides beyond the last class share it).  A batch lists its jobs

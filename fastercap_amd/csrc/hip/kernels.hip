@@ -3,8 +3,8 @@
 //  fill        one link per lane: PotEstimateOpt(row node, source node)
 //              (Solver/Autorefine.cpp:3474-3733, 2D :4156-4325); self terms
 //              (SelfPotential, Solver/Autorefine.cpp:2358-2506).  The mixed mode
-//              evaluates the far-field branches (centroid distance >= 4 max sides)
-//              in FP32 and the near-field integrals in FP64, storing FP32.
+//              evaluates the analytic near-field integrals in FP64 and the
+//              quadrature and point branches in FP32, storing FP32.
 //  product     v = P q (Solver/MultiplyHierarchical.cpp:72-330) in four stages:
 //              1. double-double prefix scan of the (Jacobi-scaled) input over the
 //                 DFS leaf order; every node's charge is the difference of two
@@ -74,56 +74,82 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 }
 
 // ------------------------------------------------------------------ fill
-template <typename T>
-__device__ T pot_estimate(const fcg_node_geom& p1, const fcg_node_geom& p2, int dim) {
-    const bool diel1 = (p1.flags & FCG_NODE_DIEL) != 0;
-    const T rmax = (T)(p1.maxside > p2.maxside ? p1.maxside : p2.maxside);
-    if (dim == 2) {
+// Link coefficient of PotEstimateOpt (Solver/Autorefine.cpp:3474-3733, 2D
+// :4156-4325) for one fill class: branch BR (0 analytic, 1 rule 5, 2 rule 3,
+// 3 rule 2, 4 point), evaluated in T.  The branch itself is chosen at upload on
+// the FP64 distance ratio, exactly as the reference computes it.
+enum FillBranch { kBrAnalytic = 0, kBrRule5 = 1, kBrRule3 = 2, kBrRule2 = 3, kBrPoint = 4 };
+
+template <typename T, int DIM, bool DIEL, int BR>
+__device__ __forceinline__ T link_coef(const fcg_node_geom& p1, const fcg_node_geom& p2) {
+    if constexpr (DIM == 2) {
         const T dx = (T)p1.c[0] - (T)p2.c[0], dy = (T)p1.c[1] - (T)p2.c[1];
-        const T rd = sqrt(dx * dx + dy * dy);
-        if (!diel1) {
-            if (rd / rmax < (T)4.0) return potential_opt_2d<T>((T)p1.c[0], (T)p1.c[1], p2.v);
-            return -log(rd);
+        if constexpr (!DIEL) {
+            if constexpr (BR == kBrAnalytic)
+                return potential_opt_2d<T>((T)p1.c[0], (T)p1.c[1], p2.v);
+            else
+                return -log(sqrt(dx * dx + dy * dy));
+        } else {
+            if constexpr (BR == kBrAnalytic) {
+                return enfield_2d<T>((T)p1.c[0], (T)p1.c[1], p2.v, (T)p1.dn[0], (T)p1.dn[1]);
+            } else {
+                const T rd = sqrt(dx * dx + dy * dy);
+                return (dx * (T)p1.dn[0] + dy * (T)p1.dn[1]) / (rd * rd);
+            }
         }
-        if (rd / rmax < (T)4.0) return enfield_2d<T>((T)p1.c[0], (T)p1.c[1], p2.v, (T)p1.dn[0], (T)p1.dn[1]);
-        return (dx * (T)p1.dn[0] + dy * (T)p1.dn[1]) / (rd * rd);
+    } else {
+        const Vec3<T> c1{(T)p1.c[0], (T)p1.c[1], (T)p1.c[2]};
+        const Vec3<T> dist{c1.x - (T)p2.c[0], c1.y - (T)p2.c[1], c1.z - (T)p2.c[2]};
+        const T k4pe0 = (T)FCG_FOUR_PI_TIMES_E0;
+        if constexpr (!DIEL) {
+            if constexpr (BR == kBrAnalytic) return potential_opt_3d<T>(c1, p2.v) / k4pe0;
+            if constexpr (BR == kBrRule5) return potential_quad_3d<T, 5>(c1, p2.v) / k4pe0;
+            if constexpr (BR == kBrRule3) return potential_quad_3d<T, 3>(c1, p2.v) / k4pe0;
+            if constexpr (BR == kBrRule2) return potential_quad_3d<T, 2>(c1, p2.v) / k4pe0;
+            if constexpr (BR == kBrPoint) return (T)1 / (k4pe0 * norm3(dist));
+        } else {
+            const Vec3<T> n1{(T)p1.dn[0], (T)p1.dn[1], (T)p1.dn[2]};
+            if constexpr (BR == kBrAnalytic) {
+                // divided difference of the potential along the normal (h = rd / 20)
+                const T h = norm3(dist) / (T)20;
+                const Vec3<T> ep{c1.x + n1.x * h, c1.y + n1.y * h, c1.z + n1.z * h};
+                const Vec3<T> em{c1.x - n1.x * h, c1.y - n1.y * h, c1.z - n1.z * h};
+                const T pp = potential_opt_3d<T>(ep, p2.v);
+                const T pn = potential_opt_3d<T>(em, p2.v);
+                return (pn - pp) / ((T)2 * h * k4pe0);
+            }
+            if constexpr (BR == kBrRule5) return enfield_quad_3d<T, 5>(c1, p2.v, n1) / k4pe0;
+            if constexpr (BR == kBrRule3) return enfield_quad_3d<T, 3>(c1, p2.v, n1) / k4pe0;
+            if constexpr (BR == kBrRule2) return enfield_quad_3d<T, 2>(c1, p2.v, n1) / k4pe0;
+            if constexpr (BR == kBrPoint) {
+                const T rd = norm3(dist);
+                return dot3(dist, n1) / (k4pe0 * rd * rd * rd);
+            }
+        }
     }
-    const Vec3<T> c1{(T)p1.c[0], (T)p1.c[1], (T)p1.c[2]};
-    const Vec3<T> dist{c1.x - (T)p2.c[0], c1.y - (T)p2.c[1], c1.z - (T)p2.c[2]};
-    const T rd = norm3(dist);
-    const T ratio = rd / rmax;
-    const T k4pe0 = (T)FCG_FOUR_PI_TIMES_E0;
-    if (!diel1) {
-        if (ratio < (T)1.0) return potential_opt_3d<T>(c1, p2.v) / k4pe0;
-        if (ratio < (T)2.0) return potential_quad_3d<T, 5>(c1, p2.v) / k4pe0;
-        if (ratio < (T)4.0) return potential_quad_3d<T, 3>(c1, p2.v) / k4pe0;
-        if (ratio < (T)10.0) return potential_quad_3d<T, 2>(c1, p2.v) / k4pe0;
-        return (T)1 / (k4pe0 * rd);
-    }
-    const Vec3<T> n1{(T)p1.dn[0], (T)p1.dn[1], (T)p1.dn[2]};
-    if (ratio < (T)1.0) {
-        const T h = rd / (T)20;
-        const Vec3<T> ep{c1.x + n1.x * h, c1.y + n1.y * h, c1.z + n1.z * h};
-        const Vec3<T> em{c1.x - n1.x * h, c1.y - n1.y * h, c1.z - n1.z * h};
-        const T pp = potential_opt_3d<T>(ep, p2.v);
-        const T pn = potential_opt_3d<T>(em, p2.v);
-        return (pn - pp) / ((T)2 * h * k4pe0);
-    }
-    if (ratio < (T)2.0) return enfield_quad_3d<T, 5>(c1, p2.v, n1) / k4pe0;
-    if (ratio < (T)4.0) return enfield_quad_3d<T, 3>(c1, p2.v, n1) / k4pe0;
-    if (ratio < (T)10.0) return enfield_quad_3d<T, 2>(c1, p2.v, n1) / k4pe0;
-    return dot3(dist, n1) / (k4pe0 * rd * rd * rd);
 }
 
-// Near field (centroid distance below 4 max sides) keeps the FP64 integrals;
-// the far-field quadrature and point formulas run in FP32.
-__device__ float pot_estimate_mixed(const fcg_node_geom& p1, const fcg_node_geom& p2, int dim) {
-    const double rmax = p1.maxside > p2.maxside ? p1.maxside : p2.maxside;
-    const double dx = (double)p1.c[0] - (double)p2.c[0], dy = (double)p1.c[1] - (double)p2.c[1];
-    const double dz = dim == 3 ? (double)p1.c[2] - (double)p2.c[2] : 0.0;
-    const double ratio = sqrt(dx * dx + dy * dy + dz * dz) / rmax;
-    if (ratio < 4.0) return (float)pot_estimate<double>(p1, p2, dim);
-    return pot_estimate<float>(p1, p2, dim);
+// One fill class over the batch.  Mixed: the analytic near-field integrals, whose
+// log and atan terms cancel, stay in FP64; the quadrature and point branches run
+// in FP32.  Every mixed coefficient is stored in FP32, and that rounding bounds
+// all branches alike.
+template <int DIM, bool DIEL, int BR, bool MIXED>
+__global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __restrict__ off, int64_t total, int cls) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= total) return;
+    const int jb = find_job(off, bv.njobs, g);
+    const BJob& J = bv.jobs[jb];
+    const int64_t l = J.fill_order[J.fill_start[cls] + (g - off[jb])];
+    const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
+    const fcg_node_geom& p2 = J.nodes[J.iperm[J.col[l]]];
+    if constexpr (MIXED) {
+        if constexpr (BR == kBrAnalytic)
+            J.coef32[l] = (float)link_coef<double, DIM, DIEL, BR>(p1, p2);
+        else
+            J.coef32[l] = link_coef<float, DIM, DIEL, BR>(p1, p2);
+    } else {
+        J.coef64[l] = link_coef<double, DIM, DIEL, BR>(p1, p2);
+    }
 }
 
 __global__ void k_link_rows(BatchView bv) {
@@ -134,20 +160,6 @@ __global__ void k_link_rows(BatchView bv) {
     const int i = (int)(g - bv.node_off[jb]);
     int32_t* lr = const_cast<int32_t*>(J.link_row);
     for (int64_t l = J.row_ptr[i]; l < J.row_ptr[i + 1]; ++l) lr[l] = i;
-}
-
-__global__ void __launch_bounds__(256) k_fill(BatchView bv, int mixed) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= bv.links) return;
-    const int jb = find_job(bv.link_off, bv.njobs, g);
-    const BJob& J = bv.jobs[jb];
-    const int64_t l = J.fill_order[g - bv.link_off[jb]];
-    const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
-    const fcg_node_geom& p2 = J.nodes[J.iperm[J.col[l]]];
-    if (mixed)
-        J.coef32[l] = pot_estimate_mixed(p1, p2, J.dim);
-    else
-        J.coef64[l] = pot_estimate<double>(p1, p2, J.dim);
 }
 
 __global__ void k_self(BatchView bv) {
@@ -538,8 +550,29 @@ hipError_t launch_link_rows(const BatchView& b, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_fill(const BatchView& b, int precision, hipStream_t s) {
-    if (b.links > 0) k_fill<<<blocks_for((size_t)b.links, 256), 256, 0, s>>>(b, precision == FCG_PREC_MIXED);
+template <bool MIXED>
+static void fill_class(const BatchView& b, const int64_t* off, int64_t n, int c, hipStream_t s) {
+    const unsigned grid = blocks_for((size_t)n, 256);
+#define FCG_F(C, DIM, DIEL, BR) \
+    case C: k_fill<DIM, DIEL, BR, MIXED><<<grid, 256, 0, s>>>(b, off, n, c); break;
+    switch (c) {
+        FCG_F(0, 3, false, 0) FCG_F(1, 3, false, 1) FCG_F(2, 3, false, 2) FCG_F(3, 3, false, 3) FCG_F(4, 3, false, 4)
+        FCG_F(5, 3, true, 0) FCG_F(6, 3, true, 1) FCG_F(7, 3, true, 2) FCG_F(8, 3, true, 3) FCG_F(9, 3, true, 4)
+        FCG_F(10, 2, false, 0) FCG_F(11, 2, false, 4) FCG_F(12, 2, true, 0) FCG_F(13, 2, true, 4)
+        default: break;
+    }
+#undef FCG_F
+}
+
+hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int64_t* total, int precision,
+                       hipStream_t s) {
+    for (int c = 0; c < kFillClasses; ++c) {
+        if (total[c] <= 0) continue;
+        if (precision == FCG_PREC_MIXED)
+            fill_class<true>(b, off[c], total[c], c, s);
+        else
+            fill_class<false>(b, off[c], total[c], c, s);
+    }
     if (b.leaves > 0) k_self<<<blocks_for((size_t)b.leaves, 256), 256, 0, s>>>(b);
     return hipGetLastError();
 }
