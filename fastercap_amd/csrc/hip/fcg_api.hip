@@ -216,7 +216,7 @@ struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
-        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range;
+        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -238,6 +238,7 @@ struct Job {
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
+                         col16.bytes +
                          scr.bytes());
     }
 
@@ -254,6 +255,7 @@ struct Job {
         b.nodes = nodes.as<fcg_node_geom>();
         b.row_ptr = row_ptr.as<int64_t>();
         b.col = col.as<int32_t>();
+        b.col16 = col16.as<uint16_t>();
         b.perm = perm.as<int32_t>();
         b.iperm = iperm.as<int32_t>();
         b.rorder = rorder.as<int32_t>();
@@ -384,8 +386,12 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     if (n == 0) return FCG_OK;
     std::vector<int> order((size_t)n);
     for (int i = 0; i < n; ++i) order[(size_t)i] = i;
-    std::stable_sort(order.begin(), order.end(),
-                     [&](int a, int b) { return ents[(size_t)a].sc->kp < ents[(size_t)b].sc->kp; });
+    // by row stride, then 16-bit before 32-bit source indices
+    auto wide = [&](int i) { return ents[(size_t)i].j->col16.p ? 0 : 1; };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+        const int ka = ents[(size_t)a].sc->kp, kb = ents[(size_t)b].sc->kp;
+        return ka != kb ? ka < kb : wide(a) < wide(b);
+    });
     if (order_out) *order_out = order;
     std::vector<BJob> recs((size_t)n);
     std::vector<int64_t> off((size_t)5 * (n + 1), 0);
@@ -410,13 +416,15 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
         any2d |= (J.dim == 2);
         if (plan) {
             const int cls = fcg::gather_class(std::max(sc.kp, 1));
-            if (plan->n == 0 || plan->seg[plan->n - 1].cls != cls) {
+            const int i16 = J.col16.p ? 1 : 0;
+            if (plan->n == 0 || plan->seg[plan->n - 1].cls != cls || plan->seg[plan->n - 1].idx16 != i16) {
                 fcg::GatherSeg& g = plan->seg[plan->n++];
                 g.row_lo = no[i];
                 g.kpmin = std::max(sc.kp, 1);
                 g.kmax = 0;
                 g.cls = cls;
                 g.variant = -1;
+                g.idx16 = i16;
             }
             fcg::GatherSeg& g = plan->seg[plan->n - 1];
             g.row_hi = no[i + 1];
@@ -464,8 +472,10 @@ double gather_alg_bytes(const std::vector<BatchEntry>& ents, int prec) {
     const double bc = prec == fcg::kMvp64 ? 8.0 : 4.0;
     const double bn = prec == fcg::kMvp32 ? 4.0 : 8.0;
     double b = 0.0;
-    for (const auto& e : ents)
-        b += (double)e.j->L * (bc + 4.0) + 16.0 * e.j->M + 8.0 + 2.0 * e.j->M * (double)e.sc->k * bn;
+    for (const auto& e : ents) {
+        const double bi = e.j->col16.p ? 2.0 : 4.0;  // source index width
+        b += (double)e.j->L * (bc + bi) + 16.0 * e.j->M + 8.0 + 2.0 * e.j->M * (double)e.sc->k * bn;
+    }
     return b;
 }
 
@@ -1416,6 +1426,12 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         HIPCHECK(ctx, upload(j->fill_start, fstart.data(), fstart.size(), s));
         cols.resize(cols.size() + kLinkSpare, 0);
         HIPCHECK(ctx, upload(j->col, cols.data(), cols.size(), s));
+        if (j->M <= 65536) {
+            std::vector<uint16_t> c16(cols.size());
+            for (size_t l = 0; l < cols.size(); ++l) c16[l] = (uint16_t)cols[l];
+            HIPCHECK(ctx, upload(j->col16, c16.data(), c16.size(), s));
+            HIPCHECK(ctx, hipStreamSynchronize(s));  // c16 is released here
+        }
         j->h_row_ptr.assign(d->row_ptr, d->row_ptr + j->M + 1);
         HIPCHECK(ctx, hipStreamSynchronize(s));  // the host vectors above are released here
     }

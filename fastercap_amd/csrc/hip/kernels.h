@@ -40,6 +40,7 @@ struct BJob {
     const fcg_node_geom* nodes;
     const int64_t* row_ptr;
     const int32_t* col;       // [L] source slot (permuted node index), ascending within a row
+    const uint16_t* col16;    // the same in 16 bits when M <= 65536, else null
     const int32_t* perm;      // [M] node -> slot of its charge / potential row
     const int32_t* iperm;     // [M] slot -> node
     const int32_t* rorder;    // [M] gather row order -> node
@@ -129,8 +130,9 @@ hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int6
 
 // Rhs classes of the gather: jobs whose node-major row stride is kp = 2^c share
 // class c (strides beyond the last class share it).  A batch lists its jobs
-// ordered by stride, so each class owns one contiguous range of batch rows and
-// the gather runs one launch per class, each with its own lane shape.
+// ordered by stride and then by source index width, so each (class, width) owns
+// one contiguous range of batch rows and the gather runs one launch per range,
+// each with its own lane shape and index width.
 constexpr int kGatherClasses = 12;
 inline int gather_class(int kp) {
     int c = 0;
@@ -141,10 +143,11 @@ struct GatherSeg {
     int64_t row_lo, row_hi;  // batch rows [row_lo, row_hi)
     int kpmin, kmax;         // smallest row stride and largest rhs count in the range
     int cls, variant;        // rhs class and gather lane shape
+    int idx16;               // every job of the range has 16-bit source indices
 };
 struct GatherPlan {
     int n = 0;
-    GatherSeg seg[kGatherClasses];
+    GatherSeg seg[2 * kGatherClasses];
 };
 
 // v = P q on every job of the batch (operands in BJob::q / BJob::v).  jacobi

@@ -328,7 +328,7 @@ struct alignas(sizeof(T) * VR) VecT {
     T v[VR];
 };
 
-template <typename TC, typename TQ, typename TA, int KR, int VR, int U, int RPW>
+template <typename TC, typename TQ, typename TA, typename TI, int KR, int VR, int U, int RPW>
 __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd, int64_t row_lo, int64_t row_hi) {
     constexpr int G = 64 / RPW;  // lanes per destination row
     constexpr int SL = G / KR;
@@ -350,6 +350,9 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
     const TQ* charge =
         (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.charge64) : reinterpret_cast<const TQ*>(J.charge32);
     TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.pot64) : reinterpret_cast<TQ*>(J.pot32);
+    // source indices: 16 bits for jobs of at most 65536 nodes (a quarter of the
+    // FP32 link stream), 32 bits otherwise
+    const TI* cix = (sizeof(TI) == 2) ? reinterpret_cast<const TI*>(J.col16) : reinterpret_cast<const TI*>(J.col);
     const int64_t beg = J.row_ptr[row], end = J.row_ptr[row + 1];
     const TA sf = (TA)J.diag[row];
     const size_t orow = (size_t)J.perm[row] * kp;
@@ -368,7 +371,7 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
             for (int u = 0; u < U; ++u) {
                 const int64_t ll = l + (int64_t)u * SL;
                 if (ll < end) {
-                    c[u] = J.col[ll];
+                    c[u] = (int)cix[ll];
                     w[u] = (TA)coef[ll];
                 } else {
                     c[u] = 0;
@@ -468,7 +471,10 @@ static hipError_t gather_dispatch(const BatchView& bv, const GatherSeg& sg, hipS
     if (grid == 0) return hipSuccess;
 #define FCG_G(KR, VR, U, R)                                                                    \
     if (c.kr == KR && c.vr == VR && c.u == U && c.rpw == R) {                                  \
-        k_gather<TC, TQ, TA, KR, VR, U, R><<<grid, 256, 0, s>>>(bv, (int)per_xcd, sg.row_lo, sg.row_hi); \
+        if (sg.idx16)                                                                          \
+            k_gather<TC, TQ, TA, uint16_t, KR, VR, U, R><<<grid, 256, 0, s>>>(bv, (int)per_xcd, sg.row_lo, sg.row_hi); \
+        else                                                                                   \
+            k_gather<TC, TQ, TA, int32_t, KR, VR, U, R><<<grid, 256, 0, s>>>(bv, (int)per_xcd, sg.row_lo, sg.row_hi); \
         return hipGetLastError();                                                              \
     }
     // every shape gather_shape can return for the table above

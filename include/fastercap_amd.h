@@ -185,9 +185,10 @@ int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* o
  * on (HIP events around the gather of every product of the solve engine; a
  * product's gather is one kernel launch per rhs class of its jobs, counted in
  * gather_kernels).  Algorithmic bytes of one product's gather = sum over its jobs
- * of L (b_coef + 4) + 16 M + 8 + 2 M k b_node (b_coef, b_node = 8 for FP64 and 4
- * for FP32 coefficients / node values).  No reference counterpart (a measurement
- * hook). */
+ * of L (b_coef + b_idx) + 16 M + 8 + 2 M k b_node (b_coef, b_node = 8 for FP64
+ * and 4 for FP32 coefficients / node values; b_idx = 2 for jobs of at most 65536
+ * nodes, whose source indices are 16-bit, else 4).  No reference counterpart (a
+ * measurement hook). */
 typedef struct fcg_profile {
     double gather_ms[3];
     double gather_bytes[3];
