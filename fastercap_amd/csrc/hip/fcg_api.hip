@@ -1549,8 +1549,9 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
         HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
         offp[c] = reinterpret_cast<const int64_t*>(d);
     }
+    // no wait: later work on the context's stream runs after the fill, and every
+    // reuse of the context's tables waits for the stream first
     HIPCHECK(ctx, fcg::launch_fill(bv, offp, total, precision, s));
-    HIPCHECK(ctx, hipStreamSynchronize(s));
     for (auto& e : ents) {
         if (precision == FCG_PREC_MIXED)
             e.j->filled32 = true;
