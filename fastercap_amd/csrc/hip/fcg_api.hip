@@ -242,7 +242,7 @@ struct Job {
                          scr.bytes());
     }
 
-    BJob rec(const Scratch& sc, const double* q, double* v) const {
+    BJob rec(const Scratch& sc, const double* q, double* v, const float* q32 = nullptr) const {
         BJob b{};
         b.N = N;
         b.M = M;
@@ -281,6 +281,7 @@ struct Job {
         b.pot32 = sc.pot32.as<float>();
         b.lt = sc.lt.as<double>();
         b.q = q;
+        b.q32 = q32;
         b.v = v;
         return b;
     }
@@ -374,6 +375,7 @@ struct BatchEntry {
     const Scratch* sc;
     const double* q;
     double* v;
+    const float* q32 = nullptr;  // FP32 input instead of q
 };
 // Jobs are listed by row stride (stable), so every rhs class owns one contiguous
 // range of batch rows; `plan` (optional) receives those ranges with no lane shape
@@ -405,7 +407,7 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
         const BatchEntry& e = ents[(size_t)order[(size_t)i]];
         const Job& J = *e.j;
         const Scratch& sc = *e.sc;
-        recs[(size_t)i] = J.rec(sc, e.q, e.v);
+        recs[(size_t)i] = J.rec(sc, e.q, e.v, e.q32);
         lo[i + 1] = lo[i] + J.L;
         to[i + 1] = to[i] + (int64_t)sc.k * sc.nblk;
         no[i + 1] = no[i] + J.M;
@@ -744,7 +746,8 @@ struct Solve {
     Scratch scr;
     // Krylov basis: blocks of vectors and their addresses (sent with each step's tables)
     std::vector<std::unique_ptr<PBuf>> qblocks;
-    std::vector<double*> hq;
+    std::vector<void*> hq;
+    bool b32 = false;  // FP32 basis vectors (inner solves of the mixed path)
     int phase = kInit;
     int i = 0;  // Arnoldi step index within the cycle (the next product uses Q[i])
     int cycles = 0;
@@ -835,17 +838,18 @@ int ensure_pbuf(fcg_ctx* ctx, PBuf& b, size_t bytes) {
 // make sure basis vector index `idx` exists (device memory from the pool only)
 int ensure_basis(fcg_ctx* ctx, Solve& S, int idx) {
     const size_t vec = (size_t)S.k * S.N;
+    const size_t elem = S.b32 ? sizeof(float) : sizeof(double);
     while ((int)S.hq.size() <= idx) {
         // blocks of about 64 MiB, 4 to 64 vectors
         const size_t per = std::max<size_t>(4, std::min<size_t>(64, ((size_t)64 << 20) / (vec * 8 + 1)));
         S.qblocks.emplace_back(new PBuf());
-        HIPCHECK(ctx, S.qblocks.back()->get(&ctx->pool, sizeof(double) * vec * per));
-        for (size_t c = 0; c < per; ++c) S.hq.push_back(S.qblocks.back()->as<double>() + c * vec);
+        HIPCHECK(ctx, S.qblocks.back()->get(&ctx->pool, elem * vec * per));
+        for (size_t c = 0; c < per; ++c) S.hq.push_back(S.qblocks.back()->as<char>() + c * vec * elem);
     }
     return FCG_OK;
 }
 
-KJob kjob_of(Solve& S, double* const* qtab, double* w, int nv, int zoff) {
+KJob kjob_of(Solve& S, void* const* qtab, double* w, int nv, int zoff) {
     KJob J{};
     J.Q = qtab;
     J.w = w;
@@ -877,7 +881,7 @@ int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, Krylo
     std::vector<Chunk> chunks;
     int maxnv = 0;
     // basis address tables of every entry, back to back
-    std::vector<double*> qaddr;
+    std::vector<void*> qaddr;
     std::vector<size_t> qoff;
     for (size_t e = 0; e < ents.size(); ++e) {
         Solve& S = *ents[e].S;
@@ -886,8 +890,8 @@ int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, Krylo
         qaddr.insert(qaddr.end(), S.hq.begin(), S.hq.begin() + need);
     }
     void* qd = nullptr;
-    HIPCHECK(ctx, tabs.push(qaddr.data(), sizeof(double*) * qaddr.size(), ctx->stream, &qd));
-    double* const* qbase = reinterpret_cast<double* const*>(qd);
+    HIPCHECK(ctx, tabs.push(qaddr.data(), sizeof(void*) * qaddr.size(), ctx->stream, &qd));
+    void* const* qbase = reinterpret_cast<void* const*>(qd);
     for (size_t e = 0; e < ents.size(); ++e) {
         Solve& S = *ents[e].S;
         kj.push_back(kjob_of(S, qbase + qoff[e], ents[e].w, ents[e].nv, ents[e].zoff));
@@ -913,6 +917,7 @@ int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, Krylo
     kv->nsegs = (int)segs.size();
     kv->nchunks = (int)chunks.size();
     kv->maxnv = maxnv;
+    kv->basis32 = ents.empty() ? 0 : (ents[0].S->b32 ? 1 : 0);  // one engine, one precision
     return FCG_OK;
 }
 
@@ -1023,6 +1028,7 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->N = J->N;
     S->ncol = J->ncol;
     S->mixed = mixed;
+    S->b32 = mixed;
     S->jac = E->opts.precond == FCG_PRECOND_JACOBI;
     S->want_x = E->want_x;
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
@@ -1203,7 +1209,9 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         std::vector<KEnt> ke;
         for (Solve* S : arn) {
             RCCHECK(ensure_basis(ctx, *S, S->i + 1));
-            be.push_back(BatchEntry{S->J, &S->scr, S->hq[(size_t)S->i], S->w.as<double>()});
+            void* qi = S->hq[(size_t)S->i];
+            be.push_back(BatchEntry{S->J, &S->scr, S->b32 ? nullptr : static_cast<const double*>(qi),
+                                    S->w.as<double>(), S->b32 ? static_cast<const float*>(qi) : nullptr});
             ke.push_back(KEnt{S, S->w.as<double>(), S->i + 1, 0});
             S->n_mvp++;
         }

@@ -66,8 +66,10 @@ struct BJob {
     double* pot64;
     float* pot32;  // [M][kp], row perm[node]
     double* lt;    // [2k] 2D fix-up scratch
-    // current product operands ([k][N] each)
+    // current product operands ([k][N] each): the input is q, or q32 when it is
+    // an FP32 basis vector (then q is null)
     const double* q;
+    const float* q32;
     double* v;
 };
 
@@ -88,7 +90,7 @@ struct BatchView {
 
 // Krylov bookkeeping of one job for the batched GMRES kernels.
 struct KJob {
-    double* const* Q;       // device array of basis vector addresses ([k][N] each)
+    void* const* Q;         // device array of basis vector addresses ([k][N] each, FP32 when basis32)
     double* w;              // work vector [k][N]
     double* x;              // current solution [k][N]
     double* r;              // residual [k][N]
@@ -112,6 +114,7 @@ struct KrylovView {
     const Seg* segs;
     const Chunk* chunks;
     int nsegs, nchunks, maxnv;
+    int basis32;  // the basis vectors are FP32 (inner solves of the mixed path)
 };
 
 // Fill classes: which integral branch of PotEstimateOpt a link takes (brackets of
@@ -189,7 +192,6 @@ hipError_t launch_add(const double* x, const double* y, double* out, size_t n, h
 hipError_t launch_sub(const double* x, const double* y, double* out, size_t n, hipStream_t s);
 hipError_t launch_jacobi(const double* q, const double* self, int N, int k, double* pq, hipStream_t s);
 // y[r][n] += sum_jj z[jj][r] Q_jj[r][n] for jj < nvec
-hipError_t launch_combine(double* const* Q, int nvec, const double* z, double* y, int N, int k, hipStream_t s);
 // Conductor charge sums of several finished solves in one launch: for each job,
 // out[off + r ncol + c] = sum over the leaves of conductor column c of w[n] x[r][n].
 struct ChargeSumJob {

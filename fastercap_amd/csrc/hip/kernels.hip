@@ -209,7 +209,8 @@ constexpr int kScanThreads = 256;
 constexpr int kScanPer = kScanTile / kScanThreads;  // 8
 
 __device__ __forceinline__ double scan_input(const BJob& J, int r, int n, int jacobi) {
-    const double x = J.q[(size_t)r * J.N + n];
+    const size_t i = (size_t)r * J.N + n;
+    const double x = J.q32 ? (double)J.q32[i] : J.q[i];
     return jacobi ? x / J.self[n] : x;
 }
 
@@ -623,7 +624,8 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
 // ------------------------------------------------------------------ krylov (batched)
 constexpr int kKThreads = 256;
 
-// grid (nchunks, maxnv)
+// grid (nchunks, maxnv); TB = basis element type
+template <typename TB>
 __global__ void __launch_bounds__(kKThreads) k_dots(KrylovView kv, double* __restrict__ partial) {
     __shared__ double sh[kKThreads / 64];
     const int c = blockIdx.x, jj = blockIdx.y;
@@ -631,10 +633,10 @@ __global__ void __launch_bounds__(kKThreads) k_dots(KrylovView kv, double* __res
     const Seg sg = kv.segs[ch.seg];
     const KJob kj = kv.kjobs[sg.job];
     if (jj >= kj.nv) return;
-    const double* q = kj.Q[jj] + (size_t)sg.r * kj.N;
+    const TB* q = reinterpret_cast<TB* const*>(kj.Q)[jj] + (size_t)sg.r * kj.N;
     const double* w = kj.w + (size_t)sg.r * kj.N;
     double s = 0.0;
-    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += kKThreads) s += q[n] * w[n];
+    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += kKThreads) s += (double)q[n] * w[n];
     s = block_sum(s, sh);
     if (threadIdx.x == 0) partial[(size_t)c * kv.maxnv + jj] = s;
 }
@@ -653,6 +655,7 @@ __global__ void k_reduce(KrylovView kv, const double* __restrict__ partial, doub
 }
 
 // grid (nchunks): w -= sum_jj h[jj] Q_jj on the chunk; optional |w|^2 partial[c]
+template <typename TB>
 __global__ void __launch_bounds__(kKThreads) k_update(KrylovView kv, const double* __restrict__ hbuf, int which,
                                                       int norm, double* __restrict__ partial) {
     __shared__ double sh[kKThreads / 64];
@@ -665,7 +668,8 @@ __global__ void __launch_bounds__(kKThreads) k_update(KrylovView kv, const doubl
     double ss = 0.0;
     for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += kKThreads) {
         double acc = w[n];
-        for (int jj = 0; jj < kj.nv; ++jj) acc -= h[(size_t)jj * kj.k] * kj.Q[jj][(size_t)sg.r * kj.N + n];
+        for (int jj = 0; jj < kj.nv; ++jj)
+            acc -= h[(size_t)jj * kj.k] * (double)reinterpret_cast<TB* const*>(kj.Q)[jj][(size_t)sg.r * kj.N + n];
         w[n] = acc;
         ss += acc * acc;
     }
@@ -687,6 +691,7 @@ __global__ void k_reduce_norm(KrylovView kv, const double* __restrict__ partial,
 }
 
 // grid (nchunks): Q_nv[r] = w[r] / d[seg]
+template <typename TB>
 __global__ void k_scale_new(KrylovView kv, const double* __restrict__ d) {
     const int c = blockIdx.x;
     const Chunk ch = kv.chunks[c];
@@ -694,8 +699,8 @@ __global__ void k_scale_new(KrylovView kv, const double* __restrict__ d) {
     const KJob kj = kv.kjobs[sg.job];
     const double den = d[ch.seg];
     const double* w = kj.w + (size_t)sg.r * kj.N;
-    double* q = kj.Q[kj.nv] + (size_t)sg.r * kj.N;
-    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += blockDim.x) q[n] = den != 0.0 ? w[n] / den : 0.0;
+    TB* q = reinterpret_cast<TB* const*>(kj.Q)[kj.nv] + (size_t)sg.r * kj.N;
+    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += blockDim.x) q[n] = (TB)(den != 0.0 ? w[n] / den : 0.0);
 }
 
 __global__ void k_init_x(KrylovView kv) {
@@ -727,6 +732,7 @@ __global__ void __launch_bounds__(kKThreads) k_resid(KrylovView kv, double* __re
     if (threadIdx.x == 0) partial[blockIdx.x] = ss;
 }
 
+template <typename TB>
 __global__ void k_xupdate(KrylovView kv, const double* __restrict__ z) {
     const Chunk ch = kv.chunks[blockIdx.x];
     const Seg sg = kv.segs[ch.seg];
@@ -737,7 +743,7 @@ __global__ void k_xupdate(KrylovView kv, const double* __restrict__ z) {
         double acc = 0.0;
         for (int jj = 0; jj < kj.nz; ++jj) {
             const double c = zz[(size_t)jj * kj.k];
-            if (c != 0.0) acc += c * kj.Q[jj][o + n];
+            if (c != 0.0) acc += c * (double)reinterpret_cast<TB* const*>(kj.Q)[jj][o + n];
         }
         kj.x[o + n] += kj.jacobi ? acc / kj.self[n] : acc;
     }
@@ -755,13 +761,19 @@ hipError_t launch_resid(const KrylovView& kv, double* partial, hipStream_t s) {
 }
 hipError_t launch_xupdate(const KrylovView& kv, const double* z, hipStream_t s) {
     if (kv.nchunks == 0) return hipSuccess;
-    k_xupdate<<<kv.nchunks, 256, 0, s>>>(kv, z);
+    if (kv.basis32)
+        k_xupdate<float><<<kv.nchunks, 256, 0, s>>>(kv, z);
+    else
+        k_xupdate<double><<<kv.nchunks, 256, 0, s>>>(kv, z);
     return hipGetLastError();
 }
 
 hipError_t launch_dots(const KrylovView& kv, double* partial, hipStream_t s) {
     if (kv.nchunks == 0 || kv.maxnv == 0) return hipSuccess;
-    k_dots<<<dim3(kv.nchunks, kv.maxnv), kKThreads, 0, s>>>(kv, partial);
+    if (kv.basis32)
+        k_dots<float><<<dim3(kv.nchunks, kv.maxnv), kKThreads, 0, s>>>(kv, partial);
+    else
+        k_dots<double><<<dim3(kv.nchunks, kv.maxnv), kKThreads, 0, s>>>(kv, partial);
     return hipGetLastError();
 }
 hipError_t launch_reduce(const KrylovView& kv, const double* partial, double* hbuf, int which, hipStream_t s) {
@@ -772,7 +784,10 @@ hipError_t launch_reduce(const KrylovView& kv, const double* partial, double* hb
 hipError_t launch_update(const KrylovView& kv, const double* hbuf, int which, int norm, double* partial,
                          hipStream_t s) {
     if (kv.nchunks == 0) return hipSuccess;
-    k_update<<<kv.nchunks, kKThreads, 0, s>>>(kv, hbuf, which, norm, partial);
+    if (kv.basis32)
+        k_update<float><<<kv.nchunks, kKThreads, 0, s>>>(kv, hbuf, which, norm, partial);
+    else
+        k_update<double><<<kv.nchunks, kKThreads, 0, s>>>(kv, hbuf, which, norm, partial);
     return hipGetLastError();
 }
 hipError_t launch_reduce_norm(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s) {
@@ -782,7 +797,10 @@ hipError_t launch_reduce_norm(const KrylovView& kv, const double* partial, doubl
 }
 hipError_t launch_scale_new(const KrylovView& kv, const double* d, hipStream_t s) {
     if (kv.nchunks == 0) return hipSuccess;
-    k_scale_new<<<kv.nchunks, 256, 0, s>>>(kv, d);
+    if (kv.basis32)
+        k_scale_new<float><<<kv.nchunks, 256, 0, s>>>(kv, d);
+    else
+        k_scale_new<double><<<kv.nchunks, 256, 0, s>>>(kv, d);
     return hipGetLastError();
 }
 
@@ -833,23 +851,6 @@ hipError_t launch_jacobi(const double* q, const double* self, int N, int k, doub
     return hipGetLastError();
 }
 
-__global__ void k_combine(double* const* __restrict__ Q, int nvec, const double* __restrict__ z,
-                          double* __restrict__ y, int N, int k) {
-    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (size_t)N * k) return;
-    const int r = (int)(t / N);
-    double acc = 0.0;
-    for (int jj = 0; jj < nvec; ++jj) {
-        const double zz = z[(size_t)jj * k + r];
-        if (zz != 0.0) acc += Q[jj][t] * zz;
-    }
-    y[t] += acc;
-}
-hipError_t launch_combine(double* const* Q, int nvec, const double* z, double* y, int N, int k, hipStream_t s) {
-    if (nvec <= 0) return hipSuccess;
-    k_combine<<<blocks_for((size_t)N * k, 256), 256, 0, s>>>(Q, nvec, z, y, N, k);
-    return hipGetLastError();
-}
 
 // grid (ncol, k)
 // one block per (job, rhs, conductor column)
