@@ -27,6 +27,7 @@ from ._ffi import (
     FCG_MVP_FP64,
     FCG_PREC_FP64,
     FCG_PREC_MIXED,
+    FCG_PRECOND_BLOCK,
     FCG_PRECOND_JACOBI,
     FCG_PRECOND_NONE,
     HostOptions,
@@ -49,6 +50,7 @@ __all__ = [
     "FCG_MVP_FP32_ACC64",
     "FCG_PREC_FP64",
     "FCG_PREC_MIXED",
+    "FCG_PRECOND_BLOCK",
     "FCG_PRECOND_JACOBI",
     "FCG_PRECOND_NONE",
     "TWO_PI_TIMES_E0",
@@ -219,6 +221,15 @@ class Device:
         self._check(self.lib.fcg_precond_apply(self._h, job, _dptr(q), _dptr(v), k), "fcg_precond_apply")
         return v
 
+    def block_precond(self, job: int, q: np.ndarray, block_size: int = 32) -> np.ndarray:
+        """B q with the job's block preconditioner (blocks of at most block_size leaves)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        k = 1 if q.ndim == 1 else q.shape[0]
+        v = np.empty_like(q)
+        self._check(self.lib.fcg_block_precond_apply(self._h, job, block_size, _dptr(q), _dptr(v), k),
+                    "fcg_block_precond_apply")
+        return v
+
     def solve(
         self,
         job: int,
@@ -229,12 +240,14 @@ class Device:
         precision: int = FCG_PREC_FP64,
         want_charges: bool = False,
         max_refine: int = 8,
+        block_size: int = 32,
     ) -> SolveResult:
         ncond = int((mesh.group_is_diel == 0).sum())
         rows_max = ncond
         cap = np.zeros(rows_max * ncond, dtype=np.float64)
         charges = np.zeros(rows_max * mesh.N, dtype=np.float64) if want_charges else None
-        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine)
+        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine,
+                      block_size=block_size)
         st = SolveStats()
         rows, cols = C.c_int(), C.c_int()
         self._check(
@@ -270,6 +283,7 @@ class Device:
         precond: int = FCG_PRECOND_JACOBI,
         precision: int = FCG_PREC_FP64,
         max_refine: int = 8,
+        block_size: int = 32,
     ):
         """Solve several filled jobs together (one launch per stage for all of
         them); returns one SolveResult per job, in order."""
@@ -280,7 +294,8 @@ class Device:
         rows = (C.c_int * len(jobs))()
         cols = (C.c_int * len(jobs))()
         sts = (SolveStats * len(jobs))()
-        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine)
+        o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine,
+                      block_size=block_size)
         self._check(
             self.lib.fcg_solve_batch(self._h, ja, len(jobs), C.byref(o), _dptr(caps), rows, cols, sts),
             "fcg_solve_batch",
@@ -316,9 +331,9 @@ class Device:
         return n.value
 
     def engine(self, tol: float = 0.01, max_iter: int = 1000, precond: int = FCG_PRECOND_JACOBI,
-               precision: int = FCG_PREC_FP64, max_refine: int = 8) -> "Engine":
+               precision: int = FCG_PREC_FP64, max_refine: int = 8, block_size: int = 32) -> "Engine":
         return Engine(self, SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision,
-                                      max_refine=max_refine))
+                                      max_refine=max_refine, block_size=block_size))
 
     def mvp_timing(self, job: int, k: int, reps: int = 20) -> dict:
         t = _ffi.MvpTiming()

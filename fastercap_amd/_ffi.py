@@ -55,6 +55,7 @@ class SolveOpts(C.Structure):
         ("precond", C.c_int32),
         ("precision", C.c_int32),
         ("max_refine", C.c_int32),
+        ("block_size", C.c_int32),
     ]
 
 
@@ -134,6 +135,7 @@ FCG_PREC_FP64 = 0
 FCG_PREC_MIXED = 1
 FCG_PRECOND_NONE = 0
 FCG_PRECOND_JACOBI = 1
+FCG_PRECOND_BLOCK = 2
 FCG_MVP_FP64 = 0
 FCG_MVP_FP32 = 1
 FCG_MVP_FP32_ACC64 = 2
@@ -152,6 +154,7 @@ DEVICE_SYMBOLS = (
     "fcg_mvp",
     "fcg_mvp_device",
     "fcg_precond_apply",
+    "fcg_block_precond_apply",
     "fcg_solve",
     "fcg_job_conductors",
     "fcg_last_error",
@@ -236,6 +239,7 @@ def device_lib() -> C.CDLL:
         lib.fcg_mvp.argtypes = [vp, C.c_int, dp, dp, C.c_int]
         lib.fcg_mvp_device.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         lib.fcg_precond_apply.argtypes = [vp, C.c_int, dp, dp, C.c_int]
+        lib.fcg_block_precond_apply.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]
         lib.fcg_solve.argtypes = [
             vp,
             C.c_int,

@@ -36,6 +36,7 @@ class Args:
     auto_err: float = 0.01
     auto_precond: bool = False
     precond: int = 1  # FCG_PRECOND_JACOBI
+    block_size: int = 32  # -pb<n> (SolverGlobal.cpp:60)
     precision: int = 0  # FCG_PREC_FP64
     verbose: bool = False
     device: int = 0
@@ -68,6 +69,21 @@ def parse_args(argv) -> Args:
             a.precond = 1
         elif opt == "-pn":
             a.precond = 0
+        elif opt.startswith("-pb"):
+            # the reference adds the block preconditioner to the current choice and then
+            # runs a flexible GMRES it reports as unsupported; here -pb selects the block
+            # preconditioner alone (as -pn -pb<n> there)
+            v = _num(opt[3:], opt)
+            if v != int(v) or v < 0:
+                raise CommandLineError(f"bad preconditioner dimension '{opt[3:]}'")
+            a.precond = 2  # FCG_PRECOND_BLOCK
+            a.block_size = int(v)
+            if a.block_size > 128:
+                print("Warning: maximum block preconditioner dimension is 128, using 128", file=sys.stderr)
+                a.block_size = 128
+            elif a.block_size < 2:
+                print("Warning: minimum block preconditioner dimension is 2, using 2", file=sys.stderr)
+                a.block_size = 2
         elif opt == "-gmixed":
             a.precision = 1
         elif opt.startswith("-gdev"):
@@ -115,7 +131,7 @@ def solve_file(dev, a: Args, mesh_eps: float, tol: float):
     job = dev.upload(mesh)
     try:
         dev.fill(job, a.precision)
-        res = dev.solve(job, mesh, tol=tol, precond=a.precond, precision=a.precision)
+        res = dev.solve(job, mesh, tol=tol, precond=a.precond, precision=a.precision, block_size=a.block_size)
     finally:
         dev.release(job)
     return mesh, res

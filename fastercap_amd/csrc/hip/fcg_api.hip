@@ -221,6 +221,13 @@ struct Job {
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
     std::vector<int64_t> fill_cnt;  // links per fill class
+    // host copies of the tree (block preconditioner partition)
+    std::vector<int32_t> h_left, h_right, h_parent, h_leaf_lo, h_leaf_hi;
+    // block preconditioner of the last block size asked for (0: none yet)
+    struct BlockPre {
+        int size = 0, nb = 0;
+        DevBuf bstart, boff, P;
+    } bpre;
     std::vector<uint32_t> link_pos;
     bool filled64 = false, filled32 = false;
     std::vector<int32_t> group_is_diel, group_leaf_start;
@@ -238,7 +245,7 @@ struct Job {
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
-                         col16.bytes +
+                         col16.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes +
                          scr.bytes());
     }
 
@@ -712,6 +719,110 @@ void build_fill_order(const fcg_job_desc* d, const std::vector<int32_t>& iperm,
     for (auto& x : th) x.join();
 }
 
+// Block preconditioner partition (RecurseComputePrecond,
+// Solver/SolveCapacitance.cpp:1094-1104, 1876-1993): every conductor's tree (roots
+// in leaf order) is walked depth first, left first; the first subtree met with at
+// most bsize leaves is one block of consecutive leaves.  A leaf met before any
+// block is open (its parent has more than bsize leaves) is a block of its own;
+// the reference opens a block there that it never computes.
+std::vector<int32_t> block_partition(const Job& j, int bsize) {
+    std::vector<int32_t> roots;
+    for (int i = 0; i < j.M; ++i)
+        if (j.h_parent[(size_t)i] < 0) roots.push_back(i);
+    std::sort(roots.begin(), roots.end(),
+              [&](int a, int b) { return j.h_leaf_lo[(size_t)a] < j.h_leaf_lo[(size_t)b]; });
+    std::vector<int32_t> starts, stack;
+    for (int r : roots) {
+        stack.push_back(r);
+        while (!stack.empty()) {
+            const int a = stack.back();
+            stack.pop_back();
+            if (j.h_leaf_hi[(size_t)a] - j.h_leaf_lo[(size_t)a] <= bsize || j.h_left[(size_t)a] < 0) {
+                starts.push_back(j.h_leaf_lo[(size_t)a]);
+                continue;
+            }
+            stack.push_back(j.h_right[(size_t)a]);
+            stack.push_back(j.h_left[(size_t)a]);
+        }
+    }
+    starts.push_back(j.N);
+    return starts;
+}
+
+int clamp_block_size(int b) { return b <= 0 ? 32 : std::min(std::max(b, 2), fcg::kMaxBlockPrecond); }
+
+// the job's block preconditioner for block size bsize (built on first use; needs
+// the self terms of a fill)
+int ensure_block_precond(fcg_ctx* ctx, Job& j, int bsize) {
+    bsize = clamp_block_size(bsize);
+    if (j.bpre.size == bsize) return FCG_OK;
+    if (!j.filled64 && !j.filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    hipStream_t s = ctx->stream;
+    const std::vector<int32_t> starts = block_partition(j, bsize);
+    const int nb = (int)starts.size() - 1;
+    std::vector<int64_t> off((size_t)nb + 1, 0);
+    int maxn = 0;
+    for (int b = 0; b < nb; ++b) {
+        const int64_t n = starts[(size_t)b + 1] - starts[(size_t)b];
+        off[(size_t)b + 1] = off[(size_t)b] + n * n;
+        maxn = std::max(maxn, (int)n);
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(s));  // the buffers below may be in use; the table arena is reset
+    j.bpre.size = 0;
+    HIPCHECK(ctx, upload(j.bpre.bstart, starts.data(), starts.size(), s));
+    HIPCHECK(ctx, upload(j.bpre.boff, off.data(), off.size(), s));
+    HIPCHECK(ctx, j.bpre.P.alloc(sizeof(double) * (size_t)std::max<int64_t>(off.back(), 1)));
+    DevBuf work;
+    if (maxn > 64) HIPCHECK(ctx, work.alloc(sizeof(double) * (size_t)off.back()));
+    ctx->tabs.reset();
+    BatchView bv;
+    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, nullptr, nullptr}}, &bv));
+    HIPCHECK(ctx, fcg::launch_block_setup(bv, nb, j.bpre.bstart.as<int32_t>(), j.bpre.boff.as<int64_t>(),
+                                          j.bpre.P.as<double>(), work.as<double>(), s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));  // work and the host vectors are released below
+    j.bpre.size = bsize;
+    j.bpre.nb = nb;
+    return FCG_OK;
+}
+
+// Apply table of the block preconditioner over several solves.
+struct PEnt {
+    const Job* j;
+    const void* in;
+    bool in32;
+    double* out;
+    int k;
+};
+int make_pview(fcg_ctx* ctx, Tables& tabs, const std::vector<PEnt>& ents, fcg::PView* pv) {
+    std::memset(pv, 0, sizeof *pv);
+    if (ents.empty()) return FCG_OK;
+    std::vector<fcg::PJob> pj;
+    std::vector<int64_t> off(ents.size() + 1, 0);
+    for (size_t i = 0; i < ents.size(); ++i) {
+        const PEnt& e = ents[i];
+        fcg::PJob r{};
+        r.in = e.in;
+        r.out = e.out;
+        r.brange = e.j->rhs_range.as<int32_t>();
+        r.bstart = e.j->bpre.bstart.as<int32_t>();
+        r.boff = e.j->bpre.boff.as<int64_t>();
+        r.P = e.j->bpre.P.as<double>();
+        r.N = e.j->N;
+        r.k = e.k;
+        r.in32 = e.in32 ? 1 : 0;
+        pj.push_back(r);
+        off[i + 1] = off[i] + e.j->bpre.nb;
+    }
+    void *dj, *doff;
+    HIPCHECK(ctx, tabs.push(pj.data(), sizeof(fcg::PJob) * pj.size(), ctx->stream, &dj));
+    HIPCHECK(ctx, tabs.push(off.data(), sizeof(int64_t) * off.size(), ctx->stream, &doff));
+    pv->jobs = reinterpret_cast<const fcg::PJob*>(dj);
+    pv->njobs = (int)ents.size();
+    pv->blk_off = reinterpret_cast<const int64_t*>(doff);
+    pv->nblk = off.back();
+    return FCG_OK;
+}
+
 // One stand-alone product on device buffers of job j (batch of one).
 int run_mvp(fcg_ctx* ctx, Job& j, int prec, const double* dq, double* dv, int k, int jacobi, hipStream_t s,
             hipEvent_t* ev = nullptr) {
@@ -742,6 +853,8 @@ struct Solve {
     int64_t tag = 0;
     int k = 0, N = 0, ncol = 0;
     bool mixed = false, jac = true, want_x = false;
+    bool block = false;  // block preconditioner: P applied by its own kernel, u = P q scratch
+    PBuf u;
     PBuf x, r, w;
     Scratch scr;
     // Krylov basis: blocks of vectors and their addresses (sent with each step's tables)
@@ -765,7 +878,7 @@ struct Solve {
     clk::time_point t0;
     int hoff = 0;
     int64_t bytes() const {
-        int64_t s = (int64_t)(x.r + r.r + w.r + scr.bytes());
+        int64_t s = (int64_t)(x.r + r.r + w.r + u.r + scr.bytes());
         for (auto& q : qblocks) s += (int64_t)q->r;
         return s;
     }
@@ -864,6 +977,7 @@ KJob kjob_of(Solve& S, void* const* qtab, double* w, int nv, int zoff) {
     J.zoff = zoff;
     J.nz = S.nz;
     J.jacobi = S.jac ? 1 : 0;
+    J.y = S.block ? S.u.as<double>() : nullptr;
     return J;
 }
 
@@ -1030,6 +1144,7 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->mixed = mixed;
     S->b32 = mixed;
     S->jac = E->opts.precond == FCG_PRECOND_JACOBI;
+    S->block = E->opts.precond == FCG_PRECOND_BLOCK;
     S->want_x = E->want_x;
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
     RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, S->k, mixed));
@@ -1037,6 +1152,10 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     HIPCHECK(ctx, S->x.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->r.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->w.get(&ctx->pool, sizeof(double) * vec));
+    if (S->block) {
+        RCCHECK(ensure_block_precond(ctx, *J, E->opts.block_size));
+        HIPCHECK(ctx, S->u.get(&ctx->pool, sizeof(double) * vec));
+    }
     // unit potential on each conductor's panels (SolveCapacitance.cpp:1209-1211); the
     // kernels form b from the job's leaf ranges, so only its norm is kept here
     S->normb.assign((size_t)S->k, 0.0);
@@ -1092,12 +1211,14 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             for (auto& e : sc) e.S->need_scale = false;
         }
         std::vector<KEnt> up;
+        std::vector<PEnt> upp;  // block preconditioner: x += P (Q z)
         std::vector<double> zall;
         for (auto& u : C.active) {
             Solve& S = *u;
             if (S.phase != kUpdate) continue;
             up.push_back(KEnt{&S, S.w.as<double>(), 0, (int)zall.size()});
             zall.insert(zall.end(), S.z.begin(), S.z.end());
+            if (S.block) upp.push_back(PEnt{S.J, S.u.as<double>(), false, S.x.as<double>(), S.k});
         }
         if (!up.empty()) {
             KrylovView kv;
@@ -1105,19 +1226,35 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             void* dz;
             HIPCHECK(ctx, C.tabs.push(zall.data(), sizeof(double) * zall.size(), s, &dz));
             q.push_back([=] { return fcg::launch_xupdate(kv, reinterpret_cast<double*>(dz), s); });
+            if (!upp.empty()) {
+                fcg::PView pv;
+                RCCHECK(make_pview(ctx, C.tabs, upp, &pv));
+                q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreAdd, s); });
+            }
             for (auto& e : up) {
                 e.S->nz = 0;
                 e.S->phase = e.S->mixed ? kResid : kFinish;
             }
         }
         std::vector<KEnt> init;
-        for (auto& u : C.active)
-            if (u->phase == kInit) init.push_back(KEnt{u.get(), u->w.as<double>(), 0, 0});
+        std::vector<PEnt> initp;  // block preconditioner: x0 = P b
+        for (auto& u : C.active) {
+            if (u->phase != kInit) continue;
+            if (u->block)
+                initp.push_back(PEnt{u->J, nullptr, false, u->x.as<double>(), u->k});
+            else
+                init.push_back(KEnt{u.get(), u->w.as<double>(), 0, 0});
+            u->phase = kResid;
+        }
         if (!init.empty()) {
             KrylovView kv;
             RCCHECK(make_krylov(ctx, C.tabs, init, &kv));
             q.push_back([=] { return fcg::launch_init_x(kv, s); });
-            for (auto& e : init) e.S->phase = kResid;
+        }
+        if (!initp.empty()) {
+            fcg::PView pv;
+            RCCHECK(make_pview(ctx, C.tabs, initp, &pv));
+            q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreFromB, s); });
         }
     }
     // finishing solves: conductor charge sums (SolveCapacitance.cpp:1305-1366), one
@@ -1207,13 +1344,24 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     if (!arn.empty()) {
         std::vector<BatchEntry> be;
         std::vector<KEnt> ke;
+        std::vector<PEnt> pe;  // block preconditioner: the product runs on u = P q_i
         for (Solve* S : arn) {
             RCCHECK(ensure_basis(ctx, *S, S->i + 1));
             void* qi = S->hq[(size_t)S->i];
-            be.push_back(BatchEntry{S->J, &S->scr, S->b32 ? nullptr : static_cast<const double*>(qi),
-                                    S->w.as<double>(), S->b32 ? static_cast<const float*>(qi) : nullptr});
+            if (S->block) {
+                pe.push_back(PEnt{S->J, qi, S->b32, S->u.as<double>(), S->k});
+                be.push_back(BatchEntry{S->J, &S->scr, S->u.as<double>(), S->w.as<double>(), nullptr});
+            } else {
+                be.push_back(BatchEntry{S->J, &S->scr, S->b32 ? nullptr : static_cast<const double*>(qi),
+                                        S->w.as<double>(), S->b32 ? static_cast<const float*>(qi) : nullptr});
+            }
             ke.push_back(KEnt{S, S->w.as<double>(), S->i + 1, 0});
             S->n_mvp++;
+        }
+        if (!pe.empty()) {
+            fcg::PView pv;
+            RCCHECK(make_pview(ctx, C.tabs, pe, &pv));
+            q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreSet, s); });
         }
         BatchView bv;
         fcg::GatherPlan gp;
@@ -1418,6 +1566,11 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     HIPCHECK(ctx, upload(j->parent, d->parent, (size_t)j->M, s));
     HIPCHECK(ctx, upload(j->leaf_lo, d->leaf_lo, (size_t)j->M, s));
     HIPCHECK(ctx, upload(j->leaf_hi, d->leaf_hi, (size_t)j->M, s));
+    j->h_left.assign(d->left, d->left + j->M);
+    j->h_right.assign(d->right, d->right + j->M);
+    j->h_parent.assign(d->parent, d->parent + j->M);
+    j->h_leaf_lo.assign(d->leaf_lo, d->leaf_lo + j->M);
+    j->h_leaf_hi.assign(d->leaf_hi, d->leaf_hi + j->M);
     HIPCHECK(ctx, upload(j->leaf_node, d->leaf_node, (size_t)j->N, s));
     HIPCHECK(ctx, upload(j->row_ptr, d->row_ptr, (size_t)j->M + 1, s));
     {
@@ -1902,9 +2055,28 @@ int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k)
     return FCG_OK;
 }
 
+int fcg_block_precond_apply(fcg_ctx* ctx, int job, int block_size, const double* q, double* pq, int k) {
+    Job* j = get_job(ctx, job);
+    if (!j || !q || !pq || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    RCCHECK(ensure_block_precond(ctx, *j, block_size));
+    DevBuf dq, dp;
+    const size_t n = (size_t)j->N * k;
+    HIPCHECK(ctx, upload(dq, q, n, ctx->stream));
+    HIPCHECK(ctx, dp.alloc(sizeof(double) * n));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));  // the table arena is reset below
+    ctx->tabs.reset();
+    fcg::PView pv;
+    RCCHECK(make_pview(ctx, ctx->tabs, {PEnt{j, dq.p, false, dp.as<double>(), k}}, &pv));
+    HIPCHECK(ctx, fcg::launch_block_apply(pv, fcg::kPreSet, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(pq, dp.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return FCG_OK;
+}
+
 int fcg_engine_create(fcg_ctx* ctx, const fcg_solve_opts* opts, fcg_engine** out) {
     if (!ctx || !opts || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (opts->precond != FCG_PRECOND_NONE && opts->precond != FCG_PRECOND_JACOBI)
+    if (opts->precond != FCG_PRECOND_NONE && opts->precond != FCG_PRECOND_JACOBI && opts->precond != FCG_PRECOND_BLOCK)
         return fail(ctx, FCG_INVALID_ARGUMENT, "unsupported preconditioner");
     if (opts->precision != FCG_PREC_FP64 && opts->precision != FCG_PREC_MIXED)
         return fail(ctx, FCG_INVALID_ARGUMENT, "unknown precision mode");

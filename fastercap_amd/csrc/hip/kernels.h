@@ -100,6 +100,7 @@ struct KJob {
     int hoff;            // offset of this job's block in the step buffer
     int zoff, nz;        // solution-update coefficients z[zoff + jj*k + r], jj < nz
     int jacobi;
+    double* y;           // block preconditioner: the update writes Q z here instead of adding to x
 };
 
 // Segment (job, rhs) and chunk tables for the Krylov kernels.
@@ -179,6 +180,36 @@ hipError_t launch_update(const KrylovView& kv, const double* hbuf, int which, in
 hipError_t launch_reduce_norm(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s);
 // Q_nv[seg] = w[seg] / d[seg]   (d per segment, 0 -> zero vector)
 hipError_t launch_scale_new(const KrylovView& kv, const double* d, hipStream_t s);
+// Block preconditioner (ComputePrecondVectFast block branch,
+// Solver/SolveCapacitance.cpp:3527-3544): blocks of consecutive leaves
+// [bstart[b], bstart[b+1]) of at most 128 leaves, each block's inverse stored
+// column-major at P + boff[b].
+constexpr int kMaxBlockPrecond = 128;
+// Build and invert every block of one job (the batch view holds that job only):
+// the block's collocation matrix (self terms on the diagonal, link coefficients
+// elsewhere), LU with partial pivoting, inverse.  work: scratch of boff[nb]
+// doubles, used by blocks of more than 64 leaves.
+hipError_t launch_block_setup(const BatchView& one_job, int nb, const int32_t* bstart, const int64_t* boff, double* P,
+                              double* work, hipStream_t s);
+// Apply of the block preconditioner to several solves in one launch.
+struct PJob {
+    const void* in;          // [k][N] FP64, or FP32 when in32
+    double* out;             // [k][N]
+    const int32_t* brange;   // conductor leaf ranges (mode kPreFromB: in = b)
+    const int32_t* bstart;
+    const int64_t* boff;
+    const double* P;
+    int N, k, in32;
+};
+struct PView {
+    const PJob* jobs;
+    int njobs;
+    const int64_t* blk_off;  // cumulative block counts (njobs + 1)
+    int64_t nblk;
+};
+enum PreMode { kPreSet = 0, kPreAdd = 1, kPreFromB = 2 };  // out = P in, out += P in, out = P b
+hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s);
+
 // x = P b (Jacobi) or b
 hipError_t launch_init_x(const KrylovView& kv, hipStream_t s);
 // r = b - w with |r|^2 chunk partials (the product A x is in w)

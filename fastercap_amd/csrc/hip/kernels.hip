@@ -745,7 +745,10 @@ __global__ void k_xupdate(KrylovView kv, const double* __restrict__ z) {
             const double c = zz[(size_t)jj * kj.k];
             if (c != 0.0) acc += c * (double)reinterpret_cast<TB* const*>(kj.Q)[jj][o + n];
         }
-        kj.x[o + n] += kj.jacobi ? acc / kj.self[n] : acc;
+        if (kj.y)
+            kj.y[o + n] = acc;
+        else
+            kj.x[o + n] += kj.jacobi ? acc / kj.self[n] : acc;
     }
 }
 
@@ -888,6 +891,202 @@ __global__ void k_sqnorms(const double* __restrict__ x, int N, double* __restric
 }
 hipError_t launch_sqnorms(const double* x, int N, int k, double* out, hipStream_t s) {
     k_sqnorms<<<k, 256, 0, s>>>(x, N, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------ block preconditioner
+// Link coefficient with the branch chosen at run time (setup only): the ratio is
+// computed in FP64 without contractions, as the reference and the upload-time
+// classification compute it.
+__device__ __noinline__ double pair_coef(const fcg_node_geom& p1, const fcg_node_geom& p2, int dim) {
+    double ratio;
+    {
+#pragma clang fp contract(off)
+        const double dx = (double)p1.c[0] - (double)p2.c[0], dy = (double)p1.c[1] - (double)p2.c[1];
+        const double dz = dim == 3 ? (double)p1.c[2] - (double)p2.c[2] : 0.0;
+        const double rmax = p1.maxside > p2.maxside ? p1.maxside : p2.maxside;
+        ratio = (dim == 3 ? sqrt(dx * dx + dy * dy + dz * dz) : sqrt(dx * dx + dy * dy)) / rmax;
+    }
+    const bool diel = (p1.flags & FCG_NODE_DIEL) != 0;
+    if (dim == 2) {
+        if (ratio < 4.0)
+            return diel ? link_coef<double, 2, true, kBrAnalytic>(p1, p2) : link_coef<double, 2, false, kBrAnalytic>(p1, p2);
+        return diel ? link_coef<double, 2, true, kBrPoint>(p1, p2) : link_coef<double, 2, false, kBrPoint>(p1, p2);
+    }
+    if (diel) {
+        if (ratio < 1.0) return link_coef<double, 3, true, kBrAnalytic>(p1, p2);
+        if (ratio < 2.0) return link_coef<double, 3, true, kBrRule5>(p1, p2);
+        if (ratio < 4.0) return link_coef<double, 3, true, kBrRule3>(p1, p2);
+        if (ratio < 10.0) return link_coef<double, 3, true, kBrRule2>(p1, p2);
+        return link_coef<double, 3, true, kBrPoint>(p1, p2);
+    }
+    if (ratio < 1.0) return link_coef<double, 3, false, kBrAnalytic>(p1, p2);
+    if (ratio < 2.0) return link_coef<double, 3, false, kBrRule5>(p1, p2);
+    if (ratio < 4.0) return link_coef<double, 3, false, kBrRule3>(p1, p2);
+    if (ratio < 10.0) return link_coef<double, 3, false, kBrRule2>(p1, p2);
+    return link_coef<double, 3, false, kBrPoint>(p1, p2);
+}
+
+constexpr int kSetupThreads = 256;
+constexpr int kSetupLds = 64;  // blocks of at most this many leaves are factored in LDS
+
+// one workgroup per block (ComputeBlockPrecond + InvertMatrix,
+// Solver/SolveCapacitance.cpp:2068-2258)
+__global__ void __launch_bounds__(kSetupThreads) k_block_setup(BatchView bv, const int32_t* __restrict__ bstart,
+                                                               const int64_t* __restrict__ boff,
+                                                               double* __restrict__ P, double* __restrict__ work) {
+    __shared__ double sh[kSetupLds * kSetupLds];
+    __shared__ double rv[kSetupThreads];
+    __shared__ int ri[kSetupThreads];
+    __shared__ int piv[kMaxBlockPrecond];
+    const BJob& J = bv.jobs[0];
+    const int b = blockIdx.x;
+    const int b0 = bstart[b], n = bstart[b + 1] - b0;
+    const int t = threadIdx.x;
+    double* A = n <= kSetupLds ? sh : work + boff[b];  // row-major n x n
+    for (int e = t; e < n * n; e += kSetupThreads) {
+        const int i = e / n, j = e - i * n;
+        A[e] = (i == j) ? J.self[b0 + i] : pair_coef(J.nodes[J.leaf_node[b0 + i]], J.nodes[J.leaf_node[b0 + j]], J.dim);
+    }
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        // pivot: the first row of largest |A[i][j]|, i >= j
+        double best = -1.0;
+        int bi = n;
+        for (int i = j + t; i < n; i += kSetupThreads) {
+            const double v = fabs(A[i * n + j]);
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        rv[t] = best;
+        ri[t] = bi;
+        __syncthreads();
+        for (int off = kSetupThreads / 2; off > 0; off >>= 1) {
+            if (t < off) {
+                const double v2 = rv[t + off];
+                const int i2 = ri[t + off];
+                if (v2 > rv[t] || (v2 == rv[t] && i2 < ri[t])) {
+                    rv[t] = v2;
+                    ri[t] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        const int p = ri[0];
+        if (t == 0) piv[j] = p;
+        if (p != j)
+            for (int c = t; c < n; c += kSetupThreads) {
+                const double tmp = A[j * n + c];
+                A[j * n + c] = A[p * n + c];
+                A[p * n + c] = tmp;
+            }
+        __syncthreads();
+        if (j < n - 1) {
+            const double rcp = 1.0 / A[j * n + j];
+            for (int i = j + 1 + t; i < n; i += kSetupThreads) A[i * n + j] *= rcp;
+            __syncthreads();
+            const int m = n - 1 - j;
+            for (int e = t; e < m * m; e += kSetupThreads) {
+                const int i = j + 1 + e / m, c = j + 1 + e % m;
+                A[i * n + c] -= A[i * n + j] * A[j * n + c];
+            }
+        }
+        __syncthreads();
+    }
+    // inverse: column k from the unit vector e_k, solved in place in column k of
+    // the output (column-major storage)
+    double* Pb = P + boff[b];
+    for (int k = t; k < n; k += kSetupThreads) {
+        double* y = Pb + (size_t)k * n;
+        for (int i = 0; i < n; ++i) y[i] = (i == k) ? 1.0 : 0.0;
+        for (int i = 0; i < n; ++i) {
+            const int ip = piv[i];
+            double sum = y[ip];
+            y[ip] = y[i];
+            for (int c = 0; c < i; ++c) sum -= A[i * n + c] * y[c];
+            y[i] = sum;
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double sum = y[i];
+            for (int c = i + 1; c < n; ++c) sum -= A[i * n + c] * y[c];
+            y[i] = sum / A[i * n + i];
+        }
+    }
+}
+
+hipError_t launch_block_setup(const BatchView& one_job, int nb, const int32_t* bstart, const int64_t* boff, double* P,
+                              double* work, hipStream_t s) {
+    if (nb <= 0) return hipSuccess;
+    k_block_setup<<<(unsigned)nb, kSetupThreads, 0, s>>>(one_job, bstart, boff, P, work);
+    return hipGetLastError();
+}
+
+// one wavefront per (solve, block): lanes over the block's rows, up to kPreRhs
+// right-hand sides per pass so each pass reads the block's inverse once
+constexpr int kPreRhs = 8;
+template <int MODE>
+__global__ void __launch_bounds__(256) k_block_apply(PView pv) {
+    const int64_t gb = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (gb >= pv.nblk) return;
+    const int jb = find_job(pv.blk_off, pv.njobs, gb);
+    const PJob& J = pv.jobs[jb];
+    const int b = (int)(gb - pv.blk_off[jb]);
+    const int b0 = J.bstart[b], n = J.bstart[b + 1] - b0;
+    const double* Pb = J.P + J.boff[b];
+    const int lane = threadIdx.x & 63;
+    const double* in64 = static_cast<const double*>(J.in);
+    const float* in32 = static_cast<const float*>(J.in);
+    for (int r0 = 0; r0 < J.k; r0 += kPreRhs) {
+        double acc[kPreRhs][2];
+#pragma unroll
+        for (int r = 0; r < kPreRhs; ++r) acc[r][0] = acc[r][1] = 0.0;
+        for (int j = 0; j < n; ++j) {
+            const double p0 = lane < n ? Pb[(size_t)j * n + lane] : 0.0;
+            const double p1 = lane + 64 < n ? Pb[(size_t)j * n + lane + 64] : 0.0;
+#pragma unroll
+            for (int r = 0; r < kPreRhs; ++r) {
+                if (r0 + r >= J.k) break;
+                const size_t o = (size_t)(r0 + r) * J.N + b0 + j;
+                double x;
+                if (MODE == kPreFromB) {
+                    const int leaf = b0 + j;
+                    x = (leaf >= J.brange[2 * (r0 + r)] && leaf < J.brange[2 * (r0 + r) + 1]) ? 1.0 : 0.0;
+                } else {
+                    x = J.in32 ? (double)in32[o] : in64[o];
+                }
+                acc[r][0] += p0 * x;
+                acc[r][1] += p1 * x;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kPreRhs; ++r) {
+            if (r0 + r >= J.k) break;
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int i = lane + 64 * h;
+                if (i < n) {
+                    double* o = J.out + (size_t)(r0 + r) * J.N + b0 + i;
+                    if (MODE == kPreAdd)
+                        *o += acc[r][h];
+                    else
+                        *o = acc[r][h];
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s) {
+    if (pv.nblk == 0) return hipSuccess;
+    const unsigned grid = (unsigned)((pv.nblk + 3) / 4);
+    if (mode == kPreSet)
+        k_block_apply<kPreSet><<<grid, 256, 0, s>>>(pv);
+    else if (mode == kPreAdd)
+        k_block_apply<kPreAdd><<<grid, 256, 0, s>>>(pv);
+    else
+        k_block_apply<kPreFromB><<<grid, 256, 0, s>>>(pv);
     return hipGetLastError();
 }
 

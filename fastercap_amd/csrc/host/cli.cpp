@@ -29,6 +29,7 @@ struct Options {
     double mesh_eps = 0.1, eps_ratio = 1.0, curv = 3.0, tol = 0.01, auto_err = 0.01;
     bool autom = false, verbose = false, auto_precond = false;
     int precond = FCG_PRECOND_JACOBI;
+    int block_size = 32;  // -pb<n> (SolverGlobal.cpp:60)
     int precision = FCG_PREC_FP64;
     int device = 0;
 };
@@ -37,12 +38,13 @@ void usage() {
     std::printf(
         "Usage: fastercap <input file> [options]\n"
         "  -a<err>  Automatically refine, stop when the relative C-matrix change is below err (default 0.01)\n"
-        "  -ap      Automatic preconditioner choice (only Jacobi is available here)\n"
+        "  -ap      Automatic preconditioner choice (Jacobi here: the two-level one is not built)\n"
         "  -m<eps>  Mesh relative refinement value (default 0.1)\n"
         "  -mc<c>   Mesh curvature coefficient (default 3)\n"
         "  -t<tol>  GMRES iteration tolerance (default 0.01)\n"
         "  -d<r>    Direct potential interaction coefficient to mesh refinement ratio (default 1)\n"
         "  -pj      Jacobi preconditioner (default)\n"
+        "  -pb<n>   Block preconditioner, blocks of at most n panels (2..128)\n"
         "  -pn      No preconditioner\n"
         "  -v       Verbose output\n"
         "  -b       Console mode (always)\n"
@@ -88,6 +90,23 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
             o.precond = FCG_PRECOND_JACOBI;
         } else if (!std::strcmp(a, "-pn")) {
             o.precond = FCG_PRECOND_NONE;
+        } else if (!std::strncmp(a, "-pb", 3)) {
+            // the reference adds the block preconditioner to the current choice and
+            // then runs a flexible GMRES that it reports as unsupported; here -pb
+            // selects the block preconditioner alone (as -pn -pb<n> there)
+            double v;
+            ok = num(a + 3, &v) && v >= 0 && v == (int)v;
+            if (ok) {
+                o.precond = FCG_PRECOND_BLOCK;
+                o.block_size = (int)v;
+                if (o.block_size > 128) {
+                    std::fprintf(stderr, "Warning: maximum block preconditioner dimension is 128, using 128\n");
+                    o.block_size = 128;
+                } else if (o.block_size < 2) {
+                    std::fprintf(stderr, "Warning: minimum block preconditioner dimension is 2, using 2\n");
+                    o.block_size = 2;
+                }
+            }
         } else if (!std::strcmp(a, "-gmixed")) {
             o.precision = FCG_PREC_MIXED;
         } else if (!std::strncmp(a, "-gdev", 5)) {
@@ -155,7 +174,7 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
     rc = fcg_job_upload(ctx, &d, &job);
     if (rc == FCG_OK) rc = fcg_fill(ctx, job, o.precision);
     if (rc == FCG_OK) {
-        fcg_solve_opts so{tol, 1000, o.precond, o.precision, 8};
+        fcg_solve_opts so{tol, 1000, o.precond, o.precision, 8, o.block_size};
         const int nc = (int)r.names.size();
         r.cap.assign((size_t)nc * nc, 0.0);
         rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, nullptr, &r.ss);
@@ -226,7 +245,7 @@ int main(int argc, char** argv) {
         o.curv = 1.0;
     }
     if (o.auto_precond)
-        std::fprintf(stderr, "Warning: automatic preconditioner requested; only Jacobi is available, using Jacobi\n");
+        std::fprintf(stderr, "Warning: automatic preconditioner requested; the two-level preconditioner is not built, using Jacobi\n");
     std::printf("Running fastercap (gfx950 solve path)\nInput file: %s\n", o.file.c_str());
     fcg_ctx* ctx = nullptr;
     if (fcg_ctx_create(o.device, &ctx) != FCG_OK) {

@@ -48,6 +48,7 @@ extern "C" {
 /* Preconditioner bit flags: values of AUTOREFINE_PRECOND_* (Solver/SolverGlobal.h:59-68). */
 #define FCG_PRECOND_NONE 0
 #define FCG_PRECOND_JACOBI 1
+#define FCG_PRECOND_BLOCK 2 /* block preconditioner (-pb<n>), fcg_solve_opts.block_size */
 
 /*
  * Geometry of one tree node (leaf panel, refined parent or super node), 96 bytes.
@@ -101,6 +102,7 @@ typedef struct fcg_solve_opts {
     int32_t precond;   /* FCG_PRECOND_* */
     int32_t precision; /* FCG_PREC_FP64 or FCG_PREC_MIXED */
     int32_t max_refine;/* outer FP64 refinement sweeps in FCG_PREC_MIXED */
+    int32_t block_size;/* FCG_PRECOND_BLOCK: most leaves per block (-pb<n>); 0 = 32, clamped to 2..128 */
 } fcg_solve_opts;
 
 typedef struct fcg_solve_stats {
@@ -156,6 +158,17 @@ int fcg_mvp_prec(fcg_ctx* ctx, int job, int mvp_prec, const double* q, double* v
 /* Pq = P^-1 q (Jacobi) for k vectors, host buffers.
  * Replaces CSolveCap::ComputePrecondVectFast (Solver/SolveCapacitance.cpp:3458-3526). */
 int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k);
+
+/* Pq = B q for k vectors (host buffers) with the block preconditioner B of block
+ * size block_size (0 = 32, clamped to 2..128): blocks of consecutive leaves, each
+ * the inverse of its dense collocation matrix.  Built on first use (the job must
+ * be filled) and kept for later calls and solves.
+ * Replaces CSolveCap::RecurseComputePrecond, ComputeBlockPrecond, InvertMatrix
+ * (Solver/SolveCapacitance.cpp:1094-1104, 1876-2258) and the block branch of
+ * ComputePrecondVectFast (:3527-3544).  A leaf met before any block is open (its
+ * parent has more than block_size leaves) is a block of its own; the reference
+ * opens a block there that it never computes. */
+int fcg_block_precond_apply(fcg_ctx* ctx, int job, int block_size, const double* q, double* pq, int k);
 
 /* Solve every conductor right-hand side of the job with right-preconditioned GMRES
  * and assemble the Maxwell capacitance matrix cap[rows][n_cond] (row-major; rows =

@@ -8,6 +8,7 @@
 
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -362,11 +363,24 @@ int orc_mvp(const fcg_job_desc* d, const double* coef, const double* self_coef, 
     return 0;
 }
 
-// gmresPrecondSFastAll (SolveCapacitance.cpp:2288-2574), Jacobi or no preconditioner.
+// Block preconditioner data (see orc_block_precond).
+struct BlockPre {
+    int nblocks = 0;
+    std::vector<int> starts;
+    std::vector<long long> poff;
+    std::vector<double> P;
+};
+
+// gmresPrecondSFastAll (SolveCapacitance.cpp:2288-2574): no preconditioner, Jacobi,
+// or the block preconditioner when bp is given.
 static int gmres(const fcg_job_desc* d, const double* coef, const double* self_coef, const std::vector<double>& b,
-                 std::vector<double>& x, double tol, int precond, int nthreads) {
+                 std::vector<double>& x, double tol, int precond, int nthreads, const BlockPre* bp = nullptr) {
     const long n = d->n_leaves;
     auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
+        if (bp) {
+            orc_block_apply(bp->nblocks, bp->starts.data(), bp->poff.data(), bp->P.data(), in.data(), out.data());
+            return;
+        }
         for (long i = 0; i < n; ++i) out[i] = precond ? in[i] / self_coef[i] : in[i];
     };
     auto nrm = [&](const std::vector<double>& a) {
@@ -463,8 +477,30 @@ int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef
     return orc_solve_rows(d, coef, self_coef, tol, precond, -1, cap, rows, cols, iters, nthreads);
 }
 
+static int solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
+                      int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads,
+                      const BlockPre* bp);
+
 int orc_solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
                    int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads) {
+    return solve_rows(d, coef, self_coef, tol, precond, max_rows, cap, rows, cols, iters, nthreads, nullptr);
+}
+
+int orc_solve_block(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int bsize,
+                    double* cap, int* rows, int* cols, int* iters, int nthreads) {
+    BlockPre bp;
+    const long n = d->n_leaves;
+    bp.starts.assign((size_t)n + 1, 0);
+    bp.poff.assign((size_t)n + 1, 0);
+    bp.P.assign((size_t)n * (size_t)(bsize < 2 ? 2 : bsize), 0.0);
+    const int rc = orc_block_precond(d, bsize, bp.starts.data(), &bp.nblocks, bp.poff.data(), bp.P.data());
+    if (rc) return rc;
+    return solve_rows(d, coef, self_coef, tol, 2, -1, cap, rows, cols, iters, nthreads, &bp);
+}
+
+static int solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
+                      int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads,
+                      const BlockPre* bp) {
     const long n = d->n_leaves;
     int ncond = 0;
     for (int g = 0; g < d->n_groups; ++g)
@@ -480,7 +516,7 @@ int orc_solve_rows(const fcg_job_desc* d, const double* coef, const double* self
         if (d->dim == 2 && hi == n) continue;  // last conductor row is the balance row (SolveCapacitance.cpp:1233-1240)
         if (max_rows >= 0 && row >= max_rows) break;
         for (int k = lo; k < hi; ++k) b[k] = 1.0;
-        const int it = gmres(d, coef, self_coef, b, x, tol, precond, nthreads);
+        const int it = gmres(d, coef, self_coef, b, x, tol, precond, nthreads, bp);
         if (iters) iters[row] = it;
         int col = 0;
         for (int g2 = 0; g2 < d->n_groups; ++g2) {
@@ -539,3 +575,121 @@ long long orc_link_coverage(const fcg_job_desc* d) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- block preconditioner
+// RecurseComputePrecond (Solver/SolveCapacitance.cpp:1094-1104, 1876-1993): each
+// conductor's tree is walked depth first, left before right, conductors in leaf
+// order; the first subtree met with at most bsize leaves forms one block of
+// consecutive leaves.  A leaf met before any block is open (its parent has more
+// than bsize leaves) forms a block of one: the reference opens a block there but
+// never computes it.
+extern "C" int orc_block_partition(const fcg_job_desc* d, int bsize, int* starts, int* nblocks) {
+    if (bsize < 2) bsize = 2;
+    if (bsize > 128) bsize = 128;
+    std::vector<int> roots;
+    for (int i = 0; i < d->n_nodes; ++i)
+        if (d->parent[i] < 0) roots.push_back(i);
+    std::sort(roots.begin(), roots.end(), [&](int a, int b) { return d->leaf_lo[a] < d->leaf_lo[b]; });
+    int nb = 0;
+    std::vector<int> stack;
+    for (int r : roots) {
+        stack.push_back(r);
+        while (!stack.empty()) {
+            const int a = stack.back();
+            stack.pop_back();
+            const int cnt = d->leaf_hi[a] - d->leaf_lo[a];
+            if (cnt <= bsize || d->left[a] < 0) {
+                starts[nb++] = d->leaf_lo[a];
+                continue;
+            }
+            stack.push_back(d->right[a]);  // left is visited first
+            stack.push_back(d->left[a]);
+        }
+    }
+    starts[nb] = d->n_leaves;
+    *nblocks = nb;
+    return 0;
+}
+
+// InvertMatrix (Solver/SolveCapacitance.cpp:2116-2258): right-looking LU with
+// partial pivoting (first largest |entry| of the column), then the inverse column
+// by column from forward and backward substitution of the unit vectors.
+static void invert_block(std::vector<double>& A, int n, double* inv) {
+    std::vector<int> piv((size_t)n);
+    for (int j = 0; j < n; ++j) {
+        int p = j;
+        double best = std::fabs(A[(size_t)j * n + j]);
+        for (int i = j + 1; i < n; ++i)
+            if (std::fabs(A[(size_t)i * n + j]) > best) {
+                best = std::fabs(A[(size_t)i * n + j]);
+                p = i;
+            }
+        piv[(size_t)j] = p;
+        if (p != j)
+            for (int c = 0; c < n; ++c) std::swap(A[(size_t)j * n + c], A[(size_t)p * n + c]);
+        if (j < n - 1) {
+            const double rcp = 1.0 / A[(size_t)j * n + j];
+            for (int i = j + 1; i < n; ++i) A[(size_t)i * n + j] *= rcp;
+            for (int i = j + 1; i < n; ++i)
+                for (int c = j + 1; c < n; ++c) A[(size_t)i * n + c] -= A[(size_t)i * n + j] * A[(size_t)j * n + c];
+        }
+    }
+    std::vector<double> y((size_t)n);
+    for (int k = 0; k < n; ++k) {
+        for (int i = 0; i < n; ++i) y[(size_t)i] = (i == k) ? 1.0 : 0.0;
+        for (int i = 0; i < n; ++i) {  // L (unit diagonal), pivots applied in order
+            const int ip = piv[(size_t)i];
+            double sum = y[(size_t)ip];
+            y[(size_t)ip] = y[(size_t)i];
+            for (int j = 0; j < i; ++j) sum -= A[(size_t)i * n + j] * y[(size_t)j];
+            y[(size_t)i] = sum;
+        }
+        for (int i = n - 1; i >= 0; --i) {  // U
+            double sum = y[(size_t)i];
+            for (int j = i + 1; j < n; ++j) sum -= A[(size_t)i * n + j] * y[(size_t)j];
+            y[(size_t)i] = sum / A[(size_t)i * n + i];
+        }
+        for (int i = 0; i < n; ++i) inv[(size_t)i * n + k] = y[(size_t)i];
+    }
+}
+
+// ComputeBlockPrecond (Solver/SolveCapacitance.cpp:2068-2113): the block's dense
+// collocation matrix (self terms on the diagonal, PotEstimateOpt between leaves
+// elsewhere; the reference's two-sided call returns the two one-sided values),
+// inverted.
+extern "C" int orc_block_precond(const fcg_job_desc* d, int bsize, int* starts, int* nblocks, long long* poff,
+                                 double* P) {
+    orc_block_partition(d, bsize, starts, nblocks);
+    const int nb = *nblocks;
+    poff[0] = 0;
+    for (int b = 0; b < nb; ++b) {
+        const long long n = starts[b + 1] - starts[b];
+        poff[b + 1] = poff[b] + n * n;
+    }
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int b = 0; b < nb; ++b) {
+        const int b0 = starts[b], n = starts[b + 1] - starts[b];
+        std::vector<double> A((size_t)n * n);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j)
+                A[(size_t)i * n + j] = (i == j) ? orc_self(d, b0 + i)
+                                                : orc_pot_estimate(d, d->leaf_node[b0 + i], d->leaf_node[b0 + j]);
+        invert_block(A, n, P + poff[b]);
+    }
+    return 0;
+}
+
+// ComputePrecondVectFast block branch (Solver/SolveCapacitance.cpp:3527-3544).
+extern "C" void orc_block_apply(int nblocks, const int* starts, const long long* poff, const double* P,
+                                const double* q, double* out) {
+    for (int b = 0; b < nblocks; ++b) {
+        const int b0 = starts[b], n = starts[b + 1] - starts[b];
+        const double* Pb = P + poff[b];
+        for (int i = 0; i < n; ++i) {
+            double s = 0.0;
+            for (int j = 0; j < n; ++j) s += Pb[(size_t)i * n + j] * q[b0 + j];
+            out[b0 + i] = s;
+        }
+    }
+}
+

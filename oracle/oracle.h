@@ -24,6 +24,13 @@
  *                          hierarchical approximation), for closed-form checks
  *   orc_link_coverage      link-partition invariant: every ordered leaf pair i != j is covered
  *                          by exactly one link
+ *   orc_block_partition    leaf blocks of the block preconditioner (RecurseComputePrecond,
+ *                          Solver/SolveCapacitance.cpp:1094-1104, 1876-1993)
+ *   orc_block_precond      each block's collocation matrix inverted (ComputeBlockPrecond,
+ *                          InvertMatrix, :2068-2258)
+ *   orc_block_apply        Pq = block-diagonal product (ComputePrecondVectFast block branch,
+ *                          :3527-3544)
+ *   orc_solve_block        orc_solve with the block preconditioner
  */
 #ifndef FASTERCAP_ORACLE_H
 #define FASTERCAP_ORACLE_H
@@ -70,6 +77,18 @@ int orc_leaf_dense(const fcg_job_desc* d, double* P, int nthreads);
 /* Returns the number of ordered leaf pairs whose link cover count differs from the
  * expected value (1 for i != j, 0 for i == j); 0 means the links partition the matrix. */
 long long orc_link_coverage(const fcg_job_desc* d);
+
+/* Block preconditioner of block size bsize (2..128).  starts[0..nblocks] are the
+ * blocks' first leaves (capacity n_leaves + 1); P holds each block's inverse,
+ * row-major, block after block, at poff[b] (capacity n_leaves * bsize doubles,
+ * nblocks + 1 offsets).  A leaf reached before any block is open forms a block of
+ * its own (the reference leaves that block unset). */
+int orc_block_partition(const fcg_job_desc* d, int bsize, int* starts, int* nblocks);
+int orc_block_precond(const fcg_job_desc* d, int bsize, int* starts, int* nblocks, long long* poff, double* P);
+void orc_block_apply(int nblocks, const int* starts, const long long* poff, const double* P, const double* q,
+                     double* out);
+int orc_solve_block(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int bsize,
+                    double* cap, int* rows, int* cols, int* iters, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -47,6 +47,13 @@ def lib():
         L.orc_leaf_dense.argtypes = [jd, dp, C.c_int]
         L.orc_link_coverage.argtypes = [jd]
         L.orc_link_coverage.restype = C.c_longlong
+        ip = C.POINTER(C.c_int)
+        llp = C.POINTER(C.c_longlong)
+        L.orc_block_partition.argtypes = [jd, C.c_int, ip, ip]
+        L.orc_block_precond.argtypes = [jd, C.c_int, ip, ip, llp, dp]
+        L.orc_block_apply.argtypes = [C.c_int, ip, llp, dp, dp, dp]
+        L.orc_block_apply.restype = None
+        L.orc_solve_block.argtypes = [jd, dp, dp, C.c_double, C.c_int, dp, ip, ip, ip, C.c_int]
         _lib = L
     return _lib
 
@@ -144,3 +151,50 @@ def link_coverage_errors(mesh):
 
 def pot_estimate(mesh, i, j):
     return lib().orc_pot_estimate(C.byref(mesh.desc), i, j)
+
+
+class BlockPrecond:
+    """Block preconditioner of one mesh: blocks[b] = leaf range [starts[b],
+    starts[b+1]), inverse P[b] (n x n, row-major)."""
+
+    def __init__(self, mesh, bsize):
+        n = mesh.N
+        self.starts = np.zeros(n + 1, dtype=np.int32)
+        self.poff = np.zeros(n + 1, dtype=np.int64)
+        self.P = np.zeros(n * max(2, min(128, bsize)), dtype=np.float64)
+        nb = C.c_int()
+        lib().orc_block_precond(C.byref(mesh.desc), bsize, self.starts.ctypes.data_as(C.POINTER(C.c_int)),
+                                C.byref(nb), self.poff.ctypes.data_as(C.POINTER(C.c_longlong)), _d(self.P))
+        self.nblocks = nb.value
+        self.starts = self.starts[: self.nblocks + 1].copy()
+        self.poff = self.poff[: self.nblocks + 1].copy()
+
+    def block(self, b):
+        n = int(self.starts[b + 1] - self.starts[b])
+        return self.P[self.poff[b]: self.poff[b] + n * n].reshape(n, n)
+
+    def apply(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty_like(q)
+        lib().orc_block_apply(self.nblocks, self.starts.ctypes.data_as(C.POINTER(C.c_int)),
+                              self.poff.ctypes.data_as(C.POINTER(C.c_longlong)), _d(self.P), _d(q), _d(out))
+        return out
+
+
+def block_partition(mesh, bsize):
+    starts = np.zeros(mesh.N + 1, dtype=np.int32)
+    nb = C.c_int()
+    lib().orc_block_partition(C.byref(mesh.desc), bsize, starts.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nb))
+    return starts[: nb.value + 1].copy()
+
+
+def solve_block(mesh, coef, selfc, bsize, tol=1e-8, nthreads=0):
+    ncond = int((mesh.group_is_diel == 0).sum())
+    cap = np.zeros(ncond * ncond, dtype=np.float64)
+    iters = np.zeros(ncond, dtype=np.int32)
+    rows, cols = C.c_int(), C.c_int()
+    lib().orc_solve_block(C.byref(mesh.desc), _d(coef), _d(selfc), tol, bsize, _d(cap), C.byref(rows),
+                          C.byref(cols), iters.ctypes.data_as(C.POINTER(C.c_int)), nthreads)
+    r, c = rows.value, cols.value
+    return cap[: r * c].reshape(r, c), iters[:r]
+

@@ -19,11 +19,21 @@ def test_switches():
     assert cli.parse_args(["x", "-mc0.5"]).curv_coeff == 1.0  # clamped as the reference does
 
 
-@pytest.mark.parametrize("bad", [["x", "-g"], ["x", "-pb32"], ["x", "-mabc"], [], ["a", "b"]])
+@pytest.mark.parametrize("bad", [["x", "-g"], ["x", "-ps32"], ["x", "-pbx"], ["x", "-mabc"], [], ["a", "b"]])
 def test_errors(bad):
     with pytest.raises(cli.CommandLineError):
         cli.parse_args(bad)
     assert cli.main(bad) == cli.COMMAND_LINE_ERROR
+
+
+def test_block_preconditioner_switch():
+    a = cli.parse_args(["x", "-pb16"])
+    assert a.precond == 2 and a.block_size == 16
+    # the reference clamps the dimension to 2..128 with a warning
+    assert cli.parse_args(["x", "-pb500"]).block_size == 128
+    assert cli.parse_args(["x", "-pb1"]).block_size == 2
+    # a later -pj or -pn replaces it
+    assert cli.parse_args(["x", "-pb16", "-pj"]).precond == 1
 
 
 def test_matrix_printout():
