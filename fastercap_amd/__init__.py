@@ -30,6 +30,7 @@ from ._ffi import (
     FCG_PRECOND_BLOCK,
     FCG_PRECOND_JACOBI,
     FCG_PRECOND_NONE,
+    FCG_PRECOND_SUPER,
     HostOptions,
     HostStats,
     JobDesc,
@@ -53,6 +54,7 @@ __all__ = [
     "FCG_PRECOND_BLOCK",
     "FCG_PRECOND_JACOBI",
     "FCG_PRECOND_NONE",
+    "FCG_PRECOND_SUPER",
     "TWO_PI_TIMES_E0",
 ]
 
@@ -230,6 +232,17 @@ class Device:
                     "fcg_block_precond_apply")
         return v
 
+    def super_precond(self, job: int, q: np.ndarray, dim: int = 128) -> np.ndarray:
+        """S q with the job's two-level preconditioner of dimension dim; raises
+        FcgError when it is unavailable (at least dim conductor and dielectric
+        roots)."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        k = 1 if q.ndim == 1 else q.shape[0]
+        v = np.empty_like(q)
+        self._check(self.lib.fcg_super_precond_apply(self._h, job, dim, _dptr(q), _dptr(v), k),
+                    "fcg_super_precond_apply")
+        return v
+
     def solve(
         self,
         job: int,
@@ -240,14 +253,14 @@ class Device:
         precision: int = FCG_PREC_FP64,
         want_charges: bool = False,
         max_refine: int = 8,
-        block_size: int = 32,
+        precond_size: int = 0,
     ) -> SolveResult:
         ncond = int((mesh.group_is_diel == 0).sum())
         rows_max = ncond
         cap = np.zeros(rows_max * ncond, dtype=np.float64)
         charges = np.zeros(rows_max * mesh.N, dtype=np.float64) if want_charges else None
         o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine,
-                      block_size=block_size)
+                      precond_size=precond_size)
         st = SolveStats()
         rows, cols = C.c_int(), C.c_int()
         self._check(
@@ -283,7 +296,7 @@ class Device:
         precond: int = FCG_PRECOND_JACOBI,
         precision: int = FCG_PREC_FP64,
         max_refine: int = 8,
-        block_size: int = 32,
+        precond_size: int = 0,
     ):
         """Solve several filled jobs together (one launch per stage for all of
         them); returns one SolveResult per job, in order."""
@@ -295,7 +308,7 @@ class Device:
         cols = (C.c_int * len(jobs))()
         sts = (SolveStats * len(jobs))()
         o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine,
-                      block_size=block_size)
+                      precond_size=precond_size)
         self._check(
             self.lib.fcg_solve_batch(self._h, ja, len(jobs), C.byref(o), _dptr(caps), rows, cols, sts),
             "fcg_solve_batch",
@@ -331,9 +344,9 @@ class Device:
         return n.value
 
     def engine(self, tol: float = 0.01, max_iter: int = 1000, precond: int = FCG_PRECOND_JACOBI,
-               precision: int = FCG_PREC_FP64, max_refine: int = 8, block_size: int = 32) -> "Engine":
+               precision: int = FCG_PREC_FP64, max_refine: int = 8, precond_size: int = 0) -> "Engine":
         return Engine(self, SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision,
-                                      max_refine=max_refine, block_size=block_size))
+                                      max_refine=max_refine, precond_size=precond_size))
 
     def mvp_timing(self, job: int, k: int, reps: int = 20) -> dict:
         t = _ffi.MvpTiming()

@@ -55,7 +55,7 @@ class SolveOpts(C.Structure):
         ("precond", C.c_int32),
         ("precision", C.c_int32),
         ("max_refine", C.c_int32),
-        ("block_size", C.c_int32),
+        ("precond_size", C.c_int32),
     ]
 
 
@@ -136,6 +136,7 @@ FCG_PREC_MIXED = 1
 FCG_PRECOND_NONE = 0
 FCG_PRECOND_JACOBI = 1
 FCG_PRECOND_BLOCK = 2
+FCG_PRECOND_SUPER = 3
 FCG_MVP_FP64 = 0
 FCG_MVP_FP32 = 1
 FCG_MVP_FP32_ACC64 = 2
@@ -155,6 +156,7 @@ DEVICE_SYMBOLS = (
     "fcg_mvp_device",
     "fcg_precond_apply",
     "fcg_block_precond_apply",
+    "fcg_super_precond_apply",
     "fcg_solve",
     "fcg_job_conductors",
     "fcg_last_error",
@@ -240,6 +242,7 @@ def device_lib() -> C.CDLL:
         lib.fcg_mvp_device.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         lib.fcg_precond_apply.argtypes = [vp, C.c_int, dp, dp, C.c_int]
         lib.fcg_block_precond_apply.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]
+        lib.fcg_super_precond_apply.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]
         lib.fcg_solve.argtypes = [
             vp,
             C.c_int,

@@ -37,6 +37,8 @@ class Args:
     auto_precond: bool = False
     precond: int = 1  # FCG_PRECOND_JACOBI
     block_size: int = 32  # -pb<n> (SolverGlobal.cpp:60)
+    super_dim: int = 128  # -ps<n> (SolverGlobal.cpp:59)
+    jacobi_set: bool = False  # -pj given (kept only without -ps, SolveCapacitance.cpp:122-125)
     precision: int = 0  # FCG_PREC_FP64
     verbose: bool = False
     device: int = 0
@@ -66,9 +68,12 @@ def parse_args(argv) -> Args:
         elif opt == "-ap":
             a.auto_precond = True
         elif opt == "-pj":
-            a.precond = 1
+            if a.precond != 3:
+                a.precond = 1
+            a.jacobi_set = True
         elif opt == "-pn":
             a.precond = 0
+            a.jacobi_set = False
         elif opt.startswith("-pb"):
             # the reference adds the block preconditioner to the current choice and then
             # runs a flexible GMRES it reports as unsupported; here -pb selects the block
@@ -84,6 +89,20 @@ def parse_args(argv) -> Args:
             elif a.block_size < 2:
                 print("Warning: minimum block preconditioner dimension is 2, using 2", file=sys.stderr)
                 a.block_size = 2
+        elif opt.startswith("-ps"):
+            # the two-level preconditioner replaces Jacobi (SolveCapacitance.cpp:122-125)
+            # and, here, a block preconditioner given before it
+            v = _num(opt[3:], opt)
+            if v != int(v) or v < 0:
+                raise CommandLineError(f"bad preconditioner dimension '{opt[3:]}'")
+            a.precond = 3  # FCG_PRECOND_SUPER
+            a.super_dim = int(min(v, 1e9))
+            if a.super_dim > 4096:
+                print("Warning: maximum two-levels preconditioner dimension is 4096, using 4096", file=sys.stderr)
+                a.super_dim = 4096
+            elif a.super_dim < 16:
+                print("Warning: minimum two-levels preconditioner dimension is 16, using 16", file=sys.stderr)
+                a.super_dim = 16
         elif opt == "-gmixed":
             a.precision = 1
         elif opt.startswith("-gdev"):
@@ -102,6 +121,9 @@ def parse_args(argv) -> Args:
                 a.auto_err = _num(opt[2:], opt)
         else:
             raise CommandLineError(f"unsupported parameter '{opt}' (this build covers the solve path only)")
+    if a.precond == 3 and a.jacobi_set:
+        print("Warning: both two-levels preconditioner and Jacobi preconditioner specified, keeping only the "
+              "two-levels preconditioner", file=sys.stderr)
     if not a.file and not a.stop:
         raise CommandLineError("no input file")
     if a.curv_coeff < 1.0:
@@ -124,6 +146,23 @@ def format_cap(cap: np.ndarray, names, dim: int, verbose: bool = False) -> str:
     return "\n".join(lines) + "\n"
 
 
+def auto_preconditioner(mutual_links: int, conductors: int):
+    """(precond, size) of CSolveCap::AutoSetPrecondType (Solver/SolveCapacitance.cpp:
+    580-608): from thousands of mutual links times conductors, Jacobi up to 500,
+    then the two-level preconditioner of dimension 128 / 512 / 1024 above 500 /
+    5000 / 10000."""
+    lx = (mutual_links // 1000) * conductors
+    if lx <= 500:
+        return 1, 128
+    return 3, (128 if lx <= 5000 else 512 if lx <= 10000 else 1024)
+
+
+def preconditioner(a: Args, mesh):
+    if a.auto_precond:
+        return auto_preconditioner(int(mesh.stats.links), len(mesh.conductor_names))
+    return a.precond, (a.block_size if a.precond == 2 else a.super_dim)
+
+
 def solve_file(dev, a: Args, mesh_eps: float, tol: float):
     from . import Mesh
 
@@ -131,7 +170,8 @@ def solve_file(dev, a: Args, mesh_eps: float, tol: float):
     job = dev.upload(mesh)
     try:
         dev.fill(job, a.precision)
-        res = dev.solve(job, mesh, tol=tol, precond=a.precond, precision=a.precision, block_size=a.block_size)
+        precond, size = preconditioner(a, mesh)
+        res = dev.solve(job, mesh, tol=tol, precond=precond, precision=a.precision, precond_size=size)
     finally:
         dev.release(job)
     return mesh, res

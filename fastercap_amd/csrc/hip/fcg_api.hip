@@ -228,6 +228,12 @@ struct Job {
         int size = 0, nb = 0;
         DevBuf bstart, boff, P;
     } bpre;
+    // two-level preconditioner of the last dimension asked for (0: none yet)
+    struct SuperPre {
+        int dim = 0, m = 0;
+        bool avail = false;
+        DevBuf snode, sstart, sidx, Sinv, w;
+    } spre;
     std::vector<uint32_t> link_pos;
     bool filled64 = false, filled32 = false;
     std::vector<int32_t> group_is_diel, group_leaf_start;
@@ -245,7 +251,8 @@ struct Job {
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
-                         col16.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes +
+                         col16.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
+                         spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
                          scr.bytes());
     }
 
@@ -823,6 +830,131 @@ int make_pview(fcg_ctx* ctx, Tables& tabs, const std::vector<PEnt>& ents, fcg::P
     return FCG_OK;
 }
 
+// Two-level preconditioner elements (MakePreconditioner, Solver/SolveCapacitance.cpp:
+// 1067-1104, and the super branch of RecurseComputePrecond, :1876-1993): with R
+// tree roots and dimension dim, every root's tree is walked to level
+// L = log2(dim / R) (integer division; root level 0); a non-leaf at level L is
+// one element standing for its leaves, a leaf at level <= L is one.  Elements
+// follow leaf order.  Unavailable (false) when R >= dim.
+bool super_partition(const Job& j, int dim, std::vector<int32_t>& nodes, std::vector<int32_t>& starts) {
+    std::vector<int32_t> roots;
+    for (int i = 0; i < j.M; ++i)
+        if (j.h_parent[(size_t)i] < 0) roots.push_back(i);
+    std::sort(roots.begin(), roots.end(),
+              [&](int a, int b) { return j.h_leaf_lo[(size_t)a] < j.h_leaf_lo[(size_t)b]; });
+    nodes.clear();
+    starts.clear();
+    const unsigned R = (unsigned)roots.size();
+    if (R >= (unsigned)dim) return false;
+    const int L = (int)(unsigned)(std::log((double)((unsigned)dim / R)) / std::log(2.0));
+    std::vector<std::pair<int, int>> stack;
+    for (int r : roots) {
+        stack.push_back({r, 0});
+        while (!stack.empty()) {
+            const auto [a, lev] = stack.back();
+            stack.pop_back();
+            if (j.h_left[(size_t)a] < 0 || lev == L) {
+                nodes.push_back(a);
+                starts.push_back(j.h_leaf_lo[(size_t)a]);
+                continue;
+            }
+            stack.push_back({j.h_right[(size_t)a], lev + 1});
+            stack.push_back({j.h_left[(size_t)a], lev + 1});
+        }
+    }
+    starts.push_back(j.N);
+    return true;
+}
+
+int clamp_super_dim(int d) { return d <= 0 ? 128 : std::min(std::max(d, 16), 4096); }
+
+// the job's two-level preconditioner of dimension dim (built on first use; needs
+// a fill)
+int ensure_super_precond(fcg_ctx* ctx, Job& j, int dim) {
+    dim = clamp_super_dim(dim);
+    if (j.spre.dim == dim) return FCG_OK;
+    if (!j.filled64 && !j.filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    hipStream_t s = ctx->stream;
+    std::vector<int32_t> nodes, starts;
+    const bool avail = super_partition(j, dim, nodes, starts);
+    HIPCHECK(ctx, hipStreamSynchronize(s));  // the buffers below may be in use; the table arena is reset
+    j.spre.dim = 0;
+    j.spre.avail = avail;
+    j.spre.m = (int)nodes.size();
+    if (avail) {
+        const int m = j.spre.m;
+        std::vector<int32_t> sidx((size_t)j.N);
+        for (int e = 0; e < m; ++e)
+            for (int i = starts[(size_t)e]; i < starts[(size_t)e + 1]; ++i) sidx[(size_t)i] = e;
+        HIPCHECK(ctx, upload(j.spre.snode, nodes.data(), nodes.size(), s));
+        HIPCHECK(ctx, upload(j.spre.sstart, starts.data(), starts.size(), s));
+        HIPCHECK(ctx, upload(j.spre.sidx, sidx.data(), sidx.size(), s));
+        HIPCHECK(ctx, j.spre.Sinv.alloc(sizeof(double) * (size_t)m * m));
+        HIPCHECK(ctx, j.spre.w.alloc(sizeof(double) * (size_t)j.N));
+        DevBuf A, piv;
+        HIPCHECK(ctx, A.alloc(sizeof(double) * (size_t)m * m));
+        HIPCHECK(ctx, piv.alloc(sizeof(int) * (size_t)m));
+        ctx->tabs.reset();
+        BatchView bv;
+        RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, nullptr, nullptr}}, &bv));
+        HIPCHECK(ctx, fcg::launch_super_setup(bv, m, j.N, j.spre.snode.as<int32_t>(), j.spre.sidx.as<int32_t>(),
+                                              A.as<double>(), piv.as<int>(), j.spre.Sinv.as<double>(),
+                                              j.spre.w.as<double>(), s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));  // A, piv and the host vectors are released below
+    }
+    j.spre.dim = dim;
+    return FCG_OK;
+}
+
+// Apply table of the two-level preconditioner over several solves.
+struct SEnt {
+    const Job* j;
+    const void* in;
+    bool in32;
+    double* out;
+    int k;
+    double* scratch;  // [2][k][m]
+};
+int make_sview(fcg_ctx* ctx, Tables& tabs, const std::vector<SEnt>& ents, fcg::SView* sv) {
+    std::memset(sv, 0, sizeof *sv);
+    if (ents.empty()) return FCG_OK;
+    std::vector<fcg::SJob> sj;
+    std::vector<int64_t> off(2 * (ents.size() + 1), 0);
+    int64_t* eo = off.data();
+    int64_t* lo = eo + ents.size() + 1;
+    for (size_t i = 0; i < ents.size(); ++i) {
+        const SEnt& e = ents[i];
+        const Job::SuperPre& sp = e.j->spre;
+        fcg::SJob r{};
+        r.in = e.in;
+        r.out = e.out;
+        r.brange = e.j->rhs_range.as<int32_t>();
+        r.sstart = sp.sstart.as<int32_t>();
+        r.sidx = sp.sidx.as<int32_t>();
+        r.w = sp.w.as<double>();
+        r.Sinv = sp.Sinv.as<double>();
+        r.pot = e.scratch;
+        r.chg = e.scratch + (size_t)e.k * sp.m;
+        r.N = e.j->N;
+        r.k = e.k;
+        r.m = sp.m;
+        r.in32 = e.in32 ? 1 : 0;
+        sj.push_back(r);
+        eo[i + 1] = eo[i] + (int64_t)e.k * sp.m;
+        lo[i + 1] = lo[i] + (int64_t)e.k * e.j->N;
+    }
+    void *dj, *doff;
+    HIPCHECK(ctx, tabs.push(sj.data(), sizeof(fcg::SJob) * sj.size(), ctx->stream, &dj));
+    HIPCHECK(ctx, tabs.push(off.data(), sizeof(int64_t) * off.size(), ctx->stream, &doff));
+    sv->jobs = reinterpret_cast<const fcg::SJob*>(dj);
+    sv->njobs = (int)ents.size();
+    sv->el_off = reinterpret_cast<const int64_t*>(doff);
+    sv->lf_off = reinterpret_cast<const int64_t*>(doff) + ents.size() + 1;
+    sv->nel = eo[ents.size()];
+    sv->nlf = lo[ents.size()];
+    return FCG_OK;
+}
+
 // One stand-alone product on device buffers of job j (batch of one).
 int run_mvp(fcg_ctx* ctx, Job& j, int prec, const double* dq, double* dv, int k, int jacobi, hipStream_t s,
             hipEvent_t* ev = nullptr) {
@@ -853,8 +985,9 @@ struct Solve {
     int64_t tag = 0;
     int k = 0, N = 0, ncol = 0;
     bool mixed = false, jac = true, want_x = false;
-    bool block = false;  // block preconditioner: P applied by its own kernel, u = P q scratch
-    PBuf u;
+    bool block = false;   // block preconditioner: P applied by its own kernel, u = P q scratch
+    bool twolev = false;  // two-level preconditioner, likewise; spw = its [2][k][m] scratch
+    PBuf u, spw;
     PBuf x, r, w;
     Scratch scr;
     // Krylov basis: blocks of vectors and their addresses (sent with each step's tables)
@@ -878,7 +1011,7 @@ struct Solve {
     clk::time_point t0;
     int hoff = 0;
     int64_t bytes() const {
-        int64_t s = (int64_t)(x.r + r.r + w.r + u.r + scr.bytes());
+        int64_t s = (int64_t)(x.r + r.r + w.r + u.r + spw.r + scr.bytes());
         for (auto& q : qblocks) s += (int64_t)q->r;
         return s;
     }
@@ -977,7 +1110,7 @@ KJob kjob_of(Solve& S, void* const* qtab, double* w, int nv, int zoff) {
     J.zoff = zoff;
     J.nz = S.nz;
     J.jacobi = S.jac ? 1 : 0;
-    J.y = S.block ? S.u.as<double>() : nullptr;
+    J.y = (S.block || S.twolev) ? S.u.as<double>() : nullptr;
     return J;
 }
 
@@ -1145,6 +1278,12 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->b32 = mixed;
     S->jac = E->opts.precond == FCG_PRECOND_JACOBI;
     S->block = E->opts.precond == FCG_PRECOND_BLOCK;
+    if (E->opts.precond == FCG_PRECOND_SUPER) {
+        RCCHECK(ensure_super_precond(ctx, *J, E->opts.precond_size));
+        // unavailable (at least dim roots): the reference drops it and solves
+        // unpreconditioned
+        S->twolev = J->spre.avail;
+    }
     S->want_x = E->want_x;
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
     RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, S->k, mixed));
@@ -1153,8 +1292,12 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     HIPCHECK(ctx, S->r.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->w.get(&ctx->pool, sizeof(double) * vec));
     if (S->block) {
-        RCCHECK(ensure_block_precond(ctx, *J, E->opts.block_size));
+        RCCHECK(ensure_block_precond(ctx, *J, E->opts.precond_size));
         HIPCHECK(ctx, S->u.get(&ctx->pool, sizeof(double) * vec));
+    }
+    if (S->twolev) {
+        HIPCHECK(ctx, S->u.get(&ctx->pool, sizeof(double) * vec));
+        HIPCHECK(ctx, S->spw.get(&ctx->pool, sizeof(double) * 2 * (size_t)S->k * J->spre.m));
     }
     // unit potential on each conductor's panels (SolveCapacitance.cpp:1209-1211); the
     // kernels form b from the job's leaf ranges, so only its norm is kept here
@@ -1212,6 +1355,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         }
         std::vector<KEnt> up;
         std::vector<PEnt> upp;  // block preconditioner: x += P (Q z)
+        std::vector<SEnt> ups;  // two-level preconditioner, likewise
         std::vector<double> zall;
         for (auto& u : C.active) {
             Solve& S = *u;
@@ -1219,6 +1363,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             up.push_back(KEnt{&S, S.w.as<double>(), 0, (int)zall.size()});
             zall.insert(zall.end(), S.z.begin(), S.z.end());
             if (S.block) upp.push_back(PEnt{S.J, S.u.as<double>(), false, S.x.as<double>(), S.k});
+            if (S.twolev) ups.push_back(SEnt{S.J, S.u.as<double>(), false, S.x.as<double>(), S.k, S.spw.as<double>()});
         }
         if (!up.empty()) {
             KrylovView kv;
@@ -1231,6 +1376,11 @@ int engine_launch(fcg_engine* E, Cohort& C) {
                 RCCHECK(make_pview(ctx, C.tabs, upp, &pv));
                 q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreAdd, s); });
             }
+            if (!ups.empty()) {
+                fcg::SView sv;
+                RCCHECK(make_sview(ctx, C.tabs, ups, &sv));
+                q.push_back([=] { return fcg::launch_super_apply(sv, fcg::kPreAdd, s); });
+            }
             for (auto& e : up) {
                 e.S->nz = 0;
                 e.S->phase = e.S->mixed ? kResid : kFinish;
@@ -1238,10 +1388,13 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         }
         std::vector<KEnt> init;
         std::vector<PEnt> initp;  // block preconditioner: x0 = P b
+        std::vector<SEnt> inits;  // two-level preconditioner, likewise
         for (auto& u : C.active) {
             if (u->phase != kInit) continue;
             if (u->block)
                 initp.push_back(PEnt{u->J, nullptr, false, u->x.as<double>(), u->k});
+            else if (u->twolev)
+                inits.push_back(SEnt{u->J, nullptr, false, u->x.as<double>(), u->k, u->spw.as<double>()});
             else
                 init.push_back(KEnt{u.get(), u->w.as<double>(), 0, 0});
             u->phase = kResid;
@@ -1255,6 +1408,11 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             fcg::PView pv;
             RCCHECK(make_pview(ctx, C.tabs, initp, &pv));
             q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreFromB, s); });
+        }
+        if (!inits.empty()) {
+            fcg::SView sv;
+            RCCHECK(make_sview(ctx, C.tabs, inits, &sv));
+            q.push_back([=] { return fcg::launch_super_apply(sv, fcg::kPreFromB, s); });
         }
     }
     // finishing solves: conductor charge sums (SolveCapacitance.cpp:1305-1366), one
@@ -1345,11 +1503,15 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         std::vector<BatchEntry> be;
         std::vector<KEnt> ke;
         std::vector<PEnt> pe;  // block preconditioner: the product runs on u = P q_i
+        std::vector<SEnt> se;  // two-level preconditioner, likewise
         for (Solve* S : arn) {
             RCCHECK(ensure_basis(ctx, *S, S->i + 1));
             void* qi = S->hq[(size_t)S->i];
-            if (S->block) {
-                pe.push_back(PEnt{S->J, qi, S->b32, S->u.as<double>(), S->k});
+            if (S->block || S->twolev) {
+                if (S->block)
+                    pe.push_back(PEnt{S->J, qi, S->b32, S->u.as<double>(), S->k});
+                else
+                    se.push_back(SEnt{S->J, qi, S->b32, S->u.as<double>(), S->k, S->spw.as<double>()});
                 be.push_back(BatchEntry{S->J, &S->scr, S->u.as<double>(), S->w.as<double>(), nullptr});
             } else {
                 be.push_back(BatchEntry{S->J, &S->scr, S->b32 ? nullptr : static_cast<const double*>(qi),
@@ -1362,6 +1524,11 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             fcg::PView pv;
             RCCHECK(make_pview(ctx, C.tabs, pe, &pv));
             q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreSet, s); });
+        }
+        if (!se.empty()) {
+            fcg::SView sv;
+            RCCHECK(make_sview(ctx, C.tabs, se, &sv));
+            q.push_back([=] { return fcg::launch_super_apply(sv, fcg::kPreSet, s); });
         }
         BatchView bv;
         fcg::GatherPlan gp;
@@ -2074,9 +2241,32 @@ int fcg_block_precond_apply(fcg_ctx* ctx, int job, int block_size, const double*
     return FCG_OK;
 }
 
+int fcg_super_precond_apply(fcg_ctx* ctx, int job, int dim, const double* q, double* pq, int k) {
+    Job* j = get_job(ctx, job);
+    if (!j || !q || !pq || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    RCCHECK(ensure_super_precond(ctx, *j, dim));
+    if (!j->spre.avail)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "two-level preconditioner unavailable: at least as many roots as its dimension");
+    DevBuf dq, dp, sw;
+    const size_t n = (size_t)j->N * k;
+    HIPCHECK(ctx, upload(dq, q, n, ctx->stream));
+    HIPCHECK(ctx, dp.alloc(sizeof(double) * n));
+    HIPCHECK(ctx, sw.alloc(sizeof(double) * 2 * (size_t)k * j->spre.m));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));  // the table arena is reset below
+    ctx->tabs.reset();
+    fcg::SView sv;
+    RCCHECK(make_sview(ctx, ctx->tabs, {SEnt{j, dq.p, false, dp.as<double>(), k, sw.as<double>()}}, &sv));
+    HIPCHECK(ctx, fcg::launch_super_apply(sv, fcg::kPreSet, ctx->stream));
+    HIPCHECK(ctx, hipMemcpyAsync(pq, dp.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    return FCG_OK;
+}
+
 int fcg_engine_create(fcg_ctx* ctx, const fcg_solve_opts* opts, fcg_engine** out) {
     if (!ctx || !opts || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (opts->precond != FCG_PRECOND_NONE && opts->precond != FCG_PRECOND_JACOBI && opts->precond != FCG_PRECOND_BLOCK)
+    if (opts->precond != FCG_PRECOND_NONE && opts->precond != FCG_PRECOND_JACOBI && opts->precond != FCG_PRECOND_BLOCK &&
+        opts->precond != FCG_PRECOND_SUPER)
         return fail(ctx, FCG_INVALID_ARGUMENT, "unsupported preconditioner");
     if (opts->precision != FCG_PREC_FP64 && opts->precision != FCG_PREC_MIXED)
         return fail(ctx, FCG_INVALID_ARGUMENT, "unknown precision mode");

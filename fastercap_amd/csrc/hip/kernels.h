@@ -210,6 +210,34 @@ struct PView {
 enum PreMode { kPreSet = 0, kPreAdd = 1, kPreFromB = 2 };  // out = P in, out += P in, out = P b
 hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s);
 
+// Two-level ("super") preconditioner: m elements (tree nodes snode[m]) over
+// consecutive leaf ranges, the inverse Sinv of their m x m collocation matrix
+// (column-major), and per leaf its element sidx[i] and weight w[i] (leaf
+// dimension over element dimension).  Setup of one job: A (m x m scratch), piv
+// (m ints scratch).
+hipError_t launch_super_setup(const BatchView& one_job, int m, int N, const int32_t* snode, const int32_t* sidx,
+                              double* A, int* piv, double* Sinv, double* w, hipStream_t s);
+struct SJob {
+    const void* in;  // [k][N] FP64, or FP32 when in32
+    double* out;     // [k][N]
+    const int32_t* brange;
+    const int32_t* sstart;  // [m + 1]
+    const int32_t* sidx;    // [N]
+    const double* w;        // [N]
+    const double* Sinv;     // [m][m] column-major
+    double* pot;            // [k][m] scratch of this solve
+    double* chg;            // [k][m]
+    int N, k, m, in32;
+};
+struct SView {
+    const SJob* jobs;
+    int njobs;
+    const int64_t* el_off;  // cumulative k * m per solve
+    const int64_t* lf_off;  // cumulative k * N per solve
+    int64_t nel, nlf;
+};
+hipError_t launch_super_apply(const SView& sv, int mode, hipStream_t s);
+
 // x = P b (Jacobi) or b
 hipError_t launch_init_x(const KrylovView& kv, hipStream_t s);
 // r = b - w with |r|^2 chunk partials (the product A x is in w)

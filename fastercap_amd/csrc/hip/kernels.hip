@@ -162,26 +162,28 @@ __global__ void k_link_rows(BatchView bv) {
     for (int64_t l = J.row_ptr[i]; l < J.row_ptr[i + 1]; ++l) lr[l] = i;
 }
 
+// SelfPotential of any tree node (leaf, refined parent or super node)
+// (Solver/Autorefine.cpp:2358-2506)
+__device__ double node_self(const BJob& J, const fcg_node_geom& nd) {
+    if (nd.flags & FCG_NODE_DIEL) {
+        const double eb = J.gout[nd.cond], ea = J.gin[nd.cond];
+        if (J.dim == 3) return (ea + eb) / (FCG_TWO_TIMES_E0 * (eb - ea) * nd.area);
+        return FCG_PI * (ea + eb) / ((eb - ea) * nd.area);
+    }
+    if (J.dim == 3) {
+        const Vec3<double> c{(double)nd.c[0], (double)nd.c[1], (double)nd.c[2]};
+        return potential_opt_3d<double>(c, nd.v) / FCG_FOUR_PI_TIMES_E0;
+    }
+    return potential_opt_2d<double>((double)nd.c[0], (double)nd.c[1], nd.v);
+}
+
 __global__ void k_self(BatchView bv) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= bv.leaves) return;
     const int jb = find_job(bv.leaf_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int k = (int)(g - bv.leaf_off[jb]);
-    const fcg_node_geom& nd = J.nodes[J.leaf_node[k]];
-    double s;
-    if (nd.flags & FCG_NODE_DIEL) {
-        const double eb = J.gout[nd.cond], ea = J.gin[nd.cond];
-        if (J.dim == 3)
-            s = (ea + eb) / (FCG_TWO_TIMES_E0 * (eb - ea) * nd.area);
-        else
-            s = FCG_PI * (ea + eb) / ((eb - ea) * nd.area);
-    } else if (J.dim == 3) {
-        const Vec3<double> c{(double)nd.c[0], (double)nd.c[1], (double)nd.c[2]};
-        s = potential_opt_3d<double>(c, nd.v) / FCG_FOUR_PI_TIMES_E0;
-    } else {
-        s = potential_opt_2d<double>((double)nd.c[0], (double)nd.c[1], nd.v);
-    }
+    const double s = node_self(J, J.nodes[J.leaf_node[k]]);
     J.self[k] = s;
     J.diag[J.leaf_node[k]] = s;  // non-leaf entries stay zero
 }
@@ -930,31 +932,17 @@ __device__ __noinline__ double pair_coef(const fcg_node_geom& p1, const fcg_node
 constexpr int kSetupThreads = 256;
 constexpr int kSetupLds = 64;  // blocks of at most this many leaves are factored in LDS
 
-// one workgroup per block (ComputeBlockPrecond + InvertMatrix,
-// Solver/SolveCapacitance.cpp:2068-2258)
-__global__ void __launch_bounds__(kSetupThreads) k_block_setup(BatchView bv, const int32_t* __restrict__ bstart,
-                                                               const int64_t* __restrict__ boff,
-                                                               double* __restrict__ P, double* __restrict__ work) {
-    __shared__ double sh[kSetupLds * kSetupLds];
-    __shared__ double rv[kSetupThreads];
-    __shared__ int ri[kSetupThreads];
-    __shared__ int piv[kMaxBlockPrecond];
-    const BJob& J = bv.jobs[0];
-    const int b = blockIdx.x;
-    const int b0 = bstart[b], n = bstart[b + 1] - b0;
+// LU with partial pivoting (the first row of largest |entry| in the column, as
+// InvertMatrix, Solver/SolveCapacitance.cpp:2116-2258) of the n x n row-major
+// matrix A, then its inverse, column k from the unit vector e_k, written
+// column-major to out.  Run by a whole workgroup of kSetupThreads threads.
+__device__ void lu_invert(double* A, int n, int* piv, double* out, double* rv, int* ri) {
     const int t = threadIdx.x;
-    double* A = n <= kSetupLds ? sh : work + boff[b];  // row-major n x n
-    for (int e = t; e < n * n; e += kSetupThreads) {
-        const int i = e / n, j = e - i * n;
-        A[e] = (i == j) ? J.self[b0 + i] : pair_coef(J.nodes[J.leaf_node[b0 + i]], J.nodes[J.leaf_node[b0 + j]], J.dim);
-    }
-    __syncthreads();
     for (int j = 0; j < n; ++j) {
-        // pivot: the first row of largest |A[i][j]|, i >= j
         double best = -1.0;
         int bi = n;
         for (int i = j + t; i < n; i += kSetupThreads) {
-            const double v = fabs(A[i * n + j]);
+            const double v = fabs(A[(size_t)i * n + j]);
             if (v > best) {
                 best = v;
                 bi = i;
@@ -978,42 +966,95 @@ __global__ void __launch_bounds__(kSetupThreads) k_block_setup(BatchView bv, con
         if (t == 0) piv[j] = p;
         if (p != j)
             for (int c = t; c < n; c += kSetupThreads) {
-                const double tmp = A[j * n + c];
-                A[j * n + c] = A[p * n + c];
-                A[p * n + c] = tmp;
+                const double tmp = A[(size_t)j * n + c];
+                A[(size_t)j * n + c] = A[(size_t)p * n + c];
+                A[(size_t)p * n + c] = tmp;
             }
         __syncthreads();
         if (j < n - 1) {
-            const double rcp = 1.0 / A[j * n + j];
-            for (int i = j + 1 + t; i < n; i += kSetupThreads) A[i * n + j] *= rcp;
+            const double rcp = 1.0 / A[(size_t)j * n + j];
+            for (int i = j + 1 + t; i < n; i += kSetupThreads) A[(size_t)i * n + j] *= rcp;
             __syncthreads();
             const int m = n - 1 - j;
-            for (int e = t; e < m * m; e += kSetupThreads) {
-                const int i = j + 1 + e / m, c = j + 1 + e % m;
-                A[i * n + c] -= A[i * n + j] * A[j * n + c];
+            for (int64_t e = t; e < (int64_t)m * m; e += kSetupThreads) {
+                const int i = j + 1 + (int)(e / m), c = j + 1 + (int)(e % m);
+                A[(size_t)i * n + c] -= A[(size_t)i * n + j] * A[(size_t)j * n + c];
             }
         }
         __syncthreads();
     }
-    // inverse: column k from the unit vector e_k, solved in place in column k of
-    // the output (column-major storage)
-    double* Pb = P + boff[b];
     for (int k = t; k < n; k += kSetupThreads) {
-        double* y = Pb + (size_t)k * n;
+        double* y = out + (size_t)k * n;
         for (int i = 0; i < n; ++i) y[i] = (i == k) ? 1.0 : 0.0;
         for (int i = 0; i < n; ++i) {
             const int ip = piv[i];
             double sum = y[ip];
             y[ip] = y[i];
-            for (int c = 0; c < i; ++c) sum -= A[i * n + c] * y[c];
+            for (int c = 0; c < i; ++c) sum -= A[(size_t)i * n + c] * y[c];
             y[i] = sum;
         }
         for (int i = n - 1; i >= 0; --i) {
             double sum = y[i];
-            for (int c = i + 1; c < n; ++c) sum -= A[i * n + c] * y[c];
-            y[i] = sum / A[i * n + i];
+            for (int c = i + 1; c < n; ++c) sum -= A[(size_t)i * n + c] * y[c];
+            y[i] = sum / A[(size_t)i * n + i];
         }
     }
+}
+
+// one workgroup per block (ComputeBlockPrecond, Solver/SolveCapacitance.cpp:2068-2113)
+__global__ void __launch_bounds__(kSetupThreads) k_block_setup(BatchView bv, const int32_t* __restrict__ bstart,
+                                                               const int64_t* __restrict__ boff,
+                                                               double* __restrict__ P, double* __restrict__ work) {
+    __shared__ double sh[kSetupLds * kSetupLds];
+    __shared__ double rv[kSetupThreads];
+    __shared__ int ri[kSetupThreads];
+    __shared__ int piv[kMaxBlockPrecond];
+    const BJob& J = bv.jobs[0];
+    const int b = blockIdx.x;
+    const int b0 = bstart[b], n = bstart[b + 1] - b0;
+    double* A = n <= kSetupLds ? sh : work + boff[b];  // row-major n x n
+    for (int e = threadIdx.x; e < n * n; e += kSetupThreads) {
+        const int i = e / n, j = e - i * n;
+        A[e] = (i == j) ? J.self[b0 + i] : pair_coef(J.nodes[J.leaf_node[b0 + i]], J.nodes[J.leaf_node[b0 + j]], J.dim);
+    }
+    __syncthreads();
+    lu_invert(A, n, piv, P + boff[b], rv, ri);
+}
+
+// Two-level preconditioner matrix (ComputeSuperPrecond,
+// Solver/SolveCapacitance.cpp:2007-2066): element i's node self term on the
+// diagonal, link coefficients between element nodes elsewhere; row-major m x m.
+__global__ void k_super_build(BatchView bv, const int32_t* __restrict__ snode, int m, double* __restrict__ A) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (int64_t)m * m) return;
+    const BJob& J = bv.jobs[0];
+    const int i = (int)(e / m), j = (int)(e % m);
+    const fcg_node_geom& ni = J.nodes[snode[i]];
+    A[e] = (i == j) ? node_self(J, ni) : pair_coef(ni, J.nodes[snode[j]], J.dim);
+}
+
+// per leaf: its element's index weight (leaf dimension over element dimension)
+__global__ void k_super_weights(BatchView bv, const int32_t* __restrict__ snode, const int32_t* __restrict__ sidx,
+                                double* __restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const BJob& J = bv.jobs[0];
+    if (i >= J.N) return;
+    w[i] = J.nodes[J.leaf_node[i]].area / J.nodes[snode[sidx[i]]].area;
+}
+
+__global__ void __launch_bounds__(kSetupThreads) k_super_invert(double* A, int m, int* piv, double* Sinv) {
+    __shared__ double rv[kSetupThreads];
+    __shared__ int ri[kSetupThreads];
+    lu_invert(A, m, piv, Sinv, rv, ri);
+}
+
+hipError_t launch_super_setup(const BatchView& one_job, int m, int N, const int32_t* snode, const int32_t* sidx,
+                              double* A, int* piv, double* Sinv, double* w, hipStream_t s) {
+    if (m <= 0) return hipSuccess;
+    k_super_build<<<blocks_for((size_t)m * m, 256), 256, 0, s>>>(one_job, snode, m, A);
+    k_super_weights<<<blocks_for((size_t)N, 256), 256, 0, s>>>(one_job, snode, sidx, w);
+    k_super_invert<<<1, kSetupThreads, 0, s>>>(A, m, piv, Sinv);
+    return hipGetLastError();
 }
 
 hipError_t launch_block_setup(const BatchView& one_job, int nb, const int32_t* bstart, const int64_t* boff, double* P,
@@ -1087,6 +1128,90 @@ hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s) {
         k_block_apply<kPreAdd><<<grid, 256, 0, s>>>(pv);
     else
         k_block_apply<kPreFromB><<<grid, 256, 0, s>>>(pv);
+    return hipGetLastError();
+}
+
+// Two-level preconditioner apply (ComputePrecondVectFast super branch,
+// Solver/SolveCapacitance.cpp:3545-3592), batched over solves.
+// collect: one wavefront per (solve, rhs, element): pot = sum of w_i in_i
+template <int MODE>
+__global__ void __launch_bounds__(256) k_super_collect(SView sv) {
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= sv.nel) return;
+    const int jb = find_job(sv.el_off, sv.njobs, g);
+    const SJob& J = sv.jobs[jb];
+    const int local = (int)(g - sv.el_off[jb]);
+    const int r = local / J.m, e = local - r * J.m;
+    const int lane = threadIdx.x & 63;
+    const double* in64 = static_cast<const double*>(J.in);
+    const float* in32 = static_cast<const float*>(J.in);
+    double acc = 0.0;
+    for (int i = J.sstart[e] + lane; i < J.sstart[e + 1]; i += 64) {
+        double x;
+        if (MODE == kPreFromB)
+            x = (i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
+        else
+            x = J.in32 ? (double)in32[(size_t)r * J.N + i] : in64[(size_t)r * J.N + i];
+        acc += J.w[i] * x;
+    }
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (lane == 0) J.pot[(size_t)r * J.m + e] = acc;
+}
+
+// multiply: one thread per (solve, rhs, element s): chg = sum over t != s of
+// Sinv[s][t] pot[t] (the diagonal is applied per leaf in the distribution)
+__global__ void k_super_mult(SView sv) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= sv.nel) return;
+    const int jb = find_job(sv.el_off, sv.njobs, g);
+    const SJob& J = sv.jobs[jb];
+    const int local = (int)(g - sv.el_off[jb]);
+    const int r = local / J.m, e = local - r * J.m;
+    const double* pot = J.pot + (size_t)r * J.m;
+    double acc = 0.0;
+    for (int t = 0; t < J.m; ++t)
+        if (t != e) acc += J.Sinv[(size_t)t * J.m + e] * pot[t];
+    J.chg[(size_t)r * J.m + e] = acc;
+}
+
+// distribute: one thread per (solve, rhs, leaf)
+template <int MODE>
+__global__ void k_super_dist(SView sv) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= sv.nlf) return;
+    const int jb = find_job(sv.lf_off, sv.njobs, g);
+    const SJob& J = sv.jobs[jb];
+    const int64_t local = g - sv.lf_off[jb];
+    const int r = (int)(local / J.N), i = (int)(local - (int64_t)r * J.N);
+    const int e = J.sidx[i];
+    const size_t o = (size_t)r * J.N + i;
+    double x;
+    if (MODE == kPreFromB)
+        x = (i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
+    else
+        x = J.in32 ? (double)static_cast<const float*>(J.in)[o] : static_cast<const double*>(J.in)[o];
+    const double v = J.w[i] * (J.chg[(size_t)r * J.m + e] + J.Sinv[(size_t)e * J.m + e] * x);
+    if (MODE == kPreAdd)
+        J.out[o] += v;
+    else
+        J.out[o] = v;
+}
+
+hipError_t launch_super_apply(const SView& sv, int mode, hipStream_t s) {
+    if (sv.nel == 0) return hipSuccess;
+    const unsigned gc = (unsigned)((sv.nel + 3) / 4);
+    if (mode == kPreFromB)
+        k_super_collect<kPreFromB><<<gc, 256, 0, s>>>(sv);
+    else
+        k_super_collect<kPreSet><<<gc, 256, 0, s>>>(sv);
+    k_super_mult<<<blocks_for((size_t)sv.nel, 256), 256, 0, s>>>(sv);
+    const unsigned gd = blocks_for((size_t)sv.nlf, 256);
+    if (mode == kPreFromB)
+        k_super_dist<kPreFromB><<<gd, 256, 0, s>>>(sv);
+    else if (mode == kPreAdd)
+        k_super_dist<kPreAdd><<<gd, 256, 0, s>>>(sv);
+    else
+        k_super_dist<kPreSet><<<gd, 256, 0, s>>>(sv);
     return hipGetLastError();
 }
 

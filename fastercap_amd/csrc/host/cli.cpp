@@ -7,6 +7,7 @@
 // printed as CSolveCap::OutputCapMtx does (Solver/SolveCapacitance.cpp:724-809).
 // The mesh and links come from the host builder (include/fastercap_host.h), the
 // fill and the solve from the device library (include/fastercap_amd.h).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -29,7 +30,9 @@ struct Options {
     double mesh_eps = 0.1, eps_ratio = 1.0, curv = 3.0, tol = 0.01, auto_err = 0.01;
     bool autom = false, verbose = false, auto_precond = false;
     int precond = FCG_PRECOND_JACOBI;
-    int block_size = 32;  // -pb<n> (SolverGlobal.cpp:60)
+    int block_size = 32;   // -pb<n> (SolverGlobal.cpp:60)
+    int super_dim = 128;   // -ps<n> (SolverGlobal.cpp:59)
+    bool jacobi_set = false;  // -pj given (kept only without -ps, SolveCapacitance.cpp:122-125)
     int precision = FCG_PREC_FP64;
     int device = 0;
 };
@@ -38,13 +41,14 @@ void usage() {
     std::printf(
         "Usage: fastercap <input file> [options]\n"
         "  -a<err>  Automatically refine, stop when the relative C-matrix change is below err (default 0.01)\n"
-        "  -ap      Automatic preconditioner choice (Jacobi here: the two-level one is not built)\n"
+        "  -ap      Automatic preconditioner choice (Jacobi or two-level, from links x conductors)\n"
         "  -m<eps>  Mesh relative refinement value (default 0.1)\n"
         "  -mc<c>   Mesh curvature coefficient (default 3)\n"
         "  -t<tol>  GMRES iteration tolerance (default 0.01)\n"
         "  -d<r>    Direct potential interaction coefficient to mesh refinement ratio (default 1)\n"
         "  -pj      Jacobi preconditioner (default)\n"
         "  -pb<n>   Block preconditioner, blocks of at most n panels (2..128)\n"
+        "  -ps<n>   Two-level preconditioner of dimension n (16..4096)\n"
         "  -pn      No preconditioner\n"
         "  -v       Verbose output\n"
         "  -b       Console mode (always)\n"
@@ -87,9 +91,11 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
         } else if (!std::strcmp(a, "-ap")) {
             o.auto_precond = true;
         } else if (!std::strcmp(a, "-pj")) {
-            o.precond = FCG_PRECOND_JACOBI;
+            if (o.precond != FCG_PRECOND_SUPER) o.precond = FCG_PRECOND_JACOBI;
+            o.jacobi_set = true;
         } else if (!std::strcmp(a, "-pn")) {
             o.precond = FCG_PRECOND_NONE;
+            o.jacobi_set = false;
         } else if (!std::strncmp(a, "-pb", 3)) {
             // the reference adds the block preconditioner to the current choice and
             // then runs a flexible GMRES that it reports as unsupported; here -pb
@@ -105,6 +111,22 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
                 } else if (o.block_size < 2) {
                     std::fprintf(stderr, "Warning: minimum block preconditioner dimension is 2, using 2\n");
                     o.block_size = 2;
+                }
+            }
+        } else if (!std::strncmp(a, "-ps", 3)) {
+            // the two-level preconditioner replaces Jacobi (SolveCapacitance.cpp:
+            // 122-125) and, here, a block preconditioner given before it
+            double v;
+            ok = num(a + 3, &v) && v >= 0 && v == (int)v;
+            if (ok) {
+                o.precond = FCG_PRECOND_SUPER;
+                o.super_dim = (int)std::min(v, 1e9);
+                if (o.super_dim > 4096) {
+                    std::fprintf(stderr, "Warning: maximum two-levels preconditioner dimension is 4096, using 4096\n");
+                    o.super_dim = 4096;
+                } else if (o.super_dim < 16) {
+                    std::fprintf(stderr, "Warning: minimum two-levels preconditioner dimension is 16, using 16\n");
+                    o.super_dim = 16;
                 }
             }
         } else if (!std::strcmp(a, "-gmixed")) {
@@ -133,6 +155,9 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
             return FCG_COMMAND_LINE_ERROR;
         }
     }
+    if (o.precond == FCG_PRECOND_SUPER && o.jacobi_set)
+        std::fprintf(stderr, "Warning: both two-levels preconditioner and Jacobi preconditioner specified, keeping only "
+                             "the two-levels preconditioner\n");
     if (o.file.empty() && !stop) {
         std::fprintf(stderr, "Error: no input file\n");
         usage();
@@ -170,11 +195,25 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
     r.names.clear();
     for (int g = 0; g < d.n_groups; ++g)
         if (!d.group_is_diel[g]) r.names.push_back(fch_group_name(mesh, g));
+    int precond = o.precond, size = o.precond == FCG_PRECOND_BLOCK ? o.block_size : o.super_dim;
+    if (o.auto_precond) {
+        // CSolveCap::AutoSetPrecondType (Solver/SolveCapacitance.cpp:580-608):
+        // thousands of mutual links times conductors
+        const long long lx = (long long)(r.hs.links / 1000) * (long long)r.names.size();
+        precond = lx <= 500 ? FCG_PRECOND_JACOBI : FCG_PRECOND_SUPER;
+        size = lx <= 5000 ? 128 : lx <= 10000 ? 512 : 1024;
+        if (o.verbose) {
+            if (precond == FCG_PRECOND_JACOBI)
+                std::printf("Automatic preconditioner: Jacobi\n");
+            else
+                std::printf("Automatic preconditioner: two-levels, dimension %d\n", size);
+        }
+    }
     int job = -1;
     rc = fcg_job_upload(ctx, &d, &job);
     if (rc == FCG_OK) rc = fcg_fill(ctx, job, o.precision);
     if (rc == FCG_OK) {
-        fcg_solve_opts so{tol, 1000, o.precond, o.precision, 8, o.block_size};
+        fcg_solve_opts so{tol, 1000, precond, o.precision, 8, size};
         const int nc = (int)r.names.size();
         r.cap.assign((size_t)nc * nc, 0.0);
         rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, nullptr, &r.ss);
@@ -244,8 +283,6 @@ int main(int argc, char** argv) {
                      o.curv);
         o.curv = 1.0;
     }
-    if (o.auto_precond)
-        std::fprintf(stderr, "Warning: automatic preconditioner requested; the two-level preconditioner is not built, using Jacobi\n");
     std::printf("Running fastercap (gfx950 solve path)\nInput file: %s\n", o.file.c_str());
     fcg_ctx* ctx = nullptr;
     if (fcg_ctx_create(o.device, &ctx) != FCG_OK) {

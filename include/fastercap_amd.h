@@ -48,7 +48,8 @@ extern "C" {
 /* Preconditioner bit flags: values of AUTOREFINE_PRECOND_* (Solver/SolverGlobal.h:59-68). */
 #define FCG_PRECOND_NONE 0
 #define FCG_PRECOND_JACOBI 1
-#define FCG_PRECOND_BLOCK 2 /* block preconditioner (-pb<n>), fcg_solve_opts.block_size */
+#define FCG_PRECOND_BLOCK 2 /* block preconditioner (-pb<n>), fcg_solve_opts.precond_size */
+#define FCG_PRECOND_SUPER 3 /* two-level preconditioner (-ps<n>), fcg_solve_opts.precond_size */
 
 /*
  * Geometry of one tree node (leaf panel, refined parent or super node), 96 bytes.
@@ -102,7 +103,8 @@ typedef struct fcg_solve_opts {
     int32_t precond;   /* FCG_PRECOND_* */
     int32_t precision; /* FCG_PREC_FP64 or FCG_PREC_MIXED */
     int32_t max_refine;/* outer FP64 refinement sweeps in FCG_PREC_MIXED */
-    int32_t block_size;/* FCG_PRECOND_BLOCK: most leaves per block (-pb<n>); 0 = 32, clamped to 2..128 */
+    int32_t precond_size; /* FCG_PRECOND_BLOCK: most leaves per block (-pb<n>; 0 = 32, clamped to 2..128);
+                             FCG_PRECOND_SUPER: dimension (-ps<n>; 0 = 128, clamped to 16..4096) */
 } fcg_solve_opts;
 
 typedef struct fcg_solve_stats {
@@ -169,6 +171,20 @@ int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k)
  * parent has more than block_size leaves) is a block of its own; the reference
  * opens a block there that it never computes. */
 int fcg_block_precond_apply(fcg_ctx* ctx, int job, int block_size, const double* q, double* pq, int k);
+
+/* Pq = S q for k vectors (host buffers) with the two-level preconditioner of
+ * dimension dim (0 = 128, clamped to 16..4096): the tree nodes at depth
+ * log2(dim / roots) of every conductor and dielectric tree (or shallower
+ * leaves), the inverse of their collocation matrix, applied by collecting each
+ * node's leaves (area weighted), multiplying by the off-diagonal part and
+ * distributing back with the diagonal term.  Built on first use (the job must be
+ * filled).  FCG_INVALID_ARGUMENT when unavailable (at least dim roots; solves
+ * then run unpreconditioned, as the reference does).
+ * Replaces MakePreconditioner, the super branch of RecurseComputePrecond,
+ * ComputeSuperPrecond and InvertMatrix (Solver/SolveCapacitance.cpp:1067-1104,
+ * 1876-2066, 2116-2258) and the super branch of ComputePrecondVectFast
+ * (:3545-3592). */
+int fcg_super_precond_apply(fcg_ctx* ctx, int job, int dim, const double* q, double* pq, int k);
 
 /* Solve every conductor right-hand side of the job with right-preconditioned GMRES
  * and assemble the Maxwell capacitance matrix cap[rows][n_cond] (row-major; rows =

@@ -288,8 +288,9 @@ double orc_pot_estimate(const fcg_job_desc* d, int i, int j) {
     return vdot(dist, n1) / (kFourPiE0 * rd * rd * rd);
 }
 
-double orc_self(const fcg_job_desc* d, int leaf) {
-    const int node = d->leaf_node[leaf];
+double orc_self(const fcg_job_desc* d, int leaf) { return orc_node_self(d, d->leaf_node[leaf]); }
+
+double orc_node_self(const fcg_job_desc* d, int node) {
     const fcg_node_geom& g = d->nodes[node];
     if (g.flags & FCG_NODE_DIEL) {
         const double eb = d->group_outperm[g.cond], ea = d->group_inperm[g.cond];
@@ -370,13 +371,24 @@ struct BlockPre {
     std::vector<long long> poff;
     std::vector<double> P;
 };
+// Two-level ("super") preconditioner data (see orc_super_precond).
+struct SuperPre {
+    int m = 0;
+    std::vector<int> nodes, starts;
+    std::vector<double> Sinv;
+};
 
 // gmresPrecondSFastAll (SolveCapacitance.cpp:2288-2574): no preconditioner, Jacobi,
 // or the block preconditioner when bp is given.
 static int gmres(const fcg_job_desc* d, const double* coef, const double* self_coef, const std::vector<double>& b,
-                 std::vector<double>& x, double tol, int precond, int nthreads, const BlockPre* bp = nullptr) {
+                 std::vector<double>& x, double tol, int precond, int nthreads, const BlockPre* bp = nullptr,
+                 const SuperPre* sp = nullptr) {
     const long n = d->n_leaves;
     auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
+        if (sp) {
+            orc_super_apply(d, sp->m, sp->nodes.data(), sp->starts.data(), sp->Sinv.data(), in.data(), out.data());
+            return;
+        }
         if (bp) {
             orc_block_apply(bp->nblocks, bp->starts.data(), bp->poff.data(), bp->P.data(), in.data(), out.data());
             return;
@@ -479,7 +491,18 @@ int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef
 
 static int solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
                       int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads,
-                      const BlockPre* bp);
+                      const BlockPre* bp, const SuperPre* sp = nullptr);
+
+int orc_solve_super(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int dim,
+                    double* cap, int* rows, int* cols, int* iters, int nthreads) {
+    SuperPre sp;
+    sp.nodes.assign((size_t)std::max(dim, 1), 0);
+    sp.starts.assign((size_t)std::max(dim, 1) + 1, 0);
+    sp.Sinv.assign((size_t)dim * (size_t)dim, 0.0);
+    if (orc_super_precond(d, dim, sp.nodes.data(), sp.starts.data(), &sp.m, sp.Sinv.data()) != 0)
+        return solve_rows(d, coef, self_coef, tol, 0, -1, cap, rows, cols, iters, nthreads, nullptr);  // unavailable
+    return solve_rows(d, coef, self_coef, tol, 3, -1, cap, rows, cols, iters, nthreads, nullptr, &sp);
+}
 
 int orc_solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
                    int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads) {
@@ -500,7 +523,7 @@ int orc_solve_block(const fcg_job_desc* d, const double* coef, const double* sel
 
 static int solve_rows(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
                       int max_rows, double* cap, int* rows, int* cols, int* iters, int nthreads,
-                      const BlockPre* bp) {
+                      const BlockPre* bp, const SuperPre* sp) {
     const long n = d->n_leaves;
     int ncond = 0;
     for (int g = 0; g < d->n_groups; ++g)
@@ -516,7 +539,7 @@ static int solve_rows(const fcg_job_desc* d, const double* coef, const double* s
         if (d->dim == 2 && hi == n) continue;  // last conductor row is the balance row (SolveCapacitance.cpp:1233-1240)
         if (max_rows >= 0 && row >= max_rows) break;
         for (int k = lo; k < hi; ++k) b[k] = 1.0;
-        const int it = gmres(d, coef, self_coef, b, x, tol, precond, nthreads, bp);
+        const int it = gmres(d, coef, self_coef, b, x, tol, precond, nthreads, bp, sp);
         if (iters) iters[row] = it;
         int col = 0;
         for (int g2 = 0; g2 < d->n_groups; ++g2) {
@@ -689,6 +712,85 @@ extern "C" void orc_block_apply(int nblocks, const int* starts, const long long*
             double s = 0.0;
             for (int j = 0; j < n; ++j) s += Pb[(size_t)i * n + j] * q[b0 + j];
             out[b0 + i] = s;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- two-level preconditioner
+// Super elements (MakePreconditioner, Solver/SolveCapacitance.cpp:1067-1104, and the
+// super branch of RecurseComputePrecond, :1876-1993): with R tree roots (conductors
+// and dielectric interfaces) and dimension dim, every root's tree is walked to
+// level L = log2(dim / R) (integer division; the root is level 0): a non-leaf at
+// level L is one element standing for all its leaves, a leaf at level <= L is
+// one.  Elements follow leaf order, so each covers consecutive leaves.  With
+// R >= dim the preconditioner is unavailable (returns 1; the reference then drops
+// it and solves unpreconditioned).
+extern "C" int orc_super_partition(const fcg_job_desc* d, int dim, int* nodes, int* starts, int* m) {
+    std::vector<int> roots;
+    for (int i = 0; i < d->n_nodes; ++i)
+        if (d->parent[i] < 0) roots.push_back(i);
+    std::sort(roots.begin(), roots.end(), [&](int a, int b) { return d->leaf_lo[a] < d->leaf_lo[b]; });
+    *m = 0;
+    const unsigned R = (unsigned)roots.size();
+    if (dim <= 0 || R >= (unsigned)dim) return 1;
+    const int L = (int)(unsigned)(std::log((double)((unsigned)dim / R)) / std::log(2.0));
+    int cnt = 0;
+    std::vector<std::pair<int, int>> stack;
+    for (int r : roots) {
+        stack.push_back({r, 0});
+        while (!stack.empty()) {
+            const auto [a, lev] = stack.back();
+            stack.pop_back();
+            if (d->left[a] < 0 || lev == L) {
+                nodes[cnt] = a;
+                starts[cnt] = d->leaf_lo[a];
+                ++cnt;
+                continue;
+            }
+            stack.push_back({d->right[a], lev + 1});
+            stack.push_back({d->left[a], lev + 1});
+        }
+    }
+    starts[cnt] = d->n_leaves;
+    *m = cnt;
+    return 0;
+}
+
+// ComputeSuperPrecond (Solver/SolveCapacitance.cpp:2007-2066): the elements' dense
+// collocation matrix (self potential of each element node on the diagonal,
+// PotEstimateOpt between element nodes elsewhere), inverted (InvertMatrix).
+// Sinv: m x m row-major (capacity dim * dim).
+extern "C" int orc_super_precond(const fcg_job_desc* d, int dim, int* nodes, int* starts, int* m, double* Sinv) {
+    if (orc_super_partition(d, dim, nodes, starts, m) != 0) return 1;
+    const int n = *m;
+    std::vector<double> A((size_t)n * n);
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            A[(size_t)i * n + j] = (i == j) ? orc_node_self(d, nodes[i]) : orc_pot_estimate(d, nodes[i], nodes[j]);
+    invert_block(A, n, Sinv);
+    return 0;
+}
+
+// ComputePrecondVectFast super branch (Solver/SolveCapacitance.cpp:3545-3592):
+// collect each element's area-weighted input, multiply by the off-diagonal part
+// of the inverse, distribute back with the leaf's area weight, and add the
+// diagonal entry times the leaf's own weighted input.
+extern "C" void orc_super_apply(const fcg_job_desc* d, int m, const int* nodes, const int* starts,
+                                const double* Sinv, const double* q, double* out) {
+    std::vector<double> pot((size_t)m, 0.0), chg((size_t)m, 0.0);
+    for (int s = 0; s < m; ++s) {
+        const double dimS = d->nodes[nodes[s]].area;
+        for (int i = starts[s]; i < starts[s + 1]; ++i) pot[(size_t)s] += q[i] * d->nodes[d->leaf_node[i]].area / dimS;
+    }
+    for (int s = 0; s < m; ++s)
+        for (int t = 0; t < m; ++t)
+            if (s != t) chg[(size_t)s] += Sinv[(size_t)s * m + t] * pot[(size_t)t];
+    for (int s = 0; s < m; ++s) {
+        const double dimS = d->nodes[nodes[s]].area;
+        for (int i = starts[s]; i < starts[s + 1]; ++i) {
+            const double w = d->nodes[d->leaf_node[i]].area / dimS;
+            out[i] = chg[(size_t)s] * w + Sinv[(size_t)s * m + s] * q[i] * w;
         }
     }
 }

@@ -31,6 +31,13 @@
  *   orc_block_apply        Pq = block-diagonal product (ComputePrecondVectFast block branch,
  *                          :3527-3544)
  *   orc_solve_block        orc_solve with the block preconditioner
+ *   orc_super_partition    two-level preconditioner elements (MakePreconditioner and the super
+ *                          branch of RecurseComputePrecond, Solver/SolveCapacitance.cpp:1067-1104,
+ *                          1876-1993)
+ *   orc_super_precond      their collocation matrix inverted (ComputeSuperPrecond, :2007-2066)
+ *   orc_super_apply        collect / multiply / distribute (ComputePrecondVectFast, :3545-3592)
+ *   orc_solve_super        orc_solve with the two-level preconditioner
+ *   orc_node_self          SelfPotential of any tree node (leaf, refined parent or super node)
  */
 #ifndef FASTERCAP_ORACLE_H
 #define FASTERCAP_ORACLE_H
@@ -88,6 +95,17 @@ int orc_block_precond(const fcg_job_desc* d, int bsize, int* starts, int* nblock
 void orc_block_apply(int nblocks, const int* starts, const long long* poff, const double* P, const double* q,
                      double* out);
 int orc_solve_block(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int bsize,
+                    double* cap, int* rows, int* cols, int* iters, int nthreads);
+
+/* Two-level preconditioner of dimension dim: m elements (nodes[m], leaf ranges
+ * starts[0..m], capacity dim and dim + 1), Sinv m x m row-major (capacity dim^2).
+ * Returns 1 when unavailable (at least dim tree roots). */
+double orc_node_self(const fcg_job_desc* d, int node);
+int orc_super_partition(const fcg_job_desc* d, int dim, int* nodes, int* starts, int* m);
+int orc_super_precond(const fcg_job_desc* d, int dim, int* nodes, int* starts, int* m, double* Sinv);
+void orc_super_apply(const fcg_job_desc* d, int m, const int* nodes, const int* starts, const double* Sinv,
+                     const double* q, double* out);
+int orc_solve_super(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int dim,
                     double* cap, int* rows, int* cols, int* iters, int nthreads);
 
 #ifdef __cplusplus

@@ -54,6 +54,13 @@ def lib():
         L.orc_block_apply.argtypes = [C.c_int, ip, llp, dp, dp, dp]
         L.orc_block_apply.restype = None
         L.orc_solve_block.argtypes = [jd, dp, dp, C.c_double, C.c_int, dp, ip, ip, ip, C.c_int]
+        L.orc_node_self.argtypes = [jd, C.c_int]
+        L.orc_node_self.restype = C.c_double
+        L.orc_super_partition.argtypes = [jd, C.c_int, ip, ip, ip]
+        L.orc_super_precond.argtypes = [jd, C.c_int, ip, ip, ip, dp]
+        L.orc_super_apply.argtypes = [jd, C.c_int, ip, ip, dp, dp, dp]
+        L.orc_super_apply.restype = None
+        L.orc_solve_super.argtypes = [jd, dp, dp, C.c_double, C.c_int, dp, ip, ip, ip, C.c_int]
         _lib = L
     return _lib
 
@@ -194,6 +201,51 @@ def solve_block(mesh, coef, selfc, bsize, tol=1e-8, nthreads=0):
     iters = np.zeros(ncond, dtype=np.int32)
     rows, cols = C.c_int(), C.c_int()
     lib().orc_solve_block(C.byref(mesh.desc), _d(coef), _d(selfc), tol, bsize, _d(cap), C.byref(rows),
+                          C.byref(cols), iters.ctypes.data_as(C.POINTER(C.c_int)), nthreads)
+    r, c = rows.value, cols.value
+    return cap[: r * c].reshape(r, c), iters[:r]
+
+
+class SuperPrecond:
+    """Two-level preconditioner of one mesh: m elements (tree nodes) with leaf
+    ranges [starts[s], starts[s+1]) and the inverse Sinv of their matrix;
+    available is False when the mesh has at least dim tree roots."""
+
+    def __init__(self, mesh, dim):
+        ip = C.POINTER(C.c_int)
+        self.mesh = mesh
+        self.nodes = np.zeros(max(dim, 1), dtype=np.int32)
+        self.starts = np.zeros(max(dim, 1) + 1, dtype=np.int32)
+        Sinv = np.zeros(dim * dim, dtype=np.float64)
+        m = C.c_int()
+        rc = lib().orc_super_precond(C.byref(mesh.desc), dim, self.nodes.ctypes.data_as(ip),
+                                     self.starts.ctypes.data_as(ip), C.byref(m), _d(Sinv))
+        self.available = rc == 0
+        self.m = m.value
+        self.nodes = self.nodes[: self.m].copy()
+        self.starts = self.starts[: self.m + 1].copy()
+        self.Sinv = Sinv[: self.m * self.m].reshape(self.m, self.m).copy()
+
+    def apply(self, q):
+        ip = C.POINTER(C.c_int)
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        out = np.empty_like(q)
+        S = np.ascontiguousarray(self.Sinv)
+        lib().orc_super_apply(C.byref(self.mesh.desc), self.m, self.nodes.ctypes.data_as(ip),
+                              self.starts.ctypes.data_as(ip), _d(S), _d(q), _d(out))
+        return out
+
+
+def node_self(mesh, node):
+    return lib().orc_node_self(C.byref(mesh.desc), node)
+
+
+def solve_super(mesh, coef, selfc, dim, tol=1e-8, nthreads=0):
+    ncond = int((mesh.group_is_diel == 0).sum())
+    cap = np.zeros(ncond * ncond, dtype=np.float64)
+    iters = np.zeros(ncond, dtype=np.int32)
+    rows, cols = C.c_int(), C.c_int()
+    lib().orc_solve_super(C.byref(mesh.desc), _d(coef), _d(selfc), tol, dim, _d(cap), C.byref(rows),
                           C.byref(cols), iters.ctypes.data_as(C.POINTER(C.c_int)), nthreads)
     r, c = rows.value, cols.value
     return cap[: r * c].reshape(r, c), iters[:r]

@@ -19,7 +19,7 @@ def test_switches():
     assert cli.parse_args(["x", "-mc0.5"]).curv_coeff == 1.0  # clamped as the reference does
 
 
-@pytest.mark.parametrize("bad", [["x", "-g"], ["x", "-ps32"], ["x", "-pbx"], ["x", "-mabc"], [], ["a", "b"]])
+@pytest.mark.parametrize("bad", [["x", "-g"], ["x", "-psx"], ["x", "-ps-3"], ["x", "-pbx"], ["x", "-mabc"], [], ["a", "b"]])
 def test_errors(bad):
     with pytest.raises(cli.CommandLineError):
         cli.parse_args(bad)
@@ -34,6 +34,32 @@ def test_block_preconditioner_switch():
     assert cli.parse_args(["x", "-pb1"]).block_size == 2
     # a later -pj or -pn replaces it
     assert cli.parse_args(["x", "-pb16", "-pj"]).precond == 1
+
+
+def test_two_level_preconditioner_switch(capsys):
+    a = cli.parse_args(["x", "-ps512"])
+    assert a.precond == 3 and a.super_dim == 512
+    # the reference clamps the dimension to 16..4096 with a warning
+    assert cli.parse_args(["x", "-ps5000"]).super_dim == 4096
+    assert cli.parse_args(["x", "-ps3"]).super_dim == 16
+    # with Jacobi, in either order, only the two-level preconditioner is kept
+    capsys.readouterr()
+    assert cli.parse_args(["x", "-pj", "-ps64"]).precond == 3
+    assert cli.parse_args(["x", "-ps64", "-pj"]).precond == 3
+    assert capsys.readouterr().err.count("keeping only the two-levels preconditioner") == 2
+    assert cli.parse_args(["x", "-ps64", "-pn"]).precond == 0
+    assert cli.parse_args(["x", "-ps64", "-pb8"]).precond == 2
+
+
+@pytest.mark.parametrize("links,cond,want", [
+    (0, 5, (1, 128)), (250_999, 2, (1, 128)), (251_000, 2, (3, 128)),
+    (2_500_999, 2, (3, 128)), (2_501_000, 2, (3, 512)), (5_000_999, 2, (3, 512)),
+    (5_001_000, 2, (3, 1024)), (10_000_000_000, 1, (3, 1024)),
+])
+def test_auto_preconditioner_rule(links, cond, want):
+    # CSolveCap::AutoSetPrecondType (Solver/SolveCapacitance.cpp:580-608): links are
+    # counted in whole thousands before the product with the conductor count
+    assert cli.auto_preconditioner(links, cond) == want
 
 
 def test_matrix_printout():

@@ -61,7 +61,7 @@ def test_block_solve_matches_oracle_and_jacobi(dev, meshes, name):
     job = dev.upload(m)
     dev.fill(job, fc.FCG_PREC_FP64)
     coef, selfc = dev.coeffs(job, m)
-    blk = dev.solve(job, m, tol=1e-8, precond=fc.FCG_PRECOND_BLOCK, block_size=32)
+    blk = dev.solve(job, m, tol=1e-8, precond=fc.FCG_PRECOND_BLOCK, precond_size=32)
     jac = dev.solve(job, m, tol=1e-8, precond=fc.FCG_PRECOND_JACOBI)
     cap_o, it_o = orc.solve_block(m, coef, selfc, 32, tol=1e-8)
     e_o, e_j = _c_err(blk.cap, cap_o), _c_err(blk.cap, jac.cap)
@@ -79,14 +79,14 @@ def test_block_mixed_and_batched(dev, meshes):
     jobs = [dev.upload(m) for m in ms]
     for j in jobs:
         dev.fill(j, fc.FCG_PREC_FP64)
-    ref = [dev.solve(j, m, tol=1e-8, precond=fc.FCG_PRECOND_BLOCK, block_size=16) for j, m in zip(jobs, ms)]
-    batch = dev.solve_batch(jobs, ms, tol=1e-8, precond=fc.FCG_PRECOND_BLOCK, block_size=16)
+    ref = [dev.solve(j, m, tol=1e-8, precond=fc.FCG_PRECOND_BLOCK, precond_size=16) for j, m in zip(jobs, ms)]
+    batch = dev.solve_batch(jobs, ms, tol=1e-8, precond=fc.FCG_PRECOND_BLOCK, precond_size=16)
     for n, a, b in zip(names, ref, batch):
         assert _c_err(b.cap, a.cap) < 1e-6, n
     for j in jobs:
         dev.fill(j, fc.FCG_PREC_MIXED)
     mixed = dev.solve_batch(jobs, ms, tol=1e-5, precision=fc.FCG_PREC_MIXED, precond=fc.FCG_PRECOND_BLOCK,
-                            block_size=16)
+                            precond_size=16)
     for n, a, b in zip(names, ref, mixed):
         err = _c_err(b.cap, a.cap)
         print(f"mixed block {n}: C err {err:.2e} iters {list(b.iterations)}", flush=True)
