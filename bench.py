@@ -121,7 +121,7 @@ def bench_batch(args, fc, dev, tmp):
 
     # plain FP64 path: one job at a time (fill + solve), as the reference runs
     dev.fill_batch(jobs, F64)
-    for j, m in zip(jobs, meshes):  # warm (picks each job's gather configuration)
+    for j, m in zip(jobs, meshes):  # warm
         dev.solve(j, m, tol=tol)
     a = time.perf_counter()
     plain = []

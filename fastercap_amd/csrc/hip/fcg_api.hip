@@ -891,16 +891,15 @@ int ensure_super_precond(fcg_ctx* ctx, Job& j, int dim) {
         HIPCHECK(ctx, upload(j.spre.sidx, sidx.data(), sidx.size(), s));
         HIPCHECK(ctx, j.spre.Sinv.alloc(sizeof(double) * (size_t)m * m));
         HIPCHECK(ctx, j.spre.w.alloc(sizeof(double) * (size_t)j.N));
-        DevBuf A, piv;
+        DevBuf A;
         HIPCHECK(ctx, A.alloc(sizeof(double) * (size_t)m * m));
-        HIPCHECK(ctx, piv.alloc(sizeof(int) * (size_t)m));
         ctx->tabs.reset();
         BatchView bv;
         RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, nullptr, nullptr}}, &bv));
         HIPCHECK(ctx, fcg::launch_super_setup(bv, m, j.N, j.spre.snode.as<int32_t>(), j.spre.sidx.as<int32_t>(),
-                                              A.as<double>(), piv.as<int>(), j.spre.Sinv.as<double>(),
+                                              A.as<double>(), j.spre.Sinv.as<double>(),
                                               j.spre.w.as<double>(), s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));  // A, piv and the host vectors are released below
+        HIPCHECK(ctx, hipStreamSynchronize(s));  // A and the host vectors are released below
     }
     j.spre.dim = dim;
     return FCG_OK;
@@ -2375,16 +2374,9 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     Job* j = get_job(ctx, job);
     if (!j || !opts || !cap) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
-    const bool mixed = opts->precision == FCG_PREC_MIXED;
-    const int k = (int)j->rhs.size();
-    // first solve of a job in this precision: pick its gather shapes (inner and,
-    // when mixed, refinement products)
-    const int p = mixed ? fcg::kMvp32 : fcg::kMvp64;
-    if (k > 0 && j->tuned[p] == 0 && (mixed ? j->filled32 : j->filled64)) {
-        RCCHECK(autotune_prec(ctx, j, p, k, 3, nullptr, nullptr));
-        if (mixed && j->tuned[fcg::kMvp32Acc] == 0)
-            RCCHECK(autotune_prec(ctx, j, fcg::kMvp32Acc, k, 3, nullptr, nullptr));
-    }
+    // the gather shapes are the rhs class defaults set at upload; measuring every
+    // shape costs about a hundred products, more than a one-off solve, so it is
+    // left to fcg_autotune / fcg_autotune_batch
     return run_batch(ctx, &job, 1, opts, charges != nullptr, cap, rows_out, cols_out, stats, charges);
 }
 

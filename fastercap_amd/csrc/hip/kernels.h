@@ -213,10 +213,10 @@ hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s);
 // Two-level ("super") preconditioner: m elements (tree nodes snode[m]) over
 // consecutive leaf ranges, the inverse Sinv of their m x m collocation matrix
 // (column-major), and per leaf its element sidx[i] and weight w[i] (leaf
-// dimension over element dimension).  Setup of one job: A (m x m scratch), piv
-// (m ints scratch).
+// dimension over element dimension).  Setup of one job: A (m x m scratch); the
+// inverse comes from a Gauss-Jordan elimination, 2m launches.
 hipError_t launch_super_setup(const BatchView& one_job, int m, int N, const int32_t* snode, const int32_t* sidx,
-                              double* A, int* piv, double* Sinv, double* w, hipStream_t s);
+                              double* A, double* Sinv, double* w, hipStream_t s);
 struct SJob {
     const void* in;  // [k][N] FP64, or FP32 when in32
     double* out;     // [k][N]

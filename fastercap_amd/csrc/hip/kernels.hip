@@ -1023,14 +1023,17 @@ __global__ void __launch_bounds__(kSetupThreads) k_block_setup(BatchView bv, con
 
 // Two-level preconditioner matrix (ComputeSuperPrecond,
 // Solver/SolveCapacitance.cpp:2007-2066): element i's node self term on the
-// diagonal, link coefficients between element nodes elsewhere; row-major m x m.
-__global__ void k_super_build(BatchView bv, const int32_t* __restrict__ snode, int m, double* __restrict__ A) {
+// diagonal, link coefficients between element nodes elsewhere; column-major
+// m x m (A[j * m + i] = row i, column j).  X is set to the identity.
+__global__ void k_super_build(BatchView bv, const int32_t* __restrict__ snode, int m, double* __restrict__ A,
+                              double* __restrict__ X) {
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= (int64_t)m * m) return;
     const BJob& J = bv.jobs[0];
-    const int i = (int)(e / m), j = (int)(e % m);
+    const int j = (int)(e / m), i = (int)(e % m);
     const fcg_node_geom& ni = J.nodes[snode[i]];
     A[e] = (i == j) ? node_self(J, ni) : pair_coef(ni, J.nodes[snode[j]], J.dim);
+    X[e] = (i == j) ? 1.0 : 0.0;
 }
 
 // per leaf: its element's index weight (leaf dimension over element dimension)
@@ -1042,18 +1045,89 @@ __global__ void k_super_weights(BatchView bv, const int32_t* __restrict__ snode,
     w[i] = J.nodes[J.leaf_node[i]].area / J.nodes[snode[sidx[i]]].area;
 }
 
-__global__ void __launch_bounds__(kSetupThreads) k_super_invert(double* A, int m, int* piv, double* Sinv) {
+// Gauss-Jordan inversion of the column-major n x n matrix A into X (which holds
+// the identity on entry), spread over the whole device: per column j, one
+// workgroup picks the pivot (the first row of largest |entry| at or below the
+// diagonal, the choice of InvertMatrix's partial pivoting,
+// Solver/SolveCapacitance.cpp:2116-2258), swaps it into row j and scales that
+// row; then every other row is cleared in column j by a grid of workgroups.
+// Column j of A is left in place (it holds the multipliers) and never read
+// again.  X ends as the inverse, column-major.
+__global__ void __launch_bounds__(kSetupThreads) k_gj_pivot(double* __restrict__ A, double* __restrict__ X, int n,
+                                                            int j) {
     __shared__ double rv[kSetupThreads];
     __shared__ int ri[kSetupThreads];
-    lu_invert(A, m, piv, Sinv, rv, ri);
+    const int t = threadIdx.x;
+    double best = -1.0;
+    int bi = n;
+    for (int i = j + t; i < n; i += kSetupThreads) {
+        const double v = fabs(A[(size_t)j * n + i]);
+        if (v > best) {
+            best = v;
+            bi = i;
+        }
+    }
+    rv[t] = best;
+    ri[t] = bi;
+    __syncthreads();
+    for (int off = kSetupThreads / 2; off > 0; off >>= 1) {
+        if (t < off) {
+            const double v2 = rv[t + off];
+            const int i2 = ri[t + off];
+            if (v2 > rv[t] || (v2 == rv[t] && i2 < ri[t])) {
+                rv[t] = v2;
+                ri[t] = i2;
+            }
+        }
+        __syncthreads();
+    }
+    const int p = ri[0];
+    const double pv = A[(size_t)j * n + p];
+    __syncthreads();  // every thread holds the pivot before column j changes
+    const double rcp = 1.0 / pv;
+    // swap rows j and p and scale the new row j (columns < j of A are unit
+    // vectors by now and are left alone); with p == j the second store wins
+    for (int c = j + t; c < n; c += kSetupThreads) {
+        double* M = A + (size_t)c * n;
+        const double a = M[j], b = M[p];
+        M[p] = a;
+        M[j] = b * rcp;
+    }
+    for (int c = t; c < n; c += kSetupThreads) {
+        double* M = X + (size_t)c * n;
+        const double a = M[j], b = M[p];
+        M[p] = a;
+        M[j] = b * rcp;
+    }
+}
+
+constexpr int kGjCols = 8;  // columns per thread in the elimination
+__global__ void __launch_bounds__(256) k_gj_elim(double* __restrict__ A, double* __restrict__ X, int n, int j) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n || i == j) return;
+    const double f = A[(size_t)j * n + i];
+    // columns j+1..n-1 of A, then 0..n-1 of X: index c' in [0, 2n - j - 1)
+    const int c0 = blockIdx.y * kGjCols;
+    const int ncol = 2 * n - j - 1;
+#pragma unroll
+    for (int q = 0; q < kGjCols; ++q) {
+        const int c = c0 + q;
+        if (c >= ncol) break;
+        double* M = c < n - j - 1 ? A + (size_t)(j + 1 + c) * n : X + (size_t)(c - (n - j - 1)) * n;
+        M[i] -= f * M[j];
+    }
 }
 
 hipError_t launch_super_setup(const BatchView& one_job, int m, int N, const int32_t* snode, const int32_t* sidx,
-                              double* A, int* piv, double* Sinv, double* w, hipStream_t s) {
+                              double* A, double* Sinv, double* w, hipStream_t s) {
     if (m <= 0) return hipSuccess;
-    k_super_build<<<blocks_for((size_t)m * m, 256), 256, 0, s>>>(one_job, snode, m, A);
+    k_super_build<<<blocks_for((size_t)m * m, 256), 256, 0, s>>>(one_job, snode, m, A, Sinv);
     k_super_weights<<<blocks_for((size_t)N, 256), 256, 0, s>>>(one_job, snode, sidx, w);
-    k_super_invert<<<1, kSetupThreads, 0, s>>>(A, m, piv, Sinv);
+    for (int j = 0; j < m; ++j) {
+        k_gj_pivot<<<1, kSetupThreads, 0, s>>>(A, Sinv, m, j);
+        const dim3 grid((unsigned)((m + 255) / 256), (unsigned)((2 * m - j - 1 + kGjCols - 1) / kGjCols));
+        if (grid.y > 0) k_gj_elim<<<grid, 256, 0, s>>>(A, Sinv, m, j);
+    }
     return hipGetLastError();
 }
 

@@ -239,8 +239,10 @@ int fcg_mvp_timing_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec,
 
 /* Time every gather configuration (lanes per link, right-hand sides per lane,
  * links in flight) on this job with k right-hand sides and keep the fastest for
- * the job's later products.  fcg_solve calls it once per job and rhs count.  No
- * reference counterpart (the reference has a single CPU loop). */
+ * the job's later products.  Without it a job gathers with its rhs class's
+ * default configuration; a sweep costs about a hundred products, so it pays
+ * only for jobs solved many times.  No reference counterpart (the reference has
+ * a single CPU loop). */
 int fcg_autotune(fcg_ctx* ctx, int job, int k, int reps, int* variant, double* best_ms);
 
 /* Pin the gather configuration of a job (variant < 0 only queries the count). */
