@@ -16,6 +16,7 @@ Python mirror of the reference interfaces:
 from __future__ import annotations
 
 import ctypes as C
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -34,6 +35,7 @@ from ._ffi import (
     HostOptions,
     HostStats,
     JobDesc,
+    LinkParams,
     NodeGeom,
     SolveOpts,
     SolveStats,
@@ -110,15 +112,25 @@ class Mesh:
         self.group_names = [self.lib.fch_group_name(C.c_void_p(handle), g).decode() for g in range(self.G)]
 
     @classmethod
-    def build(cls, path: str, mesh_eps: float = 0.1, eps_ratio: float = 1.0, curv_coeff: float = 3.0) -> "Mesh":
+    def build(cls, path: str, mesh_eps: float = 0.1, eps_ratio: float = 1.0, curv_coeff: float = 3.0,
+              device: "Device | None" = None) -> "Mesh":
+        """Refine the geometry of `path` and discover its links: on the host, or
+        with `device` on that GPU (fcg_links_discover; the same link set)."""
         lib = _ffi.host_lib()
         o = HostOptions()
         lib.fch_default_options(C.byref(o))
         o.mesh_eps, o.eps_ratio, o.curv_coeff = mesh_eps, eps_ratio, curv_coeff
+        o.links_on_device = 1 if device is not None else 0
         h = C.c_void_p()
         rc = lib.fch_build(str(path).encode(), C.byref(o), C.byref(h))
         if rc != 0:
             raise FasterCapError(rc, lib.fch_last_error().decode())
+        if device is not None:
+            try:
+                device.discover_links(h.value)
+            except BaseException:
+                lib.fch_free(h)
+                raise
         return cls(h.value)
 
     def node_geom(self) -> np.ndarray:
@@ -177,6 +189,28 @@ class Device:
         if rc != 0:
             msg = self.lib.fcg_last_error(self._h)
             raise FasterCapError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def discover_links(self, mesh_handle: int) -> int:
+        """Discover the links of a mesh built with links_on_device on this device
+        and install them in the mesh; returns the link count."""
+        hl = _ffi.host_lib()
+        mh = C.c_void_p(mesh_handle)
+        d = JobDesc()
+        p = LinkParams()
+        if hl.fch_get_desc(mh, C.byref(d)) != 0 or hl.fch_get_link_params(mh, C.byref(p)) != 0:
+            raise FasterCapError(65, "bad mesh handle")
+        t0 = time.perf_counter()
+        n = C.c_int64()
+        self._check(self.lib.fcg_links_discover(self._h, C.byref(d), C.byref(p), C.byref(n)), "fcg_links_discover")
+        row = np.empty(d.n_nodes + 1, dtype=np.int64)
+        col = np.empty(max(1, n.value), dtype=np.int32)
+        self._check(self.lib.fcg_links_fetch(self._h, row.ctypes.data_as(C.POINTER(C.c_int64)),
+                                             col.ctypes.data_as(C.POINTER(C.c_int32))), "fcg_links_fetch")
+        rc = hl.fch_set_links(mh, row.ctypes.data_as(C.POINTER(C.c_int64)), col.ctypes.data_as(C.POINTER(C.c_int32)),
+                              n.value, time.perf_counter() - t0)
+        if rc != 0:
+            raise FasterCapError(rc, "fch_set_links rejected the discovered links")
+        return n.value
 
     def upload(self, mesh: Mesh) -> int:
         j = C.c_int()

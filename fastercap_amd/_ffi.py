@@ -48,6 +48,17 @@ class JobDesc(C.Structure):
     ]
 
 
+class LinkParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("reserved", C.c_int32),
+        ("link_eps", C.c_double),
+        ("max_side", C.c_double),
+        ("max_area", C.c_double),
+        ("max_length", C.c_double),
+    ]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [
         ("tol", C.c_double),
@@ -106,6 +117,7 @@ class HostOptions(C.Structure):
         ("eps_ratio", C.c_double),
         ("curv_coeff", C.c_double),
         ("verbose", C.c_int32),
+        ("links_on_device", C.c_int32),
     ]
 
 
@@ -177,6 +189,8 @@ DEVICE_SYMBOLS = (
     "fcg_ctx_profile",
     "fcg_autotune_batch",
     "fcg_set_batch_variant",
+    "fcg_links_discover",
+    "fcg_links_fetch",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -186,6 +200,8 @@ HOST_SYMBOLS = (
     "fch_group_name",
     "fch_last_error",
     "fch_free",
+    "fch_get_link_params",
+    "fch_set_links",
 )
 
 
@@ -218,6 +234,11 @@ def host_lib() -> C.CDLL:
         lib.fch_last_error.restype = C.c_char_p
         lib.fch_free.argtypes = [C.c_void_p]
         lib.fch_free.restype = None
+        lib.fch_get_link_params.argtypes = [C.c_void_p, C.POINTER(LinkParams)]
+        lib.fch_get_link_params.restype = C.c_int
+        lib.fch_set_links.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int64,
+                                      C.c_double]
+        lib.fch_set_links.restype = C.c_int
         _host = lib
     return _host
 
@@ -274,6 +295,8 @@ def device_lib() -> C.CDLL:
         lib.fcg_engine_bytes.argtypes = [vp, i64p]
         lib.fcg_engine_destroy.argtypes = [vp]
         lib.fcg_job_footprint.argtypes = [vp, C.c_int, C.c_int, C.c_int, i64p, i64p]
+        lib.fcg_links_discover.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(LinkParams), i64p]
+        lib.fcg_links_fetch.argtypes = [vp, i64p, C.POINTER(C.c_int32)]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p
         for name in DEVICE_SYMBOLS:

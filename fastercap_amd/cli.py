@@ -42,6 +42,7 @@ class Args:
     precision: int = 0  # FCG_PREC_FP64
     verbose: bool = False
     device: int = 0
+    host_links: bool = False  # -glinks-host: link discovery on the host builder
     stop: bool = False  # -b? / -bv
 
 
@@ -103,6 +104,8 @@ def parse_args(argv) -> Args:
             elif a.super_dim < 16:
                 print("Warning: minimum two-levels preconditioner dimension is 16, using 16", file=sys.stderr)
                 a.super_dim = 16
+        elif opt == "-glinks-host":
+            a.host_links = True
         elif opt == "-gmixed":
             a.precision = 1
         elif opt.startswith("-gdev"):
@@ -166,7 +169,8 @@ def preconditioner(a: Args, mesh):
 def solve_file(dev, a: Args, mesh_eps: float, tol: float):
     from . import Mesh
 
-    mesh = Mesh.build(a.file, mesh_eps=mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff)
+    mesh = Mesh.build(a.file, mesh_eps=mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff,
+                      device=None if a.host_links else dev)
     job = dev.upload(mesh)
     try:
         dev.fill(job, a.precision)

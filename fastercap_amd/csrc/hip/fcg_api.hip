@@ -52,6 +52,10 @@ struct DevBuf {
     T* as() const {
         return reinterpret_cast<T*>(p);
     }
+    void swap(DevBuf& o) {
+        std::swap(p, o.p);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 struct PinnedBuf {
@@ -315,6 +319,19 @@ struct fcg_ctx {
     std::vector<std::unique_ptr<Job>> jobs;
     std::string err;
     Tables tabs;  // tables of stand-alone calls (reset per call)
+    // result of the last fcg_links_discover, until fcg_links_fetch
+    struct LinkOut {
+        std::vector<int64_t> row_ptr;
+        DevBuf col;
+        int64_t L = 0;
+        bool ready = false;
+        void reset() {
+            row_ptr = std::vector<int64_t>();
+            col.release();
+            L = 0;
+            ready = false;
+        }
+    } links;
 
     fcg_ctx() {
         for (auto& row : batch_variant)
@@ -2378,6 +2395,149 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     // shape costs about a hundred products, more than a one-off solve, so it is
     // left to fcg_autotune / fcg_autotune_batch
     return run_batch(ctx, &job, 1, opts, charges != nullptr, cap, rows_out, cols_out, stats, charges);
+}
+
+// ---------------------------------------------------------------- link discovery
+// CAutoRefine::ComputeLinks and the record pass of AutoRefineLinks
+// (Solver/Autorefine.cpp:1132-1171, 481-756) on the device: a work stack of node
+// pairs processed in slices of kLinkSlice items (links.hip), then the
+// compressed rows, each sorted by source node.
+namespace {
+constexpr int64_t kLinkSlice = 8ll << 20;
+
+// b holds at least need bytes, keeping its first keep bytes
+int ensure_bytes(fcg_ctx* ctx, DevBuf& b, size_t need, size_t keep, hipStream_t s) {
+    if (b.bytes >= need) return FCG_OK;
+    DevBuf nb;
+    HIPCHECK(ctx, nb.alloc(std::max(need, b.bytes + b.bytes / 2)));
+    if (keep) HIPCHECK(ctx, hipMemcpyAsync(nb.p, b.p, keep, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    b.swap(nb);
+    return FCG_OK;
+}
+}  // namespace
+
+int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_params* lp, int64_t* n_links) {
+    if (!ctx || !d || !lp || !n_links) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    const int M = d->n_nodes;
+    if (M < 1 || !d->nodes || !d->left || !d->right || !d->parent || (d->dim != 2 && d->dim != 3) ||
+        lp->dim != d->dim || !(lp->link_eps > 0) || !(lp->max_side > 0 || d->dim == 2) ||
+        !(lp->max_area > 0 || d->dim == 2) || !(lp->max_length > 0 || d->dim == 3))
+        return fail(ctx, FCG_INVALID_ARGUMENT, "bad tree or link parameters");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    ctx->links.reset();
+    // post order (children before their parent) bounds the traversal
+    for (int i = 0; i < M; ++i) {
+        const int32_t l = d->left[i], r = d->right[i];
+        if ((l < 0) != (r < 0) || l >= i || r >= i)
+            return fail(ctx, FCG_INVALID_ARGUMENT, "bad tree: children must precede their parent");
+    }
+    DevBuf nodes, left, right;
+    HIPCHECK(ctx, upload(nodes, d->nodes, (size_t)M, s));
+    HIPCHECK(ctx, upload(left, d->left, (size_t)M, s));
+    HIPCHECK(ctx, upload(right, d->right, (size_t)M, s));
+    // ComputeLinks: every pair of roots, then every root's self item
+    std::vector<int32_t> roots;
+    for (int i = 0; i < M; ++i)
+        if (d->parent[i] < 0) roots.push_back(i);
+    std::vector<uint64_t> init;
+    for (size_t i = 0; i < roots.size(); ++i)
+        for (size_t j = i + 1; j < roots.size(); ++j)
+            init.push_back(((uint64_t)(uint32_t)roots[i] << 32) | (uint32_t)roots[j]);
+    for (int32_t r : roots) init.push_back((1ull << 63) | ((uint64_t)(uint32_t)r << 32));
+    DevBuf stack, out, links, ctr;
+    HIPCHECK(ctx, stack.alloc(sizeof(uint64_t) * std::max<size_t>(init.size(), (size_t)kLinkSlice * 4)));
+    HIPCHECK(ctx, hipMemcpyAsync(stack.p, init.data(), sizeof(uint64_t) * init.size(), hipMemcpyHostToDevice, s));
+    HIPCHECK(ctx, out.alloc(sizeof(uint64_t) * 3 * (size_t)kLinkSlice));
+    HIPCHECK(ctx, links.alloc(sizeof(int2) * std::max<size_t>((size_t)kLinkSlice, (size_t)M * 64)));
+    HIPCHECK(ctx, ctr.alloc(2 * sizeof(unsigned long long)));
+    unsigned long long* d_out = ctr.as<unsigned long long>();
+    unsigned long long* d_link = d_out + 1;
+    HIPCHECK(ctx, hipMemsetAsync(ctr.p, 0, ctr.bytes, s));
+    const fcg::LinkParams P{d->dim, lp->link_eps, lp->max_side, lp->max_area, lp->max_length};
+    int64_t top = (int64_t)init.size();
+    unsigned long long nlinks = 0;
+    while (top > 0) {
+        const int64_t n = std::min(top, kLinkSlice);
+        RCCHECK(ensure_bytes(ctx, links, sizeof(int2) * (size_t)(nlinks + (unsigned long long)n),
+                             sizeof(int2) * (size_t)nlinks, s));
+        HIPCHECK(ctx, hipMemsetAsync(d_out, 0, sizeof(unsigned long long), s));
+        HIPCHECK(ctx, fcg::launch_links_step(stack.as<uint64_t>() + (top - n), n, out.as<uint64_t>(), d_out,
+                                             links.as<int2>(), d_link, nodes.as<fcg_node_geom>(),
+                                             left.as<int32_t>(), right.as<int32_t>(), P, s));
+        unsigned long long h[2];
+        HIPCHECK(ctx, hipMemcpyAsync(h, ctr.p, sizeof h, hipMemcpyDeviceToHost, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        if (h[0] > 3ull * (unsigned long long)n || h[1] < nlinks || h[1] - nlinks > (unsigned long long)n)
+            return fail(ctx, FCG_DEVICE_ERROR, "link discovery: inconsistent step counts");
+        nlinks = h[1];
+        top -= n;
+        RCCHECK(ensure_bytes(ctx, stack, sizeof(uint64_t) * (size_t)(top + (int64_t)h[0]),
+                             sizeof(uint64_t) * (size_t)top, s));
+        if (h[0])
+            HIPCHECK(ctx, hipMemcpyAsync(stack.as<uint64_t>() + top, out.p, sizeof(uint64_t) * h[0],
+                                         hipMemcpyDeviceToDevice, s));
+        top += (int64_t)h[0];
+    }
+    stack.release();
+    out.release();
+    // compressed rows: degrees, offsets (host scan), scatter, per-row sort
+    const int64_t Pn = (int64_t)nlinks, L = 2 * Pn;
+    DevBuf deg, cursor, col;
+    HIPCHECK(ctx, deg.alloc(sizeof(int32_t) * (size_t)M));
+    HIPCHECK(ctx, hipMemsetAsync(deg.p, 0, deg.bytes, s));
+    HIPCHECK(ctx, fcg::launch_links_degree(links.as<int2>(), Pn, deg.as<int32_t>(), s));
+    std::vector<int32_t> hdeg((size_t)M);
+    HIPCHECK(ctx, hipMemcpyAsync(hdeg.data(), deg.p, sizeof(int32_t) * (size_t)M, hipMemcpyDeviceToHost, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    std::vector<int64_t> row_ptr((size_t)M + 1, 0);
+    for (int i = 0; i < M; ++i) row_ptr[(size_t)i + 1] = row_ptr[(size_t)i] + hdeg[(size_t)i];
+    if (row_ptr[(size_t)M] != L) return fail(ctx, FCG_DEVICE_ERROR, "link discovery: degree sum mismatch");
+    DevBuf drow;
+    HIPCHECK(ctx, upload(drow, row_ptr.data(), row_ptr.size(), s));
+    HIPCHECK(ctx, cursor.alloc(sizeof(unsigned long long) * (size_t)M));
+    HIPCHECK(ctx, hipMemcpyAsync(cursor.p, drow.p, sizeof(int64_t) * (size_t)M, hipMemcpyDeviceToDevice, s));
+    HIPCHECK(ctx, col.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(L, 1)));
+    HIPCHECK(ctx, fcg::launch_links_scatter(links.as<int2>(), Pn, cursor.as<unsigned long long>(),
+                                            col.as<int32_t>(), s));
+    HIPCHECK(ctx, fcg::launch_links_sort_rows(drow.as<int64_t>(), col.as<int32_t>(), M, s));
+    // rows beyond the workgroup sort's capacity are sorted here
+    const int cap = fcg::links_sort_lds_max();
+    std::vector<int32_t> tmp;
+    for (int i = 0; i < M; ++i) {
+        const int64_t b = row_ptr[(size_t)i], n = row_ptr[(size_t)i + 1] - b;
+        if (n <= cap) continue;
+        tmp.resize((size_t)n);
+        HIPCHECK(ctx, hipMemcpyAsync(tmp.data(), col.as<int32_t>() + b, sizeof(int32_t) * (size_t)n,
+                                     hipMemcpyDeviceToHost, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        std::sort(tmp.begin(), tmp.end());
+        HIPCHECK(ctx, hipMemcpyAsync(col.as<int32_t>() + b, tmp.data(), sizeof(int32_t) * (size_t)n,
+                                     hipMemcpyHostToDevice, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    ctx->links.row_ptr = std::move(row_ptr);
+    ctx->links.col.swap(col);
+    ctx->links.L = L;
+    ctx->links.ready = true;
+    *n_links = L;
+    return FCG_OK;
+}
+
+int fcg_links_fetch(fcg_ctx* ctx, int64_t* row_ptr, int32_t* col) {
+    if (!ctx || !row_ptr || (!col && ctx->links.L > 0)) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (!ctx->links.ready) return fail(ctx, FCG_INVALID_ARGUMENT, "no discovered links to fetch");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    std::memcpy(row_ptr, ctx->links.row_ptr.data(), sizeof(int64_t) * ctx->links.row_ptr.size());
+    if (ctx->links.L > 0) {
+        HIPCHECK(ctx, hipMemcpyAsync(col, ctx->links.col.p, sizeof(int32_t) * (size_t)ctx->links.L,
+                                     hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    ctx->links.reset();
+    return FCG_OK;
 }
 
 }  // extern "C"

@@ -265,4 +265,22 @@ hipError_t launch_charge_sums(const ChargeSumJob* jobs, int njobs, int64_t nbloc
 // squared norms of the k segments of a [k][N] vector into out[k]
 hipError_t launch_sqnorms(const double* x, int N, int k, double* out, hipStream_t s);
 
+// Link discovery (links.hip).  A work item is a node pair (a << 32 | b) or, with
+// bit 63 set, the self item of node a.
+struct LinkParams {
+    int dim;
+    double eps;  // link eps (eps ratio x mesh eps)
+    double max_side, max_area, max_length;
+};
+hipError_t launch_links_step(const uint64_t* in, int64_t n, uint64_t* out, unsigned long long* out_cnt, int2* links,
+                             unsigned long long* link_cnt, const fcg_node_geom* nodes, const int32_t* left,
+                             const int32_t* right, const LinkParams& P, hipStream_t s);
+hipError_t launch_links_degree(const int2* links, int64_t P, int32_t* deg, hipStream_t s);
+hipError_t launch_links_scatter(const int2* links, int64_t P, unsigned long long* cursor, int32_t* col,
+                                hipStream_t s);
+// sorts every row of at most links_sort_lds_max() entries; longer rows are left
+// to the caller
+hipError_t launch_links_sort_rows(const int64_t* row_ptr, int32_t* col, int M, hipStream_t s);
+int links_sort_lds_max();
+
 }  // namespace fcg

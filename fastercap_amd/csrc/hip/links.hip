@@ -1,0 +1,237 @@
+// Link discovery on the device: the dual-tree recursion of CAutoRefine::ComputeLinks
+// (Solver/Autorefine.cpp:1132-1171) with RefineMutual (:3096-3235 3D, :3239-3447
+// 2D), RefineSelf (:2115-2229, structure part) and RefineCriteria (:2928-3003 3D,
+// :3007-3092 2D), followed by the compressed-row build of the link record pass of
+// AutoRefineLinks (:481-756).
+//
+// The recursion becomes a work stack of node pairs processed in slices: a thread
+// takes one pair, evaluates the refinement criterion and either records a link or
+// pushes the two child pairs (a self item pushes the pair of its children and the
+// two children's self items).  Every decision must equal the host builder's, so
+// this file is compiled without floating-point contraction (no fused
+// multiply-add), evaluates every expression in the host's order and uses the
+// same quadrature routines.
+#pragma clang fp contract(off)
+
+#include <hip/hip_runtime.h>
+
+#include "../common/potential.h"
+#include "kernels.h"
+
+namespace fcg {
+namespace {
+
+struct D3 {
+    double x, y, z;
+};
+__device__ inline D3 centroid_d(const fcg_node_geom& g) { return D3{(double)g.c[0], (double)g.c[1], (double)g.c[2]}; }
+__device__ inline double dot_d(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+
+// RefineCriteria with curvature coefficient 1 (the link pass): true when the
+// pair must be refined further.
+__device__ bool link_refine(const fcg_node_geom& p1, const fcg_node_geom& p2, const LinkParams& P) {
+    const D3 c1 = centroid_d(p1), c2 = centroid_d(p2);
+    const D3 dist{c1.x - c2.x, c1.y - c2.y, c1.z - c2.z};
+    const double rdist = sqrt(dot_d(dist, dist));
+    const double r1 = p1.maxside, r2 = p2.maxside;  // 3D: longest side; 2D: length
+    const double rmax = (r1 > r2) ? r1 : r2;
+    const bool d1 = (p1.flags & FCG_NODE_DIEL) != 0, d2 = (p2.flags & FCG_NODE_DIEL) != 0;
+    const D3 n1{p1.dn[0], p1.dn[1], p1.dn[2]}, n2{p2.dn[0], p2.dn[1], p2.dn[2]};
+    const D3 mdist{-dist.x, -dist.y, -dist.z};
+    const double ccoeff = 1.0;
+    double pot1, pot2, e1, e2;
+    if (P.dim == 3) {
+        if (rdist / rmax < 2.0) {
+            const Vec3<double> r1v{c1.x, c1.y, c1.z}, r2v{c2.x, c2.y, c2.z};
+            pot1 = d1 ? enfield_num_3d<double>(r1v, p2.v, Vec3<double>{n1.x, n1.y, n1.z}, 2)
+                      : potential_num_3d<double>(r1v, p2.v, 2);
+            pot2 = d2 ? enfield_num_3d<double>(r2v, p1.v, Vec3<double>{n2.x, n2.y, n2.z}, 2)
+                      : potential_num_3d<double>(r2v, p1.v, 2);
+        } else {
+            pot1 = d1 ? dot_d(dist, n1) / (rdist * rdist * rdist) : 1.0 / rdist;
+            pot2 = d2 ? dot_d(mdist, n2) / (rdist * rdist * rdist) : 1.0 / rdist;
+        }
+        e1 = d1 ? fabs(pot1 * p2.area * p2.maxside / (P.max_side * P.max_area * ccoeff))
+                : fabs(pot1 * p2.area / (P.max_area * ccoeff));
+        e2 = d2 ? fabs(pot2 * p1.area * p1.maxside / (P.max_side * P.max_area * ccoeff))
+                : fabs(pot2 * p1.area / (P.max_area * ccoeff));
+    } else {
+        if (rdist / rmax < 2.0) {
+            pot1 = d1 ? enfield_2d<double>(c1.x, c1.y, p2.v, n1.x, n1.y) : potential_opt_2d<double>(c1.x, c1.y, p2.v);
+            pot2 = d2 ? enfield_2d<double>(c2.x, c2.y, p1.v, n2.x, n2.y) : potential_opt_2d<double>(c2.x, c2.y, p1.v);
+        } else {
+            pot1 = d1 ? dot_d(dist, n1) / (rdist * rdist) : -log(rdist);
+            pot2 = d2 ? dot_d(mdist, n2) / (rdist * rdist) : -log(rdist);
+        }
+        e1 = d1 ? fabs(pot1 * p2.area * p2.area / (P.max_length * P.max_length * ccoeff))
+                : fabs((pot1 + 0.0) * p2.area / (P.max_length * ccoeff));
+        e2 = d2 ? fabs(pot2 * p1.area * p1.area / (P.max_length * P.max_length * ccoeff))
+                : fabs((pot2 + 0.0) * p1.area / (P.max_length * ccoeff));
+    }
+    return e1 > P.eps || e2 > P.eps;
+}
+
+// Reserve cnt slots per lane from a global counter with one atomic per
+// wavefront; returns the lane's first slot.  All 64 lanes must call it.
+__device__ unsigned long long wave_reserve(unsigned long long* ctr, unsigned cnt) {
+    const int lane = threadIdx.x & 63;
+    unsigned incl = cnt;
+    for (int d = 1; d < 64; d <<= 1) {
+        const unsigned v = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += v;
+    }
+    const unsigned total = __shfl(incl, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 63 && total) base = atomicAdd(ctr, (unsigned long long)total);
+    base = __shfl(base, 63, 64);
+    return base + (incl - cnt);
+}
+
+constexpr uint64_t kSelfBit = 1ull << 63;
+__device__ inline uint64_t pair_item(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
+
+// One slice of the work stack.  out receives at most 3 items per input, links
+// at most one pair per input (the caller sizes both).
+__global__ void __launch_bounds__(256) k_links_step(const uint64_t* __restrict__ in, int64_t n,
+                                                    uint64_t* __restrict__ out, unsigned long long* out_cnt,
+                                                    int2* __restrict__ links, unsigned long long* link_cnt,
+                                                    const fcg_node_geom* __restrict__ nodes,
+                                                    const int32_t* __restrict__ left,
+                                                    const int32_t* __restrict__ right, LinkParams P) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t e0 = 0, e1 = 0, e2 = 0;
+    unsigned ne = 0;
+    int2 lk{0, 0};
+    unsigned nl = 0;
+    if (i < n) {
+        const uint64_t it = in[i];
+        const int32_t a = (int32_t)((it >> 32) & 0x7fffffffu), b = (int32_t)(uint32_t)it;
+        if (it & kSelfBit) {
+            const int32_t l = left[a];
+            if (l >= 0) {
+                const int32_t r = right[a];
+                e0 = pair_item(l, r);
+                e1 = kSelfBit | pair_item(l, 0);
+                e2 = kSelfBit | pair_item(r, 0);
+                ne = 3;
+            }
+        } else {
+            const int32_t l1 = left[a], l2 = left[b];
+            bool force = false;
+            if (l1 >= 0 || l2 >= 0) force = link_refine(nodes[a], nodes[b], P);
+            if (force) {
+                int which;
+                if (l1 >= 0 && l2 < 0)
+                    which = 1;
+                else if (l1 < 0 && l2 >= 0)
+                    which = 2;
+                else if (P.dim == 3)
+                    which = (nodes[a].area > nodes[b].area) ? 1 : 2;
+                else
+                    which = (nodes[a].maxside > nodes[b].maxside) ? 1 : 2;
+                if (which == 1) {
+                    e0 = pair_item(l1, b);
+                    e1 = pair_item(right[a], b);
+                } else {
+                    e0 = pair_item(a, l2);
+                    e1 = pair_item(a, right[b]);
+                }
+                ne = 2;
+            } else {
+                lk = int2{a, b};
+                nl = 1;
+            }
+        }
+    }
+    const unsigned long long eo = wave_reserve(out_cnt, ne);
+    if (ne > 0) out[eo] = e0;
+    if (ne > 1) out[eo + 1] = e1;
+    if (ne > 2) out[eo + 2] = e2;
+    const unsigned long long lo = wave_reserve(link_cnt, nl);
+    if (nl) links[lo] = lk;
+}
+
+// entries per node: a link (a, b) is an entry of row a (source b) and of row b
+__global__ void k_links_degree(const int2* __restrict__ links, int64_t P, int32_t* __restrict__ deg) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int2 l = links[i];
+    atomicAdd(&deg[l.x], 1);
+    atomicAdd(&deg[l.y], 1);
+}
+
+__global__ void k_links_scatter(const int2* __restrict__ links, int64_t P, unsigned long long* __restrict__ cursor,
+                                int32_t* __restrict__ col) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= P) return;
+    const int2 l = links[i];
+    col[atomicAdd(&cursor[l.x], 1ull)] = l.y;
+    col[atomicAdd(&cursor[l.y], 1ull)] = l.x;
+}
+
+// Sort every row of at most kSortLds entries ascending: one workgroup per row,
+// bitonic network in LDS over the row padded to a power of two.
+constexpr int kSortLds = 4096;
+__global__ void __launch_bounds__(256) k_links_sort_rows(const int64_t* __restrict__ row_ptr, int32_t* __restrict__ col,
+                                                         int M) {
+    __shared__ int32_t s[kSortLds];
+    const int row = blockIdx.x;
+    if (row >= M) return;
+    const int64_t b = row_ptr[row];
+    const int n = (int)(row_ptr[row + 1] - b);
+    if (n < 2 || n > kSortLds) return;
+    int P = 2;
+    while (P < n) P <<= 1;
+    for (int i = threadIdx.x; i < P; i += 256) s[i] = i < n ? col[b + i] : INT32_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += 256) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int32_t x = s[i], y = s[ixj];
+                    if (((i & k) == 0) ? (x > y) : (x < y)) {
+                        s[i] = y;
+                        s[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += 256) col[b + i] = s[i];
+}
+
+}  // namespace
+
+int links_sort_lds_max() { return kSortLds; }
+
+hipError_t launch_links_step(const uint64_t* in, int64_t n, uint64_t* out, unsigned long long* out_cnt, int2* links,
+                             unsigned long long* link_cnt, const fcg_node_geom* nodes, const int32_t* left,
+                             const int32_t* right, const LinkParams& P, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_links_step<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, n, out, out_cnt, links, link_cnt, nodes, left,
+                                                              right, P);
+    return hipGetLastError();
+}
+
+hipError_t launch_links_degree(const int2* links, int64_t P, int32_t* deg, hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    k_links_degree<<<(unsigned)((P + 255) / 256), 256, 0, s>>>(links, P, deg);
+    return hipGetLastError();
+}
+
+hipError_t launch_links_scatter(const int2* links, int64_t P, unsigned long long* cursor, int32_t* col,
+                                hipStream_t s) {
+    if (P <= 0) return hipSuccess;
+    k_links_scatter<<<(unsigned)((P + 255) / 256), 256, 0, s>>>(links, P, cursor, col);
+    return hipGetLastError();
+}
+
+hipError_t launch_links_sort_rows(const int64_t* row_ptr, int32_t* col, int M, hipStream_t s) {
+    if (M <= 0) return hipSuccess;
+    k_links_sort_rows<<<(unsigned)M, 256, 0, s>>>(row_ptr, col, M);
+    return hipGetLastError();
+}
+
+}  // namespace fcg

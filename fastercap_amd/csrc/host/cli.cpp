@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -35,6 +36,7 @@ struct Options {
     bool jacobi_set = false;  // -pj given (kept only without -ps, SolveCapacitance.cpp:122-125)
     int precision = FCG_PREC_FP64;
     int device = 0;
+    bool host_links = false;  // -glinks-host: link discovery on the host builder
 };
 
 void usage() {
@@ -55,7 +57,8 @@ void usage() {
         "  -b?      This text\n"
         "  -bv      Print only the version\n"
         "  -gmixed  Mixed precision (FP32 far field and products, FP64 refinement)\n"
-        "  -gdev<n> HIP device (default 0)\n");
+        "  -gdev<n> HIP device (default 0)\n"
+        "  -glinks-host  Discover the links on the host (default: on the device, same links)\n");
 }
 
 // returns 0 or FC_COMMAND_LINE_ERROR
@@ -129,6 +132,8 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
                     o.super_dim = 16;
                 }
             }
+        } else if (!std::strcmp(a, "-glinks-host")) {
+            o.host_links = true;
         } else if (!std::strcmp(a, "-gmixed")) {
             o.precision = FCG_PREC_MIXED;
         } else if (!std::strncmp(a, "-gdev", 5)) {
@@ -182,6 +187,7 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
     ho.eps_ratio = o.eps_ratio;
     ho.curv_coeff = o.curv;
     ho.verbose = o.verbose ? 1 : 0;
+    ho.links_on_device = o.host_links ? 0 : 1;
     fch_mesh* mesh = nullptr;
     int rc = fch_build(o.file.c_str(), &ho, &mesh);
     if (rc != 0) {
@@ -190,6 +196,25 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
     }
     fcg_job_desc d;
     fch_get_desc(mesh, &d);
+    if (!o.host_links) {
+        // the link structure from the device (fcg_links_discover), installed in the mesh
+        const auto t0 = std::chrono::steady_clock::now();
+        fcg_link_params lp;
+        fch_get_link_params(mesh, &lp);
+        int64_t L = 0;
+        rc = fcg_links_discover(ctx, &d, &lp, &L);
+        std::vector<int64_t> row_ptr((size_t)d.n_nodes + 1);
+        std::unique_ptr<int32_t[]> col(new int32_t[(size_t)std::max<int64_t>(L, 1)]);  // filled by the fetch
+        if (rc == FCG_OK) rc = fcg_links_fetch(ctx, row_ptr.data(), col.get());
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc == FCG_OK) rc = fch_set_links(mesh, row_ptr.data(), col.get(), L, t);
+        if (rc != FCG_OK) {
+            std::fprintf(stderr, "Error: link discovery: %s\n", fcg_last_error(ctx));
+            fch_free(mesh);
+            return rc;
+        }
+        fch_get_desc(mesh, &d);
+    }
     fch_get_stats(mesh, &r.hs);
     r.dim = d.dim;
     r.names.clear();

@@ -1125,6 +1125,7 @@ struct fch_mesh {
     std::vector<double> group_outperm, group_inperm, leaf_outperm;
     std::vector<std::string> group_names;
     fch_stats stats{};
+    fcg_link_params link_params{};
 };
 
 extern "C" {
@@ -1134,6 +1135,7 @@ void fch_default_options(fch_options* o) {
     o->eps_ratio = 1.0;
     o->curv_coeff = 3.0;
     o->verbose = 0;
+    o->links_on_device = 0;
 }
 
 const char* fch_last_error(void) { return g_err.c_str(); }
@@ -1195,7 +1197,7 @@ int fch_build(const char* path, const fch_options* opts, fch_mesh** out) {
         }
         b.computeLinks = false;
         b.linksNum = 0;
-        b.computeLinksPass();
+        if (!o.links_on_device) b.computeLinksPass();
         auto t3 = clk::now();
 
         // AutoRefineLinks structure part: RecurseIndex, then the record pass.
@@ -1214,11 +1216,16 @@ int fch_build(const char* path, const fch_options* opts, fch_mesh** out) {
             throw std::runtime_error("mesh too large");
         b.colOut.assign((size_t)b.linksNum, -1);
         b.computeLinks = true;
-        b.computeLinksPass();
+        if (!o.links_on_device) b.computeLinksPass();
         auto t4 = clk::now();
 
         auto m = std::make_unique<fch_mesh>();
         m->dim = b.dim;
+        m->link_params.dim = b.dim;
+        m->link_params.link_eps = b.linkEps;
+        m->link_params.max_side = b.maxSide;
+        m->link_params.max_area = b.maxArea;
+        m->link_params.max_length = b.maxLength;
         const int64_t M = b.nodeNum, N = b.panelNum, L = b.linksNum;
         m->nodes.resize((size_t)M);
         m->left.assign((size_t)M, -1);
@@ -1357,6 +1364,27 @@ int fch_get_desc(const fch_mesh* m, fcg_job_desc* d) {
     d->group_outperm = m->group_outperm.data();
     d->group_inperm = m->group_inperm.data();
     d->leaf_outperm = m->leaf_outperm.data();
+    return FCG_OK;
+}
+
+int fch_get_link_params(const fch_mesh* m, fcg_link_params* p) {
+    if (!m || !p) return FCG_INVALID_ARGUMENT;
+    *p = m->link_params;
+    return FCG_OK;
+}
+
+int fch_set_links(fch_mesh* m, const int64_t* row_ptr, const int32_t* col, int64_t L, double t_links_s) {
+    if (!m || !row_ptr || L < 0 || (L > 0 && !col)) return FCG_INVALID_ARGUMENT;
+    const size_t M = m->nodes.size();
+    if (row_ptr[0] != 0 || row_ptr[M] != L) return FCG_INVALID_ARGUMENT;
+    for (size_t i = 0; i < M; ++i)
+        if (row_ptr[i + 1] < row_ptr[i]) return FCG_INVALID_ARGUMENT;
+    for (int64_t k = 0; k < L; ++k)
+        if (col[k] < 0 || (size_t)col[k] >= M) return FCG_INVALID_ARGUMENT;
+    m->row_ptr.assign(row_ptr, row_ptr + M + 1);
+    m->col.assign(col, col + L);
+    m->stats.links = L;
+    m->stats.t_links_s = t_links_s;
     return FCG_OK;
 }
 

@@ -186,6 +186,38 @@ int fcg_block_precond_apply(fcg_ctx* ctx, int job, int block_size, const double*
  * (:3545-3592). */
 int fcg_super_precond_apply(fcg_ctx* ctx, int job, int dim, const double* q, double* pq, int k);
 
+/* ---------------------------------------------------------------- link discovery */
+
+/* Link criterion parameters of a refined mesh (fch_get_link_params). */
+typedef struct fcg_link_params {
+    int32_t dim;        /* 3 or 2 */
+    int32_t reserved;
+    double link_eps;    /* eps ratio x mesh eps (-d, -m; 0 -> 0.01) */
+    double max_side;    /* 3D: longest panel side of the input */
+    double max_area;    /* 3D: largest panel area of the input */
+    double max_length;  /* 2D: longest segment of the input */
+} fcg_link_params;
+
+/* Discover the link structure of a refined tree on the device: every pair of
+ * group roots and every root's own tree are walked as CAutoRefine::ComputeLinks
+ * does, a pair becoming a link when RefineCriteria (curvature coefficient 1) no
+ * longer asks for refinement or both are leaves, else the larger node is split.
+ * The criterion is evaluated as the host builder evaluates it (FP64, no fused
+ * operations), so the link set equals the host builder's.  Uses d->dim, n_nodes,
+ * nodes, left, right (children before their parent) and parent (-1 at roots);
+ * the CSR fields are ignored.  *n_links receives L; the rows stay on the device
+ * until fcg_links_fetch.
+ * Replaces ComputeLinks, RefineMutual, RefineSelf (structure part),
+ * RefineCriteria and the record pass of AutoRefineLinks
+ * (Solver/Autorefine.cpp:1132-1171, 3096-3447, 2115-2229, 2928-3092, 481-756). */
+int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_params* lp, int64_t* n_links);
+
+/* Copy the last discovery's rows to the host: row_ptr[n_nodes + 1], col[L],
+ * every row sorted by source node id (the host builder records each row in
+ * traversal order; the device layout sorts rows anyway).  Frees them on the
+ * device. */
+int fcg_links_fetch(fcg_ctx* ctx, int64_t* row_ptr, int32_t* col);
+
 /* Solve every conductor right-hand side of the job with right-preconditioned GMRES
  * and assemble the Maxwell capacitance matrix cap[rows][n_cond] (row-major; rows =
  * n_cond in 3D, n_cond-1 in 2D; 2D values unscaled, as the reference keeps them

@@ -12,6 +12,11 @@
  *               record pass of AutoRefineLinks (Solver/Autorefine.cpp:4568-4674,
  *               6410-6544, 293-479, 481-756; RecurseIndex :1173-1220)
  *   fch_options CAutoRefGlobalVars mesh fields and defaults (Solver/SolverGlobal.cpp:238-279)
+ *
+ * With fch_options.links_on_device the builder stops after the discretization
+ * and the node indexing; the caller discovers the links on the device
+ * (fcg_links_discover with fch_get_link_params, then fcg_links_fetch) and hands
+ * them back with fch_set_links.
  */
 #ifndef FASTERCAP_HOST_H
 #define FASTERCAP_HOST_H
@@ -27,6 +32,7 @@ typedef struct fch_options {
     double eps_ratio;  /* -d  m_dEpsRatio (default 1.0); link eps = eps_ratio * mesh_eps */
     double curv_coeff; /* -mc m_dMeshCurvCoeff (default 3.0, clamped to >= 1) */
     int32_t verbose;   /* -v */
+    int32_t links_on_device; /* 1: no link passes here; see fch_set_links */
 } fch_options;
 
 typedef struct fch_stats {
@@ -44,6 +50,14 @@ void fch_default_options(fch_options* opts);
 
 /* Build a refined mesh with its link structure. Returns an FC_* code. */
 int fch_build(const char* path, const fch_options* opts, fch_mesh** out);
+
+/* Parameters of the link criterion of this mesh (for fcg_links_discover). */
+int fch_get_link_params(const fch_mesh* mesh, fcg_link_params* params);
+
+/* Install a link structure discovered elsewhere: row_ptr[nodes + 1] (row_ptr[0]
+ * = 0, non-decreasing, row_ptr[nodes] = n_links) and col[n_links] (node ids).
+ * t_links_s is recorded in the statistics. */
+int fch_set_links(fch_mesh* mesh, const int64_t* row_ptr, const int32_t* col, int64_t n_links, double t_links_s);
 
 /* Point desc at the mesh's arrays (valid while the mesh lives). */
 int fch_get_desc(const fch_mesh* mesh, fcg_job_desc* desc);

@@ -70,3 +70,40 @@ def test_matrix_printout():
     txt = cli.format_cap(np.array([[1.0, -1.0]]), ["inner", "outer"], 2)
     assert txt.splitlines()[1] == "Dimension 1 x 1"
     assert txt.splitlines()[2] == "inner  5.56325e-11 "
+
+
+def test_host_build_without_links_and_set_links(tmp_path):
+    # links_on_device stops the host builder before its link passes; fch_set_links
+    # installs rows found elsewhere and rejects malformed ones
+    import ctypes as C
+
+    import fastercap_amd as fc
+    from fastercap_amd import _ffi, workloads
+
+    path = workloads.two_plates(str(tmp_path))
+    ref = fc.Mesh.build(path, mesh_eps=0.1)
+    lib = _ffi.host_lib()
+    o = _ffi.HostOptions()
+    lib.fch_default_options(C.byref(o))
+    o.mesh_eps, o.links_on_device = 0.1, 1
+    h = C.c_void_p()
+    assert lib.fch_build(path.encode(), C.byref(o), C.byref(h)) == 0
+    bare = fc.Mesh(h.value)
+    assert (bare.N, bare.M, bare.L) == (ref.N, ref.M, 0)
+    p = _ffi.LinkParams()
+    assert lib.fch_get_link_params(h, C.byref(p)) == 0
+    assert p.dim == 3 and p.link_eps == 0.1 and p.max_side > 0 and p.max_area > 0
+    i64 = C.POINTER(C.c_int64)
+    i32 = C.POINTER(C.c_int32)
+    row, col = ref.row_ptr.copy(), ref.col.copy()
+    bad = row.copy()
+    bad[1] = -1
+    assert lib.fch_set_links(h, bad.ctypes.data_as(i64), col.ctypes.data_as(i32), ref.L, 0.0) != 0
+    badc = col.copy()
+    badc[0] = ref.M
+    assert lib.fch_set_links(h, row.ctypes.data_as(i64), badc.ctypes.data_as(i32), ref.L, 0.0) != 0
+    assert lib.fch_set_links(h, row.ctypes.data_as(i64), col.ctypes.data_as(i32), ref.L, 0.5) == 0
+    full = fc.Mesh(h.value)
+    bare._h = None  # the second wrapper owns the handle
+    assert full.L == ref.L and full.stats.links == ref.L and full.stats.t_links_s == 0.5
+    np.testing.assert_array_equal(full.col, ref.col)

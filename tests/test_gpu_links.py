@@ -1,0 +1,63 @@
+"""GPU tests of link discovery on the device (fcg_links_discover): the link set
+of every mesh must equal the host builder's (the rows compared after sorting each
+by source node), on 3D and 2D meshes, with and without dielectric interfaces,
+and on a mesh whose traversal needs several work-stack slices; a solve on the
+device-discovered mesh must reproduce the host-built one."""
+import os
+
+import numpy as np
+import pytest
+
+import fastercap_amd as fc
+from fastercap_amd import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def dev():
+    d = fc.Device(0)
+    yield d
+    d.close()
+
+
+def _cases(geo_dir):
+    os.makedirs(geo_dir, exist_ok=True)
+    return {
+        "plates": (workloads.two_plates(os.path.join(geo_dir, "lk_plates")), 0.02),
+        "boxes_interface": (workloads.random_boxes(os.path.join(geo_dir, "lk_boxes"), n=6, seed=4), 0.15),
+        "coax2d": (workloads.coated_coax_2d(os.path.join(geo_dir, "lk_coax")), 0.02),
+        "sphere_shell": (workloads.sphere_in_shell(os.path.join(geo_dir, "lk_sphere")), 0.05),
+        "bus": (workloads.crossing_bus(os.path.join(geo_dir, "lk_bus")), 600.0),  # 48M links: many slices
+    }
+
+
+def _sorted_rows(m):
+    rows = np.repeat(np.arange(m.M, dtype=np.int64), np.diff(m.row_ptr))
+    order = np.lexsort((m.col, rows))
+    return m.col[order]
+
+
+@pytest.mark.parametrize("name", ["plates", "boxes_interface", "coax2d", "sphere_shell", "bus"])
+def test_device_links_equal_host_links(dev, geo_dir, name):
+    path, eps = _cases(geo_dir)[name]
+    host = fc.Mesh.build(path, mesh_eps=eps)
+    devm = fc.Mesh.build(path, mesh_eps=eps, device=dev)
+    print(f"links {name}: N={host.N} M={host.M} L={host.L} host {host.stats.t_links_s:.3f}s "
+          f"device {devm.stats.t_links_s:.3f}s", flush=True)
+    assert (devm.N, devm.M, devm.L) == (host.N, host.M, host.L)
+    np.testing.assert_array_equal(devm.row_ptr, host.row_ptr)
+    np.testing.assert_array_equal(devm.col, _sorted_rows(host))
+    assert devm.stats.links == host.stats.links
+
+
+def test_solve_on_device_links_matches_host_mesh(dev, geo_dir):
+    path, eps = _cases(geo_dir)["boxes_interface"]
+    res = []
+    for kw in ({}, {"device": dev}):
+        m = fc.Mesh.build(path, mesh_eps=eps, **kw)
+        job = dev.upload(m)
+        dev.fill(job, fc.FCG_PREC_FP64)
+        res.append(dev.solve(job, m, tol=1e-8).cap)
+        dev.release(job)
+    np.testing.assert_allclose(res[1], res[0], rtol=1e-12, atol=1e-12 * np.abs(res[0]).max())
