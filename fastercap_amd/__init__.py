@@ -201,13 +201,15 @@ class Device:
             raise FasterCapError(65, "bad mesh handle")
         t0 = time.perf_counter()
         n = C.c_int64()
-        self._check(self.lib.fcg_links_discover(self._h, C.byref(d), C.byref(p), C.byref(n)), "fcg_links_discover")
+        lev = C.c_int32()
+        self._check(self.lib.fcg_links_discover(self._h, C.byref(d), C.byref(p), C.byref(n), C.byref(lev)),
+                    "fcg_links_discover")
         row = np.empty(d.n_nodes + 1, dtype=np.int64)
         col = np.empty(max(1, n.value), dtype=np.int32)
         self._check(self.lib.fcg_links_fetch(self._h, row.ctypes.data_as(C.POINTER(C.c_int64)),
                                              col.ctypes.data_as(C.POINTER(C.c_int32))), "fcg_links_fetch")
         rc = hl.fch_set_links(mh, row.ctypes.data_as(C.POINTER(C.c_int64)), col.ctypes.data_as(C.POINTER(C.c_int32)),
-                              n.value, time.perf_counter() - t0)
+                              n.value, lev.value, time.perf_counter() - t0)
         if rc != 0:
             raise FasterCapError(rc, "fch_set_links rejected the discovered links")
         return n.value

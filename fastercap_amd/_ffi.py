@@ -237,7 +237,7 @@ def host_lib() -> C.CDLL:
         lib.fch_get_link_params.argtypes = [C.c_void_p, C.POINTER(LinkParams)]
         lib.fch_get_link_params.restype = C.c_int
         lib.fch_set_links.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int64,
-                                      C.c_double]
+                                      C.c_int32, C.c_double]
         lib.fch_set_links.restype = C.c_int
         _host = lib
     return _host
@@ -295,7 +295,8 @@ def device_lib() -> C.CDLL:
         lib.fcg_engine_bytes.argtypes = [vp, i64p]
         lib.fcg_engine_destroy.argtypes = [vp]
         lib.fcg_job_footprint.argtypes = [vp, C.c_int, C.c_int, C.c_int, i64p, i64p]
-        lib.fcg_links_discover.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(LinkParams), i64p]
+        lib.fcg_links_discover.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(LinkParams), i64p,
+                                           C.POINTER(C.c_int32)]
         lib.fcg_links_fetch.argtypes = [vp, i64p, C.POINTER(C.c_int32)]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p

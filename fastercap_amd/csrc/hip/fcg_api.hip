@@ -2417,7 +2417,8 @@ int ensure_bytes(fcg_ctx* ctx, DevBuf& b, size_t need, size_t keep, hipStream_t 
 }
 }  // namespace
 
-int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_params* lp, int64_t* n_links) {
+int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_params* lp, int64_t* n_links,
+                       int32_t* max_level) {
     if (!ctx || !d || !lp || !n_links) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     const int M = d->n_nodes;
     if (M < 1 || !d->nodes || !d->left || !d->right || !d->parent || (d->dim != 2 && d->dim != 3) ||
@@ -2446,14 +2447,21 @@ int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_param
         for (size_t j = i + 1; j < roots.size(); ++j)
             init.push_back(((uint64_t)(uint32_t)roots[i] << 32) | (uint32_t)roots[j]);
     for (int32_t r : roots) init.push_back((1ull << 63) | ((uint64_t)(uint32_t)r << 32));
-    DevBuf stack, out, links, ctr;
-    HIPCHECK(ctx, stack.alloc(sizeof(uint64_t) * std::max<size_t>(init.size(), (size_t)kLinkSlice * 4)));
+    // the items' recursion levels (both kinds start at 1, as the top calls)
+    const std::vector<uint8_t> init_lev(init.size(), 1);
+    DevBuf stack, slev, out, olev, links, ctr;
+    const size_t cap0 = std::max<size_t>(init.size(), (size_t)kLinkSlice * 4);
+    HIPCHECK(ctx, stack.alloc(sizeof(uint64_t) * cap0));
+    HIPCHECK(ctx, slev.alloc(cap0));
     HIPCHECK(ctx, hipMemcpyAsync(stack.p, init.data(), sizeof(uint64_t) * init.size(), hipMemcpyHostToDevice, s));
+    HIPCHECK(ctx, hipMemcpyAsync(slev.p, init_lev.data(), init_lev.size(), hipMemcpyHostToDevice, s));
     HIPCHECK(ctx, out.alloc(sizeof(uint64_t) * 3 * (size_t)kLinkSlice));
+    HIPCHECK(ctx, olev.alloc(3 * (size_t)kLinkSlice));
     HIPCHECK(ctx, links.alloc(sizeof(int2) * std::max<size_t>((size_t)kLinkSlice, (size_t)M * 64)));
-    HIPCHECK(ctx, ctr.alloc(2 * sizeof(unsigned long long)));
+    HIPCHECK(ctx, ctr.alloc(3 * sizeof(unsigned long long)));
     unsigned long long* d_out = ctr.as<unsigned long long>();
     unsigned long long* d_link = d_out + 1;
+    int* d_lev = reinterpret_cast<int*>(d_out + 2);
     HIPCHECK(ctx, hipMemsetAsync(ctr.p, 0, ctr.bytes, s));
     const fcg::LinkParams P{d->dim, lp->link_eps, lp->max_side, lp->max_area, lp->max_length};
     int64_t top = (int64_t)init.size();
@@ -2463,9 +2471,10 @@ int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_param
         RCCHECK(ensure_bytes(ctx, links, sizeof(int2) * (size_t)(nlinks + (unsigned long long)n),
                              sizeof(int2) * (size_t)nlinks, s));
         HIPCHECK(ctx, hipMemsetAsync(d_out, 0, sizeof(unsigned long long), s));
-        HIPCHECK(ctx, fcg::launch_links_step(stack.as<uint64_t>() + (top - n), n, out.as<uint64_t>(), d_out,
-                                             links.as<int2>(), d_link, nodes.as<fcg_node_geom>(),
-                                             left.as<int32_t>(), right.as<int32_t>(), P, s));
+        HIPCHECK(ctx, fcg::launch_links_step(stack.as<uint64_t>() + (top - n), slev.as<uint8_t>() + (top - n), n,
+                                             out.as<uint64_t>(), olev.as<uint8_t>(), d_out, links.as<int2>(), d_link,
+                                             d_lev, nodes.as<fcg_node_geom>(), left.as<int32_t>(),
+                                             right.as<int32_t>(), P, s));
         unsigned long long h[2];
         HIPCHECK(ctx, hipMemcpyAsync(h, ctr.p, sizeof h, hipMemcpyDeviceToHost, s));
         HIPCHECK(ctx, hipStreamSynchronize(s));
@@ -2475,13 +2484,21 @@ int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_param
         top -= n;
         RCCHECK(ensure_bytes(ctx, stack, sizeof(uint64_t) * (size_t)(top + (int64_t)h[0]),
                              sizeof(uint64_t) * (size_t)top, s));
-        if (h[0])
+        RCCHECK(ensure_bytes(ctx, slev, (size_t)(top + (int64_t)h[0]), (size_t)top, s));
+        if (h[0]) {
             HIPCHECK(ctx, hipMemcpyAsync(stack.as<uint64_t>() + top, out.p, sizeof(uint64_t) * h[0],
                                          hipMemcpyDeviceToDevice, s));
+            HIPCHECK(ctx, hipMemcpyAsync(slev.as<uint8_t>() + top, olev.p, h[0], hipMemcpyDeviceToDevice, s));
+        }
         top += (int64_t)h[0];
     }
+    int maxlev = 0;
+    HIPCHECK(ctx, hipMemcpyAsync(&maxlev, d_lev, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
     stack.release();
+    slev.release();
     out.release();
+    olev.release();
     // compressed rows: degrees, offsets (host scan), scatter, per-row sort
     const int64_t Pn = (int64_t)nlinks, L = 2 * Pn;
     DevBuf deg, cursor, col;
@@ -2523,6 +2540,7 @@ int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_param
     ctx->links.L = L;
     ctx->links.ready = true;
     *n_links = L;
+    if (max_level) *max_level = maxlev;
     return FCG_OK;
 }
 

@@ -272,9 +272,10 @@ struct LinkParams {
     double eps;  // link eps (eps ratio x mesh eps)
     double max_side, max_area, max_length;
 };
-hipError_t launch_links_step(const uint64_t* in, int64_t n, uint64_t* out, unsigned long long* out_cnt, int2* links,
-                             unsigned long long* link_cnt, const fcg_node_geom* nodes, const int32_t* left,
-                             const int32_t* right, const LinkParams& P, hipStream_t s);
+hipError_t launch_links_step(const uint64_t* in, const uint8_t* in_lev, int64_t n, uint64_t* out, uint8_t* out_lev,
+                             unsigned long long* out_cnt, int2* links, unsigned long long* link_cnt, int* max_lev,
+                             const fcg_node_geom* nodes, const int32_t* left, const int32_t* right,
+                             const LinkParams& P, hipStream_t s);
 hipError_t launch_links_degree(const int2* links, int64_t P, int32_t* deg, hipStream_t s);
 hipError_t launch_links_scatter(const int2* links, int64_t P, unsigned long long* cursor, int32_t* col,
                                 hipStream_t s);

@@ -91,10 +91,14 @@ constexpr uint64_t kSelfBit = 1ull << 63;
 __device__ inline uint64_t pair_item(int32_t a, int32_t b) { return ((uint64_t)(uint32_t)a << 32) | (uint32_t)b; }
 
 // One slice of the work stack.  out receives at most 3 items per input, links
-// at most one pair per input (the caller sizes both).
-__global__ void __launch_bounds__(256) k_links_step(const uint64_t* __restrict__ in, int64_t n,
-                                                    uint64_t* __restrict__ out, unsigned long long* out_cnt,
-                                                    int2* __restrict__ links, unsigned long long* link_cnt,
+// at most one pair per input (the caller sizes both).  Every item carries its
+// recursion level (in_lev / out_lev: RefineMutual and RefineSelf count one per
+// call); max_lev receives the deepest.
+__global__ void __launch_bounds__(256) k_links_step(const uint64_t* __restrict__ in,
+                                                    const uint8_t* __restrict__ in_lev, int64_t n,
+                                                    uint64_t* __restrict__ out, uint8_t* __restrict__ out_lev,
+                                                    unsigned long long* out_cnt, int2* __restrict__ links,
+                                                    unsigned long long* link_cnt, int* max_lev,
                                                     const fcg_node_geom* __restrict__ nodes,
                                                     const int32_t* __restrict__ left,
                                                     const int32_t* __restrict__ right, LinkParams P) {
@@ -103,7 +107,9 @@ __global__ void __launch_bounds__(256) k_links_step(const uint64_t* __restrict__
     unsigned ne = 0;
     int2 lk{0, 0};
     unsigned nl = 0;
+    int lev = 0;
     if (i < n) {
+        lev = in_lev[i];
         const uint64_t it = in[i];
         const int32_t a = (int32_t)((it >> 32) & 0x7fffffffu), b = (int32_t)(uint32_t)it;
         if (it & kSelfBit) {
@@ -144,11 +150,23 @@ __global__ void __launch_bounds__(256) k_links_step(const uint64_t* __restrict__
         }
     }
     const unsigned long long eo = wave_reserve(out_cnt, ne);
-    if (ne > 0) out[eo] = e0;
-    if (ne > 1) out[eo + 1] = e1;
-    if (ne > 2) out[eo + 2] = e2;
+    const uint8_t cl = (uint8_t)(lev < 255 ? lev + 1 : 255);
+    if (ne > 0) {
+        out[eo] = e0;
+        out_lev[eo] = cl;
+    }
+    if (ne > 1) {
+        out[eo + 1] = e1;
+        out_lev[eo + 1] = cl;
+    }
+    if (ne > 2) {
+        out[eo + 2] = e2;
+        out_lev[eo + 2] = cl;
+    }
     const unsigned long long lo = wave_reserve(link_cnt, nl);
     if (nl) links[lo] = lk;
+    for (int d = 32; d > 0; d >>= 1) lev = max(lev, __shfl_xor(lev, d, 64));
+    if ((threadIdx.x & 63) == 0 && lev > 0) atomicMax(max_lev, lev);
 }
 
 // entries per node: a link (a, b) is an entry of row a (source b) and of row b
@@ -206,12 +224,13 @@ __global__ void __launch_bounds__(256) k_links_sort_rows(const int64_t* __restri
 
 int links_sort_lds_max() { return kSortLds; }
 
-hipError_t launch_links_step(const uint64_t* in, int64_t n, uint64_t* out, unsigned long long* out_cnt, int2* links,
-                             unsigned long long* link_cnt, const fcg_node_geom* nodes, const int32_t* left,
-                             const int32_t* right, const LinkParams& P, hipStream_t s) {
+hipError_t launch_links_step(const uint64_t* in, const uint8_t* in_lev, int64_t n, uint64_t* out, uint8_t* out_lev,
+                             unsigned long long* out_cnt, int2* links, unsigned long long* link_cnt, int* max_lev,
+                             const fcg_node_geom* nodes, const int32_t* left, const int32_t* right,
+                             const LinkParams& P, hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    k_links_step<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, n, out, out_cnt, links, link_cnt, nodes, left,
-                                                              right, P);
+    k_links_step<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(in, in_lev, n, out, out_lev, out_cnt, links, link_cnt,
+                                                              max_lev, nodes, left, right, P);
     return hipGetLastError();
 }
 

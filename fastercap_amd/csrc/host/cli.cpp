@@ -202,12 +202,13 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
         fcg_link_params lp;
         fch_get_link_params(mesh, &lp);
         int64_t L = 0;
-        rc = fcg_links_discover(ctx, &d, &lp, &L);
+        int32_t maxlev = 0;
+        rc = fcg_links_discover(ctx, &d, &lp, &L, &maxlev);
         std::vector<int64_t> row_ptr((size_t)d.n_nodes + 1);
         std::unique_ptr<int32_t[]> col(new int32_t[(size_t)std::max<int64_t>(L, 1)]);  // filled by the fetch
         if (rc == FCG_OK) rc = fcg_links_fetch(ctx, row_ptr.data(), col.get());
         const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (rc == FCG_OK) rc = fch_set_links(mesh, row_ptr.data(), col.get(), L, t);
+        if (rc == FCG_OK) rc = fch_set_links(mesh, row_ptr.data(), col.get(), L, maxlev, t);
         if (rc != FCG_OK) {
             std::fprintf(stderr, "Error: link discovery: %s\n", fcg_last_error(ctx));
             fch_free(mesh);

@@ -205,12 +205,14 @@ typedef struct fcg_link_params {
  * The criterion is evaluated as the host builder evaluates it (FP64, no fused
  * operations), so the link set equals the host builder's.  Uses d->dim, n_nodes,
  * nodes, left, right (children before their parent) and parent (-1 at roots);
- * the CSR fields are ignored.  *n_links receives L; the rows stay on the device
- * until fcg_links_fetch.
+ * the CSR fields are ignored.  *n_links receives L and *max_level (may be NULL)
+ * the deepest recursion level of the traversal (the top calls are level 1); the
+ * rows stay on the device until fcg_links_fetch.
  * Replaces ComputeLinks, RefineMutual, RefineSelf (structure part),
  * RefineCriteria and the record pass of AutoRefineLinks
  * (Solver/Autorefine.cpp:1132-1171, 3096-3447, 2115-2229, 2928-3092, 481-756). */
-int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_params* lp, int64_t* n_links);
+int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_params* lp, int64_t* n_links,
+                       int32_t* max_level);
 
 /* Copy the last discovery's rows to the host: row_ptr[n_nodes + 1], col[L],
  * every row sorted by source node id (the host builder records each row in

@@ -56,8 +56,10 @@ int fch_get_link_params(const fch_mesh* mesh, fcg_link_params* params);
 
 /* Install a link structure discovered elsewhere: row_ptr[nodes + 1] (row_ptr[0]
  * = 0, non-decreasing, row_ptr[nodes] = n_links) and col[n_links] (node ids).
- * t_links_s is recorded in the statistics. */
-int fch_set_links(fch_mesh* mesh, const int64_t* row_ptr, const int32_t* col, int64_t n_links, double t_links_s);
+ * max_level (the traversal's deepest recursion level) and t_links_s go into the
+ * statistics. */
+int fch_set_links(fch_mesh* mesh, const int64_t* row_ptr, const int32_t* col, int64_t n_links, int32_t max_level,
+                  double t_links_s);
 
 /* Point desc at the mesh's arrays (valid while the mesh lives). */
 int fch_get_desc(const fch_mesh* mesh, fcg_job_desc* desc);

@@ -98,12 +98,13 @@ def test_host_build_without_links_and_set_links(tmp_path):
     row, col = ref.row_ptr.copy(), ref.col.copy()
     bad = row.copy()
     bad[1] = -1
-    assert lib.fch_set_links(h, bad.ctypes.data_as(i64), col.ctypes.data_as(i32), ref.L, 0.0) != 0
+    assert lib.fch_set_links(h, bad.ctypes.data_as(i64), col.ctypes.data_as(i32), ref.L, 0, 0.0) != 0
     badc = col.copy()
     badc[0] = ref.M
-    assert lib.fch_set_links(h, row.ctypes.data_as(i64), badc.ctypes.data_as(i32), ref.L, 0.0) != 0
-    assert lib.fch_set_links(h, row.ctypes.data_as(i64), col.ctypes.data_as(i32), ref.L, 0.5) == 0
+    assert lib.fch_set_links(h, row.ctypes.data_as(i64), badc.ctypes.data_as(i32), ref.L, 0, 0.0) != 0
+    assert lib.fch_set_links(h, row.ctypes.data_as(i64), col.ctypes.data_as(i32), ref.L, 99, 0.5) == 0
     full = fc.Mesh(h.value)
     bare._h = None  # the second wrapper owns the handle
     assert full.L == ref.L and full.stats.links == ref.L and full.stats.t_links_s == 0.5
+    assert full.stats.max_level == 99
     np.testing.assert_array_equal(full.col, ref.col)

@@ -49,6 +49,7 @@ def test_device_links_equal_host_links(dev, geo_dir, name):
     np.testing.assert_array_equal(devm.row_ptr, host.row_ptr)
     np.testing.assert_array_equal(devm.col, _sorted_rows(host))
     assert devm.stats.links == host.stats.links
+    assert devm.stats.max_level == host.stats.max_level
 
 
 def test_solve_on_device_links_matches_host_mesh(dev, geo_dir):
@@ -61,3 +62,30 @@ def test_solve_on_device_links_matches_host_mesh(dev, geo_dir):
         res.append(dev.solve(job, m, tol=1e-8).cap)
         dev.release(job)
     np.testing.assert_allclose(res[1], res[0], rtol=1e-12, atol=1e-12 * np.abs(res[0]).max())
+
+
+def test_two_cubes_chain_on_device_links_matches_reference_output(dev, geo_dir):
+    """The reference help's -a0.01 two-cubes run with the links discovered on the
+    device: panel and link counts, recursion depth (the traversal's own levels
+    count) and the printed capacitance digits of every iteration."""
+    import json
+    import math
+
+    from test_golden_cpu import GOLDEN, two_cubes
+
+    gold = json.load(open(os.path.join(GOLDEN, "two_cubes_doc.json")))
+    path = two_cubes(geo_dir)
+    m = 1e32
+    for it in gold["iterations"]:
+        mesh = fc.Mesh.build(path, mesh_eps=m, device=dev)
+        assert mesh.N == it["panels"]
+        assert mesh.L == it["links_computed"]
+        assert mesh.stats.max_level == it["max_level"]
+        job = dev.upload(mesh)
+        dev.fill(job, fc.FCG_PREC_FP64)
+        res = dev.solve(job, mesh, tol=gold["gmres_tol"])
+        dev.release(job)
+        assert list(res.iterations) == [it["gmres_iters"]] * 2
+        for a, b in zip(res.cap.ravel(), np.array(it["cap"]).ravel()):
+            assert float("%.6g" % a) == pytest.approx(b, rel=1e-12), (a, b)
+        m = mesh.stats.max_mesh_eps / math.sqrt(2.0)
