@@ -191,6 +191,7 @@ DEVICE_SYMBOLS = (
     "fcg_set_batch_variant",
     "fcg_links_discover",
     "fcg_links_fetch",
+    "fcg_job_upload_links",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -298,6 +299,7 @@ def device_lib() -> C.CDLL:
         lib.fcg_links_discover.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(LinkParams), i64p,
                                            C.POINTER(C.c_int32)]
         lib.fcg_links_fetch.argtypes = [vp, i64p, C.POINTER(C.c_int32)]
+        lib.fcg_job_upload_links.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(C.c_int)]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p
         for name in DEVICE_SYMBOLS:

@@ -238,7 +238,8 @@ struct Job {
         bool avail = false;
         DevBuf snode, sstart, sidx, Sinv, w;
     } spre;
-    std::vector<uint32_t> link_pos;
+    std::vector<uint32_t> link_pos;  // descriptor-order offset of each stored link (empty: rows by node id)
+    std::vector<int32_t> h_iperm;     // charge row -> node
     bool filled64 = false, filled32 = false;
     std::vector<int32_t> group_is_diel, group_leaf_start;
     std::vector<std::pair<int, int>> rhs;  // leaf range raised to 1 V per right-hand side
@@ -553,9 +554,9 @@ struct ProfEvents {
 };
 
 // Validate the structural invariants the kernels rely on (bounds of every index).
-int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
+int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d, bool check_cols = true) {
     if (!d || !d->nodes || !d->left || !d->right || !d->parent || !d->leaf_lo || !d->leaf_hi || !d->leaf_node ||
-        !d->row_ptr || (d->n_links > 0 && !d->col) || !d->group_is_diel || !d->group_leaf_start ||
+        !d->row_ptr || (check_cols && d->n_links > 0 && !d->col) || !d->group_is_diel || !d->group_leaf_start ||
         !d->group_outperm || !d->group_inperm || !d->leaf_outperm)
         return fail(ctx, FCG_INVALID_ARGUMENT, "null array in job descriptor");
     if (d->dim != 2 && d->dim != 3) return fail(ctx, FCG_INVALID_ARGUMENT, "dim must be 2 or 3");
@@ -588,8 +589,9 @@ int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d) {
         if (nd < 0 || nd >= M || d->left[nd] >= 0 || d->leaf_lo[nd] != k)
             return fail(ctx, FCG_INVALID_ARGUMENT, "bad leaf_node map");
     }
-    for (int64_t l = 0; l < d->n_links; ++l)
-        if (d->col[l] < 0 || d->col[l] >= M) return fail(ctx, FCG_INVALID_ARGUMENT, "link source out of range");
+    if (check_cols)
+        for (int64_t l = 0; l < d->n_links; ++l)
+            if (d->col[l] < 0 || d->col[l] >= M) return fail(ctx, FCG_INVALID_ARGUMENT, "link source out of range");
     if (d->group_leaf_start[0] != 0 || d->group_leaf_start[d->n_groups] != N)
         return fail(ctx, FCG_INVALID_ARGUMENT, "group leaf ranges do not span the leaves");
     for (int g = 0; g < d->n_groups; ++g)
@@ -609,8 +611,8 @@ constexpr int64_t kLinkSpare = 4;
 // source row.  Returns perm (node -> row), iperm, the gather's row order, the
 // sorted source rows and each stored link's offset within its row in the
 // descriptor's order.
-void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector<int32_t>& iperm,
-                  std::vector<int32_t>& rorder, std::vector<int32_t>& col_sorted, std::vector<uint32_t>& link_pos) {
+void build_tree_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector<int32_t>& iperm,
+                       std::vector<int32_t>& rorder) {
     const int M = d->n_nodes;
     std::vector<int32_t> depth((size_t)M, 0);
     int maxd = 0;
@@ -645,6 +647,12 @@ void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector
             rorder[(size_t)cs[(size_t)(node / C)]++] = node;
         }
     }
+}
+
+void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector<int32_t>& iperm,
+                  std::vector<int32_t>& rorder, std::vector<int32_t>& col_sorted, std::vector<uint32_t>& link_pos) {
+    const int M = d->n_nodes;
+    build_tree_layout(d, perm, iperm, rorder);
     const int64_t L = d->n_links;
     col_sorted.assign((size_t)L, 0);
     link_pos.assign((size_t)L, 0);
@@ -1732,29 +1740,113 @@ int fcg_ctx_destroy(fcg_ctx* ctx) {
 
 const char* fcg_last_error(fcg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+}  // extern "C"
+
+namespace {
+// tree arrays of an upload (both layouts)
+int upload_tree(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
+    j.dim = d->dim;
+    j.N = d->n_leaves;
+    j.M = d->n_nodes;
+    j.G = d->n_groups;
+    j.L = d->n_links;
+    hipStream_t s = ctx->stream;
+    HIPCHECK(ctx, upload(j.nodes, d->nodes, (size_t)j.M, s));
+    HIPCHECK(ctx, upload(j.left, d->left, (size_t)j.M, s));
+    HIPCHECK(ctx, upload(j.right, d->right, (size_t)j.M, s));
+    HIPCHECK(ctx, upload(j.parent, d->parent, (size_t)j.M, s));
+    HIPCHECK(ctx, upload(j.leaf_lo, d->leaf_lo, (size_t)j.M, s));
+    HIPCHECK(ctx, upload(j.leaf_hi, d->leaf_hi, (size_t)j.M, s));
+    j.h_left.assign(d->left, d->left + j.M);
+    j.h_right.assign(d->right, d->right + j.M);
+    j.h_parent.assign(d->parent, d->parent + j.M);
+    j.h_leaf_lo.assign(d->leaf_lo, d->leaf_lo + j.M);
+    j.h_leaf_hi.assign(d->leaf_hi, d->leaf_hi + j.M);
+    HIPCHECK(ctx, upload(j.leaf_node, d->leaf_node, (size_t)j.N, s));
+    return FCG_OK;
+}
+
+// groups, right-hand sides, self and diagonal storage and the link rows (both
+// layouts); returns synchronised
+int upload_finish(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
+    hipStream_t s = ctx->stream;
+    HIPCHECK(ctx, upload(j.gout, d->group_outperm, (size_t)j.G, s));
+    HIPCHECK(ctx, upload(j.gin, d->group_inperm, (size_t)j.G, s));
+    HIPCHECK(ctx, upload(j.leaf_w, d->leaf_outperm, (size_t)j.N, s));
+    HIPCHECK(ctx, j.link_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(j.L, 1)));
+    HIPCHECK(ctx, j.self.alloc(sizeof(double) * (size_t)j.N));
+    HIPCHECK(ctx, j.diag.alloc(sizeof(double) * (size_t)j.M));
+    HIPCHECK(ctx, hipMemsetAsync(j.diag.p, 0, sizeof(double) * (size_t)j.M, s));
+    j.group_is_diel.assign(d->group_is_diel, d->group_is_diel + j.G);
+    j.group_leaf_start.assign(d->group_leaf_start, d->group_leaf_start + j.G + 1);
+    j.first_cond_leaf = j.N;
+    std::vector<int32_t> ranges;
+    for (int g = 0; g < j.G; ++g) {
+        if (j.group_is_diel[(size_t)g]) continue;
+        const int lo = j.group_leaf_start[(size_t)g], hi = j.group_leaf_start[(size_t)g + 1];
+        if (j.first_cond_leaf == j.N) j.first_cond_leaf = lo;
+        ranges.push_back(lo);
+        ranges.push_back(hi);
+        ++j.ncol;
+        // 2D: the conductor that ends the leaf list is the balance row (SolveCapacitance.cpp:1233-1240)
+        if (!(j.dim == 2 && hi == j.N)) j.rhs.emplace_back(lo, hi);
+    }
+    if (ranges.empty()) return fail(ctx, FCG_INVALID_ARGUMENT, "job has no conductor");
+    {
+        const int cls = fcg::gather_class(fcg::padded_k(std::max(1, (int)j.rhs.size())));
+        for (int p = 0; p < 3; ++p) j.gather_variant[p] = fcg::default_gather_variant(p, cls);
+    }
+    HIPCHECK(ctx, upload(j.col_range, ranges.data(), ranges.size(), s));
+    {
+        std::vector<int32_t> rr;
+        for (const auto& rg : j.rhs) {
+            rr.push_back(rg.first);
+            rr.push_back(rg.second);
+        }
+        if (rr.empty()) rr.assign(2, 0);
+        HIPCHECK(ctx, upload(j.rhs_range, rr.data(), rr.size(), s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));  // rr and ranges are released at the end of this block
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    // destination node of every link (used by the fill)
+    ctx->tabs.reset();
+    BatchView bv;
+    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, nullptr, nullptr}}, &bv));
+    HIPCHECK(ctx, fcg::launch_link_rows(bv, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    return FCG_OK;
+}
+
+// sorts every row of col (device) ascending: rows up to the workgroup sort's
+// capacity on the device, longer ones here
+int sort_rows(fcg_ctx* ctx, const DevBuf& drow, const std::vector<int64_t>& row_ptr, int32_t* col, int M) {
+    hipStream_t s = ctx->stream;
+    HIPCHECK(ctx, fcg::launch_links_sort_rows(drow.as<int64_t>(), col, M, s));
+    const int cap = fcg::links_sort_lds_max();
+    std::vector<int32_t> tmp;
+    for (int i = 0; i < M; ++i) {
+        const int64_t b = row_ptr[(size_t)i], n = row_ptr[(size_t)i + 1] - b;
+        if (n <= cap) continue;
+        tmp.resize((size_t)n);
+        HIPCHECK(ctx, hipMemcpyAsync(tmp.data(), col + b, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        std::sort(tmp.begin(), tmp.end());
+        HIPCHECK(ctx, hipMemcpyAsync(col + b, tmp.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+    }
+    return FCG_OK;
+}
+}  // namespace
+
+extern "C" {
+
 int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     if (!ctx || !job) return FCG_INVALID_ARGUMENT;
     RCCHECK(validate_desc(ctx, d));
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     auto j = std::make_unique<Job>();
-    j->dim = d->dim;
-    j->N = d->n_leaves;
-    j->M = d->n_nodes;
-    j->G = d->n_groups;
-    j->L = d->n_links;
     hipStream_t s = ctx->stream;
-    HIPCHECK(ctx, upload(j->nodes, d->nodes, (size_t)j->M, s));
-    HIPCHECK(ctx, upload(j->left, d->left, (size_t)j->M, s));
-    HIPCHECK(ctx, upload(j->right, d->right, (size_t)j->M, s));
-    HIPCHECK(ctx, upload(j->parent, d->parent, (size_t)j->M, s));
-    HIPCHECK(ctx, upload(j->leaf_lo, d->leaf_lo, (size_t)j->M, s));
-    HIPCHECK(ctx, upload(j->leaf_hi, d->leaf_hi, (size_t)j->M, s));
-    j->h_left.assign(d->left, d->left + j->M);
-    j->h_right.assign(d->right, d->right + j->M);
-    j->h_parent.assign(d->parent, d->parent + j->M);
-    j->h_leaf_lo.assign(d->leaf_lo, d->leaf_lo + j->M);
-    j->h_leaf_hi.assign(d->leaf_hi, d->leaf_hi + j->M);
-    HIPCHECK(ctx, upload(j->leaf_node, d->leaf_node, (size_t)j->N, s));
+    RCCHECK(upload_tree(ctx, d, *j));
     HIPCHECK(ctx, upload(j->row_ptr, d->row_ptr, (size_t)j->M + 1, s));
     {
         std::vector<int32_t> perm, iperm, rorder, cols;
@@ -1777,51 +1869,83 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
             HIPCHECK(ctx, hipStreamSynchronize(s));  // c16 is released here
         }
         j->h_row_ptr.assign(d->row_ptr, d->row_ptr + j->M + 1);
+        j->h_iperm = iperm;
         HIPCHECK(ctx, hipStreamSynchronize(s));  // the host vectors above are released here
     }
-    HIPCHECK(ctx, upload(j->gout, d->group_outperm, (size_t)j->G, s));
-    HIPCHECK(ctx, upload(j->gin, d->group_inperm, (size_t)j->G, s));
-    HIPCHECK(ctx, upload(j->leaf_w, d->leaf_outperm, (size_t)j->N, s));
-    HIPCHECK(ctx, j->link_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(j->L, 1)));
-    HIPCHECK(ctx, j->self.alloc(sizeof(double) * (size_t)j->N));
-    HIPCHECK(ctx, j->diag.alloc(sizeof(double) * (size_t)j->M));
-    HIPCHECK(ctx, hipMemsetAsync(j->diag.p, 0, sizeof(double) * (size_t)j->M, s));
-    j->group_is_diel.assign(d->group_is_diel, d->group_is_diel + j->G);
-    j->group_leaf_start.assign(d->group_leaf_start, d->group_leaf_start + j->G + 1);
-    j->first_cond_leaf = j->N;
-    std::vector<int32_t> ranges;
-    for (int g = 0; g < j->G; ++g) {
-        if (j->group_is_diel[(size_t)g]) continue;
-        const int lo = j->group_leaf_start[(size_t)g], hi = j->group_leaf_start[(size_t)g + 1];
-        if (j->first_cond_leaf == j->N) j->first_cond_leaf = lo;
-        ranges.push_back(lo);
-        ranges.push_back(hi);
-        ++j->ncol;
-        // 2D: the conductor that ends the leaf list is the balance row (SolveCapacitance.cpp:1233-1240)
-        if (!(j->dim == 2 && hi == j->N)) j->rhs.emplace_back(lo, hi);
-    }
-    if (ranges.empty()) return fail(ctx, FCG_INVALID_ARGUMENT, "job has no conductor");
-    {
-        const int cls = fcg::gather_class(fcg::padded_k(std::max(1, (int)j->rhs.size())));
-        for (int p = 0; p < 3; ++p) j->gather_variant[p] = fcg::default_gather_variant(p, cls);
-    }
-    HIPCHECK(ctx, upload(j->col_range, ranges.data(), ranges.size(), s));
-    {
-        std::vector<int32_t> rr;
-        for (const auto& rg : j->rhs) {
-            rr.push_back(rg.first);
-            rr.push_back(rg.second);
-        }
-        if (rr.empty()) rr.assign(2, 0);
-        HIPCHECK(ctx, upload(j->rhs_range, rr.data(), rr.size(), s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));  // rr is released at the end of this block
+    RCCHECK(upload_finish(ctx, d, *j));
+    ctx->jobs.push_back(std::move(j));
+    *job = (int)ctx->jobs.size() - 1;
+    return FCG_OK;
+}
+
+int fcg_job_upload_links(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
+    if (!ctx || !job || !d) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (!ctx->links.ready || ctx->links.row_ptr.size() != (size_t)d->n_nodes + 1)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "no discovered links for a tree of this size");
+    fcg_job_desc dd = *d;
+    dd.row_ptr = ctx->links.row_ptr.data();
+    dd.n_links = ctx->links.L;
+    dd.col = nullptr;
+    RCCHECK(validate_desc(ctx, &dd, false));
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    auto j = std::make_unique<Job>();
+    hipStream_t s = ctx->stream;
+    RCCHECK(upload_tree(ctx, &dd, *j));
+    const int M = j->M;
+    const int64_t L = j->L;
+    HIPCHECK(ctx, upload(j->row_ptr, dd.row_ptr, (size_t)M + 1, s));
+    std::vector<int32_t> perm, iperm, rorder;
+    build_tree_layout(&dd, perm, iperm, rorder);
+    HIPCHECK(ctx, upload(j->perm, perm.data(), perm.size(), s));
+    HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
+    HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
+    // stored columns: the source's charge row, every row ascending
+    HIPCHECK(ctx, j->col.alloc(sizeof(int32_t) * (size_t)(L + kLinkSpare)));
+    HIPCHECK(ctx, hipMemsetAsync(j->col.as<int32_t>() + L, 0, sizeof(int32_t) * kLinkSpare, s));
+    HIPCHECK(ctx, fcg::launch_perm_cols(ctx->links.col.as<int32_t>(), L, j->perm.as<int32_t>(), j->col.as<int32_t>(), s));
+    RCCHECK(sort_rows(ctx, j->row_ptr, ctx->links.row_ptr, j->col.as<int32_t>(), M));
+    if (M <= 65536) {
+        HIPCHECK(ctx, j->col16.alloc(sizeof(uint16_t) * (size_t)(L + kLinkSpare)));
+        HIPCHECK(ctx, fcg::launch_col16(j->col.as<int32_t>(), L + kLinkSpare, j->col16.as<uint16_t>(), s));
     }
     HIPCHECK(ctx, hipStreamSynchronize(s));
-    // destination node of every link (used by the fill)
-    ctx->tabs.reset();
-    BatchView bv;
-    RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{j.get(), &j->scr, nullptr, nullptr}}, &bv));
-    HIPCHECK(ctx, fcg::launch_link_rows(bv, s));
+    j->h_row_ptr = std::move(ctx->links.row_ptr);
+    j->h_iperm = std::move(iperm);
+    ctx->links.reset();  // the discovered rows are consumed
+    RCCHECK(upload_finish(ctx, &dd, *j));
+    // fill order: links grouped by fill class, link order kept inside a class
+    j->fill_cnt.assign(kFillClasses, 0);
+    std::vector<int64_t> fstart((size_t)kFillClasses + 1, 0);
+    if (L > 0) {
+        DevBuf cls, hist, off;
+        HIPCHECK(ctx, cls.alloc((size_t)L));
+        HIPCHECK(ctx, fcg::launch_link_fill_class(j->link_row.as<int32_t>(), j->col.as<int32_t>(),
+                                                  j->iperm.as<int32_t>(), j->nodes.as<fcg_node_geom>(), j->dim, L,
+                                                  cls.as<uint8_t>(), s));
+        const int64_t nb = (L + fcg::class_chunk() - 1) / fcg::class_chunk();
+        HIPCHECK(ctx, hist.alloc(sizeof(int32_t) * (size_t)(nb * kFillClasses)));
+        HIPCHECK(ctx, fcg::launch_class_hist(cls.as<uint8_t>(), L, hist.as<int32_t>(), s));
+        std::vector<int32_t> h((size_t)(nb * kFillClasses));
+        HIPCHECK(ctx, hipMemcpyAsync(h.data(), hist.p, sizeof(int32_t) * h.size(), hipMemcpyDeviceToHost, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        std::vector<int64_t> o((size_t)(nb * kFillClasses));
+        int64_t run = 0;
+        for (int c = 0; c < kFillClasses; ++c) {
+            fstart[(size_t)c] = run;
+            for (int64_t b = 0; b < nb; ++b) {
+                o[(size_t)(b * kFillClasses + c)] = run;
+                run += h[(size_t)(b * kFillClasses + c)];
+            }
+            j->fill_cnt[(size_t)c] = run - fstart[(size_t)c];
+        }
+        if (run != L) return fail(ctx, FCG_DEVICE_ERROR, "fill order: class counts do not add up");
+        HIPCHECK(ctx, upload(off, o.data(), o.size(), s));
+        HIPCHECK(ctx, j->fill_order.alloc(sizeof(uint32_t) * (size_t)L));
+        HIPCHECK(ctx, fcg::launch_class_scatter(cls.as<uint8_t>(), L, off.as<int64_t>(), j->fill_order.as<uint32_t>(), s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));  // cls, hist, off and the host vectors are released here
+    }
+    fstart[(size_t)kFillClasses] = L;
+    HIPCHECK(ctx, upload(j->fill_start, fstart.data(), fstart.size(), s));
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->jobs.push_back(std::move(j));
     *job = (int)ctx->jobs.size() - 1;
@@ -1923,9 +2047,23 @@ int fcg_download_coeffs(fcg_ctx* ctx, int job, double* coef, double* self_coef) 
             HIPCHECK(ctx, hipMemcpy(t32.data(), j->coef32.p, sizeof(float) * (size_t)j->L, hipMemcpyDeviceToHost));
             for (int64_t l = 0; l < j->L; ++l) tmp[(size_t)l] = t32[(size_t)l];
         }
-        for (int r = 0; r < j->M; ++r)
-            for (int64_t l = j->h_row_ptr[(size_t)r]; l < j->h_row_ptr[(size_t)r + 1]; ++l)
-                coef[j->h_row_ptr[(size_t)r] + j->link_pos[(size_t)l]] = tmp[(size_t)l];
+        if (j->link_pos.empty()) {
+            // rows discovered on the device: descriptor order is by source node id
+            std::vector<int32_t> cols((size_t)j->L);
+            HIPCHECK(ctx, hipMemcpy(cols.data(), j->col.p, sizeof(int32_t) * (size_t)j->L, hipMemcpyDeviceToHost));
+            std::vector<std::pair<int32_t, int64_t>> row;
+            for (int r = 0; r < j->M; ++r) {
+                const int64_t b = j->h_row_ptr[(size_t)r], e = j->h_row_ptr[(size_t)r + 1];
+                row.clear();
+                for (int64_t l = b; l < e; ++l) row.emplace_back(j->h_iperm[(size_t)cols[(size_t)l]], l);
+                std::sort(row.begin(), row.end());
+                for (int64_t q = b; q < e; ++q) coef[q] = tmp[(size_t)row[(size_t)(q - b)].second];
+            }
+        } else {
+            for (int r = 0; r < j->M; ++r)
+                for (int64_t l = j->h_row_ptr[(size_t)r]; l < j->h_row_ptr[(size_t)r + 1]; ++l)
+                    coef[j->h_row_ptr[(size_t)r] + j->link_pos[(size_t)l]] = tmp[(size_t)l];
+        }
     }
     if (self_coef) HIPCHECK(ctx, hipMemcpy(self_coef, j->self.p, sizeof(double) * (size_t)j->N, hipMemcpyDeviceToHost));
     return FCG_OK;
@@ -2518,22 +2656,7 @@ int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_param
     HIPCHECK(ctx, col.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(L, 1)));
     HIPCHECK(ctx, fcg::launch_links_scatter(links.as<int2>(), Pn, cursor.as<unsigned long long>(),
                                             col.as<int32_t>(), s));
-    HIPCHECK(ctx, fcg::launch_links_sort_rows(drow.as<int64_t>(), col.as<int32_t>(), M, s));
-    // rows beyond the workgroup sort's capacity are sorted here
-    const int cap = fcg::links_sort_lds_max();
-    std::vector<int32_t> tmp;
-    for (int i = 0; i < M; ++i) {
-        const int64_t b = row_ptr[(size_t)i], n = row_ptr[(size_t)i + 1] - b;
-        if (n <= cap) continue;
-        tmp.resize((size_t)n);
-        HIPCHECK(ctx, hipMemcpyAsync(tmp.data(), col.as<int32_t>() + b, sizeof(int32_t) * (size_t)n,
-                                     hipMemcpyDeviceToHost, s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));
-        std::sort(tmp.begin(), tmp.end());
-        HIPCHECK(ctx, hipMemcpyAsync(col.as<int32_t>() + b, tmp.data(), sizeof(int32_t) * (size_t)n,
-                                     hipMemcpyHostToDevice, s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));
-    }
+    RCCHECK(sort_rows(ctx, drow, row_ptr, col.as<int32_t>(), M));
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->links.row_ptr = std::move(row_ptr);
     ctx->links.col.swap(col);

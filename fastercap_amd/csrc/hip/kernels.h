@@ -283,5 +283,15 @@ hipError_t launch_links_scatter(const int2* links, int64_t P, unsigned long long
 // to the caller
 hipError_t launch_links_sort_rows(const int64_t* row_ptr, int32_t* col, int M, hipStream_t s);
 int links_sort_lds_max();
+// device layout of discovered links: stored column = charge row of the source;
+// fill class per link; stable partition by class (kClassChunk links per
+// histogram block, offsets class-major then block); 16-bit columns
+hipError_t launch_perm_cols(const int32_t* col_node, int64_t L, const int32_t* perm, int32_t* out, hipStream_t s);
+hipError_t launch_link_fill_class(const int32_t* link_row, const int32_t* col, const int32_t* iperm,
+                                  const fcg_node_geom* nodes, int dim, int64_t L, uint8_t* cls, hipStream_t s);
+int class_chunk();
+hipError_t launch_class_hist(const uint8_t* cls, int64_t L, int32_t* hist, hipStream_t s);
+hipError_t launch_class_scatter(const uint8_t* cls, int64_t L, const int64_t* off, uint32_t* order, hipStream_t s);
+hipError_t launch_col16(const int32_t* col, int64_t n, uint16_t* c16, hipStream_t s);
 
 }  // namespace fcg

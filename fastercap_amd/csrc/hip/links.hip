@@ -220,9 +220,127 @@ __global__ void __launch_bounds__(256) k_links_sort_rows(const int64_t* __restri
     for (int i = threadIdx.x; i < n; i += 256) col[b + i] = s[i];
 }
 
+// ---------------------------------------------------------------- device layout
+// of a job whose links were discovered here (the upload's host-side layout
+// build, done on the device)
+
+__global__ void k_perm_cols(const int32_t* __restrict__ col_node, int64_t L, const int32_t* __restrict__ perm,
+                            int32_t* __restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < L) out[i] = perm[col_node[i]];
+}
+
+// the upload's fill class of a link (destination a, source b): the branch of
+// PotEstimateOpt from the centroid distance over the larger max side in FP64,
+// then the dielectric row, then the dimension (same expression order as the
+// host's, no contraction)
+__device__ int fill_class_dev(const fcg_node_geom& a, const fcg_node_geom& b, int dim) {
+    const double dx = (double)a.c[0] - (double)b.c[0], dy = (double)a.c[1] - (double)b.c[1];
+    const double rmax = a.maxside > b.maxside ? a.maxside : b.maxside;
+    const int diel = (a.flags & FCG_NODE_DIEL) ? 1 : 0;
+    if (dim == 2) return 10 + 2 * diel + (sqrt(dx * dx + dy * dy) / rmax < 4.0 ? 0 : 1);
+    const double dz = (double)a.c[2] - (double)b.c[2];
+    const double ratio = sqrt(dx * dx + dy * dy + dz * dz) / rmax;
+    const int br = ratio < 1.0 ? 0 : ratio < 2.0 ? 1 : ratio < 4.0 ? 2 : ratio < 10.0 ? 3 : 4;
+    return 5 * diel + br;
+}
+
+__global__ void k_link_fill_class(const int32_t* __restrict__ link_row, const int32_t* __restrict__ col,
+                                  const int32_t* __restrict__ iperm, const fcg_node_geom* __restrict__ nodes,
+                                  int dim, int64_t L, uint8_t* __restrict__ cls) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < L) cls[i] = (uint8_t)fill_class_dev(nodes[link_row[i]], nodes[iperm[col[i]]], dim);
+}
+
+// Stable partition of the links by class: per chunk of kClassChunk links a
+// class histogram, then (with the host's exclusive offsets, class-major and
+// chunk-minor) a scatter that keeps link order inside every class.
+constexpr int kClassChunk = 4096;
+__global__ void __launch_bounds__(256) k_class_hist(const uint8_t* __restrict__ cls, int64_t L,
+                                                    int32_t* __restrict__ hist) {
+    __shared__ int32_t h[kFillClasses];
+    if (threadIdx.x < kFillClasses) h[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * kClassChunk;
+    for (int k = threadIdx.x; k < kClassChunk; k += 256) {
+        const int64_t l = b0 + k;
+        if (l < L) atomicAdd(&h[cls[l]], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x < kFillClasses) hist[(int64_t)blockIdx.x * kFillClasses + threadIdx.x] = h[threadIdx.x];
+}
+
+__global__ void __launch_bounds__(256) k_class_scatter(const uint8_t* __restrict__ cls, int64_t L,
+                                                       const int64_t* __restrict__ off,
+                                                       uint32_t* __restrict__ order) {
+    __shared__ int64_t run[kFillClasses];
+    __shared__ int32_t wc[4][kFillClasses];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (t < kFillClasses) run[t] = off[(int64_t)blockIdx.x * kFillClasses + t];
+    __syncthreads();
+    const int64_t b0 = (int64_t)blockIdx.x * kClassChunk;
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int tile = 0; tile < kClassChunk; tile += 256) {
+        const int64_t l = b0 + tile + t;
+        const int c = l < L ? (int)cls[l] : -1;
+        int rank = 0;
+        for (int cc = 0; cc < kFillClasses; ++cc) {
+            const uint64_t m = __ballot(c == cc);
+            if (c == cc) rank = __popcll(m & below);
+            if (lane == 0) wc[w][cc] = __popcll(m);
+        }
+        __syncthreads();
+        if (c >= 0) {
+            int64_t base = run[c];
+            for (int v = 0; v < w; ++v) base += wc[v][c];
+            order[base + rank] = (uint32_t)l;
+        }
+        __syncthreads();
+        if (t < kFillClasses) run[t] += wc[0][t] + wc[1][t] + wc[2][t] + wc[3][t];
+        __syncthreads();
+    }
+}
+
+__global__ void k_col16(const int32_t* __restrict__ col, int64_t n, uint16_t* __restrict__ c16) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) c16[i] = (uint16_t)col[i];
+}
+
 }  // namespace
 
 int links_sort_lds_max() { return kSortLds; }
+int class_chunk() { return kClassChunk; }
+
+hipError_t launch_perm_cols(const int32_t* col_node, int64_t L, const int32_t* perm, int32_t* out, hipStream_t s) {
+    if (L <= 0) return hipSuccess;
+    k_perm_cols<<<(unsigned)((L + 255) / 256), 256, 0, s>>>(col_node, L, perm, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_link_fill_class(const int32_t* link_row, const int32_t* col, const int32_t* iperm,
+                                  const fcg_node_geom* nodes, int dim, int64_t L, uint8_t* cls, hipStream_t s) {
+    if (L <= 0) return hipSuccess;
+    k_link_fill_class<<<(unsigned)((L + 255) / 256), 256, 0, s>>>(link_row, col, iperm, nodes, dim, L, cls);
+    return hipGetLastError();
+}
+
+hipError_t launch_class_hist(const uint8_t* cls, int64_t L, int32_t* hist, hipStream_t s) {
+    if (L <= 0) return hipSuccess;
+    k_class_hist<<<(unsigned)((L + kClassChunk - 1) / kClassChunk), 256, 0, s>>>(cls, L, hist);
+    return hipGetLastError();
+}
+
+hipError_t launch_class_scatter(const uint8_t* cls, int64_t L, const int64_t* off, uint32_t* order, hipStream_t s) {
+    if (L <= 0) return hipSuccess;
+    k_class_scatter<<<(unsigned)((L + kClassChunk - 1) / kClassChunk), 256, 0, s>>>(cls, L, off, order);
+    return hipGetLastError();
+}
+
+hipError_t launch_col16(const int32_t* col, int64_t n, uint16_t* c16, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_col16<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(col, n, c16);
+    return hipGetLastError();
+}
 
 hipError_t launch_links_step(const uint64_t* in, const uint8_t* in_lev, int64_t n, uint64_t* out, uint8_t* out_lev,
                              unsigned long long* out_cnt, int2* links, unsigned long long* link_cnt, int* max_lev,

@@ -13,7 +13,6 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <memory>
 #include <string>
 #include <vector>
 
@@ -197,24 +196,21 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
     fcg_job_desc d;
     fch_get_desc(mesh, &d);
     if (!o.host_links) {
-        // the link structure from the device (fcg_links_discover), installed in the mesh
+        // the link structure from the device (fcg_links_discover); it stays there
+        // for fcg_job_upload_links, the mesh records only its statistics
         const auto t0 = std::chrono::steady_clock::now();
         fcg_link_params lp;
         fch_get_link_params(mesh, &lp);
         int64_t L = 0;
         int32_t maxlev = 0;
         rc = fcg_links_discover(ctx, &d, &lp, &L, &maxlev);
-        std::vector<int64_t> row_ptr((size_t)d.n_nodes + 1);
-        std::unique_ptr<int32_t[]> col(new int32_t[(size_t)std::max<int64_t>(L, 1)]);  // filled by the fetch
-        if (rc == FCG_OK) rc = fcg_links_fetch(ctx, row_ptr.data(), col.get());
         const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        if (rc == FCG_OK) rc = fch_set_links(mesh, row_ptr.data(), col.get(), L, maxlev, t);
+        if (rc == FCG_OK) rc = fch_set_links(mesh, nullptr, nullptr, L, maxlev, t);
         if (rc != FCG_OK) {
             std::fprintf(stderr, "Error: link discovery: %s\n", fcg_last_error(ctx));
             fch_free(mesh);
             return rc;
         }
-        fch_get_desc(mesh, &d);
     }
     fch_get_stats(mesh, &r.hs);
     r.dim = d.dim;
@@ -236,7 +232,7 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
         }
     }
     int job = -1;
-    rc = fcg_job_upload(ctx, &d, &job);
+    rc = o.host_links ? fcg_job_upload(ctx, &d, &job) : fcg_job_upload_links(ctx, &d, &job);
     if (rc == FCG_OK) rc = fcg_fill(ctx, job, o.precision);
     if (rc == FCG_OK) {
         fcg_solve_opts so{tol, 1000, precond, o.precision, 8, size};

@@ -1375,7 +1375,14 @@ int fch_get_link_params(const fch_mesh* m, fcg_link_params* p) {
 
 int fch_set_links(fch_mesh* m, const int64_t* row_ptr, const int32_t* col, int64_t L, int32_t max_level,
                   double t_links_s) {
-    if (!m || !row_ptr || L < 0 || (L > 0 && !col)) return FCG_INVALID_ARGUMENT;
+    if (!m || L < 0) return FCG_INVALID_ARGUMENT;
+    if (!row_ptr && !col) {
+        m->stats.links = L;
+        m->stats.max_level = std::max(m->stats.max_level, max_level);
+        m->stats.t_links_s = t_links_s;
+        return FCG_OK;
+    }
+    if (!row_ptr || (L > 0 && !col)) return FCG_INVALID_ARGUMENT;
     const size_t M = m->nodes.size();
     if (row_ptr[0] != 0 || row_ptr[M] != L) return FCG_INVALID_ARGUMENT;
     for (size_t i = 0; i < M; ++i)

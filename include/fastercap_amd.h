@@ -220,6 +220,15 @@ int fcg_links_discover(fcg_ctx* ctx, const fcg_job_desc* d, const fcg_link_param
  * device. */
 int fcg_links_fetch(fcg_ctx* ctx, int64_t* row_ptr, int32_t* col);
 
+/* Create a job from a refined tree (d, CSR fields ignored) and the rows of the
+ * last fcg_links_discover on it, which it consumes: the rows never leave the
+ * device.  The stored layout (rows sorted by source charge row, fill classes
+ * and fill order, 16-bit columns) is built on the device and equals the one
+ * fcg_job_upload builds from the same links.  fcg_download_coeffs returns each
+ * row ordered by source node id.  Replaces, with fcg_job_upload, the
+ * AutoRefineLinks storage set-up (Solver/Autorefine.cpp:481-706). */
+int fcg_job_upload_links(fcg_ctx* ctx, const fcg_job_desc* d, int* job);
+
 /* Solve every conductor right-hand side of the job with right-preconditioned GMRES
  * and assemble the Maxwell capacitance matrix cap[rows][n_cond] (row-major; rows =
  * n_cond in 3D, n_cond-1 in 2D; 2D values unscaled, as the reference keeps them

@@ -57,7 +57,8 @@ int fch_get_link_params(const fch_mesh* mesh, fcg_link_params* params);
 /* Install a link structure discovered elsewhere: row_ptr[nodes + 1] (row_ptr[0]
  * = 0, non-decreasing, row_ptr[nodes] = n_links) and col[n_links] (node ids).
  * max_level (the traversal's deepest recursion level) and t_links_s go into the
- * statistics. */
+ * statistics.  With row_ptr and col both NULL only the statistics are recorded
+ * (the rows stay on the device, fcg_job_upload_links). */
 int fch_set_links(fch_mesh* mesh, const int64_t* row_ptr, const int32_t* col, int64_t n_links, int32_t max_level,
                   double t_links_s);
 

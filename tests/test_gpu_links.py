@@ -89,3 +89,65 @@ def test_two_cubes_chain_on_device_links_matches_reference_output(dev, geo_dir):
         for a, b in zip(res.cap.ravel(), np.array(it["cap"]).ravel()):
             assert float("%.6g" % a) == pytest.approx(b, rel=1e-12), (a, b)
         m = mesh.stats.max_mesh_eps / math.sqrt(2.0)
+
+
+def _upload_on_device_links(dev, path, eps):
+    """Tree from the host builder, links discovered on the device and kept
+    there, job created from them (fcg_job_upload_links)."""
+    import ctypes as C
+
+    from fastercap_amd import _ffi
+
+    lib = _ffi.host_lib()
+    o = _ffi.HostOptions()
+    lib.fch_default_options(C.byref(o))
+    o.mesh_eps, o.links_on_device = eps, 1
+    h = C.c_void_p()
+    assert lib.fch_build(path.encode(), C.byref(o), C.byref(h)) == 0
+    tree = fc.Mesh(h.value)
+    p = _ffi.LinkParams()
+    assert lib.fch_get_link_params(h, C.byref(p)) == 0
+    n, lev = C.c_int64(), C.c_int32()
+    dev._check(dev.lib.fcg_links_discover(dev._h, C.byref(tree.desc), C.byref(p), C.byref(n), C.byref(lev)),
+               "fcg_links_discover")
+    job = C.c_int()
+    dev._check(dev.lib.fcg_job_upload_links(dev._h, C.byref(tree.desc), C.byref(job)), "fcg_job_upload_links")
+    return tree, job.value, n.value
+
+
+@pytest.mark.parametrize("name", ["plates", "coax2d", "sphere_shell", "bus"])
+def test_device_layout_equals_host_layout(dev, geo_dir, name):
+    import ctypes as C
+
+    path, eps = _cases(geo_dir)[name]
+    host = fc.Mesh.build(path, mesh_eps=eps)
+    ja = dev.upload(host)
+    tree, jb, L = _upload_on_device_links(dev, path, eps)
+    assert L == host.L
+    rng = np.random.default_rng(17)
+    q = rng.standard_normal((3, host.N))
+    for prec in (fc.FCG_PREC_FP64, fc.FCG_PREC_MIXED):
+        dev.fill(ja, prec)
+        dev.fill(jb, prec)
+        mp = fc.FCG_MVP_FP64 if prec == fc.FCG_PREC_FP64 else fc.FCG_MVP_FP32
+        va, vb = dev.mvp_prec(ja, q, mp), dev.mvp_prec(jb, q, mp)
+        # the same stored layout and fill order: the same bits
+        np.testing.assert_array_equal(va, vb)
+    dev.fill(ja, fc.FCG_PREC_FP64)
+    dev.fill(jb, fc.FCG_PREC_FP64)
+    ca, sa = dev.coeffs(ja, host)
+    cb = np.empty(L)
+    sb = np.empty(host.N)
+    dp = C.POINTER(C.c_double)
+    dev._check(dev.lib.fcg_download_coeffs(dev._h, jb, cb.ctypes.data_as(dp), sb.ctypes.data_as(dp)),
+               "fcg_download_coeffs")
+    # device-layout jobs return each row by source node id
+    rows = np.repeat(np.arange(host.M, dtype=np.int64), np.diff(host.row_ptr))
+    order = np.lexsort((host.col, rows))
+    np.testing.assert_array_equal(cb, ca[order])
+    np.testing.assert_array_equal(sb, sa)
+    ra = dev.solve(ja, host, tol=1e-8)
+    rb = dev.solve(jb, host, tol=1e-8)
+    np.testing.assert_array_equal(ra.cap, rb.cap)
+    dev.release(ja)
+    dev.release(jb)
