@@ -1818,22 +1818,30 @@ int upload_finish(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
 }
 
 // sorts every row of col (device) ascending: rows up to the workgroup sort's
-// capacity on the device, longer ones here
+// LDS capacity in LDS, the longer ones in a scratch copy (one launch each)
 int sort_rows(fcg_ctx* ctx, const DevBuf& drow, const std::vector<int64_t>& row_ptr, int32_t* col, int M) {
     hipStream_t s = ctx->stream;
     HIPCHECK(ctx, fcg::launch_links_sort_rows(drow.as<int64_t>(), col, M, s));
     const int cap = fcg::links_sort_lds_max();
-    std::vector<int32_t> tmp;
+    std::vector<int32_t> rows;
+    std::vector<int64_t> off(1, 0);
     for (int i = 0; i < M; ++i) {
-        const int64_t b = row_ptr[(size_t)i], n = row_ptr[(size_t)i + 1] - b;
+        const int64_t n = row_ptr[(size_t)i + 1] - row_ptr[(size_t)i];
         if (n <= cap) continue;
-        tmp.resize((size_t)n);
-        HIPCHECK(ctx, hipMemcpyAsync(tmp.data(), col + b, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));
-        std::sort(tmp.begin(), tmp.end());
-        HIPCHECK(ctx, hipMemcpyAsync(col + b, tmp.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));
+        if (n > (int64_t)INT32_MAX / 2) return fail(ctx, FCG_INVALID_ARGUMENT, "link row too long");
+        int64_t P = 2;
+        while (P < n) P <<= 1;
+        rows.push_back(i);
+        off.push_back(off.back() + P);
     }
+    if (rows.empty()) return FCG_OK;
+    DevBuf drows, doff, scratch;
+    HIPCHECK(ctx, upload(drows, rows.data(), rows.size(), s));
+    HIPCHECK(ctx, upload(doff, off.data(), off.size(), s));
+    HIPCHECK(ctx, scratch.alloc(sizeof(int32_t) * (size_t)off.back()));
+    HIPCHECK(ctx, fcg::launch_links_sort_long(drow.as<int64_t>(), drows.as<int32_t>(), (int)rows.size(),
+                                              doff.as<int64_t>(), col, scratch.as<int32_t>(), s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));  // the scratch and the lists are released here
     return FCG_OK;
 }
 }  // namespace

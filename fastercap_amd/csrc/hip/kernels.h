@@ -279,10 +279,13 @@ hipError_t launch_links_step(const uint64_t* in, const uint8_t* in_lev, int64_t 
 hipError_t launch_links_degree(const int2* links, int64_t P, int32_t* deg, hipStream_t s);
 hipError_t launch_links_scatter(const int2* links, int64_t P, unsigned long long* cursor, int32_t* col,
                                 hipStream_t s);
-// sorts every row of at most links_sort_lds_max() entries; longer rows are left
-// to the caller
+// sorts every row of at most links_sort_lds_max() entries; the longer ones are
+// listed in rows[nrows] for launch_links_sort_long, with scratch offsets (each
+// row padded to a power of two)
 hipError_t launch_links_sort_rows(const int64_t* row_ptr, int32_t* col, int M, hipStream_t s);
 int links_sort_lds_max();
+hipError_t launch_links_sort_long(const int64_t* row_ptr, const int32_t* rows, int nrows, const int64_t* scr_off,
+                                  int32_t* col, int32_t* scratch, hipStream_t s);
 // device layout of discovered links: stored column = charge row of the source;
 // fill class per link; stable partition by class (kClassChunk links per
 // histogram block, offsets class-major then block); 16-bit columns

@@ -220,6 +220,39 @@ __global__ void __launch_bounds__(256) k_links_sort_rows(const int64_t* __restri
     for (int i = threadIdx.x; i < n; i += 256) col[b + i] = s[i];
 }
 
+// Rows longer than kSortLds: one workgroup per row, the same bitonic network on
+// a scratch copy of the row padded to a power of two (scr_off: each listed
+// row's scratch offset).
+__global__ void __launch_bounds__(1024) k_links_sort_long(const int64_t* __restrict__ row_ptr,
+                                                          const int32_t* __restrict__ rows,
+                                                          const int64_t* __restrict__ scr_off,
+                                                          int32_t* __restrict__ col, int32_t* __restrict__ scratch) {
+    const int r = rows[blockIdx.x];
+    const int64_t b = row_ptr[r];
+    const int n = (int)(row_ptr[r + 1] - b);
+    int P = 2;
+    while (P < n) P <<= 1;
+    int32_t* S = scratch + scr_off[blockIdx.x];
+    for (int i = threadIdx.x; i < P; i += 1024) S[i] = i < n ? col[b + i] : INT32_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < P; i += 1024) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int32_t x = S[i], y = S[ixj];
+                    if (((i & k) == 0) ? (x > y) : (x < y)) {
+                        S[i] = y;
+                        S[ixj] = x;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += 1024) col[b + i] = S[i];
+}
+
 // ---------------------------------------------------------------- device layout
 // of a job whose links were discovered here (the upload's host-side layout
 // build, done on the device)
@@ -310,6 +343,13 @@ __global__ void k_col16(const int32_t* __restrict__ col, int64_t n, uint16_t* __
 
 int links_sort_lds_max() { return kSortLds; }
 int class_chunk() { return kClassChunk; }
+
+hipError_t launch_links_sort_long(const int64_t* row_ptr, const int32_t* rows, int nrows, const int64_t* scr_off,
+                                  int32_t* col, int32_t* scratch, hipStream_t s) {
+    if (nrows <= 0) return hipSuccess;
+    k_links_sort_long<<<(unsigned)nrows, 1024, 0, s>>>(row_ptr, rows, scr_off, col, scratch);
+    return hipGetLastError();
+}
 
 hipError_t launch_perm_cols(const int32_t* col_node, int64_t L, const int32_t* perm, int32_t* out, hipStream_t s) {
     if (L <= 0) return hipSuccess;

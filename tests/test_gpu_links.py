@@ -29,6 +29,8 @@ def _cases(geo_dir):
         "coax2d": (workloads.coated_coax_2d(os.path.join(geo_dir, "lk_coax")), 0.02),
         "sphere_shell": (workloads.sphere_in_shell(os.path.join(geo_dir, "lk_sphere")), 0.05),
         "bus": (workloads.crossing_bus(os.path.join(geo_dir, "lk_bus")), 600.0),  # 48M links: many slices
+        # rows of up to 11177 links: the long-row sort (rows beyond the LDS sort)
+        "boxes_long_rows": (workloads.random_boxes(os.path.join(geo_dir, "lk_boxes16"), n=16, seed=0), 1e-5),
     }
 
 
@@ -38,7 +40,7 @@ def _sorted_rows(m):
     return m.col[order]
 
 
-@pytest.mark.parametrize("name", ["plates", "boxes_interface", "coax2d", "sphere_shell", "bus"])
+@pytest.mark.parametrize("name", ["plates", "boxes_interface", "coax2d", "sphere_shell", "bus", "boxes_long_rows"])
 def test_device_links_equal_host_links(dev, geo_dir, name):
     path, eps = _cases(geo_dir)[name]
     host = fc.Mesh.build(path, mesh_eps=eps)
@@ -50,6 +52,8 @@ def test_device_links_equal_host_links(dev, geo_dir, name):
     np.testing.assert_array_equal(devm.col, _sorted_rows(host))
     assert devm.stats.links == host.stats.links
     assert devm.stats.max_level == host.stats.max_level
+    if name == "boxes_long_rows":
+        assert np.diff(host.row_ptr).max() > 4096
 
 
 def test_solve_on_device_links_matches_host_mesh(dev, geo_dir):
@@ -115,7 +119,7 @@ def _upload_on_device_links(dev, path, eps):
     return tree, job.value, n.value
 
 
-@pytest.mark.parametrize("name", ["plates", "coax2d", "sphere_shell", "bus"])
+@pytest.mark.parametrize("name", ["plates", "coax2d", "sphere_shell", "bus", "boxes_long_rows"])
 def test_device_layout_equals_host_layout(dev, geo_dir, name):
     import ctypes as C
 
