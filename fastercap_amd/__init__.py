@@ -113,20 +113,25 @@ class Mesh:
 
     @classmethod
     def build(cls, path: str, mesh_eps: float = 0.1, eps_ratio: float = 1.0, curv_coeff: float = 3.0,
-              device: "Device | None" = None) -> "Mesh":
+              device: "Device | None" = None, refine_on_device: bool = False) -> "Mesh":
         """Refine the geometry of `path` and discover its links: on the host, or
-        with `device` on that GPU (fcg_links_discover; the same link set)."""
+        with `device` the links on that GPU (fcg_links_discover) and, with
+        refine_on_device, the panel refinement too (fcg_refine).  The mesh is the
+        same either way."""
         lib = _ffi.host_lib()
         o = HostOptions()
         lib.fch_default_options(C.byref(o))
         o.mesh_eps, o.eps_ratio, o.curv_coeff = mesh_eps, eps_ratio, curv_coeff
         o.links_on_device = 1 if device is not None else 0
+        o.refine_on_device = 1 if (device is not None and refine_on_device) else 0
         h = C.c_void_p()
         rc = lib.fch_build(str(path).encode(), C.byref(o), C.byref(h))
         if rc != 0:
             raise FasterCapError(rc, lib.fch_last_error().decode())
         if device is not None:
             try:
+                if o.refine_on_device:
+                    device.refine(h.value)
                 device.discover_links(h.value)
             except BaseException:
                 lib.fch_free(h)
@@ -189,6 +194,29 @@ class Device:
         if rc != 0:
             msg = self.lib.fcg_last_error(self._h)
             raise FasterCapError(rc, f"{what}: {msg.decode() if msg else ''}")
+
+    def refine(self, mesh_handle: int) -> int:
+        """Refine the coarse tree of a mesh built with refine_on_device on this
+        device and finish its build; returns the node count."""
+        hl = _ffi.host_lib()
+        mh = C.c_void_p(mesh_handle)
+        nodes = C.POINTER(_ffi.RefineNode)()
+        tops = C.POINTER(C.c_int32)()
+        n = C.c_int()
+        p = _ffi.RefineParams()
+        if hl.fch_get_coarse_tree(mh, C.byref(nodes), C.byref(n), C.byref(tops), C.byref(p)) != 0:
+            raise FasterCapError(65, "mesh is not waiting for a refinement")
+        t0 = time.perf_counter()
+        n_out, lev = C.c_int(), C.c_int32()
+        eps = C.c_double()
+        self._check(self.lib.fcg_refine(self._h, nodes, n.value, tops, C.byref(p), C.byref(n_out), C.byref(eps),
+                                        C.byref(lev)), "fcg_refine")
+        out = (_ffi.RefineNode * n_out.value)()
+        self._check(self.lib.fcg_refine_fetch(self._h, out), "fcg_refine_fetch")
+        rc = hl.fch_set_refined_tree(mh, out, n_out.value, eps.value, lev.value, time.perf_counter() - t0)
+        if rc != 0:
+            raise FasterCapError(rc, "fch_set_refined_tree: " + hl.fch_last_error().decode())
+        return n_out.value
 
     def discover_links(self, mesh_handle: int) -> int:
         """Discover the links of a mesh built with links_on_device on this device

@@ -59,6 +59,36 @@ class LinkParams(C.Structure):
     ]
 
 
+class RefineNode(C.Structure):
+    _fields_ = [
+        ("v", C.c_float * 9),
+        ("c", C.c_float * 3),
+        ("n", C.c_float * 3),
+        ("pad0", C.c_float),
+        ("dim", C.c_double),
+        ("maxside", C.c_double),
+        ("left", C.c_int32),
+        ("right", C.c_int32),
+        ("group", C.c_int32),
+        ("type", C.c_uint8),
+        ("max_side_idx", C.c_uint8),
+        ("diel_index", C.c_uint8),
+        ("pad1", C.c_uint8),
+    ]
+
+
+class RefineParams(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("n_groups", C.c_int32),
+        ("mesh_eps", C.c_double),
+        ("curv", C.c_double),
+        ("max_side", C.c_double),
+        ("max_area", C.c_double),
+        ("max_length", C.c_double),
+    ]
+
+
 class SolveOpts(C.Structure):
     _fields_ = [
         ("tol", C.c_double),
@@ -118,6 +148,7 @@ class HostOptions(C.Structure):
         ("curv_coeff", C.c_double),
         ("verbose", C.c_int32),
         ("links_on_device", C.c_int32),
+        ("refine_on_device", C.c_int32),
     ]
 
 
@@ -192,6 +223,8 @@ DEVICE_SYMBOLS = (
     "fcg_links_discover",
     "fcg_links_fetch",
     "fcg_job_upload_links",
+    "fcg_refine",
+    "fcg_refine_fetch",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -203,6 +236,8 @@ HOST_SYMBOLS = (
     "fch_free",
     "fch_get_link_params",
     "fch_set_links",
+    "fch_get_coarse_tree",
+    "fch_set_refined_tree",
 )
 
 
@@ -240,6 +275,12 @@ def host_lib() -> C.CDLL:
         lib.fch_set_links.argtypes = [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int32), C.c_int64,
                                       C.c_int32, C.c_double]
         lib.fch_set_links.restype = C.c_int
+        lib.fch_get_coarse_tree.argtypes = [C.c_void_p, C.POINTER(C.POINTER(RefineNode)), C.POINTER(C.c_int),
+                                            C.POINTER(C.POINTER(C.c_int32)), C.POINTER(RefineParams)]
+        lib.fch_get_coarse_tree.restype = C.c_int
+        lib.fch_set_refined_tree.argtypes = [C.c_void_p, C.POINTER(RefineNode), C.c_int, C.c_double, C.c_int32,
+                                             C.c_double]
+        lib.fch_set_refined_tree.restype = C.c_int
         _host = lib
     return _host
 
@@ -300,6 +341,9 @@ def device_lib() -> C.CDLL:
                                            C.POINTER(C.c_int32)]
         lib.fcg_links_fetch.argtypes = [vp, i64p, C.POINTER(C.c_int32)]
         lib.fcg_job_upload_links.argtypes = [vp, C.POINTER(JobDesc), C.POINTER(C.c_int)]
+        lib.fcg_refine.argtypes = [vp, C.POINTER(RefineNode), C.c_int, C.POINTER(C.c_int32), C.POINTER(RefineParams),
+                                   C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+        lib.fcg_refine_fetch.argtypes = [vp, C.POINTER(RefineNode)]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p
         for name in DEVICE_SYMBOLS:

@@ -333,6 +333,12 @@ struct fcg_ctx {
             ready = false;
         }
     } links;
+    // result of the last fcg_refine, until fcg_refine_fetch
+    struct RefineOut {
+        DevBuf pool;
+        int n = 0;
+        bool ready = false;
+    } refined;
 
     fcg_ctx() {
         for (auto& row : batch_variant)
@@ -2686,6 +2692,194 @@ int fcg_links_fetch(fcg_ctx* ctx, int64_t* row_ptr, int32_t* col) {
         HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     }
     ctx->links.reset();
+    return FCG_OK;
+}
+
+// ---------------------------------------------------------------- adaptive refinement
+// AutoRefinePanels' discretization (Solver/Autorefine.cpp:293-479, 2638-2921) on
+// the device: the pairs of group tops, then the groups' self passes round by
+// round (depth by depth), each pass's pair work drained from a work stack in
+// slices of evaluate / split / expand.
+int fcg_refine(fcg_ctx* ctx, const fcg_refine_node* nodes, int n_nodes, const int32_t* tops,
+               const fcg_refine_params* p, int* n_out, double* max_mesh_eps, int32_t* max_level) {
+    if (!ctx || !nodes || n_nodes < 1 || !tops || !p || !n_out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if ((p->dim != 2 && p->dim != 3) || p->n_groups < 1 || !(p->mesh_eps > 0) || !(p->curv >= 1.0))
+        return fail(ctx, FCG_INVALID_ARGUMENT, "bad refinement parameters");
+    // pre-order (children after their parent) bounds every walk
+    for (int i = 0; i < n_nodes; ++i) {
+        const int l = nodes[i].left, r = nodes[i].right;
+        if ((l < 0) != (r < 0) || (l >= 0 && (l <= i || r <= i || l >= n_nodes || r >= n_nodes)))
+            return fail(ctx, FCG_INVALID_ARGUMENT, "bad tree: children must follow their parent");
+    }
+    for (int g = 0; g < p->n_groups; ++g)
+        if (tops[g] < 0 || tops[g] >= n_nodes) return fail(ctx, FCG_INVALID_ARGUMENT, "bad group top");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = ctx->stream;
+    ctx->refined.pool.release();
+    ctx->refined.ready = false;
+    const fcg::RefineParams P{p->dim, p->mesh_eps, p->curv, p->max_side, p->max_area, p->max_length};
+    const int64_t S = kLinkSlice;
+    // node pool and split flags (grown together; new flags start at 0)
+    DevBuf pool, want;
+    int64_t cap = std::max<int64_t>(2 * (int64_t)n_nodes, 1 << 20);
+    HIPCHECK(ctx, pool.alloc(sizeof(fcg_refine_node) * (size_t)cap));
+    HIPCHECK(ctx, want.alloc(sizeof(int32_t) * (size_t)cap));
+    HIPCHECK(ctx, hipMemcpyAsync(pool.p, nodes, sizeof(fcg_refine_node) * (size_t)n_nodes, hipMemcpyHostToDevice, s));
+    HIPCHECK(ctx, hipMemsetAsync(want.p, 0, want.bytes, s));
+    // counters: pool size, pending, stack output, next-round size, max eps bits, max level
+    DevBuf ctr;
+    HIPCHECK(ctx, ctr.alloc(6 * sizeof(unsigned long long)));
+    HIPCHECK(ctx, hipMemsetAsync(ctr.p, 0, ctr.bytes, s));
+    unsigned long long* c_pool = ctr.as<unsigned long long>();
+    unsigned long long* c_pend = c_pool + 1;
+    unsigned long long* c_out = c_pool + 2;
+    unsigned long long* c_next = c_pool + 3;
+    unsigned long long* c_eps = c_pool + 4;
+    int* c_lev = reinterpret_cast<int*>(c_pool + 5);
+    const unsigned long long n0 = (unsigned long long)n_nodes;
+    HIPCHECK(ctx, hipMemcpyAsync(c_pool, &n0, sizeof n0, hipMemcpyHostToDevice, s));
+    int64_t pool_n = n_nodes;
+    auto grow_pool = [&](int64_t need) -> int {
+        if (need <= cap) return FCG_OK;
+        const int64_t nc = std::max(need, cap + cap / 2);
+        DevBuf np, nw;
+        HIPCHECK(ctx, np.alloc(sizeof(fcg_refine_node) * (size_t)nc));
+        HIPCHECK(ctx, nw.alloc(sizeof(int32_t) * (size_t)nc));
+        HIPCHECK(ctx, hipMemcpyAsync(np.p, pool.p, sizeof(fcg_refine_node) * (size_t)pool_n, hipMemcpyDeviceToDevice, s));
+        HIPCHECK(ctx, hipMemsetAsync(nw.p, 0, nw.bytes, s));
+        HIPCHECK(ctx, hipMemcpyAsync(nw.p, want.p, sizeof(int32_t) * (size_t)pool_n, hipMemcpyDeviceToDevice, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        pool.swap(np);
+        want.swap(nw);
+        cap = nc;
+        return FCG_OK;
+    };
+    DevBuf stack, slev, pend, plev, out, olev;
+    HIPCHECK(ctx, stack.alloc(sizeof(uint64_t) * (size_t)(4 * S)));
+    HIPCHECK(ctx, slev.alloc((size_t)(4 * S)));
+    HIPCHECK(ctx, pend.alloc(sizeof(uint64_t) * (size_t)S));
+    HIPCHECK(ctx, plev.alloc((size_t)S));
+    HIPCHECK(ctx, out.alloc(sizeof(uint64_t) * (size_t)(2 * S)));
+    HIPCHECK(ctx, olev.alloc((size_t)(2 * S)));
+    int64_t top = 0;
+    // push n items from out / olev onto the stack
+    auto push_out = [&](int64_t n) -> int {
+        if (n <= 0) return FCG_OK;
+        RCCHECK(ensure_bytes(ctx, stack, sizeof(uint64_t) * (size_t)(top + n), sizeof(uint64_t) * (size_t)top, s));
+        RCCHECK(ensure_bytes(ctx, slev, (size_t)(top + n), (size_t)top, s));
+        HIPCHECK(ctx, hipMemcpyAsync(stack.as<uint64_t>() + top, out.p, sizeof(uint64_t) * (size_t)n,
+                                     hipMemcpyDeviceToDevice, s));
+        HIPCHECK(ctx, hipMemcpyAsync(slev.as<uint8_t>() + top, olev.p, (size_t)n, hipMemcpyDeviceToDevice, s));
+        top += n;
+        return FCG_OK;
+    };
+    // run the stack's pair work to completion
+    auto drain = [&]() -> int {
+        while (top > 0) {
+            const int64_t n = std::min(top, S);
+            top -= n;
+            HIPCHECK(ctx, hipMemsetAsync(c_pend, 0, sizeof(unsigned long long), s));
+            HIPCHECK(ctx, fcg::launch_refine_eval(stack.as<uint64_t>() + top, slev.as<uint8_t>() + top, n,
+                                                  pool.as<fcg_refine_node>(), P, want.as<int32_t>(),
+                                                  pend.as<uint64_t>(), plev.as<uint8_t>(), c_pend, c_eps, c_lev, s));
+            unsigned long long h[2];
+            HIPCHECK(ctx, hipMemcpyAsync(h, c_pool, sizeof h, hipMemcpyDeviceToHost, s));
+            HIPCHECK(ctx, hipStreamSynchronize(s));
+            const int64_t np = (int64_t)h[1];
+            pool_n = (int64_t)h[0];
+            if (np > n || pool_n < n_nodes) return fail(ctx, FCG_DEVICE_ERROR, "refinement: inconsistent step counts");
+            if (np == 0) continue;
+            if (pool_n + 2 * np > (int64_t)INT32_MAX) return fail(ctx, FCG_OUT_OF_MEMORY, "refinement: too many nodes");
+            RCCHECK(grow_pool(pool_n + 2 * np));
+            HIPCHECK(ctx, fcg::launch_refine_split(pend.as<uint64_t>(), np, pool.as<fcg_refine_node>(),
+                                                   want.as<int32_t>(), c_pool, P.dim, s));
+            HIPCHECK(ctx, hipMemsetAsync(c_out, 0, sizeof(unsigned long long), s));
+            HIPCHECK(ctx, fcg::launch_refine_expand(pend.as<uint64_t>(), plev.as<uint8_t>(), np,
+                                                    pool.as<fcg_refine_node>(), out.as<uint64_t>(),
+                                                    olev.as<uint8_t>(), c_out, s));
+            RCCHECK(push_out(2 * np));  // every pending item gives two pairs
+        }
+        return FCG_OK;
+    };
+    // the pairs of group tops (DiscretizeMutual from AutoRefinePanels), level 1
+    {
+        std::vector<uint64_t> init;
+        for (int i = 0; i < p->n_groups; ++i)
+            for (int j = i + 1; j < p->n_groups; ++j)
+                init.push_back(((uint64_t)(uint32_t)tops[i] << 32) | (uint32_t)tops[j]);
+        const std::vector<uint8_t> lev(init.size(), 1);
+        RCCHECK(ensure_bytes(ctx, stack, sizeof(uint64_t) * init.size(), 0, s));
+        RCCHECK(ensure_bytes(ctx, slev, init.size(), 0, s));
+        if (!init.empty()) {
+            HIPCHECK(ctx, hipMemcpyAsync(stack.p, init.data(), sizeof(uint64_t) * init.size(), hipMemcpyHostToDevice, s));
+            HIPCHECK(ctx, hipMemcpyAsync(slev.p, lev.data(), lev.size(), hipMemcpyHostToDevice, s));
+        }
+        top = (int64_t)init.size();
+        RCCHECK(drain());
+    }
+    // DiscretizeSelf, round by round: the tops (split when a leaf) at level 1,
+    // then every round's children after the round's pair work has finished
+    int maxlev = 0;
+    {
+        DevBuf front, next;
+        std::vector<int32_t> t0(tops, tops + p->n_groups);
+        std::sort(t0.begin(), t0.end());
+        t0.erase(std::unique(t0.begin(), t0.end()), t0.end());
+        HIPCHECK(ctx, upload(front, t0.data(), t0.size(), s));
+        int64_t nf = (int64_t)t0.size();
+        unsigned long long pc = 0;
+        HIPCHECK(ctx, hipMemcpyAsync(&pc, c_pool, sizeof pc, hipMemcpyDeviceToHost, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+        pool_n = (int64_t)pc;
+        RCCHECK(grow_pool(pool_n + 2 * nf));
+        HIPCHECK(ctx, fcg::launch_refine_split_tops(front.as<int32_t>(), (int)nf, pool.as<fcg_refine_node>(), c_pool,
+                                                    P.dim, s));
+        for (int lev = 1; nf > 0; ++lev) {
+            maxlev = std::max(maxlev, std::min(lev, 255));
+            RCCHECK(ensure_bytes(ctx, next, sizeof(int32_t) * (size_t)(2 * nf), 0, s));
+            RCCHECK(ensure_bytes(ctx, out, sizeof(uint64_t) * (size_t)nf, 0, s));
+            RCCHECK(ensure_bytes(ctx, olev, (size_t)nf, 0, s));
+            HIPCHECK(ctx, hipMemsetAsync(c_out, 0, 2 * sizeof(unsigned long long), s));  // c_out, c_next
+            HIPCHECK(ctx, fcg::launch_refine_self(front.as<int32_t>(), nf, lev, pool.as<fcg_refine_node>(),
+                                                  out.as<uint64_t>(), olev.as<uint8_t>(), c_out, next.as<int32_t>(),
+                                                  c_next, s));
+            unsigned long long h[2];
+            HIPCHECK(ctx, hipMemcpyAsync(h, c_out, sizeof h, hipMemcpyDeviceToHost, s));
+            HIPCHECK(ctx, hipStreamSynchronize(s));
+            if ((int64_t)h[0] > nf || h[1] != 2 * h[0])
+                return fail(ctx, FCG_DEVICE_ERROR, "refinement: inconsistent self round");
+            RCCHECK(push_out((int64_t)h[0]));
+            RCCHECK(drain());
+            front.swap(next);
+            nf = (int64_t)h[1];
+        }
+    }
+    unsigned long long h[6];
+    HIPCHECK(ctx, hipMemcpyAsync(h, ctr.p, sizeof h, hipMemcpyDeviceToHost, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    double eps = 0.0;
+    std::memcpy(&eps, &h[4], sizeof eps);
+    int dlev = 0;
+    std::memcpy(&dlev, &h[5], sizeof dlev);
+    ctx->refined.pool.swap(pool);
+    ctx->refined.n = (int)h[0];
+    ctx->refined.ready = true;
+    *n_out = (int)h[0];
+    if (max_mesh_eps) *max_mesh_eps = eps;
+    if (max_level) *max_level = std::max(maxlev, dlev);
+    return FCG_OK;
+}
+
+int fcg_refine_fetch(fcg_ctx* ctx, fcg_refine_node* out) {
+    if (!ctx || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (!ctx->refined.ready) return fail(ctx, FCG_INVALID_ARGUMENT, "no refined tree to fetch");
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    HIPCHECK(ctx, hipMemcpyAsync(out, ctx->refined.pool.p, sizeof(fcg_refine_node) * (size_t)ctx->refined.n,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->refined.pool.release();
+    ctx->refined.ready = false;
+    ctx->refined.n = 0;
     return FCG_OK;
 }
 

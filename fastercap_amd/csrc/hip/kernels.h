@@ -286,6 +286,27 @@ hipError_t launch_links_sort_rows(const int64_t* row_ptr, int32_t* col, int M, h
 int links_sort_lds_max();
 hipError_t launch_links_sort_long(const int64_t* row_ptr, const int32_t* rows, int nrows, const int64_t* scr_off,
                                   int32_t* col, int32_t* scratch, hipStream_t s);
+
+// Adaptive refinement (links.hip).  A work item is a node pair (a << 32 | b),
+// bit 63 set inside one group (curvature coefficient applies).
+struct RefineParams {
+    int dim;
+    double mesh_eps, curv;
+    double max_side, max_area, max_length;
+};
+hipError_t launch_refine_eval(const uint64_t* in, const uint8_t* in_lev, int64_t n, const fcg_refine_node* nodes,
+                              const RefineParams& P, int32_t* want, uint64_t* pend, uint8_t* pend_lev,
+                              unsigned long long* pend_cnt, unsigned long long* max_eps, int* max_lev, hipStream_t s);
+hipError_t launch_refine_split(const uint64_t* pend, int64_t n, fcg_refine_node* nodes, int32_t* want,
+                               unsigned long long* pool_cnt, int dim, hipStream_t s);
+hipError_t launch_refine_expand(const uint64_t* pend, const uint8_t* pend_lev, int64_t n,
+                                const fcg_refine_node* nodes, uint64_t* out, uint8_t* out_lev,
+                                unsigned long long* out_cnt, hipStream_t s);
+hipError_t launch_refine_split_tops(const int32_t* list, int n, fcg_refine_node* nodes, unsigned long long* pool_cnt,
+                                    int dim, hipStream_t s);
+hipError_t launch_refine_self(const int32_t* front, int64_t n, int lev, const fcg_refine_node* nodes, uint64_t* out,
+                              uint8_t* out_lev, unsigned long long* out_cnt, int32_t* next,
+                              unsigned long long* next_cnt, hipStream_t s);
 // device layout of discovered links: stored column = charge row of the source;
 // fill class per link; stable partition by class (kClassChunk links per
 // histogram block, offsets class-major then block); 16-bit columns

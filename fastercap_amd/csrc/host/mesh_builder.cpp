@@ -1116,6 +1116,15 @@ struct Builder {
 }  // namespace
 
 struct fch_mesh {
+    // a build stopped after the super hierarchy (refine_on_device): the builder,
+    // its coarse tree in flat form and the elements behind the flat indices
+    std::unique_ptr<Builder> pending;
+    std::vector<fcg_refine_node> coarse;
+    std::vector<Elem*> coarse_elems;
+    std::vector<int32_t> coarse_tops;
+    fcg_refine_params refine_params{};
+    double t_read_s = 0, t_superh_s = 0;
+
     int dim = 3;
     std::vector<fcg_node_geom> nodes;
     std::vector<int32_t> left, right, parent, leaf_lo, leaf_hi, leaf_node;
@@ -1136,70 +1145,40 @@ void fch_default_options(fch_options* o) {
     o->curv_coeff = 3.0;
     o->verbose = 0;
     o->links_on_device = 0;
+    o->refine_on_device = 0;
 }
 
 const char* fch_last_error(void) { return g_err.c_str(); }
 
-int fch_build(const char* path, const fch_options* opts, fch_mesh** out) {
-    using clk = std::chrono::steady_clock;
-    g_err.clear();
-    if (!path || !out) {
-        g_err = "null argument";
-        return FCG_INVALID_ARGUMENT;
-    }
-    *out = nullptr;
+}  // extern "C"
+
+namespace {
+using clk = std::chrono::steady_clock;
+double seconds(clk::time_point a, clk::time_point c) { return std::chrono::duration<double>(c - a).count(); }
+
+// runs f, mapping the builder's exceptions to FC_* codes
+template <typename F>
+int guarded(F&& f) {
     try {
-        Builder b;
-        fch_options o;
-        if (opts)
-            o = *opts;
-        else
-            fch_default_options(&o);
-        b.opt = o;
-        b.meshEps = o.mesh_eps;
-        // CSolveCap::Run: link eps = ratio * mesh eps, 0 -> 0.01; curvature >= 1 (SolveCapacitance.cpp:108-119)
-        b.linkEps = o.eps_ratio * o.mesh_eps;
-        if (b.linkEps == 0.0) b.linkEps = 0.01;
-        b.curv = o.curv_coeff < 1.0 ? 1.0 : o.curv_coeff;
-        for (int i = 0; i < kMaxParseLevel + 2; ++i) b.groupNum[i] = 1;
+        return f();
+    } catch (const std::bad_alloc&) {
+        g_err = "out of memory";
+        return FCG_OUT_OF_MEMORY;
+    } catch (const std::runtime_error& e) {
+        g_err = e.what();
+        if (g_err.rfind("cannot open", 0) == 0) return FCG_CANNOT_OPEN_FILE;
+        return FCG_FILE_ERROR;
+    } catch (...) {
+        g_err = "unhandled exception";
+        return FCG_EXCEPTION_ERROR;
+    }
+}
 
-        std::string p(path);
-        const size_t slash = p.find_last_of('/');
-        std::string fname = p;
-        if (slash != std::string::npos) {
-            b.baseDir = p.substr(0, slash);
-            fname = p.substr(slash + 1);
-        }
-        auto t0 = clk::now();
-        // ReadFastCapFile: 2D when the first line mentions "2D" (Autorefine.cpp:4650)
-        {
-            auto ft = b.loadFile(fname);
-            const std::string first = ft->lines.empty() ? std::string() : ft->lines[0];
-            b.dim = (first.find("2D") != std::string::npos || first.find("2d") != std::string::npos) ? 2 : 3;
-        }
-        b.parseFile(fname, nullptr, false, V3{}, 1.0, 0.0, "", 0.0, 0.0, V3{});
-        if (b.groups.empty()) throw std::runtime_error("no panels found in the input file");
-        auto t1 = clk::now();
-        b.buildSuperHierarchy();
-        auto t2 = clk::now();
-
-        // AutoRefinePanels (Autorefine.cpp:293-479): mutual then self discretization, then
-        // the counting pass of ComputeLinks.
-        b.level = 0;
-        b.maxLevel = 0;
-        b.maxMeshEps = 0.0;
-        for (size_t i = 0; i < b.groups.size(); ++i)
-            for (size_t j = i + 1; j < b.groups.size(); ++j)
-                b.discretizeMutual(b.groups[i]->top, b.groups[j]->top, false);
-        for (auto& g : b.groups) {
-            b.level = 0;
-            b.discretizeSelf(g->top);
-        }
-        b.computeLinks = false;
-        b.linksNum = 0;
-        if (!o.links_on_device) b.computeLinksPass();
+// RecurseIndex, the link record pass (unless the links come from the device)
+// and the flattening into the mesh arrays
+std::unique_ptr<fch_mesh> finish_build(Builder& b, bool links_on_device, double t_read, double t_superh,
+                                       double t_disc) {
         auto t3 = clk::now();
-
         // AutoRefineLinks structure part: RecurseIndex, then the record pass.
         b.nodes.reserve((size_t)b.nodeNum);
         b.baseLinks = 0;
@@ -1216,7 +1195,7 @@ int fch_build(const char* path, const fch_options* opts, fch_mesh** out) {
             throw std::runtime_error("mesh too large");
         b.colOut.assign((size_t)b.linksNum, -1);
         b.computeLinks = true;
-        if (!o.links_on_device) b.computeLinksPass();
+        if (!links_on_device) b.computeLinksPass();
         auto t4 = clk::now();
 
         auto m = std::make_unique<fch_mesh>();
@@ -1322,28 +1301,220 @@ int fch_build(const char* path, const fch_options* opts, fch_mesh** out) {
         s.dim = b.dim;
         s.max_level = b.maxLevel;
         s.max_mesh_eps = b.maxMeshEps;
-        auto sec = [](clk::time_point a, clk::time_point c) { return std::chrono::duration<double>(c - a).count(); };
-        s.t_read_s = sec(t0, t1);
-        s.t_superh_s = sec(t1, t2);
-        s.t_discretize_s = sec(t2, t3);
-        s.t_links_s = sec(t3, t4);
+        s.t_read_s = t_read;
+        s.t_superh_s = t_superh;
+        s.t_discretize_s = t_disc;
+        s.t_links_s = seconds(t3, t4);
+        return m;
+}
+}  // namespace
+
+extern "C" {
+
+int fch_build(const char* path, const fch_options* opts, fch_mesh** out) {
+    g_err.clear();
+    if (!path || !out) {
+        g_err = "null argument";
+        return FCG_INVALID_ARGUMENT;
+    }
+    *out = nullptr;
+    return guarded([&]() -> int {
+        auto bp = std::make_unique<Builder>();
+        Builder& b = *bp;
+        fch_options o;
+        if (opts)
+            o = *opts;
+        else
+            fch_default_options(&o);
+        b.opt = o;
+        b.meshEps = o.mesh_eps;
+        // CSolveCap::Run: link eps = ratio * mesh eps, 0 -> 0.01; curvature >= 1 (SolveCapacitance.cpp:108-119)
+        b.linkEps = o.eps_ratio * o.mesh_eps;
+        if (b.linkEps == 0.0) b.linkEps = 0.01;
+        b.curv = o.curv_coeff < 1.0 ? 1.0 : o.curv_coeff;
+        for (int i = 0; i < kMaxParseLevel + 2; ++i) b.groupNum[i] = 1;
+
+        std::string p(path);
+        const size_t slash = p.find_last_of('/');
+        std::string fname = p;
+        if (slash != std::string::npos) {
+            b.baseDir = p.substr(0, slash);
+            fname = p.substr(slash + 1);
+        }
+        auto t0 = clk::now();
+        // ReadFastCapFile: 2D when the first line mentions "2D" (Autorefine.cpp:4650)
+        {
+            auto ft = b.loadFile(fname);
+            const std::string first = ft->lines.empty() ? std::string() : ft->lines[0];
+            b.dim = (first.find("2D") != std::string::npos || first.find("2d") != std::string::npos) ? 2 : 3;
+        }
+        b.parseFile(fname, nullptr, false, V3{}, 1.0, 0.0, "", 0.0, 0.0, V3{});
+        if (b.groups.empty()) throw std::runtime_error("no panels found in the input file");
+        auto t1 = clk::now();
+        b.buildSuperHierarchy();
+        auto t2 = clk::now();
+
+        if (o.refine_on_device) {
+            // stop here: the coarse tree goes to the device (fch_get_coarse_tree,
+            // fcg_refine, fch_set_refined_tree); pre-order, children after parents
+            auto m = std::make_unique<fch_mesh>();
+            m->dim = b.dim;
+            m->t_read_s = seconds(t0, t1);
+            m->t_superh_s = seconds(t1, t2);
+            for (size_t gi = 0; gi < b.groups.size(); ++gi) {
+                std::vector<Elem*> st{b.groups[gi]->top};
+                m->coarse_tops.push_back((int32_t)m->coarse.size());
+                while (!st.empty()) {
+                    Elem* e = st.back();
+                    st.pop_back();
+                    e->nodeId = (int32_t)m->coarse.size();
+                    fcg_refine_node r{};
+                    for (int k = 0; k < 3; ++k) {
+                        r.v[3 * k] = e->v[k].x;
+                        r.v[3 * k + 1] = e->v[k].y;
+                        r.v[3 * k + 2] = e->v[k].z;
+                    }
+                    r.c[0] = e->c.x;
+                    r.c[1] = e->c.y;
+                    r.c[2] = e->c.z;
+                    r.n[0] = e->n.x;
+                    r.n[1] = e->n.y;
+                    r.n[2] = e->n.z;
+                    r.dim = e->dim;
+                    r.maxside = e->maxside;
+                    r.left = r.right = -1;
+                    r.group = (int32_t)gi;
+                    r.type = e->type;
+                    r.max_side_idx = e->maxSideIdx;
+                    r.diel_index = e->dielIndex;
+                    m->coarse.push_back(r);
+                    m->coarse_elems.push_back(e);
+                    if (!e->isLeaf()) {
+                        st.push_back(e->right);
+                        st.push_back(e->left);
+                    }
+                }
+            }
+            // children indices, now that every element has one
+            for (size_t i = 0; i < m->coarse.size(); ++i) {
+                Elem* e = m->coarse_elems[i];
+                if (e->isLeaf()) continue;
+                m->coarse[i].left = e->left->nodeId;
+                m->coarse[i].right = e->right->nodeId;
+            }
+            fcg_refine_params& rp = m->refine_params;
+            rp.dim = b.dim;
+            rp.n_groups = (int32_t)b.groups.size();
+            rp.mesh_eps = b.meshEps;
+            rp.curv = b.curv;
+            rp.max_side = b.maxSide;
+            rp.max_area = b.maxArea;
+            rp.max_length = b.maxLength;
+            m->pending = std::move(bp);
+            *out = m.release();
+            return FCG_OK;
+        }
+
+        // AutoRefinePanels (Autorefine.cpp:293-479): mutual then self discretization, then
+        // the counting pass of ComputeLinks.
+        b.level = 0;
+        b.maxLevel = 0;
+        b.maxMeshEps = 0.0;
+        for (size_t i = 0; i < b.groups.size(); ++i)
+            for (size_t j = i + 1; j < b.groups.size(); ++j)
+                b.discretizeMutual(b.groups[i]->top, b.groups[j]->top, false);
+        for (auto& g : b.groups) {
+            b.level = 0;
+            b.discretizeSelf(g->top);
+        }
+        b.computeLinks = false;
+        b.linksNum = 0;
+        if (!o.links_on_device) b.computeLinksPass();
+        auto t3 = clk::now();
+        auto m = finish_build(b, o.links_on_device != 0, seconds(t0, t1), seconds(t1, t2), seconds(t2, t3));
         *out = m.release();
         return FCG_OK;
-    } catch (const std::bad_alloc&) {
-        g_err = "out of memory";
-        return FCG_OUT_OF_MEMORY;
-    } catch (const std::runtime_error& e) {
-        g_err = e.what();
-        if (g_err.rfind("cannot open", 0) == 0) return FCG_CANNOT_OPEN_FILE;
-        return FCG_FILE_ERROR;
-    } catch (...) {
-        g_err = "unhandled exception";
-        return FCG_EXCEPTION_ERROR;
+    });
+}
+
+int fch_get_coarse_tree(const fch_mesh* m, const fcg_refine_node** nodes, int* n_nodes, const int32_t** tops,
+                        fcg_refine_params* params) {
+    if (!m || !nodes || !n_nodes || !tops || !params) return FCG_INVALID_ARGUMENT;
+    if (!m->pending) {
+        g_err = "mesh is not waiting for a refinement";
+        return FCG_INVALID_ARGUMENT;
     }
+    *nodes = m->coarse.data();
+    *n_nodes = (int)m->coarse.size();
+    *tops = m->coarse_tops.data();
+    *params = m->refine_params;
+    return FCG_OK;
+}
+
+int fch_set_refined_tree(fch_mesh* m, const fcg_refine_node* nodes, int n, double max_mesh_eps, int32_t max_level,
+                         double t_discretize_s) {
+    if (!m || !nodes || !m->pending) return FCG_INVALID_ARGUMENT;
+    const int K0 = (int)m->coarse.size();
+    if (n < K0) return FCG_INVALID_ARGUMENT;
+    return guarded([&]() -> int {
+        Builder& b = *m->pending;
+        std::vector<Elem*> E((size_t)n);
+        for (int i = 0; i < n; ++i) E[(size_t)i] = i < K0 ? m->coarse_elems[(size_t)i] : b.newElem();
+        for (int i = 0; i < n; ++i) {
+            const fcg_refine_node& r = nodes[i];
+            Elem& e = *E[(size_t)i];
+            if (i >= K0) {
+                for (int k = 0; k < 3; ++k) {
+                    e.v[k].x = r.v[3 * k];
+                    e.v[k].y = r.v[3 * k + 1];
+                    e.v[k].z = r.v[3 * k + 2];
+                }
+                e.c = F3{r.c[0], r.c[1], r.c[2]};
+                e.n = F3{r.n[0], r.n[1], r.n[2]};
+                e.dim = r.dim;
+                e.maxside = r.maxside;
+                e.maxSideIdx = r.max_side_idx;
+                e.dielIndex = r.diel_index;
+                e.group = r.group;
+            }
+            e.type = r.type;
+            if ((r.left < 0) != (r.right < 0) || (r.left >= 0 && (r.left <= i || r.right <= i || r.left >= n ||
+                                                                   r.right >= n)) ||
+                ((r.left < 0) != e.isLeaf()))
+                throw std::runtime_error("refined tree: bad children");
+            e.left = r.left >= 0 ? E[(size_t)r.left] : nullptr;
+            e.right = r.right >= 0 ? E[(size_t)r.right] : nullptr;
+        }
+        // DiscretizeSelf's counts: every node of every group tree, and the leaves
+        b.nodeNum = b.panelNum = 0;
+        for (auto& g : b.groups) {
+            std::vector<Elem*> st{g->top};
+            while (!st.empty()) {
+                Elem* e = st.back();
+                st.pop_back();
+                ++b.nodeNum;
+                if (e->isLeaf()) {
+                    ++b.panelNum;
+                } else {
+                    st.push_back(e->right);
+                    st.push_back(e->left);
+                }
+            }
+        }
+        b.maxMeshEps = max_mesh_eps;
+        b.maxLevel = max_level;
+        auto fm = finish_build(b, true, m->t_read_s, m->t_superh_s, t_discretize_s);
+        *m = std::move(*fm);
+        return FCG_OK;
+    });
 }
 
 int fch_get_desc(const fch_mesh* m, fcg_job_desc* d) {
     if (!m || !d) return FCG_INVALID_ARGUMENT;
+    if (m->pending) {
+        g_err = "mesh is waiting for its refined tree (fch_set_refined_tree)";
+        return FCG_INVALID_ARGUMENT;
+    }
     std::memset(d, 0, sizeof *d);
     d->dim = m->dim;
     d->n_leaves = (int32_t)m->leaf_node.size();

@@ -229,6 +229,57 @@ int fcg_links_fetch(fcg_ctx* ctx, int64_t* row_ptr, int32_t* col);
  * AutoRefineLinks storage set-up (Solver/Autorefine.cpp:481-706). */
 int fcg_job_upload_links(fcg_ctx* ctx, const fcg_job_desc* d, int* job);
 
+/* ---------------------------------------------------------------- adaptive refinement */
+
+/* One node of a tree under refinement (the host builder's element record):
+ * children are indices into the same array, -1 for leaves. */
+typedef struct fcg_refine_node {
+    float v[9];           /* vertices (2D: two) */
+    float c[3];           /* centroid */
+    float n[3];           /* stored normal */
+    float pad0;
+    double dim;           /* area (3D) or length (2D) */
+    double maxside;       /* 3D: longest side */
+    int32_t left, right;  /* children, -1 for leaves */
+    int32_t group;        /* owning group (deque order) */
+    uint8_t type;         /* FCG_NODE_* flags */
+    uint8_t max_side_idx; /* 3D: index of the longest side */
+    uint8_t diel_index;   /* outer-medium index of a conductor panel */
+    uint8_t pad1;
+} fcg_refine_node;
+
+typedef struct fcg_refine_params {
+    int32_t dim;         /* 3 or 2 */
+    int32_t n_groups;
+    double mesh_eps;     /* -m */
+    double curv;         /* -mc, clamped to >= 1 */
+    double max_side;     /* 3D: longest side of the group tops */
+    double max_area;     /* 3D: largest group top area */
+    double max_length;   /* 2D: longest group top */
+} fcg_refine_params;
+
+/* Refine a super hierarchy on the device as CAutoRefine::AutoRefinePanels does
+ * with DiscretizeMutual / DiscretizeSelf: every pair of group tops, then every
+ * group's own tree, a pair's larger node split (longest-side bisection, 2D
+ * halving) while RefineCriteria with the mesh eps (and, inside one group, the
+ * curvature coefficient) asks for it.  A node's self pass runs after every pass
+ * that can split it, as in the reference's recursion, so the refined tree is the
+ * host builder's.  nodes[n_nodes] with tops[n_groups] (the group roots); *n_out
+ * receives the refined node count (the input nodes keep their indices, new ones
+ * follow), *max_mesh_eps the largest relative estimate of an accepted pair of
+ * non-super nodes and *max_level the deepest recursion level.  The refined nodes
+ * stay on the device until fcg_refine_fetch.
+ * Replaces AutoRefinePanels' discretization, DiscretizeMutual, DiscretizeSelf,
+ * RefineCriteria and CAutoPanel/CAutoSegment::Subdivide
+ * (Solver/Autorefine.cpp:293-479, 2638-2921, 2928-3092; AutoPanel.cpp:382-523;
+ * AutoSegment.cpp:152-230). */
+int fcg_refine(fcg_ctx* ctx, const fcg_refine_node* nodes, int n_nodes, const int32_t* tops,
+               const fcg_refine_params* p, int* n_out, double* max_mesh_eps, int32_t* max_level);
+
+/* Copy the refined nodes (n_out of the last fcg_refine) to the host and free
+ * them on the device. */
+int fcg_refine_fetch(fcg_ctx* ctx, fcg_refine_node* out);
+
 /* Solve every conductor right-hand side of the job with right-preconditioned GMRES
  * and assemble the Maxwell capacitance matrix cap[rows][n_cond] (row-major; rows =
  * n_cond in 3D, n_cond-1 in 2D; 2D values unscaled, as the reference keeps them

@@ -17,6 +17,12 @@
  * and the node indexing; the caller discovers the links on the device
  * (fcg_links_discover with fch_get_link_params, then fcg_links_fetch) and hands
  * them back with fch_set_links.
+ *
+ * With fch_options.refine_on_device it stops earlier, after the super
+ * hierarchy: fch_get_coarse_tree gives the tree and parameters for fcg_refine,
+ * and fch_set_refined_tree takes the refined tree back and finishes the build
+ * (node indexing, no link passes: the links then come from the device as
+ * above).  fch_get_desc fails on a mesh waiting for its refined tree.
  */
 #ifndef FASTERCAP_HOST_H
 #define FASTERCAP_HOST_H
@@ -33,6 +39,7 @@ typedef struct fch_options {
     double curv_coeff; /* -mc m_dMeshCurvCoeff (default 3.0, clamped to >= 1) */
     int32_t verbose;   /* -v */
     int32_t links_on_device; /* 1: no link passes here; see fch_set_links */
+    int32_t refine_on_device; /* 1: stop after the super hierarchy; see fch_get_coarse_tree */
 } fch_options;
 
 typedef struct fch_stats {
@@ -61,6 +68,18 @@ int fch_get_link_params(const fch_mesh* mesh, fcg_link_params* params);
  * (the rows stay on the device, fcg_job_upload_links). */
 int fch_set_links(fch_mesh* mesh, const int64_t* row_ptr, const int32_t* col, int64_t n_links, int32_t max_level,
                   double t_links_s);
+
+/* The coarse tree of a mesh built with refine_on_device (pre-order: children
+ * after their parent), its group tops and the refinement parameters; valid while
+ * the mesh waits for fch_set_refined_tree. */
+int fch_get_coarse_tree(const fch_mesh* mesh, const fcg_refine_node** nodes, int* n_nodes, const int32_t** tops,
+                        fcg_refine_params* params);
+
+/* Finish a refine_on_device build with the refined tree (fcg_refine_fetch: the
+ * coarse nodes first, in their order, then the new ones), the largest accepted
+ * relative estimate and the recursion depth of the refinement. */
+int fch_set_refined_tree(fch_mesh* mesh, const fcg_refine_node* nodes, int n_nodes, double max_mesh_eps,
+                         int32_t max_level, double t_discretize_s);
 
 /* Point desc at the mesh's arrays (valid while the mesh lives). */
 int fch_get_desc(const fch_mesh* mesh, fcg_job_desc* desc);
