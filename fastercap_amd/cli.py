@@ -42,7 +42,7 @@ class Args:
     precision: int = 0  # FCG_PREC_FP64
     verbose: bool = False
     device: int = 0
-    host_links: bool = False  # -glinks-host: link discovery on the host builder
+    host_links: bool = False  # -glinks-host: refinement and link discovery on the host builder
     stop: bool = False  # -b? / -bv
 
 
@@ -170,7 +170,7 @@ def solve_file(dev, a: Args, mesh_eps: float, tol: float):
     from . import Mesh
 
     mesh = Mesh.build(a.file, mesh_eps=mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff,
-                      device=None if a.host_links else dev)
+                      device=None if a.host_links else dev, refine_on_device=not a.host_links)
     job = dev.upload(mesh)
     try:
         dev.fill(job, a.precision)

@@ -35,7 +35,7 @@ struct Options {
     bool jacobi_set = false;  // -pj given (kept only without -ps, SolveCapacitance.cpp:122-125)
     int precision = FCG_PREC_FP64;
     int device = 0;
-    bool host_links = false;  // -glinks-host: link discovery on the host builder
+    bool host_links = false;  // -glinks-host: refinement and link discovery on the host builder
 };
 
 void usage() {
@@ -57,7 +57,8 @@ void usage() {
         "  -bv      Print only the version\n"
         "  -gmixed  Mixed precision (FP32 far field and products, FP64 refinement)\n"
         "  -gdev<n> HIP device (default 0)\n"
-        "  -glinks-host  Discover the links on the host (default: on the device, same links)\n");
+        "  -glinks-host  Refine the panels and discover the links on the host (default: on the device,\n"
+        "                the same mesh and links)\n");
 }
 
 // returns 0 or FC_COMMAND_LINE_ERROR
@@ -187,11 +188,36 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
     ho.curv_coeff = o.curv;
     ho.verbose = o.verbose ? 1 : 0;
     ho.links_on_device = o.host_links ? 0 : 1;
+    ho.refine_on_device = o.host_links ? 0 : 1;
     fch_mesh* mesh = nullptr;
     int rc = fch_build(o.file.c_str(), &ho, &mesh);
     if (rc != 0) {
         std::fprintf(stderr, "Error: %s\n", fch_last_error());
         return rc;
+    }
+    if (!o.host_links) {
+        // the panel refinement on the device (fcg_refine), handed back to the builder
+        const auto t0 = std::chrono::steady_clock::now();
+        const fcg_refine_node* coarse = nullptr;
+        const int32_t* tops = nullptr;
+        int n = 0, n_out = 0;
+        fcg_refine_params rp;
+        double eps = 0.0;
+        int32_t lev = 0;
+        rc = fch_get_coarse_tree(mesh, &coarse, &n, &tops, &rp);
+        if (rc == FCG_OK) rc = fcg_refine(ctx, coarse, n, tops, &rp, &n_out, &eps, &lev);
+        std::vector<fcg_refine_node> refined;
+        if (rc == FCG_OK) {
+            refined.resize((size_t)n_out);
+            rc = fcg_refine_fetch(ctx, refined.data());
+        }
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rc == FCG_OK) rc = fch_set_refined_tree(mesh, refined.data(), n_out, eps, lev, t);
+        if (rc != FCG_OK) {
+            std::fprintf(stderr, "Error: refinement: %s %s\n", fcg_last_error(ctx), fch_last_error());
+            fch_free(mesh);
+            return rc;
+        }
     }
     fcg_job_desc d;
     fch_get_desc(mesh, &d);

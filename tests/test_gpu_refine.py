@@ -81,3 +81,21 @@ def test_two_cubes_chain_refined_on_device_matches_reference_output(dev, geo_dir
         for a, b in zip(res.cap.ravel(), np.array(it["cap"]).ravel()):
             assert float("%.6g" % a) == pytest.approx(b, rel=1e-12), (a, b)
         m = mesh.stats.max_mesh_eps / math.sqrt(2.0)
+
+
+@pytest.mark.parametrize("name", ["plates", "coax2d", "boxes_interface"])
+def test_binary_device_build_prints_host_build_output(geo_dir, name):
+    # the command line's default (refinement and links on the device) against
+    # -glinks-host (the host builder's passes): the same statistics and matrix
+    import subprocess
+
+    binp = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "fastercap_amd", "bin",
+                        "fastercap")
+    path, eps = _cases(geo_dir)[name]
+    outs = []
+    for extra in ([], ["-glinks-host"]):
+        r = subprocess.run([binp, path, f"-m{eps}", "-t1e-6"] + extra, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        keep = [ln for ln in r.stdout.splitlines() if not ln.startswith(("Total", "Input file", "Running"))]
+        outs.append("\n".join(keep))
+    assert outs[0] == outs[1]
