@@ -238,6 +238,7 @@ HOST_SYMBOLS = (
     "fch_set_links",
     "fch_get_coarse_tree",
     "fch_set_refined_tree",
+    "fch_write_fastcap",
 )
 
 
@@ -281,6 +282,8 @@ def host_lib() -> C.CDLL:
         lib.fch_set_refined_tree.argtypes = [C.c_void_p, C.POINTER(RefineNode), C.c_int, C.c_double, C.c_int32,
                                              C.c_double]
         lib.fch_set_refined_tree.restype = C.c_int
+        lib.fch_write_fastcap.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.POINTER(C.c_double)]
+        lib.fch_write_fastcap.restype = C.c_int
         _host = lib
     return _host
 

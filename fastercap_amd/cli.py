@@ -43,6 +43,9 @@ class Args:
     verbose: bool = False
     device: int = 0
     host_links: bool = False  # -glinks-host: refinement and link discovery on the host builder
+    out_csv: bool = False  # -e
+    out_geo: bool = False  # -o
+    out_charge: bool = False  # -c
     stop: bool = False  # -b? / -bv
 
 
@@ -106,6 +109,12 @@ def parse_args(argv) -> Args:
                 a.super_dim = 16
         elif opt == "-glinks-host":
             a.host_links = True
+        elif opt == "-e":
+            a.out_csv = True
+        elif opt == "-o":
+            a.out_geo = True
+        elif opt == "-c":
+            a.out_charge = True
         elif opt == "-gmixed":
             a.precision = 1
         elif opt.startswith("-gdev"):
@@ -175,10 +184,50 @@ def solve_file(dev, a: Args, mesh_eps: float, tol: float):
     try:
         dev.fill(job, a.precision)
         precond, size = preconditioner(a, mesh)
-        res = dev.solve(job, mesh, tol=tol, precond=precond, precision=a.precision, precond_size=size)
+        res = dev.solve(job, mesh, tol=tol, precond=precond, precision=a.precision, precond_size=size,
+                        want_charges=a.out_charge)
     finally:
         dev.release(job)
     return mesh, res
+
+
+def write_csv(path: str, cap: np.ndarray, dim: int, out=sys.stdout):
+    """CSolveCap::OutputCapMtxToFile (Solver/SolveCapacitance.cpp:835-911)."""
+    from . import TWO_PI_TIMES_E0
+
+    base = path[: path.rfind(".")] if "." in path else path
+    name = base + ".csv"
+    M, N = cap.shape
+    if dim == 2 and N > 1:
+        N -= 1
+    scale = TWO_PI_TIMES_E0 if dim == 2 else 1.0
+    out.write(f'Generating capacitance matrix file "{name}"\n')
+    with open(name, "w") as f:
+        for i in range(M):
+            f.write(", ".join("%g" % (cap[i, j] * scale) for j in range(N)) + "\n")
+
+
+def write_outputs(a: Args, mesh, res, out=sys.stdout):
+    """The file outputs after the last solve (Solver/SolveCapacitance.cpp:533-566)."""
+    import ctypes as C
+
+    from . import _ffi
+
+    if a.out_csv:
+        write_csv(a.file, res.cap, mesh.dim, out)
+    lib = _ffi.host_lib()
+    h = C.c_void_p(mesh._h)
+    if a.out_charge and res.charges is not None:
+        out.write("\n")
+        for i, name in enumerate(mesh.conductor_names[: res.cap.shape[0]]):
+            out.write(f"Outputting charge densities for conductor {name}..\n")
+            q = np.ascontiguousarray(res.charges[i], dtype=np.float64)
+            lib.fch_write_fastcap(h, a.file.encode(), name.encode(), q.ctypes.data_as(C.POINTER(C.c_double)))
+            out.write("Done\n")
+    if a.out_geo:
+        out.write("\nOutputting refined geometry in FastCap compatible format..\n")
+        lib.fch_write_fastcap(h, a.file.encode(), b"ref", None)
+        out.write("Done\n")
 
 
 def run(a: Args, out=sys.stdout):
@@ -190,6 +239,7 @@ def run(a: Args, out=sys.stdout):
         if not a.auto:
             mesh, res = solve_file(dev, a, a.mesh_eps, a.tol)
             out.write(format_cap(res.cap, mesh.conductor_names, mesh.dim, a.verbose))
+            write_outputs(a, mesh, res, out)
             return mesh, res
         tol = a.auto_err / 2.0
         out.write(f"Auto calculation with max error: {a.auto_err:g}\n\nIteration number #0\n")
@@ -212,6 +262,7 @@ def run(a: Args, out=sys.stdout):
             old_p, old_l = mesh.N, mesh.L
             if diff <= a.auto_err:
                 break
+        write_outputs(a, mesh, res, out)
         return mesh, res
     finally:
         dev.close()

@@ -13,6 +13,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -36,6 +37,9 @@ struct Options {
     int precision = FCG_PREC_FP64;
     int device = 0;
     bool host_links = false;  // -glinks-host: refinement and link discovery on the host builder
+    bool out_csv = false;     // -e: matrix to <input>.csv
+    bool out_geo = false;     // -o: refined geometry in FastCap format
+    bool out_charge = false;  // -c: charge densities per conductor in FastCap format
 };
 
 void usage() {
@@ -51,6 +55,9 @@ void usage() {
         "  -pb<n>   Block preconditioner, blocks of at most n panels (2..128)\n"
         "  -ps<n>   Two-level preconditioner of dimension n (16..4096)\n"
         "  -pn      No preconditioner\n"
+        "  -e       Also write the capacitance matrix to <input>.csv\n"
+        "  -o       Write the refined geometry in FastCap format (<input>_ref.lst)\n"
+        "  -c       Write each conductor's charge densities in FastCap format (<input>_<conductor>.lst)\n"
         "  -v       Verbose output\n"
         "  -b       Console mode (always)\n"
         "  -b?      This text\n"
@@ -134,6 +141,12 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
             }
         } else if (!std::strcmp(a, "-glinks-host")) {
             o.host_links = true;
+        } else if (!std::strcmp(a, "-e")) {
+            o.out_csv = true;
+        } else if (!std::strcmp(a, "-o")) {
+            o.out_geo = true;
+        } else if (!std::strcmp(a, "-c")) {
+            o.out_charge = true;
         } else if (!std::strcmp(a, "-gmixed")) {
             o.precision = FCG_PREC_MIXED;
         } else if (!std::strncmp(a, "-gdev", 5)) {
@@ -172,6 +185,8 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
 }
 
 struct Result {
+    std::shared_ptr<fch_mesh> mesh;  // kept for the -o / -c writers
+    std::vector<double> charges;     // [rows][N] with -c
     std::vector<double> cap;
     int rows = 0, cols = 0, dim = 3;
     std::vector<std::string> names;
@@ -264,12 +279,64 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
         fcg_solve_opts so{tol, 1000, precond, o.precision, 8, size};
         const int nc = (int)r.names.size();
         r.cap.assign((size_t)nc * nc, 0.0);
-        rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, nullptr, &r.ss);
+        if (o.out_charge) r.charges.assign((size_t)nc * (size_t)d.n_leaves, 0.0);
+        rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, o.out_charge ? r.charges.data() : nullptr,
+                       &r.ss);
     }
     if (rc != FCG_OK) std::fprintf(stderr, "Error: %s\n", fcg_last_error(ctx));
     if (job >= 0) fcg_job_release(ctx, job);
-    fch_free(mesh);
+    r.mesh = std::shared_ptr<fch_mesh>(mesh, fch_free);
     return rc;
+}
+
+// CSolveCap::OutputCapMtxToFile (Solver/SolveCapacitance.cpp:835-911): <input
+// without extension>.csv, ", " between columns, 2D in F/m without the balance column
+void write_csv(const Options& o, const Result& r) {
+    std::string base = o.file;
+    const size_t dot = base.rfind('.');
+    if (dot != std::string::npos) base.resize(dot);
+    const std::string name = base + ".csv";
+    FILE* f = std::fopen(name.c_str(), "w");
+    if (!f) {
+        std::fprintf(stderr, "Warning: cannot open \"%s\" for writing, skipping\n", name.c_str());
+        return;
+    }
+    std::printf("Generating capacitance matrix file \"%s\"\n", name.c_str());
+    int N = r.cols;
+    if (r.dim == 2) {
+        if (N > 1)
+            --N;
+        else
+            std::fprintf(stderr, "Error: only one input conductor defined for a 2D problem. Result is meaningless.");
+    }
+    const double scale = r.dim == 2 ? kTwoPiE0 : 1.0;
+    for (int i = 0; i < r.rows; ++i) {
+        for (int j = 0; j < N; ++j) {
+            std::fprintf(f, "%g", r.cap[(size_t)i * r.cols + j] * scale);
+            if (j < N - 1) std::fprintf(f, ", ");
+        }
+        std::fprintf(f, "\n");
+    }
+    std::fclose(f);
+}
+
+// the file outputs of SolveMain after the last solve (Solver/SolveCapacitance.cpp:533-566)
+void write_outputs(const Options& o, const Result& r) {
+    if (o.out_csv) write_csv(o, r);
+    if (o.out_charge && r.mesh) {
+        std::printf("\n");
+        for (int i = 0; i < r.rows && i < (int)r.names.size(); ++i) {
+            std::printf("Outputting charge densities for conductor %s..\n", r.names[(size_t)i].c_str());
+            fch_write_fastcap(r.mesh.get(), o.file.c_str(), r.names[(size_t)i].c_str(),
+                              r.charges.data() + (size_t)i * (r.charges.size() / std::max<size_t>(r.names.size(), 1)));
+            std::printf("Done\n");
+        }
+    }
+    if (o.out_geo && r.mesh) {
+        std::printf("\nOutputting refined geometry in FastCap compatible format..\n");
+        fch_write_fastcap(r.mesh.get(), o.file.c_str(), "ref", nullptr);
+        std::printf("Done\n");
+    }
 }
 
 void print_cap(const Options& o, const Result& r) {
@@ -384,6 +451,7 @@ int main(int argc, char** argv) {
             if (rel <= o.auto_err) break;
         }
     }
+    if (rc == FCG_OK) write_outputs(o, r);
     const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     std::printf("Total time: %g s\n", secs);
     fcg_ctx_destroy(ctx);

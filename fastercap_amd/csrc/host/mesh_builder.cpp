@@ -1133,6 +1133,11 @@ struct fch_mesh {
     std::vector<int32_t> group_is_diel, group_leaf_start;
     std::vector<double> group_outperm, group_inperm, leaf_outperm;
     std::vector<std::string> group_names;
+    // for the FastCap-format writers: each group's outer media (conductors),
+    // dielectric reference point, and each leaf's medium index
+    std::vector<std::vector<std::pair<double, double>>> group_surf;
+    std::vector<V3> group_diel_ref;
+    std::vector<uint8_t> leaf_diel_index;
     fch_stats stats{};
     fcg_link_params link_params{};
 };
@@ -1280,13 +1285,17 @@ std::unique_ptr<fch_mesh> finish_build(Builder& b, bool links_on_device, double 
             m->group_outperm[gi] = g.outperm[0];
             m->group_inperm[gi] = g.inperm[0];
             m->group_names[gi] = g.name;
+            m->group_surf.push_back(g.surfOutperm);
+            m->group_diel_ref.push_back(g.dielRef);
             ls += g.leafNum;
         }
+        m->leaf_diel_index.assign((size_t)N, 0);
         m->group_leaf_start[G] = (int32_t)ls;
         for (int64_t i = 0; i < M; ++i) {
             Elem* e = b.nodes[(size_t)i];
             if (!e->isLeaf()) continue;
             Group& g = *b.groups[(size_t)e->group];
+            m->leaf_diel_index[(size_t)e->leafIndex] = e->dielIndex;
             if (!g.isDiel) m->leaf_outperm[(size_t)e->leafIndex] = g.surfOutperm[e->dielIndex].first;
         }
         fch_stats& s = m->stats;
@@ -1577,6 +1586,85 @@ int fch_get_stats(const fch_mesh* m, fch_stats* s) {
 const char* fch_group_name(const fch_mesh* m, int g) {
     if (!m || g < 0 || (size_t)g >= m->group_names.size()) return nullptr;
     return m->group_names[(size_t)g].c_str();
+}
+
+// CAutoRefine::OutputFastCapFile / OutputPanelFile / OutputPanelTree
+// (Solver/Autorefine.cpp:1230-1470): a list file <base>_<suffix>.lst naming one
+// panel file per conductor surface (outer medium) and per dielectric, each
+// listing its group's leaves in tree order; with charges, every panel line ends
+// with the leaf's charge density.
+int fch_write_fastcap(const fch_mesh* m, const char* filein, const char* suffix, const double* charges) {
+    if (!m || !filein || !suffix || m->pending) return FCG_INVALID_ARGUMENT;
+    constexpr double kZ2D = 1.0;                     // AUTOREFINE_2D_ZCOORD
+    constexpr double kTwoPiE0 = 5.563250279878e-11;  // TWO_PI_TIMES_E0, Solver/Potential.h:42
+    std::string base = filein;
+    const size_t dot = base.rfind('.');
+    if (dot != std::string::npos) base.resize(dot);
+    const std::string lst = base + "_" + suffix + ".lst";
+    FILE* fl = std::fopen(lst.c_str(), "w");
+    if (!fl) {
+        std::fprintf(stderr, "  Warning: cannot open \"%s\" for writing, skipping\n", lst.c_str());
+        return FCG_FILE_ERROR;
+    }
+    std::printf("  generating list file \"%s\"\n", lst.c_str());
+    std::fprintf(fl, "* Refined FastCap file derived from %s\n*\n* Total number of panels %lu\n*\n", filein,
+                 (unsigned long)m->leaf_node.size());
+    auto panel_file = [&](const std::string& name, size_t g, int diel_index) {
+        FILE* f = std::fopen(name.c_str(), "w");
+        if (!f) {
+            std::fprintf(stderr, "  Warning: cannot open \"%s\" for writing, skipping\n", name.c_str());
+            return;
+        }
+        std::fprintf(f, "0 Refined FastCap file derived from %s, referenced in list file %s\n*\n", filein,
+                     lst.c_str());
+        const char* cname = m->group_names[g].c_str();
+        for (int32_t k = m->group_leaf_start[g]; k < m->group_leaf_start[g + 1]; ++k) {
+            if (diel_index >= 0 && m->leaf_diel_index[(size_t)k] != diel_index) continue;
+            const fcg_node_geom& e = m->nodes[(size_t)m->leaf_node[(size_t)k]];
+            if (m->dim == 3) {
+                std::fprintf(f, "T %s  %g %g %g  %g %g %g  %g %g %g", cname, e.v[0], e.v[1], e.v[2], e.v[3], e.v[4],
+                             e.v[5], e.v[6], e.v[7], e.v[8]);
+            } else {
+                std::fprintf(f, "Q %s  %g %g %g %g %g %g %g %g %g %g %g %g", cname, e.v[0], e.v[1], 0.0, e.v[3],
+                             e.v[4], 0.0, e.v[3], e.v[4], kZ2D, e.v[0], e.v[1], kZ2D);
+            }
+            if (e.flags & FCG_NODE_OUTPERM_ELEMENT_LEVEL) {
+                // local dielectric reference point: centroid plus dielectric normal
+                const double rx = (double)e.c[0] + e.dn[0], ry = (double)e.c[1] + e.dn[1];
+                const double rz = m->dim == 3 ? (double)e.c[2] + e.dn[2] : kZ2D / 2.0;
+                std::fprintf(f, "  %g %g %g", rx, ry, rz);
+            }
+            if (charges) {
+                const double q = m->dim == 3 ? charges[k] : charges[k] * kTwoPiE0;
+                std::fprintf(f, "  %e\n", q / e.area);
+            } else {
+                std::fprintf(f, "\n");
+            }
+        }
+        std::fclose(f);
+    };
+    for (size_t g = 0; g < m->group_names.size(); ++g) {
+        std::string cf = base + "_" + m->group_names[g] + "_" + suffix;
+        // the reference strips a path up to the last backslash only
+        const size_t bs = cf.rfind('\\');
+        const std::string nopath = bs != std::string::npos ? cf.substr(bs + 1) : cf;
+        if (!m->group_is_diel[g]) {
+            const auto& surf = m->group_surf[g];
+            for (size_t i = 0; i < surf.size(); ++i) {
+                const std::string ext = "_" + std::to_string(i) + ".txt";
+                std::fprintf(fl, "C %s  %g  0.0 0.0 0.0%s\n", (nopath + ext).c_str(), surf[i].first,
+                             i + 1 == surf.size() ? "" : " +");
+                panel_file(cf + ext, g, (int)i);
+            }
+        } else {
+            const V3& r = m->group_diel_ref[g];
+            std::fprintf(fl, "D %s  %g %g  0.0 0.0 0.0  %g %g %g\n", (nopath + ".txt").c_str(), m->group_outperm[g],
+                         m->group_inperm[g], r.x, r.y, m->dim == 3 ? r.z : kZ2D / 2.0);
+            panel_file(cf + ".txt", g, -1);
+        }
+    }
+    std::fclose(fl);
+    return FCG_OK;
 }
 
 void fch_free(fch_mesh* m) { delete m; }

@@ -86,6 +86,15 @@ int fch_get_desc(const fch_mesh* mesh, fcg_job_desc* desc);
 
 int fch_get_stats(const fch_mesh* mesh, fch_stats* stats);
 
+/* Write the refined mesh in FastCap format, as the reference's -o (suffix
+ * "ref", charges NULL) and -c (suffix = a conductor's name, charges = that
+ * conductor's solve, [panels], raw; written as densities, 2D scaled by 2 pi e0):
+ * <filein without extension>_<suffix>.lst naming one panel file per conductor
+ * surface (outer medium) and per dielectric.  Replaces
+ * CAutoRefine::OutputFastCapFile, OutputPanelFile and OutputPanelTree
+ * (Solver/Autorefine.cpp:1230-1470). */
+int fch_write_fastcap(const fch_mesh* mesh, const char* filein, const char* suffix, const double* charges);
+
 /* Names of the groups in deque order (dielectrics first, as the reference orders them). */
 const char* fch_group_name(const fch_mesh* mesh, int group);
 

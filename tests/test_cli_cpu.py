@@ -1,4 +1,6 @@
 """Command-line parsing and matrix printout of the drop-in CLI (no GPU)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -108,3 +110,24 @@ def test_host_build_without_links_and_set_links(tmp_path):
     assert full.L == ref.L and full.stats.links == ref.L and full.stats.t_links_s == 0.5
     assert full.stats.max_level == 99
     np.testing.assert_array_equal(full.col, ref.col)
+
+
+def test_refined_geometry_file_rebuilds_the_refined_panels(tmp_path):
+    # fch_write_fastcap (-o) writes FastCap input: read back without refinement it
+    # gives the same panels (count, total area per conductor) as the mesh it came from
+    import ctypes as C
+
+    import fastercap_amd as fc
+    from fastercap_amd import _ffi, workloads
+
+    path = workloads.two_plates(str(tmp_path))
+    m = fc.Mesh.build(path, mesh_eps=0.05)
+    lib = _ffi.host_lib()
+    assert lib.fch_write_fastcap(C.c_void_p(m._h), path.encode(), b"ref", None) == 0
+    lst = path[: path.rfind(".")] + "_ref.lst"
+    assert os.path.exists(lst)
+    back = fc.Mesh.build(lst, mesh_eps=1e32)
+    assert back.N == m.N
+    area = lambda mm, g: sum(mm.node_geom()["area"][mm.leaf_node[mm.group_leaf_start[g]:mm.group_leaf_start[g + 1]]])
+    for g in range(m.G):
+        assert abs(area(back, g) / area(m, g) - 1) < 1e-6

@@ -64,3 +64,43 @@ def test_binary_2d_coated_coax(geo_dir):
 def test_binary_rejects_unsupported_switch(geo_dir):
     out = subprocess.run([BIN, "x.txt", "-g"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 64
+
+
+def _tri_area(v):
+    a, b, c = np.array(v[0:3]), np.array(v[3:6]), np.array(v[6:9])
+    return 0.5 * np.linalg.norm(np.cross(b - a, c - a))
+
+
+@pytest.mark.parametrize("python_cli", [False, True])
+def test_file_outputs_csv_geometry_charges(geo_dir, python_cli):
+    # -e writes the printed matrix to <input>.csv; -o the refined geometry; -c each
+    # conductor solve's charge densities, whose area integral over conductor j's
+    # panels is the matrix entry (row of the solve, column j; outer permittivity 1)
+    d = os.path.join(geo_dir, f"cli_out_{int(python_cli)}")
+    os.makedirs(d, exist_ok=True)
+    path = two_cubes(d)
+    args = [path, "-m0.05", "-t1e-8", "-e", "-o", "-c"]
+    if python_cli:
+        buf = io.StringIO()
+        cli.run(cli.parse_args(args), out=buf)
+        text = buf.getvalue()
+    else:
+        r = subprocess.run([BIN] + args, capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        text = r.stdout
+    names, cap = _last_matrix(text)
+    base = path[: path.rfind(".")]
+    csv = np.array([[float(x) for x in ln.split(",")] for ln in open(base + ".csv").read().splitlines()])
+    np.testing.assert_allclose(csv, cap, rtol=1e-5)
+    assert os.path.exists(base + "_ref.lst")
+    for i, name in enumerate(names):
+        lst = base + "_" + name + ".lst"
+        assert os.path.exists(lst), lst
+        for j, other in enumerate(names):
+            total = 0.0
+            for ln in open(base + "_" + other + "_" + name + "_0.txt"):
+                if not ln.startswith("T "):
+                    continue
+                f = ln.split()
+                total += float(f[-1]) * _tri_area([float(x) for x in f[2:11]])
+            assert abs(total - cap[i, j]) <= 1e-5 * abs(cap[i, i]), (name, other, total, cap[i, j])
