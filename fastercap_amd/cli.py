@@ -175,7 +175,56 @@ def preconditioner(a: Args, mesh):
     return a.precond, (a.block_size if a.precond == 2 else a.super_dim)
 
 
-def solve_file(dev, a: Args, mesh_eps: float, tol: float):
+def format_stats(mesh) -> str:
+    """CSolveCap::OutputSolveStats (Solver/SolveCapacitance.cpp:684-721)."""
+    st = mesh.stats
+    n, links = int(st.panels), int(st.links) + int(st.panels)
+    ratio = np.float32(100.0) - np.float32(links) / np.float32(n * n) * 100.0
+    return ("\nSolve statistics:\n"
+            f"Number of input panels: {st.input_raw_cond_panels + st.input_raw_diel_panels} of which "
+            f"{st.input_raw_cond_panels} conductors and {st.input_raw_diel_panels} dielectric\n"
+            f"Number of input panels to solver engine: {st.input_panels}\n"
+            f"Number of panels after refinement: {n}\n"
+            f"Number of potential estimates: {int(st.links) // 2}\n"
+            f"Number of links: {links} (uncompressed {n * n}, compression ratio is {ratio:.1f}%)\n"
+            f"Max recursion level: {st.max_level}\n"
+            f"Max Mesh relative refinement value: {st.max_mesh_eps:g}\n")
+
+
+def format_params(a: Args, tol: float) -> str:
+    """CSolveCap::OutputSolveParams (Solver/SolveCapacitance.cpp:610-650)."""
+    lines = []
+    if a.out_geo:
+        lines.append("Output refined geometry in FastCap compatible format (-o)")
+    if a.out_charge:
+        lines.append("Output charge densities information (-c)")
+    if a.out_csv:
+        lines.append("Output capacitance matrix to file (-e)")
+    lines += [f"Solution scheme (-g): Collocation, GMRES tolerance (-t): {tol:g}",
+              "Out-of-core free memory to link memory condition (-f): 5",
+              f"Potential interaction coefficient to mesh refinement ratio (-d): {a.eps_ratio:g}",
+              f"Mesh curvature (-mc): {a.curv_coeff:g}"]
+    return "\n".join(lines) + "\n"
+
+
+def _precond_text(precond: int, size: int) -> str:
+    return {0: "None", 1: "Jacobi ", 2: f"Block, block preconditioner dimension (-pb): {size} ",
+            3: f"Two-levels, two-levels preconditioner dimension (-ps): {size} "}[precond]
+
+
+def _report_solve(out, a: Args, mesh, res):
+    """The link, preconditioner and GMRES lines of one solve (SolveComputeLinks,
+    OutputSolvePrecondType, gmresPrecondSFastAll's iteration line)."""
+    precond, size = preconditioner(a, mesh)
+    out.write("***************************************\nComputing the links.. \n"
+              f"Number of panels after refinement: {mesh.N}\nNumber of links to be computed: {mesh.L}\n"
+              "Done computing links\n***************************************\n"
+              f"Precond Type(s) (-p): {_precond_text(precond, size)}\n")
+    for it in res.iterations:
+        out.write("GMRES Iteration: " + "".join(f"{i} " for i in range(it)) + "\n")
+
+
+def solve_file(dev, a: Args, mesh_eps: float, tol: float, out=None):
     from . import Mesh
 
     mesh = Mesh.build(a.file, mesh_eps=mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff,
@@ -186,6 +235,8 @@ def solve_file(dev, a: Args, mesh_eps: float, tol: float):
         precond, size = preconditioner(a, mesh)
         res = dev.solve(job, mesh, tol=tol, precond=precond, precision=a.precision, precond_size=size,
                         want_charges=a.out_charge)
+        if out is not None:
+            _report_solve(out, a, mesh, res)
     finally:
         dev.release(job)
     return mesh, res
@@ -237,27 +288,43 @@ def run(a: Args, out=sys.stdout):
     dev = Device(a.device)
     try:
         if not a.auto:
-            mesh, res = solve_file(dev, a, a.mesh_eps, a.tol)
+            out.write("\n" + format_params(a, a.tol))
+            out.write("***************************************\nRefining the geometry.. \n"
+                      f"Refinement completed\nMesh refinement (-m): {a.mesh_eps:g}\n")
+            mesh, res = solve_file(dev, a, a.mesh_eps, a.tol, out)
             out.write(format_cap(res.cap, mesh.conductor_names, mesh.dim, a.verbose))
+            out.write(format_stats(mesh))
             write_outputs(a, mesh, res, out)
             return mesh, res
         tol = a.auto_err / 2.0
-        out.write(f"Auto calculation with max error: {a.auto_err:g}\n\nIteration number #0\n")
-        mesh, res = solve_file(dev, a, 1e32, tol)
+        out.write(f"Auto calculation with max error: {a.auto_err:g}\n"
+                  "Remark: Auto option overrides all other Manual settings\n\n" + format_params(a, tol))
+        out.write("\nIteration number #0 ***************************\n\n"
+                  "***************************************\nRefining the geometry.. \n"
+                  f"Refinement completed\nMesh refinement (-m): {1e32:g}\n")
+        mesh, res = solve_file(dev, a, 1e32, tol, out)
         out.write(format_cap(res.cap, mesh.conductor_names, mesh.dim, a.verbose))
+        out.write(format_stats(mesh))
         old_p, old_l = mesh.N, mesh.L
         for i in range(1, MAX_AUTO_ITERATIONS):
-            out.write(f"\nIteration number #{i}\n")
+            out.write(f"\nIteration number #{i} ***************************\n\n"
+                      "***************************************\nIncreasing the geometric refinement.. \n")
             m = mesh.stats.max_mesh_eps
             for _ in range(MAX_AUTO_ITERATIONS):
                 m = m / math.sqrt(2.0)
                 nmesh, nres = solve_file(dev, a, m, tol)
                 if nmesh.L > old_l * AUTO_INCREMENT and nmesh.N > old_p * AUTO_INCREMENT:
                     break
+                out.write(f"Delta in refined panel and link count w.r.t. previous iteration less than "
+                          f"{int(round((AUTO_INCREMENT - 1.0) * 100.0))}%\n(Panels # {nmesh.N}, Links # {nmesh.L})\n"
+                          "Automatically increasing the refinement parameters\n")
                 m = nmesh.stats.max_mesh_eps
+            out.write(f"Refinement completed\nMesh refinement (-m): {m:g}\n")
+            _report_solve(out, a, nmesh, nres)
             diff = np.linalg.norm(nres.cap - res.cap) / np.linalg.norm(nres.cap)
             out.write(format_cap(nres.cap, nmesh.conductor_names, nmesh.dim, a.verbose))
             out.write(f"Weighted Frobenius norm of the difference between capacitance (auto option): {diff:g}\n")
+            out.write(format_stats(nmesh))
             mesh, res = nmesh, nres
             old_p, old_l = mesh.N, mesh.L
             if diff <= a.auto_err:

@@ -186,6 +186,7 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
 
 struct Result {
     std::shared_ptr<fch_mesh> mesh;  // kept for the -o / -c writers
+    double mesh_eps = 0.0;
     std::vector<double> charges;     // [rows][N] with -c
     std::vector<double> cap;
     int rows = 0, cols = 0, dim = 3;
@@ -194,8 +195,9 @@ struct Result {
     fcg_solve_stats ss{};
 };
 
-// one refinement + fill + solve at the given mesh eps
-int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Result& r) {
+// refinement and links at the given mesh eps (RefineGeoAndCountLinks +
+// SolveComputeLinks); the mesh is kept in r
+int build_mesh(fcg_ctx* ctx, const Options& o, double mesh_eps, Result& r) {
     fch_options ho;
     fch_default_options(&ho);
     ho.mesh_eps = mesh_eps;
@@ -258,6 +260,36 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
     r.names.clear();
     for (int g = 0; g < d.n_groups; ++g)
         if (!d.group_is_diel[g]) r.names.push_back(fch_group_name(mesh, g));
+    r.mesh = std::shared_ptr<fch_mesh>(mesh, fch_free);
+    r.mesh_eps = mesh_eps;
+    return FCG_OK;
+}
+
+const char* precond_line(int precond, int size) {
+    static char buf[160];
+    if (precond == FCG_PRECOND_NONE)
+        std::snprintf(buf, sizeof buf, "None");
+    else if (precond == FCG_PRECOND_JACOBI)
+        std::snprintf(buf, sizeof buf, "Jacobi ");
+    else if (precond == FCG_PRECOND_BLOCK)
+        std::snprintf(buf, sizeof buf, "Block, block preconditioner dimension (-pb): %d ", size);
+    else
+        std::snprintf(buf, sizeof buf, "Two-levels, two-levels preconditioner dimension (-ps): %d ", size);
+    return buf;
+}
+
+// upload, fill and solve of the mesh in r (SolveForCapacitance), with the
+// reference's link and preconditioner banners and per-solve GMRES lines
+int solve_mesh(fcg_ctx* ctx, const Options& o, double tol, Result& r) {
+    fch_mesh* mesh = r.mesh.get();
+    fcg_job_desc d;
+    fch_get_desc(mesh, &d);
+    int rc = FCG_OK;
+    // SolveComputeLinks (Solver/SolveCapacitance.cpp:1025-1041)
+    std::printf("***************************************\nComputing the links.. \n");
+    std::printf("Number of panels after refinement: %lld\n", (long long)r.hs.panels);
+    std::printf("Number of links to be computed: %lld\n", (long long)r.hs.links);
+    std::printf("Done computing links\n***************************************\n");
     int precond = o.precond, size = o.precond == FCG_PRECOND_BLOCK ? o.block_size : o.super_dim;
     if (o.auto_precond) {
         // CSolveCap::AutoSetPrecondType (Solver/SolveCapacitance.cpp:580-608):
@@ -272,6 +304,7 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
                 std::printf("Automatic preconditioner: two-levels, dimension %d\n", size);
         }
     }
+    std::printf("Precond Type(s) (-p): %s\n", precond_line(precond, size));
     int job = -1;
     rc = o.host_links ? fcg_job_upload(ctx, &d, &job) : fcg_job_upload_links(ctx, &d, &job);
     if (rc == FCG_OK) rc = fcg_fill(ctx, job, o.precision);
@@ -283,9 +316,14 @@ int solve_once(fcg_ctx* ctx, const Options& o, double mesh_eps, double tol, Resu
         rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, o.out_charge ? r.charges.data() : nullptr,
                        &r.ss);
     }
+    if (rc == FCG_OK)
+        for (int q = 0; q < r.rows && q < 64; ++q) {
+            std::printf("GMRES Iteration: ");
+            for (int i = 0; i < r.ss.iterations[q]; ++i) std::printf("%d ", i);
+            std::printf("\n");
+        }
     if (rc != FCG_OK) std::fprintf(stderr, "Error: %s\n", fcg_last_error(ctx));
     if (job >= 0) fcg_job_release(ctx, job);
-    r.mesh = std::shared_ptr<fch_mesh>(mesh, fch_free);
     return rc;
 }
 
@@ -370,14 +408,41 @@ void print_cap(const Options& o, const Result& r) {
     std::printf("\n");
 }
 
+// CSolveCap::OutputSolveStats (Solver/SolveCapacitance.cpp:684-721)
 void print_stats(const Result& r) {
+    const long long N = r.hs.panels, L = r.hs.links + r.hs.panels;
+    std::printf("\nSolve statistics:\n");
     std::printf("Number of input panels: %lld of which %lld conductors and %lld dielectric\n",
-                (long long)r.hs.input_panels, (long long)r.hs.input_raw_cond_panels,
-                (long long)r.hs.input_raw_diel_panels);
-    std::printf("Number of panels after refinement: %lld\n", (long long)r.hs.panels);
-    std::printf("Number of links: %lld\n", (long long)(r.hs.links + r.hs.panels));
+                (long long)(r.hs.input_raw_cond_panels + r.hs.input_raw_diel_panels),
+                (long long)r.hs.input_raw_cond_panels, (long long)r.hs.input_raw_diel_panels);
+    std::printf("Number of input panels to solver engine: %lld\n", (long long)r.hs.input_panels);
+    std::printf("Number of panels after refinement: %lld\n", N);
+    std::printf("Number of potential estimates: %lld\n", (long long)(r.hs.links / 2));
+    std::printf("Number of links: %lld (uncompressed %lld, compression ratio is %.1f%%)\n", L, N * N,
+                100.0f - ((float)L) / ((float)(N * N)) * 100.0);
+    std::printf("Max recursion level: %d\n", r.hs.max_level);
     std::printf("Max Mesh relative refinement value: %g\n", r.hs.max_mesh_eps);
-    std::printf("Max GMRES iterations: %d\n", r.ss.max_iterations);
+}
+
+// CSolveCap::PrintTime (Solver/SolveCapacitance.cpp)
+void print_time(double t) {
+    long sec = (long)t, min = sec / 60, hr = min / 60;
+    const long days = hr / 24;
+    sec -= min * 60;
+    min -= hr * 60;
+    hr -= days * 24;
+    std::printf("%fs (%ld days, %ld hours, %ld mins, %ld s)\n", t, days, hr, min, sec);
+}
+
+// CSolveCap::OutputSolveParams (Solver/SolveCapacitance.cpp:610-650)
+void print_params(const Options& o, double tol) {
+    if (o.out_geo) std::printf("Output refined geometry in FastCap compatible format (-o)\n");
+    if (o.out_charge) std::printf("Output charge densities information (-c)\n");
+    if (o.out_csv) std::printf("Output capacitance matrix to file (-e)\n");
+    std::printf("Solution scheme (-g): Collocation, GMRES tolerance (-t): %g\n", tol);
+    std::printf("Out-of-core free memory to link memory condition (-f): %g\n", 5.0);
+    std::printf("Potential interaction coefficient to mesh refinement ratio (-d): %g\n", o.eps_ratio);
+    std::printf("Mesh curvature (-mc): %g\n", o.curv);
 }
 
 double frob(const std::vector<double>& a) {
@@ -405,46 +470,76 @@ int main(int argc, char** argv) {
         return FCG_DEVICE_ERROR;
     }
     const auto t0 = std::chrono::steady_clock::now();
+    using clk = std::chrono::steady_clock;
+    auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
     Result r;
     if (!o.autom) {
-        rc = solve_once(ctx, o, o.mesh_eps, o.tol, r);
+        // manual run (Solver/SolveCapacitance.cpp:479-531)
+        std::printf("\n");
+        print_params(o, o.tol);
+        std::printf("***************************************\nRefining the geometry.. \n");
+        rc = build_mesh(ctx, o, o.mesh_eps, r);
         if (rc == FCG_OK) {
-            print_stats(r);
+            std::printf("Refinement completed\nMesh refinement (-m): %g\n", o.mesh_eps);
+            rc = solve_mesh(ctx, o, o.tol, r);
+        }
+        if (rc == FCG_OK) {
             print_cap(o, r);
+            print_stats(r);
         }
     } else {
-        // Solver/SolveCapacitance.cpp:294-477
+        // -a run (Solver/SolveCapacitance.cpp:294-477)
         std::printf("Auto calculation with max error: %g\n", o.auto_err);
+        std::printf("Remark: Auto option overrides all other Manual settings\n");
         const double tol = o.auto_err / 2.0;
+        std::printf("\n");
+        print_params(o, tol);
         std::printf("\nIteration number #0 ***************************\n\n");
-        rc = solve_once(ctx, o, 1e32, tol, r);
+        auto ti = clk::now();
+        std::printf("***************************************\nRefining the geometry.. \n");
+        rc = build_mesh(ctx, o, 1e32, r);
         if (rc == FCG_OK) {
-            print_stats(r);
+            std::printf("Refinement completed\nMesh refinement (-m): %g\n", 1e32);
+            rc = solve_mesh(ctx, o, tol, r);
+        }
+        if (rc == FCG_OK) {
             print_cap(o, r);
+            print_stats(r);
+            std::printf("Iteration time: ");
+            print_time(since(ti));
         }
         // CAutoRefine::GetLinksNum: mutual links only
         long long old_panels = r.hs.panels, old_links = r.hs.links;
         for (int i = 1; rc == FCG_OK && i < kMaxAutoIterations; ++i) {
             std::printf("\nIteration number #%d ***************************\n\n", i);
+            ti = clk::now();
+            std::printf("***************************************\nIncreasing the geometric refinement.. \n");
             Result nr;
             double m = r.hs.max_mesh_eps;
             for (int j = 0; j < kMaxAutoIterations; ++j) {
-                m = m / std::sqrt(2.0);
-                rc = solve_once(ctx, o, m, tol, nr);
+                m = (j == 0 ? r.hs.max_mesh_eps : nr.hs.max_mesh_eps) / std::sqrt(2.0);
+                rc = build_mesh(ctx, o, m, nr);
                 if (rc != FCG_OK) break;
                 const long long np = nr.hs.panels, nl = nr.hs.links;
                 if (nl > old_links * kAutoIncrement && np > old_panels * kAutoIncrement) break;
                 std::printf("Delta in refined panel and link count w.r.t. previous iteration less than %d%%\n",
                             (int)((kAutoIncrement - 1.0) * 100.0));
-                m = nr.hs.max_mesh_eps;
+                std::printf("(Panels # %lld, Links # %lld)\n", np, nl);
+                if (o.verbose) std::printf("(Mesh relative refinement value (-m): %g)\n", m);
+                std::printf("Automatically increasing the refinement parameters\n");
             }
+            if (rc != FCG_OK) break;
+            std::printf("Refinement completed\nMesh refinement (-m): %g\n", m);
+            rc = solve_mesh(ctx, o, tol, nr);
             if (rc != FCG_OK) break;
             std::vector<double> diff(nr.cap.size());
             for (size_t q = 0; q < diff.size(); ++q) diff[q] = nr.cap[q] - r.cap[q];
             const double rel = frob(diff) / frob(nr.cap);
-            print_stats(nr);
             print_cap(o, nr);
             std::printf("Weighted Frobenius norm of the difference between capacitance (auto option): %g\n", rel);
+            print_stats(nr);
+            std::printf("Iteration time: ");
+            print_time(since(ti));
             old_panels = nr.hs.panels;
             old_links = nr.hs.links;
             r = nr;
@@ -452,8 +547,8 @@ int main(int argc, char** argv) {
         }
     }
     if (rc == FCG_OK) write_outputs(o, r);
-    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    std::printf("Total time: %g s\n", secs);
+    std::printf("\nTotal time: ");
+    print_time(since(t0));
     fcg_ctx_destroy(ctx);
     return rc;
 }

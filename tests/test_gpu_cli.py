@@ -36,6 +36,10 @@ def test_binary_auto_two_cubes(geo_dir):
     names, cap = _last_matrix(out.stdout)
     assert names == ["g1_cube.txt", "g2_cube.txt"] or len(names) == 2
     assert out.stdout.count("Iteration number #") == len(gold["iterations"])
+    # every iteration's "Solve statistics" block as the reference help prints it
+    lines = out.stdout.splitlines()
+    blocks = [[m for m in lines[i + 1:i + 8]] for i, ln in enumerate(lines) if ln == "Solve statistics:"]
+    assert [b for b in blocks] == [it["stats_lines"] for it in gold["iterations"]]
     np.testing.assert_allclose(cap, np.array(gold["iterations"][-1]["cap"]), rtol=1e-5)
 
 
