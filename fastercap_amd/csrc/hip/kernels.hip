@@ -34,6 +34,21 @@ __device__ __forceinline__ int find_job(const int64_t* __restrict__ off, int njo
     return lo;
 }
 
+// find_job for a block of consecutive work items starting at g0 (every thread
+// of the block must call it): one search per block, staged in LDS, then a few
+// steps forward per thread; a block that spans many jobs falls back to the search
+__device__ __forceinline__ int find_job_block(const int64_t* __restrict__ off, int njobs, int64_t g0, int64_t g) {
+    __shared__ int jb0;
+    if (threadIdx.x == 0) jb0 = find_job(off, njobs, g0);
+    __syncthreads();
+    int jb = jb0;
+    for (int step = 0; step < 4; ++step) {
+        if (jb + 1 >= njobs || off[jb + 1] > g) return jb;
+        ++jb;
+    }
+    return find_job(off, njobs, g);
+}
+
 __device__ __forceinline__ DD dd_two_sum(double a, double b) {
     const double s = a + b;
     const double bb = s - a;
@@ -135,9 +150,10 @@ __device__ __forceinline__ T link_coef(const fcg_node_geom& p1, const fcg_node_g
 // all branches alike.
 template <int DIM, bool DIEL, int BR, bool MIXED>
 __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __restrict__ off, int64_t total, int cls) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t g = g0 + threadIdx.x;
+    const int jb = find_job_block(off, bv.njobs, g0, g < total ? g : total - 1);
     if (g >= total) return;
-    const int jb = find_job(off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int64_t l = J.fill_order[J.fill_start[cls] + (g - off[jb])];
     const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
@@ -290,9 +306,9 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_final(BatchView bv, int j
 // padding columns k..kp-1 are zero so that vector loads of a row stay finite
 template <typename TQ>
 __global__ void k_node_charge(BatchView bv, int jacobi) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    const int jb = find_job_block(bv.node_off, bv.njobs, g0, g < bv.nodes ? g : bv.nodes - 1);
     if (g >= bv.nodes) return;
-    const int jb = find_job(bv.node_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int i = (int)(g - bv.node_off[jb]);
     const int lo = J.leaf_lo[i], hi = J.leaf_hi[i];
@@ -341,8 +357,10 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
     const int lane = threadIdx.x & 63;
     // global row index over the batch; a lane group never straddles two rows
     const int64_t grow = row_lo + (gb * kGatherWaves + (threadIdx.x >> 6)) * RPW + lane / G;
+    const int64_t grow0 = row_lo + gb * kGatherWaves * RPW;
+    const int jb = find_job_block(bv.node_off, bv.njobs, grow0 < row_hi ? grow0 : row_hi - 1,
+                                  grow < row_hi ? grow : row_hi - 1);
     if (grow >= row_hi) return;
-    const int jb = find_job(bv.node_off, bv.njobs, grow);
     const BJob& J = bv.jobs[jb];
     const int row = J.rorder[grow - bv.node_off[jb]];
     const int gl = lane % G;
@@ -500,9 +518,9 @@ static hipError_t gather_dispatch(const BatchView& bv, const GatherSeg& sg, hipS
 // ------------------------------------------------------------------ down-pass
 template <typename TQ, int KC>
 __global__ void k_down(BatchView bv) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
     if (g >= bv.leaves) return;
-    const int jb = find_job(bv.leaf_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int leaf = (int)(g - bv.leaf_off[jb]);
     const TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.pot64) : reinterpret_cast<const TQ*>(J.pot32);
@@ -539,9 +557,9 @@ __global__ void k_2d_last(BatchView bv) {
 }
 
 __global__ void k_2d_fix(BatchView bv) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
     if (g >= bv.leaves) return;
-    const int jb = find_job(bv.leaf_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     if (J.dim != 2) return;
     const int n = (int)(g - bv.leaf_off[jb]);
