@@ -194,9 +194,9 @@ __device__ double node_self(const BJob& J, const fcg_node_geom& nd) {
 }
 
 __global__ void k_self(BatchView bv) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
     if (g >= bv.leaves) return;
-    const int jb = find_job(bv.leaf_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int k = (int)(g - bv.leaf_off[jb]);
     const double s = node_self(J, J.nodes[J.leaf_node[k]]);
