@@ -58,8 +58,9 @@ def ncond(mesh):
     return int((mesh.group_is_diel == 0).sum())
 
 
-def cpu_sample(meshes, idx, tol):
-    """Oracle fill + full GMRES of the sampled jobs on all host cores."""
+def cpu_sample(meshes, idx, tol, super_dim=0):
+    """Oracle fill + full GMRES of the sampled jobs on all host cores (Jacobi, or
+    the two-level preconditioner of dimension super_dim when it is positive)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as orc
 
@@ -67,7 +68,10 @@ def cpu_sample(meshes, idx, tol):
     a = time.perf_counter()
     for i in idx:
         co, so = orc.fill(meshes[i], nthreads=ncores)
-        orc.solve(meshes[i], co, so, tol=tol, nthreads=ncores)
+        if super_dim > 0:
+            orc.solve_super(meshes[i], co, so, super_dim, tol=tol, nthreads=ncores)
+        else:
+            orc.solve(meshes[i], co, so, tol=tol, nthreads=ncores)
     return time.perf_counter() - a, ncores
 
 
@@ -87,6 +91,11 @@ def bench_batch(args, fc, dev, tmp):
     log(f"uploaded: panels {sum(Ns)} (min {min(Ns)} max {max(Ns)}), links {sum(Ls)}")
     MIX, F64 = fc.FCG_PREC_MIXED, fc.FCG_PREC_FP64
     tol = args.tol
+    # preconditioner of every solve of this workload; a fill drops the two-level
+    # inverse, so each step's solve builds it again (one launch for the batch)
+    two = args.precond == "two-level"
+    PC = dict(precond=fc.FCG_PRECOND_SUPER if two else fc.FCG_PRECOND_JACOBI,
+              precond_size=args.precond_size if two else 0)
 
     # products over several jobs use the context's batch gather shape; the default
     # shape is what fcg_autotune_batch picks for this batch, so the timed run makes
@@ -97,7 +106,7 @@ def bench_batch(args, fc, dev, tmp):
         dev.profile(enable=1, reset=True)
     for _ in range(max(1, args.warmup)):
         dev.fill_batch(jobs, MIX)
-        dev.solve_batch(jobs, meshes, tol=tol, precision=MIX)
+        dev.solve_batch(jobs, meshes, tol=tol, precision=MIX, **PC)
     log("warmup done")
     lat = []
     dev.profile(enable=1, reset=not args.only_mixed)
@@ -106,7 +115,7 @@ def bench_batch(args, fc, dev, tmp):
         a = time.perf_counter()
         dev.fill_batch(jobs, MIX)
         b = time.perf_counter()
-        res = dev.solve_batch(jobs, meshes, tol=tol, precision=MIX)
+        res = dev.solve_batch(jobs, meshes, tol=tol, precision=MIX, **PC)
         c = time.perf_counter()
         lat = [r.t_solve_s for r in res]
         log(f"step {s}: fill {1e3 * (b - a):.1f} ms, batched mixed solve {1e3 * (c - b):.1f} ms")
@@ -122,22 +131,22 @@ def bench_batch(args, fc, dev, tmp):
     # plain FP64 path: one job at a time (fill + solve), as the reference runs
     dev.fill_batch(jobs, F64)
     for j, m in zip(jobs, meshes):  # warm
-        dev.solve(j, m, tol=tol)
+        dev.solve(j, m, tol=tol, **PC)
     a = time.perf_counter()
     plain = []
     for j, m in zip(jobs, meshes):
         dev.fill(j, F64)
-        plain.append(dev.solve(j, m, tol=tol))
+        plain.append(dev.solve(j, m, tol=tol, **PC))
     t_plain = time.perf_counter() - a
     log(f"plain FP64, one job at a time: {t_plain:.2f}s ({nj / t_plain:.1f} solves/s)")
     tuned["fp64"] = dev.autotune_batch(jobs, fc.FCG_MVP_FP64)
-    dev.solve_batch(jobs, meshes, tol=tol, precision=F64)  # warm
+    dev.solve_batch(jobs, meshes, tol=tol, precision=F64, **PC)  # warm
     a = time.perf_counter()
     dev.fill_batch(jobs, F64)
-    bres = dev.solve_batch(jobs, meshes, tol=tol, precision=F64)
+    bres = dev.solve_batch(jobs, meshes, tol=tol, precision=F64, **PC)
     t_b64 = time.perf_counter() - a
     log(f"batched FP64: {t_b64:.2f}s ({nj / t_b64:.1f} solves/s)")
-    ref = dev.solve_batch(jobs, meshes, tol=1e-8, precision=F64)
+    ref = dev.solve_batch(jobs, meshes, tol=1e-8, precision=F64, **PC)
     e_conv = c_errors(mixed_caps, [r.cap for r in ref])
     e_plain = c_errors(mixed_caps, [r.cap for r in plain])
     e_plain_conv = c_errors([r.cap for r in plain], [r.cap for r in ref])
@@ -161,17 +170,18 @@ def bench_batch(args, fc, dev, tmp):
             traffic = ratio * g["gather_bytes"] / max(1, g["gather_launches"])
     log(f"timed-region products: {prof}")
 
+    pc_name = f"two-level, dim {PC['precond_size']}" if two else "Jacobi"
     cpu = None
     if not args.no_cpu:
         stride = max(1, nj // args.cpu_jobs)
         idx = list(range(0, nj, stride))
-        t_cpu, ncores = cpu_sample(meshes, idx, tol)
+        t_cpu, ncores = cpu_sample(meshes, idx, tol, PC["precond_size"])
         cpu = {
             "value": len(idx) / t_cpu,
             "unit": "solves/s",
             "cores": ncores,
             "kind": "port",
-            "sample": f"oracle fill + FP64 GMRES of every conductor of every {stride}th job "
+            "sample": f"oracle fill + FP64 GMRES ({pc_name}) of every conductor of every {stride}th job "
                       f"({len(idx)} of {nj} jobs, {t_cpu:.1f}s)",
         }
         log(f"cpu baseline: {cpu}")
@@ -191,6 +201,8 @@ def bench_batch(args, fc, dev, tmp):
             "panels_max": int(max(Ns)),
             "links_total": int(sum(Ls)),
             "gmres_tol": tol,
+            "preconditioner": pc_name + (" (FasterCap -ps%d), inverse built again after each step's fill" % PC["precond_size"]
+                                         if two else " (FasterCap default)"),
             "precision": "mixed: FP32 far-field links and inner products, FP64 near field, Arnoldi, refinement",
         },
         "latency_ms": {"p50": 1e3 * pct(lat, 50), "p99": 1e3 * pct(lat, 99)},
@@ -421,6 +433,9 @@ def main():
     ap.add_argument("--serve-load", type=float, default=1.5, help="arrival rate / closed-batch capacity")
     ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="single-job workloads")
+    ap.add_argument("--precond", default="two-level", choices=["two-level", "jacobi"],
+                    help="batch: preconditioner of every solve")
+    ap.add_argument("--precond-size", type=int, default=128, help="batch: two-level dimension")
     ap.add_argument("--mesh-eps", type=float, default=None)
     ap.add_argument("--tol", type=float, default=None)
     ap.add_argument("--n2d", type=int, default=1024)

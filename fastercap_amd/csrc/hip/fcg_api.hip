@@ -234,7 +234,8 @@ struct Job {
     } bpre;
     // two-level preconditioner of the last dimension asked for (0: none yet)
     struct SuperPre {
-        int dim = 0, m = 0;
+        int dim = 0, m = 0;  // dim: Sinv valid for this dimension (0: to be built)
+        int part_dim = 0;    // dimension of the partition, snode .. w
         bool avail = false;
         DevBuf snode, sstart, sidx, Sinv, w;
     } spre;
@@ -899,17 +900,18 @@ bool super_partition(const Job& j, int dim, std::vector<int32_t>& nodes, std::ve
 
 int clamp_super_dim(int d) { return d <= 0 ? 128 : std::min(std::max(d, 16), 4096); }
 
-// the job's two-level preconditioner of dimension dim (built on first use; needs
-// a fill)
-int ensure_super_precond(fcg_ctx* ctx, Job& j, int dim) {
+// Structure of the job's two-level preconditioner of dimension dim: the element
+// partition, its device arrays and the leaf weights.  These depend on the tree
+// alone and are kept until another dimension is asked for.
+int super_structure(fcg_ctx* ctx, Job& j, int dim) {
     dim = clamp_super_dim(dim);
-    if (j.spre.dim == dim) return FCG_OK;
-    if (!j.filled64 && !j.filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    if (j.spre.part_dim == dim) return FCG_OK;
     hipStream_t s = ctx->stream;
     std::vector<int32_t> nodes, starts;
     const bool avail = super_partition(j, dim, nodes, starts);
     HIPCHECK(ctx, hipStreamSynchronize(s));  // the buffers below may be in use; the table arena is reset
     j.spre.dim = 0;
+    j.spre.part_dim = 0;
     j.spre.avail = avail;
     j.spre.m = (int)nodes.size();
     if (avail) {
@@ -922,17 +924,71 @@ int ensure_super_precond(fcg_ctx* ctx, Job& j, int dim) {
         HIPCHECK(ctx, upload(j.spre.sidx, sidx.data(), sidx.size(), s));
         HIPCHECK(ctx, j.spre.Sinv.alloc(sizeof(double) * (size_t)m * m));
         HIPCHECK(ctx, j.spre.w.alloc(sizeof(double) * (size_t)j.N));
-        DevBuf A;
-        HIPCHECK(ctx, A.alloc(sizeof(double) * (size_t)m * m));
         ctx->tabs.reset();
         BatchView bv;
         RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, nullptr, nullptr}}, &bv));
-        HIPCHECK(ctx, fcg::launch_super_setup(bv, m, j.N, j.spre.snode.as<int32_t>(), j.spre.sidx.as<int32_t>(),
-                                              A.as<double>(), j.spre.Sinv.as<double>(),
-                                              j.spre.w.as<double>(), s));
-        HIPCHECK(ctx, hipStreamSynchronize(s));  // A and the host vectors are released below
+        HIPCHECK(ctx, fcg::launch_super_weights(bv, j.N, j.spre.snode.as<int32_t>(), j.spre.sidx.as<int32_t>(),
+                                                j.spre.w.as<double>(), s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));  // the host vectors are released below
+    }
+    j.spre.part_dim = dim;
+    return FCG_OK;
+}
+
+// The job's two-level preconditioner of dimension dim, its inverse included
+// (built on first use after each fill; needs a fill).  At most kSuperLds
+// elements: one workgroup in LDS; more: Gauss-Jordan over the whole device.
+int ensure_super_precond(fcg_ctx* ctx, Job& j, int dim) {
+    dim = clamp_super_dim(dim);
+    if (!j.filled64 && !j.filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
+    RCCHECK(super_structure(ctx, j, dim));
+    if (j.spre.dim == dim) return FCG_OK;
+    if (j.spre.avail) {
+        hipStream_t s = ctx->stream;
+        const int m = j.spre.m;
+        HIPCHECK(ctx, hipStreamSynchronize(s));  // the table arena is reset
+        ctx->tabs.reset();
+        BatchView bv;
+        RCCHECK(make_batch(ctx, ctx->tabs, {BatchEntry{&j, &j.scr, nullptr, nullptr}}, &bv));
+        if (m <= fcg::kSuperLds) {
+            const fcg::SSetup st{j.spre.snode.as<int32_t>(), j.spre.Sinv.as<double>(), m};
+            void* d;
+            HIPCHECK(ctx, ctx->tabs.push(&st, sizeof st, s, &d));
+            HIPCHECK(ctx, fcg::launch_super_invert_lds(bv, reinterpret_cast<const fcg::SSetup*>(d), 1, s));
+        } else {
+            DevBuf A;
+            HIPCHECK(ctx, A.alloc(sizeof(double) * (size_t)m * m));
+            HIPCHECK(ctx, fcg::launch_super_invert_global(bv, m, j.spre.snode.as<int32_t>(), A.as<double>(),
+                                                          j.spre.Sinv.as<double>(), s));
+            HIPCHECK(ctx, hipStreamSynchronize(s));  // A is released below
+        }
     }
     j.spre.dim = dim;
+    return FCG_OK;
+}
+
+// Queue the two-level preconditioner inverses of jobs (all with a structure of
+// dimension dim and at most kSuperLds elements) as one launch, on the tables
+// of tabs; q receives the launch.
+int queue_super_inverses(fcg_ctx* ctx, Tables& tabs, const std::vector<Job*>& jobs,
+                         std::vector<std::function<hipError_t()>>& q) {
+    if (jobs.empty()) return FCG_OK;
+    std::vector<BatchEntry> ents;
+    for (Job* j : jobs) ents.push_back(BatchEntry{j, &j->scr, nullptr, nullptr});
+    BatchView bv;
+    std::vector<int> order;
+    RCCHECK(make_batch(ctx, tabs, ents, &bv, nullptr, &order));
+    std::vector<fcg::SSetup> st;
+    for (int i : order) {
+        const Job* j = jobs[(size_t)i];
+        st.push_back(fcg::SSetup{j->spre.snode.as<int32_t>(), j->spre.Sinv.as<double>(), j->spre.m});
+    }
+    void* d;
+    HIPCHECK(ctx, tabs.push(st.data(), sizeof(fcg::SSetup) * st.size(), ctx->stream, &d));
+    const auto* tab = reinterpret_cast<const fcg::SSetup*>(d);
+    const int n = (int)st.size();
+    hipStream_t s = ctx->stream;
+    q.push_back([=] { return fcg::launch_super_invert_lds(bv, tab, n, s); });
     return FCG_OK;
 }
 
@@ -1309,10 +1365,13 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->jac = E->opts.precond == FCG_PRECOND_JACOBI;
     S->block = E->opts.precond == FCG_PRECOND_BLOCK;
     if (E->opts.precond == FCG_PRECOND_SUPER) {
-        RCCHECK(ensure_super_precond(ctx, *J, E->opts.precond_size));
+        RCCHECK(super_structure(ctx, *J, E->opts.precond_size));
         // unavailable (at least dim roots): the reference drops it and solves
         // unpreconditioned
         S->twolev = J->spre.avail;
+        // inverses of at most kSuperLds elements are built by the cohort's first
+        // step, for all its starting solves in one launch; larger ones here
+        if (S->twolev && J->spre.m > fcg::kSuperLds) RCCHECK(ensure_super_precond(ctx, *J, E->opts.precond_size));
     }
     S->want_x = E->want_x;
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
@@ -1419,6 +1478,20 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         std::vector<KEnt> init;
         std::vector<PEnt> initp;  // block preconditioner: x0 = P b
         std::vector<SEnt> inits;  // two-level preconditioner, likewise
+        // two-level inverses still to build for the starting solves: one launch,
+        // queued ahead of their first use
+        std::vector<Job*> sinv;
+        const int sdim = clamp_super_dim(E->opts.precond_size);
+        for (auto& u : C.active) {
+            if (u->phase != kInit || !u->twolev || u->J->spre.dim == sdim) continue;
+            if (u->J->spre.m > fcg::kSuperLds) {
+                RCCHECK(ensure_super_precond(ctx, *u->J, sdim));
+                continue;
+            }
+            u->J->spre.dim = sdim;
+            sinv.push_back(u->J);
+        }
+        RCCHECK(queue_super_inverses(ctx, C.tabs, sinv, q));
         for (auto& u : C.active) {
             if (u->phase != kInit) continue;
             if (u->block)
@@ -2039,6 +2112,9 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
             e.j->filled32 = true;
         else
             e.j->filled64 = true;
+        // a fill starts a new solve: its two-level inverse is built again on first
+        // use, as the reference builds the preconditioner after each fill
+        e.j->spre.dim = 0;
     }
     return FCG_OK;
 }

@@ -213,10 +213,23 @@ hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s);
 // Two-level ("super") preconditioner: m elements (tree nodes snode[m]) over
 // consecutive leaf ranges, the inverse Sinv of their m x m collocation matrix
 // (column-major), and per leaf its element sidx[i] and weight w[i] (leaf
-// dimension over element dimension).  Setup of one job: A (m x m scratch); the
-// inverse comes from a Gauss-Jordan elimination, 2m launches.
-hipError_t launch_super_setup(const BatchView& one_job, int m, int N, const int32_t* snode, const int32_t* sidx,
-                              double* A, double* Sinv, double* w, hipStream_t s);
+// dimension over element dimension).
+//   launch_super_weights: w of one job (structural, with the partition).
+//   launch_super_invert_lds: Sinv of many jobs with m <= kSuperLds, one
+//     workgroup per job, the matrix built and inverted in place in LDS.
+//   launch_super_invert_global: Sinv of one job of any m; A (m x m scratch),
+//     Gauss-Jordan over the whole device, 2m launches.
+constexpr int kSuperLds = 128;
+struct SSetup {
+    const int32_t* snode;  // [m]
+    double* Sinv;          // [m][m] column-major
+    int m;
+};
+hipError_t launch_super_weights(const BatchView& one_job, int N, const int32_t* snode, const int32_t* sidx, double* w,
+                                hipStream_t s);
+hipError_t launch_super_invert_lds(const BatchView& bv, const SSetup* tab, int njobs, hipStream_t s);
+hipError_t launch_super_invert_global(const BatchView& one_job, int m, const int32_t* snode, double* A, double* Sinv,
+                                      hipStream_t s);
 struct SJob {
     const void* in;  // [k][N] FP64, or FP32 when in32
     double* out;     // [k][N]

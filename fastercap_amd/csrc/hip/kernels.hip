@@ -1136,11 +1136,105 @@ __global__ void __launch_bounds__(256) k_gj_elim(double* __restrict__ A, double*
     }
 }
 
-hipError_t launch_super_setup(const BatchView& one_job, int m, int N, const int32_t* snode, const int32_t* sidx,
-                              double* A, double* Sinv, double* w, hipStream_t s) {
+// Two-level preconditioner inverse of job blockIdx.x (m <= kSuperLds), one
+// workgroup per job: the element matrix (as k_super_build) is built in LDS and
+// inverted in place by Gauss-Jordan elimination with the pivot choice of
+// InvertMatrix (Solver/SolveCapacitance.cpp:2116-2258: first row of largest
+// |entry| at or below the diagonal).  Per column j: the pivot row p is swapped
+// into row j and scaled (its column-j entry taken as 1 first), every other row
+// i loses f_i times it (its column-j entry taken as 0 first), which leaves
+// column j of the inverse of the row-permuted matrix in place; at the end the
+// columns are swapped back in reverse order of the row swaps.
+constexpr int kSuperThreads = 1024;
+__global__ void __launch_bounds__(kSuperThreads) k_super_invert_lds(BatchView bv, const SSetup* __restrict__ tab) {
+    __shared__ double A[kSuperLds * kSuperLds];  // column-major: A[c * m + i] = row i, column c
+    __shared__ double rowj[kSuperLds], colj[kSuperLds];
+    __shared__ int perm[kSuperLds];
+    const BJob& J = bv.jobs[blockIdx.x];
+    const SSetup st = tab[blockIdx.x];
+    const int m = st.m, t = threadIdx.x, mm = m * m;
+    for (int e = t; e < mm; e += kSuperThreads) {
+        const int c = e / m, i = e - c * m;
+        const fcg_node_geom& ni = J.nodes[st.snode[i]];
+        A[e] = (i == c) ? node_self(J, ni) : pair_coef(ni, J.nodes[st.snode[c]], J.dim);
+    }
+    __syncthreads();
+    for (int j = 0; j < m; ++j) {
+        if (t < 64) {  // one wavefront picks the pivot row
+            double best = -1.0;
+            int bi = m;
+            for (int i = j + t; i < m; i += 64) {
+                const double v = fabs(A[j * m + i]);
+                if (v > best) {
+                    best = v;
+                    bi = i;
+                }
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                const double v2 = __shfl_down(best, off);
+                const int i2 = __shfl_down(bi, off);
+                if (v2 > best || (v2 == best && i2 < bi)) {
+                    best = v2;
+                    bi = i2;
+                }
+            }
+            if (t == 0) perm[j] = bi;
+        }
+        __syncthreads();
+        const int p = perm[j];
+        const double rcp = 1.0 / A[j * m + p];
+        double a = 0.0, b = 0.0, f = 0.0;
+        if (t < m) {  // thread t: column t of rows j and p, row t of column j (after the swap)
+            a = A[t * m + j];
+            b = A[t * m + p];
+            f = A[j * m + (t == p ? j : t)];
+        }
+        __syncthreads();
+        if (t < m) {
+            const double r = (t == j ? 1.0 : b) * rcp;
+            rowj[t] = r;
+            colj[t] = f;
+            if (p != j) A[t * m + p] = a;
+            A[t * m + j] = r;
+        }
+        __syncthreads();
+        for (int e = t; e < mm; e += kSuperThreads) {
+            const int c = e / m, i = e - c * m;
+            if (i == j) continue;
+            A[e] = (c == j ? 0.0 : A[e]) - colj[i] * rowj[c];
+        }
+        __syncthreads();
+    }
+    for (int j = m - 1; j >= 0; --j) {
+        const int p = perm[j];  // uniform over the workgroup
+        if (p == j) continue;
+        if (t < m) {
+            const double x = A[j * m + t];
+            A[j * m + t] = A[p * m + t];
+            A[p * m + t] = x;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < mm; e += kSuperThreads) st.Sinv[e] = A[e];
+}
+
+hipError_t launch_super_weights(const BatchView& one_job, int N, const int32_t* snode, const int32_t* sidx, double* w,
+                                hipStream_t s) {
+    if (N <= 0) return hipSuccess;
+    k_super_weights<<<blocks_for((size_t)N, 256), 256, 0, s>>>(one_job, snode, sidx, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_super_invert_lds(const BatchView& bv, const SSetup* tab, int njobs, hipStream_t s) {
+    if (njobs <= 0) return hipSuccess;
+    k_super_invert_lds<<<(unsigned)njobs, kSuperThreads, 0, s>>>(bv, tab);
+    return hipGetLastError();
+}
+
+hipError_t launch_super_invert_global(const BatchView& one_job, int m, const int32_t* snode, double* A, double* Sinv,
+                                      hipStream_t s) {
     if (m <= 0) return hipSuccess;
     k_super_build<<<blocks_for((size_t)m * m, 256), 256, 0, s>>>(one_job, snode, m, A, Sinv);
-    k_super_weights<<<blocks_for((size_t)N, 256), 256, 0, s>>>(one_job, snode, sidx, w);
     for (int j = 0; j < m; ++j) {
         k_gj_pivot<<<1, kSetupThreads, 0, s>>>(A, Sinv, m, j);
         const dim3 grid((unsigned)((m + 255) / 256), (unsigned)((2 * m - j - 1 + kGjCols - 1) / kGjCols));

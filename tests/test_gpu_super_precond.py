@@ -101,6 +101,38 @@ def test_super_mixed_and_batched(dev, meshes):
         dev.release(j)
 
 
+def test_super_batched_inverses_match_oracle(dev, meshes):
+    """The engine builds the inverses of all starting solves in one launch, its
+    jobs in the batch's row-stride order (2D, 3D with 2 and 6 conductors here);
+    each job's inverse, applied right after, is the oracle's.  A refill drops
+    the inverse and the next solve builds it again to the same C."""
+    names = ["coax2d", "plates", "boxes", "many"]
+    ms = [meshes[n] for n in names]
+    jobs = [dev.upload(m) for m in ms]
+    for j in jobs:
+        dev.fill(j, fc.FCG_PREC_FP64)
+    first = dev.solve_batch(jobs, ms, tol=1e-8, precond=fc.FCG_PRECOND_SUPER, precond_size=64)
+    rng = np.random.default_rng(8)
+    for n, m, j in zip(names, ms, jobs):
+        sp = orc.SuperPrecond(m, 64)
+        print(f"batched two-level inverse: {n} N={m.N} elements={sp.m} available={sp.available}", flush=True)
+        if not sp.available:
+            continue
+        q = rng.standard_normal((2, m.N))
+        got = dev.super_precond(j, q, 64)
+        for r in range(2):
+            ref = sp.apply(q[r])
+            np.testing.assert_allclose(got[r], ref, rtol=1e-9, atol=1e-9 * np.abs(ref).max(), err_msg=n)
+    for j in jobs:
+        dev.fill(j, fc.FCG_PREC_FP64)
+    again = dev.solve_batch(jobs, ms, tol=1e-8, precond=fc.FCG_PRECOND_SUPER, precond_size=64)
+    for n, a, b in zip(names, first, again):
+        assert _c_err(b.cap, a.cap) < 1e-12, n
+        assert list(a.iterations) == list(b.iterations), n
+    for j in jobs:
+        dev.release(j)
+
+
 def test_super_unavailable_solves_unpreconditioned(dev, meshes):
     m = meshes["many"]
     roots = int((m.parent < 0).sum())
