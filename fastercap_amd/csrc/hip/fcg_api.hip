@@ -1159,10 +1159,12 @@ struct fcg_engine {
 
 namespace {
 
-// step buffers grow geometrically: a step's size creeps up with the Arnoldi index
+// step buffers of a cohort grow geometrically: a step's size creeps up with the
+// Arnoldi index.  No wait: the cohort's previous step has completed before its
+// next one is planned, so its buffers are idle (the other cohort's work in flight
+// uses buffers of its own).
 int ensure_pbuf(fcg_ctx* ctx, PBuf& b, size_t bytes) {
     if (b.p && b.r >= bytes) return FCG_OK;
-    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     HIPCHECK(ctx, b.get(&ctx->pool, std::max(bytes, 2 * b.r)));
     return FCG_OK;
 }
