@@ -1005,9 +1005,12 @@ int make_sview(fcg_ctx* ctx, Tables& tabs, const std::vector<SEnt>& ents, fcg::S
     std::memset(sv, 0, sizeof *sv);
     if (ents.empty()) return FCG_OK;
     std::vector<fcg::SJob> sj;
-    std::vector<int64_t> off(2 * (ents.size() + 1), 0);
+    const size_t n1 = ents.size() + 1;
+    std::vector<int64_t> off(4 * n1, 0);
     int64_t* eo = off.data();
-    int64_t* lo = eo + ents.size() + 1;
+    int64_t* lo = eo + n1;
+    int64_t* no = lo + n1;  // collection lanes, whole wavefronts per solve
+    int64_t* mo = no + n1;  // elements, whole workgroups of 64 per solve
     for (size_t i = 0; i < ents.size(); ++i) {
         const SEnt& e = ents[i];
         const Job::SuperPre& sp = e.j->spre;
@@ -1025,19 +1028,31 @@ int make_sview(fcg_ctx* ctx, Tables& tabs, const std::vector<SEnt>& ents, fcg::S
         r.k = e.k;
         r.m = sp.m;
         r.in32 = e.in32 ? 1 : 0;
+        // lanes per element: the power of two at or above the mean leaves per
+        // element, at most a wavefront
+        const int per = (e.j->N + sp.m - 1) / std::max(sp.m, 1);
+        r.lpe = 1;
+        while (r.lpe < per && r.lpe < 64) r.lpe <<= 1;
         sj.push_back(r);
         eo[i + 1] = eo[i] + (int64_t)e.k * sp.m;
         lo[i + 1] = lo[i] + (int64_t)e.k * e.j->N;
+        no[i + 1] = no[i] + ((int64_t)e.k * sp.m * r.lpe + 63) / 64 * 64;
+        mo[i + 1] = mo[i] + ((int64_t)sp.m + 63) / 64 * 64;
     }
     void *dj, *doff;
     HIPCHECK(ctx, tabs.push(sj.data(), sizeof(fcg::SJob) * sj.size(), ctx->stream, &dj));
     HIPCHECK(ctx, tabs.push(off.data(), sizeof(int64_t) * off.size(), ctx->stream, &doff));
     sv->jobs = reinterpret_cast<const fcg::SJob*>(dj);
     sv->njobs = (int)ents.size();
-    sv->el_off = reinterpret_cast<const int64_t*>(doff);
-    sv->lf_off = reinterpret_cast<const int64_t*>(doff) + ents.size() + 1;
+    const int64_t* d = reinterpret_cast<const int64_t*>(doff);
+    sv->el_off = d;
+    sv->lf_off = d + n1;
+    sv->ln_off = d + 2 * n1;
+    sv->em_off = d + 3 * n1;
     sv->nel = eo[ents.size()];
     sv->nlf = lo[ents.size()];
+    sv->nln = no[ents.size()];
+    sv->nem = mo[ents.size()];
     return FCG_OK;
 }
 

@@ -241,13 +241,16 @@ struct SJob {
     double* pot;            // [k][m] scratch of this solve
     double* chg;            // [k][m]
     int N, k, m, in32;
+    int lpe;                // lanes per element in the collection (power of two, at most 64)
 };
 struct SView {
     const SJob* jobs;
     int njobs;
     const int64_t* el_off;  // cumulative k * m per solve
     const int64_t* lf_off;  // cumulative k * N per solve
-    int64_t nel, nlf;
+    const int64_t* ln_off;  // cumulative collection lanes per solve: k * m * lpe rounded up to 64
+    const int64_t* em_off;  // cumulative m per solve, each rounded up to 64
+    int64_t nel, nlf, nln, nem;
 };
 hipError_t launch_super_apply(const SView& sv, int mode, hipStream_t s);
 

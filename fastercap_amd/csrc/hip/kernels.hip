@@ -1319,45 +1319,82 @@ hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s) {
 
 // Two-level preconditioner apply (ComputePrecondVectFast super branch,
 // Solver/SolveCapacitance.cpp:3545-3592), batched over solves.
-// collect: one wavefront per (solve, rhs, element): pot = sum of w_i in_i
+// collect: a group of lpe lanes per (solve, rhs, element): pot = sum of w_i in_i
+// over the element's leaves.  lpe follows the solve's leaves per element; each
+// solve's lanes start on a wavefront boundary, so the groups are aligned and a
+// group's shuffles stay inside it (padding lanes form whole groups and write
+// nothing).
 template <int MODE>
 __global__ void __launch_bounds__(256) k_super_collect(SView sv) {
-    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= sv.nel) return;
-    const int jb = find_job(sv.el_off, sv.njobs, g);
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= sv.nln) return;
+    const int jb = find_job(sv.ln_off, sv.njobs, g);
     const SJob& J = sv.jobs[jb];
-    const int local = (int)(g - sv.el_off[jb]);
-    const int r = local / J.m, e = local - r * J.m;
-    const int lane = threadIdx.x & 63;
+    const int64_t local = g - sv.ln_off[jb];
+    const int lpe = J.lpe;
+    const int64_t grp = local / lpe;
+    const int sub = (int)(local - grp * lpe);
+    const bool live = grp < (int64_t)J.k * J.m;
+    const int r = live ? (int)(grp / J.m) : 0, e = live ? (int)(grp - (int64_t)r * J.m) : 0;
     const double* in64 = static_cast<const double*>(J.in);
     const float* in32 = static_cast<const float*>(J.in);
     double acc = 0.0;
-    for (int i = J.sstart[e] + lane; i < J.sstart[e + 1]; i += 64) {
-        double x;
-        if (MODE == kPreFromB)
-            x = (i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
-        else
-            x = J.in32 ? (double)in32[(size_t)r * J.N + i] : in64[(size_t)r * J.N + i];
-        acc += J.w[i] * x;
+    if (live) {
+        for (int i = J.sstart[e] + sub; i < J.sstart[e + 1]; i += lpe) {
+            double x;
+            if (MODE == kPreFromB)
+                x = (i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
+            else
+                x = J.in32 ? (double)in32[(size_t)r * J.N + i] : in64[(size_t)r * J.N + i];
+            acc += J.w[i] * x;
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
-    if (lane == 0) J.pot[(size_t)r * J.m + e] = acc;
+    for (int off = lpe >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
+    if (live && sub == 0) J.pot[(size_t)r * J.m + e] = acc;
 }
 
-// multiply: one thread per (solve, rhs, element s): chg = sum over t != s of
-// Sinv[s][t] pot[t] (the diagonal is applied per leaf in the distribution)
-__global__ void k_super_mult(SView sv) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= sv.nel) return;
-    const int jb = find_job(sv.el_off, sv.njobs, g);
+// multiply: chg = sum over t != s of Sinv[s][t] pot[t] (the diagonal is applied
+// per leaf in the distribution).  A workgroup of 4 wavefronts takes 64 elements s
+// of one solve (each solve's elements padded to whole workgroups); wavefront q
+// sums over the quarter q of t, the right-hand sides in passes of kSuperRhs over
+// Sinv's rows, and the partial sums meet in LDS.
+constexpr int kSuperRhs = 4;
+__global__ void __launch_bounds__(256) k_super_mult(SView sv) {
+    __shared__ double part[3][kSuperRhs][64];
+    const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
+    const int64_t b0 = (int64_t)blockIdx.x * 64;
+    if (b0 >= sv.nem) return;  // uniform over the workgroup
+    const int jb = find_job(sv.em_off, sv.njobs, b0);
     const SJob& J = sv.jobs[jb];
-    const int local = (int)(g - sv.el_off[jb]);
-    const int r = local / J.m, e = local - r * J.m;
-    const double* pot = J.pot + (size_t)r * J.m;
-    double acc = 0.0;
-    for (int t = 0; t < J.m; ++t)
-        if (t != e) acc += J.Sinv[(size_t)t * J.m + e] * pot[t];
-    J.chg[(size_t)r * J.m + e] = acc;
+    const int m = J.m, e = (int)(b0 - sv.em_off[jb]) + lane;
+    const bool live = e < m;
+    const int tq = (m + 3) / 4, t0 = q * tq, t1 = min(m, t0 + tq);
+    const double* __restrict__ S = J.Sinv + (live ? e : 0);  // S[t * m] = Sinv[e][t]
+    for (int r0 = 0; r0 < J.k; r0 += kSuperRhs) {
+        const int nr = min(kSuperRhs, J.k - r0);
+        const double* pot = J.pot + (size_t)r0 * m;
+        double acc[kSuperRhs] = {};
+        if (live) {
+            for (int t = t0; t < t1; ++t) {
+                const double a = (t != e) ? S[(size_t)t * m] : 0.0;
+#pragma unroll
+                for (int r = 0; r < kSuperRhs; ++r)
+                    if (r < nr) acc[r] += a * pot[(size_t)r * m + t];
+            }
+        }
+        if (q > 0) {
+#pragma unroll
+            for (int r = 0; r < kSuperRhs; ++r) part[q - 1][r][lane] = acc[r];
+        }
+        __syncthreads();
+        if (q == 0 && live) {
+#pragma unroll
+            for (int r = 0; r < kSuperRhs; ++r)
+                if (r < nr)
+                    J.chg[(size_t)(r0 + r) * m + e] = acc[r] + part[0][r][lane] + part[1][r][lane] + part[2][r][lane];
+        }
+        __syncthreads();
+    }
 }
 
 // distribute: one thread per (solve, rhs, leaf)
@@ -1385,12 +1422,12 @@ __global__ void k_super_dist(SView sv) {
 
 hipError_t launch_super_apply(const SView& sv, int mode, hipStream_t s) {
     if (sv.nel == 0) return hipSuccess;
-    const unsigned gc = (unsigned)((sv.nel + 3) / 4);
+    const unsigned gc = blocks_for((size_t)sv.nln, 256);
     if (mode == kPreFromB)
         k_super_collect<kPreFromB><<<gc, 256, 0, s>>>(sv);
     else
         k_super_collect<kPreSet><<<gc, 256, 0, s>>>(sv);
-    k_super_mult<<<blocks_for((size_t)sv.nel, 256), 256, 0, s>>>(sv);
+    k_super_mult<<<(unsigned)(sv.nem / 64), 256, 0, s>>>(sv);
     const unsigned gd = blocks_for((size_t)sv.nlf, 256);
     if (mode == kPreFromB)
         k_super_dist<kPreFromB><<<gd, 256, 0, s>>>(sv);
