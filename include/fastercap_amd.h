@@ -134,7 +134,9 @@ int fcg_job_release(fcg_ctx* ctx, int job);
  * calling PotEstimateOpt (Solver/Autorefine.cpp:3450-3733, 2D :4156-4325), and the self
  * terms of RefineSelf/SelfPotential (Solver/Autorefine.cpp:2115-2229, 2358-2506). */
 int fcg_fill(fcg_ctx* ctx, int job, int precision);
-/* fcg_fill of several jobs in one launch (same arithmetic per link). */
+/* fcg_fill of several jobs in one launch (same arithmetic per link).
+ * A fill starts a new solve: the job's two-level preconditioner inverse is
+ * built again on its next use (its element partition is kept). */
 int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision);
 
 /* Copy the filled coefficients to host memory (coef [L] and self [N], either may be NULL). */
