@@ -1505,10 +1505,13 @@ int engine_launch(fcg_engine* E, Cohort& C) {
                 RCCHECK(ensure_super_precond(ctx, *u->J, sdim));
                 continue;
             }
-            u->J->spre.dim = sdim;
+            u->J->spre.dim = sdim;  // marks the job as queued (a job may start several solves)
             sinv.push_back(u->J);
         }
-        RCCHECK(queue_super_inverses(ctx, C.tabs, sinv, q));
+        if (const int rc = queue_super_inverses(ctx, C.tabs, sinv, q); rc != FCG_OK) {
+            for (Job* j : sinv) j->spre.dim = 0;  // not queued: built again on the next use
+            return rc;
+        }
         for (auto& u : C.active) {
             if (u->phase != kInit) continue;
             if (u->block)
