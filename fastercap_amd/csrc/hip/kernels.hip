@@ -1150,6 +1150,7 @@ __global__ void __launch_bounds__(kSuperThreads) k_super_invert_lds(BatchView bv
     __shared__ double A[kSuperLds * kSuperLds];  // column-major: A[c * m + i] = row i, column c
     __shared__ double rowj[kSuperLds], colj[kSuperLds];
     __shared__ int perm[kSuperLds];
+    static_assert(kSuperThreads % kSuperLds == 0, "whole columns per thread group");
     const BJob& J = bv.jobs[blockIdx.x];
     const SSetup st = tab[blockIdx.x];
     const int m = st.m, t = threadIdx.x, mm = m * m;
@@ -1198,10 +1199,15 @@ __global__ void __launch_bounds__(kSuperThreads) k_super_invert_lds(BatchView bv
             A[t * m + j] = r;
         }
         __syncthreads();
-        for (int e = t; e < mm; e += kSuperThreads) {
-            const int c = e / m, i = e - c * m;
-            if (i == j) continue;
-            A[e] = (c == j ? 0.0 : A[e]) - colj[i] * rowj[c];
+        // thread t: row t % kSuperLds of the columns congruent to t / kSuperLds
+        // modulo kSuperThreads / kSuperLds (no division by m in the loop)
+        const int ei = t % kSuperLds;
+        if (ei < m && ei != j) {
+            const double f = colj[ei];
+            for (int c = t / kSuperLds; c < m; c += kSuperThreads / kSuperLds) {
+                double* a = &A[c * m + ei];
+                *a = (c == j ? 0.0 : *a) - f * rowj[c];
+            }
         }
         __syncthreads();
     }
