@@ -1014,6 +1014,8 @@ int make_sview(fcg_ctx* ctx, Tables& tabs, const std::vector<SEnt>& ents, fcg::S
     for (size_t i = 0; i < ents.size(); ++i) {
         const SEnt& e = ents[i];
         const Job::SuperPre& sp = e.j->spre;
+        if ((int64_t)e.k * e.j->N >= ((int64_t)1 << 31))
+            return fail(ctx, FCG_INVALID_ARGUMENT, "two-level preconditioner: k x N exceeds 2^31");
         fcg::SJob r{};
         r.in = e.in;
         r.out = e.out;

@@ -1336,12 +1336,13 @@ __global__ void __launch_bounds__(256) k_super_collect(SView sv) {
     if (g >= sv.nln) return;
     const int jb = find_job(sv.ln_off, sv.njobs, g);
     const SJob& J = sv.jobs[jb];
-    const int64_t local = g - sv.ln_off[jb];
+    // a solve's lanes fit in 32 bits (k * m * lpe, at most 32 * 4096 * 64)
+    const int local = (int)(g - sv.ln_off[jb]);
     const int lpe = J.lpe;
-    const int64_t grp = local / lpe;
-    const int sub = (int)(local - grp * lpe);
-    const bool live = grp < (int64_t)J.k * J.m;
-    const int r = live ? (int)(grp / J.m) : 0, e = live ? (int)(grp - (int64_t)r * J.m) : 0;
+    const int grp = local >> __builtin_ctz(lpe);  // lpe is a power of two
+    const int sub = local & (lpe - 1);
+    const bool live = grp < J.k * J.m;
+    const int r = live ? grp / J.m : 0, e = live ? grp - r * J.m : 0;
     const double* in64 = static_cast<const double*>(J.in);
     const float* in32 = static_cast<const float*>(J.in);
     double acc = 0.0;
@@ -1410,8 +1411,8 @@ __global__ void k_super_dist(SView sv) {
     if (g >= sv.nlf) return;
     const int jb = find_job(sv.lf_off, sv.njobs, g);
     const SJob& J = sv.jobs[jb];
-    const int64_t local = g - sv.lf_off[jb];
-    const int r = (int)(local / J.N), i = (int)(local - (int64_t)r * J.N);
+    const int local = (int)(g - sv.lf_off[jb]);  // k * N < 2^31 (checked when the table is built)
+    const int r = local / J.N, i = local - r * J.N;
     const int e = J.sidx[i];
     const size_t o = (size_t)r * J.N + i;
     double x;
