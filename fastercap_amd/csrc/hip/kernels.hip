@@ -661,17 +661,20 @@ __global__ void __launch_bounds__(kKThreads) k_dots(KrylovView kv, double* __res
     if (threadIdx.x == 0) partial[(size_t)c * kv.maxnv + jj] = s;
 }
 
-// grid (nsegs, maxnv): hbuf[hoff + which*nv*k + jj*k + r]
-__global__ void k_reduce(KrylovView kv, const double* __restrict__ partial, double* __restrict__ hbuf, int which) {
-    __shared__ double sh[4];
-    const int si = blockIdx.x, jj = blockIdx.y;
+// hbuf[hoff + which*nv*k + jj*k + r] = sum over the segment of the dots partials
+// one thread per (segment, basis vector): the segment's chunk partials summed in
+// chunk order (a segment has N / kChunk chunks, a few for most jobs)
+__global__ void __launch_bounds__(256) k_reduce(KrylovView kv, const double* __restrict__ partial,
+                                                double* __restrict__ hbuf, int which) {
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= kv.nsegs * kv.maxnv) return;
+    const int si = g / kv.maxnv, jj = g - si * kv.maxnv;
     const Seg sg = kv.segs[si];
     const KJob kj = kv.kjobs[sg.job];
     if (jj >= kj.nv) return;
     double s = 0.0;
-    for (int c = sg.c_lo + threadIdx.x; c < sg.c_hi; c += blockDim.x) s += partial[(size_t)c * kv.maxnv + jj];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) hbuf[(size_t)kj.hoff + (size_t)which * kj.nv * kj.k + (size_t)jj * kj.k + sg.r] = s;
+    for (int c = sg.c_lo; c < sg.c_hi; ++c) s += partial[(size_t)c * kv.maxnv + jj];
+    hbuf[(size_t)kj.hoff + (size_t)which * kj.nv * kj.k + (size_t)jj * kj.k + sg.r] = s;
 }
 
 // grid (nchunks): w -= sum_jj h[jj] Q_jj on the chunk; optional |w|^2 partial[c]
@@ -801,7 +804,7 @@ hipError_t launch_dots(const KrylovView& kv, double* partial, hipStream_t s) {
 }
 hipError_t launch_reduce(const KrylovView& kv, const double* partial, double* hbuf, int which, hipStream_t s) {
     if (kv.nsegs == 0 || kv.maxnv == 0) return hipSuccess;
-    k_reduce<<<dim3(kv.nsegs, kv.maxnv), 256, 0, s>>>(kv, partial, hbuf, which);
+    k_reduce<<<blocks_for((size_t)kv.nsegs * kv.maxnv, 256), 256, 0, s>>>(kv, partial, hbuf, which);
     return hipGetLastError();
 }
 hipError_t launch_update(const KrylovView& kv, const double* hbuf, int which, int norm, double* partial,
