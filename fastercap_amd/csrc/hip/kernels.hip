@@ -250,28 +250,31 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(BatchView bv, int 
     if (threadIdx.x == 0) J.block_sums[(size_t)r * J.nblk + b] = total;
 }
 
-// exclusive scan of the block sums, one block per (job, rhs)
-__global__ void __launch_bounds__(1024) k_scan_top(BatchView bv) {
-    __shared__ DD sh[1024];
-    const int64_t g = blockIdx.x;
+// exclusive scan of the block sums, one wavefront per (job, rhs): chunks of 64
+// tile sums, each scanned by shuffles (the additions of the block scan above),
+// a running carry between chunks
+__global__ void __launch_bounds__(256) k_scan_top(BatchView bv) {
+    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (g >= bv.rhs) return;  // whole wavefronts
+    const int lane = threadIdx.x & 63;
     const int jb = find_job(bv.rhs_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int r = (int)(g - bv.rhs_off[jb]);
     DD* bs = J.block_sums + (size_t)r * J.nblk;
     const int nblk = J.nblk;
-    const int per = (nblk + 1023) / 1024;
-    const int b0 = threadIdx.x * per;
-    DD acc{0.0, 0.0};
-    for (int i = 0; i < per; ++i)
-        if (b0 + i < nblk) acc = dd_add(acc, bs[b0 + i]);
-    DD total;
-    DD off = block_exclusive_scan<1024>(acc, sh, &total);
-    for (int i = 0; i < per; ++i) {
-        if (b0 + i < nblk) {
-            const DD v = bs[b0 + i];
-            bs[b0 + i] = off;
-            off = dd_add(off, v);
+    DD carry{0.0, 0.0};
+    for (int b0 = 0; b0 < nblk; b0 += 64) {
+        const int b = b0 + lane;
+        DD inc = (b < nblk) ? bs[b] : DD{0.0, 0.0};
+        for (int off = 1; off < 64; off <<= 1) {
+            const DD left{__shfl_up(inc.hi, off, 64), __shfl_up(inc.lo, off, 64)};
+            if (lane >= off) inc = dd_add(inc, left);
         }
+        const DD prev{__shfl_up(inc.hi, 1, 64), __shfl_up(inc.lo, 1, 64)};
+        const DD excl = (lane > 0) ? prev : DD{0.0, 0.0};
+        if (b < nblk) bs[b] = (b0 == 0) ? excl : dd_add(carry, excl);
+        const DD chunk{__shfl(inc.hi, 63, 64), __shfl(inc.lo, 63, 64)};
+        carry = (b0 == 0) ? chunk : dd_add(carry, chunk);
     }
 }
 
@@ -608,7 +611,7 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
     if (b.njobs == 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], s);
     k_scan_reduce<<<(unsigned)b.tiles, kScanThreads, 0, s>>>(b, jacobi);
-    k_scan_top<<<(unsigned)b.rhs, 1024, 0, s>>>(b);
+    k_scan_top<<<blocks_for((size_t)b.rhs, 4), 256, 0, s>>>(b);
     k_scan_final<<<(unsigned)b.tiles, kScanThreads, 0, s>>>(b, jacobi);
     if (ev) (void)hipEventRecord(ev[1], s);
     const unsigned nb = blocks_for((size_t)b.nodes, 256);
