@@ -205,22 +205,32 @@ __global__ void k_self(BatchView bv) {
 }
 
 // ------------------------------------------------------------------ up-pass scan
+// Exclusive scan of one DD per thread over a block of NT threads (NT a multiple
+// of 64): an inclusive shuffle scan inside each wavefront, the wavefront totals
+// through LDS (sh: NT / 64 entries), then each wavefront adds the totals of the
+// ones before it.  *total is the block's sum.
 template <int NT>
 __device__ DD block_exclusive_scan(DD v, DD* sh, DD* total) {
-    const int t = threadIdx.x;
-    sh[t] = v;
-    __syncthreads();
-    for (int off = 1; off < NT; off <<= 1) {
-        DD add = DD{0.0, 0.0};
-        if (t >= off) add = sh[t - off];
-        __syncthreads();
-        if (t >= off) sh[t] = dd_add(sh[t], add);
-        __syncthreads();
+    constexpr int NW = NT / 64;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    DD inc = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        const DD left{__shfl_up(inc.hi, off, 64), __shfl_up(inc.lo, off, 64)};
+        if (lane >= off) inc = dd_add(inc, left);
     }
-    const DD excl = (t > 0) ? sh[t - 1] : DD{0.0, 0.0};
-    *total = sh[NT - 1];
+    if (lane == 63) sh[wv] = inc;
     __syncthreads();
-    return excl;
+    DD before{0.0, 0.0}, all{0.0, 0.0};
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+        if (w == wv) before = all;
+        all = (w == 0) ? sh[0] : dd_add(all, sh[w]);
+    }
+    const DD prev{__shfl_up(inc.hi, 1, 64), __shfl_up(inc.lo, 1, 64)};
+    const DD excl = (lane > 0) ? prev : DD{0.0, 0.0};
+    *total = all;
+    __syncthreads();  // sh may be reused by the caller
+    return wv == 0 ? excl : dd_add(before, excl);
 }
 
 constexpr int kScanThreads = 256;
