@@ -312,6 +312,8 @@ struct Job {
 struct fcg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr;  // engine table uploads (default priority, like stream)
+    hipStream_t down_stream = nullptr;  // engine result downloads (default priority)
     bool profiling = false;
     fcg_profile prof{};
     // gather lane shape of products over several jobs, per product precision and
@@ -1140,7 +1142,9 @@ struct Cohort {
     Tables tabs;
     PBuf partial_a, partial_b, hbuf;
     ProfEvents pe_res, pe_arn;
-    hipEvent_t ev = nullptr;  // end of the step in flight
+    hipEvent_t ev = nullptr;      // end of the step in flight
+    hipEvent_t tab_ev = nullptr;  // end of the step's table upload
+    hipEvent_t kev = nullptr;     // end of the step's kernels
     bool inflight = false;
     int64_t load = 0;  // links x rhs of the active solves (balances submissions)
     // the step in flight, as its bookkeeping needs it
@@ -1156,6 +1160,8 @@ struct Cohort {
     Cohort& operator=(const Cohort&) = delete;
     ~Cohort() {
         if (ev) (void)hipEventDestroy(ev);
+        if (tab_ev) (void)hipEventDestroy(tab_ev);
+        if (kev) (void)hipEventDestroy(kev);
     }
 };
 constexpr int kCohorts = 2;
@@ -1171,6 +1177,14 @@ struct fcg_engine {
         size_t n = 0;
         for (const Cohort& c : co) n += c.active.size();
         return n;
+    }
+    // a step still in flight (an error path) must end before the cohorts' buffers
+    // and staging are freed
+    ~fcg_engine() {
+        if (!ctx) return;
+        (void)hipStreamSynchronize(ctx->stream);
+        (void)hipStreamSynchronize(ctx->copy_stream);
+        (void)hipStreamSynchronize(ctx->down_stream);
     }
 };
 
@@ -1440,7 +1454,14 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     const bool mixed = E->opts.precision == FCG_PREC_MIXED;
     // Every table of the step is written to pinned staging first and reaches the
     // device in one copy; the launches are queued here and issued after that copy.
-    std::vector<std::function<hipError_t()>> q;
+    // The results go back to the host after the step's last kernel, on a stream of
+    // their own (dq), so the other cohort's next kernels do not wait for them.
+    std::vector<std::function<hipError_t()>> q, dq;
+    // The side streams pay off only while another cohort has work in flight;
+    // alone (a single solve), the copies stay on the compute stream.
+    bool overlap = false;
+    for (const Cohort& c : E->co) overlap |= (&c != &C && c.inflight);
+    const hipStream_t us = overlap ? ctx->copy_stream : s, ds = overlap ? ctx->down_stream : s;
 
     // ---- A: pending basis scalings, solution updates and starts
     {
@@ -1561,7 +1582,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             hx = reinterpret_cast<double*>(hxv);
             const void* xs = S.x.p;
             const size_t nx = sizeof(double) * (size_t)S.k * S.N;
-            q.push_back([=] { return hipMemcpyAsync(hx, xs, nx, hipMemcpyDeviceToHost, s); });
+            dq.push_back([=] { return hipMemcpyAsync(hx, xs, nx, hipMemcpyDeviceToHost, ds); });
         }
         fin_x.push_back(hx);
     }
@@ -1579,7 +1600,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         double* out = fin_dev.as<double>();
         const size_t nbytes = sizeof(double) * (size_t)fin_vals;
         q.push_back([=] { return fcg::launch_charge_sums(jt, nj, nb, out, s); });
-        q.push_back([=] { return hipMemcpyAsync(h, out, nbytes, hipMemcpyDeviceToHost, s); });
+        dq.push_back([=] { return hipMemcpyAsync(h, out, nbytes, hipMemcpyDeviceToHost, ds); });
     }
 
     // ---- B: refinement residuals r = b - A x
@@ -1686,12 +1707,27 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         hh = reinterpret_cast<double*>(h);
         const void* src = C.hbuf.p;
         const size_t nb = sizeof(double) * (size_t)hoff;
-        q.push_back([=] { return hipMemcpyAsync(hh, src, nb, hipMemcpyDeviceToHost, s); });
+        dq.push_back([=] { return hipMemcpyAsync(hh, src, nb, hipMemcpyDeviceToHost, ds); });
     }
-    HIPCHECK(ctx, C.tabs.flush(s));
+    // The tables go up on the copy stream, which does not queue behind the other
+    // cohort's work in flight on the compute stream (they belong to this cohort,
+    // whose previous step has completed); the compute stream waits for the copy
+    // alone, so the step's first kernel follows the other cohort's last one.
+    HIPCHECK(ctx, C.tabs.flush(us));
+    if (us != s) {
+        if (!C.tab_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&C.tab_ev, hipEventDisableTiming));
+        HIPCHECK(ctx, hipEventRecord(C.tab_ev, us));
+        HIPCHECK(ctx, hipStreamWaitEvent(s, C.tab_ev, 0));
+    }
     for (auto& f : q) HIPCHECK(ctx, f());
+    if (ds != s) {
+        if (!C.kev) HIPCHECK(ctx, hipEventCreateWithFlags(&C.kev, hipEventDisableTiming));
+        HIPCHECK(ctx, hipEventRecord(C.kev, s));
+        HIPCHECK(ctx, hipStreamWaitEvent(ds, C.kev, 0));
+    }
+    for (auto& f : dq) HIPCHECK(ctx, f());
     if (!C.ev) HIPCHECK(ctx, hipEventCreateWithFlags(&C.ev, hipEventDisableTiming));
-    HIPCHECK(ctx, hipEventRecord(C.ev, s));
+    HIPCHECK(ctx, hipEventRecord(C.ev, ds));  // the kernels and the downloads of the step
     C.inflight = true;
     C.res = std::move(res);
     C.arn = std::move(arn);
@@ -1825,6 +1861,15 @@ int fcg_ctx_create(int device, fcg_ctx** out) {
     auto c = std::make_unique<fcg_ctx>();
     c->device = device;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FCG_DEVICE_ERROR;
+    if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        return FCG_DEVICE_ERROR;
+    }
+    if (hipStreamCreateWithFlags(&c->down_stream, hipStreamNonBlocking) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        (void)hipStreamDestroy(c->copy_stream);
+        return FCG_DEVICE_ERROR;
+    }
     *out = c.release();
     return FCG_OK;
 }
@@ -1833,9 +1878,13 @@ int fcg_ctx_destroy(fcg_ctx* ctx) {
     if (!ctx) return FCG_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    hipStream_t s = ctx->stream;
+    (void)hipStreamSynchronize(ctx->copy_stream);
+    (void)hipStreamSynchronize(ctx->down_stream);
+    hipStream_t s = ctx->stream, cs = ctx->copy_stream, ds = ctx->down_stream;
     delete ctx;
     (void)hipStreamDestroy(s);
+    (void)hipStreamDestroy(cs);
+    (void)hipStreamDestroy(ds);
     return FCG_OK;
 }
 
@@ -2588,8 +2637,7 @@ int fcg_engine_bytes(fcg_engine* eng, int64_t* bytes) {
 
 int fcg_engine_destroy(fcg_engine* eng) {
     if (!eng) return FCG_INVALID_ARGUMENT;
-    (void)hipStreamSynchronize(eng->ctx->stream);
-    delete eng;
+    delete eng;  // waits for the engine's work in flight
     return FCG_OK;
 }
 
