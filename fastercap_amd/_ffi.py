@@ -179,7 +179,8 @@ FCG_PREC_MIXED = 1
 FCG_PRECOND_NONE = 0
 FCG_PRECOND_JACOBI = 1
 FCG_PRECOND_BLOCK = 2
-FCG_PRECOND_SUPER = 3
+FCG_PRECOND_SUPER = 4
+FCG_PRECOND_HIER = 8
 FCG_MVP_FP64 = 0
 FCG_MVP_FP32 = 1
 FCG_MVP_FP32_ACC64 = 2

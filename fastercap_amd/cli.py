@@ -72,7 +72,7 @@ def parse_args(argv) -> Args:
         elif opt == "-ap":
             a.auto_precond = True
         elif opt == "-pj":
-            if a.precond != 3:
+            if a.precond != 4:  # FCG_PRECOND_SUPER
                 a.precond = 1
             a.jacobi_set = True
         elif opt == "-pn":
@@ -99,7 +99,7 @@ def parse_args(argv) -> Args:
             v = _num(opt[3:], opt)
             if v != int(v) or v < 0:
                 raise CommandLineError(f"bad preconditioner dimension '{opt[3:]}'")
-            a.precond = 3  # FCG_PRECOND_SUPER
+            a.precond = 4  # FCG_PRECOND_SUPER
             a.super_dim = int(min(v, 1e9))
             if a.super_dim > 4096:
                 print("Warning: maximum two-levels preconditioner dimension is 4096, using 4096", file=sys.stderr)
@@ -133,7 +133,7 @@ def parse_args(argv) -> Args:
                 a.auto_err = _num(opt[2:], opt)
         else:
             raise CommandLineError(f"unsupported parameter '{opt}' (this build covers the solve path only)")
-    if a.precond == 3 and a.jacobi_set:
+    if a.precond == 4 and a.jacobi_set:
         print("Warning: both two-levels preconditioner and Jacobi preconditioner specified, keeping only the "
               "two-levels preconditioner", file=sys.stderr)
     if not a.file and not a.stop:
@@ -166,7 +166,7 @@ def auto_preconditioner(mutual_links: int, conductors: int):
     lx = (mutual_links // 1000) * conductors
     if lx <= 500:
         return 1, 128
-    return 3, (128 if lx <= 5000 else 512 if lx <= 10000 else 1024)
+    return 4, (128 if lx <= 5000 else 512 if lx <= 10000 else 1024)
 
 
 def preconditioner(a: Args, mesh):
@@ -209,7 +209,7 @@ def format_params(a: Args, tol: float) -> str:
 
 def _precond_text(precond: int, size: int) -> str:
     return {0: "None", 1: "Jacobi ", 2: f"Block, block preconditioner dimension (-pb): {size} ",
-            3: f"Two-levels, two-levels preconditioner dimension (-ps): {size} "}[precond]
+            4: f"Two-levels, two-levels preconditioner dimension (-ps): {size} "}[precond]
 
 
 def _report_solve(out, a: Args, mesh, res):

@@ -1037,6 +1037,9 @@ int make_sview(fcg_ctx* ctx, Tables& tabs, const std::vector<SEnt>& ents, fcg::S
         const int per = (e.j->N + sp.m - 1) / std::max(sp.m, 1);
         r.lpe = 1;
         while (r.lpe < per && r.lpe < 64) r.lpe <<= 1;
+        // k_super_collect indexes a solve's lanes in 32 bits
+        if (((int64_t)e.k * sp.m * r.lpe + 63) / 64 * 64 >= ((int64_t)1 << 31))
+            return fail(ctx, FCG_INVALID_ARGUMENT, "two-level preconditioner: collection lanes exceed 2^31");
         sj.push_back(r);
         eo[i + 1] = eo[i] + (int64_t)e.k * sp.m;
         lo[i + 1] = lo[i] + (int64_t)e.k * e.j->N;
@@ -1457,6 +1460,20 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     // The results go back to the host after the step's last kernel, on a stream of
     // their own (dq), so the other cohort's next kernels do not wait for them.
     std::vector<std::function<hipError_t()>> q, dq;
+    // two-level inverses still to build for the step's starting solves (one
+    // launch, queued ahead of their first use).  A job listed here is marked as
+    // having its inverse; until the step's launches have all been issued that
+    // is not sure, so every error exit clears the marks and the next use builds
+    // the inverse again.
+    std::vector<Job*> sinv;
+    struct MarkGuard {
+        std::vector<Job*>& jobs;
+        bool armed;
+        ~MarkGuard() {
+            if (armed)
+                for (Job* j : jobs) j->spre.dim = 0;
+        }
+    } sinv_guard{sinv, true};
     // The side streams pay off only while another cohort has work in flight;
     // alone (a single solve), the copies stay on the compute stream.
     bool overlap = false;
@@ -1518,9 +1535,6 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         std::vector<KEnt> init;
         std::vector<PEnt> initp;  // block preconditioner: x0 = P b
         std::vector<SEnt> inits;  // two-level preconditioner, likewise
-        // two-level inverses still to build for the starting solves: one launch,
-        // queued ahead of their first use
-        std::vector<Job*> sinv;
         const int sdim = clamp_super_dim(E->opts.precond_size);
         for (auto& u : C.active) {
             if (u->phase != kInit || !u->twolev || u->J->spre.dim == sdim) continue;
@@ -1531,10 +1545,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             u->J->spre.dim = sdim;  // marks the job as queued (a job may start several solves)
             sinv.push_back(u->J);
         }
-        if (const int rc = queue_super_inverses(ctx, C.tabs, sinv, q); rc != FCG_OK) {
-            for (Job* j : sinv) j->spre.dim = 0;  // not queued: built again on the next use
-            return rc;
-        }
+        RCCHECK(queue_super_inverses(ctx, C.tabs, sinv, q));
         for (auto& u : C.active) {
             if (u->phase != kInit) continue;
             if (u->block)
@@ -1720,6 +1731,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         HIPCHECK(ctx, hipStreamWaitEvent(s, C.tab_ev, 0));
     }
     for (auto& f : q) HIPCHECK(ctx, f());
+    sinv_guard.armed = false;  // every inverse launch of the step is queued
     if (ds != s) {
         if (!C.kev) HIPCHECK(ctx, hipEventCreateWithFlags(&C.kev, hipEventDisableTiming));
         HIPCHECK(ctx, hipEventRecord(C.kev, s));
@@ -2580,15 +2592,27 @@ int fcg_super_precond_apply(fcg_ctx* ctx, int job, int dim, const double* q, dou
 
 int fcg_engine_create(fcg_ctx* ctx, const fcg_solve_opts* opts, fcg_engine** out) {
     if (!ctx || !opts || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
-    if (opts->precond != FCG_PRECOND_NONE && opts->precond != FCG_PRECOND_JACOBI && opts->precond != FCG_PRECOND_BLOCK &&
-        opts->precond != FCG_PRECOND_SUPER)
-        return fail(ctx, FCG_INVALID_ARGUMENT, "unsupported preconditioner");
+    // the reference's m_ucPrecondType bit set (SolveCapacitance.cpp:122-125, 1092-1108)
+    // reduced to the one preconditioner this engine runs
+    const int pbits = opts->precond;
+    if (pbits & ~(FCG_PRECOND_JACOBI | FCG_PRECOND_BLOCK | FCG_PRECOND_SUPER))
+        return fail(ctx, FCG_INVALID_ARGUMENT,
+                    (pbits & FCG_PRECOND_HIER) ? "hierarchical preconditioner (-ph) is not built"
+                                               : "unsupported preconditioner bits");
+    int precond = FCG_PRECOND_NONE;
+    if (pbits & FCG_PRECOND_SUPER)
+        precond = FCG_PRECOND_SUPER;
+    else if (pbits & FCG_PRECOND_BLOCK)
+        precond = FCG_PRECOND_BLOCK;
+    else if (pbits & FCG_PRECOND_JACOBI)
+        precond = FCG_PRECOND_JACOBI;
     if (opts->precision != FCG_PREC_FP64 && opts->precision != FCG_PREC_MIXED)
         return fail(ctx, FCG_INVALID_ARGUMENT, "unknown precision mode");
     if (!(opts->tol > 0.0)) return fail(ctx, FCG_INVALID_ARGUMENT, "tolerance must be positive");
     auto e = std::make_unique<fcg_engine>();
     e->ctx = ctx;
     e->opts = *opts;
+    e->opts.precond = precond;
     *out = e.release();
     return FCG_OK;
 }

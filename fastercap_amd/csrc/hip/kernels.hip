@@ -1352,7 +1352,8 @@ __global__ void __launch_bounds__(256) k_super_collect(SView sv) {
     if (g >= sv.nln) return;
     const int jb = find_job(sv.ln_off, sv.njobs, g);
     const SJob& J = sv.jobs[jb];
-    // a solve's lanes fit in 32 bits (k * m * lpe, at most 32 * 4096 * 64)
+    // a solve's lanes fit in 32 bits: the host rejects solves whose padded lane
+    // count (k m lpe rounded up to a wavefront) reaches 2^31
     const int local = (int)(g - sv.ln_off[jb]);
     const int lpe = J.lpe;
     const int grp = local >> __builtin_ctz(lpe);  // lpe is a power of two

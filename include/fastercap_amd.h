@@ -45,11 +45,18 @@ extern "C" {
 #define FCG_PREC_FP64 0  /* every coefficient and accumulation in FP64 */
 #define FCG_PREC_MIXED 1 /* FP32 far-field coefficients/products, FP64 near field + refinement */
 
-/* Preconditioner bit flags: values of AUTOREFINE_PRECOND_* (Solver/SolverGlobal.h:59-68). */
+/* Preconditioner bit flags: values of AUTOREFINE_PRECOND_* (Solver/SolverGlobal.h:59-68),
+ * so a caller passes CAutoRefGlobalVars::m_ucPrecondType unchanged.  fcg_solve_opts.precond
+ * is read as the reference reads that bit set (Solver/SolveCapacitance.cpp:122-125,
+ * 1092-1108): SUPER with anything else is SUPER alone (the reference drops JACOBI
+ * with a warning), BLOCK with JACOBI is BLOCK alone (the reference then runs a
+ * flexible GMRES it reports as unsupported), JACOBI is JACOBI, 0 is none.  HIER
+ * (the hierarchical preconditioner, -ph) is not built: FCG_INVALID_ARGUMENT. */
 #define FCG_PRECOND_NONE 0
 #define FCG_PRECOND_JACOBI 1
 #define FCG_PRECOND_BLOCK 2 /* block preconditioner (-pb<n>), fcg_solve_opts.precond_size */
-#define FCG_PRECOND_SUPER 3 /* two-level preconditioner (-ps<n>), fcg_solve_opts.precond_size */
+#define FCG_PRECOND_SUPER 4 /* two-level preconditioner (-ps<n>), fcg_solve_opts.precond_size */
+#define FCG_PRECOND_HIER 8  /* reserved: the reference's value for -ph, rejected here */
 
 /*
  * Geometry of one tree node (leaf panel, refined parent or super node), 96 bytes.

@@ -31,3 +31,47 @@ def test_exports(header, loader, listed):
 
 def test_node_struct_size():
     assert ctypes.sizeof(_ffi.NodeGeom) == 96
+
+
+def _defines(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    return {m.group(1): int(m.group(2)) for m in re.finditer(r"#define\s+(FCG_\w+)\s+(\d+)u?\b", text)}
+
+
+# Values of the reference's own macros, transcribed with their file:line:
+#   AUTOREFINE_PRECOND_NONE/JACOBI/BLOCK/SUPER/HIER  Solver/SolverGlobal.h:60-68
+#   FC_NORMAL_END .. FC_USER_BREAK                    Solver/SolverGlobal.h:39-49
+#   AUTOPANEL_IS_LEAF .. AUTOPANEL_IS_SUPER_NODE      Solver/AutoElement.h:44-48
+REFERENCE_VALUES = {
+    "FCG_PRECOND_NONE": 0, "FCG_PRECOND_JACOBI": 1, "FCG_PRECOND_BLOCK": 2, "FCG_PRECOND_SUPER": 4,
+    "FCG_PRECOND_HIER": 8,
+    "FCG_OK": 0, "FCG_GENERIC_ERROR": 1, "FCG_COMMAND_LINE_ERROR": 64, "FCG_CANNOT_OPEN_FILE": 66,
+    "FCG_OUT_OF_MEMORY": 71, "FCG_FILE_ERROR": 74, "FCG_EXCEPTION_ERROR": 98, "FCG_USER_BREAK": 125,
+    "FCG_NODE_LEAF": 1, "FCG_NODE_DIEL": 2, "FCG_NODE_OUTPERM_NORMAL_DIR": 4, "FCG_NODE_OUTPERM_ELEMENT_LEVEL": 8,
+    "FCG_NODE_SUPER": 128,
+}
+
+
+def test_boundary_values_are_the_references():
+    """A caller of the reference passes m_ucPrecondType, FC_* codes and m_ucType
+    flags unchanged: the header's values must be the reference's."""
+    d = _defines("fastercap_amd.h")
+    for name, val in REFERENCE_VALUES.items():
+        assert d.get(name) == val, (name, d.get(name), val)
+        if hasattr(_ffi, name):
+            assert getattr(_ffi, name) == val, name
+    # the precond bit set of the reference combines flags: each flag is one bit
+    for name in ("FCG_PRECOND_JACOBI", "FCG_PRECOND_BLOCK", "FCG_PRECOND_SUPER", "FCG_PRECOND_HIER"):
+        v = d[name]
+        assert v & (v - 1) == 0
+
+
+def test_cli_precond_values():
+    """The Python command line stores the reference's bit values."""
+    from fastercap_amd import cli
+
+    assert cli.parse_args(["-ps256", "x.txt"]).precond == _ffi.FCG_PRECOND_SUPER
+    assert cli.parse_args(["-pb16", "x.txt"]).precond == _ffi.FCG_PRECOND_BLOCK
+    assert cli.parse_args(["-ps256", "-pj", "x.txt"]).precond == _ffi.FCG_PRECOND_SUPER
+    assert cli.auto_preconditioner(10_000_000, 4)[0] == _ffi.FCG_PRECOND_SUPER
+    assert cli.auto_preconditioner(1000, 1)[0] == _ffi.FCG_PRECOND_JACOBI

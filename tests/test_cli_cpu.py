@@ -40,23 +40,23 @@ def test_block_preconditioner_switch():
 
 def test_two_level_preconditioner_switch(capsys):
     a = cli.parse_args(["x", "-ps512"])
-    assert a.precond == 3 and a.super_dim == 512
+    assert a.precond == 4 and a.super_dim == 512  # AUTOREFINE_PRECOND_SUPER
     # the reference clamps the dimension to 16..4096 with a warning
     assert cli.parse_args(["x", "-ps5000"]).super_dim == 4096
     assert cli.parse_args(["x", "-ps3"]).super_dim == 16
     # with Jacobi, in either order, only the two-level preconditioner is kept
     capsys.readouterr()
-    assert cli.parse_args(["x", "-pj", "-ps64"]).precond == 3
-    assert cli.parse_args(["x", "-ps64", "-pj"]).precond == 3
+    assert cli.parse_args(["x", "-pj", "-ps64"]).precond == 4
+    assert cli.parse_args(["x", "-ps64", "-pj"]).precond == 4
     assert capsys.readouterr().err.count("keeping only the two-levels preconditioner") == 2
     assert cli.parse_args(["x", "-ps64", "-pn"]).precond == 0
     assert cli.parse_args(["x", "-ps64", "-pb8"]).precond == 2
 
 
 @pytest.mark.parametrize("links,cond,want", [
-    (0, 5, (1, 128)), (250_999, 2, (1, 128)), (251_000, 2, (3, 128)),
-    (2_500_999, 2, (3, 128)), (2_501_000, 2, (3, 512)), (5_000_999, 2, (3, 512)),
-    (5_001_000, 2, (3, 1024)), (10_000_000_000, 1, (3, 1024)),
+    (0, 5, (1, 128)), (250_999, 2, (1, 128)), (251_000, 2, (4, 128)),
+    (2_500_999, 2, (4, 128)), (2_501_000, 2, (4, 512)), (5_000_999, 2, (4, 512)),
+    (5_001_000, 2, (4, 1024)), (10_000_000_000, 1, (4, 1024)),
 ])
 def test_auto_preconditioner_rule(links, cond, want):
     # CSolveCap::AutoSetPrecondType (Solver/SolveCapacitance.cpp:580-608): links are
