@@ -198,6 +198,7 @@ DEVICE_SYMBOLS = (
     "fcg_download_coeffs",
     "fcg_mvp",
     "fcg_mvp_device",
+    "fcg_mvp_device_noalloc",
     "fcg_precond_apply",
     "fcg_block_precond_apply",
     "fcg_super_precond_apply",
@@ -307,6 +308,7 @@ def device_lib() -> C.CDLL:
         lib.fcg_download_coeffs.argtypes = [vp, C.c_int, dp, dp]
         lib.fcg_mvp.argtypes = [vp, C.c_int, dp, dp, C.c_int]
         lib.fcg_mvp_device.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        lib.fcg_mvp_device_noalloc.argtypes = [vp, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
         lib.fcg_precond_apply.argtypes = [vp, C.c_int, dp, dp, C.c_int]
         lib.fcg_block_precond_apply.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]
         lib.fcg_super_precond_apply.argtypes = [vp, C.c_int, C.c_int, dp, dp, C.c_int]

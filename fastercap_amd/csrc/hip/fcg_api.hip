@@ -12,6 +12,7 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -309,11 +310,54 @@ struct Job {
 
 }  // namespace
 
+// Scratch of fcg_mvp_device_noalloc: one static device array, present on every
+// GPU the library runs on until the process ends.  Host code never uses its name
+// as an address: each device's copy has its own address, read once per device
+// with hipGetSymbolAddress (scratch_address) and passed to the kernels.
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// device address of `scratch` on `device`, which must be the current device
+hipError_t scratch_address(int device, void** out) {
+    *out = nullptr;
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+// The job table of a no-alloc product, written into the scratch array by a
+// kernel from its by-value argument (no staging buffer, no copy call).
+struct NoAllocTable {
+    BJob job;
+    int64_t off[10];  // link, tile, node, leaf and rhs offsets of a batch of one
+};
+static_assert(sizeof(NoAllocTable) % 8 == 0, "table copied in 8-byte words");
+constexpr size_t kNoAllocTableBytes = (sizeof(NoAllocTable) + 255) & ~(size_t)255;
+
+__global__ void k_noalloc_table(NoAllocTable t, uint64_t* __restrict__ dst) {
+    const uint64_t* src = reinterpret_cast<const uint64_t*>(&t);
+    for (unsigned i = threadIdx.x; i < sizeof(NoAllocTable) / 8; i += blockDim.x) dst[i] = src[i];
+}
+}  // namespace
+
 struct fcg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     hipStream_t copy_stream = nullptr;  // engine table uploads (default priority, like stream)
     hipStream_t down_stream = nullptr;  // engine result downloads (default priority)
+    void* noalloc = nullptr;            // this device's address of `scratch`
     bool profiling = false;
     fcg_profile prof{};
     // gather lane shape of products over several jobs, per product precision and
@@ -1872,6 +1916,7 @@ int fcg_ctx_create(int device, fcg_ctx** out) {
     if (hipSetDevice(device) != hipSuccess) return FCG_DEVICE_ERROR;
     auto c = std::make_unique<fcg_ctx>();
     c->device = device;
+    if (scratch_address(device, &c->noalloc) != hipSuccess || !c->noalloc) return FCG_DEVICE_ERROR;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FCG_DEVICE_ERROR;
     if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
@@ -2247,6 +2292,81 @@ int fcg_mvp_device(fcg_ctx* ctx, int job, const double* d_q, double* d_v, int k,
     if (!j || !d_q || !d_v || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     return run_mvp(ctx, *j, fcg::kMvp64, d_q, d_v, k, 0, stream ? (hipStream_t)stream : ctx->stream);
+}
+
+int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v, int k, void* stream) {
+    Job* j = get_job(ctx, job);
+    if (!j || !d_q || !d_v || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (!j->filled64) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in FP64");
+    if (!ctx->noalloc) return fail(ctx, FCG_DEVICE_ERROR, "no scratch address on this device");
+    // scratch layout: job table, leaf prefix sums [k][N+1], scan tile sums [k][nblk],
+    // node charges and potentials [M][kp], 2D fix-up values [2k]
+    const int N = j->N, M = j->M, kp = fcg::padded_k(k);
+    const int nblk = (N + fcg::kScanTile - 1) / fcg::kScanTile;
+    auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_prefix = kNoAllocTableBytes;
+    const size_t o_bsum = o_prefix + al(sizeof(DD) * (size_t)k * (N + 1));
+    const size_t o_charge = o_bsum + al(sizeof(DD) * (size_t)k * nblk);
+    const size_t o_pot = o_charge + al(sizeof(double) * (size_t)kp * M);
+    const size_t o_lt = o_pot + al(sizeof(double) * (size_t)kp * M);
+    const size_t need = o_lt + al(sizeof(double) * 2 * (size_t)k);
+    if (need > kNoAllocScratchBytes)
+        return fail(ctx, FCG_INVALID_ARGUMENT,
+                    "product needs " + std::to_string(need) + " bytes of scratch, the no-alloc limit is " +
+                        std::to_string(kNoAllocScratchBytes));
+    HIPCHECK(ctx, hipSetDevice(ctx->device));
+    hipStream_t s = stream ? (hipStream_t)stream : ctx->stream;
+    char* base = static_cast<char*>(ctx->noalloc);
+    Scratch sc;  // sizes only: the buffers are the scratch array's
+    sc.k = k;
+    sc.kp = kp;
+    sc.nblk = nblk;
+    NoAllocTable t{};
+    t.job = j->rec(sc, d_q, d_v);
+    t.job.prefix = reinterpret_cast<DD*>(base + o_prefix);
+    t.job.block_sums = reinterpret_cast<DD*>(base + o_bsum);
+    t.job.charge64 = reinterpret_cast<double*>(base + o_charge);
+    t.job.pot64 = reinterpret_cast<double*>(base + o_pot);
+    t.job.lt = reinterpret_cast<double*>(base + o_lt);
+    t.job.charge32 = nullptr;
+    t.job.pot32 = nullptr;
+    const int64_t sizes[5] = {j->L, (int64_t)k * nblk, M, N, k};
+    for (int i = 0; i < 5; ++i) {
+        t.off[2 * i] = 0;
+        t.off[2 * i + 1] = sizes[i];
+    }
+    k_noalloc_table<<<1, 64, 0, s>>>(t, reinterpret_cast<uint64_t*>(base));
+    HIPCHECK(ctx, hipGetLastError());
+    const auto* tab = reinterpret_cast<const NoAllocTable*>(base);
+    BatchView bv{};
+    bv.jobs = &tab->job;
+    bv.njobs = 1;
+    bv.link_off = tab->off;
+    bv.tile_off = tab->off + 2;
+    bv.node_off = tab->off + 4;
+    bv.leaf_off = tab->off + 6;
+    bv.rhs_off = tab->off + 8;
+    bv.links = j->L;
+    bv.tiles = (int64_t)k * nblk;
+    bv.nodes = M;
+    bv.leaves = N;
+    bv.rhs = k;
+    bv.kmax = k;
+    bv.kpmin = kp;
+    bv.any2d = j->dim == 2;
+    // the lane plan of a batch of this one job (make_batch + choose_variants)
+    fcg::GatherPlan gp;
+    gp.n = 1;
+    fcg::GatherSeg& g = gp.seg[0];
+    g.row_lo = 0;
+    g.row_hi = M;
+    g.kpmin = kp;
+    g.kmax = k;
+    g.cls = fcg::gather_class(kp);
+    g.idx16 = j->col16.p ? 1 : 0;
+    g.variant = j->gather_variant[fcg::kMvp64];
+    HIPCHECK(ctx, fcg::launch_mvp(bv, fcg::kMvp64, 0, gp, s, nullptr));
+    return FCG_OK;
 }
 
 int fcg_mvp_prec(fcg_ctx* ctx, int job, int mvp_prec, const double* q, double* v, int k) {

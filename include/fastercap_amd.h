@@ -157,6 +157,22 @@ int fcg_mvp(fcg_ctx* ctx, int job, const double* q, double* v, int k);
  * the context stream).  No host synchronisation. */
 int fcg_mvp_device(fcg_ctx* ctx, int job, const double* d_q, double* d_v, int k, void* stream);
 
+/* fcg_mvp_device that allocates nothing.  No reference counterpart: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same product, kernels and result as
+ * fcg_mvp_device (FP64 job, k right-hand sides, device buffers, enqueued on
+ * `stream`, NULL = the context stream, no host synchronisation); its scratch
+ * (the job table, leaf prefix sums 16 k (N+1) B, scan tile sums, node charges
+ * and potentials 2 x 8 kp M B with kp = k rounded up to a power of two) lives in
+ * one static device array of 32 MiB per GPU, held until the process ends,
+ * instead of the library's pool.  No hipMalloc, hipFree or allocating call.
+ * FCG_INVALID_ARGUMENT when the product needs more than that array (about
+ * 16 k N + 32 kp N bytes for M ~ 2N); FCG_INVALID_ARGUMENT as well when the job
+ * is not filled in FP64.  Every call on one GPU shares its array: calls that
+ * overlap on one GPU (different streams without ordering) race and must be
+ * serialised by the caller. */
+int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v, int k, void* stream);
+
 /* Product precisions of fcg_mvp_prec. */
 #define FCG_MVP_FP64 0       /* FP64 coefficients, charges and accumulation */
 #define FCG_MVP_FP32 1       /* FP32 coefficients, charges and accumulation (inner solves, mixed path) */
