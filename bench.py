@@ -29,8 +29,13 @@ import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# the CPU baseline's OpenMP threads: one per core, packed (read when the oracle's
+# OpenMP runtime starts, which is after this point)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 PEAK_HBM_GBS = 8000.0
+ACCURACY_BAR = 1e-4  # largest per-entry relative C error against the FP64 path (tol 1e-8)
 
 
 def log(msg):
@@ -52,6 +57,59 @@ def c_errors(caps, refs):
         pe = max(pe, float(np.max(d / np.abs(r))))
         pd = max(pd, float(np.max(d / np.abs(np.diag(r))[:, None])))
     return pe, pd
+
+
+def cpu_info():
+    """CPU model, sockets and cores of this host (lscpu), the OpenMP placement."""
+    import subprocess
+
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except Exception:
+        pass
+    sockets = int(info.get("Socket(s)", "0") or 0)
+    cps = int(info.get("Core(s) per socket", "0") or 0)
+    return {"model": info.get("Model name", "unknown"), "sockets": sockets, "cores_per_socket": cps,
+            "threads_per_core": int(info.get("Thread(s) per core", "0") or 0),
+            "host_cores": sockets * cps, "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+            "omp_places": os.environ.get("OMP_PLACES")}
+
+
+def copy_ceiling(mib=1024, reps=5):
+    """Achievable HBM rate of a plain device-to-device copy of `mib` MiB (read +
+    write bytes / time), through the HIP runtime the library links."""
+    import ctypes as C
+
+    from fastercap_amd import _ffi
+
+    _ffi.device_lib()
+    h = C.CDLL("libamdhip64.so.7")
+    h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    h.hipFree.argtypes = [C.c_void_p]
+    h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    h.hipDeviceSynchronize.argtypes = []
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    if h.hipMalloc(C.byref(a), n) != 0 or h.hipMalloc(C.byref(b), n) != 0:
+        return None
+    try:
+        h.hipMemcpy(b, a, n, 3)  # warm
+        h.hipDeviceSynchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            if h.hipMemcpy(b, a, n, 3) != 0:  # hipMemcpyDeviceToDevice
+                return None
+        h.hipDeviceSynchronize()
+        dt = (time.perf_counter() - t) / reps
+    finally:
+        h.hipFree(a)
+        h.hipFree(b)
+    return {"gbps": 2 * n / dt / 1e9, "bytes": n, "reps": reps,
+            "kind": "hipMemcpy device-to-device, read + write bytes"}
 
 
 def ncond(mesh):
@@ -111,6 +169,7 @@ def bench_batch(args, fc, dev, tmp):
     lat = []
     dev.profile(enable=1, reset=not args.only_mixed)
     t_start = time.perf_counter()
+    res = []
     for s in range(args.steps):
         a = time.perf_counter()
         dev.fill_batch(jobs, MIX)
@@ -124,6 +183,8 @@ def bench_batch(args, fc, dev, tmp):
     value = args.steps * nj / t_mixed
     mixed_caps = [r.cap for r in res]
     iters_mixed = [max(r.iterations) for r in res]
+    unconverged = sum(1 for r in res if not r.converged)
+    cycles_mixed = [r.cycles for r in res]
     if args.only_mixed:  # profiling aid: the mixed solves alone, warm-up included
         log(f"all products of the process: {prof}")
         return None
@@ -150,8 +211,17 @@ def bench_batch(args, fc, dev, tmp):
     e_conv = c_errors(mixed_caps, [r.cap for r in ref])
     e_plain = c_errors(mixed_caps, [r.cap for r in plain])
     e_plain_conv = c_errors([r.cap for r in plain], [r.cap for r in ref])
+    unconverged += sum(1 for r in plain + bres + ref if not r.converged)
     log(f"mixed vs FP64 tol 1e-8: per-entry {e_conv[0]:.3e} vs-diag {e_conv[1]:.3e}; "
         f"plain FP64 at tol {tol} vs tol 1e-8: {e_plain_conv[0]:.3e}")
+    # the mixed path's accuracy floor: mixed at tol 1e-8 against FP64 at tol 1e-8
+    dev.fill_batch(jobs, MIX)
+    floor = dev.solve_batch(jobs, meshes, tol=1e-8, precision=MIX, **PC)
+    unconverged += sum(1 for r in floor if not r.converged)
+    e_floor = c_errors([r.cap for r in floor], [r.cap for r in ref])
+    log(f"mixed floor (tol 1e-8 vs FP64 tol 1e-8): per-entry {e_floor[0]:.3e} vs-diag {e_floor[1]:.3e}; "
+        f"unconverged solves {unconverged}")
+    bar_met = e_conv[0] <= ACCURACY_BAR and unconverged == 0
 
     # dominant kernel: the FP32 gather of the inner products of the timed solves
     # (HIP events around the gather of every product, which is one launch per rhs
@@ -183,11 +253,15 @@ def bench_batch(args, fc, dev, tmp):
             "kind": "port",
             "sample": f"oracle fill + FP64 GMRES ({pc_name}) of every conductor of every {stride}th job "
                       f"({len(idx)} of {nj} jobs, {t_cpu:.1f}s)",
+            "host": cpu_info(),
         }
         log(f"cpu baseline: {cpu}")
 
+    ceiling = copy_ceiling()
+    metric = ("solves/s (batched uneven jobs, mixed precision, <=1e-4 rel C error vs FP64 path)" if bar_met else
+              "solves/s (batched uneven jobs, mixed precision; accuracy bar NOT met, see accuracy_bar_met)")
     return {
-        "metric": "solves/s (batched uneven jobs, mixed precision, <=1e-4 rel C error vs FP64 path)",
+        "metric": metric,
         "value": value,
         "unit": "solves/s",
         "ms_per_step": 1e3 * t_mixed / args.steps,
@@ -212,7 +286,15 @@ def bench_batch(args, fc, dev, tmp):
         "fp64_batched_solves_per_s": nj / t_b64,
         "speedup_vs_fp64_plain": value / (nj / t_plain),
         "gmres_iterations_max_per_job": {"p50": pct(iters_mixed, 50), "max": max(iters_mixed)},
+        "accuracy_bar_met": bar_met,
+        "accuracy_bar": f"max per-entry |C_mixed - C_FP64| / |C_FP64| <= {ACCURACY_BAR:g} (FP64 at tol 1e-8), "
+                        "every solve converged",
+        "unconverged_solves": unconverged,
+        "refine_cycles_mixed": {"p50": pct(cycles_mixed, 50), "max": max(cycles_mixed)},
         "c_rel_err_mixed_vs_fp64_tol1e-8": {"max_per_entry": e_conv[0], "max_vs_diagonal": e_conv[1]},
+        "mixed_floor_per_entry": e_floor[0],
+        "mixed_floor": "mixed at tol 1e-8 against FP64 at tol 1e-8 (max per entry)",
+        "c_rel_err_fp64_plain_vs_fp64_tol1e-8": {"max_per_entry": e_plain_conv[0], "max_vs_diagonal": e_plain_conv[1]},
         "c_rel_err_mixed_vs_fp64_plain_same_tol": {"max_per_entry": e_plain[0], "max_vs_diagonal": e_plain[1]},
         "mvp": {
             "inner_fp32": {"launches": g["mvp_launches"], "avg_ms": g["mvp_ms"] / max(1, g["mvp_launches"])},
@@ -233,6 +315,8 @@ def bench_batch(args, fc, dev, tmp):
             "kernel_launches": g["gather_kernels"],
             "avg_gather_ms_per_product": g["gather_ms"] / max(1, g["gather_launches"]),
             "alg_bytes_per_product": g["gather_bytes"] / max(1, g["gather_launches"]),
+            "achievable": ceiling["gbps"] if ceiling else None,
+            "achievable_source": ceiling,
         },
         "cpu_baseline": cpu,
     }
@@ -272,23 +356,29 @@ def bench_serve(args, fc, dev, tmp):
         arr = serving.poisson_arrivals(rate, max(8, int(rate * window)), jobs, seed)
         rep = srv.run(arr)
         srv.close()
-        errs = [float(np.max(np.abs(r.result.cap - refs[r.job]) / np.abs(np.diag(refs[r.job]))[:, None]))
-                for r in rep.requests]
-        return rep, max(errs) if errs else float("nan")
+        errs = [c_errors([r.result.cap], [refs[r.job]]) for r in rep.requests]
+        unconv = sum(1 for r in rep.requests if not r.result.converged)
+        if not errs:
+            return rep, (float("nan"), float("nan")), unconv
+        return rep, (max(e[0] for e in errs), max(e[1] for e in errs)), unconv
 
     serve(cap_rate, 2.0, 1)  # warm
     dev.profile(enable=1, reset=True)
     rate = args.serve_load * cap_rate
-    rep, err = serve(rate, args.serve_window, 7)
+    rep, err, unc = serve(rate, args.serve_window, 7)
     prof = dev.profile(enable=0, reset=True)
     log(f"load {args.serve_load:.2f}x capacity ({rate:.1f}/s): {rep.completed} solves in {rep.elapsed_s:.1f}s = "
-        f"{rep.solves_per_s:.1f}/s, latency {rep.latency_ms}, max active {rep.max_active}, C err {err:.2e}")
-    half, err_h = serve(0.5 * cap_rate, args.serve_window / 2, 11)
+        f"{rep.solves_per_s:.1f}/s, latency {rep.latency_ms}, max active {rep.max_active}, C err {err}")
+    half, err_h, unc_h = serve(0.5 * cap_rate, args.serve_window / 2, 11)
+    bar_met = max(err[0], err_h[0]) <= ACCURACY_BAR and unc + unc_h == 0
     log(f"half load: {half.solves_per_s:.1f}/s, latency {half.latency_ms}")
     g = prof["fp32"]
     achieved = g["gather_bytes"] / (g["gather_ms"] * 1e-3) / 1e9 if g["gather_ms"] > 0 else 0.0
     return {
-        "metric": "served solves/s (Poisson arrivals over resident jobs, mixed precision, <=1e-4 rel C error)",
+        "metric": ("served solves/s (Poisson arrivals over resident jobs, mixed precision, <=1e-4 rel C error)"
+                   if bar_met else "served solves/s (Poisson arrivals, mixed precision; accuracy bar NOT met)"),
+        "accuracy_bar_met": bar_met,
+        "unconverged_solves": unc + unc_h,
         "value": rep.solves_per_s,
         "unit": "solves/s",
         "ms_per_step": 1e3 * rep.elapsed_s / max(1, rep.steps),
@@ -310,7 +400,7 @@ def bench_serve(args, fc, dev, tmp):
         "half_load": {"rate_per_s": 0.5 * cap_rate, "solves_per_s": half.solves_per_s, "latency_ms": half.latency_ms},
         "closed_batch_capacity_solves_per_s": cap_rate,
         "max_active": rep.max_active,
-        "c_rel_err_vs_fp64_tol1e-8_max_vs_diagonal": max(err, err_h),
+        "c_rel_err_vs_fp64_tol1e-8": {"max_per_entry": max(err[0], err_h[0]), "max_vs_diagonal": max(err[1], err_h[1])},
         "roofline": {"kernel": "k_gather<float,float,float,*> (inner FP32 products while serving)", "bound": "hbm",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": None, "products": g["gather_launches"], "kernel_launches": g["gather_kernels"],
@@ -429,7 +519,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="batch", choices=["batch", "serve", "bus", "plates", "boxes"])
     ap.add_argument("--serve-jobs", type=int, default=32)
-    ap.add_argument("--serve-max-panels", type=int, default=200_000)
+    ap.add_argument("--serve-max-panels", type=int, default=1_000_000)
     ap.add_argument("--serve-load", type=float, default=1.5, help="arrival rate / closed-batch capacity")
     ap.add_argument("--serve-window", type=float, default=20.0, help="seconds of arrivals")
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"], help="single-job workloads")
@@ -453,7 +543,9 @@ def main():
                                    "the design notes, not run", "n_gpus": args.gpus}), flush=True)
         sys.exit(2)
     if args.tol is None:
-        args.tol = 1e-5 if args.workload in ("batch", "serve", "boxes") else 1e-4
+        # the batch and serving workloads are quoted at <=1e-4 per-entry C error
+        # against FP64, which GMRES tol 1e-6 meets on the uneven batch
+        args.tol = 1e-6 if args.workload in ("batch", "serve", "boxes") else 1e-4
 
     if args.calibrate_copy_mib > 0:
         import torch
@@ -498,6 +590,9 @@ def main():
     line.update(out)
     print(json.dumps(line), flush=True)
     dev.close()
+    if line.get("unconverged_solves"):
+        log(f"{line['unconverged_solves']} solves ended without meeting their tolerance")
+        sys.exit(1)
 
 
 if __name__ == "__main__":

@@ -173,6 +173,8 @@ class SolveResult:
     t_mvp_s: float = 0.0
     n_mvp: int = 0
     charges: np.ndarray | None = None
+    converged: bool = True  # every rhs met the tolerance (mixed: the FP64 residual check)
+    cycles: int = 0  # inner GMRES cycles (mixed: refinement cycles)
 
 
 def _dptr(a: np.ndarray):
@@ -345,6 +347,8 @@ class Device:
             cap=cap[: r * c].reshape(r, c),
             iterations=[st.iterations[i] for i in range(min(r, 64))],
             residual=st.final_residual,
+            converged=bool(st.converged),
+            cycles=int(st.cycles),
             t_solve_s=st.t_solve_s,
             t_mvp_s=st.t_mvp_s,
             n_mvp=st.n_mvp,
@@ -386,6 +390,8 @@ class Device:
                     cap=caps[off : off + r * c].reshape(r, c).copy(),
                     iterations=[st.iterations[q] for q in range(min(r, 64))],
                     residual=st.final_residual,
+                    converged=bool(st.converged),
+                    cycles=int(st.cycles),
                     t_solve_s=st.t_solve_s,
                     n_mvp=st.n_mvp,
                 )
@@ -517,6 +523,8 @@ class Engine:
             cap=cap[: r * c].reshape(r, c).copy(),
             iterations=[st.iterations[q] for q in range(min(r, 64))],
             residual=st.final_residual,
+            converged=bool(st.converged),
+            cycles=int(st.cycles),
             t_solve_s=st.t_solve_s,
             n_mvp=st.n_mvp,
         )

@@ -110,6 +110,8 @@ class SolveStats(C.Structure):
         ("t_solve_s", C.c_double),
         ("t_mvp_s", C.c_double),
         ("n_mvp", C.c_int64),
+        ("converged", C.c_int32),
+        ("cycles", C.c_int32),
     ]
 
 

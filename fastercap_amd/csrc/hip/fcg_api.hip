@@ -240,6 +240,10 @@ struct Job {
         bool avail = false;
         DevBuf snode, sstart, sidx, Sinv, w;
     } spre;
+    // FP64 near-field side list (built on the first mixed fill; kernels.h BJob)
+    DevBuf near_ptr, near_col, near_link, near_dc;
+    int64_t Ln = 0;
+    bool near_ready = false;
     std::vector<uint32_t> link_pos;  // descriptor-order offset of each stored link (empty: rows by node id)
     std::vector<int32_t> h_iperm;     // charge row -> node
     bool filled64 = false, filled32 = false;
@@ -260,7 +264,7 @@ struct Job {
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
                          col16.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
-                         scr.bytes());
+                         near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + scr.bytes());
     }
 
     BJob rec(const Scratch& sc, const double* q, double* v, const float* q32 = nullptr) const {
@@ -285,6 +289,10 @@ struct Job {
         b.link_row = link_row.as<int32_t>();
         b.coef64 = coef64.as<double>();
         b.coef32 = coef32.as<float>();
+        b.near_ptr = near_ready ? near_ptr.as<int64_t>() : nullptr;
+        b.near_col = near_col.as<int32_t>();
+        b.near_link = near_link.as<uint32_t>();
+        b.near_dc = near_dc.as<double>();
         b.diag = diag.as<double>();
         b.self = self.as<double>();
         b.left = left.as<int32_t>();
@@ -562,6 +570,8 @@ double gather_alg_bytes(const std::vector<BatchEntry>& ents, int prec) {
     for (const auto& e : ents) {
         const double bi = e.j->col16.p ? 2.0 : 4.0;  // source index width
         b += (double)e.j->L * (bc + bi) + 16.0 * e.j->M + 8.0 + 2.0 * e.j->M * (double)e.sc->k * bn;
+        // the residual product's FP64 near-field correction: its side list
+        if (prec == fcg::kMvp32Acc && e.j->near_ready) b += 12.0 * (double)e.j->Ln + 8.0 * (e.j->M + 1);
     }
     return b;
 }
@@ -802,6 +812,51 @@ void build_fill_order(const fcg_job_desc* d, const std::vector<int32_t>& iperm,
         });
     }
     for (auto& x : th) x.join();
+}
+
+// FP64 near-field side list of a job, built on its first mixed fill: the links of
+// the analytic PotEstimateOpt branch (3D fill classes 0 and 5, centroid distance
+// below one max side; 2D classes 10 and 12, below four), whose log and atan terms
+// cancel, row by row in stored link order.  Their fill_order ranges are flagged,
+// counted per row, scanned on the host and scattered.
+int ensure_near(fcg_ctx* ctx, Job& j) {
+    if (j.near_ready) return FCG_OK;
+    hipStream_t s = ctx->stream;
+    const int M = j.M;
+    int64_t fstart[kFillClasses + 1];
+    fstart[0] = 0;
+    for (int c = 0; c < kFillClasses; ++c) fstart[c + 1] = fstart[c] + j.fill_cnt[(size_t)c];
+    DevBuf flag, cnt;
+    HIPCHECK(ctx, flag.alloc((size_t)std::max<int64_t>(j.L, 1)));
+    HIPCHECK(ctx, hipMemsetAsync(flag.p, 0, (size_t)std::max<int64_t>(j.L, 1), s));
+    int64_t want = 0;
+    for (int c = 0; c < kFillClasses; ++c) {
+        if (!fcg::near_class(c) || j.fill_cnt[(size_t)c] == 0) continue;
+        HIPCHECK(ctx, fcg::launch_near_flag(j.fill_order.as<uint32_t>(), fstart[c], j.fill_cnt[(size_t)c],
+                                            flag.as<uint8_t>(), s));
+        want += j.fill_cnt[(size_t)c];
+    }
+    HIPCHECK(ctx, cnt.alloc(sizeof(int32_t) * (size_t)M));
+    HIPCHECK(ctx, fcg::launch_near_count(j.row_ptr.as<int64_t>(), flag.as<uint8_t>(), M, cnt.as<int32_t>(), s));
+    std::vector<int32_t> h((size_t)M);
+    HIPCHECK(ctx, hipMemcpyAsync(h.data(), cnt.p, sizeof(int32_t) * (size_t)M, hipMemcpyDeviceToHost, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    std::vector<int64_t> ptr((size_t)M + 1, 0);
+    for (int i = 0; i < M; ++i) ptr[(size_t)i + 1] = ptr[(size_t)i] + h[(size_t)i];
+    const int64_t Ln = ptr[(size_t)M];
+    if (Ln != want) return fail(ctx, FCG_DEVICE_ERROR, "near-field side list: counts do not add up");
+    HIPCHECK(ctx, upload(j.near_ptr, ptr.data(), ptr.size(), s));
+    HIPCHECK(ctx, j.near_col.alloc(sizeof(int32_t) * (size_t)Ln));
+    HIPCHECK(ctx, j.near_link.alloc(sizeof(uint32_t) * (size_t)Ln));
+    HIPCHECK(ctx, j.near_dc.alloc(sizeof(double) * (size_t)Ln));
+    if (Ln > 0)
+        HIPCHECK(ctx, fcg::launch_near_scatter(j.row_ptr.as<int64_t>(), flag.as<uint8_t>(), j.col.as<int32_t>(), M,
+                                               j.near_ptr.as<int64_t>(), j.near_col.as<int32_t>(),
+                                               j.near_link.as<uint32_t>(), s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));  // flag, cnt and the host vectors are released here
+    j.Ln = Ln;
+    j.near_ready = true;
+    return FCG_OK;
 }
 
 // Block preconditioner partition (RecurseComputePrecond,
@@ -1149,6 +1204,7 @@ struct Solve {
     int phase = kInit;
     int i = 0;  // Arnoldi step index within the cycle (the next product uses Q[i])
     int cycles = 0;
+    bool converged = false;     // every rhs met the tolerance (see fcg_solve_stats)
     bool need_scale = false;    // Q[i] = src / d pending
     bool scale_from_r = false;  // src is r (cycle start) or w (continuation)
     std::vector<double> d;      // scale divisors per rhs
@@ -1831,6 +1887,8 @@ int engine_finish(fcg_engine* E, Cohort& C) {
         rd.stats.max_iterations = mx;
         rd.stats.final_residual = S.rel;
         rd.stats.n_mvp = S.n_mvp;
+        rd.stats.converged = S.converged ? 1 : 0;
+        rd.stats.cycles = S.cycles;
         rd.stats.t_solve_s = std::chrono::duration<double>(clk::now() - S.t0).count();
         E->ready.push_back(std::move(rd));
         S.phase = kDone;
@@ -1845,7 +1903,12 @@ int engine_finish(fcg_engine* E, Cohort& C) {
             worst = std::max(worst, rel);
             if (!(rel < E->opts.tol)) all = false;
         }
-        if (S->mixed) S->rel = worst;
+        if (S->mixed) {
+            S->rel = worst;
+            S->converged = all;
+        }
+        // a mixed solve that reaches max_refine cycles finishes unconverged
+        // (stats.converged = 0, final_residual above tol)
         if (S->mixed && (all || S->cycles >= std::max(1, E->opts.max_refine)))
             S->phase = kFinish;
         else
@@ -1864,7 +1927,10 @@ int engine_finish(fcg_engine* E, Cohort& C) {
             const int it = S->last_i[(size_t)q];
             if (it >= 0) worst = std::max(worst, std::fabs(S->g[(size_t)q][(size_t)it + 1]) / S->normb[(size_t)q]);
         }
-        if (!S->mixed) S->rel = worst;
+        if (!S->mixed) {
+            S->rel = worst;
+            S->converged = all;
+        }
         if (all || S->i + 1 >= S->cap_iter) {
             prepare_update(*S);
         } else {
@@ -2189,6 +2255,12 @@ int fcg_job_footprint(fcg_ctx* ctx, int job, int precision, int iters, int64_t* 
     const int64_t k = (int64_t)j->rhs.size();
     int64_t res = j->resident_bytes();
     if (precision == FCG_PREC_MIXED && !j->coef32.p) res += 4 * j->L;
+    if (precision == FCG_PREC_MIXED && !j->near_ready) {
+        int64_t ln = 0;
+        for (int c = 0; c < kFillClasses; ++c)
+            if (fcg::near_class(c)) ln += j->fill_cnt[(size_t)c];
+        res += 16 * ln + 8 * ((int64_t)j->M + 1);
+    }
     if (precision != FCG_PREC_MIXED && !j->coef64.p) res += 8 * j->L;
     if (resident) *resident = res;
     // vectors b, x, r, w and the basis, plus the solve's own product scratch
@@ -2211,6 +2283,7 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
             if (e.j == j) return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
         if (precision == FCG_PREC_MIXED && !j->coef32.p)
             HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)(j->L + kLinkSpare)));
+        if (precision == FCG_PREC_MIXED) RCCHECK(ensure_near(ctx, *j));
         if (precision == FCG_PREC_FP64 && !j->coef64.p)
             HIPCHECK(ctx, j->coef64.alloc(sizeof(double) * (size_t)(j->L + kLinkSpare)));
         ents.push_back(BatchEntry{j, &j->scr, nullptr, nullptr});
@@ -2232,9 +2305,20 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
         HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
         offp[c] = reinterpret_cast<const int64_t*>(d);
     }
+    // mixed: the analytic classes from the near-field side lists
+    const int64_t* near_off = nullptr;
+    int64_t near_total = 0;
+    if (precision == FCG_PREC_MIXED) {
+        std::vector<int64_t> off(order.size() + 1, 0);
+        for (size_t i = 0; i < order.size(); ++i) off[i + 1] = off[i] + ents[(size_t)order[i]].j->Ln;
+        near_total = off.back();
+        void* d;
+        HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
+        near_off = reinterpret_cast<const int64_t*>(d);
+    }
     // no wait: later work on the context's stream runs after the fill, and every
     // reuse of the context's tables waits for the stream first
-    HIPCHECK(ctx, fcg::launch_fill(bv, offp, total, precision, s));
+    HIPCHECK(ctx, fcg::launch_fill(bv, offp, total, precision, near_off, near_total, s));
     for (auto& e : ents) {
         if (precision == FCG_PREC_MIXED)
             e.j->filled32 = true;

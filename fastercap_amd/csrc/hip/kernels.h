@@ -49,6 +49,13 @@ struct BJob {
     const int32_t* link_row;  // [L] destination node of each link (fill only)
     double* coef64;           // [L] or null
     float* coef32;            // [L] or null
+    // FP64 near-field side list (mixed precision): the links of the analytic
+    // PotEstimateOpt branch, row by row in stored link order; near_dc is the
+    // difference between the FP64 coefficient and its FP32 copy in coef32
+    const int64_t* near_ptr;    // [M+1] or null
+    const int32_t* near_col;    // [Ln] source charge row
+    const uint32_t* near_link;  // [Ln] stored link index
+    double* near_dc;            // [Ln]
     double* diag;             // [M] self coefficient of leaves, 0 elsewhere
     double* self;             // [N]
     const int32_t* left;
@@ -129,8 +136,19 @@ constexpr int kFillClasses = 14;
 // off[c] (device, njobs + 1 entries) is the prefix over the batch's jobs of their
 // class-c link counts, total[c] its last entry.
 hipError_t launch_link_rows(const BatchView& b, hipStream_t s);
+// In mixed precision the analytic classes (near_class) are filled from the jobs'
+// near-field side lists instead: near_off (device, njobs + 1) is the prefix of
+// their near link counts, near_total its last entry.
 hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int64_t* total, int precision,
-                       hipStream_t s);
+                       const int64_t* near_off, int64_t near_total, hipStream_t s);
+inline bool near_class(int c) { return c == 0 || c == 5 || c == 10 || c == 12; }
+// Near-field side list of one job: flag[l] = 1 for the fill_order entries
+// [p0, p0 + n); per-row counts of flagged links; the rows' flagged links
+// (charge-row column, link index) scattered at near_ptr.
+hipError_t launch_near_flag(const uint32_t* fill_order, int64_t p0, int64_t n, uint8_t* flag, hipStream_t s);
+hipError_t launch_near_count(const int64_t* row_ptr, const uint8_t* flag, int M, int32_t* cnt, hipStream_t s);
+hipError_t launch_near_scatter(const int64_t* row_ptr, const uint8_t* flag, const int32_t* col, int M,
+                               const int64_t* near_ptr, int32_t* near_col, uint32_t* near_link, hipStream_t s);
 
 // Rhs classes of the gather: jobs whose node-major row stride is kp = 2^c share
 // class c (strides beyond the last class share it).  A batch lists its jobs

@@ -168,6 +168,60 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __res
     }
 }
 
+// Mixed precision, analytic branch: the coefficient integrated in FP64 once,
+// stored as its FP32 copy (read by every FP32 product) and, in the job's near-field
+// side list, the difference to FP64 (added by the refinement residual's product).
+// One near link per lane; the row node's dielectric flag picks the kernel of
+// PotEstimateOpt's conductor or dielectric branch as the fill class does.
+__global__ void __launch_bounds__(256) k_fill_near(BatchView bv, const int64_t* __restrict__ off, int64_t total) {
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t g = g0 + threadIdx.x;
+    const int jb = find_job_block(off, bv.njobs, g0, g < total ? g : total - 1);
+    if (g >= total) return;
+    const BJob& J = bv.jobs[jb];
+    const int64_t sl = g - off[jb];
+    const uint32_t l = J.near_link[sl];
+    const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
+    const fcg_node_geom& p2 = J.nodes[J.iperm[J.col[l]]];
+    const bool diel = (p1.flags & FCG_NODE_DIEL) != 0;
+    double c;
+    if (J.dim == 2)
+        c = diel ? link_coef<double, 2, true, kBrAnalytic>(p1, p2) : link_coef<double, 2, false, kBrAnalytic>(p1, p2);
+    else
+        c = diel ? link_coef<double, 3, true, kBrAnalytic>(p1, p2) : link_coef<double, 3, false, kBrAnalytic>(p1, p2);
+    const float c32 = (float)c;
+    J.coef32[l] = c32;
+    J.near_dc[sl] = c - (double)c32;
+}
+
+__global__ void k_near_flag(const uint32_t* __restrict__ fill_order, int64_t p0, int64_t n, uint8_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[fill_order[p0 + i]] = 1;
+}
+
+__global__ void k_near_count(const int64_t* __restrict__ row_ptr, const uint8_t* __restrict__ flag, int M,
+                             int32_t* __restrict__ cnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    int32_t c = 0;
+    for (int64_t l = row_ptr[i]; l < row_ptr[i + 1]; ++l) c += flag[l];
+    cnt[i] = c;
+}
+
+__global__ void k_near_scatter(const int64_t* __restrict__ row_ptr, const uint8_t* __restrict__ flag,
+                               const int32_t* __restrict__ col, int M, const int64_t* __restrict__ near_ptr,
+                               int32_t* __restrict__ near_col, uint32_t* __restrict__ near_link) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M) return;
+    int64_t o = near_ptr[i];
+    for (int64_t l = row_ptr[i]; l < row_ptr[i + 1]; ++l)
+        if (flag[l]) {
+            near_col[o] = col[l];
+            near_link[o] = (uint32_t)l;
+            ++o;
+        }
+}
+
 __global__ void k_link_rows(BatchView bv) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= bv.nodes) return;
@@ -605,16 +659,70 @@ static void fill_class(const BatchView& b, const int64_t* off, int64_t n, int c,
 }
 
 hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int64_t* total, int precision,
-                       hipStream_t s) {
+                       const int64_t* near_off, int64_t near_total, hipStream_t s) {
+    const bool mixed = precision == FCG_PREC_MIXED;
     for (int c = 0; c < kFillClasses; ++c) {
         if (total[c] <= 0) continue;
-        if (precision == FCG_PREC_MIXED)
+        if (mixed && near_off && near_class(c)) continue;  // filled from the side lists below
+        if (mixed)
             fill_class<true>(b, off[c], total[c], c, s);
         else
             fill_class<false>(b, off[c], total[c], c, s);
     }
+    if (mixed && near_off && near_total > 0)
+        k_fill_near<<<blocks_for((size_t)near_total, 256), 256, 0, s>>>(b, near_off, near_total);
     if (b.leaves > 0) k_self<<<blocks_for((size_t)b.leaves, 256), 256, 0, s>>>(b);
     return hipGetLastError();
+}
+
+hipError_t launch_near_flag(const uint32_t* fill_order, int64_t p0, int64_t n, uint8_t* flag, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    k_near_flag<<<blocks_for((size_t)n, 256), 256, 0, s>>>(fill_order, p0, n, flag);
+    return hipGetLastError();
+}
+
+hipError_t launch_near_count(const int64_t* row_ptr, const uint8_t* flag, int M, int32_t* cnt, hipStream_t s) {
+    k_near_count<<<blocks_for((size_t)M, 256), 256, 0, s>>>(row_ptr, flag, M, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_near_scatter(const int64_t* row_ptr, const uint8_t* flag, const int32_t* col, int M,
+                               const int64_t* near_ptr, int32_t* near_col, uint32_t* near_link, hipStream_t s) {
+    k_near_scatter<<<blocks_for((size_t)M, 256), 256, 0, s>>>(row_ptr, flag, col, M, near_ptr, near_col, near_link);
+    return hipGetLastError();
+}
+
+// FP64 near-field correction of the refinement residual's product (FP32
+// coefficients, FP64 charges and potentials): each node adds to its potentials
+// the near links' FP64-minus-FP32 coefficient differences times the source
+// charges, so the near field enters in FP64 and the far field in FP32.  One node
+// per lane, its near links in stored order (deterministic sums).
+template <int KC>
+__global__ void k_near_correct(BatchView bv) {
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    const int jb = find_job_block(bv.node_off, bv.njobs, g0, g < bv.nodes ? g : bv.nodes - 1);
+    if (g >= bv.nodes) return;
+    const BJob& J = bv.jobs[jb];
+    if (!J.near_ptr) return;
+    const int i = (int)(g - bv.node_off[jb]);
+    const int64_t b = J.near_ptr[i], e = J.near_ptr[i + 1];
+    if (b == e) return;
+    double* pot = J.pot64 + (size_t)J.perm[i] * J.kp;
+    for (int r0 = 0; r0 < J.k; r0 += KC) {
+        double acc[KC];
+#pragma unroll
+        for (int r = 0; r < KC; ++r) acc[r] = 0.0;
+        for (int64_t sl = b; sl < e; ++sl) {
+            const double w = J.near_dc[sl];
+            const double* ch = J.charge64 + (size_t)J.near_col[sl] * J.kp + r0;
+#pragma unroll
+            for (int r = 0; r < KC; ++r)
+                if (r0 + r < J.k) acc[r] += w * ch[r];
+        }
+#pragma unroll
+        for (int r = 0; r < KC; ++r)
+            if (r0 + r < J.k) pot[r0 + r] += acc[r];
+    }
 }
 
 hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan& plan, hipStream_t s, hipEvent_t* ev) {
@@ -640,6 +748,7 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
             ge = gather_dispatch<float, double, double>(b, plan.seg[i], s);
         if (ge != hipSuccess) return ge;
     }
+    if (prec == kMvp32Acc) k_near_correct<8><<<blocks_for((size_t)b.nodes, 256), 256, 0, s>>>(b);
     if (ev) (void)hipEventRecord(ev[3], s);
     const unsigned lb = blocks_for((size_t)b.leaves, 256);
     if (prec == kMvp32)

@@ -43,7 +43,9 @@ class ServeReport:
     queue_ms: dict
     max_active: int
     steps: int
-    peak_solve_bytes: int
+    peak_solve_bytes: int  # admitted solve bytes (footprint estimates) at their peak
+    peak_engine_bytes: int  # device bytes the engine's active solves held at their peak
+    resident_bytes: int  # resident bytes of the registered jobs
     requests: list = field(default_factory=list)
 
 
@@ -85,6 +87,7 @@ class Server:
         resident_total = sum(self.resident.values())
         active_bytes = 0
         peak = 0
+        peak_dev = 0
         steps = 0
         max_act = 0
         t0 = time.perf_counter()
@@ -114,6 +117,7 @@ class Server:
                 continue
             self.engine.step()
             steps += 1
+            peak_dev = max(peak_dev, self.engine.bytes())
             while True:
                 got = self.engine.poll()
                 if got is None:
@@ -136,6 +140,8 @@ class Server:
             max_active=max_act,
             steps=steps,
             peak_solve_bytes=peak,
+            peak_engine_bytes=peak_dev,
+            resident_bytes=resident_total,
             requests=done,
         )
 

@@ -121,6 +121,10 @@ typedef struct fcg_solve_stats {
     double final_residual;  /* largest relative residual estimate at exit */
     double t_fill_s, t_solve_s, t_mvp_s;
     int64_t n_mvp;          /* number of hierarchical products executed */
+    int32_t converged;      /* 1: every right-hand side met the tolerance (FP64: the GMRES
+                               estimate; mixed: the FP64 residual of the last refinement
+                               check); 0: the iteration cap or max_refine ended the solve */
+    int32_t cycles;         /* inner GMRES cycles (mixed: refinement cycles) */
 } fcg_solve_stats;
 
 typedef struct fcg_ctx fcg_ctx;
