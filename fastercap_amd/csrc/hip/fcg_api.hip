@@ -221,7 +221,7 @@ struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
-        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16;
+        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16, grow;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -262,7 +262,7 @@ struct Job {
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
-                         col16.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
+                         col16.bytes + grow.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
                          near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + scr.bytes());
     }
@@ -284,6 +284,7 @@ struct Job {
         b.perm = perm.as<int32_t>();
         b.iperm = iperm.as<int32_t>();
         b.rorder = rorder.as<int32_t>();
+        b.grow = grow.as<fcg::GRow>();
         b.fill_order = fill_order.as<uint32_t>();
         b.fill_start = fill_start.as<int64_t>();
         b.link_row = link_row.as<int32_t>();
@@ -2039,6 +2040,20 @@ int upload_tree(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
     return FCG_OK;
 }
 
+// gather row records (fcg::GRow) of a job: for every row in gather order, its
+// link range, charge row and node (link offsets fit in 32 bits: validate_desc)
+int upload_grow(fcg_ctx* ctx, Job& j, const int64_t* row_ptr, const std::vector<int32_t>& perm,
+                const std::vector<int32_t>& rorder) {
+    std::vector<fcg::GRow> g((size_t)j.M);
+    for (int i = 0; i < j.M; ++i) {
+        const int node = rorder[(size_t)i];
+        g[(size_t)i] = fcg::GRow{(uint32_t)row_ptr[node], (uint32_t)row_ptr[node + 1], perm[(size_t)node], node};
+    }
+    HIPCHECK(ctx, upload(j.grow, g.data(), g.size(), ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));  // g is released here
+    return FCG_OK;
+}
+
 // groups, right-hand sides, self and diagonal storage and the link rows (both
 // layouts); returns synchronised
 int upload_finish(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
@@ -2135,6 +2150,7 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         HIPCHECK(ctx, upload(j->perm, perm.data(), perm.size(), s));
         HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
         HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
+        RCCHECK(upload_grow(ctx, *j, d->row_ptr, perm, rorder));
         std::vector<uint32_t> forder;
         build_fill_order(d, iperm, cols, forder, j->fill_cnt);
         if (!forder.empty()) HIPCHECK(ctx, upload(j->fill_order, forder.data(), forder.size(), s));
@@ -2180,6 +2196,7 @@ int fcg_job_upload_links(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     HIPCHECK(ctx, upload(j->perm, perm.data(), perm.size(), s));
     HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
     HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
+    RCCHECK(upload_grow(ctx, *j, dd.row_ptr, perm, rorder));
     // stored columns: the source's charge row, every row ascending
     HIPCHECK(ctx, j->col.alloc(sizeof(int32_t) * (size_t)(L + kLinkSpare)));
     HIPCHECK(ctx, hipMemsetAsync(j->col.as<int32_t>() + L, 0, sizeof(int32_t) * kLinkSpare, s));

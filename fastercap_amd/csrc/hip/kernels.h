@@ -30,6 +30,13 @@ enum MvpPrec : int {
     kMvp32Acc = 2,  // FP32 coefficients, FP64 charges and accumulation (refinement residual)
 };
 
+// One destination row of the gather, in gather row order: its links
+// [beg, end) in the stored link arrays, its charge / potential row and its node.
+struct GRow {
+    uint32_t beg, end;
+    int32_t prow, node;
+};
+
 // Device record of one job in a batch.
 struct BJob {
     int N, M, k, dim;
@@ -44,6 +51,7 @@ struct BJob {
     const int32_t* perm;      // [M] node -> slot of its charge / potential row
     const int32_t* iperm;     // [M] slot -> node
     const int32_t* rorder;    // [M] gather row order -> node
+    const GRow* grow;         // [M] the rows in gather row order
     const uint32_t* fill_order;  // [L] links grouped by fill class
     const int64_t* fill_start;   // [kFillClasses + 1] first fill_order entry of each class
     const int32_t* link_row;  // [L] destination node of each link (fill only)

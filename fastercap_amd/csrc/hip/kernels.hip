@@ -34,12 +34,36 @@ __device__ __forceinline__ int find_job(const int64_t* __restrict__ off, int njo
     return lo;
 }
 
+// find_job for a g that is the same on every lane of the calling wavefront (all
+// 64 lanes active): a 64-way search, each step testing 64 evenly spaced offsets at
+// once, so a batch of up to 4096 jobs takes two dependent loads instead of the
+// dozen of a binary search.  off ascends with off[0] = 0 <= g; the result is the
+// last index i < njobs with off[i] <= g.
+__device__ __forceinline__ int find_job_wave(const int64_t* __restrict__ off, int njobs, int64_t g) {
+    const int lane = threadIdx.x & 63;
+    int lo = 0, hi = njobs;  // the answer lies in [lo, hi)
+    while (hi - lo > 1) {
+        const int step = (hi - lo + 63) >> 6;
+        const int idx = lo + lane * step;
+        const bool ok = idx < hi && off[idx] <= g;  // true on a prefix of the lanes (lane 0 at least)
+        const unsigned long long m = __ballot(ok);
+        const int last = 63 - __clzll((long long)m);
+        lo += last * step;
+        hi = min(hi, lo + step);
+    }
+    return lo;
+}
+
 // find_job for a block of consecutive work items starting at g0 (every thread
-// of the block must call it): one search per block, staged in LDS, then a few
-// steps forward per thread; a block that spans many jobs falls back to the search
+// of the block must call it, before any thread leaves): one wavefront search per
+// block, staged in LDS, then a few steps forward per thread; a block that spans
+// many jobs falls back to the per-thread search
 __device__ __forceinline__ int find_job_block(const int64_t* __restrict__ off, int njobs, int64_t g0, int64_t g) {
     __shared__ int jb0;
-    if (threadIdx.x == 0) jb0 = find_job(off, njobs, g0);
+    if (threadIdx.x < 64) {
+        const int j = find_job_wave(off, njobs, g0);
+        if (threadIdx.x == 0) jb0 = j;
+    }
     __syncthreads();
     int jb = jb0;
     for (int step = 0; step < 4; ++step) {
@@ -299,7 +323,7 @@ __device__ __forceinline__ double scan_input(const BJob& J, int r, int n, int ja
 __global__ void __launch_bounds__(kScanThreads) k_scan_reduce(BatchView bv, int jacobi) {
     __shared__ DD sh[kScanThreads];
     const int64_t g = blockIdx.x;
-    const int jb = find_job(bv.tile_off, bv.njobs, g);
+    const int jb = find_job_wave(bv.tile_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int t = (int)(g - bv.tile_off[jb]);
     const int r = t / J.nblk, b = t % J.nblk;
@@ -321,7 +345,7 @@ __global__ void __launch_bounds__(256) k_scan_top(BatchView bv) {
     const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
     if (g >= bv.rhs) return;  // whole wavefronts
     const int lane = threadIdx.x & 63;
-    const int jb = find_job(bv.rhs_off, bv.njobs, g);
+    const int jb = find_job_wave(bv.rhs_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int r = (int)(g - bv.rhs_off[jb]);
     DD* bs = J.block_sums + (size_t)r * J.nblk;
@@ -345,7 +369,7 @@ __global__ void __launch_bounds__(256) k_scan_top(BatchView bv) {
 __global__ void __launch_bounds__(kScanThreads) k_scan_final(BatchView bv, int jacobi) {
     __shared__ DD sh[kScanThreads];
     const int64_t g = blockIdx.x;
-    const int jb = find_job(bv.tile_off, bv.njobs, g);
+    const int jb = find_job_wave(bv.tile_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int t = (int)(g - bv.tile_off[jb]);
     const int r = t / J.nblk, b = t % J.nblk;
@@ -417,9 +441,12 @@ struct alignas(sizeof(T) * VR) VecT {
 template <typename TC, typename TQ, typename TA, typename TI, int KR, int VR, int U, int RPW>
 __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd, int64_t row_lo, int64_t row_hi) {
     constexpr int G = 64 / RPW;  // lanes per destination row
-    constexpr int SL = G / KR;
+    constexpr int SL = G / KR;   // link slots per row
     constexpr int W = KR * VR;
+    constexpr int Q = U >= 8 ? 2 : 1;  // aligned quads of links in flight per slot
     using V = VecT<TQ, VR>;
+    using CI = VecT<TI, 4>;
+    using CC = VecT<TC, 4>;
     const int64_t gb = (int64_t)(blockIdx.x % kNumXcd) * blocks_per_xcd + (blockIdx.x / kNumXcd);
     const int lane = threadIdx.x & 63;
     // global row index over the batch; a lane group never straddles two rows
@@ -429,7 +456,8 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
                                   grow < row_hi ? grow : row_hi - 1);
     if (grow >= row_hi) return;
     const BJob& J = bv.jobs[jb];
-    const int row = J.rorder[grow - bv.node_off[jb]];
+    // one record gives the row's link range, charge / potential row and node
+    const GRow rr = J.grow[grow - bv.node_off[jb]];
     const int gl = lane % G;
     const int slot = gl / KR;
     const int rl = (gl % KR) * VR;
@@ -441,45 +469,71 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
     // source indices: 16 bits for jobs of at most 65536 nodes (a quarter of the
     // FP32 link stream), 32 bits otherwise
     const TI* cix = (sizeof(TI) == 2) ? reinterpret_cast<const TI*>(J.col16) : reinterpret_cast<const TI*>(J.col);
-    const int64_t beg = J.row_ptr[row], end = J.row_ptr[row + 1];
-    const TA sf = (TA)J.diag[row];
-    const size_t orow = (size_t)J.perm[row] * kp;
+    const int64_t beg = rr.beg, end = rr.end;
+    const TA sf = (TA)J.diag[rr.node];
+    const size_t orow = (size_t)rr.prow * kp;
+    // The link streams are read as aligned quads: one vector load brings 4
+    // indices, one the 4 coefficients (the arrays start 256-byte aligned and
+    // carry 4 spare entries, so the quad of the last link is in bounds); links of
+    // a quad outside [beg, end) get weight 0 and charge row 0.  A slot takes Q
+    // quads per step, the row's slots consecutive quads.
+    const int64_t q0 = beg >> 2, q1 = (end + 3) >> 2;
     for (int r0 = 0; r0 < k; r0 += W) {
         const bool active = r0 + rl < k;  // lanes past the last rhs load nothing
+        // the row's own charges (self term), read ahead of the links
+        TQ selfc[VR];
+#pragma unroll
+        for (int v = 0; v < VR; ++v)
+            selfc[v] = (slot == 0 && r0 + rl + v < k) ? charge[orow + r0 + rl + v] : (TQ)0;
         TA acc[VR];
 #pragma unroll
         for (int v = 0; v < VR; ++v) acc[v] = (TA)0;
-        for (int64_t l = beg + slot; l < end; l += U * SL) {
-            // lanes past the row's last link skip the index and coefficient loads
-            // (a branch, measured faster than a select) and gather charge row 0
-            // with weight 0
-            int c[U];
-            TA w[U];
+        if (active) {
+            // Software pipeline: the next step's quads are loaded while the
+            // charges of the current ones arrive.
+            int c[4 * Q];
+            TA w[4 * Q];
+            auto load_quads = [&](int64_t qs, int* cc, TA* ww) {
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int64_t ll = l + (int64_t)u * SL;
-                if (ll < end) {
-                    c[u] = (int)cix[ll];
-                    w[u] = (TA)coef[ll];
-                } else {
-                    c[u] = 0;
-                    w[u] = (TA)0;
+                for (int qq = 0; qq < Q; ++qq) {
+                    const int64_t q = qs + (int64_t)qq * SL;
+                    if (q < q1) {
+                        const CI iv = *reinterpret_cast<const CI*>(cix + 4 * q);
+                        const CC cv = *reinterpret_cast<const CC*>(coef + 4 * q);
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const int64_t ll = 4 * q + t;
+                            const bool in = ll >= beg && ll < end;
+                            cc[4 * qq + t] = in ? (int)iv.v[t] : 0;
+                            ww[4 * qq + t] = in ? (TA)cv.v[t] : (TA)0;
+                        }
+                    } else {
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            cc[4 * qq + t] = 0;
+                            ww[4 * qq + t] = (TA)0;
+                        }
+                    }
+                }
+            };
+            load_quads(q0 + slot, c, w);
+            for (int64_t qs = q0 + slot; qs < q1; qs += (int64_t)Q * SL) {
+                V ch[4 * Q];
+#pragma unroll
+                for (int u = 0; u < 4 * Q; ++u) ch[u] = *reinterpret_cast<const V*>(charge + (size_t)c[u] * kp + r0 + rl);
+                int cn[4 * Q];
+                TA wn[4 * Q];
+                load_quads(qs + (int64_t)Q * SL, cn, wn);
+#pragma unroll
+                for (int u = 0; u < 4 * Q; ++u)
+#pragma unroll
+                    for (int v = 0; v < VR; ++v) acc[v] += w[u] * (TA)ch[u].v[v];
+#pragma unroll
+                for (int u = 0; u < 4 * Q; ++u) {
+                    c[u] = cn[u];
+                    w[u] = wn[u];
                 }
             }
-            V ch[U];
-#pragma unroll
-            for (int u = 0; u < U; ++u) {
-                if (active) {
-                    ch[u] = *reinterpret_cast<const V*>(charge + (size_t)c[u] * kp + r0 + rl);
-                } else {
-#pragma unroll
-                    for (int v = 0; v < VR; ++v) ch[u].v[v] = (TQ)0;
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; ++u)
-#pragma unroll
-                for (int v = 0; v < VR; ++v) acc[v] += w[u] * (TA)ch[u].v[v];
         }
         // reduce over the slots of this row's lane group
 #pragma unroll
@@ -489,10 +543,7 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
         if (slot == 0) {
 #pragma unroll
             for (int v = 0; v < VR; ++v)
-                if (r0 + rl + v < k) {
-                    const size_t o = orow + r0 + rl + v;
-                    pot[o] = (TQ)(sf * (TA)charge[o] + acc[v]);
-                }
+                if (r0 + rl + v < k) pot[orow + r0 + rl + v] = (TQ)(sf * (TA)selfc[v] + acc[v]);
         }
     }
 }
@@ -1387,8 +1438,8 @@ constexpr int kPreRhs = 8;
 template <int MODE>
 __global__ void __launch_bounds__(256) k_block_apply(PView pv) {
     const int64_t gb = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (gb >= pv.nblk) return;
-    const int jb = find_job(pv.blk_off, pv.njobs, gb);
+    if (gb >= pv.nblk) return;  // whole wavefronts
+    const int jb = find_job_wave(pv.blk_off, pv.njobs, gb);
     const PJob& J = pv.jobs[jb];
     const int b = (int)(gb - pv.blk_off[jb]);
     const int b0 = J.bstart[b], n = J.bstart[b + 1] - b0;
@@ -1497,7 +1548,7 @@ __global__ void __launch_bounds__(256) k_super_mult(SView sv) {
     const int lane = threadIdx.x & 63, q = threadIdx.x >> 6;
     const int64_t b0 = (int64_t)blockIdx.x * 64;
     if (b0 >= sv.nem) return;  // uniform over the workgroup
-    const int jb = find_job(sv.em_off, sv.njobs, b0);
+    const int jb = find_job_wave(sv.em_off, sv.njobs, b0);
     const SJob& J = sv.jobs[jb];
     const int m = J.m, e = (int)(b0 - sv.em_off[jb]) + lane;
     const bool live = e < m;
