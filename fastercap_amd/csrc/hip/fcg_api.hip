@@ -1391,6 +1391,10 @@ int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, Krylo
     return FCG_OK;
 }
 
+// Relative accuracy an inner FP32 cycle is asked for at most: a few times the
+// FP32 rounding of the stored coefficients, charges and basis vectors.
+constexpr double kInnerFloor = 2e-7;
+
 void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
     const double tol = E->opts.tol;
     S.beta.assign(beta, beta + S.k);
@@ -1407,11 +1411,12 @@ void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
             // reference stop rule |g_{i+1}| / |b| < tol (SolveCapacitance.cpp:2493-2500)
             S.thr[(size_t)q] = tol * S.normb[(size_t)q];
         } else {
-            // inner FP32 solve: the reference stop rule with a 10% margin, so that
-            // the FP64 residual check after the cycle (which differs from the inner
-            // estimate by the FP32 product error) usually passes at once; never
-            // below the FP32 product's own accuracy
-            const double rel_in = std::max(0.9 * tol * S.normb[(size_t)q] / beta[q], 2e-6);
+            // inner FP32 solve: the reference stop rule at half the tolerance, so
+            // that the FP64 residual check after the cycle (which differs from the
+            // inner estimate by the FP32 product error, about 1e-7 relative with
+            // FP32 coefficients and charges summed in FP64) passes at once; never
+            // below a small multiple of that error
+            const double rel_in = std::max(0.5 * tol * S.normb[(size_t)q] / beta[q], kInnerFloor);
             S.thr[(size_t)q] = rel_in * beta[q];
         }
         if (beta[q] / S.normb[(size_t)q] < tol) S.done[(size_t)q] = 1;

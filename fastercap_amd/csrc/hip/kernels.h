@@ -20,13 +20,13 @@ struct DD {
 };
 
 constexpr int kScanTile = 2048;  // elements per scan block (256 threads x 8)
-constexpr int kChunk = 4096;     // Krylov vector chunk per block
+constexpr int kChunk = 1024;     // Krylov vector chunk per block
 constexpr int kNumXcd = 8;       // MI355X accelerator complex dies
 
 // Precision of one hierarchical product.
 enum MvpPrec : int {
     kMvp64 = 0,     // FP64 coefficients, FP64 charges and accumulation
-    kMvp32 = 1,     // FP32 coefficients, FP32 charges and accumulation (inner solves)
+    kMvp32 = 1,     // FP32 coefficients and charges, FP64 sums in the gather (inner solves)
     kMvp32Acc = 2,  // FP32 coefficients, FP64 charges and accumulation (refinement residual)
 };
 

@@ -817,21 +817,45 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
 // ------------------------------------------------------------------ krylov (batched)
 constexpr int kKThreads = 256;
 
-// grid (nchunks, maxnv); TB = basis element type
+// grid (nchunks): the chunk's dot products with every basis vector of its job,
+// partial[c * maxnv + jj].  Each thread keeps its work-vector elements in
+// registers, so w is read once and each basis vector once; every product is
+// reduced by shuffles inside the wavefronts and the four wavefront sums meet in
+// LDS after the loop.  TB = basis element type.
+constexpr int kKPer = kChunk / kKThreads;  // chunk elements per thread
+constexpr int kMaxBasis = 1024;            // LDS bound on the basis vectors of one step
 template <typename TB>
 __global__ void __launch_bounds__(kKThreads) k_dots(KrylovView kv, double* __restrict__ partial) {
-    __shared__ double sh[kKThreads / 64];
-    const int c = blockIdx.x, jj = blockIdx.y;
+    extern __shared__ double sh[];  // [kKThreads / 64][maxnv]
+    const int c = blockIdx.x;
     const Chunk ch = kv.chunks[c];
     const Seg sg = kv.segs[ch.seg];
     const KJob kj = kv.kjobs[sg.job];
-    if (jj >= kj.nv) return;
-    const TB* q = reinterpret_cast<TB* const*>(kj.Q)[jj] + (size_t)sg.r * kj.N;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const double* w = kj.w + (size_t)sg.r * kj.N;
-    double s = 0.0;
-    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += kKThreads) s += (double)q[n] * w[n];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[(size_t)c * kv.maxnv + jj] = s;
+    double wr[kKPer];
+#pragma unroll
+    for (int t = 0; t < kKPer; ++t) {
+        const int n = ch.n0 + threadIdx.x + t * kKThreads;
+        wr[t] = n < ch.n1 ? w[n] : 0.0;
+    }
+    for (int jj = 0; jj < kj.nv; ++jj) {
+        const TB* q = reinterpret_cast<TB* const*>(kj.Q)[jj] + (size_t)sg.r * kj.N;
+        double s = 0.0;
+#pragma unroll
+        for (int t = 0; t < kKPer; ++t) {
+            const int n = ch.n0 + threadIdx.x + t * kKThreads;
+            if (n < ch.n1) s += (double)q[n] * wr[t];
+        }
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (lane == 0) sh[wv * kv.maxnv + jj] = s;
+    }
+    __syncthreads();
+    for (int jj = threadIdx.x; jj < kj.nv; jj += kKThreads) {
+        double s = 0.0;
+        for (int q = 0; q < kKThreads / 64; ++q) s += sh[q * kv.maxnv + jj];
+        partial[(size_t)c * kv.maxnv + jj] = s;
+    }
 }
 
 // hbuf[hoff + which*nv*k + jj*k + r] = sum over the segment of the dots partials
@@ -850,7 +874,9 @@ __global__ void __launch_bounds__(256) k_reduce(KrylovView kv, const double* __r
     hbuf[(size_t)kj.hoff + (size_t)which * kj.nv * kj.k + (size_t)jj * kj.k + sg.r] = s;
 }
 
-// grid (nchunks): w -= sum_jj h[jj] Q_jj on the chunk; optional |w|^2 partial[c]
+// grid (nchunks): w -= sum_jj h[jj] Q_jj on the chunk; optional |w|^2 partial[c].
+// Each thread's elements stay in registers across the basis vectors, so the loads
+// of one vector are independent of each other.
 template <typename TB>
 __global__ void __launch_bounds__(kKThreads) k_update(KrylovView kv, const double* __restrict__ hbuf, int which,
                                                       int norm, double* __restrict__ partial) {
@@ -861,13 +887,29 @@ __global__ void __launch_bounds__(kKThreads) k_update(KrylovView kv, const doubl
     const KJob kj = kv.kjobs[sg.job];
     const double* h = hbuf + (size_t)kj.hoff + (size_t)which * kj.nv * kj.k + sg.r;
     double* w = kj.w + (size_t)sg.r * kj.N;
+    double acc[kKPer];
+#pragma unroll
+    for (int t = 0; t < kKPer; ++t) {
+        const int n = ch.n0 + threadIdx.x + t * kKThreads;
+        acc[t] = n < ch.n1 ? w[n] : 0.0;
+    }
+    for (int jj = 0; jj < kj.nv; ++jj) {
+        const double hj = h[(size_t)jj * kj.k];
+        const TB* q = reinterpret_cast<TB* const*>(kj.Q)[jj] + (size_t)sg.r * kj.N;
+#pragma unroll
+        for (int t = 0; t < kKPer; ++t) {
+            const int n = ch.n0 + threadIdx.x + t * kKThreads;
+            if (n < ch.n1) acc[t] -= hj * (double)q[n];
+        }
+    }
     double ss = 0.0;
-    for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += kKThreads) {
-        double acc = w[n];
-        for (int jj = 0; jj < kj.nv; ++jj)
-            acc -= h[(size_t)jj * kj.k] * (double)reinterpret_cast<TB* const*>(kj.Q)[jj][(size_t)sg.r * kj.N + n];
-        w[n] = acc;
-        ss += acc * acc;
+#pragma unroll
+    for (int t = 0; t < kKPer; ++t) {
+        const int n = ch.n0 + threadIdx.x + t * kKThreads;
+        if (n < ch.n1) {
+            w[n] = acc[t];
+            ss += acc[t] * acc[t];
+        }
     }
     if (norm) {
         ss = block_sum(ss, sh);
@@ -969,10 +1011,12 @@ hipError_t launch_xupdate(const KrylovView& kv, const double* z, hipStream_t s) 
 
 hipError_t launch_dots(const KrylovView& kv, double* partial, hipStream_t s) {
     if (kv.nchunks == 0 || kv.maxnv == 0) return hipSuccess;
+    if (kv.maxnv > kMaxBasis) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (kKThreads / 64) * (size_t)kv.maxnv;
     if (kv.basis32)
-        k_dots<float><<<dim3(kv.nchunks, kv.maxnv), kKThreads, 0, s>>>(kv, partial);
+        k_dots<float><<<kv.nchunks, kKThreads, lds, s>>>(kv, partial);
     else
-        k_dots<double><<<dim3(kv.nchunks, kv.maxnv), kKThreads, 0, s>>>(kv, partial);
+        k_dots<double><<<kv.nchunks, kKThreads, lds, s>>>(kv, partial);
     return hipGetLastError();
 }
 hipError_t launch_reduce(const KrylovView& kv, const double* partial, double* hbuf, int which, hipStream_t s) {
