@@ -107,6 +107,12 @@ class Mesh:
         self.group_outperm = _np(d.group_outperm, self.G, np.float64)
         self.group_inperm = _np(d.group_inperm, self.G, np.float64)
         self.leaf_outperm = _np(d.leaf_outperm, self.N, np.float64)
+        # complex permittivities (the job solves the real form of the complex system)
+        self.complex = bool(d.group_outperm_im)
+        if self.complex:
+            self.group_outperm_im = _np(d.group_outperm_im, self.G, np.float64)
+            self.group_inperm_im = _np(d.group_inperm_im, self.G, np.float64)
+            self.leaf_outperm_im = _np(d.leaf_outperm_im, self.N, np.float64)
         raw = np.ctypeslib.as_array(C.cast(d.nodes, C.POINTER(C.c_uint8)), shape=(self.M * C.sizeof(NodeGeom),))
         self.nodes_raw = raw
         self.group_names = [self.lib.fch_group_name(C.c_void_p(handle), g).decode() for g in range(self.G)]
@@ -323,8 +329,9 @@ class Device:
     ) -> SolveResult:
         ncond = int((mesh.group_is_diel == 0).sum())
         rows_max = ncond
-        cap = np.zeros(rows_max * ncond, dtype=np.float64)
-        charges = np.zeros(rows_max * mesh.N, dtype=np.float64) if want_charges else None
+        nf = 2 if mesh.complex else 1  # complex jobs: real then imaginary part
+        cap = np.zeros(nf * rows_max * ncond, dtype=np.float64)
+        charges = np.zeros(nf * rows_max * mesh.N, dtype=np.float64) if want_charges else None
         o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine,
                       precond_size=precond_size)
         st = SolveStats()
@@ -343,8 +350,17 @@ class Device:
             "fcg_solve",
         )
         r, c = rows.value, cols.value
+        if mesh.complex:
+            capv = cap[: r * c].reshape(r, c) + 1j * cap[r * c: 2 * r * c].reshape(r, c)
+            if charges is not None:
+                ch = charges[: 2 * r * mesh.N].reshape(r, 2, mesh.N)
+                charges = ch[:, 0, :] + 1j * ch[:, 1, :]
+        else:
+            capv = cap[: r * c].reshape(r, c)
+            if charges is not None:
+                charges = charges[: r * mesh.N].reshape(r, mesh.N)
         return SolveResult(
-            cap=cap[: r * c].reshape(r, c),
+            cap=capv,
             iterations=[st.iterations[i] for i in range(min(r, 64))],
             residual=st.final_residual,
             converged=bool(st.converged),
@@ -352,7 +368,7 @@ class Device:
             t_solve_s=st.t_solve_s,
             t_mvp_s=st.t_mvp_s,
             n_mvp=st.n_mvp,
-            charges=None if charges is None else charges[: r * mesh.N].reshape(r, mesh.N),
+            charges=charges,
         )
 
     def solve_batch(
@@ -370,7 +386,7 @@ class Device:
         them); returns one SolveResult per job, in order."""
         jobs = list(jobs)
         ncs = [int((m.group_is_diel == 0).sum()) for m in meshes]
-        caps = np.zeros(sum(n * n for n in ncs), dtype=np.float64)
+        caps = np.zeros(sum(n * n * (2 if m.complex else 1) for n, m in zip(ncs, meshes)), dtype=np.float64)
         ja = (C.c_int * len(jobs))(*jobs)
         rows = (C.c_int * len(jobs))()
         cols = (C.c_int * len(jobs))()
@@ -385,9 +401,12 @@ class Device:
         for i, (m, n) in enumerate(zip(meshes, ncs)):
             r, c = rows[i], cols[i]
             st = sts[i]
+            capv = caps[off : off + r * c].reshape(r, c).copy()
+            if m.complex:
+                capv = capv + 1j * caps[off + r * c : off + 2 * r * c].reshape(r, c)
             out.append(
                 SolveResult(
-                    cap=caps[off : off + r * c].reshape(r, c).copy(),
+                    cap=capv,
                     iterations=[st.iterations[q] for q in range(min(r, 64))],
                     residual=st.final_residual,
                     converged=bool(st.converged),
@@ -396,7 +415,7 @@ class Device:
                     n_mvp=st.n_mvp,
                 )
             )
-            off += r * c
+            off += r * c * (2 if m.complex else 1)
         return out
 
     def footprint(self, job: int, precision: int = FCG_PREC_FP64, iters: int = 64):
@@ -407,6 +426,11 @@ class Device:
             self.lib.fcg_job_footprint(self._h, job, precision, iters, C.byref(a), C.byref(b)), "fcg_job_footprint"
         )
         return a.value, b.value
+
+    def is_complex(self, job: int) -> bool:
+        v = C.c_int()
+        self._check(self.lib.fcg_job_is_complex(self._h, job, C.byref(v)), "fcg_job_is_complex")
+        return bool(v.value)
 
     def conductors(self, job: int) -> int:
         n = C.c_int()
@@ -519,8 +543,11 @@ class Engine:
         if rc != 0:
             return None
         r, c = rows.value, cols.value
+        capv = cap[: r * c].reshape(r, c).copy()
+        if self.dev.is_complex(job.value):
+            capv = capv + 1j * cap[r * c : 2 * r * c].reshape(r, c)
         res = SolveResult(
-            cap=cap[: r * c].reshape(r, c).copy(),
+            cap=capv,
             iterations=[st.iterations[q] for q in range(min(r, 64))],
             residual=st.final_residual,
             converged=bool(st.converged),

@@ -45,6 +45,9 @@ class JobDesc(C.Structure):
         ("group_outperm", C.POINTER(C.c_double)),
         ("group_inperm", C.POINTER(C.c_double)),
         ("leaf_outperm", C.POINTER(C.c_double)),
+        ("group_outperm_im", C.POINTER(C.c_double)),  # NULL for real permittivities
+        ("group_inperm_im", C.POINTER(C.c_double)),
+        ("leaf_outperm_im", C.POINTER(C.c_double)),
     ]
 
 
@@ -205,6 +208,7 @@ DEVICE_SYMBOLS = (
     "fcg_block_precond_apply",
     "fcg_super_precond_apply",
     "fcg_solve",
+    "fcg_job_is_complex",
     "fcg_job_conductors",
     "fcg_last_error",
     "fcg_mvp_timing_run",
@@ -325,6 +329,7 @@ def device_lib() -> C.CDLL:
             C.POINTER(SolveStats),
         ]
         lib.fcg_job_conductors.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+        lib.fcg_job_is_complex.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
         lib.fcg_mvp_timing_run.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(MvpTiming)]
         lib.fcg_set_gather_variant.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int)]
         lib.fcg_autotune.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]

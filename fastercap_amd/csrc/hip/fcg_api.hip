@@ -240,6 +240,14 @@ struct Job {
         bool avail = false;
         DevBuf snode, sstart, sidx, Sinv, w;
     } spre;
+    // complex permittivities: imaginary parts of the group permittivities and of
+    // the leaf charge weights, the imaginary self terms (filled with the real ones)
+    bool cplx = false;
+    DevBuf gout_im, gin_im, leaf_w_im, self_im;
+    // length of a solve's vectors: [Re | Im] for complex jobs
+    int vlen() const { return cplx ? 2 * N : N; }
+    // product vectors per right-hand side: a complex vector is two real ones
+    int pk(int k) const { return cplx ? 2 * k : k; }
     // FP64 near-field side list (built on the first mixed fill; kernels.h BJob)
     DevBuf near_ptr, near_col, near_link, near_dc;
     int64_t Ln = 0;
@@ -264,7 +272,8 @@ struct Job {
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
                          col16.bytes + grow.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
-                         near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + scr.bytes());
+                         near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + gout_im.bytes +
+                         gin_im.bytes + leaf_w_im.bytes + self_im.bytes + scr.bytes());
     }
 
     BJob rec(const Scratch& sc, const double* q, double* v, const float* q32 = nullptr) const {
@@ -296,6 +305,9 @@ struct Job {
         b.near_dc = near_dc.as<double>();
         b.diag = diag.as<double>();
         b.self = self.as<double>();
+        b.self_im = cplx ? self_im.as<double>() : nullptr;
+        b.gout_im = gout_im.as<double>();
+        b.gin_im = gin_im.as<double>();
         b.left = left.as<int32_t>();
         b.leaf_lo = leaf_lo.as<int32_t>();
         b.leaf_hi = leaf_hi.as<int32_t>();
@@ -496,7 +508,7 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     int64_t* no = to + (n + 1);
     int64_t* fo = no + (n + 1);
     int64_t* ko = fo + (n + 1);
-    int kmax = 0, any2d = 0, kpmin = 1 << 30;
+    int kmax = 0, any2d = 0, anycplx = 0, kpmin = 1 << 30;
     for (int i = 0; i < n; ++i) {
         const BatchEntry& e = ents[(size_t)order[(size_t)i]];
         const Job& J = *e.j;
@@ -510,6 +522,7 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
         kmax = std::max(kmax, sc.k);
         kpmin = std::min(kpmin, std::max(sc.kp, 1));
         any2d |= (J.dim == 2);
+        anycplx |= J.cplx ? 1 : 0;
         if (plan) {
             const int cls = fcg::gather_class(std::max(sc.kp, 1));
             const int i16 = J.col16.p ? 1 : 0;
@@ -546,6 +559,7 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     out->kmax = kmax;
     out->kpmin = kpmin;
     out->any2d = any2d;
+    out->anycplx = anycplx;
     return FCG_OK;
 }
 
@@ -1163,12 +1177,13 @@ int make_sview(fcg_ctx* ctx, Tables& tabs, const std::vector<SEnt>& ents, fcg::S
     return FCG_OK;
 }
 
-// One stand-alone product on device buffers of job j (batch of one).
+// One stand-alone product on device buffers of job j (batch of one) on k
+// right-hand sides, each a [vlen()] vector (complex jobs: two product vectors).
 int run_mvp(fcg_ctx* ctx, Job& j, int prec, const double* dq, double* dv, int k, int jacobi, hipStream_t s,
             hipEvent_t* ev = nullptr) {
     if (prec == fcg::kMvp64 && !j.filled64) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in FP64");
     if (prec != fcg::kMvp64 && !j.filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in mixed mode");
-    RCCHECK(ensure_scratch(ctx, j.scr, j.N, j.M, k, prec == fcg::kMvp32));
+    RCCHECK(ensure_scratch(ctx, j.scr, j.N, j.M, j.pk(k), prec == fcg::kMvp32));
     HIPCHECK(ctx, hipStreamSynchronize(s));  // the table arena is reused below
     ctx->tabs.reset();
     BatchView bv;
@@ -1327,6 +1342,7 @@ KJob kjob_of(Solve& S, void* const* qtab, double* w, int nv, int zoff) {
     J.brange = S.J->rhs_range.as<int32_t>();
     J.self = S.J->self.as<double>();
     J.N = S.N;
+    J.nleaf = S.J->N;
     J.k = S.k;
     J.nv = nv;
     J.hoff = S.hoff;
@@ -1500,10 +1516,14 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->J = J;
     S->tag = tag;
     S->k = (int)J->rhs.size();
-    S->N = J->N;
+    S->N = J->vlen();  // complex: [Re | Im] vectors of the real 2N system
     S->ncol = J->ncol;
     S->mixed = mixed;
     S->b32 = mixed;
+    if (J->cplx && E->opts.precond != FCG_PRECOND_NONE && E->opts.precond != FCG_PRECOND_JACOBI)
+        return fail(ctx, FCG_INVALID_ARGUMENT,
+                    "complex permittivities: Jacobi or no preconditioner (the reference's block and two-level "
+                    "preconditioners handle the real form's first half only)");
     S->jac = E->opts.precond == FCG_PRECOND_JACOBI;
     S->block = E->opts.precond == FCG_PRECOND_BLOCK;
     if (E->opts.precond == FCG_PRECOND_SUPER) {
@@ -1517,7 +1537,7 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     }
     S->want_x = E->want_x;
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
-    RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, S->k, mixed));
+    RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, J->pk(S->k), mixed));
     const size_t vec = (size_t)S->k * S->N;
     HIPCHECK(ctx, S->x.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->r.get(&ctx->pool, sizeof(double) * vec));
@@ -1683,15 +1703,18 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     std::vector<std::pair<Solve*, int64_t>> fin;
     std::vector<double*> fin_x;
     std::vector<fcg::ChargeSumJob> csj;
-    int64_t fin_vals = 0;
+    int64_t fin_vals = 0, fin_blk = 0;
     for (auto& u : C.active) {
         Solve& S = *u;
         if (S.phase != kFinish) continue;
-        const int64_t nval = (int64_t)S.k * S.ncol;
-        csj.push_back(fcg::ChargeSumJob{S.x.as<double>(), S.J->leaf_w.as<double>(), S.J->col_range.as<int32_t>(),
-                                        S.N, S.k, S.ncol, fin_vals, fin_vals});
+        const int64_t nsum = (int64_t)S.k * S.ncol;  // blocks: one per (rhs, conductor)
+        const int64_t nval = S.J->cplx ? 2 * nsum : nsum;  // complex: real sums, then imaginary
+        csj.push_back(fcg::ChargeSumJob{S.x.as<double>(), S.J->leaf_w.as<double>(),
+                                        S.J->cplx ? S.J->leaf_w_im.as<double>() : nullptr,
+                                        S.J->col_range.as<int32_t>(), S.J->N, S.k, S.ncol, fin_blk, fin_vals});
         fin.emplace_back(&S, fin_vals);
         fin_vals += nval;
+        fin_blk += nsum;
         double* hx = nullptr;
         if (S.want_x) {
             void* hxv;
@@ -1713,7 +1736,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         fin_host = reinterpret_cast<double*>(h);
         const auto* jt = reinterpret_cast<const fcg::ChargeSumJob*>(dt);
         const int nj = (int)csj.size();
-        const int64_t nb = fin_vals;
+        const int64_t nb = fin_blk;
         double* out = fin_dev.as<double>();
         const size_t nbytes = sizeof(double) * (size_t)fin_vals;
         q.push_back([=] { return fcg::launch_charge_sums(jt, nj, nb, out, s); });
@@ -1881,7 +1904,8 @@ int engine_finish(fcg_engine* E, Cohort& C) {
         rd.job = S.job_id;
         rd.rows = S.k;
         rd.cols = S.ncol;
-        rd.cap.assign(fin_host + fin[f].second, fin_host + fin[f].second + (size_t)S.k * S.ncol);
+        rd.cap.assign(fin_host + fin[f].second,
+                      fin_host + fin[f].second + (size_t)S.k * S.ncol * (S.J->cplx ? 2 : 1));
         if (fin_x[f]) rd.x.assign(fin_x[f], fin_x[f] + (size_t)S.k * S.N);
         std::memset(&rd.stats, 0, sizeof rd.stats);
         rd.stats.n_rhs = S.k;
@@ -2066,6 +2090,15 @@ int upload_finish(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
     HIPCHECK(ctx, upload(j.gout, d->group_outperm, (size_t)j.G, s));
     HIPCHECK(ctx, upload(j.gin, d->group_inperm, (size_t)j.G, s));
     HIPCHECK(ctx, upload(j.leaf_w, d->leaf_outperm, (size_t)j.N, s));
+    if (d->group_outperm_im || d->group_inperm_im || d->leaf_outperm_im) {
+        if (!d->group_outperm_im || !d->group_inperm_im || !d->leaf_outperm_im)
+            return fail(ctx, FCG_INVALID_ARGUMENT, "complex permittivities need all three imaginary arrays");
+        j.cplx = true;
+        HIPCHECK(ctx, upload(j.gout_im, d->group_outperm_im, (size_t)j.G, s));
+        HIPCHECK(ctx, upload(j.gin_im, d->group_inperm_im, (size_t)j.G, s));
+        HIPCHECK(ctx, upload(j.leaf_w_im, d->leaf_outperm_im, (size_t)j.N, s));
+        HIPCHECK(ctx, j.self_im.alloc(sizeof(double) * (size_t)j.N));
+    }
     HIPCHECK(ctx, j.link_row.alloc(sizeof(int32_t) * (size_t)std::max<int64_t>(j.L, 1)));
     HIPCHECK(ctx, j.self.alloc(sizeof(double) * (size_t)j.N));
     HIPCHECK(ctx, j.diag.alloc(sizeof(double) * (size_t)j.M));
@@ -2271,6 +2304,13 @@ int fcg_job_conductors(fcg_ctx* ctx, int job, int* n_cond) {
     return FCG_OK;
 }
 
+int fcg_job_is_complex(fcg_ctx* ctx, int job, int* is_complex) {
+    Job* j = get_job(ctx, job);
+    if (!j || !is_complex) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
+    *is_complex = j->cplx ? 1 : 0;
+    return FCG_OK;
+}
+
 int fcg_job_footprint(fcg_ctx* ctx, int job, int precision, int iters, int64_t* resident, int64_t* solve) {
     Job* j = get_job(ctx, job);
     if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
@@ -2286,8 +2326,9 @@ int fcg_job_footprint(fcg_ctx* ctx, int job, int precision, int iters, int64_t* 
     if (precision != FCG_PREC_MIXED && !j->coef64.p) res += 8 * j->L;
     if (resident) *resident = res;
     // vectors b, x, r, w and the basis, plus the solve's own product scratch
-    const int64_t scratch = k * ((int64_t)(j->N + 1) * 16 + (int64_t)j->M * (precision == FCG_PREC_MIXED ? 24 : 16));
-    if (solve) *solve = (int64_t)8 * k * j->N * (4 + std::max(iters, 1) + 1) + scratch;
+    const int64_t pk = j->pk((int)k);
+    const int64_t scratch = pk * ((int64_t)(j->N + 1) * 16 + (int64_t)j->M * (precision == FCG_PREC_MIXED ? 24 : 16));
+    if (solve) *solve = (int64_t)8 * k * j->vlen() * (4 + std::max(iters, 1) + 1) + scratch;
     return FCG_OK;
 }
 
@@ -2405,6 +2446,7 @@ int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v
     if (!j || !d_q || !d_v || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     if (!j->filled64) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in FP64");
     if (!ctx->noalloc) return fail(ctx, FCG_DEVICE_ERROR, "no scratch address on this device");
+    k = j->pk(k);  // complex jobs: two product vectors per right-hand side
     // scratch layout: job table, leaf prefix sums [k][N+1], scan tile sums [k][nblk],
     // node charges and potentials [M][kp], 2D fix-up values [2k]
     const int N = j->N, M = j->M, kp = fcg::padded_k(k);
@@ -2460,6 +2502,7 @@ int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v
     bv.kmax = k;
     bv.kpmin = kp;
     bv.any2d = j->dim == 2;
+    bv.anycplx = j->cplx ? 1 : 0;
     // the lane plan of a batch of this one job (make_batch + choose_variants)
     fcg::GatherPlan gp;
     gp.n = 1;
@@ -2482,7 +2525,7 @@ int fcg_mvp_prec(fcg_ctx* ctx, int job, int mvp_prec, const double* q, double* v
         return fail(ctx, FCG_INVALID_ARGUMENT, "bad product precision");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     DevBuf dq, dv;
-    const size_t n = (size_t)j->N * k;
+    const size_t n = (size_t)j->vlen() * k;
     HIPCHECK(ctx, upload(dq, q, n, ctx->stream));
     HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
     RCCHECK(run_mvp(ctx, *j, mvp_prec, dq.as<double>(), dv.as<double>(), k, 0, ctx->stream));
@@ -2512,7 +2555,7 @@ int fcg_mvp_timing_run(fcg_ctx* ctx, int job, int k, int reps, fcg_mvp_timing* o
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     const int prec = j->filled64 ? fcg::kMvp64 : fcg::kMvp32;
     hipStream_t s = ctx->stream;
-    const size_t n = (size_t)j->N * k;
+    const size_t n = (size_t)j->vlen() * k;
     DevBuf dq, dv;
     HIPCHECK(ctx, dq.alloc(sizeof(double) * n));
     HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
@@ -2573,8 +2616,8 @@ int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps,
         if (mvp_prec == FCG_MVP_FP64 ? !j->filled64 : !j->filled32)
             return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in this precision");
         const int k = std::max(1, (int)j->rhs.size());
-        RCCHECK(ensure_scratch(ctx, j->scr, j->N, j->M, k, mvp_prec == FCG_MVP_FP32));
-        const size_t n = (size_t)j->N * k;
+        RCCHECK(ensure_scratch(ctx, j->scr, j->N, j->M, j->pk(k), mvp_prec == FCG_MVP_FP32));
+        const size_t n = (size_t)j->vlen() * k;
         qs.emplace_back(new DevBuf());
         vs.emplace_back(new DevBuf());
         HIPCHECK(ctx, qs.back()->alloc(sizeof(double) * n));
@@ -2705,7 +2748,7 @@ namespace {
 // measure every gather shape of job j in product precision prec and keep the fastest
 int autotune_prec(fcg_ctx* ctx, Job* j, int prec, int k, int reps, int* variant, double* best_ms) {
     hipStream_t s = ctx->stream;
-    const size_t n = (size_t)j->N * k;
+    const size_t n = (size_t)j->vlen() * k;
     DevBuf dq, dv;
     HIPCHECK(ctx, dq.alloc(sizeof(double) * n));
     HIPCHECK(ctx, dv.alloc(sizeof(double) * n));
@@ -2766,10 +2809,12 @@ int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k)
     if (!j->filled64 && !j->filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     DevBuf dq, dp;
-    const size_t n = (size_t)j->N * k;
+    const size_t n = (size_t)j->vlen() * k;
     HIPCHECK(ctx, upload(dq, q, n, ctx->stream));
     HIPCHECK(ctx, dp.alloc(sizeof(double) * n));
-    HIPCHECK(ctx, fcg::launch_jacobi(dq.as<double>(), j->self.as<double>(), j->N, k, dp.as<double>(), ctx->stream));
+    // complex: both halves divided by the real self term (SolveCapacitance.cpp:3508-3524)
+    HIPCHECK(ctx, fcg::launch_jacobi(dq.as<double>(), j->self.as<double>(), j->N, j->pk(k), dp.as<double>(),
+                                     ctx->stream));
     HIPCHECK(ctx, hipMemcpyAsync(pq, dp.p, sizeof(double) * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     return FCG_OK;
@@ -2778,6 +2823,7 @@ int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k)
 int fcg_block_precond_apply(fcg_ctx* ctx, int job, int block_size, const double* q, double* pq, int k) {
     Job* j = get_job(ctx, job);
     if (!j || !q || !pq || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (j->cplx) return fail(ctx, FCG_INVALID_ARGUMENT, "block preconditioner: real permittivities only");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     RCCHECK(ensure_block_precond(ctx, *j, block_size));
     DevBuf dq, dp;
@@ -2797,6 +2843,7 @@ int fcg_block_precond_apply(fcg_ctx* ctx, int job, int block_size, const double*
 int fcg_super_precond_apply(fcg_ctx* ctx, int job, int dim, const double* q, double* pq, int k) {
     Job* j = get_job(ctx, job);
     if (!j || !q || !pq || k < 1) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    if (j->cplx) return fail(ctx, FCG_INVALID_ARGUMENT, "two-level preconditioner: real permittivities only");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     RCCHECK(ensure_super_precond(ctx, *j, dim));
     if (!j->spre.avail)
@@ -2903,7 +2950,7 @@ int run_batch(fcg_ctx* ctx, const int* jobs, int njobs, const fcg_solve_opts* op
     for (int i = 0; i < njobs; ++i) {
         Job* j = get_job(ctx, jobs[i]);
         if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
-        off[(size_t)i + 1] = off[(size_t)i] + j->rhs.size() * (size_t)j->ncol;
+        off[(size_t)i + 1] = off[(size_t)i] + j->rhs.size() * (size_t)j->ncol * (j->cplx ? 2 : 1);
     }
     for (int i = 0; i < njobs; ++i) RCCHECK(fcg_engine_submit(e, jobs[i], i));
     int got = 0;

@@ -66,6 +66,12 @@ struct BJob {
     double* near_dc;            // [Ln]
     double* diag;             // [M] self coefficient of leaves, 0 elsewhere
     double* self;             // [N]
+    // complex permittivities: the product's 2k vectors are k pairs [Re | Im]
+    // of [k][2N] vectors, coupled by the imaginary self terms self_im [N]
+    // (null for real jobs); gout_im / gin_im [groups] are the imaginary parts
+    double* self_im;
+    const double* gout_im;
+    const double* gin_im;
     const int32_t* left;
     const int32_t* leaf_lo;
     const int32_t* leaf_hi;
@@ -101,6 +107,7 @@ struct BatchView {
     int kmax;   // largest k in the batch
     int kpmin;  // smallest row stride in the batch
     int any2d;
+    int anycplx;  // some job has complex permittivities
 };
 
 // Krylov bookkeeping of one job for the batched GMRES kernels.
@@ -110,8 +117,9 @@ struct KJob {
     double* x;              // current solution [k][N]
     double* r;              // residual [k][N]
     const int32_t* brange;  // [2k] right-hand side r is 1 on leaves [brange[2r], brange[2r+1]), 0 elsewhere
-    const double* self;  // self coefficients [N] (Jacobi)
-    int N, k, nv;        // nv = basis vectors to project against this step
+    const double* self;  // self coefficients [nleaf] (Jacobi)
+    int N, k, nv;        // N: vector length (2 nleaf for complex jobs); nv = basis vectors to project
+    int nleaf;           // leaves (the self term of element n is self[n mod nleaf])
     int hoff;            // offset of this job's block in the step buffer
     int zoff, nz;        // solution-update coefficients z[zoff + jj*k + r], jj < nz
     int jacobi;
@@ -296,8 +304,10 @@ hipError_t launch_jacobi(const double* q, const double* self, int N, int k, doub
 // Conductor charge sums of several finished solves in one launch: for each job,
 // out[off + r ncol + c] = sum over the leaves of conductor column c of w[n] x[r][n].
 struct ChargeSumJob {
-    const double* x;         // [k][N] solution
+    const double* x;         // [k][N] solution ([k][2N], Re | Im, with w_im)
     const double* w;         // [N] leaf weights
+    const double* w_im;      // [N] their imaginary parts (complex jobs; else null): the
+                             // imaginary sums follow the real ones at off + k ncol
     const int32_t* col_range;  // [2 ncol] leaf range of every conductor column
     int N, k, ncol;
     int64_t blk;  // first block of this job (ascending, first entry 0)

@@ -271,15 +271,34 @@ __device__ double node_self(const BJob& J, const fcg_node_geom& nd) {
     return potential_opt_2d<double>((double)nd.c[0], (double)nd.c[1], nd.v);
 }
 
+// Self term of a dielectric panel of a complex job (SelfPotential's complex
+// branch, Solver/Autorefine.cpp:2377-2393, 2D :2454-2471): the complex number
+// (ea + eb) / (2 e0 (eb - ea) A) in 3D, pi (ea + eb) / ((eb - ea) L) in 2D, divided
+// as numerator times the conjugate denominator over its squared modulus.
+__device__ void diel_self_cplx(const BJob& J, const fcg_node_geom& nd, double* re, double* im) {
+    const double sr = J.gin[nd.cond] + J.gout[nd.cond], si = J.gin_im[nd.cond] + J.gout_im[nd.cond];
+    const double dr = J.gout[nd.cond] - J.gin[nd.cond], di = J.gout_im[nd.cond] - J.gin_im[nd.cond];
+    const double scale = J.dim == 3 ? 1.0 / (FCG_TWO_TIMES_E0 * nd.area) : FCG_PI / nd.area;
+    const double mod2 = dr * dr + di * di;
+    *re = scale * (sr * dr + si * di) / mod2;
+    *im = scale * (si * dr - sr * di) / mod2;
+}
+
 __global__ void k_self(BatchView bv) {
     const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
     const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
     if (g >= bv.leaves) return;
     const BJob& J = bv.jobs[jb];
     const int k = (int)(g - bv.leaf_off[jb]);
-    const double s = node_self(J, J.nodes[J.leaf_node[k]]);
+    const fcg_node_geom& nd = J.nodes[J.leaf_node[k]];
+    double s, si = 0.0;
+    if (J.self_im && (nd.flags & FCG_NODE_DIEL))
+        diel_self_cplx(J, nd, &s, &si);
+    else
+        s = node_self(J, nd);
     J.self[k] = s;
     J.diag[J.leaf_node[k]] = s;  // non-leaf entries stay zero
+    if (J.self_im) J.self_im[k] = si;
 }
 
 // ------------------------------------------------------------------ up-pass scan
@@ -776,6 +795,31 @@ __global__ void k_near_correct(BatchView bv) {
     }
 }
 
+// Complex jobs (MultiplyMatByVec_fast's complex branch,
+// Solver/MultiplyHierarchical.cpp:332-455): the product's 2k vectors are the
+// [Re | Im] halves of k complex vectors, each half already multiplied by the
+// real hierarchical matrix; the imaginary self terms couple them,
+// v_re -= D u_im and v_im += D u_re, u the (Jacobi-scaled) input.  D is zero on
+// conductor panels, so this commutes with the 2D fix-up (conductor rows only).
+__global__ void k_cplx_couple(BatchView bv, int jacobi) {
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
+    const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
+    if (g >= bv.leaves) return;
+    const BJob& J = bv.jobs[jb];
+    if (!J.self_im) return;
+    const int n = (int)(g - bv.leaf_off[jb]);
+    const double c = J.self_im[n];
+    if (c == 0.0) return;
+    const double sc = jacobi ? 1.0 / J.self[n] : 1.0;
+    for (int r = 0; r < J.k; r += 2) {
+        const size_t ore = (size_t)r * J.N + n, oim = ore + J.N;
+        const double ure = (J.q32 ? (double)J.q32[ore] : J.q[ore]) * sc;
+        const double uim = (J.q32 ? (double)J.q32[oim] : J.q[oim]) * sc;
+        J.v[ore] -= c * uim;
+        J.v[oim] += c * ure;
+    }
+}
+
 hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan& plan, hipStream_t s, hipEvent_t* ev) {
     if (b.njobs == 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -810,6 +854,7 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
         k_2d_last<<<blocks_for((size_t)b.rhs, 64), 64, 0, s>>>(b);
         k_2d_fix<<<lb, 256, 0, s>>>(b);
     }
+    if (b.anycplx) k_cplx_couple<<<lb, 256, 0, s>>>(b, jacobi);
     if (ev) (void)hipEventRecord(ev[4], s);
     return hipGetLastError();
 }
@@ -839,16 +884,29 @@ __global__ void __launch_bounds__(kKThreads) k_dots(KrylovView kv, double* __res
         const int n = ch.n0 + threadIdx.x + t * kKThreads;
         wr[t] = n < ch.n1 ? w[n] : 0.0;
     }
-    for (int jj = 0; jj < kj.nv; ++jj) {
-        const TB* q = reinterpret_cast<TB* const*>(kj.Q)[jj] + (size_t)sg.r * kj.N;
-        double s = 0.0;
+    // four basis vectors at a time: their loads and shuffle reductions interleave
+    constexpr int kG = 4;
+    for (int j0 = 0; j0 < kj.nv; j0 += kG) {
+        double s[kG];
 #pragma unroll
-        for (int t = 0; t < kKPer; ++t) {
-            const int n = ch.n0 + threadIdx.x + t * kKThreads;
-            if (n < ch.n1) s += (double)q[n] * wr[t];
+        for (int g = 0; g < kG; ++g) {
+            s[g] = 0.0;
+            if (j0 + g < kj.nv) {
+                const TB* q = reinterpret_cast<TB* const*>(kj.Q)[j0 + g] + (size_t)sg.r * kj.N;
+#pragma unroll
+                for (int t = 0; t < kKPer; ++t) {
+                    const int n = ch.n0 + threadIdx.x + t * kKThreads;
+                    if (n < ch.n1) s[g] += (double)q[n] * wr[t];
+                }
+            }
         }
-        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-        if (lane == 0) sh[wv * kv.maxnv + jj] = s;
+        for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int g = 0; g < kG; ++g) s[g] += __shfl_xor(s[g], off, 64);
+        if (lane == 0)
+#pragma unroll
+            for (int g = 0; g < kG; ++g)
+                if (j0 + g < kj.nv) sh[wv * kv.maxnv + j0 + g] = s[g];
     }
     __syncthreads();
     for (int jj = threadIdx.x; jj < kj.nv; jj += kKThreads) {
@@ -893,6 +951,7 @@ __global__ void __launch_bounds__(kKThreads) k_update(KrylovView kv, const doubl
         const int n = ch.n0 + threadIdx.x + t * kKThreads;
         acc[t] = n < ch.n1 ? w[n] : 0.0;
     }
+#pragma unroll 4
     for (int jj = 0; jj < kj.nv; ++jj) {
         const double hj = h[(size_t)jj * kj.k];
         const TB* q = reinterpret_cast<TB* const*>(kj.Q)[jj] + (size_t)sg.r * kj.N;
@@ -949,7 +1008,7 @@ __global__ void k_init_x(KrylovView kv) {
     const int lo = kj.brange[2 * sg.r], hi = kj.brange[2 * sg.r + 1];
     for (int n = ch.n0 + threadIdx.x; n < ch.n1; n += blockDim.x) {
         const double b = (n >= lo && n < hi) ? 1.0 : 0.0;
-        kj.x[o + n] = kj.jacobi ? b / kj.self[n] : b;
+        kj.x[o + n] = kj.jacobi ? b / kj.self[n < kj.nleaf ? n : n - kj.nleaf] : b;
     }
 }
 
@@ -986,7 +1045,7 @@ __global__ void k_xupdate(KrylovView kv, const double* __restrict__ z) {
         if (kj.y)
             kj.y[o + n] = acc;
         else
-            kj.x[o + n] += kj.jacobi ? acc / kj.self[n] : acc;
+            kj.x[o + n] += kj.jacobi ? acc / kj.self[n < kj.nleaf ? n : n - kj.nleaf] : acc;
     }
 }
 
@@ -1108,11 +1167,28 @@ __global__ void k_charge_sums(const ChargeSumJob* __restrict__ jobs, int njobs, 
     const ChargeSumJob& J = jobs[lo];
     const int64_t loc = b - J.blk;
     const int r = (int)(loc / J.ncol), c = (int)(loc % J.ncol);
-    double s = 0.0;
-    for (int n = J.col_range[2 * c] + threadIdx.x; n < J.col_range[2 * c + 1]; n += blockDim.x)
-        s += J.w[n] * J.x[(size_t)r * J.N + n];
-    s = block_sum(s, sh);
-    if (threadIdx.x == 0) out[J.off + loc] = s;
+    if (!J.w_im) {
+        double s = 0.0;
+        for (int n = J.col_range[2 * c] + threadIdx.x; n < J.col_range[2 * c + 1]; n += blockDim.x)
+            s += J.w[n] * J.x[(size_t)r * J.N + n];
+        s = block_sum(s, sh);
+        if (threadIdx.x == 0) out[J.off + loc] = s;
+        return;
+    }
+    // complex: the complex outer permittivity times the complex charge
+    // (SolveCapacitance.cpp:1342-1354), x = [Re(N) | Im(N)] per rhs
+    const double* xr = J.x + (size_t)r * 2 * J.N;
+    double sr = 0.0, si = 0.0;
+    for (int n = J.col_range[2 * c] + threadIdx.x; n < J.col_range[2 * c + 1]; n += blockDim.x) {
+        sr += J.w[n] * xr[n] - J.w_im[n] * xr[J.N + n];
+        si += J.w_im[n] * xr[n] + J.w[n] * xr[J.N + n];
+    }
+    sr = block_sum(sr, sh);
+    si = block_sum(si, sh);
+    if (threadIdx.x == 0) {
+        out[J.off + loc] = sr;
+        out[J.off + (int64_t)J.k * J.ncol + loc] = si;
+    }
 }
 hipError_t launch_charge_sums(const ChargeSumJob* jobs, int njobs, int64_t nblocks, double* out, hipStream_t s) {
     if (njobs == 0 || nblocks == 0) return hipSuccess;

@@ -127,6 +127,7 @@ struct BBox {
 };
 
 struct Builder {
+    bool hasCplx = false;  // m_ucHasCmplxPerm: some permittivity read so far is complex
     fch_options opt{};
     int dim = 3;
     std::deque<Elem> arena;
@@ -850,7 +851,7 @@ struct Builder {
             const char* p = ls.c_str() + 1;
             int skip = 0;
             char nm[1024] = {0}, tmp[1024] = {0};
-            bool cplx = false;
+            bool& cplx = hasCplx;
             if (c0 == 'C') {
                 if (std::sscanf(p, "%1023s%n", nm, &skip) < 1) continue;
                 p += skip;
@@ -872,7 +873,6 @@ struct Builder {
                 std::sscanf(p, "%1023s%n", tmp, &skip);
                 if (tmp[0] != '+') groupNum[parseLevel]++;
                 groupNum[parseLevel + 1] = 1;
-                if (cplx) throw std::runtime_error("complex permittivities are not supported by this solver");
                 parseFile(nm, ft, false, loff, oRe, oIm, lg, 0.0, 0.0, V3{});
             } else if (c0 == 'D') {
                 if (std::sscanf(p, "%1023s%n", nm, &skip) < 1) continue;
@@ -907,8 +907,13 @@ struct Builder {
                     std::swap(oRe, iRe);
                     std::swap(oIm, iIm);
                 }
-                if (cplx) throw std::runtime_error("complex permittivities are not supported by this solver");
-                bool dummy = (iRe <= 0.0f || oRe <= 0.0f) || (std::fabs(iRe - oRe) / iRe < kRelDielTol);
+                // a dummy interface (Autorefine.cpp:4986-5004): non-positive real part, or
+                // no contrast (with complex values read so far: in both parts)
+                bool dummy = (iRe <= 0.0f || oRe <= 0.0f);
+                if (cplx)
+                    dummy = dummy || (std::fabs(iRe - oRe) / iRe < kRelDielTol && std::fabs(iIm - oIm) / iIm < kRelDielTol);
+                else
+                    dummy = dummy || std::fabs(iRe - oRe) / iRe < kRelDielTol;
                 if (dummy) {
                     std::fprintf(stderr, "Warning: dummy dielectric-dielectric interface found in the input file, skipping\n");
                 } else {
@@ -1132,6 +1137,8 @@ struct fch_mesh {
     std::vector<int32_t> col;
     std::vector<int32_t> group_is_diel, group_leaf_start;
     std::vector<double> group_outperm, group_inperm, leaf_outperm;
+    bool cplx = false;  // some permittivity is complex: the *_im arrays below are exported
+    std::vector<double> group_outperm_im, group_inperm_im, leaf_outperm_im;
     std::vector<std::string> group_names;
     // for the FastCap-format writers: each group's outer media (conductors),
     // dielectric reference point, and each leaf's medium index
@@ -1277,6 +1284,10 @@ std::unique_ptr<fch_mesh> finish_build(Builder& b, bool links_on_device, double 
         m->group_inperm.resize(G);
         m->group_names.resize(G);
         m->leaf_outperm.assign((size_t)N, 0.0);
+        m->cplx = b.hasCplx;
+        m->group_outperm_im.assign(G, 0.0);
+        m->group_inperm_im.assign(G, 0.0);
+        m->leaf_outperm_im.assign((size_t)N, 0.0);
         int64_t ls = 0;
         for (size_t gi = 0; gi < G; ++gi) {
             Group& g = *b.groups[gi];
@@ -1284,6 +1295,8 @@ std::unique_ptr<fch_mesh> finish_build(Builder& b, bool links_on_device, double 
             m->group_leaf_start[gi] = (int32_t)ls;
             m->group_outperm[gi] = g.outperm[0];
             m->group_inperm[gi] = g.inperm[0];
+            m->group_outperm_im[gi] = g.outperm[1];
+            m->group_inperm_im[gi] = g.inperm[1];
             m->group_names[gi] = g.name;
             m->group_surf.push_back(g.surfOutperm);
             m->group_diel_ref.push_back(g.dielRef);
@@ -1296,7 +1309,10 @@ std::unique_ptr<fch_mesh> finish_build(Builder& b, bool links_on_device, double 
             if (!e->isLeaf()) continue;
             Group& g = *b.groups[(size_t)e->group];
             m->leaf_diel_index[(size_t)e->leafIndex] = e->dielIndex;
-            if (!g.isDiel) m->leaf_outperm[(size_t)e->leafIndex] = g.surfOutperm[e->dielIndex].first;
+            if (!g.isDiel) {
+                m->leaf_outperm[(size_t)e->leafIndex] = g.surfOutperm[e->dielIndex].first;
+                m->leaf_outperm_im[(size_t)e->leafIndex] = g.surfOutperm[e->dielIndex].second;
+            }
         }
         fch_stats& s = m->stats;
         s.input_raw_cond_panels = b.rawCond;
@@ -1544,6 +1560,9 @@ int fch_get_desc(const fch_mesh* m, fcg_job_desc* d) {
     d->group_outperm = m->group_outperm.data();
     d->group_inperm = m->group_inperm.data();
     d->leaf_outperm = m->leaf_outperm.data();
+    d->group_outperm_im = m->cplx ? m->group_outperm_im.data() : nullptr;
+    d->group_inperm_im = m->cplx ? m->group_inperm_im.data() : nullptr;
+    d->leaf_outperm_im = m->cplx ? m->leaf_outperm_im.data() : nullptr;
     return FCG_OK;
 }
 

@@ -29,6 +29,15 @@ def _fmt(x: float) -> str:
     return repr(float(x))
 
 
+def _perm(e) -> str:
+    """A permittivity in FastCap syntax: real, or complex as 're-jim' / 're+jim'
+    (the reference's parser, Solver/Autorefine.cpp:4840-4860; a lossy medium is
+    eps' - j eps'')."""
+    if isinstance(e, complex):
+        return f"{e.real!r}{'-' if e.imag < 0 else '+'}j{abs(e.imag)!r}"
+    return repr(float(e))
+
+
 def box_quads(name: str, lo, hi, scale: float = 1.0):
     """Six 'Q' statements of an axis-aligned box [lo, hi]."""
     lines = []
@@ -204,8 +213,8 @@ def _circle_segments(name: str, r: float, n: int):
     return lines
 
 
-def coated_coax_2d(outdir: str, a: float = 0.1, b: float = 0.15, c: float = 0.2, eps_in: float = 2.0,
-                   eps_out: float = 1.0, n: int = 40) -> str:
+def coated_coax_2d(outdir: str, a: float = 0.1, b: float = 0.15, c: float = 0.2, eps_in=2.0,
+                   eps_out=1.0, n: int = 40) -> str:
     """2D coated coaxial cable (the example of the reference's 2D input-file help):
     inner conductor radius a, dielectric interface at radius b (eps_in inside,
     eps_out outside), outer conductor radius c; each circle an n-gon.  Closed form
@@ -220,10 +229,10 @@ def coated_coax_2d(outdir: str, a: float = 0.1, b: float = 0.15, c: float = 0.2,
     main = os.path.join(outdir, "coated_coax.lst")
     with open(main, "w") as f:
         f.write("* 2D coated coaxial cable\n")
-        f.write(f"C {files['inner']} {eps_in!r}  0.0 0.0\n")
+        f.write(f"C {files['inner']} {_perm(eps_in)}  0.0 0.0\n")
         # the reference point (centre) lies in the region of the first permittivity
-        f.write(f"D {files['iface']} {eps_in!r} {eps_out!r}  0.0 0.0  0.0 0.0\n")
-        f.write(f"C {files['outer']} {eps_out!r}  0.0 0.0\n")
+        f.write(f"D {files['iface']} {_perm(eps_in)} {_perm(eps_out)}  0.0 0.0  0.0 0.0\n")
+        f.write(f"C {files['outer']} {_perm(eps_out)}  0.0 0.0\n")
     return main
 
 
@@ -255,7 +264,7 @@ def _icosphere(subdiv: int):
     return verts, faces
 
 
-def sphere_in_shell(outdir: str, a: float = 1.0, b: float = 2.0, eps_shell: float = 3.0, subdiv: int = 2) -> str:
+def sphere_in_shell(outdir: str, a: float = 1.0, b: float = 2.0, eps_shell=3.0, subdiv: int = 2) -> str:
     """Conductor sphere of radius a inside a concentric dielectric sphere of radius b
     (eps_shell between them, vacuum outside), both icospheres of 20*4^subdiv 'T'
     panels.  Closed form C = 4 pi e0 / ((1/a - 1/b)/eps_shell + 1/b)."""
@@ -276,8 +285,8 @@ def sphere_in_shell(outdir: str, a: float = 1.0, b: float = 2.0, eps_shell: floa
     main = os.path.join(outdir, "sphere_shell.lst")
     with open(main, "w") as f:
         f.write("* conductor sphere in a dielectric shell\n")
-        f.write(f"C {fc} {eps_shell!r}  0 0 0\n")
-        f.write(f"D {fd} {eps_shell!r} 1.0  0 0 0  0 0 0\n")
+        f.write(f"C {fc} {_perm(eps_shell)}  0 0 0\n")
+        f.write(f"D {fd} {_perm(eps_shell)} 1.0  0 0 0  0 0 0\n")
     return main
 
 

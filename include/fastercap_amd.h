@@ -102,6 +102,17 @@ typedef struct fcg_job_desc {
     const double* group_outperm;      /* [n_groups] outer permittivity (real part) */
     const double* group_inperm;       /* [n_groups] inner permittivity (dielectrics) */
     const double* leaf_outperm;       /* [N] permittivity multiplying the leaf charge in C */
+    /* Imaginary parts of the three permittivity arrays above, or all NULL when
+     * every permittivity is real.  A job with them is complex (the reference's
+     * AUTOREFINE_CPLX_PERM, Solver/Autorefine.cpp:4840-4951): its solves run on
+     * the real form [R -D; D R] of the complex system (R the hierarchical matrix,
+     * D the imaginary self terms of the dielectric panels), every right-hand side
+     * and charge vector is [Re(N) | Im(N)], and every capacitance matrix comes
+     * back as its real part followed by its imaginary part
+     * (Solver/MultiplyHierarchical.cpp:332-455, SolveCapacitance.cpp:1305-1366). */
+    const double* group_outperm_im;
+    const double* group_inperm_im;
+    const double* leaf_outperm_im;
 } fcg_job_desc;
 
 typedef struct fcg_solve_opts {
@@ -153,7 +164,8 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision);
 /* Copy the filled coefficients to host memory (coef [L] and self [N], either may be NULL). */
 int fcg_download_coeffs(fcg_ctx* ctx, int job, double* coef, double* self_coef);
 
-/* v = P q for k right-hand sides, host buffers laid out vector after vector ([k][N]).
+/* v = P q for k right-hand sides, host buffers laid out vector after vector ([k][N];
+ * complex jobs [k][2N], each vector [Re | Im], P the real form [R -D; D R]).
  * Replaces CMultHier::MultiplyMatByVec_fast (Solver/MultiplyHierarchical.cpp:285-458). */
 int fcg_mvp(fcg_ctx* ctx, int job, const double* q, double* v, int k);
 
@@ -186,7 +198,8 @@ int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v
  * FCG_MVP_FP64 and in FCG_PREC_MIXED for the two FP32-coefficient modes). */
 int fcg_mvp_prec(fcg_ctx* ctx, int job, int mvp_prec, const double* q, double* v, int k);
 
-/* Pq = P^-1 q (Jacobi) for k vectors, host buffers.
+/* Pq = P^-1 q (Jacobi) for k vectors, host buffers (complex jobs: [k][2N], both
+ * halves divided by the real self term, as the reference does).
  * Replaces CSolveCap::ComputePrecondVectFast (Solver/SolveCapacitance.cpp:3458-3526). */
 int fcg_precond_apply(fcg_ctx* ctx, int job, const double* q, double* pq, int k);
 
@@ -312,8 +325,11 @@ int fcg_refine_fetch(fcg_ctx* ctx, fcg_refine_node* out);
 /* Solve every conductor right-hand side of the job with right-preconditioned GMRES
  * and assemble the Maxwell capacitance matrix cap[rows][n_cond] (row-major; rows =
  * n_cond in 3D, n_cond-1 in 2D; 2D values unscaled, as the reference keeps them
- * before OutputCapMtx).  Replaces CSolveCap::Solve + gmresPrecondSFastAll
- * (Solver/SolveCapacitance.cpp:1138-1425, 2288-2574). */
+ * before OutputCapMtx).  Complex jobs (fcg_job_desc *_im arrays): cap holds the
+ * real matrix followed by the imaginary one (the reference's cRe, cIm), charges
+ * [rows][2N] ([Re | Im]); Jacobi or no preconditioner only.  Replaces
+ * CSolveCap::Solve + gmresPrecondSFastAll (Solver/SolveCapacitance.cpp:1138-1425,
+ * 2288-2574). */
 int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, int* rows, int* cols,
               double* charges /* [rows][N] or NULL */, fcg_solve_stats* stats);
 
@@ -418,6 +434,11 @@ int fcg_job_footprint(fcg_ctx* ctx, int job, int precision, int iters, int64_t* 
 
 /* Number of conductors (non-dielectric groups) of a job. */
 int fcg_job_conductors(fcg_ctx* ctx, int job, int* n_cond);
+
+/* 1 when the job has complex permittivities (its vectors are [Re | Im] and its
+ * capacitance matrices come back as real then imaginary part), else 0.  No
+ * reference counterpart (the reference's m_ucHasCmplxPerm, Solver/SolverGlobal.h:169). */
+int fcg_job_is_complex(fcg_ctx* ctx, int job, int* is_complex);
 
 /* Last error message of the context (static storage, valid until the next call). */
 const char* fcg_last_error(fcg_ctx* ctx);

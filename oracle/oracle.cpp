@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <vector>
 
 namespace {
@@ -315,8 +316,9 @@ int orc_fill(const fcg_job_desc* d, double* coef, double* self_coef, int nthread
     return 0;
 }
 
-int orc_mvp(const fcg_job_desc* d, const double* coef, const double* self_coef, const double* q, double* v,
-            int nthreads) {
+// the hierarchical product without the 2D fix-up
+static void mvp_tree(const fcg_job_desc* d, const double* coef, const double* self_coef, const double* q, double* v,
+                     int nthreads) {
     if (nthreads > 0) omp_set_num_threads(nthreads);
     const int M = d->n_nodes, N = d->n_leaves;
     std::vector<double> chg(M), pot(M);
@@ -347,20 +349,29 @@ int orc_mvp(const fcg_job_desc* d, const double* coef, const double* self_coef, 
         }
     }
     for (int k = 0; k < N; ++k) v[k] = pot[d->leaf_node[k]];
-    if (d->dim == 2) {
-        // zero-total-charge row (MultiplyHierarchical.cpp:305-330)
-        int first_cond = N;
-        for (int g = 0; g < d->n_groups; ++g)
-            if (!d->group_is_diel[g]) {
-                first_cond = d->group_leaf_start[g];
-                break;
-            }
-        const double last = v[N - 1];
-        for (int i = first_cond; i < N - 1; ++i) v[i] -= last;
-        double s = 0.0;
-        for (int i = 0; i < N; ++i) s += q[i];
-        v[N - 1] = 1.0 * s;
-    }
+}
+
+// 2D zero-total-charge row of one N-vector (MultiplyHierarchical.cpp:305-330)
+static void fix_2d(const fcg_job_desc* d, const double* q, double* v) {
+    if (d->dim != 2) return;
+    const int N = d->n_leaves;
+    int first_cond = N;
+    for (int g = 0; g < d->n_groups; ++g)
+        if (!d->group_is_diel[g]) {
+            first_cond = d->group_leaf_start[g];
+            break;
+        }
+    const double last = v[N - 1];
+    for (int i = first_cond; i < N - 1; ++i) v[i] -= last;
+    double s = 0.0;
+    for (int i = 0; i < N; ++i) s += q[i];
+    v[N - 1] = 1.0 * s;
+}
+
+int orc_mvp(const fcg_job_desc* d, const double* coef, const double* self_coef, const double* q, double* v,
+            int nthreads) {
+    mvp_tree(d, coef, self_coef, q, v, nthreads);
+    fix_2d(d, q, v);
     return 0;
 }
 
@@ -378,23 +389,12 @@ struct SuperPre {
     std::vector<double> Sinv;
 };
 
-// gmresPrecondSFastAll (SolveCapacitance.cpp:2288-2574): no preconditioner, Jacobi,
-// or the block preconditioner when bp is given.
-static int gmres(const fcg_job_desc* d, const double* coef, const double* self_coef, const std::vector<double>& b,
-                 std::vector<double>& x, double tol, int precond, int nthreads, const BlockPre* bp = nullptr,
-                 const SuperPre* sp = nullptr) {
-    const long n = d->n_leaves;
-    auto prec = [&](const std::vector<double>& in, std::vector<double>& out) {
-        if (sp) {
-            orc_super_apply(d, sp->m, sp->nodes.data(), sp->starts.data(), sp->Sinv.data(), in.data(), out.data());
-            return;
-        }
-        if (bp) {
-            orc_block_apply(bp->nblocks, bp->starts.data(), bp->poff.data(), bp->P.data(), in.data(), out.data());
-            return;
-        }
-        for (long i = 0; i < n; ++i) out[i] = precond ? in[i] / self_coef[i] : in[i];
-    };
+// gmresPrecondSFastAll (SolveCapacitance.cpp:2288-2574) on any real system of
+// size n: right preconditioning (x0 = P b when a preconditioner is given), MGS
+// Arnoldi, Givens rotations, stop when |g_{i+1}| / |b| < tol.  Returns the step count.
+using Op = std::function<void(const std::vector<double>&, std::vector<double>&)>;
+static int gmres_op(long n, const Op& A, const Op* P, const std::vector<double>& b, std::vector<double>& x,
+                    double tol) {
     auto nrm = [&](const std::vector<double>& a) {
         double s = 0;
         for (long i = 0; i < n; ++i) s += a[i] * a[i];
@@ -402,9 +402,9 @@ static int gmres(const fcg_job_desc* d, const double* coef, const double* self_c
     };
     std::vector<double> x0(n, 0.0), r(n), v(n), w(n), pq(n);
     double normr, normb;
-    if (precond) {
-        prec(b, x0);
-        orc_mvp(d, coef, self_coef, x0.data(), v.data(), nthreads);
+    if (P) {
+        (*P)(b, x0);
+        A(x0, v);
         for (long k = 0; k < n; ++k) r[k] = b[k] - v[k];
         normr = nrm(r);
         normb = nrm(b);
@@ -427,11 +427,11 @@ static int gmres(const fcg_job_desc* d, const double* coef, const double* self_c
     int i;
     double rel = normr / normb;
     for (i = 0; i < maxit; ++i) {
-        if (precond) {
-            prec(Q[i], pq);
-            orc_mvp(d, coef, self_coef, pq.data(), v.data(), nthreads);
+        if (P) {
+            (*P)(Q[i], pq);
+            A(pq, v);
         } else {
-            orc_mvp(d, coef, self_coef, Q[i].data(), v.data(), nthreads);
+            A(Q[i], v);
         }
         w = v;
         H.emplace_back(i + 2, 0.0);
@@ -475,13 +475,35 @@ static int gmres(const fcg_job_desc* d, const double* coef, const double* self_c
         for (int j = 0; j <= iters; ++j) t += Q[j][k] * z[j];
         y[k] = t;
     }
-    if (precond) {
-        prec(y, x);
+    if (P) {
+        (*P)(y, x);
         for (long k = 0; k < n; ++k) x[k] = x0[k] + x[k];
     } else {
         x = y;
     }
     return iters + 1;
+}
+
+// the real system: no preconditioner, Jacobi, or the block / two-level preconditioner
+static int gmres(const fcg_job_desc* d, const double* coef, const double* self_coef, const std::vector<double>& b,
+                 std::vector<double>& x, double tol, int precond, int nthreads, const BlockPre* bp = nullptr,
+                 const SuperPre* sp = nullptr) {
+    const long n = d->n_leaves;
+    const Op A = [&](const std::vector<double>& in, std::vector<double>& out) {
+        orc_mvp(d, coef, self_coef, in.data(), out.data(), nthreads);
+    };
+    const Op P = [&](const std::vector<double>& in, std::vector<double>& out) {
+        if (sp) {
+            orc_super_apply(d, sp->m, sp->nodes.data(), sp->starts.data(), sp->Sinv.data(), in.data(), out.data());
+            return;
+        }
+        if (bp) {
+            orc_block_apply(bp->nblocks, bp->starts.data(), bp->poff.data(), bp->P.data(), in.data(), out.data());
+            return;
+        }
+        for (long i = 0; i < n; ++i) out[i] = in[i] / self_coef[i];
+    };
+    return gmres_op(n, A, precond ? &P : nullptr, b, x, tol);
 }
 
 int orc_solve(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int precond,
@@ -795,3 +817,94 @@ extern "C" void orc_super_apply(const fcg_job_desc* d, int m, const int* nodes, 
     }
 }
 
+
+// ------------------------------------------------------------------ complex permittivities
+
+// SelfPotential of a leaf of a complex job (Solver/Autorefine.cpp:2358-2393, 2D
+// :2435-2471): a dielectric panel's self term is the complex number
+// (ea + eb) / (2 e0 (eb - ea) A) in 3D, pi (ea + eb) / ((eb - ea) L) in 2D
+// (complex division by the conjugate over the squared modulus); a conductor
+// panel's is real.
+extern "C" void orc_self_cplx(const fcg_job_desc* d, int leaf, double* re, double* im) {
+    const int node = d->leaf_node[leaf];
+    const fcg_node_geom& g = d->nodes[node];
+    *im = 0.0;
+    if (!(g.flags & FCG_NODE_DIEL) || !d->group_outperm_im) {
+        *re = orc_node_self(d, node);
+        return;
+    }
+    const double sr = d->group_inperm[g.cond] + d->group_outperm[g.cond];
+    const double si = d->group_inperm_im[g.cond] + d->group_outperm_im[g.cond];
+    const double dr = d->group_outperm[g.cond] - d->group_inperm[g.cond];
+    const double di = d->group_outperm_im[g.cond] - d->group_inperm_im[g.cond];
+    const double scale = d->dim == 3 ? 1.0 / (kTwoE0 * g.area) : kPi / g.area;
+    const double mod2 = dr * dr + di * di;
+    *re = scale * (sr * dr + si * di) / mod2;
+    *im = scale * (si * dr - sr * di) / mod2;
+}
+
+// MultiplyMatByVec_fast, complex branch (Solver/MultiplyHierarchical.cpp:332-455):
+// q and v are [Re(N) | Im(N)]; v_re = R q_re - D q_im, v_im = R q_im + D q_re with
+// D = diag(self_im), then each half's 2D fix-up.
+extern "C" int orc_mvp_cplx(const fcg_job_desc* d, const double* coef, const double* self_re, const double* self_im,
+                            const double* q, double* v, int nthreads) {
+    const int N = d->n_leaves;
+    const double *qr = q, *qi = q + N;
+    double *vr = v, *vi = v + N;
+    mvp_tree(d, coef, self_re, qr, vr, nthreads);
+    for (int i = 0; i < N; ++i) vr[i] -= self_im[i] * qi[i];
+    fix_2d(d, qr, vr);
+    mvp_tree(d, coef, self_re, qi, vi, nthreads);
+    for (int i = 0; i < N; ++i) vi[i] += self_im[i] * qr[i];
+    fix_2d(d, qi, vi);
+    return 0;
+}
+
+// CSolveCap::Solve for a complex job (Solver/SolveCapacitance.cpp:1138-1366):
+// GMRES on the real 2N system with b = [1 on the conductor | 0], Jacobi divides
+// both halves by the real self term (ComputePrecondVectFast, :3498-3526), and
+// C = sum of the complex outer permittivity times the complex charge.  cap
+// receives the real matrix (rows x n_cond) followed by the imaginary one.
+extern "C" int orc_solve_cplx(const fcg_job_desc* d, const double* coef, const double* self_re,
+                              const double* self_im, double tol, int precond, double* cap, int* rows, int* cols,
+                              int* iters, int nthreads) {
+    const long N = d->n_leaves, n = 2 * N;
+    int ncond = 0;
+    for (int g = 0; g < d->n_groups; ++g)
+        if (!d->group_is_diel[g]) ++ncond;
+    const int nrow = (d->dim == 2 && ncond > 1) ? ncond - 1 : ncond;
+    *rows = nrow;
+    *cols = ncond;
+    const Op A = [&](const std::vector<double>& in, std::vector<double>& out) {
+        orc_mvp_cplx(d, coef, self_re, self_im, in.data(), out.data(), nthreads);
+    };
+    const Op P = [&](const std::vector<double>& in, std::vector<double>& out) {
+        for (long i = 0; i < n; ++i) out[i] = in[i] / self_re[i % N];
+    };
+    std::vector<double> b(n, 0.0), x(n, 0.0);
+    int row = 0;
+    for (int g = 0; g < d->n_groups; ++g) {
+        if (d->group_is_diel[g]) continue;
+        const int lo = d->group_leaf_start[g], hi = d->group_leaf_start[g + 1];
+        if (d->dim == 2 && hi == N) continue;
+        for (int k = lo; k < hi; ++k) b[k] = 1.0;
+        const int it = gmres_op(n, A, precond ? &P : nullptr, b, x, tol);
+        if (iters) iters[row] = it;
+        int col = 0;
+        for (int g2 = 0; g2 < d->n_groups; ++g2) {
+            if (d->group_is_diel[g2]) continue;
+            double sr = 0.0, si = 0.0;
+            for (int k = d->group_leaf_start[g2]; k < d->group_leaf_start[g2 + 1]; ++k) {
+                const double wr = d->leaf_outperm[k], wi = d->leaf_outperm_im[k];
+                sr += wr * x[k] - wi * x[k + N];
+                si += wi * x[k] + wr * x[k + N];
+            }
+            cap[(size_t)row * ncond + col] = sr;
+            cap[(size_t)nrow * ncond + (size_t)row * ncond + col] = si;
+            ++col;
+        }
+        for (int k = lo; k < hi; ++k) b[k] = 0.0;
+        ++row;
+    }
+    return 0;
+}

@@ -38,6 +38,10 @@
  *   orc_super_apply        collect / multiply / distribute (ComputePrecondVectFast, :3545-3592)
  *   orc_solve_super        orc_solve with the two-level preconditioner
  *   orc_node_self          SelfPotential of any tree node (leaf, refined parent or super node)
+ *   orc_self_cplx          SelfPotential of a complex job's leaf (Solver/Autorefine.cpp:2358-2393, 2435-2471)
+ *   orc_mvp_cplx           MultiplyMatByVec_fast complex branch [R -D; D R] (MultiplyHierarchical.cpp:332-455)
+ *   orc_solve_cplx         CSolveCap::Solve of a complex job: GMRES on the real 2N system, Jacobi on both
+ *                          halves, complex charge sums (SolveCapacitance.cpp:1138-1366, 3498-3526)
  */
 #ifndef FASTERCAP_ORACLE_H
 #define FASTERCAP_ORACLE_H
@@ -107,6 +111,15 @@ void orc_super_apply(const fcg_job_desc* d, int m, const int* nodes, const int* 
                      const double* q, double* out);
 int orc_solve_super(const fcg_job_desc* d, const double* coef, const double* self_coef, double tol, int dim,
                     double* cap, int* rows, int* cols, int* iters, int nthreads);
+
+/* Complex permittivities (jobs whose desc carries the *_im arrays). */
+void orc_self_cplx(const fcg_job_desc* d, int leaf, double* re, double* im);
+/* q, v [Re(N) | Im(N)] */
+int orc_mvp_cplx(const fcg_job_desc* d, const double* coef, const double* self_re, const double* self_im,
+                 const double* q, double* v, int nthreads);
+/* cap: real matrix (rows x n_cond) then the imaginary one */
+int orc_solve_cplx(const fcg_job_desc* d, const double* coef, const double* self_re, const double* self_im,
+                   double tol, int precond, double* cap, int* rows, int* cols, int* iters, int nthreads);
 
 #ifdef __cplusplus
 }

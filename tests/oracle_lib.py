@@ -61,6 +61,10 @@ def lib():
         L.orc_super_apply.argtypes = [jd, C.c_int, ip, ip, dp, dp, dp]
         L.orc_super_apply.restype = None
         L.orc_solve_super.argtypes = [jd, dp, dp, C.c_double, C.c_int, dp, ip, ip, ip, C.c_int]
+        L.orc_self_cplx.argtypes = [jd, C.c_int, dp, dp]
+        L.orc_self_cplx.restype = None
+        L.orc_mvp_cplx.argtypes = [jd, dp, dp, dp, dp, dp, C.c_int]
+        L.orc_solve_cplx.argtypes = [jd, dp, dp, dp, C.c_double, C.c_int, dp, ip, ip, ip, C.c_int]
         _lib = L
     return _lib
 
@@ -250,3 +254,42 @@ def solve_super(mesh, coef, selfc, dim, tol=1e-8, nthreads=0):
     r, c = rows.value, cols.value
     return cap[: r * c].reshape(r, c), iters[:r]
 
+
+
+# ------------------------------------------------------------------ complex permittivities
+def self_cplx(mesh):
+    """(re [N], im [N]) self terms of a complex job."""
+    re = np.empty(mesh.N)
+    im = np.empty(mesh.N)
+    a, b = C.c_double(), C.c_double()
+    for i in range(mesh.N):
+        lib().orc_self_cplx(C.byref(mesh.desc), i, C.byref(a), C.byref(b))
+        re[i], im[i] = a.value, b.value
+    return re, im
+
+
+def fill_cplx(mesh, nthreads=0):
+    """(coef [L], self_re [N], self_im [N]) of a complex job."""
+    coef, _ = fill(mesh, nthreads)
+    sre, sim = self_cplx(mesh)
+    return coef, sre, sim
+
+
+def mvp_cplx(mesh, coef, sre, sim, q, nthreads=0):
+    """[Re | Im] product of a complex job, q of length 2N."""
+    q = np.ascontiguousarray(q, dtype=np.float64)
+    v = np.empty_like(q)
+    lib().orc_mvp_cplx(C.byref(mesh.desc), _d(coef), _d(sre), _d(sim), _d(q), _d(v), nthreads)
+    return v
+
+
+def solve_cplx(mesh, coef, sre, sim, tol=1e-8, precond=1, nthreads=0):
+    """Complex C matrix (rows x n_cond) and iteration counts of a complex job."""
+    ncond = int((mesh.group_is_diel == 0).sum())
+    cap = np.zeros(2 * ncond * ncond, dtype=np.float64)
+    iters = np.zeros(ncond, dtype=np.int32)
+    rows, cols = C.c_int(), C.c_int()
+    lib().orc_solve_cplx(C.byref(mesh.desc), _d(coef), _d(sre), _d(sim), tol, precond, _d(cap), C.byref(rows),
+                         C.byref(cols), iters.ctypes.data_as(C.POINTER(C.c_int)), nthreads)
+    r, c = rows.value, cols.value
+    return cap[: r * c].reshape(r, c) + 1j * cap[r * c: 2 * r * c].reshape(r, c), iters[:r]
