@@ -143,9 +143,17 @@ def parse_args(argv) -> Args:
     return a
 
 
+def _entry(x, scale):
+    """One matrix entry as the reference prints it: %g, complex as %g%+gj."""
+    if np.iscomplexobj(x):
+        return "%g%+gj" % (x.real * scale, x.imag * scale)
+    return "%g" % (x * scale)
+
+
 def format_cap(cap: np.ndarray, names, dim: int, verbose: bool = False) -> str:
-    """The reference's matrix printout: 2D values in F/m with the balance
-    conductor's column dropped unless verbose."""
+    """The reference's matrix printout (Solver/SolveCapacitance.cpp:724-809): 2D
+    values in F/m with the balance conductor's column dropped unless verbose,
+    complex entries as re+imj."""
     from . import TWO_PI_TIMES_E0
 
     M, N = cap.shape
@@ -154,7 +162,7 @@ def format_cap(cap: np.ndarray, names, dim: int, verbose: bool = False) -> str:
     scale = TWO_PI_TIMES_E0 if dim == 2 else 1.0
     lines = ["Capacitance matrix is:", f"Dimension {M} x {N}"]
     for i in range(M):
-        lines.append(f"{names[i]}  " + "".join(f"{cap[i, j] * scale:g} " for j in range(N)))
+        lines.append(f"{names[i]}  " + "".join(_entry(cap[i, j], scale) + " " for j in range(N)))
     return "\n".join(lines) + "\n"
 
 
@@ -255,7 +263,7 @@ def write_csv(path: str, cap: np.ndarray, dim: int, out=sys.stdout):
     out.write(f'Generating capacitance matrix file "{name}"\n')
     with open(name, "w") as f:
         for i in range(M):
-            f.write(", ".join("%g" % (cap[i, j] * scale) for j in range(N)) + "\n")
+            f.write(", ".join(_entry(cap[i, j], scale) for j in range(N)) + "\n")
 
 
 def write_outputs(a: Args, mesh, res, out=sys.stdout):
@@ -272,7 +280,8 @@ def write_outputs(a: Args, mesh, res, out=sys.stdout):
         out.write("\n")
         for i, name in enumerate(mesh.conductor_names[: res.cap.shape[0]]):
             out.write(f"Outputting charge densities for conductor {name}..\n")
-            q = np.ascontiguousarray(res.charges[i], dtype=np.float64)
+            # complex solves: the real part of the charge densities
+            q = np.ascontiguousarray(np.real(res.charges[i]), dtype=np.float64)
             lib.fch_write_fastcap(h, a.file.encode(), name.encode(), q.ctypes.data_as(C.POINTER(C.c_double)))
             out.write("Done\n")
     if a.out_geo:

@@ -187,9 +187,10 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
 struct Result {
     std::shared_ptr<fch_mesh> mesh;  // kept for the -o / -c writers
     double mesh_eps = 0.0;
-    std::vector<double> charges;     // [rows][N] with -c
-    std::vector<double> cap;
+    std::vector<double> charges;     // [rows][N] with -c ([rows][2N], Re | Im, when complex)
+    std::vector<double> cap;         // rows x cols, then the imaginary part when complex
     int rows = 0, cols = 0, dim = 3;
+    bool cplx = false;               // complex permittivities (m_ucHasCmplxPerm)
     std::vector<std::string> names;
     fch_stats hs{};
     fcg_solve_stats ss{};
@@ -311,8 +312,10 @@ int solve_mesh(fcg_ctx* ctx, const Options& o, double tol, Result& r) {
     if (rc == FCG_OK) {
         fcg_solve_opts so{tol, 1000, precond, o.precision, 8, size};
         const int nc = (int)r.names.size();
-        r.cap.assign((size_t)nc * nc, 0.0);
-        if (o.out_charge) r.charges.assign((size_t)nc * (size_t)d.n_leaves, 0.0);
+        r.cplx = d.group_outperm_im != nullptr;
+        const size_t nf = r.cplx ? 2 : 1;
+        r.cap.assign(nf * nc * nc, 0.0);
+        if (o.out_charge) r.charges.assign(nf * (size_t)nc * (size_t)d.n_leaves, 0.0);
         rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, o.out_charge ? r.charges.data() : nullptr,
                        &r.ss);
     }
@@ -348,9 +351,14 @@ void write_csv(const Options& o, const Result& r) {
             std::fprintf(stderr, "Error: only one input conductor defined for a 2D problem. Result is meaningless.");
     }
     const double scale = r.dim == 2 ? kTwoPiE0 : 1.0;
+    const size_t im = (size_t)r.rows * r.cols;  // offset of the imaginary part
     for (int i = 0; i < r.rows; ++i) {
         for (int j = 0; j < N; ++j) {
-            std::fprintf(f, "%g", r.cap[(size_t)i * r.cols + j] * scale);
+            const size_t e = (size_t)i * r.cols + j;
+            if (r.cplx)
+                std::fprintf(f, "%g%+gj", r.cap[e] * scale, r.cap[im + e] * scale);
+            else
+                std::fprintf(f, "%g", r.cap[e] * scale);
             if (j < N - 1) std::fprintf(f, ", ");
         }
         std::fprintf(f, "\n");
@@ -365,8 +373,10 @@ void write_outputs(const Options& o, const Result& r) {
         std::printf("\n");
         for (int i = 0; i < r.rows && i < (int)r.names.size(); ++i) {
             std::printf("Outputting charge densities for conductor %s..\n", r.names[(size_t)i].c_str());
+            // complex: the real part of the charge densities
+            const size_t stride = r.charges.size() / std::max<size_t>(r.names.size(), 1);
             fch_write_fastcap(r.mesh.get(), o.file.c_str(), r.names[(size_t)i].c_str(),
-                              r.charges.data() + (size_t)i * (r.charges.size() / std::max<size_t>(r.names.size(), 1)));
+                              r.charges.data() + (size_t)i * stride);
             std::printf("Done\n");
         }
     }
@@ -388,9 +398,16 @@ void print_cap(const Options& o, const Result& r) {
     }
     std::printf("Dimension %d x %d\n", M, N);
     const double scale = r.dim == 2 ? kTwoPiE0 : 1.0;
+    const size_t imo = (size_t)r.rows * r.cols;  // offset of the imaginary part
+    auto im = [&](int i, int j) { return r.cplx ? r.cap[imo + (size_t)i * r.cols + j] : 0.0; };
     for (int i = 0; i < M; ++i) {
         std::printf("%s  ", i < (int)r.names.size() ? r.names[(size_t)i].c_str() : "?");
-        for (int j = 0; j < N; ++j) std::printf("%g ", r.cap[(size_t)i * r.cols + j] * scale);
+        for (int j = 0; j < N; ++j) {
+            if (r.cplx)
+                std::printf("%g%+gj ", r.cap[(size_t)i * r.cols + j] * scale, im(i, j) * scale);
+            else
+                std::printf("%g ", r.cap[(size_t)i * r.cols + j] * scale);
+        }
         std::printf("\n");
     }
     for (int i = 0; i < M; ++i)
@@ -398,11 +415,12 @@ void print_cap(const Options& o, const Result& r) {
             if (i != j && r.cap[(size_t)i * r.cols + j] >= 0.0)
                 std::fprintf(stderr, "Warning: capacitance matrix has a non-negative off-diagonal element at row %d col %d\n",
                              i + 1, j + 1);
+    // diagonal dominance on the moduli (real matrices: the absolute values)
     for (int i = 0; i < M; ++i) {
         double sum = 0.0;
         for (int j = 0; j < N; ++j)
-            if (i != j) sum += std::fabs(r.cap[(size_t)i * r.cols + j]);
-        if (std::fabs(r.cap[(size_t)i * r.cols + i]) < sum)
+            if (i != j) sum += std::hypot(r.cap[(size_t)i * r.cols + j], im(i, j));
+        if (std::hypot(r.cap[(size_t)i * r.cols + i], im(i, i)) < sum)
             std::fprintf(stderr, "Warning: capacitance matrix is not diagonally dominant due to row %d\n", i + 1);
     }
     std::printf("\n");

@@ -34,7 +34,7 @@ def test_binary_auto_two_cubes(geo_dir):
     out = subprocess.run([BIN, path, "-a0.01"], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     names, cap = _last_matrix(out.stdout)
-    assert names == ["g1_cube.txt", "g2_cube.txt"] or len(names) == 2
+    assert names == ["g1_mycube", "g2_mycube"]  # the help's printout (hlp/Output/Output_ch01.htm:65-66)
     assert out.stdout.count("Iteration number #") == len(gold["iterations"])
     # every iteration's "Solve statistics" block as the reference help prints it
     lines = out.stdout.splitlines()
@@ -63,6 +63,44 @@ def test_binary_2d_coated_coax(geo_dir):
     names, cap = _last_matrix(out.stdout)
     assert cap.shape == (1, 1)  # balance conductor's column dropped, as the reference prints 2D results
     assert abs(cap[0, 0] / 113.4e-12 - 1.0) < 0.025
+
+
+def _complex_matrix(text):
+    lines = text.splitlines()
+    start = max(i for i, l in enumerate(lines) if l.startswith("Capacitance matrix is:"))
+    m, n = [int(x) for x in lines[start + 1].split()[1::2]]
+    rows = [lines[start + 2 + i].split() for i in range(m)]
+    return np.array([[complex(v) for v in r[1:1 + n]] for r in rows])
+
+
+@pytest.mark.parametrize("python_cli", [False, True])
+def test_lossy_coax_complex_printout(geo_dir, python_cli):
+    """A lossy insulator ('2.0-j1.0' in the input file): the matrix is printed
+    as re+imj (OutputCapMtx's complex format) in F/m, the CSV likewise, and the
+    value is the closed form's within 2.5%."""
+    import math
+
+    d = os.path.join(geo_dir, f"cli_lossy_{int(python_cli)}")
+    path = workloads.coated_coax_2d(d, eps_in=2.0 - 1.0j)
+    args = [path, "-m0.1", "-t1e-8", "-e"]
+    if python_cli:
+        buf = io.StringIO()
+        cli.run(cli.parse_args(args), out=buf)
+        text = buf.getvalue()
+    else:
+        out = subprocess.run([BIN] + args, capture_output=True, text=True, timeout=120)
+        assert out.returncode == 0, out.stderr
+        text = out.stdout
+    cap = _complex_matrix(text)
+    e0 = 8.854187817e-12
+    c1 = 2 * math.pi * e0 * (2.0 - 1.0j) / math.log(1.5)
+    c2 = 2 * math.pi * e0 / math.log(2.0 / 1.5)
+    exact = c1 * c2 / (c1 + c2)
+    assert cap.shape == (1, 1)
+    assert abs(cap[0, 0] - exact) < 0.025 * abs(exact)
+    base = path[: path.rfind(".")]
+    csv = [complex(x.strip()) for x in open(base + ".csv").read().split(",")]
+    assert abs(csv[0] - cap[0, 0]) <= 1e-5 * abs(cap[0, 0])
 
 
 def test_binary_rejects_unsupported_switch(geo_dir):
