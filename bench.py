@@ -20,6 +20,7 @@ region.  Prints one JSON line.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import statistics
@@ -167,6 +168,11 @@ def bench_batch(args, fc, dev, tmp):
         dev.solve_batch(jobs, meshes, tol=tol, precision=MIX, **PC)
     log("warmup done")
     lat = []
+    # Python's collector is paused over the timed steps (as the standard library's
+    # timeit does): a full collection of the interpreter's objects otherwise lands
+    # in some step; the solver itself is native code and unaffected
+    gc.collect()
+    gc.disable()
     dev.profile(enable=1, reset=not args.only_mixed)
     t_start = time.perf_counter()
     res = []
@@ -179,6 +185,7 @@ def bench_batch(args, fc, dev, tmp):
         lat = [r.t_solve_s for r in res]
         log(f"step {s}: fill {1e3 * (b - a):.1f} ms, batched mixed solve {1e3 * (c - b):.1f} ms")
     t_mixed = time.perf_counter() - t_start
+    gc.enable()
     prof = dev.profile(enable=0, reset=True)
     value = args.steps * nj / t_mixed
     mixed_caps = [r.cap for r in res]

@@ -183,6 +183,17 @@ class SolveResult:
     cycles: int = 0  # inner GMRES cycles (mixed: refinement cycles)
 
 
+_STATS_DTYPE = None
+
+
+def _stats_dtype():
+    """numpy dtype of fcg_solve_stats (built once)."""
+    global _STATS_DTYPE
+    if _STATS_DTYPE is None:
+        _STATS_DTYPE = np.ctypeslib.as_array(SolveStats()).dtype
+    return _STATS_DTYPE
+
+
 def _dptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
@@ -397,24 +408,21 @@ class Device:
             self.lib.fcg_solve_batch(self._h, ja, len(jobs), C.byref(o), _dptr(caps), rows, cols, sts),
             "fcg_solve_batch",
         )
+        # the stats through one numpy view (per-record ctypes access costs a few
+        # ms for a thousand jobs)
+        a = np.frombuffer(sts, dtype=_stats_dtype())
+        rmax = max(max(rows[i] for i in range(len(jobs))), 1)
+        its = a["iterations"][:, : min(rmax, 64)].tolist()
+        resid, conv, cyc = a["final_residual"].tolist(), a["converged"].tolist(), a["cycles"].tolist()
+        tsol, nmvp = a["t_solve_s"].tolist(), a["n_mvp"].tolist()
         out, off = [], 0
-        for i, (m, n) in enumerate(zip(meshes, ncs)):
+        for i, m in enumerate(meshes):
             r, c = rows[i], cols[i]
-            st = sts[i]
-            capv = caps[off : off + r * c].reshape(r, c).copy()
+            capv = caps[off : off + r * c].reshape(r, c)
             if m.complex:
                 capv = capv + 1j * caps[off + r * c : off + 2 * r * c].reshape(r, c)
-            out.append(
-                SolveResult(
-                    cap=capv,
-                    iterations=[st.iterations[q] for q in range(min(r, 64))],
-                    residual=st.final_residual,
-                    converged=bool(st.converged),
-                    cycles=int(st.cycles),
-                    t_solve_s=st.t_solve_s,
-                    n_mvp=st.n_mvp,
-                )
-            )
+            out.append(SolveResult(cap=capv, iterations=its[i][: min(r, 64)], residual=resid[i],
+                                   converged=bool(conv[i]), cycles=cyc[i], t_solve_s=tsol[i], n_mvp=nmvp[i]))
             off += r * c * (2 if m.complex else 1)
         return out
 

@@ -221,7 +221,8 @@ struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
-        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16, grow;
+        col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16, grow,
+        src_geom;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -270,7 +271,7 @@ struct Job {
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
-                         col16.bytes + grow.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
+                         col16.bytes + grow.bytes + src_geom.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
                          near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + gout_im.bytes +
                          gin_im.bytes + leaf_w_im.bytes + self_im.bytes + scr.bytes());
@@ -294,6 +295,7 @@ struct Job {
         b.iperm = iperm.as<int32_t>();
         b.rorder = rorder.as<int32_t>();
         b.grow = grow.as<fcg::GRow>();
+        b.src_geom = src_geom.as<fcg::SrcGeom>();
         b.fill_order = fill_order.as<uint32_t>();
         b.fill_start = fill_start.as<int64_t>();
         b.link_row = link_row.as<int32_t>();
@@ -388,6 +390,8 @@ struct fcg_ctx {
     std::vector<std::unique_ptr<Job>> jobs;
     std::string err;
     Tables tabs;  // tables of stand-alone calls (reset per call)
+    // engine step tables kept between engines (pinned staging is slow to allocate)
+    std::vector<std::unique_ptr<Tables>> spare_tables;
     // result of the last fcg_links_discover, until fcg_links_fetch
     struct LinkOut {
         std::vector<int64_t> row_ptr;
@@ -1258,7 +1262,9 @@ namespace {
 // bookkeeping), so results do not depend on the split.
 struct Cohort {
     std::vector<std::unique_ptr<Solve>> active;
-    Tables tabs;
+    // step tables: slabs of device memory and pinned staging, lent by the
+    // context for the engine's life so that engines do not allocate them
+    Tables* tabs = nullptr;
     PBuf partial_a, partial_b, hbuf;
     ProfEvents pe_res, pe_arn;
     hipEvent_t ev = nullptr;      // end of the step in flight
@@ -1304,6 +1310,13 @@ struct fcg_engine {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipStreamSynchronize(ctx->copy_stream);
         (void)hipStreamSynchronize(ctx->down_stream);
+        for (Cohort& c : co)
+            if (c.tabs) {
+                c.tabs->reset();
+                c.tabs->defer = false;
+                ctx->spare_tables.emplace_back(c.tabs);
+                c.tabs = nullptr;
+            }
     }
 };
 
@@ -1578,8 +1591,8 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     fcg_ctx* ctx = E->ctx;
     hipStream_t s = ctx->stream;
     C.t_plan = clk::now();
-    C.tabs.reset();  // the cohort's previous step has completed (engine_finish)
-    C.tabs.defer = true;
+    C.tabs->reset();  // the cohort's previous step has completed (engine_finish)
+    C.tabs->defer = true;
     const bool mixed = E->opts.precision == FCG_PREC_MIXED;
     // Every table of the step is written to pinned staging first and reaches the
     // device in one copy; the launches are queued here and issued after that copy.
@@ -1619,9 +1632,9 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         }
         if (!sc.empty()) {
             KrylovView kv;
-            RCCHECK(make_krylov(ctx, C.tabs, sc, &kv));
+            RCCHECK(make_krylov(ctx, *C.tabs, sc, &kv));
             void* dd;
-            HIPCHECK(ctx, C.tabs.push(dvals.data(), sizeof(double) * dvals.size(), s, &dd));
+            HIPCHECK(ctx, C.tabs->push(dvals.data(), sizeof(double) * dvals.size(), s, &dd));
             q.push_back([=] { return fcg::launch_scale_new(kv, reinterpret_cast<double*>(dd), s); });
             for (auto& e : sc) e.S->need_scale = false;
         }
@@ -1639,18 +1652,18 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         }
         if (!up.empty()) {
             KrylovView kv;
-            RCCHECK(make_krylov(ctx, C.tabs, up, &kv));
+            RCCHECK(make_krylov(ctx, *C.tabs, up, &kv));
             void* dz;
-            HIPCHECK(ctx, C.tabs.push(zall.data(), sizeof(double) * zall.size(), s, &dz));
+            HIPCHECK(ctx, C.tabs->push(zall.data(), sizeof(double) * zall.size(), s, &dz));
             q.push_back([=] { return fcg::launch_xupdate(kv, reinterpret_cast<double*>(dz), s); });
             if (!upp.empty()) {
                 fcg::PView pv;
-                RCCHECK(make_pview(ctx, C.tabs, upp, &pv));
+                RCCHECK(make_pview(ctx, *C.tabs, upp, &pv));
                 q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreAdd, s); });
             }
             if (!ups.empty()) {
                 fcg::SView sv;
-                RCCHECK(make_sview(ctx, C.tabs, ups, &sv));
+                RCCHECK(make_sview(ctx, *C.tabs, ups, &sv));
                 q.push_back([=] { return fcg::launch_super_apply(sv, fcg::kPreAdd, s); });
             }
             for (auto& e : up) {
@@ -1671,7 +1684,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             u->J->spre.dim = sdim;  // marks the job as queued (a job may start several solves)
             sinv.push_back(u->J);
         }
-        RCCHECK(queue_super_inverses(ctx, C.tabs, sinv, q));
+        RCCHECK(queue_super_inverses(ctx, *C.tabs, sinv, q));
         for (auto& u : C.active) {
             if (u->phase != kInit) continue;
             if (u->block)
@@ -1684,17 +1697,17 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         }
         if (!init.empty()) {
             KrylovView kv;
-            RCCHECK(make_krylov(ctx, C.tabs, init, &kv));
+            RCCHECK(make_krylov(ctx, *C.tabs, init, &kv));
             q.push_back([=] { return fcg::launch_init_x(kv, s); });
         }
         if (!initp.empty()) {
             fcg::PView pv;
-            RCCHECK(make_pview(ctx, C.tabs, initp, &pv));
+            RCCHECK(make_pview(ctx, *C.tabs, initp, &pv));
             q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreFromB, s); });
         }
         if (!inits.empty()) {
             fcg::SView sv;
-            RCCHECK(make_sview(ctx, C.tabs, inits, &sv));
+            RCCHECK(make_sview(ctx, *C.tabs, inits, &sv));
             q.push_back([=] { return fcg::launch_super_apply(sv, fcg::kPreFromB, s); });
         }
     }
@@ -1718,7 +1731,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         double* hx = nullptr;
         if (S.want_x) {
             void* hxv;
-            HIPCHECK(ctx, C.tabs.host_area(sizeof(double) * (size_t)S.k * S.N, &hxv));
+            HIPCHECK(ctx, C.tabs->host_area(sizeof(double) * (size_t)S.k * S.N, &hxv));
             hx = reinterpret_cast<double*>(hxv);
             const void* xs = S.x.p;
             const size_t nx = sizeof(double) * (size_t)S.k * S.N;
@@ -1730,8 +1743,8 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     PBuf& fin_dev = C.fin_dev;
     if (!csj.empty()) {
         void *dt, *h;
-        HIPCHECK(ctx, C.tabs.push(csj.data(), sizeof(fcg::ChargeSumJob) * csj.size(), s, &dt));
-        HIPCHECK(ctx, C.tabs.host_area(sizeof(double) * (size_t)fin_vals, &h));
+        HIPCHECK(ctx, C.tabs->push(csj.data(), sizeof(fcg::ChargeSumJob) * csj.size(), s, &dt));
+        HIPCHECK(ctx, C.tabs->host_area(sizeof(double) * (size_t)fin_vals, &h));
         HIPCHECK(ctx, fin_dev.get(&ctx->pool, sizeof(double) * (size_t)fin_vals));
         fin_host = reinterpret_cast<double*>(h);
         const auto* jt = reinterpret_cast<const fcg::ChargeSumJob*>(dt);
@@ -1769,14 +1782,14 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         }
         BatchView bv;
         fcg::GatherPlan gp;
-        RCCHECK(make_batch(ctx, C.tabs, be, &bv, &gp));
+        RCCHECK(make_batch(ctx, *C.tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32Acc : fcg::kMvp64;
         choose_variants(ctx, prec, be, gp);
         HIPCHECK(ctx, C.pe_res.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
         hipEvent_t* ev = C.pe_res.events();
         q.push_back([=] { return fcg::launch_mvp(bv, prec, 0, gp, s, ev); });
         KrylovView kv;
-        RCCHECK(make_krylov(ctx, C.tabs, ke, &kv));
+        RCCHECK(make_krylov(ctx, *C.tabs, ke, &kv));
         RCCHECK(ensure_pbuf(ctx, C.partial_a, sizeof(double) * (size_t)kv.nchunks));
         double* pa = C.partial_a.as<double>();
         double* hb = C.hbuf.as<double>();
@@ -1808,17 +1821,17 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         }
         if (!pe.empty()) {
             fcg::PView pv;
-            RCCHECK(make_pview(ctx, C.tabs, pe, &pv));
+            RCCHECK(make_pview(ctx, *C.tabs, pe, &pv));
             q.push_back([=] { return fcg::launch_block_apply(pv, fcg::kPreSet, s); });
         }
         if (!se.empty()) {
             fcg::SView sv;
-            RCCHECK(make_sview(ctx, C.tabs, se, &sv));
+            RCCHECK(make_sview(ctx, *C.tabs, se, &sv));
             q.push_back([=] { return fcg::launch_super_apply(sv, fcg::kPreSet, s); });
         }
         BatchView bv;
         fcg::GatherPlan gp;
-        RCCHECK(make_batch(ctx, C.tabs, be, &bv, &gp));
+        RCCHECK(make_batch(ctx, *C.tabs, be, &bv, &gp));
         const int prec = mixed ? fcg::kMvp32 : fcg::kMvp64;
         choose_variants(ctx, prec, be, gp);
         HIPCHECK(ctx, C.pe_arn.arm(ctx->profiling, prec, ctx->profiling ? gather_alg_bytes(be, prec) : 0.0, gp.n));
@@ -1826,7 +1839,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         const int jac = arn[0]->jac ? 1 : 0;
         q.push_back([=] { return fcg::launch_mvp(bv, prec, jac, gp, s, ev); });
         KrylovView kv;
-        RCCHECK(make_krylov(ctx, C.tabs, ke, &kv));
+        RCCHECK(make_krylov(ctx, *C.tabs, ke, &kv));
         RCCHECK(ensure_pbuf(ctx, C.partial_b, sizeof(double) * (size_t)kv.nchunks * std::max(kv.maxnv, 1)));
         double* pb = C.partial_b.as<double>();
         double* hb = C.hbuf.as<double>();
@@ -1843,7 +1856,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     double* hh = nullptr;
     if (hoff > 0) {
         void* h;
-        HIPCHECK(ctx, C.tabs.host_area(sizeof(double) * (size_t)hoff, &h));
+        HIPCHECK(ctx, C.tabs->host_area(sizeof(double) * (size_t)hoff, &h));
         hh = reinterpret_cast<double*>(h);
         const void* src = C.hbuf.p;
         const size_t nb = sizeof(double) * (size_t)hoff;
@@ -1853,7 +1866,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     // cohort's work in flight on the compute stream (they belong to this cohort,
     // whose previous step has completed); the compute stream waits for the copy
     // alone, so the step's first kernel follows the other cohort's last one.
-    HIPCHECK(ctx, C.tabs.flush(us));
+    HIPCHECK(ctx, C.tabs->flush(us));
     if (us != s) {
         if (!C.tab_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&C.tab_ev, hipEventDisableTiming));
         HIPCHECK(ctx, hipEventRecord(C.tab_ev, us));
@@ -2083,6 +2096,19 @@ int upload_grow(fcg_ctx* ctx, Job& j, const int64_t* row_ptr, const std::vector<
     return FCG_OK;
 }
 
+// source geometry of a job by charge row (fcg::SrcGeom: the fill's link sources)
+int upload_src_geom(fcg_ctx* ctx, Job& j, const fcg_node_geom* nodes, const std::vector<int32_t>& perm) {
+    std::vector<fcg::SrcGeom> g((size_t)j.M);
+    for (int i = 0; i < j.M; ++i) {
+        fcg::SrcGeom& e = g[(size_t)perm[(size_t)i]];
+        std::memcpy(e.v, nodes[i].v, sizeof e.v);
+        std::memcpy(e.c, nodes[i].c, sizeof e.c);
+    }
+    HIPCHECK(ctx, upload(j.src_geom, g.data(), g.size(), ctx->stream));
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));  // g is released here
+    return FCG_OK;
+}
+
 // groups, right-hand sides, self and diagonal storage and the link rows (both
 // layouts); returns synchronised
 int upload_finish(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
@@ -2189,6 +2215,7 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
         HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
         RCCHECK(upload_grow(ctx, *j, d->row_ptr, perm, rorder));
+        RCCHECK(upload_src_geom(ctx, *j, d->nodes, perm));
         std::vector<uint32_t> forder;
         build_fill_order(d, iperm, cols, forder, j->fill_cnt);
         if (!forder.empty()) HIPCHECK(ctx, upload(j->fill_order, forder.data(), forder.size(), s));
@@ -2235,6 +2262,7 @@ int fcg_job_upload_links(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     HIPCHECK(ctx, upload(j->iperm, iperm.data(), iperm.size(), s));
     HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
     RCCHECK(upload_grow(ctx, *j, dd.row_ptr, perm, rorder));
+    RCCHECK(upload_src_geom(ctx, *j, dd.nodes, perm));
     // stored columns: the source's charge row, every row ascending
     HIPCHECK(ctx, j->col.alloc(sizeof(int32_t) * (size_t)(L + kLinkSpare)));
     HIPCHECK(ctx, hipMemsetAsync(j->col.as<int32_t>() + L, 0, sizeof(int32_t) * kLinkSpare, s));
@@ -2884,6 +2912,14 @@ int fcg_engine_create(fcg_ctx* ctx, const fcg_solve_opts* opts, fcg_engine** out
     if (!(opts->tol > 0.0)) return fail(ctx, FCG_INVALID_ARGUMENT, "tolerance must be positive");
     auto e = std::make_unique<fcg_engine>();
     e->ctx = ctx;
+    for (Cohort& c : e->co) {
+        if (ctx->spare_tables.empty()) {
+            c.tabs = new Tables();
+        } else {
+            c.tabs = ctx->spare_tables.back().release();
+            ctx->spare_tables.pop_back();
+        }
+    }
     e->opts = *opts;
     e->opts.precond = precond;
     *out = e.release();

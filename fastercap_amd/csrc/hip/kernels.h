@@ -37,6 +37,14 @@ struct GRow {
     int32_t prow, node;
 };
 
+// Vertices and centroid of a node as a link source (48 bytes, three 16-byte
+// loads), stored by charge row: the fill reads a link's source through its
+// stored column directly.
+struct alignas(16) SrcGeom {
+    float v[9];
+    float c[3];
+};
+
 // Device record of one job in a batch.
 struct BJob {
     int N, M, k, dim;
@@ -52,6 +60,7 @@ struct BJob {
     const int32_t* iperm;     // [M] slot -> node
     const int32_t* rorder;    // [M] gather row order -> node
     const GRow* grow;         // [M] the rows in gather row order
+    const SrcGeom* src_geom;  // [M] source geometry by charge row
     const uint32_t* fill_order;  // [L] links grouped by fill class
     const int64_t* fill_start;   // [kFillClasses + 1] first fill_order entry of each class
     const int32_t* link_row;  // [L] destination node of each link (fill only)

@@ -119,8 +119,9 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 // the FP64 distance ratio, exactly as the reference computes it.
 enum FillBranch { kBrAnalytic = 0, kBrRule5 = 1, kBrRule3 = 2, kBrRule2 = 3, kBrPoint = 4 };
 
-template <typename T, int DIM, bool DIEL, int BR>
-__device__ __forceinline__ T link_coef(const fcg_node_geom& p1, const fcg_node_geom& p2) {
+// p2: the source, a fcg_node_geom or its SrcGeom copy (vertices, centroid)
+template <typename T, int DIM, bool DIEL, int BR, typename G2>
+__device__ __forceinline__ T link_coef(const fcg_node_geom& p1, const G2& p2) {
     if constexpr (DIM == 2) {
         const T dx = (T)p1.c[0] - (T)p2.c[0], dy = (T)p1.c[1] - (T)p2.c[1];
         if constexpr (!DIEL) {
@@ -171,24 +172,60 @@ __device__ __forceinline__ T link_coef(const fcg_node_geom& p1, const fcg_node_g
 // One fill class over the batch.  Mixed: the analytic near-field integrals, whose
 // log and atan terms cancel, stay in FP64; the quadrature and point branches run
 // in FP32.  Every mixed coefficient is stored in FP32, and that rounding bounds
-// all branches alike.
+// all branches alike.  The quadrature and point classes take kFillPer links per
+// thread (a block's links are consecutive class entries, thread t taking entries
+// t, t + 256, ...): the index chain of every link (fill order -> link -> source
+// charge row -> source node) is loaded before any integral is evaluated, so the
+// chains' memory latencies overlap.  The analytic classes (FP64, a few percent of
+// the links) keep one link per thread.
+constexpr int kFillPer = 4;
+template <int BR>
+constexpr int fill_per() {
+    return BR == kBrAnalytic ? 1 : kFillPer;
+}
+
 template <int DIM, bool DIEL, int BR, bool MIXED>
 __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __restrict__ off, int64_t total, int cls) {
-    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t g = g0 + threadIdx.x;
-    const int jb = find_job_block(off, bv.njobs, g0, g < total ? g : total - 1);
-    if (g >= total) return;
-    const BJob& J = bv.jobs[jb];
-    const int64_t l = J.fill_order[J.fill_start[cls] + (g - off[jb])];
-    const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
-    const fcg_node_geom& p2 = J.nodes[J.iperm[J.col[l]]];
-    if constexpr (MIXED) {
-        if constexpr (BR == kBrAnalytic)
-            J.coef32[l] = (float)link_coef<double, DIM, DIEL, BR>(p1, p2);
-        else
-            J.coef32[l] = link_coef<float, DIM, DIEL, BR>(p1, p2);
-    } else {
-        J.coef64[l] = link_coef<double, DIM, DIEL, BR>(p1, p2);
+    constexpr int P = fill_per<BR>();
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x * P;
+    const int64_t gf = g0 + threadIdx.x;
+    int jb = find_job_block(off, bv.njobs, g0, gf < total ? gf : total - 1);
+    int64_t l[P];
+    int jl[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        const int64_t g = gf + (int64_t)i * blockDim.x;
+        l[i] = -1;
+        jl[i] = jb;
+        if (g < total) {
+            while (jb + 1 < bv.njobs && off[jb + 1] <= g) ++jb;  // entries ascend with i
+            const BJob& J = bv.jobs[jb];
+            jl[i] = jb;
+            l[i] = J.fill_order[J.fill_start[cls] + (g - off[jb])];
+        }
+    }
+    int row[P], src[P];
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        if (l[i] < 0) continue;
+        const BJob& J = bv.jobs[jl[i]];
+        row[i] = J.link_row[l[i]];
+        src[i] = J.col[l[i]];  // the source's charge row, which indexes src_geom
+    }
+#pragma unroll
+    for (int i = 0; i < P; ++i) {
+        if (l[i] < 0) continue;
+        const BJob& J = bv.jobs[jl[i]];
+        const fcg_node_geom& p1 = J.nodes[row[i]];
+        const SrcGeom& p2 = J.src_geom[src[i]];
+        if constexpr (MIXED) {
+            if constexpr (BR == kBrAnalytic)
+                J.coef32[l[i]] = (float)link_coef<double, DIM, DIEL, BR>(p1, p2);
+            else
+                J.coef32[l[i]] = link_coef<float, DIM, DIEL, BR>(p1, p2);
+        } else {
+            J.coef64[l[i]] = link_coef<double, DIM, DIEL, BR>(p1, p2);
+        }
     }
 }
 
@@ -206,7 +243,7 @@ __global__ void __launch_bounds__(256) k_fill_near(BatchView bv, const int64_t* 
     const int64_t sl = g - off[jb];
     const uint32_t l = J.near_link[sl];
     const fcg_node_geom& p1 = J.nodes[J.link_row[l]];
-    const fcg_node_geom& p2 = J.nodes[J.iperm[J.col[l]]];
+    const SrcGeom& p2 = J.src_geom[J.near_col[sl]];
     const bool diel = (p1.flags & FCG_NODE_DIEL) != 0;
     double c;
     if (J.dim == 2)
@@ -716,9 +753,10 @@ hipError_t launch_link_rows(const BatchView& b, hipStream_t s) {
 
 template <bool MIXED>
 static void fill_class(const BatchView& b, const int64_t* off, int64_t n, int c, hipStream_t s) {
-    const unsigned grid = blocks_for((size_t)n, 256);
 #define FCG_F(C, DIM, DIEL, BR) \
-    case C: k_fill<DIM, DIEL, BR, MIXED><<<grid, 256, 0, s>>>(b, off, n, c); break;
+    case C:                                                                                              \
+        k_fill<DIM, DIEL, BR, MIXED><<<blocks_for((size_t)n, 256 * fill_per<BR>()), 256, 0, s>>>(b, off, n, c); \
+        break;
     switch (c) {
         FCG_F(0, 3, false, 0) FCG_F(1, 3, false, 1) FCG_F(2, 3, false, 2) FCG_F(3, 3, false, 3) FCG_F(4, 3, false, 4)
         FCG_F(5, 3, true, 0) FCG_F(6, 3, true, 1) FCG_F(7, 3, true, 2) FCG_F(8, 3, true, 3) FCG_F(9, 3, true, 4)
