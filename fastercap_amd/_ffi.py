@@ -7,6 +7,8 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
+# another build of the host library (the sanitizer build of tests/sanitize.sh)
+HOST_LIB_ENV = "FASTERCAP_HOST_LIB"
 
 
 class NodeGeom(C.Structure):
@@ -261,7 +263,7 @@ _dev = None
 def host_lib() -> C.CDLL:
     global _host
     if _host is None:
-        path = _lib_path("libfastercap_host.so")
+        path = os.environ.get(HOST_LIB_ENV) or _lib_path("libfastercap_host.so")
         if not os.path.exists(path):
             raise RuntimeError(f"{path} is not built; run __graft_entry__.build()")
         lib = C.CDLL(path)

@@ -13,7 +13,8 @@ from fastercap_amd._ffi import JobDesc
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-LIB = os.path.join(ORACLE_DIR, "_build", "liboracle.so")
+# FASTERCAP_ORACLE_LIB names another build (the sanitizer build of tests/sanitize.sh)
+LIB = os.environ.get("FASTERCAP_ORACLE_LIB") or os.path.join(ORACLE_DIR, "_build", "liboracle.so")
 
 _lib = None
 
