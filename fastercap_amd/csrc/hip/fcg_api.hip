@@ -249,6 +249,11 @@ struct Job {
     int vlen() const { return cplx ? 2 * N : N; }
     // product vectors per right-hand side: a complex vector is two real ones
     int pk(int k) const { return cplx ? 2 * k : k; }
+    // staged gather: source list of every gather tile and each link's index in
+    // its tile's list (built before the first product or solve; jobs with 16-bit
+    // source indices only); max_tile_src is the longest list, -1 until built
+    DevBuf tsrc_off, tsrc, lsrc;
+    int max_tile_src = -1;
     // FP64 near-field side list (built on the first mixed fill; kernels.h BJob)
     DevBuf near_ptr, near_col, near_link, near_dc;
     int64_t Ln = 0;
@@ -274,7 +279,8 @@ struct Job {
                          col16.bytes + grow.bytes + src_geom.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
                          near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + gout_im.bytes +
-                         gin_im.bytes + leaf_w_im.bytes + self_im.bytes + scr.bytes());
+                         gin_im.bytes + leaf_w_im.bytes + self_im.bytes + tsrc_off.bytes + tsrc.bytes +
+                         lsrc.bytes + scr.bytes());
     }
 
     BJob rec(const Scratch& sc, const double* q, double* v, const float* q32 = nullptr) const {
@@ -295,6 +301,9 @@ struct Job {
         b.iperm = iperm.as<int32_t>();
         b.rorder = rorder.as<int32_t>();
         b.grow = grow.as<fcg::GRow>();
+        b.tsrc_off = max_tile_src >= 0 ? tsrc_off.as<int32_t>() : nullptr;
+        b.tsrc = tsrc.as<uint16_t>();
+        b.lsrc = lsrc.as<uint16_t>();
         b.src_geom = src_geom.as<fcg::SrcGeom>();
         b.fill_order = fill_order.as<uint32_t>();
         b.fill_start = fill_start.as<int64_t>();
@@ -506,12 +515,13 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     });
     if (order_out) *order_out = order;
     std::vector<BJob> recs((size_t)n);
-    std::vector<int64_t> off((size_t)5 * (n + 1), 0);
+    std::vector<int64_t> off((size_t)6 * (n + 1), 0);
     int64_t* lo = off.data();
     int64_t* to = lo + (n + 1);
     int64_t* no = to + (n + 1);
     int64_t* fo = no + (n + 1);
     int64_t* ko = fo + (n + 1);
+    int64_t* go = ko + (n + 1);
     int kmax = 0, any2d = 0, anycplx = 0, kpmin = 1 << 30;
     for (int i = 0; i < n; ++i) {
         const BatchEntry& e = ents[(size_t)order[(size_t)i]];
@@ -523,6 +533,7 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
         no[i + 1] = no[i] + J.M;
         fo[i + 1] = fo[i] + J.N;
         ko[i + 1] = ko[i] + sc.k;
+        go[i + 1] = go[i] + fcg::gather_tiles(J.M);
         kmax = std::max(kmax, sc.k);
         kpmin = std::min(kpmin, std::max(sc.kp, 1));
         any2d |= (J.dim == 2);
@@ -533,6 +544,9 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
             if (plan->n == 0 || plan->seg[plan->n - 1].cls != cls || plan->seg[plan->n - 1].idx16 != i16) {
                 fcg::GatherSeg& g = plan->seg[plan->n++];
                 g.row_lo = no[i];
+                g.tile_lo = go[i];
+                g.staged = 1;
+                g.stage_floats = 1;
                 g.kpmin = std::max(sc.kp, 1);
                 g.kmax = 0;
                 g.cls = cls;
@@ -541,7 +555,13 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
             }
             fcg::GatherSeg& g = plan->seg[plan->n - 1];
             g.row_hi = no[i + 1];
+            g.tile_hi = go[i + 1];
             g.kmax = std::max(g.kmax, sc.k);
+            // staged when every job has its lists (launch_mvp checks that the
+            // longest one fits in LDS in the product's charge type)
+            const int64_t fl = (int64_t)J.max_tile_src * std::max(sc.kp, 1);
+            if (J.max_tile_src < 0 || fl > (int64_t)fcg::kStageMaxBytes) g.staged = 0;
+            else g.stage_floats = (int)std::max<int64_t>(g.stage_floats, fl);
         }
     }
     void *dr, *doff;
@@ -555,11 +575,13 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     out->node_off = d + 2 * (n + 1);
     out->leaf_off = d + 3 * (n + 1);
     out->rhs_off = d + 4 * (n + 1);
+    out->gtile_off = d + 5 * (n + 1);
     out->links = lo[n];
     out->tiles = to[n];
     out->nodes = no[n];
     out->leaves = fo[n];
     out->rhs = ko[n];
+    out->gtiles = go[n];
     out->kmax = kmax;
     out->kpmin = kpmin;
     out->any2d = any2d;
@@ -878,6 +900,37 @@ int ensure_near(fcg_ctx* ctx, Job& j) {
     return FCG_OK;
 }
 
+// Source lists of a job's gather tiles for the staged gather (once per job,
+// two launches and a host scan of the tile counts): only for jobs with 16-bit
+// source indices, once the gather rows are known.
+int ensure_tile_sources(fcg_ctx* ctx, Job& j) {
+    if (j.max_tile_src >= 0 || !j.col16.p || !j.grow.p || j.M < 1 || j.M > 65536) return FCG_OK;
+    hipStream_t s = ctx->stream;
+    const int64_t nt = fcg::gather_tiles(j.M);
+    DevBuf cnt;
+    HIPCHECK(ctx, cnt.alloc(sizeof(int32_t) * (size_t)nt));
+    HIPCHECK(ctx, fcg::launch_tile_sources_count(j.grow.as<fcg::GRow>(), j.col16.as<uint16_t>(), j.M,
+                                                 cnt.as<int32_t>(), s));
+    std::vector<int32_t> h((size_t)nt), off((size_t)nt + 1, 0);
+    HIPCHECK(ctx, hipMemcpyAsync(h.data(), cnt.p, sizeof(int32_t) * (size_t)nt, hipMemcpyDeviceToHost, s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    int mx = 0;
+    for (int64_t t = 0; t < nt; ++t) {
+        off[(size_t)t + 1] = off[(size_t)t] + h[(size_t)t];
+        mx = std::max(mx, h[(size_t)t]);
+    }
+    HIPCHECK(ctx, upload(j.tsrc_off, off.data(), off.size(), s));
+    HIPCHECK(ctx, j.tsrc.alloc(sizeof(uint16_t) * (size_t)std::max<int32_t>(off[(size_t)nt], 1)));
+    HIPCHECK(ctx, j.lsrc.alloc(sizeof(uint16_t) * (size_t)(j.L + kLinkSpare)));
+    HIPCHECK(ctx, hipMemsetAsync(j.lsrc.p, 0, sizeof(uint16_t) * (size_t)(j.L + kLinkSpare), s));
+    HIPCHECK(ctx, fcg::launch_tile_sources_fill(j.grow.as<fcg::GRow>(), j.col16.as<uint16_t>(), j.M,
+                                                j.tsrc_off.as<int32_t>(), j.tsrc.as<uint16_t>(),
+                                                j.lsrc.as<uint16_t>(), s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));
+    j.max_tile_src = mx;
+    return FCG_OK;
+}
+
 // Block preconditioner partition (RecurseComputePrecond,
 // Solver/SolveCapacitance.cpp:1094-1104, 1876-1993): every conductor's tree (roots
 // in leaf order) is walked depth first, left first; the first subtree met with at
@@ -1188,6 +1241,7 @@ int run_mvp(fcg_ctx* ctx, Job& j, int prec, const double* dq, double* dv, int k,
     if (prec == fcg::kMvp64 && !j.filled64) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in FP64");
     if (prec != fcg::kMvp64 && !j.filled32) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in mixed mode");
     RCCHECK(ensure_scratch(ctx, j.scr, j.N, j.M, j.pk(k), prec == fcg::kMvp32));
+    RCCHECK(ensure_tile_sources(ctx, j));
     HIPCHECK(ctx, hipStreamSynchronize(s));  // the table arena is reused below
     ctx->tabs.reset();
     BatchView bv;
@@ -1551,6 +1605,7 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
     S->want_x = E->want_x;
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
     RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, J->pk(S->k), mixed));
+    RCCHECK(ensure_tile_sources(ctx, *J));
     const size_t vec = (size_t)S->k * S->N;
     HIPCHECK(ctx, S->x.get(&ctx->pool, sizeof(double) * vec));
     HIPCHECK(ctx, S->r.get(&ctx->pool, sizeof(double) * vec));
@@ -2537,6 +2592,10 @@ int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v
     fcg::GatherSeg& g = gp.seg[0];
     g.row_lo = 0;
     g.row_hi = M;
+    g.tile_lo = 0;
+    g.tile_hi = fcg::gather_tiles(M);
+    g.staged = 0;  // FP64 product
+    g.stage_floats = 0;
     g.kpmin = kp;
     g.kmax = k;
     g.cls = fcg::gather_class(kp);
@@ -2645,6 +2704,7 @@ int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps,
             return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in this precision");
         const int k = std::max(1, (int)j->rhs.size());
         RCCHECK(ensure_scratch(ctx, j->scr, j->N, j->M, j->pk(k), mvp_prec == FCG_MVP_FP32));
+        RCCHECK(ensure_tile_sources(ctx, *j));
         const size_t n = (size_t)j->vlen() * k;
         qs.emplace_back(new DevBuf());
         vs.emplace_back(new DevBuf());
@@ -2785,7 +2845,8 @@ int autotune_prec(fcg_ctx* ctx, Job* j, int prec, int k, int reps, int* variant,
     for (auto& x : ev.e) HIPCHECK(ctx, hipEventCreate(&x));
     int best = 0;
     double best_t = 1e300;
-    for (int var = 0; var < fcg::gather_variant_count(); ++var) {
+    const int nv = fcg::gather_variant_count();
+    for (int var = 0; var < nv; ++var) {
         j->gather_variant[prec] = var;
         RCCHECK(run_mvp(ctx, *j, prec, dq.as<double>(), dv.as<double>(), k, 0, s));
         double t = 0.0;

@@ -26,7 +26,7 @@ constexpr int kNumXcd = 8;       // MI355X accelerator complex dies
 // Precision of one hierarchical product.
 enum MvpPrec : int {
     kMvp64 = 0,     // FP64 coefficients, FP64 charges and accumulation
-    kMvp32 = 1,     // FP32 coefficients and charges, FP64 sums in the gather (inner solves)
+    kMvp32 = 1,     // FP32 coefficients, charges and gather sums (inner solves)
     kMvp32Acc = 2,  // FP32 coefficients, FP64 charges and accumulation (refinement residual)
 };
 
@@ -60,6 +60,11 @@ struct BJob {
     const int32_t* iperm;     // [M] slot -> node
     const int32_t* rorder;    // [M] gather row order -> node
     const GRow* grow;         // [M] the rows in gather row order
+    // staged gather (null until built): tsrc[tsrc_off[t] ..] lists the source
+    // charge rows of tile t ascending; lsrc[l] is link l's index in its tile's list
+    const int32_t* tsrc_off;  // [ceil(M / kGatherTileRows) + 1]
+    const uint16_t* tsrc;
+    const uint16_t* lsrc;     // [L + 4]
     const SrcGeom* src_geom;  // [M] source geometry by charge row
     const uint32_t* fill_order;  // [L] links grouped by fill class
     const int64_t* fill_start;   // [kFillClasses + 1] first fill_order entry of each class
@@ -112,7 +117,8 @@ struct BatchView {
     const int64_t* node_off;  // nodes
     const int64_t* leaf_off;  // leaves
     const int64_t* rhs_off;   // right-hand sides
-    int64_t links, tiles, nodes, leaves, rhs;
+    const int64_t* gtile_off;  // staged-gather tiles (kGatherTileRows rows of one job)
+    int64_t links, tiles, nodes, leaves, rhs, gtiles;
     int kmax;   // largest k in the batch
     int kpmin;  // smallest row stride in the batch
     int any2d;
@@ -187,11 +193,32 @@ inline int gather_class(int kp) {
     return c;
 }
 struct GatherSeg {
-    int64_t row_lo, row_hi;  // batch rows [row_lo, row_hi)
-    int kpmin, kmax;         // smallest row stride and largest rhs count in the range
-    int cls, variant;        // rhs class and gather lane shape
-    int idx16;               // every job of the range has 16-bit source indices
+    int64_t row_lo, row_hi;    // batch rows [row_lo, row_hi)
+    int64_t tile_lo, tile_hi;  // its staged-gather tiles [tile_lo, tile_hi)
+    int kpmin, kmax;           // smallest row stride and largest rhs count in the range
+    int cls, variant;          // rhs class and gather lane shape
+    int idx16;                 // every job of the range has 16-bit source indices
+    int staged;                // every job of the range has its tile source lists (staged gather)
+    int stage_floats;          // the range's largest tile source list times kp (LDS elements)
 };
+
+// Staged gather (products of jobs with 16-bit source indices).  A job's
+// gather rows are cut into tiles of kGatherTileRows consecutive rows (same-depth
+// neighbours).  The block of a tile first copies the charges of the tile's
+// distinct sources into LDS and then reads every link's charge from there, by
+// the link's 16-bit index in that list; the loop's memory loads are then the
+// coefficient and index streams alone.  The lists are built once per job
+// (launch_tile_sources_count / _fill); a tile's list times the row stride in
+// the charge type must fit kStageMaxBytes of LDS, or the range gathers from
+// global memory.
+constexpr int kGatherTileRows = 128;
+constexpr size_t kStageMaxBytes = 65536;
+inline int64_t gather_tiles(int M) { return ((int64_t)M + kGatherTileRows - 1) / kGatherTileRows; }
+// cnt[t] = distinct sources of tile t (jobs of at most 65536 nodes)
+hipError_t launch_tile_sources_count(const GRow* grow, const uint16_t* col16, int M, int32_t* cnt, hipStream_t s);
+// the lists at tsrc_off and every link's index in its tile's list
+hipError_t launch_tile_sources_fill(const GRow* grow, const uint16_t* col16, int M, const int32_t* tsrc_off,
+                                    uint16_t* tsrc, uint16_t* lsrc, hipStream_t s);
 struct GatherPlan {
     int n = 0;
     GatherSeg seg[2 * kGatherClasses];
@@ -204,6 +231,8 @@ struct GatherPlan {
 hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan& plan, hipStream_t s,
                       hipEvent_t* ev = nullptr);
 int gather_variant_count();
+// whether a variant stages the tiles' source charges in LDS (FP32 products)
+bool gather_variant_staged(int variant);
 // gather lane shape used for rhs class `cls` in product precision `prec` unless
 // a job or context pins another one (measured best shapes of short-row batches)
 int default_gather_variant(int prec, int cls);

@@ -604,6 +604,251 @@ __global__ void __launch_bounds__(256) k_gather(BatchView bv, int blocks_per_xcd
     }
 }
 
+// ------------------------------------------------------------------ staged gather
+// One block per tile of kGatherTileRows gather rows of one job; TC, TQ, TA are
+// the coefficient, charge and accumulation types of k_gather.  The block copies
+// the charges of the tile's distinct sources (its
+// list tsrc, ascending charge rows) into LDS, then walks the tile's rows
+// kGatherWaves * RPW at a time with the lane shape of k_gather, reading each
+// link's charge from LDS by the link's index in the list (lsrc).  The rows of a
+// tile are same-depth neighbours (gather row order): 128 rows of a 32768-panel
+// plate pair have about 1000 distinct sources for 23000 links.  With no charge
+// loads from global memory in the loop, the link quads can be fetched two steps
+// ahead (the memory counter waits on them alone).
+template <typename TC, typename TQ, typename TA, int KR, int VR, int U, int RPW>
+__global__ void __launch_bounds__(256) k_gather_staged(BatchView bv, int blocks_per_xcd, int64_t tile_lo, int64_t tile_hi) {
+    constexpr int G = 64 / RPW;  // lanes per destination row
+    constexpr int SL = G / KR;   // link slots per row
+    constexpr int W = KR * VR;
+    constexpr int Q = U >= 8 ? 2 : 1;
+    constexpr int kPass = kGatherWaves * RPW;  // rows of one pass of the block
+    using V = VecT<TQ, VR>;
+    using CI = VecT<uint16_t, 4>;
+    using CC = VecT<TC, 4>;
+    extern __shared__ __align__(16) unsigned char stage_raw[];  // the largest list of the launch times kp
+    TQ* stage = reinterpret_cast<TQ*>(stage_raw);
+    const int64_t tg = tile_lo + (int64_t)(blockIdx.x % kNumXcd) * blocks_per_xcd + (blockIdx.x / kNumXcd);
+    if (tg >= tile_hi) return;  // the whole block: tg is uniform
+    const int jb = find_job_wave(bv.gtile_off, bv.njobs, tg);
+    const BJob& J = bv.jobs[jb];
+    const int t = (int)(tg - bv.gtile_off[jb]);
+    const int k = J.k, kp = J.kp, M = J.M;
+    const TQ* charge =
+        (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.charge64) : reinterpret_cast<const TQ*>(J.charge32);
+    {
+        const int s0 = J.tsrc_off[t];
+        const int nf = (J.tsrc_off[t + 1] - s0) * kp;
+        const uint16_t* src = J.tsrc + s0;
+        const int lk = __ffs(kp) - 1;  // kp is a power of two
+        for (int f = threadIdx.x; f < nf; f += 256) stage[f] = charge[(size_t)src[f >> lk] * kp + (f & (kp - 1))];
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int gl = lane % G;
+    const int slot = gl / KR;
+    const int rl = (gl % KR) * VR;
+    const TC* coef = (sizeof(TC) == 8) ? reinterpret_cast<const TC*>(J.coef64) : reinterpret_cast<const TC*>(J.coef32);
+    const uint16_t* cix = J.lsrc;
+    TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.pot64) : reinterpret_cast<TQ*>(J.pot32);
+    const int r_beg = t * kGatherTileRows;
+    const int r_end = min(M, r_beg + kGatherTileRows);
+    for (int row = r_beg + (int)(threadIdx.x >> 6) * RPW + lane / G; row < r_end; row += kPass) {
+        const GRow rr = J.grow[row];
+        // link offsets fit in 32 bits (GRow), so the quad arithmetic is 32-bit
+        const uint32_t beg = rr.beg, end = rr.end;
+        const TA sf = (TA)J.diag[rr.node];
+        const size_t orow = (size_t)rr.prow * kp;
+        const int q0 = (int)(beg >> 2), q1 = (int)((end + 3) >> 2);
+        for (int r0 = 0; r0 < k; r0 += W) {
+            const bool active = r0 + rl < k;
+            TQ selfc[VR];
+#pragma unroll
+            for (int v = 0; v < VR; ++v) selfc[v] = (slot == 0 && r0 + rl + v < k) ? charge[orow + r0 + rl + v] : (TQ)0;
+            TA acc[VR];
+#pragma unroll
+            for (int v = 0; v < VR; ++v) acc[v] = (TA)0;
+            if (active) {
+                // Two-stage ring of raw link quads, unrolled by two so that no
+                // register copy waits on a load in flight: while step j is
+                // consumed (charges from LDS), the quads of step j + 1 are in
+                // flight and those of step j + 2 are issued.  Quads are unpacked
+                // (masked to the row's links) only where they are used.
+                struct Raw {
+                    CI iv[Q];
+                    CC cv[Q];
+                };
+                auto fetch = [&](int qs, Raw& r) {
+#pragma unroll
+                    for (int qq = 0; qq < Q; ++qq) {
+                        const int q = qs + qq * SL;
+                        if (q < q1) {
+                            r.iv[qq] = *reinterpret_cast<const CI*>(cix + 4 * (size_t)q);
+                            r.cv[qq] = *reinterpret_cast<const CC*>(coef + 4 * (size_t)q);
+                        } else {
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) {
+                                r.iv[qq].v[u] = 0;
+                                r.cv[qq].v[u] = (TC)0;
+                            }
+                        }
+                    }
+                };
+                auto consume = [&](int qs, const Raw& r) {
+                    int c[4 * Q];
+                    TA w[4 * Q];
+#pragma unroll
+                    for (int qq = 0; qq < Q; ++qq)
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) {
+                            const uint32_t ll = 4u * (uint32_t)(qs + qq * SL) + (uint32_t)u;
+                            const bool in = ll >= beg && ll < end;
+                            c[4 * qq + u] = in ? (int)r.iv[qq].v[u] : 0;  // a row with links has list entry 0
+                            w[4 * qq + u] = in ? (TA)r.cv[qq].v[u] : (TA)0;
+                        }
+                    V ch[4 * Q];
+#pragma unroll
+                    for (int u = 0; u < 4 * Q; ++u) ch[u] = *reinterpret_cast<const V*>(stage + c[u] * kp + r0 + rl);
+#pragma unroll
+                    for (int u = 0; u < 4 * Q; ++u)
+#pragma unroll
+                        for (int v = 0; v < VR; ++v) acc[v] += w[u] * (TA)ch[u].v[v];
+                };
+                constexpr int kStep = Q * SL;
+                Raw ra, rb;
+                int qa = q0 + slot, qb = qa + kStep;
+                fetch(qa, ra);
+                fetch(qb, rb);
+                while (qa < q1) {
+                    consume(qa, ra);
+                    qa = qb + kStep;
+                    fetch(qa, ra);
+                    if (qb >= q1) break;
+                    consume(qb, rb);
+                    qb = qa + kStep;
+                    fetch(qb, rb);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < VR; ++v)
+#pragma unroll
+                for (int off = KR; off < G; off <<= 1) acc[v] += __shfl_xor(acc[v], off, 64);
+            if (slot == 0) {
+#pragma unroll
+                for (int v = 0; v < VR; ++v)
+                    if (r0 + rl + v < k) pot[orow + r0 + rl + v] = (TQ)(sf * (TA)selfc[v] + acc[v]);
+            }
+        }
+    }
+}
+
+// Source lists of a job's tiles (one block per tile).  The tile's sources are
+// marked in a bitmap over the job's charge rows (M <= 65536: 8 KB of LDS); the
+// count is the bitmap's population, the list its set bits in order, and a
+// link's index the population below its source's bit.
+constexpr int kTileBitWords = 65536 / 32;
+
+__device__ void tile_source_bitmap(const GRow* __restrict__ grow, const uint16_t* __restrict__ col16, int M, int t,
+                                   uint32_t* bits) {
+    const int nw = (M + 31) >> 5;
+    for (int i = threadIdx.x; i < nw; i += blockDim.x) bits[i] = 0u;
+    __syncthreads();
+    const int r1 = min(M, (t + 1) * kGatherTileRows);
+    for (int r = t * kGatherTileRows + (int)(threadIdx.x >> 6); r < r1; r += (int)(blockDim.x >> 6)) {
+        const GRow g = grow[r];
+        for (uint32_t l = g.beg + (threadIdx.x & 63); l < g.end; l += 64) {
+            const int c = col16[l];
+            atomicOr(&bits[c >> 5], 1u << (c & 31));
+        }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) k_tile_src_count(const GRow* __restrict__ grow,
+                                                        const uint16_t* __restrict__ col16, int M,
+                                                        int32_t* __restrict__ cnt) {
+    __shared__ uint32_t bits[kTileBitWords];
+    __shared__ int wsum[kGatherWaves];
+    tile_source_bitmap(grow, col16, M, blockIdx.x, bits);
+    const int nw = (M + 31) >> 5;
+    int n = 0;
+    for (int i = threadIdx.x; i < nw; i += 256) n += __popc(bits[i]);
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = n;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int tot = 0;
+        for (int w = 0; w < kGatherWaves; ++w) tot += wsum[w];
+        cnt[blockIdx.x] = tot;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_tile_src_fill(const GRow* __restrict__ grow,
+                                                       const uint16_t* __restrict__ col16, int M,
+                                                       const int32_t* __restrict__ tsrc_off,
+                                                       uint16_t* __restrict__ tsrc, uint16_t* __restrict__ lsrc) {
+    constexpr int kPer = kTileBitWords / 256;  // bitmap words per thread
+    __shared__ uint32_t bits[kTileBitWords];
+    __shared__ int pre[kTileBitWords];  // set bits below each word
+    __shared__ int wsum[kGatherWaves];
+    const int t = blockIdx.x;
+    tile_source_bitmap(grow, col16, M, t, bits);
+    // exclusive prefix of the words' populations: each thread owns kPer
+    // consecutive words; the thread totals are scanned by shuffles inside the
+    // wavefronts and the wavefront totals in LDS
+    const int nw = (M + 31) >> 5;
+    const int w0 = threadIdx.x * kPer;
+    int mine = 0;
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) mine += (w0 + i < nw) ? __popc(bits[w0 + i]) : 0;
+    const int lane = threadIdx.x & 63;
+    int incl = mine;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += o;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    int base = incl - mine;
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) base += wsum[w];
+    const int s0 = tsrc_off[t];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int wi = w0 + i;
+        if (wi >= nw) break;
+        pre[wi] = base;
+        uint32_t b = bits[wi];
+        int r = base;
+        while (b) {
+            const int bit = __ffs(b) - 1;
+            tsrc[s0 + r++] = (uint16_t)(wi * 32 + bit);
+            b &= b - 1u;
+        }
+        base = r;
+    }
+    __syncthreads();
+    const int r1 = min(M, (t + 1) * kGatherTileRows);
+    for (int row = t * kGatherTileRows + (int)(threadIdx.x >> 6); row < r1; row += kGatherWaves) {
+        const GRow g = grow[row];
+        for (uint32_t l = g.beg + lane; l < g.end; l += 64) {
+            const int c = col16[l];
+            lsrc[l] = (uint16_t)(pre[c >> 5] + __popc(bits[c >> 5] & ((1u << (c & 31)) - 1u)));
+        }
+    }
+}
+
+hipError_t launch_tile_sources_count(const GRow* grow, const uint16_t* col16, int M, int32_t* cnt, hipStream_t s) {
+    if (M < 1 || M > 65536) return hipErrorInvalidValue;
+    k_tile_src_count<<<(unsigned)gather_tiles(M), 256, 0, s>>>(grow, col16, M, cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_sources_fill(const GRow* grow, const uint16_t* col16, int M, const int32_t* tsrc_off,
+                                    uint16_t* tsrc, uint16_t* lsrc, hipStream_t s) {
+    if (M < 1 || M > 65536) return hipErrorInvalidValue;
+    k_tile_src_fill<<<(unsigned)gather_tiles(M), 256, 0, s>>>(grow, col16, M, tsrc_off, tsrc, lsrc);
+    return hipGetLastError();
+}
+
 struct GatherCfg {
     int kr, vr, u, rpw;
 };
@@ -614,13 +859,19 @@ static const GatherCfg kGatherCfgs[] = {
     {1, 4, 4, 1}, {16, 1, 2, 1}, {2, 2, 4, 1}, {4, 8, 2, 1}, {8, 4, 2, 1}, {16, 2, 2, 1}, {2, 2, 4, 2},
     {1, 4, 4, 2}, {2, 4, 4, 2}, {4, 2, 4, 2}, {1, 2, 4, 4}, {2, 2, 4, 4}, {1, 2, 2, 4}, {1, 2, 4, 8},
     {2, 2, 4, 8}, {1, 1, 4, 8}, {1, 4, 4, 8}, {1, 2, 2, 8}, {1, 2, 4, 16}, {1, 1, 4, 16}, {1, 2, 2, 16},
+    {1, 2, 8, 8}, {1, 4, 8, 8}, {1, 2, 8, 4},
 };
 constexpr int kNumGatherCfgs = sizeof(kGatherCfgs) / sizeof(kGatherCfgs[0]);
-int gather_variant_count() { return kNumGatherCfgs; }
+// variants [0, kNumGatherCfgs) read charges from global memory; the next
+// kNumGatherCfgs are the same lane shapes with the tile's window staged in LDS
+// (k_gather_staged), used where the range allows it and the shape otherwise
+int gather_variant_count() { return 2 * kNumGatherCfgs; }
+bool gather_variant_staged(int variant) { return variant >= kNumGatherCfgs; }
 
 // VR must divide every job's row stride (vector loads stay inside one row), and
 // lanes beyond the largest rhs count are dropped
 static GatherCfg gather_shape(int variant, int kpmin, int kmax) {
+    if (variant >= kNumGatherCfgs) variant -= kNumGatherCfgs;
     GatherCfg c = kGatherCfgs[(variant >= 0 && variant < kNumGatherCfgs) ? variant : 0];
     const int G = 64 / c.rpw;
     while (c.vr > kpmin) {
@@ -643,16 +894,52 @@ static int find_cfg(int kr, int vr, int u, int rpw) {
 int default_gather_variant(int prec, int cls) {
     // fastest shapes measured per rhs class: few rhs want several rows per wavefront
     // (rows average 70-160 links there), many rhs want one row per wavefront with
-    // the lanes spread over the rhs
+    // the lanes spread over the rhs.  Every default stages the tiles' source
+    // charges in LDS (on the batch: FP32 gather 1.05 -> 0.80 ms, residual 1.59 ->
+    // 1.08 ms, FP64 1.47 -> 1.20 ms); a range whose lists do not fit in LDS
+    // gathers with the same shape from global memory.
     const bool f32 = prec == kMvp32;
+    const int st = kNumGatherCfgs;
     switch (cls) {
         case 0:
-        case 1: return find_cfg(1, 2, 4, 8);
-        case 2: return f32 ? find_cfg(1, 4, 4, 8) : find_cfg(2, 2, 4, 8);
-        case 3: return find_cfg(2, 4, 4, 2);
-        case 4: return find_cfg(8, 4, 2, 1);
-        default: return f32 ? find_cfg(4, 8, 2, 1) : find_cfg(8, 4, 2, 1);
+        case 1: return st + find_cfg(1, 2, 4, 8);
+        case 2: return st + (f32 ? find_cfg(1, 4, 4, 8) : find_cfg(2, 2, 4, 8));
+        case 3: return st + find_cfg(2, 4, 4, 2);
+        case 4: return st + find_cfg(8, 4, 2, 1);
+        default: return st + (f32 ? find_cfg(4, 8, 2, 1) : find_cfg(8, 4, 2, 1));
     }
+}
+
+template <typename TC, typename TQ, typename TA>
+static hipError_t gather_dispatch_staged(const BatchView& bv, const GatherSeg& sg, hipStream_t s) {
+    const GatherCfg c = gather_shape(sg.variant, sg.kpmin, sg.kmax);
+    const int64_t blocks = sg.tile_hi - sg.tile_lo;
+    const int64_t per_xcd = (blocks + kNumXcd - 1) / kNumXcd;
+    const unsigned grid = (unsigned)(per_xcd * kNumXcd);
+    if (grid == 0) return hipSuccess;
+    const size_t lds = sizeof(TQ) * (size_t)sg.stage_floats;
+    if (sg.stage_floats < 1 || lds > kStageMaxBytes) return hipErrorInvalidConfiguration;
+#define FCG_S(KR, VR, U, R)                                                                                    \
+    if (c.kr == KR && c.vr == VR && c.u == U && c.rpw == R) {                                                  \
+        k_gather_staged<TC, TQ, TA, KR, VR, U, R><<<grid, 256, lds, s>>>(bv, (int)per_xcd, sg.tile_lo, sg.tile_hi); \
+        return hipGetLastError();                                                                              \
+    }
+    // every shape gather_shape can return
+    FCG_S(1, 1, 2, 1) FCG_S(1, 1, 4, 1) FCG_S(1, 2, 2, 1) FCG_S(1, 2, 4, 1) FCG_S(1, 4, 2, 1)
+    FCG_S(1, 4, 4, 1) FCG_S(1, 8, 2, 1) FCG_S(1, 8, 4, 1) FCG_S(2, 1, 2, 1) FCG_S(2, 1, 4, 1)
+    FCG_S(2, 2, 2, 1) FCG_S(2, 2, 4, 1) FCG_S(2, 4, 2, 1) FCG_S(2, 4, 4, 1) FCG_S(2, 8, 2, 1)
+    FCG_S(4, 1, 2, 1) FCG_S(4, 1, 4, 1) FCG_S(4, 2, 2, 1) FCG_S(4, 2, 4, 1) FCG_S(4, 4, 2, 1)
+    FCG_S(4, 8, 2, 1) FCG_S(8, 1, 2, 1) FCG_S(8, 1, 4, 1) FCG_S(8, 2, 2, 1) FCG_S(8, 4, 2, 1)
+    FCG_S(16, 1, 2, 1) FCG_S(16, 2, 2, 1) FCG_S(1, 1, 4, 2) FCG_S(1, 2, 4, 2) FCG_S(1, 4, 4, 2)
+    FCG_S(2, 1, 4, 2) FCG_S(2, 2, 4, 2) FCG_S(2, 4, 4, 2) FCG_S(4, 1, 4, 2) FCG_S(4, 2, 4, 2)
+    FCG_S(8, 1, 4, 2) FCG_S(1, 1, 2, 4) FCG_S(1, 1, 4, 4) FCG_S(1, 2, 2, 4) FCG_S(1, 2, 4, 4)
+    FCG_S(2, 1, 2, 4) FCG_S(2, 1, 4, 4) FCG_S(2, 2, 4, 4) FCG_S(4, 1, 4, 4) FCG_S(1, 1, 2, 8)
+    FCG_S(1, 1, 4, 8) FCG_S(1, 2, 2, 8) FCG_S(1, 2, 4, 8) FCG_S(1, 4, 4, 8) FCG_S(2, 1, 2, 8)
+    FCG_S(2, 1, 4, 8) FCG_S(2, 2, 4, 8) FCG_S(1, 1, 2, 16) FCG_S(1, 1, 4, 16) FCG_S(1, 2, 2, 16)
+    FCG_S(1, 2, 4, 16) FCG_S(1, 2, 8, 8) FCG_S(1, 4, 8, 8) FCG_S(1, 2, 8, 4) FCG_S(1, 1, 8, 8)
+    FCG_S(1, 1, 8, 4) FCG_S(2, 1, 8, 4) FCG_S(2, 1, 8, 8) FCG_S(1, 4, 8, 4) FCG_S(2, 2, 8, 8) FCG_S(2, 2, 8, 4)
+#undef FCG_S
+    return hipErrorInvalidConfiguration;
 }
 
 template <typename TC, typename TQ, typename TA>
@@ -684,7 +971,8 @@ static hipError_t gather_dispatch(const BatchView& bv, const GatherSeg& sg, hipS
     FCG_G(2, 1, 2, 4) FCG_G(2, 1, 4, 4) FCG_G(2, 2, 4, 4) FCG_G(4, 1, 4, 4) FCG_G(1, 1, 2, 8)
     FCG_G(1, 1, 4, 8) FCG_G(1, 2, 2, 8) FCG_G(1, 2, 4, 8) FCG_G(1, 4, 4, 8) FCG_G(2, 1, 2, 8)
     FCG_G(2, 1, 4, 8) FCG_G(2, 2, 4, 8) FCG_G(1, 1, 2, 16) FCG_G(1, 1, 4, 16) FCG_G(1, 2, 2, 16)
-    FCG_G(1, 2, 4, 16)
+    FCG_G(1, 2, 4, 16) FCG_G(1, 2, 8, 8) FCG_G(1, 4, 8, 8) FCG_G(1, 2, 8, 4) FCG_G(1, 1, 8, 8)
+    FCG_G(1, 1, 8, 4) FCG_G(2, 1, 8, 4) FCG_G(2, 1, 8, 8) FCG_G(1, 4, 8, 4) FCG_G(2, 2, 8, 8) FCG_G(2, 2, 8, 4)
 #undef FCG_G
     return hipErrorInvalidConfiguration;
 }
@@ -873,12 +1161,20 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int i = 0; i < plan.n; ++i) {
         hipError_t ge;
+        const GatherSeg& sg = plan.seg[i];
+        // staged when the variant asks for it, every job has its tile source lists
+        // and the longest list fits in LDS in this charge type
+        const bool st = gather_variant_staged(sg.variant) && sg.staged && sg.idx16 &&
+                        (size_t)sg.stage_floats * (prec == kMvp32 ? 4 : 8) <= kStageMaxBytes;
         if (prec == kMvp64)
-            ge = gather_dispatch<double, double, double>(b, plan.seg[i], s);
+            ge = st ? gather_dispatch_staged<double, double, double>(b, sg, s)
+                    : gather_dispatch<double, double, double>(b, sg, s);
         else if (prec == kMvp32)
-            ge = gather_dispatch<float, float, float>(b, plan.seg[i], s);
+            ge = st ? gather_dispatch_staged<float, float, float>(b, sg, s)
+                    : gather_dispatch<float, float, float>(b, sg, s);
         else
-            ge = gather_dispatch<float, double, double>(b, plan.seg[i], s);
+            ge = st ? gather_dispatch_staged<float, double, double>(b, sg, s)
+                    : gather_dispatch<float, double, double>(b, sg, s);
         if (ge != hipSuccess) return ge;
     }
     if (prec == kMvp32Acc) k_near_correct<8><<<blocks_for((size_t)b.nodes, 256), 256, 0, s>>>(b);
