@@ -114,7 +114,7 @@ def copy_ceiling(mib=1024, reps=5):
 
 
 def ncond(mesh):
-    return int((mesh.group_is_diel == 0).sum())
+    return mesh.n_cond
 
 
 def cpu_sample(meshes, idx, tol, super_dim=0):
@@ -161,6 +161,8 @@ def bench_batch(args, fc, dev, tmp):
     # no extra tuning launches (the rocprof averages of the same command then cover
     # the solve launches only)
     tuned = {}
+    if args.gather_variant >= 0:  # one pinned FP32 gather shape for every rhs class (comparisons)
+        dev.set_batch_variant(fc.FCG_MVP_FP32, args.gather_variant)
     if args.only_mixed:  # profiling aid: account for every product of the process
         dev.profile(enable=1, reset=True)
     for _ in range(max(1, args.warmup)):
@@ -310,7 +312,7 @@ def bench_batch(args, fc, dev, tmp):
                 "avg_ms": prof["fp32_acc64"]["mvp_ms"] / max(1, prof["fp32_acc64"]["mvp_launches"])},
         },
         "roofline": {
-            "kernel": "k_gather<float,float,float,*> (inner FP32 products of the timed batched solves)",
+            "kernel": "k_gather_staged<float,float,float,*> (inner FP32 products of the timed batched solves)",
             "bound": "hbm",
             "achieved": achieved,
             "peak": PEAK_HBM_GBS,
@@ -408,7 +410,7 @@ def bench_serve(args, fc, dev, tmp):
         "closed_batch_capacity_solves_per_s": cap_rate,
         "max_active": rep.max_active,
         "c_rel_err_vs_fp64_tol1e-8": {"max_per_entry": max(err[0], err_h[0]), "max_vs_diagonal": max(err[1], err_h[1])},
-        "roofline": {"kernel": "k_gather<float,float,float,*> (inner FP32 products while serving)", "bound": "hbm",
+        "roofline": {"kernel": "k_gather_staged<float,float,float,*> (inner FP32 products while serving)", "bound": "hbm",
                      "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS,
                      "traffic": None, "products": g["gather_launches"], "kernel_launches": g["gather_kernels"],
                      "avg_gather_ms_per_product": g["gather_ms"] / max(1, g["gather_launches"])},
@@ -509,7 +511,9 @@ def bench_single(args, fc, dev, tmp):
                 "stage_ms": None if tm is None else {s: tm["ms_" + s] for s in ("scan", "charge", "gather", "down")},
                 "gather_variant": None if tm is None else tm["variant"]},
         "c_rel_err_vs_fp64_tol1e-8": {"max_per_entry": e_pe, "max_vs_diagonal": e_pd},
-        "roofline": {"kernel": "k_gather (products of the timed solves)", "bound": "hbm", "achieved": achieved,
+        # jobs of at most 65536 nodes gather with their tiles' sources staged in LDS
+        "roofline": {"kernel": ("k_gather_staged" if mesh.M <= 65536 else "k_gather") + " (products of the timed solves)",
+                     "bound": "hbm", "achieved": achieved,
                      "peak": PEAK_HBM_GBS, "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
                      "products": g["gather_launches"], "kernel_launches": g["gather_kernels"],
                      "avg_gather_ms_per_product": g["gather_ms"] / max(1, g["gather_launches"]),
@@ -540,6 +544,8 @@ def main():
     ap.add_argument("--reps", type=int, default=10, help="timed repetitions of the product for the roofline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only-mixed", action="store_true", help="batch: run only the timed mixed solves (profiling)")
+    ap.add_argument("--gather-variant", type=int, default=-1,
+                    help="batch: pin the FP32 gather shape of every rhs class (fcg_set_batch_variant)")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="only copy this many MiB device-to-device (reference kernel for memory counters) and exit")
     ap.add_argument("--cpu-rows", type=int, default=1, help="single-job CPU sample: right-hand sides")

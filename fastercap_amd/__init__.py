@@ -103,6 +103,7 @@ class Mesh:
         self.leaf_hi = _np(d.leaf_hi, self.M, np.int32)
         self.leaf_node = _np(d.leaf_node, self.N, np.int32)
         self.group_is_diel = _np(d.group_is_diel, self.G, np.int32)
+        self.n_cond = int((self.group_is_diel == 0).sum())  # conductor groups: the C matrix order
         self.group_leaf_start = _np(d.group_leaf_start, self.G + 1, np.int32)
         self.group_outperm = _np(d.group_outperm, self.G, np.float64)
         self.group_inperm = _np(d.group_inperm, self.G, np.float64)
@@ -338,7 +339,7 @@ class Device:
         max_refine: int = 8,
         precond_size: int = 0,
     ) -> SolveResult:
-        ncond = int((mesh.group_is_diel == 0).sum())
+        ncond = mesh.n_cond
         rows_max = ncond
         nf = 2 if mesh.complex else 1  # complex jobs: real then imaginary part
         cap = np.zeros(nf * rows_max * ncond, dtype=np.float64)
@@ -396,7 +397,7 @@ class Device:
         """Solve several filled jobs together (one launch per stage for all of
         them); returns one SolveResult per job, in order."""
         jobs = list(jobs)
-        ncs = [int((m.group_is_diel == 0).sum()) for m in meshes]
+        ncs = [m.n_cond for m in meshes]
         caps = np.zeros(sum(n * n * (2 if m.complex else 1) for n, m in zip(ncs, meshes)), dtype=np.float64)
         ja = (C.c_int * len(jobs))(*jobs)
         rows = (C.c_int * len(jobs))()

@@ -1,6 +1,6 @@
 """Summaries of rocprofv3 CSV output for this library's kernels.
 
-    python -m fastercap_amd.rocprof_summary stats KERNEL_STATS.csv [--top N] [--group PREFIX ...]
+    python -m fastercap_amd.rocprof_summary stats KERNEL_STATS.csv|RESULTS.db [--top N] [--group PREFIX ...]
         per-kernel totals (time, calls, average) and, for every PREFIX, the sum over
         all kernels whose name starts with it (template instantiations of one kernel)
     python -m fastercap_amd.rocprof_summary pmc COUNTER_COLLECTION.csv COUNTER PREFIX [--scale F]
@@ -22,6 +22,8 @@ import sys
 
 
 def kernel_stats(path: str):
+    if path.endswith(".db"):
+        return _db_kernel_stats(path)
     with open(path) as f:
         rows = list(csv.DictReader(f))
     out = []
@@ -33,6 +35,24 @@ def kernel_stats(path: str):
             "avg_us": float(r["AverageNs"]) / 1e3,
             "pct": float(r["Percentage"]),
         })
+    out.sort(key=lambda r: -r["total_ms"])
+    return out
+
+
+def _db_kernel_stats(db: str):
+    """The same per-kernel totals from a rocprofv3 database (the default rocpd
+    output of --kernel-trace --stats)."""
+    import sqlite3
+
+    con = sqlite3.connect(db)
+    per = {}
+    for name, a, b in con.execute("select name, start, end from kernels"):
+        e = per.setdefault(name, [0, 0.0])
+        e[0] += 1
+        e[1] += (b - a) / 1e6
+    tot = sum(v[1] for v in per.values()) or 1.0
+    out = [{"name": n, "calls": v[0], "total_ms": v[1], "avg_us": 1e3 * v[1] / v[0], "pct": 100.0 * v[1] / tot}
+           for n, v in per.items()]
     out.sort(key=lambda r: -r["total_ms"])
     return out
 
