@@ -1279,21 +1279,24 @@ struct Solve {
     int i = 0;  // Arnoldi step index within the cycle (the next product uses Q[i])
     int cycles = 0;
     bool converged = false;     // every rhs met the tolerance (see fcg_solve_stats)
-    bool need_scale = false;    // Q[i] = src / d pending
-    bool scale_from_r = false;  // src is r (cycle start) or w (continuation)
+    bool need_scale = false;    // Q[0] = r / d pending (cycle start)
     std::vector<double> d;      // scale divisors per rhs
     std::vector<double> normb, beta, thr;
-    std::vector<std::vector<double>> H, cs, sn, g;
+    // Arnoldi state (Hessenberg columns, rotations, g) on the device: one DCGS2
+    // record per rhs (kernels.h dcgs_rec) of hcap columns; gres: the last
+    // residual estimate |g| of each rhs
+    PBuf hd;
+    int hcap = 0;
+    std::vector<double> gres;
     std::vector<int> done, last_i, total_steps;
-    std::vector<double> z;  // pending solution update [nz][k]
-    int nz = 0;
+    int nz = 0;  // columns of the pending solution update
     int cap_iter = 0;
     int64_t n_mvp = 0;
     double rel = 0.0;
     clk::time_point t0;
-    int hoff = 0;
+    int hoff = 0, ooff = 0;  // the step's reduction and host-output offsets in the step buffer
     int64_t bytes() const {
-        int64_t s = (int64_t)(x.r + r.r + w.r + u.r + spw.r + scr.bytes());
+        int64_t s = (int64_t)(x.r + r.r + w.r + u.r + spw.r + hd.r + scr.bytes());
         for (auto& q : qblocks) s += (int64_t)q->r;
         return s;
     }
@@ -1320,6 +1323,7 @@ struct Cohort {
     // context for the engine's life so that engines do not allocate them
     Tables* tabs = nullptr;
     PBuf partial_a, partial_b, hbuf;
+    PBuf coefb, zbuf;  // DCGS2 update coefficients and solution-update coefficients of a step
     ProfEvents pe_res, pe_arn;
     hipEvent_t ev = nullptr;      // end of the step in flight
     hipEvent_t tab_ev = nullptr;  // end of the step's table upload
@@ -1387,6 +1391,29 @@ int ensure_pbuf(fcg_ctx* ctx, PBuf& b, size_t bytes) {
 }
 
 // make sure basis vector index `idx` exists (device memory from the pool only)
+// Device Arnoldi records of a solve for `cols` Hessenberg columns, grown by
+// doubling.  The record layout does not depend on the capacity (dcgs_rec), so a
+// larger one starts with a copy of the smaller; the copy is ordered on the
+// compute stream, which is synchronised before the old block returns to the
+// pool (growth is rare: 48, 96, 192, ... columns).
+int ensure_hstate(fcg_ctx* ctx, Solve& S, int cols, hipStream_t s) {
+    if (cols <= S.hcap) return FCG_OK;
+    const int cap = std::max(cols, std::max(2 * S.hcap, 48));
+    const int64_t st_new = fcg::dcgs_rec(cap), st_old = fcg::dcgs_rec(S.hcap);
+    PBuf nb;
+    HIPCHECK(ctx, nb.get(&ctx->pool, sizeof(double) * (size_t)(st_new * S.k)));
+    if (S.hcap > 0) {
+        HIPCHECK(ctx, hipMemcpy2DAsync(nb.p, sizeof(double) * (size_t)st_new, S.hd.p, sizeof(double) * (size_t)st_old,
+                                       sizeof(double) * (size_t)st_old, (size_t)S.k, hipMemcpyDeviceToDevice, s));
+        HIPCHECK(ctx, hipStreamSynchronize(s));
+    }
+    std::swap(S.hd.p, nb.p);
+    std::swap(S.hd.r, nb.r);
+    std::swap(S.hd.pool, nb.pool);
+    S.hcap = cap;
+    return FCG_OK;
+}
+
 int ensure_basis(fcg_ctx* ctx, Solve& S, int idx) {
     const size_t vec = (size_t)S.k * S.N;
     const size_t elem = S.b32 ? sizeof(float) : sizeof(double);
@@ -1426,6 +1453,8 @@ struct KEnt {
     double* w;
     int nv;
     int zoff;
+    int coff = 0;      // DCGS2 update: coefficient offset
+    bool next = false;  // DCGS2 update: write the next provisional vector
 };
 int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, KrylovView* kv) {
     std::memset(kv, 0, sizeof *kv);
@@ -1438,16 +1467,41 @@ int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, Krylo
     std::vector<size_t> qoff;
     for (size_t e = 0; e < ents.size(); ++e) {
         Solve& S = *ents[e].S;
-        const size_t need = std::min(S.hq.size(), (size_t)std::max(ents[e].nv + 1, S.nz));
+        // slots 0..nv, the next one for a DCGS2 update, 0..nz-1 for a solution update
+        const size_t need = std::min(S.hq.size(), (size_t)std::max(ents[e].nv + (ents[e].next ? 2 : 1), S.nz));
         qoff.push_back(qaddr.size());
         qaddr.insert(qaddr.end(), S.hq.begin(), S.hq.begin() + need);
     }
     void* qd = nullptr;
     HIPCHECK(ctx, tabs.push(qaddr.data(), sizeof(void*) * qaddr.size(), ctx->stream, &qd));
     void* const* qbase = reinterpret_cast<void* const*>(qd);
+    // per rhs: Arnoldi flags (met the tolerance), last columns and cycle-start
+    // norms, for the DCGS2 kernels
+    std::vector<int32_t> iv;
+    std::vector<double> bv;
+    std::vector<size_t> ioff;
+    for (size_t e = 0; e < ents.size(); ++e) {
+        const Solve& S = *ents[e].S;
+        ioff.push_back(iv.size());
+        for (int q = 0; q < S.k; ++q) iv.push_back(q < (int)S.done.size() ? S.done[(size_t)q] : 0);
+        for (int q = 0; q < S.k; ++q) iv.push_back(q < (int)S.last_i.size() ? S.last_i[(size_t)q] : -1);
+        for (int q = 0; q < S.k; ++q) bv.push_back(q < (int)S.beta.size() ? S.beta[(size_t)q] : 0.0);
+    }
+    void *id = nullptr, *bd = nullptr;
+    HIPCHECK(ctx, tabs.push(iv.data(), sizeof(int32_t) * std::max<size_t>(iv.size(), 1), ctx->stream, &id));
+    HIPCHECK(ctx, tabs.push(bv.data(), sizeof(double) * std::max<size_t>(bv.size(), 1), ctx->stream, &bd));
     for (size_t e = 0; e < ents.size(); ++e) {
         Solve& S = *ents[e].S;
         kj.push_back(kjob_of(S, qbase + qoff[e], ents[e].w, ents[e].nv, ents[e].zoff));
+        KJob& J = kj.back();
+        J.coff = ents[e].coff;
+        J.next = ents[e].next ? 1 : 0;
+        J.hd = S.hd.as<double>();
+        J.hstride = fcg::dcgs_rec(S.hcap);
+        J.ooff = S.ooff;
+        J.flags = reinterpret_cast<const int32_t*>(id) + ioff[e];
+        J.last = J.flags + S.k;
+        J.beta = reinterpret_cast<const double*>(bd) + ioff[e] / 2;
         maxnv = std::max(maxnv, ents[e].nv);
         for (int r = 0; r < S.k; ++r) {
             Seg sg;
@@ -1484,12 +1538,8 @@ void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
     S.thr.assign((size_t)S.k, 0.0);
     S.done.assign((size_t)S.k, 0);
     S.last_i.assign((size_t)S.k, -1);
+    S.gres.assign((size_t)S.k, 0.0);
     for (int q = 0; q < S.k; ++q) {
-        S.H[(size_t)q].clear();
-        S.cs[(size_t)q].clear();
-        S.sn[(size_t)q].clear();
-        S.g[(size_t)q].assign((size_t)S.cap_iter + 2, 0.0);
-        S.g[(size_t)q][0] = beta[q];
         if (!S.mixed) {
             // reference stop rule |g_{i+1}| / |b| < tol (SolveCapacitance.cpp:2493-2500)
             S.thr[(size_t)q] = tol * S.normb[(size_t)q];
@@ -1507,7 +1557,6 @@ void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
     S.i = 0;
     S.cycles++;
     S.need_scale = true;
-    S.scale_from_r = true;
     S.d.assign(beta, beta + S.k);
     S.phase = kArn;
     bool any = false;
@@ -1516,60 +1565,16 @@ void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
 }
 
 // back substitution of every rhs; z[jj][r], zero beyond each rhs's own count
+// The cycle ends: its solution update uses the least squares over each rhs's
+// columns (k_dcgs_lsq on the device), nz = the longest.
 void prepare_update(Solve& S) {
     int nz = 0;
     for (int q = 0; q < S.k; ++q) nz = std::max(nz, S.last_i[(size_t)q] + 1);
     S.nz = nz;
-    S.z.assign((size_t)nz * S.k, 0.0);
-    for (int q = 0; q < S.k; ++q) {
-        const int it = S.last_i[(size_t)q];
-        if (it < 0) continue;
-        std::vector<double> zz(S.g[(size_t)q].begin(), S.g[(size_t)q].begin() + it + 1);
-        const std::vector<double>& Hq = S.H[(size_t)q];
-        for (int ii = it; ii >= 0; --ii) {
-            const double* col = Hq.data() + (size_t)ii * (ii + 1) / 2;
-            zz[(size_t)ii] = zz[(size_t)ii] / col[ii];
-            for (int jj = ii - 1; jj >= 0; --jj) zz[(size_t)jj] -= col[jj] * zz[(size_t)ii];
-        }
-        for (int jj = 0; jj <= it; ++jj) S.z[(size_t)jj * S.k + q] = zz[(size_t)jj];
-    }
     S.phase = kUpdate;
 }
 
 // Givens update of one Arnoldi step of every rhs (h1 + h2 = column, |w| = subdiagonal)
-void givens_step(Solve& S, const double* h1, const double* h2, const double* w2) {
-    const int i = S.i;
-    const int k = S.k;
-    S.d.assign((size_t)k, 0.0);
-    for (int q = 0; q < k; ++q) {
-        const double hn = std::sqrt(w2[q]);
-        S.d[(size_t)q] = hn;
-        if (S.done[(size_t)q]) continue;
-        std::vector<double>& Hq = S.H[(size_t)q];
-        // rotated columns packed: column i (rows 0..i) at i(i+1)/2
-        Hq.resize((size_t)(i + 1) * (i + 2) / 2);
-        double* colv = Hq.data() + (size_t)i * (i + 1) / 2;
-        for (int jj = 0; jj <= i; ++jj) colv[jj] = h1[(size_t)jj * k + q] + h2[(size_t)jj * k + q];
-        auto& c = S.cs[(size_t)q];
-        auto& sv = S.sn[(size_t)q];
-        for (int jj = 0; jj < i; ++jj) {
-            const double t1 = colv[jj], t2 = colv[jj + 1];
-            colv[jj] = c[(size_t)jj] * t1 - sv[(size_t)jj] * t2;
-            colv[jj + 1] = c[(size_t)jj] * t2 + sv[(size_t)jj] * t1;
-        }
-        const double t1 = colv[i], t2 = hn;
-        const double len = std::sqrt(t1 * t1 + t2 * t2);
-        c.push_back(t1 / len);
-        sv.push_back(-t2 / len);
-        colv[i] = c[(size_t)i] * t1 - sv[(size_t)i] * t2;
-        const double gi = S.g[(size_t)q][(size_t)i];
-        S.g[(size_t)q][(size_t)i] = c[(size_t)i] * gi;
-        S.g[(size_t)q][(size_t)i + 1] = sv[(size_t)i] * gi;
-        S.last_i[(size_t)q] = i;
-        S.total_steps[(size_t)q]++;
-        if (std::fabs(S.g[(size_t)q][(size_t)i + 1]) < S.thr[(size_t)q]) S.done[(size_t)q] = 1;
-    }
-}
 
 int engine_submit(fcg_engine* E, int job, int64_t tag) {
     fcg_ctx* ctx = E->ctx;
@@ -1626,10 +1631,6 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
         S->normb[(size_t)q] = std::sqrt((double)(rg.second - rg.first));
     }
     RCCHECK(ensure_basis(ctx, *S, 1));
-    S->H.assign((size_t)S->k, {});
-    S->cs.assign((size_t)S->k, {});
-    S->sn.assign((size_t)S->k, {});
-    S->g.assign((size_t)S->k, {});
     S->total_steps.assign((size_t)S->k, 0);
     S->t0 = clk::now();
     S->phase = kInit;
@@ -1674,7 +1675,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
     for (const Cohort& c : E->co) overlap |= (&c != &C && c.inflight);
     const hipStream_t us = overlap ? ctx->copy_stream : s, ds = overlap ? ctx->down_stream : s;
 
-    // ---- A: pending basis scalings, solution updates and starts
+    // ---- A: cycle starts' first basis vectors, solution updates and starts
     {
         std::vector<KEnt> sc;
         std::vector<double> dvals;
@@ -1682,7 +1683,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
             Solve& S = *up;
             if (S.phase != kArn || !S.need_scale) continue;
             RCCHECK(ensure_basis(ctx, S, S.i + 1));
-            sc.push_back(KEnt{&S, S.scale_from_r ? S.r.as<double>() : S.w.as<double>(), S.i, 0});
+            sc.push_back(KEnt{&S, S.r.as<double>(), S.i, 0});
             for (int q = 0; q < S.k; ++q) dvals.push_back(S.d[(size_t)q]);
         }
         if (!sc.empty()) {
@@ -1696,21 +1697,23 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         std::vector<KEnt> up;
         std::vector<PEnt> upp;  // block preconditioner: x += P (Q z)
         std::vector<SEnt> ups;  // two-level preconditioner, likewise
-        std::vector<double> zall;
+        int64_t nzall = 0;
         for (auto& u : C.active) {
             Solve& S = *u;
             if (S.phase != kUpdate) continue;
-            up.push_back(KEnt{&S, S.w.as<double>(), 0, (int)zall.size()});
-            zall.insert(zall.end(), S.z.begin(), S.z.end());
+            up.push_back(KEnt{&S, S.w.as<double>(), 0, (int)nzall});
+            nzall += (int64_t)S.nz * S.k;
             if (S.block) upp.push_back(PEnt{S.J, S.u.as<double>(), false, S.x.as<double>(), S.k});
             if (S.twolev) ups.push_back(SEnt{S.J, S.u.as<double>(), false, S.x.as<double>(), S.k, S.spw.as<double>()});
         }
         if (!up.empty()) {
             KrylovView kv;
             RCCHECK(make_krylov(ctx, *C.tabs, up, &kv));
-            void* dz;
-            HIPCHECK(ctx, C.tabs->push(zall.data(), sizeof(double) * zall.size(), s, &dz));
-            q.push_back([=] { return fcg::launch_xupdate(kv, reinterpret_cast<double*>(dz), s); });
+            RCCHECK(ensure_pbuf(ctx, C.zbuf, sizeof(double) * (size_t)std::max<int64_t>(nzall, 1)));
+            double* dz = C.zbuf.as<double>();
+            // the least squares of every ended cycle, then x += P (Q z)
+            q.push_back([=] { return fcg::launch_dcgs_lsq(kv, dz, s); });
+            q.push_back([=] { return fcg::launch_xupdate(kv, dz, s); });
             if (!upp.empty()) {
                 fcg::PView pv;
                 RCCHECK(make_pview(ctx, *C.tabs, upp, &pv));
@@ -1817,14 +1820,21 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         if (u->phase == kResid) res.push_back(u.get());
         if (u->phase == kArn) arn.push_back(u.get());
     }
+    // step buffer: what the host reads back first (residual norms, then each
+    // Arnoldi rhs's status and residual estimate), then the Arnoldi projections
     int hoff = 0;
     for (Solve* S : res) {
         S->hoff = hoff;
         hoff += S->k;
     }
     for (Solve* S : arn) {
+        S->ooff = hoff;
+        hoff += 2 * S->k;
+    }
+    const int hdown = hoff;
+    for (Solve* S : arn) {
         S->hoff = hoff;
-        hoff += (2 * (S->i + 1) + 1) * S->k;
+        hoff += (2 * S->i + 3) * S->k;  // DCGS2 projection of step i
     }
     RCCHECK(ensure_pbuf(ctx, C.hbuf, sizeof(double) * (size_t)std::max(hoff, 1)));
     if (!res.empty()) {
@@ -1852,14 +1862,18 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         q.push_back([=] { return fcg::launch_reduce_norm(kv, pa, hb, s); });
     }
 
-    // ---- C: one Arnoldi step (product + two Gram-Schmidt passes) of every active cycle
+    // ---- C: one Arnoldi step of every active cycle: the product on the
+    // provisional vector, its projection, the Hessenberg column and the update
+    // that completes the basis vector (DCGS2, kernels.hip k_dcgs_*)
     if (!arn.empty()) {
         std::vector<BatchEntry> be;
         std::vector<KEnt> ke;
         std::vector<PEnt> pe;  // block preconditioner: the product runs on u = P q_i
         std::vector<SEnt> se;  // two-level preconditioner, likewise
+        int64_t ncoef = 0;
         for (Solve* S : arn) {
             RCCHECK(ensure_basis(ctx, *S, S->i + 1));
+            RCCHECK(ensure_hstate(ctx, *S, S->i + 1, s));
             void* qi = S->hq[(size_t)S->i];
             if (S->block || S->twolev) {
                 if (S->block)
@@ -1871,7 +1885,11 @@ int engine_launch(fcg_engine* E, Cohort& C) {
                 be.push_back(BatchEntry{S->J, &S->scr, S->b32 ? nullptr : static_cast<const double*>(qi),
                                         S->w.as<double>(), S->b32 ? static_cast<const float*>(qi) : nullptr});
             }
-            ke.push_back(KEnt{S, S->w.as<double>(), S->i + 1, 0});
+            KEnt e{S, S->w.as<double>(), S->i, 0};  // final vectors 0..i-1, provisional i
+            e.coff = (int)ncoef;
+            e.next = true;
+            ncoef += (int64_t)(2 * S->i + 3) * S->k;
+            ke.push_back(e);
             S->n_mvp++;
         }
         if (!pe.empty()) {
@@ -1895,26 +1913,26 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         q.push_back([=] { return fcg::launch_mvp(bv, prec, jac, gp, s, ev); });
         KrylovView kv;
         RCCHECK(make_krylov(ctx, *C.tabs, ke, &kv));
-        RCCHECK(ensure_pbuf(ctx, C.partial_b, sizeof(double) * (size_t)kv.nchunks * std::max(kv.maxnv, 1)));
+        RCCHECK(ensure_pbuf(ctx, C.partial_b,
+                            sizeof(double) * (size_t)kv.nchunks * 4 * (2 * (size_t)kv.maxnv + 3)));
         double* pb = C.partial_b.as<double>();
         double* hb = C.hbuf.as<double>();
-        q.push_back([=] { return fcg::launch_dots(kv, pb, s); });
-        q.push_back([=] { return fcg::launch_reduce(kv, pb, hb, 0, s); });
-        q.push_back([=] { return fcg::launch_update(kv, hb, 0, 0, pb, s); });
-        q.push_back([=] { return fcg::launch_dots(kv, pb, s); });
-        q.push_back([=] { return fcg::launch_reduce(kv, pb, hb, 1, s); });
-        q.push_back([=] { return fcg::launch_update(kv, hb, 1, 1, pb, s); });
-        q.push_back([=] { return fcg::launch_reduce_norm(kv, pb, hb, s); });
+        RCCHECK(ensure_pbuf(ctx, C.coefb, sizeof(double) * (size_t)std::max<int64_t>(ncoef, 1)));
+        double* cb = C.coefb.as<double>();
+        q.push_back([=] { return fcg::launch_dcgs_dots(kv, pb, s); });
+        q.push_back([=] { return fcg::launch_dcgs_reduce(kv, pb, hb, s); });
+        q.push_back([=] { return fcg::launch_dcgs_coef(kv, hb, cb, s); });
+        q.push_back([=] { return fcg::launch_dcgs_update(kv, cb, s); });
     }
 
     // ---- D: one table copy, the queued launches, one copy back, one synchronisation
     double* hh = nullptr;
-    if (hoff > 0) {
+    if (hdown > 0) {
         void* h;
-        HIPCHECK(ctx, C.tabs->host_area(sizeof(double) * (size_t)hoff, &h));
+        HIPCHECK(ctx, C.tabs->host_area(sizeof(double) * (size_t)hdown, &h));
         hh = reinterpret_cast<double*>(h);
         const void* src = C.hbuf.p;
-        const size_t nb = sizeof(double) * (size_t)hoff;
+        const size_t nb = sizeof(double) * (size_t)hdown;
         dq.push_back([=] { return hipMemcpyAsync(hh, src, nb, hipMemcpyDeviceToHost, ds); });
     }
     // The tables go up on the copy stream, which does not queue behind the other
@@ -2013,29 +2031,38 @@ int engine_finish(fcg_engine* E, Cohort& C) {
             start_cycle(E, *S, beta.data());
     }
     for (Solve* S : arn) {
-        const int nv = S->i + 1;
-        const double* h1 = hh + S->hoff;
-        const double* h2 = h1 + (size_t)nv * S->k;
-        const double* w2 = h2 + (size_t)nv * S->k;
-        givens_step(*S, h1, h2, w2);
+        // per rhs: status (0 iterated, 1 met the tolerance before, 2 the
+        // provisional vector lay in the basis, 3 exact breakdown) and |g_(i+1)|
+        const double* out = hh + S->ooff;
+        for (int q = 0; q < S->k; ++q) {
+            const double st = out[2 * q], gr = out[2 * q + 1];
+            if (st == 1.0) continue;
+            if (st == 2.0) {
+                S->done[(size_t)q] = 1;
+                continue;
+            }
+            S->total_steps[(size_t)q]++;
+            S->last_i[(size_t)q] = S->i;
+            S->gres[(size_t)q] = gr;
+            if (gr < S->thr[(size_t)q] || st == 3.0) S->done[(size_t)q] = 1;
+        }
         bool all = true;
         double worst = 0.0;
         for (int q = 0; q < S->k; ++q) {
             if (!S->done[(size_t)q]) all = false;
             const int it = S->last_i[(size_t)q];
-            if (it >= 0) worst = std::max(worst, std::fabs(S->g[(size_t)q][(size_t)it + 1]) / S->normb[(size_t)q]);
+            if (it >= 0) worst = std::max(worst, S->gres[(size_t)q] / S->normb[(size_t)q]);
         }
         if (!S->mixed) {
             S->rel = worst;
             S->converged = all;
         }
-        if (all || S->i + 1 >= S->cap_iter) {
+        // basis vector i is final (the step's update also wrote the next
+        // provisional vector, slot i + 1)
+        if (all || S->i + 1 >= S->cap_iter)
             prepare_update(*S);
-        } else {
+        else
             S->i += 1;
-            S->need_scale = true;
-            S->scale_from_r = false;
-        }
     }
     // drop finished solves (their device memory returns to the pool)
     for (auto& u : C.active)

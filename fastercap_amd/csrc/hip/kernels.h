@@ -139,7 +139,23 @@ struct KJob {
     int zoff, nz;        // solution-update coefficients z[zoff + jj*k + r], jj < nz
     int jacobi;
     double* y;           // block preconditioner: the update writes Q z here instead of adding to x
+    int coff, next;      // DCGS2 update: coefficient offset, and whether the next vector is written
+    // DCGS2 Arnoldi state on the device, per rhs at hd + r * hstride (dcgs_rec):
+    // the Hessenberg columns, their rotations and the rotated right-hand side
+    double* hd;
+    int64_t hstride;
+    int ooff;                // step outputs for the host: [k][2] = status, |g_(nv+1)|
+    const int32_t* flags;    // [k] 1: the rhs met the tolerance before this step
+    const int32_t* last;     // [k] last Arnoldi column of each rhs (least squares), -1: none
+    const double* beta;      // [k] |r| of the cycle's start (first step)
 };
+
+// Per (job, rhs) DCGS2 record, laid out independently of its capacity so that a
+// larger one starts with a copy of the smaller: [0] the last column's provisional
+// subdiagonal, [1] g at that column before its rotation, [2] g_0, then column c
+// at dcgs_rec(c): the unrotated column (rows 0..c+1), the rotated one (rows
+// 0..c), its rotation (cos, sin) and g_(c+1).
+__host__ __device__ inline int64_t dcgs_rec(int c) { return 3 + (int64_t)c * (c + 5); }
 
 // Segment (job, rhs) and chunk tables for the Krylov kernels.
 struct Seg {
@@ -243,15 +259,27 @@ inline int padded_k(int k) {
     return p;
 }
 
-// Krylov helpers (batched over segments).  Step buffer layout per job at hoff:
-// h1 [nv][k], h2 [nv][k], |w|^2 [k].
-hipError_t launch_dots(const KrylovView& kv, double* partial, hipStream_t s);
-hipError_t launch_reduce(const KrylovView& kv, const double* partial, double* hbuf, int which, hipStream_t s);
-hipError_t launch_update(const KrylovView& kv, const double* hbuf, int which, int norm, double* partial,
-                         hipStream_t s);
+// Krylov helpers (batched over segments).
+// hbuf[hoff + 2 nv k + r] = the sum of the segment's chunk partials (|r|^2 of k_resid)
 hipError_t launch_reduce_norm(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s);
 // Q_nv[seg] = w[seg] / d[seg]   (d per segment, 0 -> zero vector)
 hipError_t launch_scale_new(const KrylovView& kv, const double* d, hipStream_t s);
+// Arnoldi with delayed reorthogonalisation (DCGS2).  Step buffer layout per job
+// at hoff: [2 nv + 3][k] = Q^T v [nv], Q^T w [nv], v.v, v.w, w.w (v: slot nv,
+// w: the product); partial holds nchunks * 4 * (2 maxnv + 3) doubles.
+hipError_t launch_dcgs_dots(const KrylovView& kv, double* partial, hipStream_t s);
+hipError_t launch_dcgs_reduce(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s);
+// the final q in slot nv and, for jobs with `next`, the next provisional vector
+// in slot nv + 1; coefficients per (job, rhs) at coef[coff + r (2 nv + 3)]
+hipError_t launch_dcgs_update(const KrylovView& kv, const double* coef, hipStream_t s);
+// per (job, rhs): the step's Hessenberg columns, rotations and update
+// coefficients from the projection in hbuf (at hoff), and the host's outputs at
+// hbuf[ooff + 2 r] (status: 0 iterated, 1 met the tolerance before, 2 the
+// provisional vector lies in the basis; then |g_(nv+1)|)
+hipError_t launch_dcgs_coef(const KrylovView& kv, double* hbuf, double* coef, hipStream_t s);
+// per (job, rhs): the least-squares solution over columns 0..last of the rotated
+// system, z[zoff + jj k + r] for jj < nz (0 beyond last)
+hipError_t launch_dcgs_lsq(const KrylovView& kv, double* z, hipStream_t s);
 // Block preconditioner (ComputePrecondVectFast block branch,
 // Solver/SolveCapacitance.cpp:3527-3544): blocks of consecutive leaves
 // [bstart[b], bstart[b+1]) of at most 128 leaves, each block's inverse stored

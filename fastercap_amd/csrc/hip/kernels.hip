@@ -1196,117 +1196,320 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
 // ------------------------------------------------------------------ krylov (batched)
 constexpr int kKThreads = 256;
 
-// grid (nchunks): the chunk's dot products with every basis vector of its job,
-// partial[c * maxnv + jj].  Each thread keeps its work-vector elements in
-// registers, so w is read once and each basis vector once; every product is
-// reduced by shuffles inside the wavefronts and the four wavefront sums meet in
-// LDS after the loop.  TB = basis element type.
 constexpr int kKPer = kChunk / kKThreads;  // chunk elements per thread
-constexpr int kMaxBasis = 1024;            // LDS bound on the basis vectors of one step
+
+// ---- Arnoldi with delayed reorthogonalisation (DCGS2)
+// Classical Gram-Schmidt applied twice, with the second pass of basis vector j
+// folded into the projection of step j + 1 (Bielich, Langou, Thomas,
+// Swirydowicz, Yamazaki, Boman, "Low-synch Gram-Schmidt with delayed
+// reorthogonalization for Krylov solvers", Parallel Computing 112, 2022): the
+// product runs on the provisional vector v_j (basis slot nv = j), one block
+// projection gives every dot product the host needs, and one update pass writes
+// the final q_j and the next provisional vector.  The basis is read twice per
+// step instead of four times.
+constexpr int kDcgsWaves = kKThreads / 64;
+
+// grid (nchunks): the chunk's dot products of v (slot nv) and of the product w
+// with the nv final basis vectors, and v.v, v.w, w.w.  Each wavefront writes its
+// own sums: partial[(c * kDcgsWaves + wave) * P + e], P = 2 maxnv + 3, with
+//   e < nv: Q_e . v,   nv <= e < 2 nv: Q_(e-nv) . w,   2 nv: v.v,   2 nv + 1: v.w,   2 nv + 2: w.w
 template <typename TB>
-__global__ void __launch_bounds__(kKThreads) k_dots(KrylovView kv, double* __restrict__ partial) {
-    extern __shared__ double sh[];  // [kKThreads / 64][maxnv]
+__global__ void __launch_bounds__(kKThreads) k_dcgs_dots(KrylovView kv, double* __restrict__ partial) {
+    const int P = 2 * kv.maxnv + 3;
     const int c = blockIdx.x;
     const Chunk ch = kv.chunks[c];
     const Seg sg = kv.segs[ch.seg];
     const KJob kj = kv.kjobs[sg.job];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const double* w = kj.w + (size_t)sg.r * kj.N;
-    double wr[kKPer];
+    const int nv = kj.nv;
+    const size_t ro = (size_t)sg.r * kj.N;
+    const TB* v = reinterpret_cast<TB* const*>(kj.Q)[nv] + ro;
+    const double* w = kj.w + ro;
+    double* out = partial + ((size_t)c * kDcgsWaves + wv) * P;
+    double vr[kKPer], wr[kKPer];
+    double svv = 0.0, svw = 0.0, sww = 0.0;
 #pragma unroll
     for (int t = 0; t < kKPer; ++t) {
         const int n = ch.n0 + threadIdx.x + t * kKThreads;
+        vr[t] = n < ch.n1 ? (double)v[n] : 0.0;
         wr[t] = n < ch.n1 ? w[n] : 0.0;
+        svv += vr[t] * vr[t];
+        svw += vr[t] * wr[t];
+        sww += wr[t] * wr[t];
     }
-    // four basis vectors at a time: their loads and shuffle reductions interleave
+    // four basis vectors at a time, each read once for both dot products
     constexpr int kG = 4;
-    for (int j0 = 0; j0 < kj.nv; j0 += kG) {
-        double s[kG];
+    for (int j0 = 0; j0 < nv; j0 += kG) {
+        double a[kG], b[kG];
 #pragma unroll
         for (int g = 0; g < kG; ++g) {
-            s[g] = 0.0;
-            if (j0 + g < kj.nv) {
-                const TB* q = reinterpret_cast<TB* const*>(kj.Q)[j0 + g] + (size_t)sg.r * kj.N;
+            a[g] = b[g] = 0.0;
+            if (j0 + g < nv) {
+                const TB* q = reinterpret_cast<TB* const*>(kj.Q)[j0 + g] + ro;
 #pragma unroll
                 for (int t = 0; t < kKPer; ++t) {
                     const int n = ch.n0 + threadIdx.x + t * kKThreads;
-                    if (n < ch.n1) s[g] += (double)q[n] * wr[t];
+                    if (n < ch.n1) {
+                        const double qq = (double)q[n];
+                        a[g] += qq * vr[t];
+                        b[g] += qq * wr[t];
+                    }
                 }
             }
         }
         for (int off = 32; off > 0; off >>= 1)
 #pragma unroll
-            for (int g = 0; g < kG; ++g) s[g] += __shfl_xor(s[g], off, 64);
+            for (int g = 0; g < kG; ++g) {
+                a[g] += __shfl_xor(a[g], off, 64);
+                b[g] += __shfl_xor(b[g], off, 64);
+            }
         if (lane == 0)
 #pragma unroll
             for (int g = 0; g < kG; ++g)
-                if (j0 + g < kj.nv) sh[wv * kv.maxnv + j0 + g] = s[g];
+                if (j0 + g < nv) {
+                    out[j0 + g] = a[g];
+                    out[nv + j0 + g] = b[g];
+                }
     }
-    __syncthreads();
-    for (int jj = threadIdx.x; jj < kj.nv; jj += kKThreads) {
-        double s = 0.0;
-        for (int q = 0; q < kKThreads / 64; ++q) s += sh[q * kv.maxnv + jj];
-        partial[(size_t)c * kv.maxnv + jj] = s;
+    for (int off = 32; off > 0; off >>= 1) {
+        svv += __shfl_xor(svv, off, 64);
+        svw += __shfl_xor(svw, off, 64);
+        sww += __shfl_xor(sww, off, 64);
+    }
+    if (lane == 0) {
+        out[2 * nv] = svv;
+        out[2 * nv + 1] = svw;
+        out[2 * nv + 2] = sww;
     }
 }
 
-// hbuf[hoff + which*nv*k + jj*k + r] = sum over the segment of the dots partials
-// one thread per (segment, basis vector): the segment's chunk partials summed in
-// chunk order (a segment has N / kChunk chunks, a few for most jobs)
-__global__ void __launch_bounds__(256) k_reduce(KrylovView kv, const double* __restrict__ partial,
-                                                double* __restrict__ hbuf, int which) {
-    const int g = blockIdx.x * 256 + threadIdx.x;
-    if (g >= kv.nsegs * kv.maxnv) return;
-    const int si = g / kv.maxnv, jj = g - si * kv.maxnv;
+// one thread per (segment, entry e < 2 nv + 3): hbuf[hoff + e k + r] = the sum
+// of the segment's chunk and wavefront partials, in chunk order
+__global__ void __launch_bounds__(256) k_dcgs_reduce(KrylovView kv, const double* __restrict__ partial,
+                                                     double* __restrict__ hbuf) {
+    const int P = 2 * kv.maxnv + 3;
+    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (g >= (int64_t)kv.nsegs * P) return;
+    const int si = (int)(g / P), e = (int)(g - (int64_t)si * P);
     const Seg sg = kv.segs[si];
     const KJob kj = kv.kjobs[sg.job];
-    if (jj >= kj.nv) return;
+    if (e >= 2 * kj.nv + 3) return;
     double s = 0.0;
-    for (int c = sg.c_lo; c < sg.c_hi; ++c) s += partial[(size_t)c * kv.maxnv + jj];
-    hbuf[(size_t)kj.hoff + (size_t)which * kj.nv * kj.k + (size_t)jj * kj.k + sg.r] = s;
+    for (int c = sg.c_lo; c < sg.c_hi; ++c)
+        for (int q = 0; q < kDcgsWaves; ++q) s += partial[((size_t)c * kDcgsWaves + q) * P + e];
+    hbuf[(size_t)kj.hoff + (size_t)e * kj.k + sg.r] = s;
 }
 
-// grid (nchunks): w -= sum_jj h[jj] Q_jj on the chunk; optional |w|^2 partial[c].
-// Each thread's elements stay in registers across the basis vectors, so the loads
-// of one vector are independent of each other.
+// grid (nchunks): the update that completes a DCGS2 step, with the host's
+// coefficients per (job, rhs) at coef[coff + r (2 nv + 3)]: s[nv], a[nv], then
+// inv_nu, cw, cv.  The final basis vector
+//     q = (v - sum_{i<nv} s_i Q_i) inv_nu                   over v in slot nv,
+// and, when `next` is set, the next provisional vector
+//     v' = cw w - sum_{i<nv} a_i Q_i - cv v                 in slot nv + 1.
+// Each thread keeps its elements of both in registers across the basis vectors.
 template <typename TB>
-__global__ void __launch_bounds__(kKThreads) k_update(KrylovView kv, const double* __restrict__ hbuf, int which,
-                                                      int norm, double* __restrict__ partial) {
-    __shared__ double sh[kKThreads / 64];
+__global__ void __launch_bounds__(kKThreads) k_dcgs_update(KrylovView kv, const double* __restrict__ coef) {
     const int c = blockIdx.x;
     const Chunk ch = kv.chunks[c];
     const Seg sg = kv.segs[ch.seg];
     const KJob kj = kv.kjobs[sg.job];
-    const double* h = hbuf + (size_t)kj.hoff + (size_t)which * kj.nv * kj.k + sg.r;
-    double* w = kj.w + (size_t)sg.r * kj.N;
-    double acc[kKPer];
+    const int nv = kj.nv;
+    const size_t ro = (size_t)sg.r * kj.N;
+    const double* cf = coef + kj.coff + (size_t)sg.r * (2 * nv + 3);
+    const double inv_nu = cf[2 * nv], cw = cf[2 * nv + 1], cv = cf[2 * nv + 2];
+    const bool next = kj.next != 0;
+    TB* vq = reinterpret_cast<TB* const*>(kj.Q)[nv] + ro;
+    const double* w = kj.w + ro;
+    double aq[kKPer], av[kKPer];
 #pragma unroll
     for (int t = 0; t < kKPer; ++t) {
         const int n = ch.n0 + threadIdx.x + t * kKThreads;
-        acc[t] = n < ch.n1 ? w[n] : 0.0;
+        const double vv = n < ch.n1 ? (double)vq[n] : 0.0;
+        const double ww = (next && n < ch.n1) ? w[n] : 0.0;
+        aq[t] = vv;
+        av[t] = cw * ww - cv * vv;
     }
 #pragma unroll 4
-    for (int jj = 0; jj < kj.nv; ++jj) {
-        const double hj = h[(size_t)jj * kj.k];
-        const TB* q = reinterpret_cast<TB* const*>(kj.Q)[jj] + (size_t)sg.r * kj.N;
+    for (int i = 0; i < nv; ++i) {
+        const double si = cf[i], ai = cf[nv + i];
+        const TB* q = reinterpret_cast<TB* const*>(kj.Q)[i] + ro;
 #pragma unroll
         for (int t = 0; t < kKPer; ++t) {
             const int n = ch.n0 + threadIdx.x + t * kKThreads;
-            if (n < ch.n1) acc[t] -= hj * (double)q[n];
+            if (n < ch.n1) {
+                const double qq = (double)q[n];
+                aq[t] -= si * qq;
+                av[t] -= ai * qq;
+            }
         }
     }
-    double ss = 0.0;
+    TB* vn = next ? reinterpret_cast<TB* const*>(kj.Q)[nv + 1] + ro : nullptr;
 #pragma unroll
     for (int t = 0; t < kKPer; ++t) {
         const int n = ch.n0 + threadIdx.x + t * kKThreads;
         if (n < ch.n1) {
-            w[n] = acc[t];
-            ss += acc[t] * acc[t];
+            vq[n] = (TB)(aq[t] * inv_nu);
+            if (next) vn[n] = (TB)av[t];
         }
     }
-    if (norm) {
-        ss = block_sum(ss, sh);
-        if (threadIdx.x == 0) partial[c] = ss;
+}
+
+// Rotation of column c of a DCGS2 record: the rotations 0..c-1 applied to the
+// unrotated column (rows 0..c+1), the rotated rows 0..c stored beside it, and the
+// rotation c that zeroes row c+1 (GMRES's Givens rotation, as the reference's
+// gmresPrecondSFastAll: cos = a / |(a, b)|, sin = -b / |(a, b)|).
+__device__ void dcgs_rotate(double* __restrict__ D, int c) {
+    const double* H = D + dcgs_rec(c);
+    double* R = D + dcgs_rec(c) + c + 2;
+    double a = H[0];
+    for (int r = 0; r < c; ++r) {
+        const double* rr = D + dcgs_rec(r) + 2 * r + 3;
+        const double cs = rr[0], sn = rr[1];
+        const double b = H[r + 1];
+        R[r] = cs * a - sn * b;
+        a = cs * b + sn * a;
+    }
+    const double b = H[c + 1];
+    const double len = sqrt(a * a + b * b);
+    const double cs = len > 0.0 ? a / len : 1.0, sn = len > 0.0 ? -b / len : 0.0;
+    R[c] = cs * a - sn * b;
+    double* rc = D + dcgs_rec(c) + 2 * c + 3;
+    rc[0] = cs;
+    rc[1] = sn;
+}
+__device__ __forceinline__ double& dcgs_g(double* D, int c) { return c == 0 ? D[2] : D[dcgs_rec(c - 1) + 2 * c + 3]; }
+
+// grid (nsegs), one wavefront per (job, rhs): the host part of a DCGS2 step, on
+// the device.  With s = Q^T v_j, z = Q^T w and the norms from the projection:
+// nu^2 = v.v - s.s gives the final q_j = (v_j - Q s) / nu.  Since v_j = nu q_j +
+// Q s, column j-1 (A P q_(j-1) = Q h + btil v_j) becomes final: h += btil s and the
+// subdiagonal btil nu.  Then A P q_j = (w - Q_j t) / nu with t = H s over the final
+// columns, so column j is h = ([z; (v.w - s.z) / nu] - t) / nu, with the
+// provisional subdiagonal btil_j = |A P q_j - Q_j h| from the norms, and the next
+// provisional vector is v_(j+1) = (w / nu - Q_j (t / nu + h)) / btil_j; its
+// coefficients over Q_(j-1), v_j and w go to coef for k_dcgs_update.  Lane 0 then
+// rotates the final column j-1 and, provisionally, column j, and reports
+// |g_(j+1)|.  A rhs that met the tolerance before gets coefficients that keep its
+// vectors.
+__global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, double* __restrict__ hbuf, double* __restrict__ coef) {
+    __shared__ double tj_sh;
+    const Seg sg = kv.segs[blockIdx.x];
+    const KJob kj = kv.kjobs[sg.job];
+    const int j = kj.nv, k = kj.k, r = sg.r, lane = threadIdx.x;
+    const double* hv = hbuf + kj.hoff;
+    auto at = [&](int e) { return hv[(size_t)e * k + r]; };
+    double* cf = coef + kj.coff + (size_t)r * (2 * j + 3);
+    double* out = hbuf + kj.ooff + 2 * r;
+    double* D = kj.hd + (size_t)r * kj.hstride;
+    auto keep = [&](double status) {
+        for (int e = lane; e < 2 * j + 3; e += 64) cf[e] = (e == 2 * j) ? 1.0 : 0.0;
+        if (lane == 0) {
+            out[0] = status;
+            out[1] = 0.0;
+        }
+    };
+    if (kj.flags[r]) {
+        keep(1.0);
+        return;
+    }
+    double ss = 0.0, sz = 0.0;
+    for (int i = lane; i < j; i += 64) {
+        ss += at(i) * at(i);
+        sz += at(i) * at(j + i);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        ss += __shfl_xor(ss, off, 64);
+        sz += __shfl_xor(sz, off, 64);
+    }
+    const double vv = at(2 * j), vw = at(2 * j + 1), ww = at(2 * j + 2);
+    const double nu2 = vv - ss;
+    if (!(vv > 0.0) || !(nu2 > 1e-28 * vv)) {
+        keep(2.0);
+        return;
+    }
+    const double nu = sqrt(nu2);
+    if (j == 0) {
+        if (lane == 0) D[2] = kj.beta[r] * nu;  // r = beta v_0 = beta nu q_0
+    } else {
+        double* Hp = D + dcgs_rec(j - 1);
+        const double bt0 = D[0];
+        for (int rr = lane; rr < j; rr += 64) Hp[rr] += bt0 * at(rr);
+        if (lane == 0) Hp[j] = bt0 * nu;
+    }
+    __syncthreads();
+    double* Hj = D + dcgs_rec(j);
+    double tq = 0.0, tt = 0.0, hh = 0.0;
+    for (int rr = lane; rr <= j; rr += 64) {
+        double tr = 0.0;
+        for (int c = rr > 0 ? rr - 1 : 0; c < j; ++c) tr += D[dcgs_rec(c) + rr] * at(c);
+        const double qw = rr < j ? at(j + rr) : (vw - sz) / nu;
+        const double hr = (qw - tr) / nu;
+        Hj[rr] = hr;
+        if (rr < j) cf[j + rr] = tr;  // t_rr, turned into the coefficient below
+        else tj_sh = tr;
+        tq += tr * qw;
+        tt += tr * tr;
+        hh += hr * hr;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        tq += __shfl_xor(tq, off, 64);
+        tt += __shfl_xor(tt, off, 64);
+        hh += __shfl_xor(hh, off, 64);
+    }
+    __syncthreads();
+    const double b2 = (ww - 2.0 * tq + tt) / nu2 - hh;
+    const double bt = b2 > 0.0 ? sqrt(b2) : 0.0;
+    const double cj = tj_sh / nu + Hj[j];
+    for (int i = lane; i < j; i += 64) {
+        const double si = at(i);
+        cf[i] = si;
+        cf[j + i] = bt > 0.0 ? (cf[j + i] / nu + Hj[i] - cj / nu * si) / bt : 0.0;
+    }
+    if (lane == 0) {
+        Hj[j + 1] = bt;
+        D[0] = bt;
+        cf[2 * j] = 1.0 / nu;
+        cf[2 * j + 1] = bt > 0.0 ? 1.0 / (nu * bt) : 0.0;
+        cf[2 * j + 2] = bt > 0.0 ? cj / (nu * bt) : 0.0;
+    }
+    __syncthreads();
+    if (lane == 0) {
+        if (j >= 1) {
+            dcgs_rotate(D, j - 1);
+            const double* rc = D + dcgs_rec(j - 1) + 2 * (j - 1) + 3;
+            const double gp = D[1];
+            dcgs_g(D, j - 1) = rc[0] * gp;
+            dcgs_g(D, j) = rc[1] * gp;
+        }
+        dcgs_rotate(D, j);
+        const double* rc = D + dcgs_rec(j) + 2 * j + 3;
+        const double gp = dcgs_g(D, j);
+        D[1] = gp;
+        dcgs_g(D, j) = rc[0] * gp;
+        dcgs_g(D, j + 1) = rc[1] * gp;
+        out[0] = bt > 0.0 ? 0.0 : 3.0;  // 3: exact breakdown, the residual is final
+        out[1] = fabs(rc[1] * gp);
+    }
+}
+
+// grid (nsegs), one wavefront per (job, rhs): back substitution of the rotated
+// Hessenberg system over columns 0..last (R y = g), y into z (0 beyond last)
+__global__ void __launch_bounds__(64) k_dcgs_lsq(KrylovView kv, double* __restrict__ z) {
+    const Seg sg = kv.segs[blockIdx.x];
+    const KJob kj = kv.kjobs[sg.job];
+    const int k = kj.k, r = sg.r, lane = threadIdx.x;
+    const int it = kj.last[r];
+    double* D = kj.hd + (size_t)r * kj.hstride;
+    double* y = z + kj.zoff + r;  // y[jj * k]
+    for (int jj = lane; jj < kj.nz; jj += 64) y[(size_t)jj * k] = jj <= it ? dcgs_g(D, jj) : 0.0;
+    __syncthreads();
+    for (int ii = it; ii >= 0; --ii) {
+        const double* R = D + dcgs_rec(ii) + ii + 2;  // rotated column ii, rows 0..ii
+        const double yi = y[(size_t)ii * k] / R[ii];
+        __syncthreads();
+        if (lane == 0) y[(size_t)ii * k] = yi;
+        for (int jj = lane; jj < ii; jj += 64) y[(size_t)jj * k] -= R[jj] * yi;
+        __syncthreads();
     }
 }
 
@@ -1402,28 +1605,35 @@ hipError_t launch_xupdate(const KrylovView& kv, const double* z, hipStream_t s) 
     return hipGetLastError();
 }
 
-hipError_t launch_dots(const KrylovView& kv, double* partial, hipStream_t s) {
-    if (kv.nchunks == 0 || kv.maxnv == 0) return hipSuccess;
-    if (kv.maxnv > kMaxBasis) return hipErrorInvalidValue;
-    const size_t lds = sizeof(double) * (kKThreads / 64) * (size_t)kv.maxnv;
-    if (kv.basis32)
-        k_dots<float><<<kv.nchunks, kKThreads, lds, s>>>(kv, partial);
-    else
-        k_dots<double><<<kv.nchunks, kKThreads, lds, s>>>(kv, partial);
-    return hipGetLastError();
-}
-hipError_t launch_reduce(const KrylovView& kv, const double* partial, double* hbuf, int which, hipStream_t s) {
-    if (kv.nsegs == 0 || kv.maxnv == 0) return hipSuccess;
-    k_reduce<<<blocks_for((size_t)kv.nsegs * kv.maxnv, 256), 256, 0, s>>>(kv, partial, hbuf, which);
-    return hipGetLastError();
-}
-hipError_t launch_update(const KrylovView& kv, const double* hbuf, int which, int norm, double* partial,
-                         hipStream_t s) {
+hipError_t launch_dcgs_dots(const KrylovView& kv, double* partial, hipStream_t s) {
     if (kv.nchunks == 0) return hipSuccess;
     if (kv.basis32)
-        k_update<float><<<kv.nchunks, kKThreads, 0, s>>>(kv, hbuf, which, norm, partial);
+        k_dcgs_dots<float><<<kv.nchunks, kKThreads, 0, s>>>(kv, partial);
     else
-        k_update<double><<<kv.nchunks, kKThreads, 0, s>>>(kv, hbuf, which, norm, partial);
+        k_dcgs_dots<double><<<kv.nchunks, kKThreads, 0, s>>>(kv, partial);
+    return hipGetLastError();
+}
+hipError_t launch_dcgs_reduce(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s) {
+    if (kv.nsegs == 0) return hipSuccess;
+    k_dcgs_reduce<<<blocks_for((size_t)kv.nsegs * (2 * (size_t)kv.maxnv + 3), 256), 256, 0, s>>>(kv, partial, hbuf);
+    return hipGetLastError();
+}
+hipError_t launch_dcgs_update(const KrylovView& kv, const double* coef, hipStream_t s) {
+    if (kv.nchunks == 0) return hipSuccess;
+    if (kv.basis32)
+        k_dcgs_update<float><<<kv.nchunks, kKThreads, 0, s>>>(kv, coef);
+    else
+        k_dcgs_update<double><<<kv.nchunks, kKThreads, 0, s>>>(kv, coef);
+    return hipGetLastError();
+}
+hipError_t launch_dcgs_coef(const KrylovView& kv, double* hbuf, double* coef, hipStream_t s) {
+    if (kv.nsegs == 0) return hipSuccess;
+    k_dcgs_coef<<<kv.nsegs, 64, 0, s>>>(kv, hbuf, coef);
+    return hipGetLastError();
+}
+hipError_t launch_dcgs_lsq(const KrylovView& kv, double* z, hipStream_t s) {
+    if (kv.nsegs == 0) return hipSuccess;
+    k_dcgs_lsq<<<kv.nsegs, 64, 0, s>>>(kv, z);
     return hipGetLastError();
 }
 hipError_t launch_reduce_norm(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s) {
