@@ -222,7 +222,7 @@ struct Job {
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
         col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16, grow,
-        src_geom;
+        src_geom, fill_rs;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -276,7 +276,7 @@ struct Job {
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
-                         col16.bytes + grow.bytes + src_geom.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
+                         col16.bytes + grow.bytes + src_geom.bytes + fill_rs.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
                          near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + gout_im.bytes +
                          gin_im.bytes + leaf_w_im.bytes + self_im.bytes + tsrc_off.bytes + tsrc.bytes +
@@ -306,6 +306,7 @@ struct Job {
         b.lsrc = lsrc.as<uint16_t>();
         b.src_geom = src_geom.as<fcg::SrcGeom>();
         b.fill_order = fill_order.as<uint32_t>();
+        b.fill_rs = fill_rs.as<int2>();
         b.fill_start = fill_start.as<int64_t>();
         b.link_row = link_row.as<int32_t>();
         b.coef64 = coef64.as<double>();
@@ -853,6 +854,16 @@ void build_fill_order(const fcg_job_desc* d, const std::vector<int32_t>& iperm,
         });
     }
     for (auto& x : th) x.join();
+}
+
+// Rows and sources of a job's links in fill order, built on its first fill (one
+// gather over the links), so that the fill kernels read them coalesced.
+int ensure_fill_rs(fcg_ctx* ctx, Job& j) {
+    if (j.fill_rs.p || j.L == 0) return FCG_OK;
+    HIPCHECK(ctx, j.fill_rs.alloc(sizeof(int2) * (size_t)j.L));
+    HIPCHECK(ctx, fcg::launch_fill_rs(j.fill_order.as<uint32_t>(), j.link_row.as<int32_t>(), j.col.as<int32_t>(), j.L,
+                                      j.fill_rs.as<int2>(), ctx->stream));
+    return FCG_OK;
 }
 
 // FP64 near-field side list of a job, built on its first mixed fill: the links of
@@ -2457,6 +2468,7 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
         if (precision == FCG_PREC_MIXED && !j->coef32.p)
             HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)(j->L + kLinkSpare)));
         if (precision == FCG_PREC_MIXED) RCCHECK(ensure_near(ctx, *j));
+        RCCHECK(ensure_fill_rs(ctx, *j));
         if (precision == FCG_PREC_FP64 && !j->coef64.p)
             HIPCHECK(ctx, j->coef64.alloc(sizeof(double) * (size_t)(j->L + kLinkSpare)));
         ents.push_back(BatchEntry{j, &j->scr, nullptr, nullptr});

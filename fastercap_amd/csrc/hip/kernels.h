@@ -67,6 +67,7 @@ struct BJob {
     const uint16_t* lsrc;     // [L + 4]
     const SrcGeom* src_geom;  // [M] source geometry by charge row
     const uint32_t* fill_order;  // [L] links grouped by fill class
+    const int2* fill_rs;         // [L] (destination node, source charge row) of each fill_order entry
     const int64_t* fill_start;   // [kFillClasses + 1] first fill_order entry of each class
     const int32_t* link_row;  // [L] destination node of each link (fill only)
     double* coef64;           // [L] or null
@@ -193,6 +194,10 @@ inline bool near_class(int c) { return c == 0 || c == 5 || c == 10 || c == 12; }
 // [p0, p0 + n); per-row counts of flagged links; the rows' flagged links
 // (charge-row column, link index) scattered at near_ptr.
 hipError_t launch_near_flag(const uint32_t* fill_order, int64_t p0, int64_t n, uint8_t* flag, hipStream_t s);
+// rs[g] = (link_row[l], col[l]) for l = fill_order[g]: the fill reads its links'
+// rows and sources in fill order, coalesced
+hipError_t launch_fill_rs(const uint32_t* fill_order, const int32_t* link_row, const int32_t* col, int64_t L,
+                          int2* rs, hipStream_t s);
 hipError_t launch_near_count(const int64_t* row_ptr, const uint8_t* flag, int M, int32_t* cnt, hipStream_t s);
 hipError_t launch_near_scatter(const int64_t* row_ptr, const uint8_t* flag, const int32_t* col, int M,
                                const int64_t* near_ptr, int32_t* near_col, uint32_t* near_link, hipStream_t s);

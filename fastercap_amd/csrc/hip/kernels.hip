@@ -192,6 +192,7 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __res
     int jb = find_job_block(off, bv.njobs, g0, gf < total ? gf : total - 1);
     int64_t l[P];
     int jl[P];
+    int row[P], src[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
         const int64_t g = gf + (int64_t)i * blockDim.x;
@@ -201,16 +202,12 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __res
             while (jb + 1 < bv.njobs && off[jb + 1] <= g) ++jb;  // entries ascend with i
             const BJob& J = bv.jobs[jb];
             jl[i] = jb;
-            l[i] = J.fill_order[J.fill_start[cls] + (g - off[jb])];
+            const int64_t f = J.fill_start[cls] + (g - off[jb]);
+            l[i] = J.fill_order[f];
+            const int2 rs = J.fill_rs[f];  // the source's charge row indexes src_geom
+            row[i] = rs.x;
+            src[i] = rs.y;
         }
-    }
-    int row[P], src[P];
-#pragma unroll
-    for (int i = 0; i < P; ++i) {
-        if (l[i] < 0) continue;
-        const BJob& J = bv.jobs[jl[i]];
-        row[i] = J.link_row[l[i]];
-        src[i] = J.col[l[i]];  // the source's charge row, which indexes src_geom
     }
 #pragma unroll
     for (int i = 0; i < P; ++i) {
@@ -1071,6 +1068,19 @@ hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int6
     return hipGetLastError();
 }
 
+__global__ void k_fill_rs(const uint32_t* __restrict__ fill_order, const int32_t* __restrict__ link_row,
+                          const int32_t* __restrict__ col, int64_t L, int2* __restrict__ rs) {
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= L) return;
+    const uint32_t l = fill_order[g];
+    rs[g] = make_int2(link_row[l], col[l]);
+}
+hipError_t launch_fill_rs(const uint32_t* fill_order, const int32_t* link_row, const int32_t* col, int64_t L,
+                          int2* rs, hipStream_t s) {
+    if (L <= 0) return hipSuccess;
+    k_fill_rs<<<blocks_for((size_t)L, 256), 256, 0, s>>>(fill_order, link_row, col, L, rs);
+    return hipGetLastError();
+}
 hipError_t launch_near_flag(const uint32_t* fill_order, int64_t p0, int64_t n, uint8_t* flag, hipStream_t s) {
     if (n <= 0) return hipSuccess;
     k_near_flag<<<blocks_for((size_t)n, 256), 256, 0, s>>>(fill_order, p0, n, flag);
