@@ -343,14 +343,23 @@ def bench_serve(args, fc, dev, tmp):
     kinds = [k for _, k, _ in pool]
     log(f"built {len(meshes)} serving jobs in {time.perf_counter() - t0:.1f}s: panels "
         f"{min(m.N for m in meshes)}..{max(m.N for m in meshes)}, links {sum(m.L for m in meshes)}")
-    jobs = [dev.upload(m) for m in meshes]
+    # progress lines at every stage: the largest pools take minutes to set up
+    jobs = []
+    for i, m in enumerate(meshes):
+        jobs.append(dev.upload(m))
+        if m.N > 100_000 or i == len(meshes) - 1:
+            log(f"uploaded {i + 1}/{len(meshes)} jobs (this one {m.N} panels, {m.L} links)")
     MIX, F64 = fc.FCG_PREC_MIXED, fc.FCG_PREC_FP64
     dev.fill_batch(jobs, F64)
+    log("FP64 fill done")
     dev.autotune_batch(jobs, fc.FCG_MVP_FP64)
+    log("FP64 gather shapes tuned")
     refs = {j: r.cap for j, r in zip(jobs, dev.solve_batch(jobs, meshes, tol=1e-8, precision=F64))}
+    log("reference FP64 solves (tol 1e-8) done")
     dev.fill_batch(jobs, MIX)
     dev.autotune_batch(jobs, fc.FCG_MVP_FP32)
     dev.autotune_batch(jobs, fc.FCG_MVP_FP32_ACC64)
+    log("mixed fill and gather shapes done")
     # closed-batch capacity: every job once, all together
     dev.solve_batch(jobs, meshes, tol=args.tol, precision=MIX)
     a = time.perf_counter()
@@ -372,6 +381,7 @@ def bench_serve(args, fc, dev, tmp):
         return rep, (max(e[0] for e in errs), max(e[1] for e in errs)), unconv
 
     serve(cap_rate, 2.0, 1)  # warm
+    log("serving warm-up done")
     dev.profile(enable=1, reset=True)
     rate = args.serve_load * cap_rate
     rep, err, unc = serve(rate, args.serve_window, 7)

@@ -22,7 +22,6 @@
 using fcg::BatchView;
 using fcg::BJob;
 using fcg::Chunk;
-using fcg::DD;
 using fcg::KJob;
 using fcg::KrylovView;
 using fcg::Seg;
@@ -207,14 +206,12 @@ struct Tables {
     }
 };
 
-// Scratch of one hierarchical product on k right-hand sides: leaf prefix sums,
-// node charges and node potentials.  Every concurrently running solve owns one.
+// Scratch of one hierarchical product on k right-hand sides: node charges and
+// node potentials.  Every concurrently running solve owns one.
 struct Scratch {
-    int k = 0, kp = 0, nblk = 0;
-    PBuf prefix, block_sums, charge64, charge32, pot64, pot32, lt;
-    size_t bytes() const {
-        return prefix.r + block_sums.r + charge64.r + charge32.r + pot64.r + pot32.r + lt.r;
-    }
+    int k = 0, kp = 0;
+    PBuf charge64, charge32, pot64, pot32, lt;
+    size_t bytes() const { return charge64.r + charge32.r + pot64.r + pot32.r + lt.r; }
 };
 
 struct Job {
@@ -223,6 +220,10 @@ struct Job {
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
         col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16, grow,
         src_geom, fill_rs;
+    // tree passes of the product (kernels.h BJob): inner nodes per chunk and
+    // level, top nodes per level, top flags, group roots
+    DevBuf tb_rec, tb_ppm, tb_lev, tb_lvoff, tt_node, tt_rec, tt_lev, ttop, troots;
+    int tt_nlev = 0, ntop = 0, nroots = 0;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -276,7 +277,10 @@ struct Job {
                          leaf_node.bytes + row_ptr.bytes + col.bytes + link_row.bytes + gout.bytes + gin.bytes +
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
-                         col16.bytes + grow.bytes + src_geom.bytes + fill_rs.bytes + bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
+                         col16.bytes + grow.bytes + src_geom.bytes + fill_rs.bytes + tb_rec.bytes +
+                         tb_ppm.bytes + tb_lev.bytes +
+                         tb_lvoff.bytes + tt_node.bytes + tt_rec.bytes + tt_lev.bytes + ttop.bytes + troots.bytes +
+                         bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
                          near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + gout_im.bytes +
                          gin_im.bytes + leaf_w_im.bytes + self_im.bytes + tsrc_off.bytes + tsrc.bytes +
@@ -291,7 +295,6 @@ struct Job {
         b.kp = sc.kp;
         b.dim = dim;
         b.first_cond_leaf = first_cond_leaf;
-        b.nblk = sc.nblk;
         b.L = L;
         b.nodes = nodes.as<fcg_node_geom>();
         b.row_ptr = row_ptr.as<int64_t>();
@@ -300,6 +303,18 @@ struct Job {
         b.perm = perm.as<int32_t>();
         b.iperm = iperm.as<int32_t>();
         b.rorder = rorder.as<int32_t>();
+        b.tb_rec = tb_rec.as<int4>();
+        b.tb_ppm = tb_ppm.as<int32_t>();
+        b.tb_lev = tb_lev.as<int32_t>();
+        b.tb_lvoff = tb_lvoff.as<int32_t>();
+        b.tt_node = tt_node.as<int32_t>();
+        b.tt_rec = tt_rec.as<int4>();
+        b.tt_lev = tt_lev.as<int32_t>();
+        b.ttop = ttop.as<uint8_t>();
+        b.troots = troots.as<int32_t>();
+        b.tt_nlev = tt_nlev;
+        b.ntop = ntop;
+        b.nroots = nroots;
         b.grow = grow.as<fcg::GRow>();
         b.tsrc_off = max_tile_src >= 0 ? tsrc_off.as<int32_t>() : nullptr;
         b.tsrc = tsrc.as<uint16_t>();
@@ -321,14 +336,13 @@ struct Job {
         b.gout_im = gout_im.as<double>();
         b.gin_im = gin_im.as<double>();
         b.left = left.as<int32_t>();
+        b.right = right.as<int32_t>();
         b.leaf_lo = leaf_lo.as<int32_t>();
         b.leaf_hi = leaf_hi.as<int32_t>();
         b.parent = parent.as<int32_t>();
         b.leaf_node = leaf_node.as<int32_t>();
         b.gout = gout.as<double>();
         b.gin = gin.as<double>();
-        b.prefix = sc.prefix.as<DD>();
-        b.block_sums = sc.block_sums.as<DD>();
         b.charge64 = sc.charge64.as<double>();
         b.charge32 = sc.charge32.as<float>();
         b.pot64 = sc.pot64.as<double>();
@@ -374,7 +388,7 @@ hipError_t scratch_address(int device, void** out) {
 // kernel from its by-value argument (no staging buffer, no copy call).
 struct NoAllocTable {
     BJob job;
-    int64_t off[10];  // link, tile, node, leaf and rhs offsets of a batch of one
+    int64_t off[10];  // link, tree-chunk, node, leaf and rhs offsets of a batch of one
 };
 static_assert(sizeof(NoAllocTable) % 8 == 0, "table copied in 8-byte words");
 constexpr size_t kNoAllocTableBytes = (sizeof(NoAllocTable) + 255) & ~(size_t)255;
@@ -464,15 +478,13 @@ hipError_t upload(DevBuf& b, const T* src, size_t n, hipStream_t s) {
 
 // product scratch for k right-hand sides of an (N leaves, M nodes) job
 int ensure_scratch(fcg_ctx* ctx, Scratch& sc, int N, int M, int k, bool need32) {
-    const int nblk = (N + fcg::kScanTile - 1) / fcg::kScanTile;
+    (void)N;
     const int kp = fcg::padded_k(k);
     if (sc.k == k && (!need32 || sc.charge32.p)) return FCG_OK;
     // buffers already in use may be read by queued kernels: drain before replacing
     if (sc.k != 0) HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
     Pool* pl = &ctx->pool;
     if (sc.k != k) {
-        HIPCHECK(ctx, sc.prefix.get(pl, sizeof(DD) * (size_t)k * (N + 1)));
-        HIPCHECK(ctx, sc.block_sums.get(pl, sizeof(DD) * (size_t)k * nblk));
         HIPCHECK(ctx, sc.charge64.get(pl, sizeof(double) * (size_t)kp * M));
         HIPCHECK(ctx, sc.pot64.get(pl, sizeof(double) * (size_t)kp * M));
         HIPCHECK(ctx, sc.lt.get(pl, sizeof(double) * 2 * (size_t)k));
@@ -485,7 +497,6 @@ int ensure_scratch(fcg_ctx* ctx, Scratch& sc, int N, int M, int k, bool need32) 
     }
     sc.k = k;
     sc.kp = kp;
-    sc.nblk = nblk;
     return FCG_OK;
 }
 
@@ -530,7 +541,7 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
         const Scratch& sc = *e.sc;
         recs[(size_t)i] = J.rec(sc, e.q, e.v, e.q32);
         lo[i + 1] = lo[i] + J.L;
-        to[i + 1] = to[i] + (int64_t)sc.k * sc.nblk;
+        to[i + 1] = to[i] + fcg::tree_chunks(J.M);
         no[i + 1] = no[i] + J.M;
         fo[i + 1] = fo[i] + J.N;
         ko[i + 1] = ko[i] + sc.k;
@@ -572,13 +583,13 @@ int make_batch(fcg_ctx* ctx, Tables& tabs, const std::vector<BatchEntry>& ents, 
     out->jobs = reinterpret_cast<const BJob*>(dr);
     out->njobs = n;
     out->link_off = d;
-    out->tile_off = d + (n + 1);
+    out->tchunk_off = d + (n + 1);
     out->node_off = d + 2 * (n + 1);
     out->leaf_off = d + 3 * (n + 1);
     out->rhs_off = d + 4 * (n + 1);
     out->gtile_off = d + 5 * (n + 1);
     out->links = lo[n];
-    out->tiles = to[n];
+    out->tchunks = to[n];
     out->nodes = no[n];
     out->leaves = fo[n];
     out->rhs = ko[n];
@@ -752,6 +763,94 @@ void build_tree_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::v
             rorder[(size_t)cs[(size_t)(node / C)]++] = node;
         }
     }
+}
+
+// Tables of the product's tree passes (kernels.h kTreeChunk): post-order nodes in
+// chunks of kTreeChunk; a node whose subtree (post-order range [start, n]) lies in
+// its chunk is inner, the others are top nodes (their subtrees straddle a chunk
+// boundary; a top node's parent is a top node).  Inner nodes are listed chunk by
+// chunk grouped by depth, top nodes grouped by depth, both ascending.
+int upload_tree_passes(fcg_ctx* ctx, Job& j, const fcg_job_desc* d, const std::vector<int32_t>& perm) {
+    hipStream_t s = ctx->stream;
+    const int M = d->n_nodes;
+    constexpr int C = fcg::kTreeChunk;
+    std::vector<int32_t> depth((size_t)M, 0), start((size_t)M, 0), roots;
+    int maxd = 0;
+    for (int i = M - 1; i >= 0; --i) {  // parents follow their children
+        const int p = d->parent[i];
+        depth[(size_t)i] = p >= 0 ? depth[(size_t)p] + 1 : 0;
+        maxd = std::max(maxd, depth[(size_t)i]);
+        if (p < 0) roots.push_back(i);
+    }
+    for (int i = 0; i < M; ++i) {
+        int st = i;
+        if (d->left[i] >= 0) st = std::min(st, start[(size_t)d->left[i]]);
+        if (d->right[i] >= 0) st = std::min(st, start[(size_t)d->right[i]]);
+        start[(size_t)i] = st;
+    }
+    std::vector<uint8_t> top((size_t)M, 0);
+    for (int i = 0; i < M; ++i) top[(size_t)i] = start[(size_t)i] < (i / C) * C ? 1 : 0;
+    const int nc = (int)fcg::tree_chunks(M);
+    std::vector<int32_t> tb_node, tb_lev, tb_lvoff((size_t)nc + 1, 0), tt_node, tt_lev;
+    std::vector<std::vector<int32_t>> byd((size_t)maxd + 1);
+    for (int t = 0; t < nc; ++t) {
+        for (auto& v : byd) v.clear();
+        for (int i = t * C; i < std::min(M, (t + 1) * C); ++i)
+            if (!top[(size_t)i]) byd[(size_t)depth[(size_t)i]].push_back(i);
+        for (const auto& v : byd) {
+            if (v.empty()) continue;
+            tb_lev.push_back((int32_t)tb_node.size());
+            tb_node.insert(tb_node.end(), v.begin(), v.end());
+        }
+        tb_lev.push_back((int32_t)tb_node.size());
+        tb_lvoff[(size_t)t + 1] = (int32_t)tb_lev.size();
+    }
+    for (auto& v : byd) v.clear();
+    for (int i = 0; i < M; ++i)
+        if (top[(size_t)i]) byd[(size_t)depth[(size_t)i]].push_back(i);
+    for (const auto& v : byd) {
+        if (v.empty()) continue;
+        tt_lev.push_back((int32_t)tt_node.size());
+        tt_node.insert(tt_node.end(), v.begin(), v.end());
+    }
+    j.tt_nlev = (int)tt_lev.size();
+    tt_lev.push_back((int32_t)tt_node.size());
+    j.ntop = (int)tt_node.size();
+    j.nroots = (int)roots.size();
+    // per top node: its slot's parent and top children slots (-1: not a top node)
+    std::vector<int32_t> slot((size_t)M, -1);
+    for (size_t i = 0; i < tt_node.size(); ++i) slot[(size_t)tt_node[i]] = (int32_t)i;
+    std::vector<int4> rec(std::max<size_t>(tt_node.size(), 1));
+    for (size_t i = 0; i < tt_node.size(); ++i) {
+        const int n = tt_node[i];
+        const int p = d->parent[n], lc = d->left[n], rc = d->right[n];
+        rec[i] = make_int4(n, p >= 0 ? slot[(size_t)p] : -1, lc >= 0 ? slot[(size_t)lc] : -1,
+                           rc >= 0 ? slot[(size_t)rc] : -1);
+    }
+    // records of the inner nodes in list order (kernels.h BJob tb_rec / tb_ppm)
+    std::vector<int4> tbr(std::max<size_t>(tb_node.size(), 1));
+    std::vector<int32_t> ppm(std::max<size_t>(tb_node.size(), 1), -1);
+    for (size_t i = 0; i < tb_node.size(); ++i) {
+        const int n = tb_node[i], c0 = (n / C) * C;
+        const int lc = d->left[n], rc = d->right[n], p = d->parent[n];
+        const bool lds_parent = p >= c0 && p < c0 + C && !top[(size_t)p];
+        auto pack = [](int lo, int hi) { return (int32_t)((uint32_t)(lo & 0xffff) | ((uint32_t)hi << 16)); };
+        tbr[i] = make_int4(pack(n - c0, lc >= 0 ? lc - c0 : -1), pack(rc >= 0 ? rc - c0 : -1, lds_parent ? p - c0 : -1),
+                           perm[(size_t)n], lc < 0 ? d->leaf_lo[n] : -1);
+        if (p >= 0 && !lds_parent) ppm[i] = perm[(size_t)p];
+    }
+    if (tt_node.empty()) tt_node.push_back(0);
+    HIPCHECK(ctx, upload(j.tb_rec, tbr.data(), tbr.size(), s));
+    HIPCHECK(ctx, upload(j.tb_ppm, ppm.data(), ppm.size(), s));
+    HIPCHECK(ctx, upload(j.tb_lev, tb_lev.data(), tb_lev.size(), s));
+    HIPCHECK(ctx, upload(j.tb_lvoff, tb_lvoff.data(), tb_lvoff.size(), s));
+    HIPCHECK(ctx, upload(j.tt_node, tt_node.data(), tt_node.size(), s));
+    HIPCHECK(ctx, upload(j.tt_rec, rec.data(), rec.size(), s));
+    HIPCHECK(ctx, upload(j.tt_lev, tt_lev.data(), tt_lev.size(), s));
+    HIPCHECK(ctx, upload(j.ttop, top.data(), top.size(), s));
+    HIPCHECK(ctx, upload(j.troots, roots.data(), roots.size(), s));
+    HIPCHECK(ctx, hipStreamSynchronize(s));  // the host vectors are released here
+    return FCG_OK;
 }
 
 void build_layout(const fcg_job_desc* d, std::vector<int32_t>& perm, std::vector<int32_t>& iperm,
@@ -2309,6 +2408,7 @@ int fcg_job_upload(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
         HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
         RCCHECK(upload_grow(ctx, *j, d->row_ptr, perm, rorder));
         RCCHECK(upload_src_geom(ctx, *j, d->nodes, perm));
+        RCCHECK(upload_tree_passes(ctx, *j, d, perm));
         std::vector<uint32_t> forder;
         build_fill_order(d, iperm, cols, forder, j->fill_cnt);
         if (!forder.empty()) HIPCHECK(ctx, upload(j->fill_order, forder.data(), forder.size(), s));
@@ -2356,6 +2456,7 @@ int fcg_job_upload_links(fcg_ctx* ctx, const fcg_job_desc* d, int* job) {
     HIPCHECK(ctx, upload(j->rorder, rorder.data(), rorder.size(), s));
     RCCHECK(upload_grow(ctx, *j, dd.row_ptr, perm, rorder));
     RCCHECK(upload_src_geom(ctx, *j, dd.nodes, perm));
+    RCCHECK(upload_tree_passes(ctx, *j, &dd, perm));
     // stored columns: the source's charge row, every row ascending
     HIPCHECK(ctx, j->col.alloc(sizeof(int32_t) * (size_t)(L + kLinkSpare)));
     HIPCHECK(ctx, hipMemsetAsync(j->col.as<int32_t>() + L, 0, sizeof(int32_t) * kLinkSpare, s));
@@ -2569,14 +2670,11 @@ int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v
     if (!j->filled64) return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in FP64");
     if (!ctx->noalloc) return fail(ctx, FCG_DEVICE_ERROR, "no scratch address on this device");
     k = j->pk(k);  // complex jobs: two product vectors per right-hand side
-    // scratch layout: job table, leaf prefix sums [k][N+1], scan tile sums [k][nblk],
-    // node charges and potentials [M][kp], 2D fix-up values [2k]
+    // scratch layout: job table, node charges and potentials [M][kp], 2D fix-up
+    // values [2k]
     const int N = j->N, M = j->M, kp = fcg::padded_k(k);
-    const int nblk = (N + fcg::kScanTile - 1) / fcg::kScanTile;
     auto al = [](size_t b) { return (b + 255) & ~(size_t)255; };
-    const size_t o_prefix = kNoAllocTableBytes;
-    const size_t o_bsum = o_prefix + al(sizeof(DD) * (size_t)k * (N + 1));
-    const size_t o_charge = o_bsum + al(sizeof(DD) * (size_t)k * nblk);
+    const size_t o_charge = kNoAllocTableBytes;
     const size_t o_pot = o_charge + al(sizeof(double) * (size_t)kp * M);
     const size_t o_lt = o_pot + al(sizeof(double) * (size_t)kp * M);
     const size_t need = o_lt + al(sizeof(double) * 2 * (size_t)k);
@@ -2590,17 +2688,14 @@ int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v
     Scratch sc;  // sizes only: the buffers are the scratch array's
     sc.k = k;
     sc.kp = kp;
-    sc.nblk = nblk;
     NoAllocTable t{};
     t.job = j->rec(sc, d_q, d_v);
-    t.job.prefix = reinterpret_cast<DD*>(base + o_prefix);
-    t.job.block_sums = reinterpret_cast<DD*>(base + o_bsum);
     t.job.charge64 = reinterpret_cast<double*>(base + o_charge);
     t.job.pot64 = reinterpret_cast<double*>(base + o_pot);
     t.job.lt = reinterpret_cast<double*>(base + o_lt);
     t.job.charge32 = nullptr;
     t.job.pot32 = nullptr;
-    const int64_t sizes[5] = {j->L, (int64_t)k * nblk, M, N, k};
+    const int64_t sizes[5] = {j->L, fcg::tree_chunks(M), M, N, k};
     for (int i = 0; i < 5; ++i) {
         t.off[2 * i] = 0;
         t.off[2 * i + 1] = sizes[i];
@@ -2612,12 +2707,12 @@ int fcg_mvp_device_noalloc(fcg_ctx* ctx, int job, const double* d_q, double* d_v
     bv.jobs = &tab->job;
     bv.njobs = 1;
     bv.link_off = tab->off;
-    bv.tile_off = tab->off + 2;
+    bv.tchunk_off = tab->off + 2;
     bv.node_off = tab->off + 4;
     bv.leaf_off = tab->off + 6;
     bv.rhs_off = tab->off + 8;
     bv.links = j->L;
-    bv.tiles = (int64_t)k * nblk;
+    bv.tchunks = fcg::tree_chunks(M);
     bv.nodes = M;
     bv.leaves = N;
     bv.rhs = k;

@@ -15,11 +15,15 @@
 namespace fcg {
 
 // Double-double value used by the prefix scan of the up-pass.
-struct DD {
-    double hi, lo;
-};
-
-constexpr int kScanTile = 2048;  // elements per scan block (256 threads x 8)
+// Tree passes of the product (up: node charges, down: leaf potentials) work on
+// chunks of kTreeChunk consecutive post-order nodes.  A node whose subtree lies in
+// its chunk is inner: its chunk's block computes it level by level in LDS.  The
+// others (the nodes whose subtrees straddle a chunk boundary: the top of the tree,
+// closed under parents) are done by one block per job.
+constexpr int kTreeChunk = 1024;
+constexpr int kTreeRhs = 4;      // right-hand sides per LDS pass of the tree kernels
+constexpr int kTreeTopLds = 1024;  // top nodes a job's top pass holds in LDS (more: global memory)
+inline int64_t tree_chunks(int M) { return ((int64_t)M + kTreeChunk - 1) / kTreeChunk; }
 constexpr int kChunk = 1024;     // Krylov vector chunk per block
 constexpr int kNumXcd = 8;       // MI355X accelerator complex dies
 
@@ -50,7 +54,6 @@ struct BJob {
     int N, M, k, dim;
     int kp;               // node-major row stride: k rounded up to a power of two
     int first_cond_leaf;  // 2D zero-total-charge fix-up start
-    int nblk;             // scan tiles per right-hand side
     int64_t L;
     const fcg_node_geom* nodes;
     const int64_t* row_ptr;
@@ -59,6 +62,25 @@ struct BJob {
     const int32_t* perm;      // [M] node -> slot of its charge / potential row
     const int32_t* iperm;     // [M] slot -> node
     const int32_t* rorder;    // [M] gather row order -> node
+    // tree passes: the inner nodes of every chunk grouped by depth (ascending),
+    // one record per node, tb_rec[tb_lev[tb_lvoff[t] + l] ...] the nodes of level
+    // l of chunk t (the chunk's last tb_lev entry ends its list); a record holds
+    // x = slot | left child slot << 16, y = right child slot | parent slot << 16
+    // (slots are the node's offset in its chunk, -1 for none or a top parent),
+    // z = charge row (perm), w = leaf index (-1 for internal nodes); tb_ppm: the
+    // charge row of a top parent (-1 otherwise); the top nodes by depth, level l
+    // at tt_node[tt_lev[l] ..]; ttop[n] = 1 for a top node; the group roots
+    const int4* tb_rec;
+    const int32_t* tb_ppm;
+    const int32_t* tb_lev;
+    const int32_t* tb_lvoff;   // [chunks + 1]
+    const int32_t* tt_node;
+    const int4* tt_rec;        // per top node: (node, slot of its parent, of its left child, of its right
+                               // child), -1 where that node is not a top node (slots index tt_node)
+    const int32_t* tt_lev;     // [tt_nlev + 1]
+    const uint8_t* ttop;       // [M]
+    const int32_t* troots;     // [nroots]
+    int tt_nlev, ntop, nroots;
     const GRow* grow;         // [M] the rows in gather row order
     // staged gather (null until built): tsrc[tsrc_off[t] ..] lists the source
     // charge rows of tile t ascending; lsrc[l] is link l's index in its tile's list
@@ -88,6 +110,7 @@ struct BJob {
     const double* gout_im;
     const double* gin_im;
     const int32_t* left;
+    const int32_t* right;
     const int32_t* leaf_lo;
     const int32_t* leaf_hi;
     const int32_t* parent;
@@ -95,8 +118,6 @@ struct BJob {
     const double* gout;
     const double* gin;
     // product scratch sized for k right-hand sides
-    DD* prefix;      // [k][N+1]
-    DD* block_sums;  // [k][nblk]
     double* charge64;
     float* charge32;  // [M][kp], row perm[node]
     double* pot64;
@@ -114,12 +135,12 @@ struct BatchView {
     const BJob* jobs;
     int njobs;
     const int64_t* link_off;  // links
-    const int64_t* tile_off;  // k * nblk scan tiles
+    const int64_t* tchunk_off;  // tree chunks (kTreeChunk post-order nodes)
     const int64_t* node_off;  // nodes
     const int64_t* leaf_off;  // leaves
     const int64_t* rhs_off;   // right-hand sides
     const int64_t* gtile_off;  // staged-gather tiles (kGatherTileRows rows of one job)
-    int64_t links, tiles, nodes, leaves, rhs, gtiles;
+    int64_t links, tchunks, nodes, leaves, rhs, gtiles;
     int kmax;   // largest k in the batch
     int kpmin;  // smallest row stride in the batch
     int any2d;

@@ -5,16 +5,17 @@
 //              (SelfPotential, Solver/Autorefine.cpp:2358-2506).  The mixed mode
 //              evaluates the analytic near-field integrals in FP64 and the
 //              quadrature and point branches in FP32, storing FP32.
-//  product     v = P q (Solver/MultiplyHierarchical.cpp:72-330) in four stages:
-//              1. double-double prefix scan of the (Jacobi-scaled) input over the
-//                 DFS leaf order; every node's charge is the difference of two
-//                 prefixes because each subtree owns a contiguous leaf range
-//              2. node charges (leaves copy their own input exactly), node-major [M][k]
-//              3. CSR gather: one wavefront per destination node, lanes = (link
-//                 slot, rhs), coalesced coef/col streaming, XCD-aware row ranges
-//              4. down-pass: each leaf sums the potentials of its ancestor chain
-//  krylov      batched dot products / updates over (job, rhs) segments for CGS2
-//              GMRES (Solver/SolveCapacitance.cpp:2288-2574)
+//  product     v = P q (Solver/MultiplyHierarchical.cpp:72-330) in three stages:
+//              1. up-pass: node charges as subtree sums, level by level over
+//                 chunks of post-order nodes (the chunk-straddling top of the
+//                 tree by one block per job), node-major [M][kp]
+//              2. CSR gather: lanes = (link slot, rhs), each 128-row tile's
+//                 source charges staged in LDS, coalesced coefficient and index
+//                 streams, XCD-aware tile ranges
+//              3. down-pass: path sums of the node potentials, top nodes first,
+//                 then level by level over the chunks
+//  krylov      batched projections and updates over (job, rhs) segments for
+//              GMRES with delayed reorthogonalisation (Solver/SolveCapacitance.cpp:2288-2574)
 #include "kernels.h"
 
 #include "../common/potential.h"
@@ -71,31 +72,6 @@ __device__ __forceinline__ int find_job_block(const int64_t* __restrict__ off, i
         ++jb;
     }
     return find_job(off, njobs, g);
-}
-
-__device__ __forceinline__ DD dd_two_sum(double a, double b) {
-    const double s = a + b;
-    const double bb = s - a;
-    const double e = (a - (s - bb)) + (b - bb);
-    return DD{s, e};
-}
-__device__ __forceinline__ DD dd_quick(double a, double b) {
-    const double s = a + b;
-    return DD{s, b - (s - a)};
-}
-// accurate double-double addition (two two-sums, two renormalisations)
-__device__ __forceinline__ DD dd_add(DD a, DD b) {
-    DD s = dd_two_sum(a.hi, b.hi);
-    DD t = dd_two_sum(a.lo, b.lo);
-    s.lo += t.hi;
-    s = dd_quick(s.hi, s.lo);
-    s.lo += t.lo;
-    return dd_quick(s.hi, s.lo);
-}
-__device__ __forceinline__ DD dd_neg(DD a) { return DD{-a.hi, -a.lo}; }
-__device__ __forceinline__ double dd_diff(DD a, DD b) {
-    const DD d = dd_add(a, dd_neg(b));
-    return d.hi + d.lo;
 }
 
 static inline unsigned blocks_for(size_t n, unsigned t) { return (unsigned)((n + t - 1) / t); }
@@ -335,37 +311,17 @@ __global__ void k_self(BatchView bv) {
     if (J.self_im) J.self_im[k] = si;
 }
 
-// ------------------------------------------------------------------ up-pass scan
-// Exclusive scan of one DD per thread over a block of NT threads (NT a multiple
-// of 64): an inclusive shuffle scan inside each wavefront, the wavefront totals
-// through LDS (sh: NT / 64 entries), then each wavefront adds the totals of the
-// ones before it.  *total is the block's sum.
-template <int NT>
-__device__ DD block_exclusive_scan(DD v, DD* sh, DD* total) {
-    constexpr int NW = NT / 64;
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    DD inc = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        const DD left{__shfl_up(inc.hi, off, 64), __shfl_up(inc.lo, off, 64)};
-        if (lane >= off) inc = dd_add(inc, left);
-    }
-    if (lane == 63) sh[wv] = inc;
-    __syncthreads();
-    DD before{0.0, 0.0}, all{0.0, 0.0};
-#pragma unroll
-    for (int w = 0; w < NW; ++w) {
-        if (w == wv) before = all;
-        all = (w == 0) ? sh[0] : dd_add(all, sh[w]);
-    }
-    const DD prev{__shfl_up(inc.hi, 1, 64), __shfl_up(inc.lo, 1, 64)};
-    const DD excl = (lane > 0) ? prev : DD{0.0, 0.0};
-    *total = all;
-    __syncthreads();  // sh may be reused by the caller
-    return wv == 0 ? excl : dd_add(before, excl);
-}
-
-constexpr int kScanThreads = 256;
-constexpr int kScanPer = kScanTile / kScanThreads;  // 8
+// ------------------------------------------------------------------ tree passes
+// Up-pass (ComputePanelCharges_fast, Solver/MultiplyHierarchical.cpp:72-127): the
+// charge of a node is the sum of its leaves' inputs, so a node's charge is its
+// children's sum.  Down-pass (ComputeLeafPotentials_fast, :231-283): a leaf's
+// result is the sum of the potentials of the nodes on its path to the root.  Both
+// run over chunks of kTreeChunk consecutive post-order nodes: the inner nodes of a
+// chunk (subtree inside the chunk) level by level in LDS, one block per chunk,
+// and the top nodes (subtrees across chunk boundaries) by one block per job.
+// Sums are accumulated in FP64; charges and potentials are stored as TQ, one
+// kp-row per node at row perm[node] (padding columns zero).
+constexpr int kTreeThreads = 1024;
 
 __device__ __forceinline__ double scan_input(const BJob& J, int r, int n, int jacobi) {
     const size_t i = (size_t)r * J.N + n;
@@ -373,103 +329,120 @@ __device__ __forceinline__ double scan_input(const BJob& J, int r, int n, int ja
     return jacobi ? x / J.self[n] : x;
 }
 
-__global__ void __launch_bounds__(kScanThreads) k_scan_reduce(BatchView bv, int jacobi) {
-    __shared__ DD sh[kScanThreads];
-    const int64_t g = blockIdx.x;
-    const int jb = find_job_wave(bv.tile_off, bv.njobs, g);
-    const BJob& J = bv.jobs[jb];
-    const int t = (int)(g - bv.tile_off[jb]);
-    const int r = t / J.nblk, b = t % J.nblk;
-    const int base = b * kScanTile + threadIdx.x * kScanPer;
-    DD acc{0.0, 0.0};
-    for (int i = 0; i < kScanPer; ++i) {
-        const int n = base + i;
-        if (n < J.N) acc = dd_add(acc, DD{scan_input(J, r, n, jacobi), 0.0});
-    }
-    DD total;
-    block_exclusive_scan<kScanThreads>(acc, sh, &total);
-    if (threadIdx.x == 0) J.block_sums[(size_t)r * J.nblk + b] = total;
-}
+// Inner node charges of one chunk.  The chunk's records go to LDS and the
+// leaves' inputs are read in one parallel pass; the levels (deepest first) then
+// add children in LDS alone, and one parallel pass writes every node's charge row.
+__device__ __forceinline__ int tree_lo(int v) { return (int)(int16_t)(v & 0xffff); }
+__device__ __forceinline__ int tree_hi(int v) { return v >> 16; }
 
-// exclusive scan of the block sums, one wavefront per (job, rhs): chunks of 64
-// tile sums, each scanned by shuffles (the additions of the block scan above),
-// a running carry between chunks
-__global__ void __launch_bounds__(256) k_scan_top(BatchView bv) {
-    const int64_t g = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (g >= bv.rhs) return;  // whole wavefronts
-    const int lane = threadIdx.x & 63;
-    const int jb = find_job_wave(bv.rhs_off, bv.njobs, g);
-    const BJob& J = bv.jobs[jb];
-    const int r = (int)(g - bv.rhs_off[jb]);
-    DD* bs = J.block_sums + (size_t)r * J.nblk;
-    const int nblk = J.nblk;
-    DD carry{0.0, 0.0};
-    for (int b0 = 0; b0 < nblk; b0 += 64) {
-        const int b = b0 + lane;
-        DD inc = (b < nblk) ? bs[b] : DD{0.0, 0.0};
-        for (int off = 1; off < 64; off <<= 1) {
-            const DD left{__shfl_up(inc.hi, off, 64), __shfl_up(inc.lo, off, 64)};
-            if (lane >= off) inc = dd_add(inc, left);
-        }
-        const DD prev{__shfl_up(inc.hi, 1, 64), __shfl_up(inc.lo, 1, 64)};
-        const DD excl = (lane > 0) ? prev : DD{0.0, 0.0};
-        if (b < nblk) bs[b] = (b0 == 0) ? excl : dd_add(carry, excl);
-        const DD chunk{__shfl(inc.hi, 63, 64), __shfl(inc.lo, 63, 64)};
-        carry = (b0 == 0) ? chunk : dd_add(carry, chunk);
-    }
-}
-
-__global__ void __launch_bounds__(kScanThreads) k_scan_final(BatchView bv, int jacobi) {
-    __shared__ DD sh[kScanThreads];
-    const int64_t g = blockIdx.x;
-    const int jb = find_job_wave(bv.tile_off, bv.njobs, g);
-    const BJob& J = bv.jobs[jb];
-    const int t = (int)(g - bv.tile_off[jb]);
-    const int r = t / J.nblk, b = t % J.nblk;
-    const int base = b * kScanTile + threadIdx.x * kScanPer;
-    double x[kScanPer];
-    DD acc{0.0, 0.0};
-    for (int i = 0; i < kScanPer; ++i) {
-        const int n = base + i;
-        x[i] = (n < J.N) ? scan_input(J, r, n, jacobi) : 0.0;
-        acc = dd_add(acc, DD{x[i], 0.0});
-    }
-    DD total;
-    DD run = block_exclusive_scan<kScanThreads>(acc, sh, &total);
-    run = dd_add(run, J.block_sums[(size_t)r * J.nblk + b]);
-    DD* P = J.prefix + (size_t)r * (J.N + 1);
-    for (int i = 0; i < kScanPer; ++i) {
-        const int n = base + i;
-        if (n < J.N) P[n] = run;
-        run = dd_add(run, DD{x[i], 0.0});
-        if (n == J.N - 1) P[J.N] = run;
-    }
-}
-
-// node charges, node-major [M][kp] at row perm[node], in FP64 or FP32; the
-// padding columns k..kp-1 are zero so that vector loads of a row stay finite
 template <typename TQ>
-__global__ void k_node_charge(BatchView bv, int jacobi) {
-    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
-    const int jb = find_job_block(bv.node_off, bv.njobs, g0, g < bv.nodes ? g : bv.nodes - 1);
-    if (g >= bv.nodes) return;
+__global__ void __launch_bounds__(kTreeThreads) k_up_inner(BatchView bv, int jacobi) {
+    __shared__ double tacc[kTreeChunk * kTreeRhs];
+    __shared__ int4 rec[kTreeChunk];
+    const int64_t g = blockIdx.x;
+    const int jb = find_job_wave(bv.tchunk_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
-    const int i = (int)(g - bv.node_off[jb]);
-    const int lo = J.leaf_lo[i], hi = J.leaf_hi[i];
-    const bool leaf = J.left[i] < 0;
-    TQ* out = ((sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.charge64) : reinterpret_cast<TQ*>(J.charge32)) +
-              (size_t)J.perm[i] * J.kp;
-    for (int r = 0; r < J.k; ++r) {
-        double c;
-        if (leaf) {
-            c = scan_input(J, r, lo, jacobi);
-        } else {
-            const DD* P = J.prefix + (size_t)r * (J.N + 1);
-            c = dd_diff(P[hi], P[lo]);
+    const int t = (int)(g - bv.tchunk_off[jb]);
+    const int k = J.k, kp = J.kp;
+    TQ* out = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.charge64) : reinterpret_cast<TQ*>(J.charge32);
+    const int l0 = J.tb_lvoff[t], l1 = J.tb_lvoff[t + 1];  // l1 - l0 - 1 levels
+    const int e0 = J.tb_lev[l0], e1 = J.tb_lev[l1 - 1];    // the chunk's list
+    for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) rec[i] = J.tb_rec[e0 + i];
+    __syncthreads();
+    for (int rg = 0; rg < kp; rg += kTreeRhs) {
+        const int nr = min(kTreeRhs, kp - rg);
+        for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) {  // the leaves' inputs
+            const int4 rc = rec[i];
+            if (rc.w < 0) continue;
+            const int sl = tree_lo(rc.x);
+            for (int r = 0; r < nr; ++r)
+                tacc[sl * kTreeRhs + r] = rg + r < k ? scan_input(J, rg + r, rc.w, jacobi) : 0.0;
         }
-        out[r] = (TQ)c;
+        __syncthreads();
+        for (int lv = l1 - 2; lv >= l0; --lv) {
+            for (int i = J.tb_lev[lv] - e0 + (int)threadIdx.x; i < J.tb_lev[lv + 1] - e0; i += kTreeThreads) {
+                const int4 rc = rec[i];
+                if (rc.w >= 0) continue;
+                const int sl = tree_lo(rc.x), lc = tree_hi(rc.x), rch = tree_lo(rc.y);
+                for (int r = 0; r < nr; ++r)
+                    tacc[sl * kTreeRhs + r] = (lc >= 0 ? tacc[lc * kTreeRhs + r] : 0.0) +
+                                              (rch >= 0 ? tacc[rch * kTreeRhs + r] : 0.0);
+            }
+            __syncthreads();
+        }
+        for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) {
+            const int4 rc = rec[i];
+            const int sl = tree_lo(rc.x);
+            TQ* o = out + (size_t)rc.z * kp + rg;
+            for (int r = 0; r < nr; ++r) o[r] = (TQ)tacc[sl * kTreeRhs + r];
+        }
+        __syncthreads();  // tacc is reused by the next group
     }
-    for (int r = J.k; r < J.kp; ++r) out[r] = (TQ)0;
+}
+
+// Top node charges of one job (one block).  Each top node starts from its inner
+// children's charges (global: k_up_inner), then the levels (deepest first) add
+// the top children in LDS; every top node's row is written at the end.  Jobs
+// with more than kTreeTopLds top nodes take the same steps in global memory.
+template <typename TQ>
+__global__ void __launch_bounds__(kTreeThreads) k_up_top(BatchView bv) {
+    __shared__ double tacc[kTreeTopLds * kTreeRhs];
+    const BJob& J = bv.jobs[blockIdx.x];
+    const int k = J.k, kp = J.kp, nt = J.ntop;
+    if (nt == 0) return;
+    TQ* ch = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.charge64) : reinterpret_cast<TQ*>(J.charge32);
+    if (nt > kTreeTopLds) {
+        for (int lv = J.tt_nlev - 1; lv >= 0; --lv) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+                const int n = J.tt_node[i];
+                const int lc = J.left[n], rc = J.right[n];
+                TQ* o = ch + (size_t)J.perm[n] * kp;
+                for (int r = 0; r < kp; ++r) {
+                    double c = 0.0;
+                    if (r < k) {
+                        if (lc >= 0) c += (double)ch[(size_t)J.perm[lc] * kp + r];
+                        if (rc >= 0) c += (double)ch[(size_t)J.perm[rc] * kp + r];
+                    }
+                    o[r] = (TQ)c;
+                }
+            }
+            __syncthreads();
+        }
+        return;
+    }
+    for (int rg = 0; rg < kp; rg += kTreeRhs) {
+        const int nr = min(kTreeRhs, kp - rg);
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {  // inner children
+            const int4 rc = J.tt_rec[i];
+            const int lc = J.left[rc.x], rr = J.right[rc.x];
+            for (int r = 0; r < nr; ++r) {
+                double c = 0.0;
+                if (rg + r < k) {
+                    if (lc >= 0 && rc.z < 0) c += (double)ch[(size_t)J.perm[lc] * kp + rg + r];
+                    if (rr >= 0 && rc.w < 0) c += (double)ch[(size_t)J.perm[rr] * kp + rg + r];
+                }
+                tacc[i * kTreeRhs + r] = c;
+            }
+        }
+        __syncthreads();
+        for (int lv = J.tt_nlev - 1; lv >= 0; --lv) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+                const int4 rc = J.tt_rec[i];
+                for (int r = 0; r < nr; ++r) {
+                    double c = tacc[i * kTreeRhs + r];
+                    if (rc.z >= 0) c += tacc[rc.z * kTreeRhs + r];
+                    if (rc.w >= 0) c += tacc[rc.w * kTreeRhs + r];
+                    tacc[i * kTreeRhs + r] = c;
+                }
+            }
+            __syncthreads();
+        }
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {
+            TQ* o = ch + (size_t)J.perm[J.tt_node[i]] * kp + rg;
+            for (int r = 0; r < nr; ++r) o[r] = (TQ)tacc[i * kTreeRhs + r];
+        }
+        __syncthreads();
+    }
 }
 
 // ------------------------------------------------------------------ gather
@@ -975,35 +948,103 @@ static hipError_t gather_dispatch(const BatchView& bv, const GatherSeg& sg, hipS
 }
 
 // ------------------------------------------------------------------ down-pass
-template <typename TQ, int KC>
-__global__ void k_down(BatchView bv) {
-    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
-    const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
-    if (g >= bv.leaves) return;
-    const BJob& J = bv.jobs[jb];
-    const int leaf = (int)(g - bv.leaf_off[jb]);
-    const TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.pot64) : reinterpret_cast<const TQ*>(J.pot32);
-    const int k = J.k;
-    for (int r0 = 0; r0 < k; r0 += KC) {
-        const int kc = min(k - r0, KC);
-        double acc[KC];
-#pragma unroll
-        for (int r = 0; r < KC; ++r) acc[r] = 0.0;
-        int a = J.leaf_node[leaf];
-        while (a >= 0) {
-            const TQ* p = pot + (size_t)J.perm[a] * J.kp + r0;
-#pragma unroll
-            for (int r = 0; r < KC; ++r)
-                if (r < kc) acc[r] += (double)p[r];
-            a = J.parent[a];
+// Top nodes of one job (one block): each top node's potential row becomes the sum
+// of the potentials on its path to the root.  The rows go to LDS, the levels
+// (shallowest first) add the parent's sum in LDS, and the rows are written back;
+// jobs with more than kTreeTopLds top nodes take the same steps in global memory.
+template <typename TQ>
+__global__ void __launch_bounds__(kTreeThreads) k_down_top(BatchView bv) {
+    __shared__ double tacc[kTreeTopLds * kTreeRhs];
+    const BJob& J = bv.jobs[blockIdx.x];
+    const int k = J.k, kp = J.kp, nt = J.ntop;
+    if (nt == 0) return;
+    TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.pot64) : reinterpret_cast<TQ*>(J.pot32);
+    if (nt > kTreeTopLds) {
+        for (int lv = 0; lv < J.tt_nlev; ++lv) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+                const int n = J.tt_node[i];
+                const int p = J.parent[n];
+                if (p < 0) continue;
+                TQ* o = pot + (size_t)J.perm[n] * kp;
+                const TQ* a = pot + (size_t)J.perm[p] * kp;
+                for (int r = 0; r < k; ++r) o[r] = (TQ)((double)o[r] + (double)a[r]);
+            }
+            __syncthreads();
         }
-#pragma unroll
-        for (int r = 0; r < KC; ++r)
-            if (r < kc) J.v[(size_t)(r0 + r) * J.N + leaf] = acc[r];
+        return;
+    }
+    for (int rg = 0; rg < k; rg += kTreeRhs) {
+        const int nr = min(kTreeRhs, k - rg);
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {
+            const TQ* o = pot + (size_t)J.perm[J.tt_node[i]] * kp + rg;
+            for (int r = 0; r < nr; ++r) tacc[i * kTreeRhs + r] = (double)o[r];
+        }
+        __syncthreads();
+        for (int lv = 0; lv < J.tt_nlev; ++lv) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+                const int ps = J.tt_rec[i].y;
+                if (ps < 0) continue;
+                for (int r = 0; r < nr; ++r) tacc[i * kTreeRhs + r] += tacc[ps * kTreeRhs + r];
+            }
+            __syncthreads();
+        }
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {
+            TQ* o = pot + (size_t)J.perm[J.tt_node[i]] * kp + rg;
+            for (int r = 0; r < nr; ++r) o[r] = (TQ)tacc[i * kTreeRhs + r];
+        }
+        __syncthreads();
     }
 }
 
-__global__ void k_2d_last(BatchView bv) {
+// Inner nodes of one chunk.  One parallel pass puts every inner node's potential
+// in LDS, plus its parent's path sum when the parent is a top node (k_down_top);
+// the levels (shallowest first) add the parent's sum in LDS alone; one parallel
+// pass writes the leaves' sums to v.
+template <typename TQ>
+__global__ void __launch_bounds__(kTreeThreads) k_down_inner(BatchView bv) {
+    __shared__ double tacc[kTreeChunk * kTreeRhs];
+    __shared__ int4 rec[kTreeChunk];
+    const int64_t g = blockIdx.x;
+    const int jb = find_job_wave(bv.tchunk_off, bv.njobs, g);
+    const BJob& J = bv.jobs[jb];
+    const int t = (int)(g - bv.tchunk_off[jb]);
+    const int k = J.k, kp = J.kp;
+    const TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.pot64) : reinterpret_cast<const TQ*>(J.pot32);
+    const int l0 = J.tb_lvoff[t], l1 = J.tb_lvoff[t + 1];
+    const int e0 = J.tb_lev[l0], e1 = J.tb_lev[l1 - 1];
+    for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) rec[i] = J.tb_rec[e0 + i];
+    __syncthreads();
+    for (int rg = 0; rg < k; rg += kTreeRhs) {
+        const int nr = min(kTreeRhs, k - rg);
+        for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) {
+            const int4 rc = rec[i];
+            const int sl = tree_lo(rc.x);
+            const TQ* own = pot + (size_t)rc.z * kp + rg;
+            const int pm = J.tb_ppm[e0 + i];
+            const TQ* pp = pm >= 0 ? pot + (size_t)pm * kp + rg : nullptr;
+            for (int r = 0; r < nr; ++r) tacc[sl * kTreeRhs + r] = (double)own[r] + (pp ? (double)pp[r] : 0.0);
+        }
+        __syncthreads();
+        for (int lv = l0; lv < l1 - 1; ++lv) {
+            for (int i = J.tb_lev[lv] - e0 + (int)threadIdx.x; i < J.tb_lev[lv + 1] - e0; i += kTreeThreads) {
+                const int4 rc = rec[i];
+                const int sl = tree_lo(rc.x), ps = tree_hi(rc.y);
+                if (ps < 0) continue;
+                for (int r = 0; r < nr; ++r) tacc[sl * kTreeRhs + r] += tacc[ps * kTreeRhs + r];
+            }
+            __syncthreads();
+        }
+        for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) {
+            const int4 rc = rec[i];
+            if (rc.w < 0) continue;
+            const int sl = tree_lo(rc.x);
+            for (int r = 0; r < nr; ++r) J.v[(size_t)(rg + r) * J.N + rc.w] = tacc[sl * kTreeRhs + r];
+        }
+        __syncthreads();
+    }
+}
+
+__global__ void k_2d_last(BatchView bv, int f32) {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= bv.rhs) return;
     const int jb = find_job(bv.rhs_off, bv.njobs, g);
@@ -1011,8 +1052,13 @@ __global__ void k_2d_last(BatchView bv) {
     if (J.dim != 2) return;
     const int r = (int)(g - bv.rhs_off[jb]);
     J.lt[2 * r] = J.v[(size_t)r * J.N + J.N - 1];
-    const DD t = J.prefix[(size_t)r * (J.N + 1) + J.N];
-    J.lt[2 * r + 1] = t.hi + t.lo;
+    // the total input charge: the group roots' charges (the product's FP32 or FP64 rows)
+    double t = 0.0;
+    for (int i = 0; i < J.nroots; ++i) {
+        const size_t o = (size_t)J.perm[J.troots[i]] * J.kp + r;
+        t += f32 ? (double)J.charge32[o] : J.charge64[o];
+    }
+    J.lt[2 * r + 1] = t;
 }
 
 __global__ void k_2d_fix(BatchView bv) {
@@ -1159,15 +1205,15 @@ __global__ void k_cplx_couple(BatchView bv, int jacobi) {
 hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan& plan, hipStream_t s, hipEvent_t* ev) {
     if (b.njobs == 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], s);
-    k_scan_reduce<<<(unsigned)b.tiles, kScanThreads, 0, s>>>(b, jacobi);
-    k_scan_top<<<blocks_for((size_t)b.rhs, 4), 256, 0, s>>>(b);
-    k_scan_final<<<(unsigned)b.tiles, kScanThreads, 0, s>>>(b, jacobi);
-    if (ev) (void)hipEventRecord(ev[1], s);
-    const unsigned nb = blocks_for((size_t)b.nodes, 256);
     if (prec == kMvp32)
-        k_node_charge<float><<<nb, 256, 0, s>>>(b, jacobi);
+        k_up_inner<float><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b, jacobi);
     else
-        k_node_charge<double><<<nb, 256, 0, s>>>(b, jacobi);
+        k_up_inner<double><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b, jacobi);
+    if (ev) (void)hipEventRecord(ev[1], s);
+    if (prec == kMvp32)
+        k_up_top<float><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
+    else
+        k_up_top<double><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int i = 0; i < plan.n; ++i) {
         hipError_t ge;
@@ -1190,12 +1236,15 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
     if (prec == kMvp32Acc) k_near_correct<8><<<blocks_for((size_t)b.nodes, 256), 256, 0, s>>>(b);
     if (ev) (void)hipEventRecord(ev[3], s);
     const unsigned lb = blocks_for((size_t)b.leaves, 256);
-    if (prec == kMvp32)
-        k_down<float, 8><<<lb, 256, 0, s>>>(b);
-    else
-        k_down<double, 8><<<lb, 256, 0, s>>>(b);
+    if (prec == kMvp32) {
+        k_down_top<float><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
+        k_down_inner<float><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b);
+    } else {
+        k_down_top<double><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
+        k_down_inner<double><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b);
+    }
     if (b.any2d) {
-        k_2d_last<<<blocks_for((size_t)b.rhs, 64), 64, 0, s>>>(b);
+        k_2d_last<<<blocks_for((size_t)b.rhs, 64), 64, 0, s>>>(b, prec == kMvp32 ? 1 : 0);
         k_2d_fix<<<lb, 256, 0, s>>>(b);
     }
     if (b.anycplx) k_cplx_couple<<<lb, 256, 0, s>>>(b, jacobi);
