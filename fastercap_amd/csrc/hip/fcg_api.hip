@@ -222,8 +222,8 @@ struct Job {
         src_geom, fill_rs;
     // tree passes of the product (kernels.h BJob): inner nodes per chunk and
     // level, top nodes per level, top flags, group roots
-    DevBuf tb_rec, tb_ppm, tb_lev, tb_lvoff, tt_node, tt_rec, tt_lev, ttop, troots;
-    int tt_nlev = 0, ntop = 0, nroots = 0;
+    DevBuf tb_rec, tb_ppm, tb_lev, tb_lvoff, tt_node, tt_rec, tt_lev, ttop, troots, lpath;
+    int tt_nlev = 0, ntop = 0, nroots = 0, lpath_len = 0, lpath_top = 0;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
     std::vector<int64_t> h_row_ptr;
@@ -278,7 +278,7 @@ struct Job {
                          leaf_w.bytes + col_range.bytes + self.bytes + diag.bytes + coef64.bytes + coef32.bytes +
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
                          col16.bytes + grow.bytes + src_geom.bytes + fill_rs.bytes + tb_rec.bytes +
-                         tb_ppm.bytes + tb_lev.bytes +
+                         tb_ppm.bytes + tb_lev.bytes + lpath.bytes +
                          tb_lvoff.bytes + tt_node.bytes + tt_rec.bytes + tt_lev.bytes + ttop.bytes + troots.bytes +
                          bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
@@ -305,6 +305,9 @@ struct Job {
         b.rorder = rorder.as<int32_t>();
         b.tb_rec = tb_rec.as<int4>();
         b.tb_ppm = tb_ppm.as<int32_t>();
+        b.lpath = lpath.as<int32_t>();
+        b.lpath_len = lpath_len;
+        b.lpath_top = lpath_top;
         b.tb_lev = tb_lev.as<int32_t>();
         b.tb_lvoff = tb_lvoff.as<int32_t>();
         b.tt_node = tt_node.as<int32_t>();
@@ -839,9 +842,19 @@ int upload_tree_passes(fcg_ctx* ctx, Job& j, const fcg_job_desc* d, const std::v
                            perm[(size_t)n], lc < 0 ? d->leaf_lo[n] : -1);
         if (p >= 0 && !lds_parent) ppm[i] = perm[(size_t)p];
     }
+    // the path of the last leaf's node to its root as charge rows, root first
+    // (2D fix-up values); its top nodes come first (a top node's parent is one)
+    std::vector<int32_t> lp;
+    for (int n = d->leaf_node[d->n_leaves - 1]; n >= 0; n = d->parent[n]) lp.push_back(n);
+    std::reverse(lp.begin(), lp.end());
+    j.lpath_len = (int)lp.size();
+    j.lpath_top = 0;
+    while (j.lpath_top < j.lpath_len && top[(size_t)lp[(size_t)j.lpath_top]]) ++j.lpath_top;
+    for (int32_t& n : lp) n = perm[(size_t)n];
     if (tt_node.empty()) tt_node.push_back(0);
     HIPCHECK(ctx, upload(j.tb_rec, tbr.data(), tbr.size(), s));
     HIPCHECK(ctx, upload(j.tb_ppm, ppm.data(), ppm.size(), s));
+    HIPCHECK(ctx, upload(j.lpath, lp.data(), lp.size(), s));
     HIPCHECK(ctx, upload(j.tb_lev, tb_lev.data(), tb_lev.size(), s));
     HIPCHECK(ctx, upload(j.tb_lvoff, tb_lvoff.data(), tb_lvoff.size(), s));
     HIPCHECK(ctx, upload(j.tt_node, tt_node.data(), tt_node.size(), s));

@@ -69,9 +69,13 @@ struct BJob {
     // (slots are the node's offset in its chunk, -1 for none or a top parent),
     // z = charge row (perm), w = leaf index (-1 for internal nodes); tb_ppm: the
     // charge row of a top parent (-1 otherwise); the top nodes by depth, level l
-    // at tt_node[tt_lev[l] ..]; ttop[n] = 1 for a top node; the group roots
+    // at tt_node[tt_lev[l] ..]; ttop[n] = 1 for a top node; the group roots;
+    // lpath: the charge rows on the path of the last leaf's node to its root,
+    // root first, its first lpath_top entries top nodes (2D fix-up)
     const int4* tb_rec;
     const int32_t* tb_ppm;
+    const int32_t* lpath;
+    int lpath_len, lpath_top;
     const int32_t* tb_lev;
     const int32_t* tb_lvoff;   // [chunks + 1]
     const int32_t* tt_node;
@@ -122,7 +126,7 @@ struct BJob {
     float* charge32;  // [M][kp], row perm[node]
     double* pot64;
     float* pot32;  // [M][kp], row perm[node]
-    double* lt;    // [2k] 2D fix-up scratch
+    double* lt;    // [2k] 2D fix-up values (k_down_top)
     // current product operands ([k][N] each): the input is q, or q32 when it is
     // an FP32 basis vector (then q is null)
     const double* q;

@@ -18,6 +18,8 @@
 //              GMRES with delayed reorthogonalisation (Solver/SolveCapacitance.cpp:2288-2574)
 #include "kernels.h"
 
+#include <algorithm>
+
 #include "../common/potential.h"
 
 namespace fcg {
@@ -957,8 +959,33 @@ __global__ void __launch_bounds__(kTreeThreads) k_down_top(BatchView bv) {
     __shared__ double tacc[kTreeTopLds * kTreeRhs];
     const BJob& J = bv.jobs[blockIdx.x];
     const int k = J.k, kp = J.kp, nt = J.ntop;
-    if (nt == 0) return;
     TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.pot64) : reinterpret_cast<TQ*>(J.pot32);
+    if (J.dim == 2) {
+        // 2D fix-up values, read by k_down_inner (MultiplyMatByVec_fast,
+        // Solver/MultiplyHierarchical.cpp:305-330): lt[2r] = the last leaf's result,
+        // the potentials on its path summed root first in the order the two passes
+        // sum them (the top part rounded to TQ as its row is stored, at every level
+        // where the top rows stay in global memory); lt[2r + 1] = the total input
+        // charge, the group roots' charge rows of this product
+        const TQ* ch = (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.charge64)
+                                         : reinterpret_cast<const TQ*>(J.charge32);
+        for (int r = (int)threadIdx.x; r < k; r += kTreeThreads) {
+            double v = 0.0;
+            int a = 0;
+            for (; a < J.lpath_top; ++a) {
+                v += (double)pot[(size_t)J.lpath[a] * kp + r];
+                if (nt > kTreeTopLds) v = (double)(TQ)v;
+            }
+            v = (double)(TQ)v;
+            for (; a < J.lpath_len; ++a) v += (double)pot[(size_t)J.lpath[a] * kp + r];
+            double t = 0.0;
+            for (int i = 0; i < J.nroots; ++i) t += (double)ch[(size_t)J.perm[J.troots[i]] * kp + r];
+            J.lt[2 * r] = v;
+            J.lt[2 * r + 1] = t;
+        }
+        __syncthreads();  // the top pass below rewrites rows read here
+    }
+    if (nt == 0) return;
     if (nt > kTreeTopLds) {
         for (int lv = 0; lv < J.tt_nlev; ++lv) {
             for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
@@ -999,16 +1026,25 @@ __global__ void __launch_bounds__(kTreeThreads) k_down_top(BatchView bv) {
 // Inner nodes of one chunk.  One parallel pass puts every inner node's potential
 // in LDS, plus its parent's path sum when the parent is a top node (k_down_top);
 // the levels (shallowest first) add the parent's sum in LDS alone; one parallel
-// pass writes the leaves' sums to v.
+// pass writes the leaves' sums to v, finishing them there:
+//  * complex jobs (MultiplyMatByVec_fast's complex branch,
+//    Solver/MultiplyHierarchical.cpp:332-455): each [Re | Im] pair of results is
+//    coupled through the imaginary self terms, v_re -= D u_im, v_im += D u_re, u
+//    the (Jacobi-scaled) input (D is zero on conductor panels);
+//  * 2D jobs (:305-330): every conductor row from the first conductor leaf on,
+//    but the last, subtracts the last leaf's result, and the last row becomes the
+//    total input charge (both from k_down_top, lt).
 template <typename TQ>
-__global__ void __launch_bounds__(kTreeThreads) k_down_inner(BatchView bv) {
+__global__ void __launch_bounds__(kTreeThreads) k_down_inner(BatchView bv, int jacobi) {
     __shared__ double tacc[kTreeChunk * kTreeRhs];
     __shared__ int4 rec[kTreeChunk];
+    __shared__ double fixv[2 * kTreeRhs];  // 2D: lt of this group of right-hand sides
     const int64_t g = blockIdx.x;
     const int jb = find_job_wave(bv.tchunk_off, bv.njobs, g);
     const BJob& J = bv.jobs[jb];
     const int t = (int)(g - bv.tchunk_off[jb]);
-    const int k = J.k, kp = J.kp;
+    const int k = J.k, kp = J.kp, N = J.N;
+    const bool d2 = J.dim == 2;
     const TQ* pot = (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.pot64) : reinterpret_cast<const TQ*>(J.pot32);
     const int l0 = J.tb_lvoff[t], l1 = J.tb_lvoff[t + 1];
     const int e0 = J.tb_lev[l0], e1 = J.tb_lev[l1 - 1];
@@ -1016,6 +1052,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_down_inner(BatchView bv) {
     __syncthreads();
     for (int rg = 0; rg < k; rg += kTreeRhs) {
         const int nr = min(kTreeRhs, k - rg);
+        if (d2 && (int)threadIdx.x < 2 * nr) fixv[threadIdx.x] = J.lt[2 * rg + threadIdx.x];
         for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) {
             const int4 rc = rec[i];
             const int sl = tree_lo(rc.x);
@@ -1036,42 +1073,35 @@ __global__ void __launch_bounds__(kTreeThreads) k_down_inner(BatchView bv) {
         }
         for (int i = (int)threadIdx.x; i < e1 - e0; i += kTreeThreads) {
             const int4 rc = rec[i];
-            if (rc.w < 0) continue;
+            const int n = rc.w;
+            if (n < 0) continue;
             const int sl = tree_lo(rc.x);
-            for (int r = 0; r < nr; ++r) J.v[(size_t)(rg + r) * J.N + rc.w] = tacc[sl * kTreeRhs + r];
+            const bool fx = d2 && (n >= J.first_cond_leaf || n == N - 1);  // a 2D fix-up row
+            const double c = J.self_im ? J.self_im[n] : 0.0;
+            if (c == 0.0) {
+                for (int r = 0; r < nr; ++r) {
+                    double x = tacc[sl * kTreeRhs + r];
+                    if (fx) x = n < N - 1 ? x - fixv[2 * r] : fixv[2 * r + 1];
+                    J.v[(size_t)(rg + r) * N + n] = x;
+                }
+            } else {  // complex job: the pairs (rg + r, rg + r + 1), rg and kTreeRhs even
+                const double sc = jacobi ? 1.0 / J.self[n] : 1.0;
+                for (int r = 0; r + 1 < nr; r += 2) {
+                    const size_t ore = (size_t)(rg + r) * N + n, oim = ore + N;
+                    const double ure = (J.q32 ? (double)J.q32[ore] : J.q[ore]) * sc;
+                    const double uim = (J.q32 ? (double)J.q32[oim] : J.q[oim]) * sc;
+                    double xr = tacc[sl * kTreeRhs + r] - c * uim;
+                    double xi = tacc[sl * kTreeRhs + r + 1] + c * ure;
+                    if (fx) {
+                        xr = n < N - 1 ? xr - fixv[2 * r] : fixv[2 * r + 1];
+                        xi = n < N - 1 ? xi - fixv[2 * r + 2] : fixv[2 * r + 3];
+                    }
+                    J.v[ore] = xr;
+                    J.v[oim] = xi;
+                }
+            }
         }
         __syncthreads();
-    }
-}
-
-__global__ void k_2d_last(BatchView bv, int f32) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= bv.rhs) return;
-    const int jb = find_job(bv.rhs_off, bv.njobs, g);
-    const BJob& J = bv.jobs[jb];
-    if (J.dim != 2) return;
-    const int r = (int)(g - bv.rhs_off[jb]);
-    J.lt[2 * r] = J.v[(size_t)r * J.N + J.N - 1];
-    // the total input charge: the group roots' charges (the product's FP32 or FP64 rows)
-    double t = 0.0;
-    for (int i = 0; i < J.nroots; ++i) {
-        const size_t o = (size_t)J.perm[J.troots[i]] * J.kp + r;
-        t += f32 ? (double)J.charge32[o] : J.charge64[o];
-    }
-    J.lt[2 * r + 1] = t;
-}
-
-__global__ void k_2d_fix(BatchView bv) {
-    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
-    const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
-    if (g >= bv.leaves) return;
-    const BJob& J = bv.jobs[jb];
-    if (J.dim != 2) return;
-    const int n = (int)(g - bv.leaf_off[jb]);
-    for (int r = 0; r < J.k; ++r) {
-        double* vr = J.v + (size_t)r * J.N;
-        if (n >= J.first_cond_leaf && n < J.N - 1) vr[n] -= J.lt[2 * r];
-        if (n == J.N - 1) vr[n] = 1.0 * J.lt[2 * r + 1];
     }
 }
 
@@ -1177,31 +1207,18 @@ __global__ void k_near_correct(BatchView bv) {
     }
 }
 
-// Complex jobs (MultiplyMatByVec_fast's complex branch,
-// Solver/MultiplyHierarchical.cpp:332-455): the product's 2k vectors are the
-// [Re | Im] halves of k complex vectors, each half already multiplied by the
-// real hierarchical matrix; the imaginary self terms couple them,
-// v_re -= D u_im and v_im += D u_re, u the (Jacobi-scaled) input.  D is zero on
-// conductor panels, so this commutes with the 2D fix-up (conductor rows only).
-__global__ void k_cplx_couple(BatchView bv, int jacobi) {
-    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x, g = g0 + threadIdx.x;
-    const int jb = find_job_block(bv.leaf_off, bv.njobs, g0, g < bv.leaves ? g : bv.leaves - 1);
-    if (g >= bv.leaves) return;
-    const BJob& J = bv.jobs[jb];
-    if (!J.self_im) return;
-    const int n = (int)(g - bv.leaf_off[jb]);
-    const double c = J.self_im[n];
-    if (c == 0.0) return;
-    const double sc = jacobi ? 1.0 / J.self[n] : 1.0;
-    for (int r = 0; r < J.k; r += 2) {
-        const size_t ore = (size_t)r * J.N + n, oim = ore + J.N;
-        const double ure = (J.q32 ? (double)J.q32[ore] : J.q[ore]) * sc;
-        const double uim = (J.q32 ? (double)J.q32[oim] : J.q[oim]) * sc;
-        J.v[ore] -= c * uim;
-        J.v[oim] += c * ure;
-    }
+
+// staged when the variant asks for it, every job has its tile source lists and
+// the longest list fits in LDS in the product's charge type
+static bool seg_staged(const GatherSeg& sg, int prec) {
+    return gather_variant_staged(sg.variant) && sg.staged && sg.idx16 &&
+           (size_t)sg.stage_floats * (prec == kMvp32 ? 4 : 8) <= kStageMaxBytes;
 }
 
+// The up-pass (chunks, then the top nodes of every job), the gather (one launch
+// per rhs class range; the refinement residual adds its near-field correction),
+// the down-pass (the top nodes of every job with the 2D fix-up values, then the
+// chunks, which also couple complex jobs and fix up 2D jobs).
 hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan& plan, hipStream_t s, hipEvent_t* ev) {
     if (b.njobs == 0) return hipSuccess;
     if (ev) (void)hipEventRecord(ev[0], s);
@@ -1218,10 +1235,7 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
     for (int i = 0; i < plan.n; ++i) {
         hipError_t ge;
         const GatherSeg& sg = plan.seg[i];
-        // staged when the variant asks for it, every job has its tile source lists
-        // and the longest list fits in LDS in this charge type
-        const bool st = gather_variant_staged(sg.variant) && sg.staged && sg.idx16 &&
-                        (size_t)sg.stage_floats * (prec == kMvp32 ? 4 : 8) <= kStageMaxBytes;
+        const bool st = seg_staged(sg, prec);
         if (prec == kMvp64)
             ge = st ? gather_dispatch_staged<double, double, double>(b, sg, s)
                     : gather_dispatch<double, double, double>(b, sg, s);
@@ -1235,19 +1249,13 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
     }
     if (prec == kMvp32Acc) k_near_correct<8><<<blocks_for((size_t)b.nodes, 256), 256, 0, s>>>(b);
     if (ev) (void)hipEventRecord(ev[3], s);
-    const unsigned lb = blocks_for((size_t)b.leaves, 256);
     if (prec == kMvp32) {
         k_down_top<float><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
-        k_down_inner<float><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b);
+        k_down_inner<float><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b, jacobi);
     } else {
         k_down_top<double><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
-        k_down_inner<double><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b);
+        k_down_inner<double><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b, jacobi);
     }
-    if (b.any2d) {
-        k_2d_last<<<blocks_for((size_t)b.rhs, 64), 64, 0, s>>>(b, prec == kMvp32 ? 1 : 0);
-        k_2d_fix<<<lb, 256, 0, s>>>(b);
-    }
-    if (b.anycplx) k_cplx_couple<<<lb, 256, 0, s>>>(b, jacobi);
     if (ev) (void)hipEventRecord(ev[4], s);
     return hipGetLastError();
 }

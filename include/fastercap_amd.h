@@ -338,10 +338,11 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
  * vectors.  No reference counterpart: plays the role of the reference's phase
  * timers (Solver/SolveCapacitance.cpp:704-720) for the product alone. */
 typedef struct fcg_mvp_timing {
-    double ms_scan;    /* prefix scan of the input (3 kernels) */
-    double ms_charge;  /* node charges */
-    double ms_gather;  /* link gather (the dominant kernel) */
-    double ms_down;    /* down-pass (+ 2D fix-up) */
+    double ms_scan;    /* up-pass: every node charge (one kernel; top nodes included) */
+    double ms_charge;  /* 0: the node charges are part of the up-pass */
+    double ms_gather;  /* link gather (the dominant kernel; with each job's top stage of
+                          the down-pass when the gather is staged) */
+    double ms_down;    /* down-pass (+ complex coupling, 2D fix-up) */
     double ms_total;   /* whole product */
     int32_t reps, k;
     int32_t variant;   /* gather configuration in use */
