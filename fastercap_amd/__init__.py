@@ -35,6 +35,7 @@ from ._ffi import (
     HostOptions,
     HostStats,
     JobDesc,
+    JobMem,
     LinkParams,
     NodeGeom,
     SolveOpts,
@@ -182,6 +183,12 @@ class SolveResult:
     charges: np.ndarray | None = None
     converged: bool = True  # every rhs met the tolerance (mixed: the FP64 residual check)
     cycles: int = 0  # inner GMRES cycles (mixed: refinement cycles)
+    # with want_residuals: per rhs, |r0| then |g_i| / |b| after each Arnoldi step (-r)
+    residuals: list | None = None
+    t_precond_s: float = 0.0
+    mem: dict = field(default_factory=dict)  # device bytes of the solve (gmres, vectors, product)
+    t_fill_s: float = 0.0  # set by callers that time the fill (command line -i)
+    job_mem: dict = field(default_factory=dict)  # fcg_job_memory of the job (command line -i)
 
 
 _STATS_DTYPE = None
@@ -338,6 +345,7 @@ class Device:
         want_charges: bool = False,
         max_refine: int = 8,
         precond_size: int = 0,
+        want_residuals: bool = False,
     ) -> SolveResult:
         ncond = mesh.n_cond
         rows_max = ncond
@@ -346,6 +354,12 @@ class Device:
         charges = np.zeros(nf * rows_max * mesh.N, dtype=np.float64) if want_charges else None
         o = SolveOpts(tol=tol, max_iter=max_iter, precond=precond, precision=precision, max_refine=max_refine,
                       precond_size=precond_size)
+        hist = None
+        if want_residuals:
+            hcap = max_iter * (max_refine + 1) + 1
+            hist = np.zeros(rows_max * hcap, dtype=np.float64)
+            o.residuals = _dptr(hist)
+            o.residuals_cap = hcap
         st = SolveStats()
         rows, cols = C.c_int(), C.c_int()
         self._check(
@@ -381,7 +395,18 @@ class Device:
             t_mvp_s=st.t_mvp_s,
             n_mvp=st.n_mvp,
             charges=charges,
+            residuals=None if hist is None else [
+                hist[q * o.residuals_cap: q * o.residuals_cap + min(st.iterations[q] + 1, o.residuals_cap)].copy()
+                for q in range(min(r, 64))],
+            t_precond_s=st.t_precond_s,
+            mem={"gmres": st.mem_gmres, "vectors": st.mem_vectors, "product": st.mem_product},
         )
+
+    def memory(self, job: int) -> dict:
+        """Device bytes a resident job holds by category (fcg_job_memory)."""
+        m = JobMem()
+        self._check(self.lib.fcg_job_memory(self._h, job, C.byref(m)), "fcg_job_memory")
+        return {"hierarchy": m.hierarchy, "links": m.links, "conductors": m.conductors, "precond": m.precond}
 
     def solve_batch(
         self,

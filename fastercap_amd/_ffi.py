@@ -102,6 +102,8 @@ class SolveOpts(C.Structure):
         ("precision", C.c_int32),
         ("max_refine", C.c_int32),
         ("precond_size", C.c_int32),
+        ("residuals", C.POINTER(C.c_double)),
+        ("residuals_cap", C.c_int32),
     ]
 
 
@@ -117,6 +119,19 @@ class SolveStats(C.Structure):
         ("n_mvp", C.c_int64),
         ("converged", C.c_int32),
         ("cycles", C.c_int32),
+        ("t_precond_s", C.c_double),
+        ("mem_gmres", C.c_int64),
+        ("mem_vectors", C.c_int64),
+        ("mem_product", C.c_int64),
+    ]
+
+
+class JobMem(C.Structure):
+    _fields_ = [
+        ("hierarchy", C.c_int64),
+        ("links", C.c_int64),
+        ("conductors", C.c_int64),
+        ("precond", C.c_int64),
     ]
 
 
@@ -235,6 +250,7 @@ DEVICE_SYMBOLS = (
     "fcg_job_upload_links",
     "fcg_refine",
     "fcg_refine_fetch",
+    "fcg_job_memory",
 )
 HOST_SYMBOLS = (
     "fch_default_options",
@@ -359,6 +375,7 @@ def device_lib() -> C.CDLL:
         lib.fcg_refine.argtypes = [vp, C.POINTER(RefineNode), C.c_int, C.POINTER(C.c_int32), C.POINTER(RefineParams),
                                    C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int32)]
         lib.fcg_refine_fetch.argtypes = [vp, C.POINTER(RefineNode)]
+        lib.fcg_job_memory.argtypes = [vp, C.c_int, C.POINTER(JobMem)]
         lib.fcg_last_error.argtypes = [vp]
         lib.fcg_last_error.restype = C.c_char_p
         for name in DEVICE_SYMBOLS:

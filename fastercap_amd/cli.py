@@ -46,6 +46,8 @@ class Args:
     out_csv: bool = False  # -e
     out_geo: bool = False  # -o
     out_charge: bool = False  # -c
+    dump_resid: bool = False  # -r
+    dump_time: bool = False  # -i
     stop: bool = False  # -b? / -bv
 
 
@@ -115,6 +117,10 @@ def parse_args(argv) -> Args:
             a.out_geo = True
         elif opt == "-c":
             a.out_charge = True
+        elif opt == "-r":
+            a.dump_resid = True
+        elif opt == "-i":
+            a.dump_time = True
         elif opt == "-gmixed":
             a.precision = 1
         elif opt.startswith("-gdev"):
@@ -183,11 +189,28 @@ def preconditioner(a: Args, mesh):
     return a.precond, (a.block_size if a.precond == 2 else a.super_dim)
 
 
-def format_stats(mesh) -> str:
-    """CSolveCap::OutputSolveStats (Solver/SolveCapacitance.cpp:684-721)."""
+def format_stats(mesh, a: Args | None = None, res=None) -> str:
+    """CSolveCap::OutputSolveStats (Solver/SolveCapacitance.cpp:684-721); with -i the
+    phase times and the device memory of the job and its solve (:704-720)."""
     st = mesh.stats
     n, links = int(st.panels), int(st.links) + int(st.panels)
     ratio = np.float32(100.0) - np.float32(links) / np.float32(n * n) * 100.0
+    extra = ""
+    if a is not None and a.dump_time and res is not None:
+        jm = res.job_mem
+        extra = (f"Time for reading input file: {st.t_read_s:f}s\n"
+                 f"Time for building super hierarchy: {st.t_superh_s:f}s\n"
+                 f"Time for discretization: {st.t_discretize_s:f}s\n"
+                 f"Time for building potential matrix: {st.t_links_s + res.t_fill_s:f}s\n"
+                 f"Time for precond calculation: {res.t_precond_s:f}s\n"
+                 f"Time for gmres solving: {res.t_solve_s:f}s\n"
+                 f"Memory allocated for panel hierarchy: {jm.get('hierarchy', 0)} bytes\n"
+                 f"Memory allocated for links structure: {jm.get('links', 0)} bytes\n"
+                 f"Memory allocated for conductor list: {jm.get('conductors', 0)} bytes\n"
+                 f"Memory allocated for Gmres: {res.mem.get('gmres', 0)} bytes\n"
+                 f"Memory allocated for preconditioner: {jm.get('precond', 0)} bytes\n"
+                 f"Memory allocated for charge vectors: {res.mem.get('vectors', 0)} bytes\n"
+                 f"Memory allocated for hierarchical multiplication: {res.mem.get('product', 0)} bytes\n")
     return ("\nSolve statistics:\n"
             f"Number of input panels: {st.input_raw_cond_panels + st.input_raw_diel_panels} of which "
             f"{st.input_raw_cond_panels} conductors and {st.input_raw_diel_panels} dielectric\n"
@@ -196,7 +219,7 @@ def format_stats(mesh) -> str:
             f"Number of potential estimates: {int(st.links) // 2}\n"
             f"Number of links: {links} (uncompressed {n * n}, compression ratio is {ratio:.1f}%)\n"
             f"Max recursion level: {st.max_level}\n"
-            f"Max Mesh relative refinement value: {st.max_mesh_eps:g}\n")
+            f"Max Mesh relative refinement value: {st.max_mesh_eps:g}\n" + extra)
 
 
 def format_params(a: Args, tol: float) -> str:
@@ -228,8 +251,12 @@ def _report_solve(out, a: Args, mesh, res):
               f"Number of panels after refinement: {mesh.N}\nNumber of links to be computed: {mesh.L}\n"
               "Done computing links\n***************************************\n"
               f"Precond Type(s) (-p): {_precond_text(precond, size)}\n")
-    for it in res.iterations:
-        out.write("GMRES Iteration: " + "".join(f"{i} " for i in range(it)) + "\n")
+    for q, it in enumerate(res.iterations):
+        h = res.residuals[q] if res.residuals is not None else None
+        if h is None:
+            out.write("GMRES Iteration: " + "".join(f"{i} " for i in range(it)) + "\n")
+        else:  # -r: the residual before each step, the final one at the end
+            out.write("GMRES Iteration: " + "".join(f"{h[i]:.3g} {i} " for i in range(it)) + f"{h[it]:.3f} \n")
 
 
 def solve_file(dev, a: Args, mesh_eps: float, tol: float, out=None):
@@ -239,10 +266,14 @@ def solve_file(dev, a: Args, mesh_eps: float, tol: float, out=None):
                       device=None if a.host_links else dev, refine_on_device=not a.host_links)
     job = dev.upload(mesh)
     try:
+        t0 = time.perf_counter()
         dev.fill(job, a.precision)
+        t_fill = time.perf_counter() - t0
         precond, size = preconditioner(a, mesh)
         res = dev.solve(job, mesh, tol=tol, precond=precond, precision=a.precision, precond_size=size,
-                        want_charges=a.out_charge)
+                        want_charges=a.out_charge, want_residuals=a.dump_resid)
+        res.t_fill_s = t_fill
+        res.job_mem = dev.memory(job)
         if out is not None:
             _report_solve(out, a, mesh, res)
     finally:
@@ -302,7 +333,7 @@ def run(a: Args, out=sys.stdout):
                       f"Refinement completed\nMesh refinement (-m): {a.mesh_eps:g}\n")
             mesh, res = solve_file(dev, a, a.mesh_eps, a.tol, out)
             out.write(format_cap(res.cap, mesh.conductor_names, mesh.dim, a.verbose))
-            out.write(format_stats(mesh))
+            out.write(format_stats(mesh, a, res))
             write_outputs(a, mesh, res, out)
             return mesh, res
         tol = a.auto_err / 2.0
@@ -313,7 +344,7 @@ def run(a: Args, out=sys.stdout):
                   f"Refinement completed\nMesh refinement (-m): {1e32:g}\n")
         mesh, res = solve_file(dev, a, 1e32, tol, out)
         out.write(format_cap(res.cap, mesh.conductor_names, mesh.dim, a.verbose))
-        out.write(format_stats(mesh))
+        out.write(format_stats(mesh, a, res))
         old_p, old_l = mesh.N, mesh.L
         for i in range(1, MAX_AUTO_ITERATIONS):
             out.write(f"\nIteration number #{i} ***************************\n\n"
@@ -333,7 +364,7 @@ def run(a: Args, out=sys.stdout):
             diff = np.linalg.norm(nres.cap - res.cap) / np.linalg.norm(nres.cap)
             out.write(format_cap(nres.cap, nmesh.conductor_names, nmesh.dim, a.verbose))
             out.write(f"Weighted Frobenius norm of the difference between capacitance (auto option): {diff:g}\n")
-            out.write(format_stats(nmesh))
+            out.write(format_stats(nmesh, a, nres))
             mesh, res = nmesh, nres
             old_p, old_l = mesh.N, mesh.L
             if diff <= a.auto_err:

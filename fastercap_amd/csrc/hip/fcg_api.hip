@@ -1416,6 +1416,9 @@ struct Solve {
     int cap_iter = 0;
     int64_t n_mvp = 0;
     double rel = 0.0;
+    // residual history per rhs (fcg_solve_opts::residuals): |r0|, then |g_i| / |b|
+    bool want_hist = false;
+    std::vector<std::vector<double>> hist;
     clk::time_point t0;
     int hoff = 0, ooff = 0;  // the step's reduction and host-output offsets in the step buffer
     int64_t bytes() const {
@@ -1429,6 +1432,7 @@ struct Ready {
     int64_t tag;
     int job;
     std::vector<double> cap, x;
+    std::vector<std::vector<double>> hist;
     int rows, cols;
     fcg_solve_stats stats;
 };
@@ -1479,6 +1483,7 @@ struct fcg_engine {
     Cohort co[kCohorts];
     std::deque<Ready> ready;
     bool want_x = false;
+    bool want_hist = false;  // record residual histories (fcg_solve with opts->residuals)
     size_t n_active() const {
         size_t n = 0;
         for (const Cohort& c : co) n += c.active.size();
@@ -1677,6 +1682,10 @@ void start_cycle(fcg_engine* E, Solve& S, const double* beta) {
         }
         if (beta[q] / S.normb[(size_t)q] < tol) S.done[(size_t)q] = 1;
     }
+    if (S.want_hist && S.cycles == 0) {
+        S.hist.assign((size_t)S.k, {});
+        for (int q = 0; q < S.k; ++q) S.hist[(size_t)q].push_back(beta[q]);
+    }
     S.i = 0;
     S.cycles++;
     S.need_scale = true;
@@ -1731,6 +1740,7 @@ int engine_submit(fcg_engine* E, int job, int64_t tag) {
         if (S->twolev && J->spre.m > fcg::kSuperLds) RCCHECK(ensure_super_precond(ctx, *J, E->opts.precond_size));
     }
     S->want_x = E->want_x;
+    S->want_hist = E->want_hist;
     S->cap_iter = std::min(E->opts.max_iter > 0 ? E->opts.max_iter : 1000, J->N);
     RCCHECK(ensure_scratch(ctx, S->scr, J->N, J->M, J->pk(S->k), mixed));
     RCCHECK(ensure_tile_sources(ctx, *J));
@@ -2129,6 +2139,11 @@ int engine_finish(fcg_engine* E, Cohort& C) {
         rd.stats.converged = S.converged ? 1 : 0;
         rd.stats.cycles = S.cycles;
         rd.stats.t_solve_s = std::chrono::duration<double>(clk::now() - S.t0).count();
+        rd.stats.mem_gmres = (int64_t)S.hd.r;
+        for (const auto& qb : S.qblocks) rd.stats.mem_gmres += (int64_t)qb->r;
+        rd.stats.mem_vectors = (int64_t)(S.x.r + S.r.r + S.w.r + S.u.r + S.spw.r);
+        rd.stats.mem_product = (int64_t)S.scr.bytes();
+        rd.hist = std::move(S.hist);
         E->ready.push_back(std::move(rd));
         S.phase = kDone;
     }
@@ -2167,6 +2182,7 @@ int engine_finish(fcg_engine* E, Cohort& C) {
             S->total_steps[(size_t)q]++;
             S->last_i[(size_t)q] = S->i;
             S->gres[(size_t)q] = gr;
+            if (S->want_hist && (size_t)q < S->hist.size()) S->hist[(size_t)q].push_back(gr / S->normb[(size_t)q]);
             if (gr < S->thr[(size_t)q] || st == 3.0) S->done[(size_t)q] = 1;
         }
         bool all = true;
@@ -3185,11 +3201,12 @@ int fcg_engine_destroy(fcg_engine* eng) {
 namespace {
 // run an engine over `jobs` to completion; results in job order
 int run_batch(fcg_ctx* ctx, const int* jobs, int njobs, const fcg_solve_opts* opts, bool want_x, double* caps,
-              int* rows, int* cols, fcg_solve_stats* stats, double* charges) {
+              int* rows, int* cols, fcg_solve_stats* stats, double* charges, bool want_hist = false) {
     fcg_engine* e = nullptr;
     RCCHECK(fcg_engine_create(ctx, opts, &e));
     std::unique_ptr<fcg_engine> guard(e);
     e->want_x = want_x;
+    e->want_hist = want_hist && opts->residuals && opts->residuals_cap > 0;
     std::vector<size_t> off((size_t)njobs + 1, 0);
     for (int i = 0; i < njobs; ++i) {
         Job* j = get_job(ctx, jobs[i]);
@@ -3209,6 +3226,11 @@ int run_batch(fcg_ctx* ctx, const int* jobs, int njobs, const fcg_solve_opts* op
             if (cols) cols[i] = r.cols;
             if (stats) stats[i] = r.stats;
             if (charges && !r.x.empty()) std::memcpy(charges, r.x.data(), sizeof(double) * r.x.size());
+            if (e->want_hist)  // a single job (fcg_solve)
+                for (size_t q = 0; q < r.hist.size(); ++q) {
+                    const size_t n = std::min(r.hist[q].size(), (size_t)opts->residuals_cap);
+                    std::memcpy(opts->residuals + q * (size_t)opts->residuals_cap, r.hist[q].data(), sizeof(double) * n);
+                }
             e->ready.pop_front();
             ++got;
         }
@@ -3230,10 +3252,41 @@ int fcg_solve(fcg_ctx* ctx, int job, const fcg_solve_opts* opts, double* cap, in
     Job* j = get_job(ctx, job);
     if (!j || !opts || !cap) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
+    // the preconditioner is built ahead of the iterations (the engine finds it
+    // built), so that its time is the reference's "precond calculation" phase
+    const auto tp0 = clk::now();
+    if (!j->cplx && (opts->precond & FCG_PRECOND_SUPER)) {
+        RCCHECK(ensure_super_precond(ctx, *j, opts->precond_size));
+    } else if (!j->cplx && (opts->precond & FCG_PRECOND_BLOCK)) {
+        RCCHECK(ensure_block_precond(ctx, *j, opts->precond_size));
+    }
+    HIPCHECK(ctx, hipStreamSynchronize(ctx->stream));
+    const double tp = std::chrono::duration<double>(clk::now() - tp0).count();
     // the gather shapes are the rhs class defaults set at upload; measuring every
     // shape costs about a hundred products, more than a one-off solve, so it is
     // left to fcg_autotune / fcg_autotune_batch
-    return run_batch(ctx, &job, 1, opts, charges != nullptr, cap, rows_out, cols_out, stats, charges);
+    RCCHECK(run_batch(ctx, &job, 1, opts, charges != nullptr, cap, rows_out, cols_out, stats, charges, true));
+    if (stats) stats->t_precond_s = tp;
+    return FCG_OK;
+}
+
+int fcg_job_memory(fcg_ctx* ctx, int job, fcg_job_mem* out) {
+    Job* j = get_job(ctx, job);
+    if (!j || !out) return fail(ctx, FCG_INVALID_ARGUMENT, "bad argument");
+    out->hierarchy = (int64_t)(j->nodes.bytes + j->left.bytes + j->right.bytes + j->parent.bytes + j->leaf_lo.bytes +
+                               j->leaf_hi.bytes + j->leaf_node.bytes + j->gout.bytes + j->gin.bytes + j->gout_im.bytes +
+                               j->gin_im.bytes + j->perm.bytes + j->iperm.bytes + j->rorder.bytes + j->tb_rec.bytes +
+                               j->tb_ppm.bytes + j->tb_lev.bytes + j->tb_lvoff.bytes + j->tt_node.bytes +
+                               j->tt_rec.bytes + j->tt_lev.bytes + j->ttop.bytes + j->troots.bytes + j->lpath.bytes +
+                               j->self.bytes + j->self_im.bytes + j->diag.bytes + j->src_geom.bytes);
+    out->links = (int64_t)(j->row_ptr.bytes + j->col.bytes + j->col16.bytes + j->link_row.bytes + j->coef64.bytes +
+                           j->coef32.bytes + j->fill_order.bytes + j->fill_start.bytes + j->fill_rs.bytes +
+                           j->grow.bytes + j->tsrc_off.bytes + j->tsrc.bytes + j->lsrc.bytes + j->near_ptr.bytes +
+                           j->near_col.bytes + j->near_link.bytes + j->near_dc.bytes);
+    out->conductors = (int64_t)(j->leaf_w.bytes + j->leaf_w_im.bytes + j->col_range.bytes + j->rhs_range.bytes);
+    out->precond = (int64_t)(j->bpre.bstart.bytes + j->bpre.boff.bytes + j->bpre.P.bytes + j->spre.snode.bytes +
+                             j->spre.sstart.bytes + j->spre.sidx.bytes + j->spre.Sinv.bytes + j->spre.w.bytes);
+    return FCG_OK;
 }
 
 // ---------------------------------------------------------------- link discovery

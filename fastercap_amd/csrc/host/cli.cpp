@@ -40,6 +40,8 @@ struct Options {
     bool out_csv = false;     // -e: matrix to <input>.csv
     bool out_geo = false;     // -o: refined geometry in FastCap format
     bool out_charge = false;  // -c: charge densities per conductor in FastCap format
+    bool dump_resid = false;  // -r: GMRES residual at each iteration
+    bool dump_time = false;   // -i: phase times and memory
 };
 
 void usage() {
@@ -58,6 +60,8 @@ void usage() {
         "  -e       Also write the capacitance matrix to <input>.csv\n"
         "  -o       Write the refined geometry in FastCap format (<input>_ref.lst)\n"
         "  -c       Write each conductor's charge densities in FastCap format (<input>_<conductor>.lst)\n"
+        "  -r       Print the GMRES residual at each iteration\n"
+        "  -i       Print phase times and device memory\n"
         "  -v       Verbose output\n"
         "  -b       Console mode (always)\n"
         "  -b?      This text\n"
@@ -147,6 +151,10 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
             o.out_geo = true;
         } else if (!std::strcmp(a, "-c")) {
             o.out_charge = true;
+        } else if (!std::strcmp(a, "-r")) {
+            o.dump_resid = true;
+        } else if (!std::strcmp(a, "-i")) {
+            o.dump_time = true;
         } else if (!std::strcmp(a, "-gmixed")) {
             o.precision = FCG_PREC_MIXED;
         } else if (!std::strncmp(a, "-gdev", 5)) {
@@ -194,6 +202,8 @@ struct Result {
     std::vector<std::string> names;
     fch_stats hs{};
     fcg_solve_stats ss{};
+    fcg_job_mem mem{};
+    double t_fill_s = 0.0;
 };
 
 // refinement and links at the given mesh eps (RefineGeoAndCountLinks +
@@ -308,9 +318,19 @@ int solve_mesh(fcg_ctx* ctx, const Options& o, double tol, Result& r) {
     std::printf("Precond Type(s) (-p): %s\n", precond_line(precond, size));
     int job = -1;
     rc = o.host_links ? fcg_job_upload(ctx, &d, &job) : fcg_job_upload_links(ctx, &d, &job);
+    const auto tf = std::chrono::steady_clock::now();
     if (rc == FCG_OK) rc = fcg_fill(ctx, job, o.precision);
+    r.t_fill_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - tf).count();
+    // -r: the residual history of every right-hand side (|r0|, then |g_i| / |b|)
+    constexpr int kHistCap = 1000 * 9 + 1;  // max_iter x (max_refine + 1) + 1
+    std::vector<double> hist;
     if (rc == FCG_OK) {
-        fcg_solve_opts so{tol, 1000, precond, o.precision, 8, size};
+        fcg_solve_opts so{tol, 1000, precond, o.precision, 8, size, nullptr, 0};
+        if (o.dump_resid) {
+            hist.assign((size_t)d.n_groups * kHistCap, 0.0);
+            so.residuals = hist.data();
+            so.residuals_cap = kHistCap;
+        }
         const int nc = (int)r.names.size();
         r.cplx = d.group_outperm_im != nullptr;
         const size_t nf = r.cplx ? 2 : 1;
@@ -319,10 +339,20 @@ int solve_mesh(fcg_ctx* ctx, const Options& o, double tol, Result& r) {
         rc = fcg_solve(ctx, job, &so, r.cap.data(), &r.rows, &r.cols, o.out_charge ? r.charges.data() : nullptr,
                        &r.ss);
     }
+    if (rc == FCG_OK) rc = fcg_job_memory(ctx, job, &r.mem);
+    // one GMRES line per right-hand side (gmresPrecondSFastAll,
+    // Solver/SolveCapacitance.cpp:2359-2514): the step numbers, with -r each
+    // preceded by the residual before it and the final residual at the end
     if (rc == FCG_OK)
         for (int q = 0; q < r.rows && q < 64; ++q) {
+            const int n = r.ss.iterations[q];
+            const double* h = hist.empty() ? nullptr : hist.data() + (size_t)q * kHistCap;
             std::printf("GMRES Iteration: ");
-            for (int i = 0; i < r.ss.iterations[q]; ++i) std::printf("%d ", i);
+            for (int i = 0; i < n; ++i) {
+                if (h && i < kHistCap) std::printf("%.3g ", h[i]);
+                std::printf("%d ", i);
+            }
+            if (h) std::printf("%.3f ", h[std::min(n, kHistCap - 1)]);
             std::printf("\n");
         }
     if (rc != FCG_OK) std::fprintf(stderr, "Error: %s\n", fcg_last_error(ctx));
@@ -427,7 +457,7 @@ void print_cap(const Options& o, const Result& r) {
 }
 
 // CSolveCap::OutputSolveStats (Solver/SolveCapacitance.cpp:684-721)
-void print_stats(const Result& r) {
+void print_stats(const Options& o, const Result& r) {
     const long long N = r.hs.panels, L = r.hs.links + r.hs.panels;
     std::printf("\nSolve statistics:\n");
     std::printf("Number of input panels: %lld of which %lld conductors and %lld dielectric\n",
@@ -440,6 +470,22 @@ void print_stats(const Result& r) {
                 100.0f - ((float)L) / ((float)(N * N)) * 100.0);
     std::printf("Max recursion level: %d\n", r.hs.max_level);
     std::printf("Max Mesh relative refinement value: %g\n", r.hs.max_mesh_eps);
+    if (!o.dump_time) return;
+    // -i (Solver/SolveCapacitance.cpp:704-720): the phases of this path and the
+    // device memory of the job and its solve
+    std::printf("Time for reading input file: %fs\n", r.hs.t_read_s);
+    std::printf("Time for building super hierarchy: %fs\n", r.hs.t_superh_s);
+    std::printf("Time for discretization: %fs\n", r.hs.t_discretize_s);
+    std::printf("Time for building potential matrix: %fs\n", r.hs.t_links_s + r.t_fill_s);
+    std::printf("Time for precond calculation: %fs\n", r.ss.t_precond_s);
+    std::printf("Time for gmres solving: %fs\n", r.ss.t_solve_s);
+    std::printf("Memory allocated for panel hierarchy: %lld bytes\n", (long long)r.mem.hierarchy);
+    std::printf("Memory allocated for links structure: %lld bytes\n", (long long)r.mem.links);
+    std::printf("Memory allocated for conductor list: %lld bytes\n", (long long)r.mem.conductors);
+    std::printf("Memory allocated for Gmres: %lld bytes\n", (long long)r.ss.mem_gmres);
+    std::printf("Memory allocated for preconditioner: %lld bytes\n", (long long)r.mem.precond);
+    std::printf("Memory allocated for charge vectors: %lld bytes\n", (long long)r.ss.mem_vectors);
+    std::printf("Memory allocated for hierarchical multiplication: %lld bytes\n", (long long)r.ss.mem_product);
 }
 
 // CSolveCap::PrintTime (Solver/SolveCapacitance.cpp)
@@ -503,7 +549,7 @@ int main(int argc, char** argv) {
         }
         if (rc == FCG_OK) {
             print_cap(o, r);
-            print_stats(r);
+            print_stats(o, r);
         }
     } else {
         // -a run (Solver/SolveCapacitance.cpp:294-477)
@@ -522,7 +568,7 @@ int main(int argc, char** argv) {
         }
         if (rc == FCG_OK) {
             print_cap(o, r);
-            print_stats(r);
+            print_stats(o, r);
             std::printf("Iteration time: ");
             print_time(since(ti));
         }
@@ -555,7 +601,7 @@ int main(int argc, char** argv) {
             const double rel = frob(diff) / frob(nr.cap);
             print_cap(o, nr);
             std::printf("Weighted Frobenius norm of the difference between capacitance (auto option): %g\n", rel);
-            print_stats(nr);
+            print_stats(o, nr);
             std::printf("Iteration time: ");
             print_time(since(ti));
             old_panels = nr.hs.panels;

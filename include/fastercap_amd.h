@@ -123,6 +123,15 @@ typedef struct fcg_solve_opts {
     int32_t max_refine;/* outer FP64 refinement sweeps in FCG_PREC_MIXED */
     int32_t precond_size; /* FCG_PRECOND_BLOCK: most leaves per block (-pb<n>; 0 = 32, clamped to 2..128);
                              FCG_PRECOND_SUPER: dimension (-ps<n>; 0 = 128, clamped to 16..4096) */
+    /* Optional residual history of fcg_solve, the values the reference prints with -r
+     * (Solver/SolveCapacitance.cpp:2374-2376, 2511-2513): residuals[q * residuals_cap + i]
+     * for right-hand side q holds, at i = 0, the norm of the initial residual |r0| and, at
+     * i >= 1, the relative residual estimate |g_i| / |b| after Arnoldi step i (the last of
+     * them the final one); at most residuals_cap values per right-hand side, the count is
+     * stats->iterations[q] + 1.  Mixed precision: the inner cycles' estimates in order.
+     * NULL: not recorded (fcg_solve_batch and the engine ignore it). */
+    double* residuals;
+    int32_t residuals_cap;
 } fcg_solve_opts;
 
 typedef struct fcg_solve_stats {
@@ -136,6 +145,13 @@ typedef struct fcg_solve_stats {
                                estimate; mixed: the FP64 residual of the last refinement
                                check); 0: the iteration cap or max_refine ended the solve */
     int32_t cycles;         /* inner GMRES cycles (mixed: refinement cycles) */
+    /* phase time and device memory of the solve, the figures the reference prints
+     * with -i (Solver/SolveCapacitance.cpp:704-720) */
+    double t_precond_s;     /* preconditioner set-up (fcg_solve builds it before the
+                               iterations; 0 in batched solves, which build it in a step) */
+    int64_t mem_gmres;      /* Krylov basis and Arnoldi records */
+    int64_t mem_vectors;    /* solution, residual and work vectors */
+    int64_t mem_product;    /* product scratch: node charges and potentials */
 } fcg_solve_stats;
 
 typedef struct fcg_ctx fcg_ctx;
@@ -144,6 +160,15 @@ typedef struct fcg_ctx fcg_ctx;
  * state lives in CSolveCap / CMultHier objects, Solver/SolveCapacitance.h:53-191). */
 int fcg_ctx_create(int device, fcg_ctx** out);
 int fcg_ctx_destroy(fcg_ctx* ctx);
+
+/* Device memory a resident job holds, by the categories the reference prints with -i
+ * (Solver/SolveCapacitance.cpp:713-719): the panel hierarchy (tree, geometry, tree-pass
+ * tables), the link structure (rows, sources, coefficients, fill and gather tables), the
+ * conductor list (leaf ranges, charge weights) and the preconditioner arrays. */
+typedef struct fcg_job_mem {
+    int64_t hierarchy, links, conductors, precond;
+} fcg_job_mem;
+int fcg_job_memory(fcg_ctx* ctx, int job, fcg_job_mem* out);
 
 /* Upload a job's hierarchy and link structure into HBM.
  * Replaces the in-memory link chunk allocation of CAutoRefine::AutoRefineLinks

@@ -75,3 +75,37 @@ def test_cli_precond_values():
     assert cli.parse_args(["-ps256", "-pj", "x.txt"]).precond == _ffi.FCG_PRECOND_SUPER
     assert cli.auto_preconditioner(10_000_000, 4)[0] == _ffi.FCG_PRECOND_SUPER
     assert cli.auto_preconditioner(1000, 1)[0] == _ffi.FCG_PRECOND_JACOBI
+
+
+def test_struct_layouts_match_header(tmp_path):
+    """Every ctypes mirror of a header struct has the C compiler's size and field
+    offsets (a C program built from include/fastercap_amd.h prints them)."""
+    structs = {
+        "fcg_solve_opts": _ffi.SolveOpts,
+        "fcg_solve_stats": _ffi.SolveStats,
+        "fcg_job_mem": _ffi.JobMem,
+        "fcg_mvp_timing": _ffi.MvpTiming,
+        "fcg_profile": _ffi.Profile,
+        "fcg_node_geom": _ffi.NodeGeom,
+        "fcg_job_desc": _ffi.JobDesc,
+    }
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "fastercap_amd.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in py._fields_:
+            lines.append(f'printf("{cname} {fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines += ["return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    import subprocess
+
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.splitlines():
+        c, f, v = ln.split()
+        got[(c, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for fname, _ in py._fields_:
+            assert got[(cname, fname)] == getattr(py, fname).offset, (cname, fname)

@@ -146,3 +146,92 @@ def test_file_outputs_csv_geometry_charges(geo_dir, python_cli):
                 f = ln.split()
                 total += float(f[-1]) * _tri_area([float(x) for x in f[2:11]])
             assert abs(total - cap[i, j]) <= 1e-5 * abs(cap[i, i]), (name, other, total, cap[i, j])
+
+
+def _run_cli(args, python_cli):
+    if python_cli:
+        buf = io.StringIO()
+        cli.run(cli.parse_args(args), out=buf)
+        return buf.getvalue()
+    out = subprocess.run([BIN] + args, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+@pytest.mark.parametrize("python_cli", [False, True])
+def test_residual_and_time_dump(geo_dir, python_cli):
+    """-r (SolveCapacitance.cpp:2374-2376, 2511-2513): each GMRES step number is
+    preceded by the residual before it (|r0| first, then |g_i| / |b|, which
+    cannot grow), and the final residual ends the line; -i (:704-720): the phase
+    times and the device memory by category follow the statistics.  Neither
+    changes the matrix or the step count."""
+    d = os.path.join(geo_dir, f"cli_dump_{int(python_cli)}")
+    os.makedirs(d, exist_ok=True)
+    path = two_cubes(d)
+    args = [path, "-m0.05", "-t1e-6"]
+    print(f"cli -r -i python={python_cli} {path}", flush=True)
+    plain = _run_cli(args, python_cli)
+    dump = _run_cli(args + ["-r", "-i"], python_cli)
+    np.testing.assert_allclose(_last_matrix(dump)[1], _last_matrix(plain)[1], rtol=1e-12)
+    pl = [ln.split()[2:] for ln in plain.splitlines() if ln.startswith("GMRES Iteration:")]
+    dl = [ln.split()[2:] for ln in dump.splitlines() if ln.startswith("GMRES Iteration:")]
+    assert len(pl) == len(dl) == 2
+    for p, t in zip(pl, dl):
+        n = len(p)
+        assert n > 3 and len(t) == 2 * n + 1
+        assert [int(x) for x in t[1:2 * n:2]] == list(range(n))
+        res = [float(x) for x in t[0:2 * n:2]]
+        assert res[0] > 0.0
+        rel = np.array(res[1:])  # printed to 3 significant digits
+        assert np.all(rel[1:] <= rel[:-1] * (1 + 5e-3))
+        assert 0.0 <= float(t[-1]) < 5e-4  # "%.3f" of a residual below the tolerance
+    lines = dump.splitlines()
+    k = max(i for i, ln in enumerate(lines) if ln.startswith("Max Mesh relative refinement value"))
+    block = lines[k + 1:k + 14]
+    keys = ["Time for reading input file", "Time for building super hierarchy", "Time for discretization",
+            "Time for building potential matrix", "Time for precond calculation", "Time for gmres solving",
+            "Memory allocated for panel hierarchy", "Memory allocated for links structure",
+            "Memory allocated for conductor list", "Memory allocated for Gmres",
+            "Memory allocated for preconditioner", "Memory allocated for charge vectors",
+            "Memory allocated for hierarchical multiplication"]
+    assert [ln.split(":")[0] for ln in block] == keys
+    vals = {ln.split(":")[0]: float(ln.split(": ")[1].split()[0].rstrip("s")) for ln in block}
+    assert all(vals[k] >= 0.0 for k in keys[:6])
+    assert vals["Time for gmres solving"] > 0.0
+    for k in keys[6:]:
+        if k != "Memory allocated for preconditioner":  # Jacobi: the self terms only
+            assert vals[k] > 0, k
+    assert "Time for reading input file" not in plain
+
+
+def test_residual_history_matches_oracle(geo_dir):
+    """The first value of the residual history is |b - A x0| with x0 = P b (Jacobi),
+    as gmresPrecondSFastAll computes it before the loop (SolveCapacitance.cpp:
+    2302-2323), checked against the oracle's product; the rest are the GMRES
+    estimates |g_i| / |b|: non-increasing, one per step, the last below tol."""
+    import fastercap_amd as fc
+    import oracle_lib as orc
+
+    path = workloads.two_plates(os.path.join(geo_dir, "resid_hist"))
+    mesh = fc.Mesh.build(path, mesh_eps=0.05)
+    dev = fc.Device(0)
+    try:
+        job = dev.upload(mesh)
+        dev.fill(job)
+        print(f"residual history: plates N={mesh.N}", flush=True)
+        res = dev.solve(job, mesh, tol=1e-8, precond=fc.FCG_PRECOND_JACOBI, want_residuals=True)
+        ocoef, oself = orc.fill(mesh)
+        conds = [g for g in range(mesh.G) if not mesh.group_is_diel[g]]
+        assert len(res.residuals) == len(conds)
+        for q, g in enumerate(conds):
+            b = np.zeros(mesh.N)
+            b[mesh.group_leaf_start[g]:mesh.group_leaf_start[g + 1]] = 1.0
+            x0 = b / oself
+            r0 = b - orc.mvp(mesh, ocoef, oself, x0)
+            h = res.residuals[q]
+            assert len(h) == res.iterations[q] + 1
+            assert abs(h[0] / np.linalg.norm(r0) - 1.0) < 1e-10
+            assert np.all(np.diff(h[1:]) <= 1e-14 * h[1])
+            assert h[-1] < 1e-8 and h[-2] >= 1e-8
+    finally:
+        dev.close()
