@@ -32,6 +32,8 @@ from ._ffi import (
     FCG_PRECOND_JACOBI,
     FCG_PRECOND_NONE,
     FCG_PRECOND_SUPER,
+    FCG_SCHEME_COLLOCATION,
+    FCG_SCHEME_GALERKIN,
     HostOptions,
     HostStats,
     JobDesc,
@@ -58,6 +60,8 @@ __all__ = [
     "FCG_PRECOND_JACOBI",
     "FCG_PRECOND_NONE",
     "FCG_PRECOND_SUPER",
+    "FCG_SCHEME_COLLOCATION",
+    "FCG_SCHEME_GALERKIN",
     "TWO_PI_TIMES_E0",
 ]
 
@@ -160,6 +164,16 @@ class Mesh:
         )
         assert dt.itemsize == 96
         return self.nodes_raw.view(dt)
+
+    @property
+    def scheme(self) -> int:
+        """Discretization scheme of the job this mesh uploads as (fcg_job_desc.scheme):
+        FCG_SCHEME_COLLOCATION (also 0) or FCG_SCHEME_GALERKIN (3D; -g)."""
+        return int(self.desc.scheme)
+
+    @scheme.setter
+    def scheme(self, value: int):
+        self.desc.scheme = int(value)
 
     @property
     def conductor_names(self):

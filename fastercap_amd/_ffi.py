@@ -50,6 +50,7 @@ class JobDesc(C.Structure):
         ("group_outperm_im", C.POINTER(C.c_double)),  # NULL for real permittivities
         ("group_inperm_im", C.POINTER(C.c_double)),
         ("leaf_outperm_im", C.POINTER(C.c_double)),
+        ("scheme", C.c_int32),
     ]
 
 
@@ -203,6 +204,8 @@ FCG_PRECOND_JACOBI = 1
 FCG_PRECOND_BLOCK = 2
 FCG_PRECOND_SUPER = 4
 FCG_PRECOND_HIER = 8
+FCG_SCHEME_COLLOCATION = 1
+FCG_SCHEME_GALERKIN = 2
 FCG_MVP_FP64 = 0
 FCG_MVP_FP32 = 1
 FCG_MVP_FP32_ACC64 = 2

@@ -21,6 +21,11 @@
 extern "C" {
 #endif
 
+/* Discretization schemes: the reference's AUTOREFINE_COLLOCATION / AUTOREFINE_GALERKIN
+ * values (Solver/SolverGlobal.h:72-73). */
+#define FCG_SCHEME_COLLOCATION 1
+#define FCG_SCHEME_GALERKIN 2
+
 /* Return codes: identical values to FasterCap's FC_* (Solver/SolverGlobal.h:39-49). */
 #define FCG_OK 0
 #define FCG_GENERIC_ERROR 1
@@ -113,6 +118,12 @@ typedef struct fcg_job_desc {
     const double* group_outperm_im;
     const double* group_inperm_im;
     const double* leaf_outperm_im;
+    /* Discretization scheme (-g; m_cScheme, Solver/SolverGlobal.h:72-73): FCG_SCHEME_GALERKIN
+     * takes the Galerkin branches of PotEstimateOpt (Solver/Autorefine.cpp:3570-3674) and
+     * SelfPotential (:2399-2401) for the fill, the preconditioners' blocks and the self terms;
+     * 0 or FCG_SCHEME_COLLOCATION collocation.  3D only: the reference solves 2D jobs by
+     * collocation (Solver/SolveCapacitance.cpp:159-161), and so does this library. */
+    int32_t scheme;
 } fcg_job_desc;
 
 typedef struct fcg_solve_opts {

@@ -20,6 +20,7 @@ namespace {
 const double kFourPiE0 = 1.1126500559756e-10;  // Potential.h:43
 const double kTwoE0 = 1.7708375634e-11;        // Potential.h:44
 const double kPi = 3.1415926535897932384626433832795;
+const double kPiE0 = 2.781625139939e-11;  // Potential.h:41
 const double kPiHalf = 1.5707963267948966192313216916398;
 const double kGeoEps = 1E-6;    // POTENTIAL_GEO_EPS
 const double kTolEps = 1E-12;   // POTENTIAL_TOL_EPS
@@ -38,10 +39,12 @@ void vcross(const double* a, const double* b, double* o) {
 }
 double vnorm(const double* a) { return std::sqrt(vdot(a, a)); }
 
-// Quadrature tables: InitNumerical rules 2, 3, 5 (Potential.cpp:112-128, 133-149, 181-206).
+// Quadrature tables: InitNumerical rules 2, 3, 5 (Potential.cpp:112-128, 133-149, 181-206)
+// and the Galerkin scheme's rules 10 (Strang-Fix #8, 9 points, :345-389) and 17 (19
+// points, precision 9, :717-800).
 struct Rule {
     int n;
-    double px[6], py[6], w[6];
+    double px[19], py[19], w[19];
 };
 Rule make_rule(int id) {
     Rule r;
@@ -58,7 +61,7 @@ Rule make_rule(int id) {
         r.px[1] = 0.5; r.py[1] = 0.0;
         r.px[2] = 0.5; r.py[2] = 0.5;
         for (int i = 0; i < 3; ++i) r.w[i] = 1.0 / 3.0;
-    } else {
+    } else if (id == 5) {
         const double a = 0.659027622374092, b = 0.231933368553031, c = 0.109039009072877;
         const double xs[6] = {a, a, b, b, c, c}, ys[6] = {b, c, a, c, a, b};
         r.n = 6;
@@ -67,11 +70,47 @@ Rule make_rule(int id) {
             r.py[i] = ys[i];
             r.w[i] = 1.0 / 6.0;
         }
+    } else if (id == 10) {
+        const double a = 0.124949503233232, b = 0.437525248383384, c = 0.797112651860071,
+                     d = 0.165409927389841, e = 0.037477420750088;
+        const double u = 0.205950504760887, v = 0.063691414286223;
+        const double xs[9] = {a, b, b, c, c, d, d, e, e}, ys[9] = {b, a, b, d, e, c, e, c, d};
+        r.n = 9;
+        for (int i = 0; i < 9; ++i) {
+            r.px[i] = xs[i];
+            r.py[i] = ys[i];
+            r.w[i] = i < 3 ? u : v;
+        }
+    } else {  // 17
+        const double a = 1.0 / 3.0, b = 0.02063496160252593, c = 0.4896825191987370, d = 0.1258208170141290,
+                     e = 0.4370895914929355, f = 0.6235929287619356, g = 0.1882035356190322,
+                     rr = 0.9105409732110941, ss = 0.04472951339445297, t = 0.7411985987844980,
+                     u = 0.03683841205473626, v = 0.22196288916076574;
+        const double w1 = 0.09713579628279610, w2 = 0.03133470022713983, w3 = 0.07782754100477543,
+                     w4 = 0.07964773892720910, w5 = 0.02557767565869810, w6 = 0.04328353937728940;
+        const double xs[19] = {a, b, c, c, d, e, e, f, g, g, rr, ss, ss, t, t, u, u, v, v};
+        const double ys[19] = {a, c, b, c, e, d, e, g, f, g, ss, rr, ss, u, v, t, v, t, u};
+        const double ws[19] = {w1, w2, w2, w2, w3, w3, w3, w4, w4, w4, w5, w5, w5, w6, w6, w6, w6, w6, w6};
+        r.n = 19;
+        for (int i = 0; i < 19; ++i) {
+            r.px[i] = xs[i];
+            r.py[i] = ys[i];
+            r.w[i] = ws[i];
+        }
     }
     return r;
 }
-const Rule kRule2 = make_rule(2), kRule3 = make_rule(3), kRule5 = make_rule(5);
-const Rule& rule_of(int id) { return id == 2 ? kRule2 : (id == 3 ? kRule3 : kRule5); }
+const Rule kRule2 = make_rule(2), kRule3 = make_rule(3), kRule5 = make_rule(5), kRule10 = make_rule(10),
+           kRule17 = make_rule(17);
+const Rule& rule_of(int id) {
+    switch (id) {
+        case 2: return kRule2;
+        case 3: return kRule3;
+        case 10: return kRule10;
+        case 17: return kRule17;
+        default: return kRule5;
+    }
+}
 
 // Quadrature of a kernel over the triangle in the reference's local frame.
 template <class F>
@@ -187,6 +226,159 @@ double orc_enfield_num_3d(const double* r, const float* v, const double* n, int 
     });
 }
 
+// ---------------------------------------------------------------- Galerkin (3D)
+// CPotential::Auto (Potential.cpp:994-1017): the self coefficient of a uniformly
+// charged triangle (its potential averaged over itself), closed form in the side
+// lengths and the semiperimeter.
+double orc_auto_3d(const float* v) {
+    double V[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) V[i][k] = v[3 * i + k];
+    double s1[3], s2[3], s3[3];
+    vsub(V[1], V[0], s1);
+    vsub(V[2], V[1], s2);
+    vsub(V[0], V[2], s3);
+    const double a = vnorm(s1), b = vnorm(s2), c = vnorm(s3);
+    const double ph = (a + b + c) / 2;
+    return -((1 / a) * std::log(1 - a / ph) + (1 / b) * std::log(1 - b / ph) + (1 / c) * std::log(1 - c / ph)) /
+           (3 * kPiE0);
+}
+
+// CPotential::MutualHalfNumerical (Potential.cpp:2485-2625): the mean over triangle
+// 1 (rule `rule`, the local frame) of triangle 2's analytic potential.
+double orc_mutual_half_3d(const float* v1, const float* v2, int rule) {
+    return tri_quadrature(v1, rule, [&](const double* p) { return orc_potential_opt_3d(p, v2); }) / kFourPiE0;
+}
+
+// CPotential::MutualDHalfNumerical (Potential.cpp:2802-2964): the same mean of the
+// divided difference of triangle 2's potential along n (step h).
+double orc_mutual_dhalf_3d(const float* v1, const float* v2, const double* n, double h, int rule) {
+    return tri_quadrature(v1, rule,
+                          [&](const double* p) {
+                              double pp[3], pm[3];
+                              for (int k = 0; k < 3; ++k) {
+                                  pp[k] = p[k] + n[k] * h;
+                                  pm[k] = p[k] - n[k] * h;
+                              }
+                              return (orc_potential_opt_3d(pm, v2) - orc_potential_opt_3d(pp, v2)) / (2.0 * h);
+                          }) /
+           kFourPiE0;
+}
+
+extern "C++" {
+namespace {
+// Rule points on a triangle as the double-quadrature routines map them
+// (Potential.cpp:3179-3260, the tNtransform form): the local frame, the 2x3
+// homogeneous transform and its Jacobian; nmod = |side1 x side2|.
+void full_points(const float* v, int n, const double* px, const double* py, double (*out)[3], double* jac,
+                 double* nmod) {
+    double V[3][3];
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) V[i][k] = v[3 * i + k];
+    double s1[3], s2[3], s3[3], nrm[3], z[3], x[3], y[3];
+    vsub(V[1], V[0], s1);
+    vsub(V[2], V[1], s2);
+    vsub(V[0], V[2], s3);
+    vcross(s1, s2, nrm);
+    const double nm = vnorm(nrm);
+    for (int k = 0; k < 3; ++k) z[k] = nrm[k] / nm;
+    const double m1 = vnorm(s1);
+    for (int k = 0; k < 3; ++k) x[k] = s1[k] / m1;
+    vcross(x, z, y);
+    const double l1x = m1, l2x = -vdot(s3, x), l2y = -vdot(s3, y);
+    const double t0x = l1x - l2x, t0y = -l2x, t0z = l2x;
+    const double t1x = -l2y, t1y = -l2y, t1z = l2y;
+    *jac = t0x * t1y - t1x * t0y;
+    *nmod = nm;
+    for (int i = 0; i < n; ++i) {
+        const double qx = t0x * px[i] + t0y * py[i] + t0z * 1.0;
+        const double qy = t1x * px[i] + t1y * py[i] + t1z * 1.0;
+        for (int k = 0; k < 3; ++k) out[i][k] = V[0][k] + x[k] * qx + y[k] * qy;
+    }
+}
+
+// the double rules of Mutual_2thOrd_FullNum / MutualD_2thOrd_FullNum (3 points per
+// triangle, :3147-3152) and Mutual_4thOrd_FullNum (6 points, :3799-3812): points
+// and the weights of each point pair, i-major
+const double kFull2P[3][2] = {{0.66666666666666666666666666666667, 0.16666666666666666666666666666667},
+                              {0.16666666666666666666666666666667, 0.66666666666666666666666666666667},
+                              {0.16666666666666666666666666666667, 0.16666666666666666666666666666667}};
+const double kFull2W = 0.11111111111111111111111111111111;
+const double kFull4P[6][2] = {{0.816847572980459, 0.091576213509771}, {0.091576213509771, 0.816847572980459},
+                              {0.091576213509771, 0.091576213509771}, {0.108103018168070, 0.445948490915965},
+                              {0.445948490915965, 0.108103018168070}, {0.445948490915965, 0.445948490915965}};
+const double kFull4Waa = 0.012089385932845641681938923684, kFull4Wab = 0.024561195285594988334146524542,
+             kFull4Wbb = 0.049899334607075270538656916121;
+double full4_weight(int i, int j) {
+    const bool a = i < 3, b = j < 3;
+    return a && b ? kFull4Waa : (a || b ? kFull4Wab : kFull4Wbb);
+}
+
+// sum over the point pairs of w(i, j) K(p_i - q_j), the Jacobians and the normal
+// moduli applied as the reference applies them
+template <class W, class K>
+double full_pairs(const float* v1, const float* v2, int n, const double* px, const double* py, W weight, K kernel) {
+    double p1[19][3], p2[19][3], j1, j2, n1, n2;
+    full_points(v1, n, px, py, p1, &j1, &n1);
+    full_points(v2, n, px, py, p2, &j2, &n2);
+    double result = 0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            double dd[3];
+            vsub(p1[i], p2[j], dd);
+            result = result + weight(i, j) * kernel(dd);
+        }
+    result = result * j1 * j2;
+    result = result / (n1 * n2);
+    return result / kFourPiE0;
+}
+}  // namespace
+}  // extern "C++"
+
+// Mutual_2thOrd_FullNum (Potential.cpp:3138-3347) and Mutual_4thOrd_FullNum
+// (:3790-4003): the double mean of 1/r by a product rule (order 2: 3 x 3 points,
+// order 4: 6 x 6 points).
+double orc_mutual_full_3d(const float* v1, const float* v2, int order) {
+    double px[6] = {}, py[6] = {};
+    const int n = order == 4 ? 6 : 3;
+    for (int i = 0; i < n; ++i) {
+        px[i] = order == 4 ? kFull4P[i][0] : kFull2P[i][0];
+        py[i] = order == 4 ? kFull4P[i][1] : kFull2P[i][1];
+    }
+    return full_pairs(
+        v1, v2, n, px, py, [&](int i, int j) { return order == 4 ? full4_weight(i, j) : kFull2W; },
+        [](const double* dd) { return 1 / vnorm(dd); });
+}
+
+// MutualDFullNumerical (Potential.cpp:4197-4377, rule >= 1) and
+// MutualD_2thOrd_FullNum (:3349-3577, rule 0): the double mean of (r1 - r2).n /
+// |r1 - r2|^3, with the product of the rule's weights (or the 3 x 3 table).
+double orc_mutual_dfull_3d(const float* v1, const float* v2, const double* n, int rule) {
+    double px[19] = {}, py[19] = {}, w[19] = {};
+    int m;
+    if (rule == 0) {
+        m = 3;
+        for (int i = 0; i < 3; ++i) {
+            px[i] = kFull2P[i][0];
+            py[i] = kFull2P[i][1];
+        }
+    } else {
+        const Rule& R = rule_of(rule);
+        m = R.n;
+        for (int i = 0; i < m; ++i) {
+            px[i] = R.px[i];
+            py[i] = R.py[i];
+            w[i] = R.w[i];
+        }
+    }
+    return full_pairs(
+        v1, v2, m, px, py, [&](int i, int j) { return rule == 0 ? kFull2W : w[i] * w[j]; },
+        [&](const double* dd) {
+            const double rd = vnorm(dd);
+            return vdot(dd, n) / (rd * rd * rd);
+        });
+}
+
 // Local segment frame shared by the 2D routines (Potential.cpp:1866-1899).
 static void seg_frame(const double* r, const float* v, double* a, double* xo, double* yo, double* len, double* ex,
                       double* ey) {
@@ -264,6 +456,20 @@ double orc_pot_estimate(const fcg_job_desc* d, int i, int j) {
     const double rd = vnorm(dist);
     const double rmax = p1.maxside > p2.maxside ? p1.maxside : p2.maxside;
     const double ratio = rd / rmax;
+    if (d->scheme == FCG_SCHEME_GALERKIN) {  // the Galerkin branches (Autorefine.cpp:3570-3674)
+        if (!diel1) {
+            if (ratio < 1.0) return orc_mutual_half_3d(p1.v, p2.v, 17);
+            if (ratio < 2.0) return orc_mutual_half_3d(p1.v, p2.v, 10);
+            if (ratio < 4.0) return orc_mutual_full_3d(p1.v, p2.v, 4);
+            if (ratio < 10.0) return orc_mutual_full_3d(p1.v, p2.v, 2);
+            return 1 / (kFourPiE0 * rd);
+        }
+        if (ratio < 1.0) return orc_mutual_dhalf_3d(p1.v, p2.v, p1.dn, rd / 20, 17);
+        if (ratio < 2.0) return orc_mutual_dfull_3d(p1.v, p2.v, p1.dn, 10);
+        if (ratio < 4.0) return orc_mutual_dfull_3d(p1.v, p2.v, p1.dn, 5);
+        if (ratio < 10.0) return orc_mutual_dfull_3d(p1.v, p2.v, p1.dn, 0);
+        return vdot(dist, p1.dn) / (kFourPiE0 * rd * rd * rd);
+    }
     if (!diel1) {
         if (ratio < 1.0) return orc_potential_opt_3d(c1, p2.v) / kFourPiE0;
         if (ratio < 2.0) return orc_potential_num_3d(c1, p2.v, 5) / kFourPiE0;
@@ -300,6 +506,7 @@ double orc_node_self(const fcg_job_desc* d, int node) {
     }
     double c[3];
     centroid(g, c);
+    if (d->dim == 3 && d->scheme == FCG_SCHEME_GALERKIN) return orc_auto_3d(g.v);  // Autorefine.cpp:2399-2401
     if (d->dim == 3) return orc_potential_opt_3d(c, g.v) / kFourPiE0;
     return orc_potential_opt_2d(c, g.v);
 }

@@ -12,7 +12,13 @@
  *   orc_enfield_num_3d     Solver/Potential.cpp:2172-2254
  *   orc_potential_opt_2d   Solver/Potential.cpp:1866-1949
  *   orc_enfield_2d         Solver/Potential.cpp:1967-2060
- *   orc_pot_estimate       CAutoRefine::PotEstimateOpt one-sided 3D/2D (Solver/Autorefine.cpp:3474-3733, 4156-4325)
+ *   orc_auto_3d            Solver/Potential.cpp:994-1017 (Galerkin self term)
+ *   orc_mutual_half_3d     Solver/Potential.cpp:2485-2625 (rules 10, 17 :345-389, 717-800)
+ *   orc_mutual_dhalf_3d    Solver/Potential.cpp:2802-2964
+ *   orc_mutual_full_3d     Solver/Potential.cpp:3138-3347, 3790-4003
+ *   orc_mutual_dfull_3d    Solver/Potential.cpp:4197-4377, 3349-3577
+ *   orc_pot_estimate       CAutoRefine::PotEstimateOpt one-sided 3D/2D, collocation and Galerkin
+ *                          (Solver/Autorefine.cpp:3474-3733, 4156-4325)
  *   orc_self               CAutoRefine::SelfPotential (Solver/Autorefine.cpp:2358-2506)
  *   orc_fill               fill loop of AutoRefineLinks (Solver/Autorefine.cpp:760-840), OpenMP over nodes
  *   orc_mvp                CMultHier::MultiplyMatByVec_fast (Solver/MultiplyHierarchical.cpp:72-330)
@@ -57,6 +63,15 @@ double orc_potential_num_3d(const double* r, const float* v, int rule);
 double orc_enfield_num_3d(const double* r, const float* v, const double* n, int rule);
 double orc_potential_opt_2d(const double* r, const float* v);
 double orc_enfield_2d(const double* r, const float* v, const double* n);
+
+/* Galerkin scheme (3D): Auto, MutualHalfNumerical, MutualDHalfNumerical, the product-rule
+ * double quadratures (order 2: Mutual_2thOrd_FullNum, 4: Mutual_4thOrd_FullNum) and
+ * MutualDFullNumerical (rule 0: MutualD_2thOrd_FullNum), all divided by 4 pi e0 */
+double orc_auto_3d(const float* v);
+double orc_mutual_half_3d(const float* v1, const float* v2, int rule);
+double orc_mutual_dhalf_3d(const float* v1, const float* v2, const double* n, double h, int rule);
+double orc_mutual_full_3d(const float* v1, const float* v2, int order);
+double orc_mutual_dfull_3d(const float* v1, const float* v2, const double* n, int rule);
 
 double orc_pot_estimate(const fcg_job_desc* d, int node_i, int node_j);
 double orc_self(const fcg_job_desc* d, int leaf);

@@ -36,6 +36,14 @@ def lib():
         for f in ("orc_potential_opt_3d", "orc_potential_num_3d", "orc_enfield_num_3d", "orc_potential_opt_2d",
                   "orc_enfield_2d", "orc_pot_estimate", "orc_self"):
             getattr(L, f).restype = C.c_double
+        L.orc_auto_3d.argtypes = [fp]
+        L.orc_mutual_half_3d.argtypes = [fp, fp, C.c_int]
+        L.orc_mutual_dhalf_3d.argtypes = [fp, fp, dp, C.c_double, C.c_int]
+        L.orc_mutual_full_3d.argtypes = [fp, fp, C.c_int]
+        L.orc_mutual_dfull_3d.argtypes = [fp, fp, dp, C.c_int]
+        for f in ("orc_auto_3d", "orc_mutual_half_3d", "orc_mutual_dhalf_3d", "orc_mutual_full_3d",
+                  "orc_mutual_dfull_3d"):
+            getattr(L, f).restype = C.c_double
         L.orc_pot_estimate.argtypes = [jd, C.c_int, C.c_int]
         L.orc_self.argtypes = [jd, C.c_int]
         L.orc_fill.argtypes = [jd, dp, dp, C.c_int]
@@ -108,6 +116,37 @@ def enfield_2d(r, v6, n):
     v = np.ascontiguousarray(v6, dtype=np.float32)
     n = np.ascontiguousarray(n, dtype=np.float64)
     return lib().orc_enfield_2d(_d(r), _f(v), _d(n))
+
+
+def _v(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def auto_3d(v9):
+    v = _v(v9)
+    return lib().orc_auto_3d(_f(v))
+
+
+def mutual_half_3d(v1, v2, rule):
+    a, b = _v(v1), _v(v2)
+    return lib().orc_mutual_half_3d(_f(a), _f(b), rule)
+
+
+def mutual_dhalf_3d(v1, v2, n, h, rule):
+    a, b = _v(v1), _v(v2)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    return lib().orc_mutual_dhalf_3d(_f(a), _f(b), _d(n), h, rule)
+
+
+def mutual_full_3d(v1, v2, order):
+    a, b = _v(v1), _v(v2)
+    return lib().orc_mutual_full_3d(_f(a), _f(b), order)
+
+
+def mutual_dfull_3d(v1, v2, n, rule):
+    a, b = _v(v1), _v(v2)
+    n = np.ascontiguousarray(n, dtype=np.float64)
+    return lib().orc_mutual_dfull_3d(_f(a), _f(b), _d(n), rule)
 
 
 def fill(mesh, nthreads=0):

@@ -42,7 +42,9 @@ def _defines(header):
 #   AUTOREFINE_PRECOND_NONE/JACOBI/BLOCK/SUPER/HIER  Solver/SolverGlobal.h:60-68
 #   FC_NORMAL_END .. FC_USER_BREAK                    Solver/SolverGlobal.h:39-49
 #   AUTOPANEL_IS_LEAF .. AUTOPANEL_IS_SUPER_NODE      Solver/AutoElement.h:44-48
+#   AUTOREFINE_COLLOCATION / AUTOREFINE_GALERKIN      Solver/SolverGlobal.h:72-73
 REFERENCE_VALUES = {
+    "FCG_SCHEME_COLLOCATION": 1, "FCG_SCHEME_GALERKIN": 2,
     "FCG_PRECOND_NONE": 0, "FCG_PRECOND_JACOBI": 1, "FCG_PRECOND_BLOCK": 2, "FCG_PRECOND_SUPER": 4,
     "FCG_PRECOND_HIER": 8,
     "FCG_OK": 0, "FCG_GENERIC_ERROR": 1, "FCG_COMMAND_LINE_ERROR": 64, "FCG_CANNOT_OPEN_FILE": 66,
