@@ -48,6 +48,7 @@ class Args:
     out_charge: bool = False  # -c
     dump_resid: bool = False  # -r
     dump_time: bool = False  # -i
+    galerkin: bool = False  # -g (3D)
     stop: bool = False  # -b? / -bv
 
 
@@ -123,6 +124,8 @@ def parse_args(argv) -> Args:
             a.dump_time = True
         elif opt == "-gmixed":
             a.precision = 1
+        elif opt == "-g":
+            a.galerkin = True
         elif opt.startswith("-gdev"):
             a.device = int(_num(opt[5:], opt))
         elif opt.startswith("-mc"):
@@ -231,7 +234,8 @@ def format_params(a: Args, tol: float) -> str:
         lines.append("Output charge densities information (-c)")
     if a.out_csv:
         lines.append("Output capacitance matrix to file (-e)")
-    lines += [f"Solution scheme (-g): Collocation, GMRES tolerance (-t): {tol:g}",
+    scheme = "Galerkin" if a.galerkin else "Collocation"
+    lines += [f"Solution scheme (-g): {scheme}, GMRES tolerance (-t): {tol:g}",
               "Out-of-core free memory to link memory condition (-f): 5",
               f"Potential interaction coefficient to mesh refinement ratio (-d): {a.eps_ratio:g}",
               f"Mesh curvature (-mc): {a.curv_coeff:g}"]
@@ -264,6 +268,10 @@ def solve_file(dev, a: Args, mesh_eps: float, tol: float, out=None):
 
     mesh = Mesh.build(a.file, mesh_eps=mesh_eps, eps_ratio=a.eps_ratio, curv_coeff=a.curv_coeff,
                       device=None if a.host_links else dev, refine_on_device=not a.host_links)
+    if a.galerkin and mesh.dim == 3:
+        from . import FCG_SCHEME_GALERKIN
+
+        mesh.scheme = FCG_SCHEME_GALERKIN
     job = dev.upload(mesh)
     try:
         t0 = time.perf_counter()
@@ -325,6 +333,14 @@ def run(a: Args, out=sys.stdout):
     """Manual or -a run; returns (mesh, SolveResult) of the final iteration."""
     from . import Device
 
+    if a.galerkin:
+        # the Galerkin scheme is 3D only: 2D inputs are solved by collocation
+        # (Solver/SolveCapacitance.cpp:159-161)
+        from . import Mesh
+
+        if Mesh.build(a.file, mesh_eps=1e32).dim == 2:
+            print("Warning: Galerkin scheme yet not supported in 2D, moving to Collocation", file=sys.stderr)
+            a.galerkin = False
     dev = Device(a.device)
     try:
         if not a.auto:

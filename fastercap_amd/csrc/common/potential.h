@@ -1,6 +1,6 @@
-// Collocation potential / field integrals of uniformly charged triangles (3D) and
-// segments (2D), shared by the host mesh builder (refinement decisions) and the
-// gfx950 fill kernels (link coefficients).  Everything is a pure function of plain
+// Collocation and Galerkin potential / field integrals of uniformly charged
+// triangles (3D) and segments (2D), shared by the host mesh builder (refinement
+// decisions) and the gfx950 fill kernels (link coefficients).  Everything is a pure function of plain
 // scalars so the same source compiles for the host (g++) and the device (hipcc).
 //
 // Semantics follow FasterCap 6.0.7:
@@ -27,6 +27,7 @@
 #endif
 
 #define FCG_FOUR_PI_TIMES_E0 1.1126500559756e-10
+#define FCG_PI_TIMES_E0 2.781625139939e-11
 #define FCG_TWO_PI_TIMES_E0 5.563250279878e-11
 #define FCG_TWO_TIMES_E0 1.7708375634e-11
 #define FCG_PI 3.1415926535897932384626433832795
@@ -263,6 +264,202 @@ FCG_UNROLL
         result += (T)q.w[i] * dot3(d, tn) * (ir * ir * ir);
     }
     return result;
+}
+
+// ---------------------------------------------------------------- Galerkin (3D)
+// The Galerkin scheme's coefficients (-g): means over the destination triangle
+// instead of values at its centroid.
+//   auto_3d          Solver/Potential.cpp:994-1017   (CPotential::Auto)
+//   mutual_half_3d   Solver/Potential.cpp:2485-2625  (MutualHalfNumerical)
+//   mutual_dhalf_3d  Solver/Potential.cpp:2802-2964  (MutualDHalfNumerical)
+//   mutual_full_3d   Solver/Potential.cpp:3138-3347, 3790-4003 (Mutual_2thOrd/_4thOrd_FullNum)
+//   mutual_dfull_3d  Solver/Potential.cpp:4197-4377, 3349-3577 (MutualDFullNumerical,
+//                    MutualD_2thOrd_FullNum)
+//   rules 10 and 17  Solver/Potential.cpp:345-389, 717-800
+// All include the 1 / (4 pi e0) factor, as the reference's do.  As for the
+// collocation quadratures, rule point (px, py) is the barycentric point
+// py V0 + px V1 + (1 - px - py) V2 and the frame's Jacobian, twice the area,
+// cancels against the area division: each mean is a weighted sum over points.
+
+// Rules 10 (Strang-Fix #8, 9 points, precision 6) and 17 (19 points, precision 9).
+struct QuadRule19 {
+    int n;
+    double x[19], y[19], w[19];
+};
+
+FCG_HD QuadRule19 quad_rule19(int rule) {
+    QuadRule19 q{};
+    if (rule == 10) {
+        const double a = 0.124949503233232, b = 0.437525248383384, c = 0.797112651860071,
+                     d = 0.165409927389841, e = 0.037477420750088;
+        const double u = 0.205950504760887, v = 0.063691414286223;
+        const double xs[9] = {a, b, b, c, c, d, d, e, e}, ys[9] = {b, a, b, d, e, c, e, c, d};
+        q.n = 9;
+        for (int i = 0; i < 9; ++i) {
+            q.x[i] = xs[i];
+            q.y[i] = ys[i];
+            q.w[i] = i < 3 ? u : v;
+        }
+    } else {  // 17
+        const double a = 1.0 / 3.0, b = 0.02063496160252593, c = 0.4896825191987370, d = 0.1258208170141290,
+                     e = 0.4370895914929355, f = 0.6235929287619356, g = 0.1882035356190322,
+                     r = 0.9105409732110941, s = 0.04472951339445297, t = 0.7411985987844980,
+                     u = 0.03683841205473626, v = 0.22196288916076574;
+        const double w1 = 0.09713579628279610, w2 = 0.03133470022713983, w3 = 0.07782754100477543,
+                     w4 = 0.07964773892720910, w5 = 0.02557767565869810, w6 = 0.04328353937728940;
+        const double xs[19] = {a, b, c, c, d, e, e, f, g, g, r, s, s, t, t, u, u, v, v};
+        const double ys[19] = {a, c, b, c, e, d, e, g, f, g, s, r, s, u, v, t, v, t, u};
+        const double ws[19] = {w1, w2, w2, w2, w3, w3, w3, w4, w4, w4, w5, w5, w5, w6, w6, w6, w6, w6, w6};
+        q.n = 19;
+        for (int i = 0; i < 19; ++i) {
+            q.x[i] = xs[i];
+            q.y[i] = ys[i];
+            q.w[i] = ws[i];
+        }
+    }
+    return q;
+}
+
+// barycentric point (px, py) of triangle v
+template <typename T>
+FCG_HD Vec3<T> bary_point(const float* v, T px, T py) {
+    const T w0 = py, w1 = px, w2 = (T)1 - px - py;
+    const Vec3<T> a = vtx<T>(v, 0), b = vtx<T>(v, 1), c = vtx<T>(v, 2);
+    return Vec3<T>{w0 * a.x + w1 * b.x + w2 * c.x, w0 * a.y + w1 * b.y + w2 * c.y, w0 * a.z + w1 * b.z + w2 * c.z};
+}
+
+// the triangle's own potential averaged over it, closed form in the side lengths
+// a, b, c and the semiperimeter p: -(ln(1 - a/p)/a + ln(1 - b/p)/b + ln(1 - c/p)/c) / (3 pi e0)
+template <typename T>
+FCG_HD T auto_3d(const float* v) {
+    const Vec3<T> V0 = vtx<T>(v, 0), V1 = vtx<T>(v, 1), V2 = vtx<T>(v, 2);
+    const T a = norm3(sub3(V1, V0)), b = norm3(sub3(V2, V1)), c = norm3(sub3(V0, V2));
+    const T ph = (a + b + c) / (T)2;
+    return -(((T)1 / a) * log((T)1 - a / ph) + ((T)1 / b) * log((T)1 - b / ph) + ((T)1 / c) * log((T)1 - c / ph)) /
+           ((T)3 * (T)FCG_PI_TIMES_E0);
+}
+
+// mean over triangle v1 (rule RULE: 10 or 17) of the analytic potential of v2
+template <typename T, int RULE>
+FCG_HD T mutual_half_3d(const float* v1, const float* v2) {
+    const QuadRule19 q = quad_rule19(RULE);
+    T result = 0;
+    for (int i = 0; i < q.n; ++i)
+        result += (T)q.w[i] * potential_opt_3d<T>(bary_point<T>(v1, (T)q.x[i], (T)q.y[i]), v2);
+    return result / (T)FCG_FOUR_PI_TIMES_E0;
+}
+
+// the same mean of the divided difference of v2's potential along n (step h)
+template <typename T, int RULE>
+FCG_HD T mutual_dhalf_3d(const float* v1, const float* v2, Vec3<T> n, T h) {
+    const QuadRule19 q = quad_rule19(RULE);
+    T result = 0;
+    for (int i = 0; i < q.n; ++i) {
+        const Vec3<T> p = bary_point<T>(v1, (T)q.x[i], (T)q.y[i]);
+        const Vec3<T> pp{p.x + n.x * h, p.y + n.y * h, p.z + n.z * h};
+        const Vec3<T> pm{p.x - n.x * h, p.y - n.y * h, p.z - n.z * h};
+        result += (T)q.w[i] * ((potential_opt_3d<T>(pm, v2) - potential_opt_3d<T>(pp, v2)) / ((T)2 * h));
+    }
+    return result / (T)FCG_FOUR_PI_TIMES_E0;
+}
+
+// Points of the double (product) rules: the 3-point rule 2 and the 6-point
+// degree-4 rule (Strang-Fix), with the pair weights the reference tabulates:
+// 1/9 for order 2; for order 4 the products of the two point weights
+// 0.109951743655322 and 0.223381589678011.
+struct FullRule {
+    int n;
+    double x[6], y[6];
+};
+FCG_HD FullRule full_rule(int order) {
+    FullRule r{};
+    if (order == 4) {
+        const double a = 0.816847572980459, b = 0.091576213509771, c = 0.108103018168070, d = 0.445948490915965;
+        const double xs[6] = {a, b, b, c, d, d}, ys[6] = {b, a, b, d, c, d};
+        r.n = 6;
+        for (int i = 0; i < 6; ++i) {
+            r.x[i] = xs[i];
+            r.y[i] = ys[i];
+        }
+    } else {
+        const double a = 0.66666666666666666666666666666667, b = 0.16666666666666666666666666666667;
+        const double xs[3] = {a, b, b}, ys[3] = {b, a, b};
+        r.n = 3;
+        for (int i = 0; i < 3; ++i) {
+            r.x[i] = xs[i];
+            r.y[i] = ys[i];
+        }
+    }
+    return r;
+}
+FCG_HD double full_weight(int order, int i, int j) {
+    if (order != 4) return 0.11111111111111111111111111111111;
+    const bool a = i < 3, b = j < 3;
+    return a && b ? 0.012089385932845641681938923684
+                  : (a || b ? 0.024561195285594988334146524542 : 0.049899334607075270538656916121);
+}
+
+// double mean of 1/r over v1 and v2 by the product rule of order ORDER (2 or 4)
+template <typename T, int ORDER>
+FCG_HD T mutual_full_3d(const float* v1, const float* v2) {
+    const FullRule q = full_rule(ORDER);
+    Vec3<T> p2[6];
+FCG_UNROLL
+    for (int j = 0; j < q.n; ++j) p2[j] = bary_point<T>(v2, (T)q.x[j], (T)q.y[j]);
+    T result = 0;
+FCG_UNROLL
+    for (int i = 0; i < q.n; ++i) {
+        const Vec3<T> p1 = bary_point<T>(v1, (T)q.x[i], (T)q.y[i]);
+FCG_UNROLL
+        for (int j = 0; j < q.n; ++j) {
+            const Vec3<T> dd = sub3(p1, p2[j]);
+            result += (T)full_weight(ORDER, i, j) * ((T)1 / norm3(dd));
+        }
+    }
+    return result / (T)FCG_FOUR_PI_TIMES_E0;
+}
+
+// double mean of (r1 - r2).n / |r1 - r2|^3 over v1 and v2: RULE 10 or 5 with the
+// products of the rule's weights, RULE 0 the 3 x 3 table of order 2
+template <typename T, int RULE>
+FCG_HD T mutual_dfull_3d(const float* v1, const float* v2, Vec3<T> n) {
+    double px[19] = {}, py[19] = {}, w[19] = {};
+    int m;
+    if (RULE == 0) {
+        const FullRule f = full_rule(2);
+        m = f.n;
+        for (int i = 0; i < m; ++i) {
+            px[i] = f.x[i];
+            py[i] = f.y[i];
+        }
+    } else if (RULE == 5) {
+        const QuadRule q = quad_rule(5);
+        m = q.n;
+        for (int i = 0; i < m; ++i) {
+            px[i] = q.x[i];
+            py[i] = q.y[i];
+            w[i] = q.w[i];
+        }
+    } else {
+        const QuadRule19 q = quad_rule19(RULE);
+        m = q.n;
+        for (int i = 0; i < m; ++i) {
+            px[i] = q.x[i];
+            py[i] = q.y[i];
+            w[i] = q.w[i];
+        }
+    }
+    T result = 0;
+    for (int i = 0; i < m; ++i) {
+        const Vec3<T> p1 = bary_point<T>(v1, (T)px[i], (T)py[i]);
+        for (int j = 0; j < m; ++j) {
+            const Vec3<T> dd = sub3(p1, bary_point<T>(v2, (T)px[j], (T)py[j]));
+            const T rd = norm3(dd);
+            const T wij = RULE == 0 ? (T)full_weight(2, i, j) : (T)(w[i] * w[j]);
+            result += wij * (dot3(dd, n) / (rd * rd * rd));
+        }
+    }
+    return result / (T)FCG_FOUR_PI_TIMES_E0;
 }
 
 // 2D: potential (up to the log constant, unscaled) at r of a unit-charge segment

@@ -216,6 +216,7 @@ struct Scratch {
 
 struct Job {
     int dim = 3, N = 0, M = 0, G = 0;
+    int scheme = FCG_SCHEME_COLLOCATION;  // discretization (fcg_job_desc::scheme)
     int64_t L = 0;
     DevBuf nodes, left, right, parent, leaf_lo, leaf_hi, leaf_node, row_ptr, col, link_row, gout, gin, leaf_w,
         col_range, self, diag, coef64, coef32, perm, iperm, rorder, fill_order, fill_start, rhs_range, col16, grow,
@@ -294,6 +295,7 @@ struct Job {
         b.k = sc.k;
         b.kp = sc.kp;
         b.dim = dim;
+        b.scheme = scheme;
         b.first_cond_leaf = first_cond_leaf;
         b.L = L;
         b.nodes = nodes.as<fcg_node_geom>();
@@ -679,6 +681,8 @@ int validate_desc(fcg_ctx* ctx, const fcg_job_desc* d, bool check_cols = true) {
         !d->group_outperm || !d->group_inperm || !d->leaf_outperm)
         return fail(ctx, FCG_INVALID_ARGUMENT, "null array in job descriptor");
     if (d->dim != 2 && d->dim != 3) return fail(ctx, FCG_INVALID_ARGUMENT, "dim must be 2 or 3");
+    if (d->scheme != 0 && d->scheme != FCG_SCHEME_COLLOCATION && d->scheme != FCG_SCHEME_GALERKIN)
+        return fail(ctx, FCG_INVALID_ARGUMENT, "unknown discretization scheme");
     const int N = d->n_leaves, M = d->n_nodes;
     if (N < 1 || M < N || d->n_groups < 1 || d->n_links < 0)
         return fail(ctx, FCG_INVALID_ARGUMENT, "bad sizes in job descriptor");
@@ -2283,6 +2287,8 @@ namespace {
 // tree arrays of an upload (both layouts)
 int upload_tree(fcg_ctx* ctx, const fcg_job_desc* d, Job& j) {
     j.dim = d->dim;
+    // Galerkin in 3D; 2D jobs are solved by collocation (SolveCapacitance.cpp:159-161)
+    j.scheme = d->dim == 3 && d->scheme == FCG_SCHEME_GALERKIN ? FCG_SCHEME_GALERKIN : FCG_SCHEME_COLLOCATION;
     j.N = d->n_leaves;
     j.M = d->n_nodes;
     j.G = d->n_groups;
@@ -2605,35 +2611,43 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
     }
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->tabs.reset();
-    BatchView bv;
-    std::vector<int> order;
-    RCCHECK(make_batch(ctx, ctx->tabs, ents, &bv, nullptr, &order));
-    // per class, the prefix of the jobs' link counts in the batch's job order
-    const int64_t* offp[kFillClasses];
-    int64_t total[kFillClasses];
-    for (int c = 0; c < kFillClasses; ++c) {
-        std::vector<int64_t> off(order.size() + 1, 0);
-        for (size_t i = 0; i < order.size(); ++i)
-            off[i + 1] = off[i] + ents[(size_t)order[i]].j->fill_cnt[(size_t)c];
-        total[c] = off.back();
-        void* d;
-        HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
-        offp[c] = reinterpret_cast<const int64_t*>(d);
+    // one fill per scheme present (the Galerkin kernels are instantiations of their own)
+    for (const int scheme : {FCG_SCHEME_COLLOCATION, FCG_SCHEME_GALERKIN}) {
+        std::vector<BatchEntry> part;
+        for (const auto& e : ents)
+            if (e.j->scheme == scheme) part.push_back(e);
+        if (part.empty()) continue;
+        BatchView bv;
+        std::vector<int> order;
+        RCCHECK(make_batch(ctx, ctx->tabs, part, &bv, nullptr, &order));
+        // per class, the prefix of the jobs' link counts in the batch's job order
+        const int64_t* offp[kFillClasses];
+        int64_t total[kFillClasses];
+        for (int c = 0; c < kFillClasses; ++c) {
+            std::vector<int64_t> off(order.size() + 1, 0);
+            for (size_t i = 0; i < order.size(); ++i)
+                off[i + 1] = off[i] + part[(size_t)order[i]].j->fill_cnt[(size_t)c];
+            total[c] = off.back();
+            void* d;
+            HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
+            offp[c] = reinterpret_cast<const int64_t*>(d);
+        }
+        // mixed: the analytic classes from the near-field side lists
+        const int64_t* near_off = nullptr;
+        int64_t near_total = 0;
+        if (precision == FCG_PREC_MIXED) {
+            std::vector<int64_t> off(order.size() + 1, 0);
+            for (size_t i = 0; i < order.size(); ++i) off[i + 1] = off[i] + part[(size_t)order[i]].j->Ln;
+            near_total = off.back();
+            void* d;
+            HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
+            near_off = reinterpret_cast<const int64_t*>(d);
+        }
+        // no wait: later work on the context's stream runs after the fill, and every
+        // reuse of the context's tables waits for the stream first
+        HIPCHECK(ctx, fcg::launch_fill(bv, offp, total, precision, near_off, near_total,
+                                       scheme == FCG_SCHEME_GALERKIN, s));
     }
-    // mixed: the analytic classes from the near-field side lists
-    const int64_t* near_off = nullptr;
-    int64_t near_total = 0;
-    if (precision == FCG_PREC_MIXED) {
-        std::vector<int64_t> off(order.size() + 1, 0);
-        for (size_t i = 0; i < order.size(); ++i) off[i + 1] = off[i] + ents[(size_t)order[i]].j->Ln;
-        near_total = off.back();
-        void* d;
-        HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
-        near_off = reinterpret_cast<const int64_t*>(d);
-    }
-    // no wait: later work on the context's stream runs after the fill, and every
-    // reuse of the context's tables waits for the stream first
-    HIPCHECK(ctx, fcg::launch_fill(bv, offp, total, precision, near_off, near_total, s));
     for (auto& e : ents) {
         if (precision == FCG_PREC_MIXED)
             e.j->filled32 = true;

@@ -52,6 +52,7 @@ struct alignas(16) SrcGeom {
 // Device record of one job in a batch.
 struct BJob {
     int N, M, k, dim;
+    int scheme;           // FCG_SCHEME_GALERKIN for a 3D Galerkin job, else collocation
     int kp;               // node-major row stride: k rounded up to a power of two
     int first_cond_leaf;  // 2D zero-total-charge fix-up start
     int64_t L;
@@ -212,8 +213,9 @@ hipError_t launch_link_rows(const BatchView& b, hipStream_t s);
 // In mixed precision the analytic classes (near_class) are filled from the jobs'
 // near-field side lists instead: near_off (device, njobs + 1) is the prefix of
 // their near link counts, near_total its last entry.
+// galerkin: every job of the batch takes the Galerkin scheme's 3D integrals
 hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int64_t* total, int precision,
-                       const int64_t* near_off, int64_t near_total, hipStream_t s);
+                       const int64_t* near_off, int64_t near_total, bool galerkin, hipStream_t s);
 inline bool near_class(int c) { return c == 0 || c == 5 || c == 10 || c == 12; }
 // Near-field side list of one job: flag[l] = 1 for the fill_order entries
 // [p0, p0 + n); per-row counts of flagged links; the rows' flagged links

@@ -98,9 +98,30 @@ __device__ __forceinline__ double block_sum(double v, double* sh) {
 enum FillBranch { kBrAnalytic = 0, kBrRule5 = 1, kBrRule3 = 2, kBrRule2 = 3, kBrPoint = 4 };
 
 // p2: the source, a fcg_node_geom or its SrcGeom copy (vertices, centroid)
-template <typename T, int DIM, bool DIEL, int BR, typename G2>
+template <typename T, int DIM, bool DIEL, int BR, bool GAL = false, typename G2>
 __device__ __forceinline__ T link_coef(const fcg_node_geom& p1, const G2& p2) {
-    if constexpr (DIM == 2) {
+    if constexpr (GAL && DIM == 3) {
+        // Galerkin branches of PotEstimateOpt (Solver/Autorefine.cpp:3570-3674)
+        const Vec3<T> dist{(T)p1.c[0] - (T)p2.c[0], (T)p1.c[1] - (T)p2.c[1], (T)p1.c[2] - (T)p2.c[2]};
+        const T k4pe0 = (T)FCG_FOUR_PI_TIMES_E0;
+        if constexpr (!DIEL) {
+            if constexpr (BR == kBrAnalytic) return mutual_half_3d<T, 17>(p1.v, p2.v);
+            if constexpr (BR == kBrRule5) return mutual_half_3d<T, 10>(p1.v, p2.v);
+            if constexpr (BR == kBrRule3) return mutual_full_3d<T, 4>(p1.v, p2.v);
+            if constexpr (BR == kBrRule2) return mutual_full_3d<T, 2>(p1.v, p2.v);
+            if constexpr (BR == kBrPoint) return (T)1 / (k4pe0 * norm3(dist));
+        } else {
+            const Vec3<T> n1{(T)p1.dn[0], (T)p1.dn[1], (T)p1.dn[2]};
+            if constexpr (BR == kBrAnalytic) return mutual_dhalf_3d<T, 17>(p1.v, p2.v, n1, norm3(dist) / (T)20);
+            if constexpr (BR == kBrRule5) return mutual_dfull_3d<T, 10>(p1.v, p2.v, n1);
+            if constexpr (BR == kBrRule3) return mutual_dfull_3d<T, 5>(p1.v, p2.v, n1);
+            if constexpr (BR == kBrRule2) return mutual_dfull_3d<T, 0>(p1.v, p2.v, n1);
+            if constexpr (BR == kBrPoint) {
+                const T rd = norm3(dist);
+                return dot3(dist, n1) / (k4pe0 * rd * rd * rd);
+            }
+        }
+    } else if constexpr (DIM == 2) {
         const T dx = (T)p1.c[0] - (T)p2.c[0], dy = (T)p1.c[1] - (T)p2.c[1];
         if constexpr (!DIEL) {
             if constexpr (BR == kBrAnalytic)
@@ -162,7 +183,7 @@ constexpr int fill_per() {
     return BR == kBrAnalytic ? 1 : kFillPer;
 }
 
-template <int DIM, bool DIEL, int BR, bool MIXED>
+template <int DIM, bool DIEL, int BR, bool MIXED, bool GAL>
 __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __restrict__ off, int64_t total, int cls) {
     constexpr int P = fill_per<BR>();
     const int64_t g0 = (int64_t)blockIdx.x * blockDim.x * P;
@@ -194,12 +215,14 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __res
         const fcg_node_geom& p1 = J.nodes[row[i]];
         const SrcGeom& p2 = J.src_geom[src[i]];
         if constexpr (MIXED) {
-            if constexpr (BR == kBrAnalytic)
-                J.coef32[l[i]] = (float)link_coef<double, DIM, DIEL, BR>(p1, p2);
+            // Galerkin: every branch integrated in FP64 (double quadratures of
+            // nearby panels), stored in FP32
+            if constexpr (BR == kBrAnalytic || GAL)
+                J.coef32[l[i]] = (float)link_coef<double, DIM, DIEL, BR, GAL>(p1, p2);
             else
                 J.coef32[l[i]] = link_coef<float, DIM, DIEL, BR>(p1, p2);
         } else {
-            J.coef64[l[i]] = link_coef<double, DIM, DIEL, BR>(p1, p2);
+            J.coef64[l[i]] = link_coef<double, DIM, DIEL, BR, GAL>(p1, p2);
         }
     }
 }
@@ -209,6 +232,7 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __res
 // side list, the difference to FP64 (added by the refinement residual's product).
 // One near link per lane; the row node's dielectric flag picks the kernel of
 // PotEstimateOpt's conductor or dielectric branch as the fill class does.
+template <bool GAL>
 __global__ void __launch_bounds__(256) k_fill_near(BatchView bv, const int64_t* __restrict__ off, int64_t total) {
     const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
     const int64_t g = g0 + threadIdx.x;
@@ -224,7 +248,8 @@ __global__ void __launch_bounds__(256) k_fill_near(BatchView bv, const int64_t* 
     if (J.dim == 2)
         c = diel ? link_coef<double, 2, true, kBrAnalytic>(p1, p2) : link_coef<double, 2, false, kBrAnalytic>(p1, p2);
     else
-        c = diel ? link_coef<double, 3, true, kBrAnalytic>(p1, p2) : link_coef<double, 3, false, kBrAnalytic>(p1, p2);
+        c = diel ? link_coef<double, 3, true, kBrAnalytic, GAL>(p1, p2)
+                 : link_coef<double, 3, false, kBrAnalytic, GAL>(p1, p2);
     const float c32 = (float)c;
     J.coef32[l] = c32;
     J.near_dc[sl] = c - (double)c32;
@@ -277,6 +302,7 @@ __device__ double node_self(const BJob& J, const fcg_node_geom& nd) {
         return FCG_PI * (ea + eb) / ((eb - ea) * nd.area);
     }
     if (J.dim == 3) {
+        if (J.scheme == FCG_SCHEME_GALERKIN) return auto_3d<double>(nd.v);  // Autorefine.cpp:2399-2401
         const Vec3<double> c{(double)nd.c[0], (double)nd.c[1], (double)nd.c[2]};
         return potential_opt_3d<double>(c, nd.v) / FCG_FOUR_PI_TIMES_E0;
     }
@@ -1112,11 +1138,11 @@ hipError_t launch_link_rows(const BatchView& b, hipStream_t s) {
     return hipGetLastError();
 }
 
-template <bool MIXED>
+template <bool MIXED, bool GAL>
 static void fill_class(const BatchView& b, const int64_t* off, int64_t n, int c, hipStream_t s) {
 #define FCG_F(C, DIM, DIEL, BR) \
     case C:                                                                                              \
-        k_fill<DIM, DIEL, BR, MIXED><<<blocks_for((size_t)n, 256 * fill_per<BR>()), 256, 0, s>>>(b, off, n, c); \
+        k_fill<DIM, DIEL, BR, MIXED, GAL && DIM == 3><<<blocks_for((size_t)n, 256 * fill_per<BR>()), 256, 0, s>>>(b, off, n, c); \
         break;
     switch (c) {
         FCG_F(0, 3, false, 0) FCG_F(1, 3, false, 1) FCG_F(2, 3, false, 2) FCG_F(3, 3, false, 3) FCG_F(4, 3, false, 4)
@@ -1128,18 +1154,23 @@ static void fill_class(const BatchView& b, const int64_t* off, int64_t n, int c,
 }
 
 hipError_t launch_fill(const BatchView& b, const int64_t* const* off, const int64_t* total, int precision,
-                       const int64_t* near_off, int64_t near_total, hipStream_t s) {
+                       const int64_t* near_off, int64_t near_total, bool galerkin, hipStream_t s) {
     const bool mixed = precision == FCG_PREC_MIXED;
     for (int c = 0; c < kFillClasses; ++c) {
         if (total[c] <= 0) continue;
         if (mixed && near_off && near_class(c)) continue;  // filled from the side lists below
         if (mixed)
-            fill_class<true>(b, off[c], total[c], c, s);
+            galerkin ? fill_class<true, true>(b, off[c], total[c], c, s) : fill_class<true, false>(b, off[c], total[c], c, s);
         else
-            fill_class<false>(b, off[c], total[c], c, s);
+            galerkin ? fill_class<false, true>(b, off[c], total[c], c, s)
+                     : fill_class<false, false>(b, off[c], total[c], c, s);
     }
-    if (mixed && near_off && near_total > 0)
-        k_fill_near<<<blocks_for((size_t)near_total, 256), 256, 0, s>>>(b, near_off, near_total);
+    if (mixed && near_off && near_total > 0) {
+        if (galerkin)
+            k_fill_near<true><<<blocks_for((size_t)near_total, 256), 256, 0, s>>>(b, near_off, near_total);
+        else
+            k_fill_near<false><<<blocks_for((size_t)near_total, 256), 256, 0, s>>>(b, near_off, near_total);
+    }
     if (b.leaves > 0) k_self<<<blocks_for((size_t)b.leaves, 256), 256, 0, s>>>(b);
     return hipGetLastError();
 }
@@ -1825,7 +1856,8 @@ hipError_t launch_sqnorms(const double* x, int N, int k, double* out, hipStream_
 // Link coefficient with the branch chosen at run time (setup only): the ratio is
 // computed in FP64 without contractions, as the reference and the upload-time
 // classification compute it.
-__device__ __noinline__ double pair_coef(const fcg_node_geom& p1, const fcg_node_geom& p2, int dim) {
+template <bool GAL>
+__device__ __noinline__ double pair_coef_s(const fcg_node_geom& p1, const fcg_node_geom& p2, int dim) {
     double ratio;
     {
 #pragma clang fp contract(off)
@@ -1841,17 +1873,22 @@ __device__ __noinline__ double pair_coef(const fcg_node_geom& p1, const fcg_node
         return diel ? link_coef<double, 2, true, kBrPoint>(p1, p2) : link_coef<double, 2, false, kBrPoint>(p1, p2);
     }
     if (diel) {
-        if (ratio < 1.0) return link_coef<double, 3, true, kBrAnalytic>(p1, p2);
-        if (ratio < 2.0) return link_coef<double, 3, true, kBrRule5>(p1, p2);
-        if (ratio < 4.0) return link_coef<double, 3, true, kBrRule3>(p1, p2);
-        if (ratio < 10.0) return link_coef<double, 3, true, kBrRule2>(p1, p2);
-        return link_coef<double, 3, true, kBrPoint>(p1, p2);
+        if (ratio < 1.0) return link_coef<double, 3, true, kBrAnalytic, GAL>(p1, p2);
+        if (ratio < 2.0) return link_coef<double, 3, true, kBrRule5, GAL>(p1, p2);
+        if (ratio < 4.0) return link_coef<double, 3, true, kBrRule3, GAL>(p1, p2);
+        if (ratio < 10.0) return link_coef<double, 3, true, kBrRule2, GAL>(p1, p2);
+        return link_coef<double, 3, true, kBrPoint, GAL>(p1, p2);
     }
-    if (ratio < 1.0) return link_coef<double, 3, false, kBrAnalytic>(p1, p2);
-    if (ratio < 2.0) return link_coef<double, 3, false, kBrRule5>(p1, p2);
-    if (ratio < 4.0) return link_coef<double, 3, false, kBrRule3>(p1, p2);
-    if (ratio < 10.0) return link_coef<double, 3, false, kBrRule2>(p1, p2);
-    return link_coef<double, 3, false, kBrPoint>(p1, p2);
+    if (ratio < 1.0) return link_coef<double, 3, false, kBrAnalytic, GAL>(p1, p2);
+    if (ratio < 2.0) return link_coef<double, 3, false, kBrRule5, GAL>(p1, p2);
+    if (ratio < 4.0) return link_coef<double, 3, false, kBrRule3, GAL>(p1, p2);
+    if (ratio < 10.0) return link_coef<double, 3, false, kBrRule2, GAL>(p1, p2);
+    return link_coef<double, 3, false, kBrPoint, GAL>(p1, p2);
+}
+
+// the scheme of the job decides the branches' integrals (3D)
+__device__ __forceinline__ double pair_coef(const BJob& J, const fcg_node_geom& p1, const fcg_node_geom& p2) {
+    return J.dim == 3 && J.scheme == FCG_SCHEME_GALERKIN ? pair_coef_s<true>(p1, p2, 3) : pair_coef_s<false>(p1, p2, J.dim);
 }
 
 constexpr int kSetupThreads = 256;
@@ -1940,7 +1977,7 @@ __global__ void __launch_bounds__(kSetupThreads) k_block_setup(BatchView bv, con
     double* A = n <= kSetupLds ? sh : work + boff[b];  // row-major n x n
     for (int e = threadIdx.x; e < n * n; e += kSetupThreads) {
         const int i = e / n, j = e - i * n;
-        A[e] = (i == j) ? J.self[b0 + i] : pair_coef(J.nodes[J.leaf_node[b0 + i]], J.nodes[J.leaf_node[b0 + j]], J.dim);
+        A[e] = (i == j) ? J.self[b0 + i] : pair_coef(J, J.nodes[J.leaf_node[b0 + i]], J.nodes[J.leaf_node[b0 + j]]);
     }
     __syncthreads();
     lu_invert(A, n, piv, P + boff[b], rv, ri);
@@ -1957,7 +1994,7 @@ __global__ void k_super_build(BatchView bv, const int32_t* __restrict__ snode, i
     const BJob& J = bv.jobs[0];
     const int j = (int)(e / m), i = (int)(e % m);
     const fcg_node_geom& ni = J.nodes[snode[i]];
-    A[e] = (i == j) ? node_self(J, ni) : pair_coef(ni, J.nodes[snode[j]], J.dim);
+    A[e] = (i == j) ? node_self(J, ni) : pair_coef(J, ni, J.nodes[snode[j]]);
     X[e] = (i == j) ? 1.0 : 0.0;
 }
 
@@ -2064,7 +2101,7 @@ __global__ void __launch_bounds__(kSuperThreads) k_super_invert_lds(BatchView bv
     for (int e = t; e < mm; e += kSuperThreads) {
         const int c = e / m, i = e - c * m;
         const fcg_node_geom& ni = J.nodes[st.snode[i]];
-        A[e] = (i == c) ? node_self(J, ni) : pair_coef(ni, J.nodes[st.snode[c]], J.dim);
+        A[e] = (i == c) ? node_self(J, ni) : pair_coef(J, ni, J.nodes[st.snode[c]]);
     }
     __syncthreads();
     for (int j = 0; j < m; ++j) {

@@ -42,6 +42,7 @@ struct Options {
     bool out_charge = false;  // -c: charge densities per conductor in FastCap format
     bool dump_resid = false;  // -r: GMRES residual at each iteration
     bool dump_time = false;   // -i: phase times and memory
+    bool galerkin = false;    // -g: Galerkin scheme (3D)
 };
 
 void usage() {
@@ -66,6 +67,7 @@ void usage() {
         "  -b       Console mode (always)\n"
         "  -b?      This text\n"
         "  -bv      Print only the version\n"
+        "  -g       Galerkin scheme (3D; 2D inputs are solved by collocation)\n"
         "  -gmixed  Mixed precision (FP32 far field and products, FP64 refinement)\n"
         "  -gdev<n> HIP device (default 0)\n"
         "  -glinks-host  Refine the panels and discover the links on the host (default: on the device,\n"
@@ -157,6 +159,8 @@ int parse(int argc, char** argv, Options& o, bool& stop) {
             o.dump_time = true;
         } else if (!std::strcmp(a, "-gmixed")) {
             o.precision = FCG_PREC_MIXED;
+        } else if (!std::strcmp(a, "-g")) {
+            o.galerkin = true;
         } else if (!std::strncmp(a, "-gdev", 5)) {
             double v;
             ok = num(a + 5, &v);
@@ -317,6 +321,7 @@ int solve_mesh(fcg_ctx* ctx, const Options& o, double tol, Result& r) {
     }
     std::printf("Precond Type(s) (-p): %s\n", precond_line(precond, size));
     int job = -1;
+    if (o.galerkin && d.dim == 3) d.scheme = FCG_SCHEME_GALERKIN;
     rc = o.host_links ? fcg_job_upload(ctx, &d, &job) : fcg_job_upload_links(ctx, &d, &job);
     const auto tf = std::chrono::steady_clock::now();
     if (rc == FCG_OK) rc = fcg_fill(ctx, job, o.precision);
@@ -499,11 +504,11 @@ void print_time(double t) {
 }
 
 // CSolveCap::OutputSolveParams (Solver/SolveCapacitance.cpp:610-650)
-void print_params(const Options& o, double tol) {
+void print_params(const Options& o, double tol, bool galerkin) {
     if (o.out_geo) std::printf("Output refined geometry in FastCap compatible format (-o)\n");
     if (o.out_charge) std::printf("Output charge densities information (-c)\n");
     if (o.out_csv) std::printf("Output capacitance matrix to file (-e)\n");
-    std::printf("Solution scheme (-g): Collocation, GMRES tolerance (-t): %g\n", tol);
+    std::printf("Solution scheme (-g): %s, GMRES tolerance (-t): %g\n", galerkin ? "Galerkin" : "Collocation", tol);
     std::printf("Out-of-core free memory to link memory condition (-f): %g\n", 5.0);
     std::printf("Potential interaction coefficient to mesh refinement ratio (-d): %g\n", o.eps_ratio);
     std::printf("Mesh curvature (-mc): %g\n", o.curv);
@@ -533,6 +538,27 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "Error: cannot open HIP device %d\n", o.device);
         return FCG_DEVICE_ERROR;
     }
+    // the Galerkin scheme is 3D only: the reference warns and solves 2D inputs by
+    // collocation (Solver/SolveCapacitance.cpp:159-161)
+    bool galerkin = o.galerkin;
+    if (galerkin) {
+        fch_options ho;
+        fch_default_options(&ho);
+        fch_mesh* probe = nullptr;
+        ho.mesh_eps = 1e32;
+        ho.links_on_device = 0;
+        ho.refine_on_device = 0;
+        if (fch_build(o.file.c_str(), &ho, &probe) == 0) {
+            fch_stats st;
+            fch_get_stats(probe, &st);
+            if (st.dim == 2) {
+                std::fprintf(stderr, "Warning: Galerkin scheme yet not supported in 2D, moving to Collocation\n");
+                galerkin = false;
+                o.galerkin = false;
+            }
+            fch_free(probe);
+        }
+    }
     const auto t0 = std::chrono::steady_clock::now();
     using clk = std::chrono::steady_clock;
     auto since = [](clk::time_point a) { return std::chrono::duration<double>(clk::now() - a).count(); };
@@ -540,7 +566,7 @@ int main(int argc, char** argv) {
     if (!o.autom) {
         // manual run (Solver/SolveCapacitance.cpp:479-531)
         std::printf("\n");
-        print_params(o, o.tol);
+        print_params(o, o.tol, galerkin);
         std::printf("***************************************\nRefining the geometry.. \n");
         rc = build_mesh(ctx, o, o.mesh_eps, r);
         if (rc == FCG_OK) {
@@ -557,7 +583,7 @@ int main(int argc, char** argv) {
         std::printf("Remark: Auto option overrides all other Manual settings\n");
         const double tol = o.auto_err / 2.0;
         std::printf("\n");
-        print_params(o, tol);
+        print_params(o, tol, galerkin);
         std::printf("\nIteration number #0 ***************************\n\n");
         auto ti = clk::now();
         std::printf("***************************************\nRefining the geometry.. \n");

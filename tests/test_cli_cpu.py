@@ -21,7 +21,7 @@ def test_switches():
     assert cli.parse_args(["x", "-mc0.5"]).curv_coeff == 1.0  # clamped as the reference does
 
 
-@pytest.mark.parametrize("bad", [["x", "-g"], ["x", "-psx"], ["x", "-ps-3"], ["x", "-pbx"], ["x", "-mabc"], [], ["a", "b"]])
+@pytest.mark.parametrize("bad", [["x", "-ph"], ["x", "-psx"], ["x", "-ps-3"], ["x", "-pbx"], ["x", "-mabc"], [], ["a", "b"]])
 def test_errors(bad):
     with pytest.raises(cli.CommandLineError):
         cli.parse_args(bad)
@@ -131,3 +131,12 @@ def test_refined_geometry_file_rebuilds_the_refined_panels(tmp_path):
     area = lambda mm, g: sum(mm.node_geom()["area"][mm.leaf_node[mm.group_leaf_start[g]:mm.group_leaf_start[g + 1]]])
     for g in range(m.G):
         assert abs(area(back, g) / area(m, g) - 1) < 1e-6
+
+
+def test_galerkin_switches():
+    """-g selects the Galerkin scheme; -r and -i the residual and time/memory dumps."""
+    a = cli.parse_args(["x.txt", "-g", "-r", "-i"])
+    assert a.galerkin and a.dump_resid and a.dump_time
+    assert not cli.parse_args(["x.txt"]).galerkin
+    assert "Solution scheme (-g): Galerkin," in cli.format_params(a, 0.01)
+    assert "Solution scheme (-g): Collocation," in cli.format_params(cli.parse_args(["x.txt"]), 0.01)

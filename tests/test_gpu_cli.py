@@ -104,7 +104,7 @@ def test_lossy_coax_complex_printout(geo_dir, python_cli):
 
 
 def test_binary_rejects_unsupported_switch(geo_dir):
-    out = subprocess.run([BIN, "x.txt", "-g"], capture_output=True, text=True, timeout=60)
+    out = subprocess.run([BIN, "x.txt", "-ph"], capture_output=True, text=True, timeout=60)
     assert out.returncode == 64
 
 
