@@ -67,11 +67,14 @@ struct PinnedBuf {
     ~PinnedBuf() {
         if (p) (void)hipHostFree(p);
     }
-    hipError_t alloc(size_t n) {
+    void* dev = nullptr;  // the device's address of a coherent buffer
+    hipError_t alloc(size_t n, bool coherent = false) {
         if (p) (void)hipHostFree(p);
-        p = nullptr;
+        p = dev = nullptr;
         bytes = n;
-        return hipHostMalloc(&p, n, hipHostMallocDefault);
+        hipError_t e = hipHostMalloc(&p, n, coherent ? hipHostMallocCoherent : hipHostMallocDefault);
+        if (e == hipSuccess && coherent) e = hipHostGetDevicePointer(&dev, p, 0);
+        return e;
     }
     template <typename T>
     T* as() const {
@@ -162,7 +165,7 @@ struct Tables {
             const size_t sz = std::max<size_t>(need, (size_t)8 << 20);
             hipError_t e = sl->dev.alloc(sz);
             if (e != hipSuccess) return e;
-            e = sl->host.alloc(sz);
+            e = sl->host.alloc(sz, true);
             if (e != hipSuccess) return e;
             slabs.push_back(std::move(sl));
         }
@@ -185,10 +188,12 @@ struct Tables {
         if (defer) return hipSuccess;
         return hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s);
     }
+    // deferred mode: one copy kernel per slab on s, the stream of the step's
+    // kernels (slab sizes are multiples of 256 bytes)
     hipError_t flush(hipStream_t s) {
         for (auto& sl : slabs) {
             if (!sl->used) continue;
-            hipError_t e = hipMemcpyAsync(sl->dev.p, sl->host.p, sl->used, hipMemcpyHostToDevice, s);
+            hipError_t e = fcg::launch_copy_in(sl->host.dev, sl->dev.p, sl->used, s);
             if (e != hipSuccess) return e;
         }
         return hipSuccess;
@@ -407,7 +412,6 @@ __global__ void k_noalloc_table(NoAllocTable t, uint64_t* __restrict__ dst) {
 struct fcg_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
-    hipStream_t copy_stream = nullptr;  // engine table uploads (default priority, like stream)
     hipStream_t down_stream = nullptr;  // engine result downloads (default priority)
     void* noalloc = nullptr;            // this device's address of `scratch`
     bool profiling = false;
@@ -1209,6 +1213,13 @@ int super_structure(fcg_ctx* ctx, Job& j, int dim) {
     hipStream_t s = ctx->stream;
     std::vector<int32_t> nodes, starts;
     const bool avail = super_partition(j, dim, nodes, starts);
+    if (avail) {
+        // the elements are ascending, non-empty leaf ranges covering [0, N), as
+        // the collection (sstart) and the distribution (sidx) read them
+        bool ok = starts.size() == nodes.size() + 1 && starts.front() == 0 && starts.back() == j.N;
+        for (size_t e = 0; ok && e + 1 < starts.size(); ++e) ok = starts[e] < starts[e + 1];
+        if (!ok) return fail(ctx, FCG_GENERIC_ERROR, "two-level preconditioner: bad element partition");
+    }
     HIPCHECK(ctx, hipStreamSynchronize(s));  // the buffers below may be in use; the table arena is reset
     j.spre.dim = 0;
     j.spre.part_dim = 0;
@@ -1457,7 +1468,6 @@ struct Cohort {
     PBuf coefb, zbuf;  // DCGS2 update coefficients and solution-update coefficients of a step
     ProfEvents pe_res, pe_arn;
     hipEvent_t ev = nullptr;      // end of the step in flight
-    hipEvent_t tab_ev = nullptr;  // end of the step's table upload
     hipEvent_t kev = nullptr;     // end of the step's kernels
     bool inflight = false;
     int64_t load = 0;  // links x rhs of the active solves (balances submissions)
@@ -1474,7 +1484,6 @@ struct Cohort {
     Cohort& operator=(const Cohort&) = delete;
     ~Cohort() {
         if (ev) (void)hipEventDestroy(ev);
-        if (tab_ev) (void)hipEventDestroy(tab_ev);
         if (kev) (void)hipEventDestroy(kev);
     }
 };
@@ -1498,7 +1507,6 @@ struct fcg_engine {
     ~fcg_engine() {
         if (!ctx) return;
         (void)hipStreamSynchronize(ctx->stream);
-        (void)hipStreamSynchronize(ctx->copy_stream);
         (void)hipStreamSynchronize(ctx->down_stream);
         for (Cohort& c : co)
             if (c.tabs) {
@@ -1806,11 +1814,11 @@ int engine_launch(fcg_engine* E, Cohort& C) {
                 for (Job* j : jobs) j->spre.dim = 0;
         }
     } sinv_guard{sinv, true};
-    // The side streams pay off only while another cohort has work in flight;
+    // The download stream pays off only while another cohort has work in flight;
     // alone (a single solve), the copies stay on the compute stream.
     bool overlap = false;
     for (const Cohort& c : E->co) overlap |= (&c != &C && c.inflight);
-    const hipStream_t us = overlap ? ctx->copy_stream : s, ds = overlap ? ctx->down_stream : s;
+    const hipStream_t ds = overlap ? ctx->down_stream : s;
 
     // ---- A: cycle starts' first basis vectors, solution updates and starts
     {
@@ -2072,16 +2080,10 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         const size_t nb = sizeof(double) * (size_t)hdown;
         dq.push_back([=] { return hipMemcpyAsync(hh, src, nb, hipMemcpyDeviceToHost, ds); });
     }
-    // The tables go up on the copy stream, which does not queue behind the other
-    // cohort's work in flight on the compute stream (they belong to this cohort,
-    // whose previous step has completed); the compute stream waits for the copy
-    // alone, so the step's first kernel follows the other cohort's last one.
-    HIPCHECK(ctx, C.tabs->flush(us));
-    if (us != s) {
-        if (!C.tab_ev) HIPCHECK(ctx, hipEventCreateWithFlags(&C.tab_ev, hipEventDisableTiming));
-        HIPCHECK(ctx, hipEventRecord(C.tab_ev, us));
-        HIPCHECK(ctx, hipStreamWaitEvent(s, C.tab_ev, 0));
-    }
+    // The tables go up by a copy kernel on the compute stream, right after the
+    // other cohort's kernels in flight (a copy on the copy engine would make the
+    // compute stream wait on another queue's signal, about 50 us per step).
+    HIPCHECK(ctx, C.tabs->flush(s));
     for (auto& f : q) HIPCHECK(ctx, f());
     sinv_guard.armed = false;  // every inverse launch of the step is queued
     if (ds != s) {
@@ -2252,13 +2254,8 @@ int fcg_ctx_create(int device, fcg_ctx** out) {
     c->device = device;
     if (scratch_address(device, &c->noalloc) != hipSuccess || !c->noalloc) return FCG_DEVICE_ERROR;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) return FCG_DEVICE_ERROR;
-    if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) {
-        (void)hipStreamDestroy(c->stream);
-        return FCG_DEVICE_ERROR;
-    }
     if (hipStreamCreateWithFlags(&c->down_stream, hipStreamNonBlocking) != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
-        (void)hipStreamDestroy(c->copy_stream);
         return FCG_DEVICE_ERROR;
     }
     *out = c.release();
@@ -2269,12 +2266,10 @@ int fcg_ctx_destroy(fcg_ctx* ctx) {
     if (!ctx) return FCG_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamSynchronize(ctx->copy_stream);
     (void)hipStreamSynchronize(ctx->down_stream);
-    hipStream_t s = ctx->stream, cs = ctx->copy_stream, ds = ctx->down_stream;
+    hipStream_t s = ctx->stream, ds = ctx->down_stream;
     delete ctx;
     (void)hipStreamDestroy(s);
-    (void)hipStreamDestroy(cs);
     (void)hipStreamDestroy(ds);
     return FCG_OK;
 }

@@ -411,6 +411,7 @@ struct ChargeSumJob {
     int64_t blk;  // first block of this job (ascending, first entry 0)
     int64_t off;  // first output value of this job
 };
+hipError_t launch_copy_in(const void* src, void* dst, size_t bytes, hipStream_t s);
 hipError_t launch_charge_sums(const ChargeSumJob* jobs, int njobs, int64_t nblocks, double* out, hipStream_t s);
 // squared norms of the k segments of a [k][N] vector into out[k]
 hipError_t launch_sqnorms(const double* x, int N, int k, double* out, hipStream_t s);

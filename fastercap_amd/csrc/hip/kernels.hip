@@ -1838,6 +1838,23 @@ hipError_t launch_charge_sums(const ChargeSumJob* jobs, int njobs, int64_t nbloc
     return hipGetLastError();
 }
 
+// Step tables from pinned host staging to device memory, as a kernel on the
+// stream that uses them: a copy on another engine would need a cross-queue
+// dependency, which costs tens of microseconds at every step boundary.  src is
+// host-coherent (fine-grained) memory written by the host before the launch.
+__global__ void __launch_bounds__(256) k_copy_in(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) dst[i] = src[i];
+}
+
+hipError_t launch_copy_in(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    if ((bytes & 15) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15)) return hipErrorInvalidValue;
+    const size_t n = bytes / 16;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+    k_copy_in<<<grid, 256, 0, s>>>(reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n);
+    return hipGetLastError();
+}
+
 // grid (k): one block per segment
 __global__ void k_sqnorms(const double* __restrict__ x, int N, double* __restrict__ out) {
     __shared__ double sh[4];
@@ -2276,9 +2293,10 @@ hipError_t launch_block_apply(const PView& pv, int mode, hipStream_t s) {
 // nothing).
 template <int MODE>
 __global__ void __launch_bounds__(256) k_super_collect(SView sv) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t g = g0 + threadIdx.x;
+    const int jb = find_job_block(sv.ln_off, sv.njobs, g0, g < sv.nln ? g : sv.nln - 1);
     if (g >= sv.nln) return;
-    const int jb = find_job(sv.ln_off, sv.njobs, g);
     const SJob& J = sv.jobs[jb];
     // a solve's lanes fit in 32 bits: the host rejects solves whose padded lane
     // count (k m lpe rounded up to a wavefront) reaches 2^31
@@ -2352,9 +2370,10 @@ __global__ void __launch_bounds__(256) k_super_mult(SView sv) {
 // distribute: one thread per (solve, rhs, leaf)
 template <int MODE>
 __global__ void k_super_dist(SView sv) {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t g = g0 + threadIdx.x;
+    const int jb = find_job_block(sv.lf_off, sv.njobs, g0, g < sv.nlf ? g : sv.nlf - 1);
     if (g >= sv.nlf) return;
-    const int jb = find_job(sv.lf_off, sv.njobs, g);
     const SJob& J = sv.jobs[jb];
     const int local = (int)(g - sv.lf_off[jb]);  // k * N < 2^31 (checked when the table is built)
     const int r = local / J.N, i = local - r * J.N;
