@@ -2065,8 +2065,7 @@ int engine_launch(fcg_engine* E, Cohort& C) {
         RCCHECK(ensure_pbuf(ctx, C.coefb, sizeof(double) * (size_t)std::max<int64_t>(ncoef, 1)));
         double* cb = C.coefb.as<double>();
         q.push_back([=] { return fcg::launch_dcgs_dots(kv, pb, s); });
-        q.push_back([=] { return fcg::launch_dcgs_reduce(kv, pb, hb, s); });
-        q.push_back([=] { return fcg::launch_dcgs_coef(kv, hb, cb, s); });
+        q.push_back([=] { return fcg::launch_dcgs_coef(kv, pb, hb, cb, s); });
         q.push_back([=] { return fcg::launch_dcgs_update(kv, cb, s); });
     }
 

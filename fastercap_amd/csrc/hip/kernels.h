@@ -300,15 +300,15 @@ hipError_t launch_scale_new(const KrylovView& kv, const double* d, hipStream_t s
 // at hoff: [2 nv + 3][k] = Q^T v [nv], Q^T w [nv], v.v, v.w, w.w (v: slot nv,
 // w: the product); partial holds nchunks * 4 * (2 maxnv + 3) doubles.
 hipError_t launch_dcgs_dots(const KrylovView& kv, double* partial, hipStream_t s);
-hipError_t launch_dcgs_reduce(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s);
 // the final q in slot nv and, for jobs with `next`, the next provisional vector
 // in slot nv + 1; coefficients per (job, rhs) at coef[coff + r (2 nv + 3)]
 hipError_t launch_dcgs_update(const KrylovView& kv, const double* coef, hipStream_t s);
-// per (job, rhs): the step's Hessenberg columns, rotations and update
-// coefficients from the projection in hbuf (at hoff), and the host's outputs at
-// hbuf[ooff + 2 r] (status: 0 iterated, 1 met the tolerance before, 2 the
-// provisional vector lies in the basis; then |g_(nv+1)|)
-hipError_t launch_dcgs_coef(const KrylovView& kv, double* hbuf, double* coef, hipStream_t s);
+// per (job, rhs): the projection summed from the partials of launch_dcgs_dots
+// into hbuf (at hoff), then the step's Hessenberg columns, rotations and update
+// coefficients, and the host's outputs at hbuf[ooff + 2 r] (status: 0 iterated,
+// 1 met the tolerance before, 2 the provisional vector lies in the basis; then
+// |g_(nv+1)|)
+hipError_t launch_dcgs_coef(const KrylovView& kv, const double* partial, double* hbuf, double* coef, hipStream_t s);
 // per (job, rhs): the least-squares solution over columns 0..last of the rotated
 // system, z[zoff + jj k + r] for jj < nz (0 beyond last)
 hipError_t launch_dcgs_lsq(const KrylovView& kv, double* z, hipStream_t s);

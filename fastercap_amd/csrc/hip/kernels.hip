@@ -1355,19 +1355,33 @@ __global__ void __launch_bounds__(kKThreads) k_dcgs_dots(KrylovView kv, double* 
                 }
             }
         }
-        for (int off = 32; off > 0; off >>= 1)
+        // the 8 sums over the wavefront by recursive halving: at lane distance
+        // 32, 16 and 8 each lane keeps half of its values and sends the other
+        // half, so lane bits 5..3 pick one value, then 4, 2 and 1 finish it
+        // (10 shuffles instead of 48)
+        double x[2 * kG];
 #pragma unroll
-            for (int g = 0; g < kG; ++g) {
-                a[g] += __shfl_xor(a[g], off, 64);
-                b[g] += __shfl_xor(b[g], off, 64);
+        for (int g = 0; g < kG; ++g) {
+            x[g] = a[g];
+            x[kG + g] = b[g];
+        }
+#pragma unroll
+        for (int h = kG, off = 32; h >= 1; h >>= 1, off >>= 1) {
+            const bool up = (lane & off) != 0;
+#pragma unroll
+            for (int i = 0; i < h; ++i) {
+                const double keep = up ? x[h + i] : x[i];
+                const double send = up ? x[i] : x[h + i];
+                x[i] = keep + __shfl_xor(send, off, 64);
             }
-        if (lane == 0)
-#pragma unroll
-            for (int g = 0; g < kG; ++g)
-                if (j0 + g < nv) {
-                    out[j0 + g] = a[g];
-                    out[nv + j0 + g] = b[g];
-                }
+        }
+        double y = x[0];
+        for (int off = 4; off > 0; off >>= 1) y += __shfl_xor(y, off, 64);
+        const int idx = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);  // value of this lane
+        if ((lane & 7) == 0) {
+            const int g = idx & (kG - 1);
+            if (j0 + g < nv) out[(idx < kG ? 0 : nv) + j0 + g] = y;
+        }
     }
     for (int off = 32; off > 0; off >>= 1) {
         svv += __shfl_xor(svv, off, 64);
@@ -1379,23 +1393,6 @@ __global__ void __launch_bounds__(kKThreads) k_dcgs_dots(KrylovView kv, double* 
         out[2 * nv + 1] = svw;
         out[2 * nv + 2] = sww;
     }
-}
-
-// one thread per (segment, entry e < 2 nv + 3): hbuf[hoff + e k + r] = the sum
-// of the segment's chunk and wavefront partials, in chunk order
-__global__ void __launch_bounds__(256) k_dcgs_reduce(KrylovView kv, const double* __restrict__ partial,
-                                                     double* __restrict__ hbuf) {
-    const int P = 2 * kv.maxnv + 3;
-    const int64_t g = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (g >= (int64_t)kv.nsegs * P) return;
-    const int si = (int)(g / P), e = (int)(g - (int64_t)si * P);
-    const Seg sg = kv.segs[si];
-    const KJob kj = kv.kjobs[sg.job];
-    if (e >= 2 * kj.nv + 3) return;
-    double s = 0.0;
-    for (int c = sg.c_lo; c < sg.c_hi; ++c)
-        for (int q = 0; q < kDcgsWaves; ++q) s += partial[((size_t)c * kDcgsWaves + q) * P + e];
-    hbuf[(size_t)kj.hoff + (size_t)e * kj.k + sg.r] = s;
 }
 
 // grid (nchunks): the update that completes a DCGS2 step, with the host's
@@ -1488,12 +1485,24 @@ __device__ __forceinline__ double& dcgs_g(double* D, int c) { return c == 0 ? D[
 // coefficients over Q_(j-1), v_j and w go to coef for k_dcgs_update.  Lane 0 then
 // rotates the final column j-1 and, provisionally, column j, and reports
 // |g_(j+1)|.  A rhs that met the tolerance before gets coefficients that keep its
-// vectors.
-__global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, double* __restrict__ hbuf, double* __restrict__ coef) {
+// vectors.  The wavefront first sums its segment's projection from the chunk and
+// wavefront partials (in chunk order) into hbuf.
+__global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, const double* __restrict__ partial,
+                                                  double* __restrict__ hbuf, double* __restrict__ coef) {
     __shared__ double tj_sh;
     const Seg sg = kv.segs[blockIdx.x];
     const KJob kj = kv.kjobs[sg.job];
     const int j = kj.nv, k = kj.k, r = sg.r, lane = threadIdx.x;
+    {
+        const int P = 2 * kv.maxnv + 3;
+        for (int e = lane; e < 2 * j + 3; e += 64) {
+            double s = 0.0;
+            for (int c = sg.c_lo; c < sg.c_hi; ++c)
+                for (int q = 0; q < kDcgsWaves; ++q) s += partial[((size_t)c * kDcgsWaves + q) * P + e];
+            hbuf[(size_t)kj.hoff + (size_t)e * k + r] = s;
+        }
+        __syncthreads();
+    }
     const double* hv = hbuf + kj.hoff;
     auto at = [&](int e) { return hv[(size_t)e * k + r]; };
     double* cf = coef + kj.coff + (size_t)r * (2 * j + 3);
@@ -1711,11 +1720,6 @@ hipError_t launch_dcgs_dots(const KrylovView& kv, double* partial, hipStream_t s
         k_dcgs_dots<double><<<kv.nchunks, kKThreads, 0, s>>>(kv, partial);
     return hipGetLastError();
 }
-hipError_t launch_dcgs_reduce(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s) {
-    if (kv.nsegs == 0) return hipSuccess;
-    k_dcgs_reduce<<<blocks_for((size_t)kv.nsegs * (2 * (size_t)kv.maxnv + 3), 256), 256, 0, s>>>(kv, partial, hbuf);
-    return hipGetLastError();
-}
 hipError_t launch_dcgs_update(const KrylovView& kv, const double* coef, hipStream_t s) {
     if (kv.nchunks == 0) return hipSuccess;
     if (kv.basis32)
@@ -1724,9 +1728,9 @@ hipError_t launch_dcgs_update(const KrylovView& kv, const double* coef, hipStrea
         k_dcgs_update<double><<<kv.nchunks, kKThreads, 0, s>>>(kv, coef);
     return hipGetLastError();
 }
-hipError_t launch_dcgs_coef(const KrylovView& kv, double* hbuf, double* coef, hipStream_t s) {
+hipError_t launch_dcgs_coef(const KrylovView& kv, const double* partial, double* hbuf, double* coef, hipStream_t s) {
     if (kv.nsegs == 0) return hipSuccess;
-    k_dcgs_coef<<<kv.nsegs, 64, 0, s>>>(kv, hbuf, coef);
+    k_dcgs_coef<<<kv.nsegs, 64, 0, s>>>(kv, partial, hbuf, coef);
     return hipGetLastError();
 }
 hipError_t launch_dcgs_lsq(const KrylovView& kv, double* z, hipStream_t s) {
@@ -2310,13 +2314,25 @@ __global__ void __launch_bounds__(256) k_super_collect(SView sv) {
     const float* in32 = static_cast<const float*>(J.in);
     double acc = 0.0;
     if (live) {
-        for (int i = J.sstart[e] + sub; i < J.sstart[e + 1]; i += lpe) {
-            double x;
-            if (MODE == kPreFromB)
-                x = (i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
-            else
-                x = J.in32 ? (double)in32[(size_t)r * J.N + i] : in64[(size_t)r * J.N + i];
-            acc += J.w[i] * x;
+        // four leaves per lane in flight; the sum runs over i ascending as one
+        // leaf per step would
+        constexpr int kU = 4;
+        const int i1 = J.sstart[e + 1];
+        for (int ib = J.sstart[e] + sub; ib < i1; ib += kU * lpe) {
+            double x[kU], wv[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) {
+                const int i = ib + u * lpe;
+                const bool in = i < i1;
+                if (MODE == kPreFromB)
+                    x[u] = (in && i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
+                else
+                    x[u] = !in ? 0.0 : J.in32 ? (double)in32[(size_t)r * J.N + i] : in64[(size_t)r * J.N + i];
+                wv[u] = in ? J.w[i] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < kU; ++u)
+                if (ib + u * lpe < i1) acc += wv[u] * x[u];
         }
     }
     for (int off = lpe >> 1; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 64);
@@ -2345,11 +2361,25 @@ __global__ void __launch_bounds__(256) k_super_mult(SView sv) {
         const double* pot = J.pot + (size_t)r0 * m;
         double acc[kSuperRhs] = {};
         if (live) {
-            for (int t = t0; t < t1; ++t) {
-                const double a = (t != e) ? S[(size_t)t * m] : 0.0;
+            // kU steps of t in flight; the sums run over t ascending as one step
+            // at a time would
+            constexpr int kU = 8;
+            for (int tb = t0; tb < t1; tb += kU) {
+                double a[kU], p[kU][kSuperRhs];
 #pragma unroll
-                for (int r = 0; r < kSuperRhs; ++r)
-                    if (r < nr) acc[r] += a * pot[(size_t)r * m + t];
+                for (int u = 0; u < kU; ++u) {
+                    const int t = tb + u;
+                    a[u] = (t < t1 && t != e) ? S[(size_t)t * m] : 0.0;
+#pragma unroll
+                    for (int r = 0; r < kSuperRhs; ++r) p[u][r] = (t < t1 && r < nr) ? pot[(size_t)r * m + t] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < kU; ++u)
+                    if (tb + u < t1) {
+#pragma unroll
+                        for (int r = 0; r < kSuperRhs; ++r)
+                            if (r < nr) acc[r] += a[u] * p[u][r];
+                    }
             }
         }
         if (q > 0) {
@@ -2367,28 +2397,54 @@ __global__ void __launch_bounds__(256) k_super_mult(SView sv) {
     }
 }
 
-// distribute: one thread per (solve, rhs, leaf)
+// distribute: kDistPer (solve, rhs, leaf) items per thread, at g0 + threadIdx.x +
+// u 256, every item's loads issued before the first result is written
+constexpr int kDistPer = 4;
 template <int MODE>
-__global__ void k_super_dist(SView sv) {
-    const int64_t g0 = (int64_t)blockIdx.x * blockDim.x;
-    const int64_t g = g0 + threadIdx.x;
-    const int jb = find_job_block(sv.lf_off, sv.njobs, g0, g < sv.nlf ? g : sv.nlf - 1);
-    if (g >= sv.nlf) return;
-    const SJob& J = sv.jobs[jb];
-    const int local = (int)(g - sv.lf_off[jb]);  // k * N < 2^31 (checked when the table is built)
-    const int r = local / J.N, i = local - r * J.N;
-    const int e = J.sidx[i];
-    const size_t o = (size_t)r * J.N + i;
-    double x;
-    if (MODE == kPreFromB)
-        x = (i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
-    else
-        x = J.in32 ? (double)static_cast<const float*>(J.in)[o] : static_cast<const double*>(J.in)[o];
-    const double v = J.w[i] * (J.chg[(size_t)r * J.m + e] + J.Sinv[(size_t)e * J.m + e] * x);
-    if (MODE == kPreAdd)
-        J.out[o] += v;
-    else
-        J.out[o] = v;
+__global__ void __launch_bounds__(256) k_super_dist(SView sv) {
+    const int64_t g0 = (int64_t)blockIdx.x * 256 * kDistPer;
+    int jb = find_job_block(sv.lf_off, sv.njobs, g0, min(g0 + (int64_t)threadIdx.x, sv.nlf - 1));
+    int jj[kDistPer], ii[kDistPer], rr[kDistPer];
+#pragma unroll
+    for (int u = 0; u < kDistPer; ++u) {
+        const int64_t g = g0 + threadIdx.x + (int64_t)u * 256;
+        jj[u] = -1;
+        if (g < sv.nlf) {
+            while (jb + 1 < sv.njobs && sv.lf_off[jb + 1] <= g) ++jb;  // items ascend with u
+            const SJob& J = sv.jobs[jb];
+            const int local = (int)(g - sv.lf_off[jb]);  // k * N < 2^31 (checked when the table is built)
+            jj[u] = jb;
+            rr[u] = local / J.N;
+            ii[u] = local - rr[u] * J.N;
+        }
+    }
+    int ee[kDistPer];
+    double x[kDistPer], wv[kDistPer];
+#pragma unroll
+    for (int u = 0; u < kDistPer; ++u) {
+        if (jj[u] < 0) continue;
+        const SJob& J = sv.jobs[jj[u]];
+        const int r = rr[u], i = ii[u];
+        const size_t o = (size_t)r * J.N + i;
+        ee[u] = J.sidx[i];
+        wv[u] = J.w[i];
+        if (MODE == kPreFromB)
+            x[u] = (i >= J.brange[2 * r] && i < J.brange[2 * r + 1]) ? 1.0 : 0.0;
+        else
+            x[u] = J.in32 ? (double)static_cast<const float*>(J.in)[o] : static_cast<const double*>(J.in)[o];
+    }
+#pragma unroll
+    for (int u = 0; u < kDistPer; ++u) {
+        if (jj[u] < 0) continue;
+        const SJob& J = sv.jobs[jj[u]];
+        const int r = rr[u], i = ii[u], e = ee[u];
+        const size_t o = (size_t)r * J.N + i;
+        const double v = wv[u] * (J.chg[(size_t)r * J.m + e] + J.Sinv[(size_t)e * J.m + e] * x[u]);
+        if (MODE == kPreAdd)
+            J.out[o] += v;
+        else
+            J.out[o] = v;
+    }
 }
 
 hipError_t launch_super_apply(const SView& sv, int mode, hipStream_t s) {
@@ -2399,7 +2455,7 @@ hipError_t launch_super_apply(const SView& sv, int mode, hipStream_t s) {
     else
         k_super_collect<kPreSet><<<gc, 256, 0, s>>>(sv);
     k_super_mult<<<(unsigned)(sv.nem / 64), 256, 0, s>>>(sv);
-    const unsigned gd = blocks_for((size_t)sv.nlf, 256);
+    const unsigned gd = blocks_for((size_t)sv.nlf, 256 * kDistPer);
     if (mode == kPreFromB)
         k_super_dist<kPreFromB><<<gd, 256, 0, s>>>(sv);
     else if (mode == kPreAdd)
