@@ -177,19 +177,21 @@ def bench_batch(args, fc, dev, tmp):
     gc.disable()
     dev.profile(enable=1, reset=not args.only_mixed)
     t_start = time.perf_counter()
-    res = []
+    res = None
     for s in range(args.steps):
         a = time.perf_counter()
         dev.fill_batch(jobs, MIX)
         b = time.perf_counter()
-        res = dev.solve_batch(jobs, meshes, tol=tol, precision=MIX, **PC)
+        # every job's C matrix and statistics as arrays (BatchSolve)
+        res = dev.solve_batch_arrays(jobs, meshes, tol=tol, precision=MIX, **PC)
         c = time.perf_counter()
-        lat = [r.t_solve_s for r in res]
+        lat = res.stats["t_solve_s"].tolist()
         log(f"step {s}: fill {1e3 * (b - a):.1f} ms, batched mixed solve {1e3 * (c - b):.1f} ms")
     t_mixed = time.perf_counter() - t_start
     gc.enable()
     prof = dev.profile(enable=0, reset=True)
     value = args.steps * nj / t_mixed
+    res = res.results()  # one SolveResult per job, for the checks below
     mixed_caps = [r.cap for r in res]
     iters_mixed = [max(r.iterations) for r in res]
     unconverged = sum(1 for r in res if not r.converged)
@@ -213,8 +215,9 @@ def bench_batch(args, fc, dev, tmp):
     dev.solve_batch(jobs, meshes, tol=tol, precision=F64, **PC)  # warm
     a = time.perf_counter()
     dev.fill_batch(jobs, F64)
-    bres = dev.solve_batch(jobs, meshes, tol=tol, precision=F64, **PC)
+    bres = dev.solve_batch_arrays(jobs, meshes, tol=tol, precision=F64, **PC)
     t_b64 = time.perf_counter() - a
+    bres = bres.results()
     log(f"batched FP64: {t_b64:.2f}s ({nj / t_b64:.1f} solves/s)")
     ref = dev.solve_batch(jobs, meshes, tol=1e-8, precision=F64, **PC)
     e_conv = c_errors(mixed_caps, [r.cap for r in ref])

@@ -17,7 +17,7 @@ from __future__ import annotations
 
 import ctypes as C
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 
@@ -49,6 +49,7 @@ __all__ = [
     "Device",
     "Engine",
     "SolveResult",
+    "BatchSolve",
     "FasterCapError",
     "run",
     "FCG_MVP_FP64",
@@ -186,10 +187,12 @@ class Mesh:
             self._h = None
 
 
-@dataclass
+@dataclass(slots=True)
 class SolveResult:
+    # (slots and no default factories: a batch of a thousand results is built in
+    # about a millisecond)
     cap: np.ndarray  # rows x n_cond, unscaled (2D: multiply by 2*pi*e0 for F/m)
-    iterations: list = field(default_factory=list)
+    iterations: list | None = None
     residual: float = 0.0
     t_solve_s: float = 0.0
     t_mvp_s: float = 0.0
@@ -200,9 +203,48 @@ class SolveResult:
     # with want_residuals: per rhs, |r0| then |g_i| / |b| after each Arnoldi step (-r)
     residuals: list | None = None
     t_precond_s: float = 0.0
-    mem: dict = field(default_factory=dict)  # device bytes of the solve (gmres, vectors, product)
+    mem: dict | None = None  # device bytes of the solve (gmres, vectors, product)
     t_fill_s: float = 0.0  # set by callers that time the fill (command line -i)
-    job_mem: dict = field(default_factory=dict)  # fcg_job_memory of the job (command line -i)
+    job_mem: dict | None = None  # fcg_job_memory of the job (command line -i)
+
+
+class BatchSolve:
+    """Results of a batch of solves as arrays: caps holds every job's C matrix
+    (rows x cols, row-major; a complex job's imaginary parts follow its real
+    ones) job after job, rows and cols their shapes, cplx the complex flags and
+    stats the fcg_solve_stats records (a structured numpy array: iterations,
+    final_residual, converged, cycles, t_solve_s, n_mvp, ...)."""
+
+    __slots__ = ("caps", "rows", "cols", "cplx", "stats")
+
+    def __init__(self, caps, rows, cols, cplx, stats):
+        self.caps, self.rows, self.cols, self.cplx, self.stats = caps, rows, cols, cplx, stats
+
+    def __len__(self):
+        return len(self.rows)
+
+    def results(self) -> list:
+        """One SolveResult per job (the stats through plain lists: per-record
+        ctypes or numpy access costs milliseconds for a thousand jobs)."""
+        a = self.stats
+        rl, cl = self.rows.tolist(), self.cols.tolist()
+        rmax = max(max(rl, default=1), 1)
+        its = a["iterations"][:, : min(rmax, 64)].tolist()
+        resid, conv, cyc = a["final_residual"].tolist(), a["converged"].tolist(), a["cycles"].tolist()
+        tsol, nmvp = a["t_solve_s"].tolist(), a["n_mvp"].tolist()
+        caps = self.caps
+        out, off = [], 0
+        for cx, r, c, it, re, cv, cy, ts, nm in zip(self.cplx, rl, cl, its, resid, conv, cyc, tsol, nmvp):
+            n = r * c
+            capv = caps[off : off + n].reshape(r, c)
+            if cx:
+                capv = capv + 1j * caps[off + n : off + 2 * n].reshape(r, c)
+                off += 2 * n
+            else:
+                off += n
+            # positional: cap, iterations, residual, t_solve_s, t_mvp_s, n_mvp, charges, converged, cycles
+            out.append(SolveResult(capv, it[: min(r, 64)], re, ts, 0.0, nm, None, bool(cv), cy))
+        return out
 
 
 _STATS_DTYPE = None
@@ -422,7 +464,7 @@ class Device:
         self._check(self.lib.fcg_job_memory(self._h, job, C.byref(m)), "fcg_job_memory")
         return {"hierarchy": m.hierarchy, "links": m.links, "conductors": m.conductors, "precond": m.precond}
 
-    def solve_batch(
+    def solve_batch_arrays(
         self,
         jobs,
         meshes,
@@ -432,12 +474,13 @@ class Device:
         precision: int = FCG_PREC_FP64,
         max_refine: int = 8,
         precond_size: int = 0,
-    ):
+    ) -> "BatchSolve":
         """Solve several filled jobs together (one launch per stage for all of
-        them); returns one SolveResult per job, in order."""
+        them); the results as arrays (BatchSolve), with no per-job objects."""
         jobs = list(jobs)
         ncs = [m.n_cond for m in meshes]
-        caps = np.zeros(sum(n * n * (2 if m.complex else 1) for n, m in zip(ncs, meshes)), dtype=np.float64)
+        cplx = [bool(m.complex) for m in meshes]
+        caps = np.zeros(sum(n * n * (2 if c else 1) for n, c in zip(ncs, cplx)), dtype=np.float64)
         ja = (C.c_int * len(jobs))(*jobs)
         rows = (C.c_int * len(jobs))()
         cols = (C.c_int * len(jobs))()
@@ -448,23 +491,12 @@ class Device:
             self.lib.fcg_solve_batch(self._h, ja, len(jobs), C.byref(o), _dptr(caps), rows, cols, sts),
             "fcg_solve_batch",
         )
-        # the stats through one numpy view (per-record ctypes access costs a few
-        # ms for a thousand jobs)
-        a = np.frombuffer(sts, dtype=_stats_dtype())
-        rmax = max(max(rows[i] for i in range(len(jobs))), 1)
-        its = a["iterations"][:, : min(rmax, 64)].tolist()
-        resid, conv, cyc = a["final_residual"].tolist(), a["converged"].tolist(), a["cycles"].tolist()
-        tsol, nmvp = a["t_solve_s"].tolist(), a["n_mvp"].tolist()
-        out, off = [], 0
-        for i, m in enumerate(meshes):
-            r, c = rows[i], cols[i]
-            capv = caps[off : off + r * c].reshape(r, c)
-            if m.complex:
-                capv = capv + 1j * caps[off + r * c : off + 2 * r * c].reshape(r, c)
-            out.append(SolveResult(cap=capv, iterations=its[i][: min(r, 64)], residual=resid[i],
-                                   converged=bool(conv[i]), cycles=cyc[i], t_solve_s=tsol[i], n_mvp=nmvp[i]))
-            off += r * c * (2 if m.complex else 1)
-        return out
+        return BatchSolve(caps, np.ctypeslib.as_array(rows).copy(), np.ctypeslib.as_array(cols).copy(), cplx,
+                          np.frombuffer(sts, dtype=_stats_dtype()))
+
+    def solve_batch(self, jobs, meshes, **kw):
+        """solve_batch_arrays with one SolveResult per job, in order."""
+        return self.solve_batch_arrays(jobs, meshes, **kw).results()
 
     def footprint(self, job: int, precision: int = FCG_PREC_FP64, iters: int = 64):
         """(resident bytes of the job, bytes one solve of it allocates with a

@@ -200,7 +200,8 @@ def format_stats(mesh, a: Args | None = None, res=None) -> str:
     ratio = np.float32(100.0) - np.float32(links) / np.float32(n * n) * 100.0
     extra = ""
     if a is not None and a.dump_time and res is not None:
-        jm = res.job_mem
+        jm = res.job_mem or {}
+        mem = res.mem or {}
         extra = (f"Time for reading input file: {st.t_read_s:f}s\n"
                  f"Time for building super hierarchy: {st.t_superh_s:f}s\n"
                  f"Time for discretization: {st.t_discretize_s:f}s\n"
@@ -210,10 +211,10 @@ def format_stats(mesh, a: Args | None = None, res=None) -> str:
                  f"Memory allocated for panel hierarchy: {jm.get('hierarchy', 0)} bytes\n"
                  f"Memory allocated for links structure: {jm.get('links', 0)} bytes\n"
                  f"Memory allocated for conductor list: {jm.get('conductors', 0)} bytes\n"
-                 f"Memory allocated for Gmres: {res.mem.get('gmres', 0)} bytes\n"
+                 f"Memory allocated for Gmres: {mem.get('gmres', 0)} bytes\n"
                  f"Memory allocated for preconditioner: {jm.get('precond', 0)} bytes\n"
-                 f"Memory allocated for charge vectors: {res.mem.get('vectors', 0)} bytes\n"
-                 f"Memory allocated for hierarchical multiplication: {res.mem.get('product', 0)} bytes\n")
+                 f"Memory allocated for charge vectors: {mem.get('vectors', 0)} bytes\n"
+                 f"Memory allocated for hierarchical multiplication: {mem.get('product', 0)} bytes\n")
     return ("\nSolve statistics:\n"
             f"Number of input panels: {st.input_raw_cond_panels + st.input_raw_diel_panels} of which "
             f"{st.input_raw_cond_panels} conductors and {st.input_raw_diel_panels} dielectric\n"
