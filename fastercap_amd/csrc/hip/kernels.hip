@@ -184,14 +184,68 @@ constexpr int fill_per() {
 }
 
 template <int DIM, bool DIEL, int BR, bool MIXED, bool GAL>
+__device__ __forceinline__ void fill_one(const BJob& J, int64_t l, const fcg_node_geom& p1, const SrcGeom& p2) {
+    if constexpr (MIXED) {
+        // Galerkin: every branch integrated in FP64 (double quadratures of
+        // nearby panels), stored in FP32
+        if constexpr (BR == kBrAnalytic || GAL)
+            J.coef32[l] = (float)link_coef<double, DIM, DIEL, BR, GAL>(p1, p2);
+        else
+            J.coef32[l] = link_coef<float, DIM, DIEL, BR>(p1, p2);
+    } else {
+        J.coef64[l] = link_coef<double, DIM, DIEL, BR, GAL>(p1, p2);
+    }
+}
+
+template <int DIM, bool DIEL, int BR, bool MIXED, bool GAL>
 __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __restrict__ off, int64_t total, int cls) {
     constexpr int P = fill_per<BR>();
     const int64_t g0 = (int64_t)blockIdx.x * blockDim.x * P;
     const int64_t gf = g0 + threadIdx.x;
     int jb = find_job_block(off, bv.njobs, g0, gf < total ? gf : total - 1);
     int64_t l[P];
-    int jl[P];
     int row[P], src[P];
+    // a block whose entries all lie in one job (the common case) reads the job's
+    // record once, with scalar loads, and no entry searches for its job
+    const int ju = __builtin_amdgcn_readfirstlane(jb);
+    const int64_t glast = min(g0 + (int64_t)blockDim.x * P, total) - 1;
+    if (ju + 1 >= bv.njobs || off[ju + 1] > glast) {
+        const BJob& J = bv.jobs[ju];
+        const int64_t fb = J.fill_start[cls] - off[ju];
+#pragma unroll
+        for (int i = 0; i < P; ++i) {
+            const int64_t g = gf + (int64_t)i * blockDim.x;
+            l[i] = -1;
+            if (g < total) {
+                l[i] = J.fill_order[fb + g];
+                const int2 rs = J.fill_rs[fb + g];  // the source's charge row indexes src_geom
+                row[i] = rs.x;
+                src[i] = rs.y;
+            }
+        }
+        if constexpr (MIXED && !GAL && BR != kBrAnalytic) {
+            // FP32 quadrature and point classes: every entry's geometry in
+            // registers before the first store (a store may alias the records as
+            // far as the compiler knows); the FP64 bodies keep one entry at a time
+            fcg_node_geom a1[P];
+            SrcGeom a2[P];
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+                if (l[i] >= 0) {
+                    a1[i] = J.nodes[row[i]];
+                    a2[i] = J.src_geom[src[i]];
+                }
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+                if (l[i] >= 0) fill_one<DIM, DIEL, BR, MIXED, GAL>(J, l[i], a1[i], a2[i]);
+        } else {
+#pragma unroll
+            for (int i = 0; i < P; ++i)
+                if (l[i] >= 0) fill_one<DIM, DIEL, BR, MIXED, GAL>(J, l[i], J.nodes[row[i]], J.src_geom[src[i]]);
+        }
+        return;
+    }
+    int jl[P];
 #pragma unroll
     for (int i = 0; i < P; ++i) {
         const int64_t g = gf + (int64_t)i * blockDim.x;
@@ -209,22 +263,11 @@ __global__ void __launch_bounds__(256) k_fill(BatchView bv, const int64_t* __res
         }
     }
 #pragma unroll
-    for (int i = 0; i < P; ++i) {
-        if (l[i] < 0) continue;
-        const BJob& J = bv.jobs[jl[i]];
-        const fcg_node_geom& p1 = J.nodes[row[i]];
-        const SrcGeom& p2 = J.src_geom[src[i]];
-        if constexpr (MIXED) {
-            // Galerkin: every branch integrated in FP64 (double quadratures of
-            // nearby panels), stored in FP32
-            if constexpr (BR == kBrAnalytic || GAL)
-                J.coef32[l[i]] = (float)link_coef<double, DIM, DIEL, BR, GAL>(p1, p2);
-            else
-                J.coef32[l[i]] = link_coef<float, DIM, DIEL, BR>(p1, p2);
-        } else {
-            J.coef64[l[i]] = link_coef<double, DIM, DIEL, BR, GAL>(p1, p2);
+    for (int i = 0; i < P; ++i)
+        if (l[i] >= 0) {
+            const BJob& J = bv.jobs[jl[i]];
+            fill_one<DIM, DIEL, BR, MIXED, GAL>(J, l[i], J.nodes[row[i]], J.src_geom[src[i]]);
         }
-    }
 }
 
 // Mixed precision, analytic branch: the coefficient integrated in FP64 once,
@@ -349,7 +392,8 @@ __global__ void k_self(BatchView bv) {
 // and the top nodes (subtrees across chunk boundaries) by one block per job.
 // Sums are accumulated in FP64; charges and potentials are stored as TQ, one
 // kp-row per node at row perm[node] (padding columns zero).
-constexpr int kTreeThreads = 1024;
+constexpr int kTreeThreads = 512;      // per chunk (inner kernels): three workgroups fit a CU's LDS
+constexpr int kTreeTopThreads = 1024;  // per job (top kernels)
 
 __device__ __forceinline__ double scan_input(const BJob& J, int r, int n, int jacobi) {
     const size_t i = (size_t)r * J.N + n;
@@ -413,7 +457,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_up_inner(BatchView bv, int jac
 // the top children in LDS; every top node's row is written at the end.  Jobs
 // with more than kTreeTopLds top nodes take the same steps in global memory.
 template <typename TQ>
-__global__ void __launch_bounds__(kTreeThreads) k_up_top(BatchView bv) {
+__global__ void __launch_bounds__(kTreeTopThreads) k_up_top(BatchView bv) {
     __shared__ double tacc[kTreeTopLds * kTreeRhs];
     const BJob& J = bv.jobs[blockIdx.x];
     const int k = J.k, kp = J.kp, nt = J.ntop;
@@ -421,7 +465,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_up_top(BatchView bv) {
     TQ* ch = (sizeof(TQ) == 8) ? reinterpret_cast<TQ*>(J.charge64) : reinterpret_cast<TQ*>(J.charge32);
     if (nt > kTreeTopLds) {
         for (int lv = J.tt_nlev - 1; lv >= 0; --lv) {
-            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeTopThreads) {
                 const int n = J.tt_node[i];
                 const int lc = J.left[n], rc = J.right[n];
                 TQ* o = ch + (size_t)J.perm[n] * kp;
@@ -440,7 +484,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_up_top(BatchView bv) {
     }
     for (int rg = 0; rg < kp; rg += kTreeRhs) {
         const int nr = min(kTreeRhs, kp - rg);
-        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {  // inner children
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {  // inner children
             const int4 rc = J.tt_rec[i];
             const int lc = J.left[rc.x], rr = J.right[rc.x];
             for (int r = 0; r < nr; ++r) {
@@ -454,7 +498,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_up_top(BatchView bv) {
         }
         __syncthreads();
         for (int lv = J.tt_nlev - 1; lv >= 0; --lv) {
-            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeTopThreads) {
                 const int4 rc = J.tt_rec[i];
                 for (int r = 0; r < nr; ++r) {
                     double c = tacc[i * kTreeRhs + r];
@@ -465,7 +509,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_up_top(BatchView bv) {
             }
             __syncthreads();
         }
-        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {
             TQ* o = ch + (size_t)J.perm[J.tt_node[i]] * kp + rg;
             for (int r = 0; r < nr; ++r) o[r] = (TQ)tacc[i * kTreeRhs + r];
         }
@@ -981,7 +1025,7 @@ static hipError_t gather_dispatch(const BatchView& bv, const GatherSeg& sg, hipS
 // (shallowest first) add the parent's sum in LDS, and the rows are written back;
 // jobs with more than kTreeTopLds top nodes take the same steps in global memory.
 template <typename TQ>
-__global__ void __launch_bounds__(kTreeThreads) k_down_top(BatchView bv) {
+__global__ void __launch_bounds__(kTreeTopThreads) k_down_top(BatchView bv) {
     __shared__ double tacc[kTreeTopLds * kTreeRhs];
     const BJob& J = bv.jobs[blockIdx.x];
     const int k = J.k, kp = J.kp, nt = J.ntop;
@@ -995,7 +1039,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_down_top(BatchView bv) {
         // charge, the group roots' charge rows of this product
         const TQ* ch = (sizeof(TQ) == 8) ? reinterpret_cast<const TQ*>(J.charge64)
                                          : reinterpret_cast<const TQ*>(J.charge32);
-        for (int r = (int)threadIdx.x; r < k; r += kTreeThreads) {
+        for (int r = (int)threadIdx.x; r < k; r += kTreeTopThreads) {
             double v = 0.0;
             int a = 0;
             for (; a < J.lpath_top; ++a) {
@@ -1014,7 +1058,7 @@ __global__ void __launch_bounds__(kTreeThreads) k_down_top(BatchView bv) {
     if (nt == 0) return;
     if (nt > kTreeTopLds) {
         for (int lv = 0; lv < J.tt_nlev; ++lv) {
-            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeTopThreads) {
                 const int n = J.tt_node[i];
                 const int p = J.parent[n];
                 if (p < 0) continue;
@@ -1028,20 +1072,20 @@ __global__ void __launch_bounds__(kTreeThreads) k_down_top(BatchView bv) {
     }
     for (int rg = 0; rg < k; rg += kTreeRhs) {
         const int nr = min(kTreeRhs, k - rg);
-        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {
             const TQ* o = pot + (size_t)J.perm[J.tt_node[i]] * kp + rg;
             for (int r = 0; r < nr; ++r) tacc[i * kTreeRhs + r] = (double)o[r];
         }
         __syncthreads();
         for (int lv = 0; lv < J.tt_nlev; ++lv) {
-            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeThreads) {
+            for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeTopThreads) {
                 const int ps = J.tt_rec[i].y;
                 if (ps < 0) continue;
                 for (int r = 0; r < nr; ++r) tacc[i * kTreeRhs + r] += tacc[ps * kTreeRhs + r];
             }
             __syncthreads();
         }
-        for (int i = (int)threadIdx.x; i < nt; i += kTreeThreads) {
+        for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {
             TQ* o = pot + (size_t)J.perm[J.tt_node[i]] * kp + rg;
             for (int r = 0; r < nr; ++r) o[r] = (TQ)tacc[i * kTreeRhs + r];
         }
@@ -1259,9 +1303,9 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
         k_up_inner<double><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b, jacobi);
     if (ev) (void)hipEventRecord(ev[1], s);
     if (prec == kMvp32)
-        k_up_top<float><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
+        k_up_top<float><<<(unsigned)b.njobs, kTreeTopThreads, 0, s>>>(b);
     else
-        k_up_top<double><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
+        k_up_top<double><<<(unsigned)b.njobs, kTreeTopThreads, 0, s>>>(b);
     if (ev) (void)hipEventRecord(ev[2], s);
     for (int i = 0; i < plan.n; ++i) {
         hipError_t ge;
@@ -1281,10 +1325,10 @@ hipError_t launch_mvp(const BatchView& b, int prec, int jacobi, const GatherPlan
     if (prec == kMvp32Acc) k_near_correct<8><<<blocks_for((size_t)b.nodes, 256), 256, 0, s>>>(b);
     if (ev) (void)hipEventRecord(ev[3], s);
     if (prec == kMvp32) {
-        k_down_top<float><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
+        k_down_top<float><<<(unsigned)b.njobs, kTreeTopThreads, 0, s>>>(b);
         k_down_inner<float><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b, jacobi);
     } else {
-        k_down_top<double><<<(unsigned)b.njobs, kTreeThreads, 0, s>>>(b);
+        k_down_top<double><<<(unsigned)b.njobs, kTreeTopThreads, 0, s>>>(b);
         k_down_inner<double><<<(unsigned)b.tchunks, kTreeThreads, 0, s>>>(b, jacobi);
     }
     if (ev) (void)hipEventRecord(ev[4], s);
