@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -2589,12 +2590,21 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
         return fail(ctx, FCG_INVALID_ARGUMENT, "unknown precision mode");
     HIPCHECK(ctx, hipSetDevice(ctx->device));
     hipStream_t s = ctx->stream;
-    std::vector<BatchEntry> ents;
+    std::vector<Job*> js;
     for (int i = 0; i < njobs; ++i) {
         Job* j = get_job(ctx, jobs[i]);
         if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
-        for (const auto& e : ents)
-            if (e.j == j) return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
+        js.push_back(j);
+    }
+    {
+        // a job listed more than once (a sorted copy: batches hold thousands of jobs)
+        std::vector<Job*> sorted(js);
+        std::sort(sorted.begin(), sorted.end());
+        if (std::adjacent_find(sorted.begin(), sorted.end()) != sorted.end())
+            return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
+    }
+    std::vector<BatchEntry> ents;
+    for (Job* j : js) {
         if (precision == FCG_PREC_MIXED && !j->coef32.p)
             HIPCHECK(ctx, j->coef32.alloc(sizeof(float) * (size_t)(j->L + kLinkSpare)));
         if (precision == FCG_PREC_MIXED) RCCHECK(ensure_near(ctx, *j));
@@ -2605,6 +2615,13 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
     }
     HIPCHECK(ctx, hipStreamSynchronize(s));
     ctx->tabs.reset();
+    // the tables of every fill go up in one copy per slab, ahead of the launches
+    ctx->tabs.defer = true;
+    struct Undefer {
+        Tables& t;
+        ~Undefer() { t.defer = false; }
+    } undefer{ctx->tabs};
+    std::vector<std::function<hipError_t()>> q;
     // one fill per scheme present (the Galerkin kernels are instantiations of their own)
     for (const int scheme : {FCG_SCHEME_COLLOCATION, FCG_SCHEME_GALERKIN}) {
         std::vector<BatchEntry> part;
@@ -2615,8 +2632,8 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
         std::vector<int> order;
         RCCHECK(make_batch(ctx, ctx->tabs, part, &bv, nullptr, &order));
         // per class, the prefix of the jobs' link counts in the batch's job order
-        const int64_t* offp[kFillClasses];
-        int64_t total[kFillClasses];
+        std::array<const int64_t*, kFillClasses> offp;
+        std::array<int64_t, kFillClasses> total;
         for (int c = 0; c < kFillClasses; ++c) {
             std::vector<int64_t> off(order.size() + 1, 0);
             for (size_t i = 0; i < order.size(); ++i)
@@ -2637,11 +2654,13 @@ int fcg_fill_batch(fcg_ctx* ctx, const int* jobs, int njobs, int precision) {
             HIPCHECK(ctx, ctx->tabs.push(off.data(), sizeof(int64_t) * off.size(), s, &d));
             near_off = reinterpret_cast<const int64_t*>(d);
         }
-        // no wait: later work on the context's stream runs after the fill, and every
-        // reuse of the context's tables waits for the stream first
-        HIPCHECK(ctx, fcg::launch_fill(bv, offp, total, precision, near_off, near_total,
-                                       scheme == FCG_SCHEME_GALERKIN, s));
+        const bool gal = scheme == FCG_SCHEME_GALERKIN;
+        q.push_back([=] { return fcg::launch_fill(bv, offp.data(), total.data(), precision, near_off, near_total, gal, s); });
     }
+    // no wait: later work on the context's stream runs after the fill, and every
+    // reuse of the context's tables waits for the stream first
+    HIPCHECK(ctx, ctx->tabs.flush(s));
+    for (auto& f : q) HIPCHECK(ctx, f());
     for (auto& e : ents) {
         if (precision == FCG_PREC_MIXED)
             e.j->filled32 = true;
@@ -2869,8 +2888,7 @@ int time_batch(fcg_ctx* ctx, const int* jobs, int njobs, int mvp_prec, int reps,
     for (int i = 0; i < njobs; ++i) {
         Job* j = get_job(ctx, jobs[i]);
         if (!j) return fail(ctx, FCG_INVALID_ARGUMENT, "bad job id");
-        for (Job* o : js)
-            if (o == j) return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
+        if (std::find(js.begin(), js.end(), j) != js.end()) return fail(ctx, FCG_INVALID_ARGUMENT, "job listed twice");
         if (mvp_prec == FCG_MVP_FP64 ? !j->filled64 : !j->filled32)
             return fail(ctx, FCG_INVALID_ARGUMENT, "job not filled in this precision");
         const int k = std::max(1, (int)j->rhs.size());

@@ -1538,11 +1538,22 @@ __global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, const double* _
     const KJob kj = kv.kjobs[sg.job];
     const int j = kj.nv, k = kj.k, r = sg.r, lane = threadIdx.x;
     {
+        // the partials in (chunk, wavefront) order, eight loads in flight
         const int P = 2 * kv.maxnv + 3;
+        const size_t base = (size_t)sg.c_lo * kDcgsWaves;
+        const int n = (sg.c_hi - sg.c_lo) * kDcgsWaves;
         for (int e = lane; e < 2 * j + 3; e += 64) {
+            const double* pp = partial + e;
             double s = 0.0;
-            for (int c = sg.c_lo; c < sg.c_hi; ++c)
-                for (int q = 0; q < kDcgsWaves; ++q) s += partial[((size_t)c * kDcgsWaves + q) * P + e];
+            int t = 0;
+            for (; t + 8 <= n; t += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = pp[(base + t + u) * P];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += v[u];
+            }
+            for (; t < n; ++t) s += pp[(base + t) * P];
             hbuf[(size_t)kj.hoff + (size_t)e * k + r] = s;
         }
         __syncthreads();
