@@ -229,7 +229,7 @@ struct Job {
         src_geom, fill_rs;
     // tree passes of the product (kernels.h BJob): inner nodes per chunk and
     // level, top nodes per level, top flags, group roots
-    DevBuf tb_rec, tb_ppm, tb_lev, tb_lvoff, tt_node, tt_rec, tt_lev, ttop, troots, lpath;
+    DevBuf tb_rec, tb_ppm, tb_lev, tb_lvoff, tt_node, tt_rec, tt_rows, tt_lev, ttop, troots, lpath;
     int tt_nlev = 0, ntop = 0, nroots = 0, lpath_len = 0, lpath_top = 0;
     // host copy of the row pointers and, for every stored link, its offset within
     // its row in the descriptor's order (the device keeps each row sorted by source)
@@ -286,7 +286,7 @@ struct Job {
                          perm.bytes + iperm.bytes + rorder.bytes + fill_order.bytes + fill_start.bytes + rhs_range.bytes +
                          col16.bytes + grow.bytes + src_geom.bytes + fill_rs.bytes + tb_rec.bytes +
                          tb_ppm.bytes + tb_lev.bytes + lpath.bytes +
-                         tb_lvoff.bytes + tt_node.bytes + tt_rec.bytes + tt_lev.bytes + ttop.bytes + troots.bytes +
+                         tb_lvoff.bytes + tt_node.bytes + tt_rec.bytes + tt_rows.bytes + tt_lev.bytes + ttop.bytes + troots.bytes +
                          bpre.bstart.bytes + bpre.boff.bytes + bpre.P.bytes + spre.snode.bytes +
                          spre.sstart.bytes + spre.sidx.bytes + spre.Sinv.bytes + spre.w.bytes +
                          near_ptr.bytes + near_col.bytes + near_link.bytes + near_dc.bytes + gout_im.bytes +
@@ -320,6 +320,7 @@ struct Job {
         b.tb_lvoff = tb_lvoff.as<int32_t>();
         b.tt_node = tt_node.as<int32_t>();
         b.tt_rec = tt_rec.as<int4>();
+        b.tt_rows = tt_rows.as<int4>();
         b.tt_lev = tt_lev.as<int32_t>();
         b.ttop = ttop.as<uint8_t>();
         b.troots = troots.as<int32_t>();
@@ -832,12 +833,14 @@ int upload_tree_passes(fcg_ctx* ctx, Job& j, const fcg_job_desc* d, const std::v
     // per top node: its slot's parent and top children slots (-1: not a top node)
     std::vector<int32_t> slot((size_t)M, -1);
     for (size_t i = 0; i < tt_node.size(); ++i) slot[(size_t)tt_node[i]] = (int32_t)i;
-    std::vector<int4> rec(std::max<size_t>(tt_node.size(), 1));
+    std::vector<int4> rec(std::max<size_t>(tt_node.size(), 1)), rows(rec.size(), make_int4(0, -1, -1, -1));
     for (size_t i = 0; i < tt_node.size(); ++i) {
         const int n = tt_node[i];
         const int p = d->parent[n], lc = d->left[n], rc = d->right[n];
         rec[i] = make_int4(n, p >= 0 ? slot[(size_t)p] : -1, lc >= 0 ? slot[(size_t)lc] : -1,
                            rc >= 0 ? slot[(size_t)rc] : -1);
+        rows[i] = make_int4(perm[(size_t)n], lc >= 0 ? perm[(size_t)lc] : -1, rc >= 0 ? perm[(size_t)rc] : -1,
+                            p >= 0 ? perm[(size_t)p] : -1);
     }
     // records of the inner nodes in list order (kernels.h BJob tb_rec / tb_ppm)
     std::vector<int4> tbr(std::max<size_t>(tb_node.size(), 1));
@@ -868,6 +871,7 @@ int upload_tree_passes(fcg_ctx* ctx, Job& j, const fcg_job_desc* d, const std::v
     HIPCHECK(ctx, upload(j.tb_lvoff, tb_lvoff.data(), tb_lvoff.size(), s));
     HIPCHECK(ctx, upload(j.tt_node, tt_node.data(), tt_node.size(), s));
     HIPCHECK(ctx, upload(j.tt_rec, rec.data(), rec.size(), s));
+    HIPCHECK(ctx, upload(j.tt_rows, rows.data(), rows.size(), s));
     HIPCHECK(ctx, upload(j.tt_lev, tt_lev.data(), tt_lev.size(), s));
     HIPCHECK(ctx, upload(j.ttop, top.data(), top.size(), s));
     HIPCHECK(ctx, upload(j.troots, roots.data(), roots.size(), s));
@@ -3303,7 +3307,7 @@ int fcg_job_memory(fcg_ctx* ctx, int job, fcg_job_mem* out) {
                                j->leaf_hi.bytes + j->leaf_node.bytes + j->gout.bytes + j->gin.bytes + j->gout_im.bytes +
                                j->gin_im.bytes + j->perm.bytes + j->iperm.bytes + j->rorder.bytes + j->tb_rec.bytes +
                                j->tb_ppm.bytes + j->tb_lev.bytes + j->tb_lvoff.bytes + j->tt_node.bytes +
-                               j->tt_rec.bytes + j->tt_lev.bytes + j->ttop.bytes + j->troots.bytes + j->lpath.bytes +
+                               j->tt_rec.bytes + j->tt_rows.bytes + j->tt_lev.bytes + j->ttop.bytes + j->troots.bytes + j->lpath.bytes +
                                j->self.bytes + j->self_im.bytes + j->diag.bytes + j->src_geom.bytes);
     out->links = (int64_t)(j->row_ptr.bytes + j->col.bytes + j->col16.bytes + j->link_row.bytes + j->coef64.bytes +
                            j->coef32.bytes + j->fill_order.bytes + j->fill_start.bytes + j->fill_rs.bytes +

@@ -82,6 +82,8 @@ struct BJob {
     const int32_t* tt_node;
     const int4* tt_rec;        // per top node: (node, slot of its parent, of its left child, of its right
                                // child), -1 where that node is not a top node (slots index tt_node)
+    const int4* tt_rows;       // per top node: the charge rows (perm) of itself, its left child, its
+                               // right child and its parent, -1 for none
     const int32_t* tt_lev;     // [tt_nlev + 1]
     const uint8_t* ttop;       // [M]
     const int32_t* troots;     // [nroots]

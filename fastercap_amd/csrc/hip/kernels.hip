@@ -466,14 +466,13 @@ __global__ void __launch_bounds__(kTreeTopThreads) k_up_top(BatchView bv) {
     if (nt > kTreeTopLds) {
         for (int lv = J.tt_nlev - 1; lv >= 0; --lv) {
             for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeTopThreads) {
-                const int n = J.tt_node[i];
-                const int lc = J.left[n], rc = J.right[n];
-                TQ* o = ch + (size_t)J.perm[n] * kp;
+                const int4 rw = J.tt_rows[i];  // rows of the node and its children
+                TQ* o = ch + (size_t)rw.x * kp;
                 for (int r = 0; r < kp; ++r) {
                     double c = 0.0;
                     if (r < k) {
-                        if (lc >= 0) c += (double)ch[(size_t)J.perm[lc] * kp + r];
-                        if (rc >= 0) c += (double)ch[(size_t)J.perm[rc] * kp + r];
+                        if (rw.y >= 0) c += (double)ch[(size_t)rw.y * kp + r];
+                        if (rw.z >= 0) c += (double)ch[(size_t)rw.z * kp + r];
                     }
                     o[r] = (TQ)c;
                 }
@@ -486,12 +485,12 @@ __global__ void __launch_bounds__(kTreeTopThreads) k_up_top(BatchView bv) {
         const int nr = min(kTreeRhs, kp - rg);
         for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {  // inner children
             const int4 rc = J.tt_rec[i];
-            const int lc = J.left[rc.x], rr = J.right[rc.x];
+            const int4 rw = J.tt_rows[i];
             for (int r = 0; r < nr; ++r) {
                 double c = 0.0;
                 if (rg + r < k) {
-                    if (lc >= 0 && rc.z < 0) c += (double)ch[(size_t)J.perm[lc] * kp + rg + r];
-                    if (rr >= 0 && rc.w < 0) c += (double)ch[(size_t)J.perm[rr] * kp + rg + r];
+                    if (rw.y >= 0 && rc.z < 0) c += (double)ch[(size_t)rw.y * kp + rg + r];
+                    if (rw.z >= 0 && rc.w < 0) c += (double)ch[(size_t)rw.z * kp + rg + r];
                 }
                 tacc[i * kTreeRhs + r] = c;
             }
@@ -510,7 +509,7 @@ __global__ void __launch_bounds__(kTreeTopThreads) k_up_top(BatchView bv) {
             __syncthreads();
         }
         for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {
-            TQ* o = ch + (size_t)J.perm[J.tt_node[i]] * kp + rg;
+            TQ* o = ch + (size_t)J.tt_rows[i].x * kp + rg;
             for (int r = 0; r < nr; ++r) o[r] = (TQ)tacc[i * kTreeRhs + r];
         }
         __syncthreads();
@@ -1059,11 +1058,10 @@ __global__ void __launch_bounds__(kTreeTopThreads) k_down_top(BatchView bv) {
     if (nt > kTreeTopLds) {
         for (int lv = 0; lv < J.tt_nlev; ++lv) {
             for (int i = J.tt_lev[lv] + (int)threadIdx.x; i < J.tt_lev[lv + 1]; i += kTreeTopThreads) {
-                const int n = J.tt_node[i];
-                const int p = J.parent[n];
-                if (p < 0) continue;
-                TQ* o = pot + (size_t)J.perm[n] * kp;
-                const TQ* a = pot + (size_t)J.perm[p] * kp;
+                const int4 rw = J.tt_rows[i];  // rows of the node and its parent
+                if (rw.w < 0) continue;
+                TQ* o = pot + (size_t)rw.x * kp;
+                const TQ* a = pot + (size_t)rw.w * kp;
                 for (int r = 0; r < k; ++r) o[r] = (TQ)((double)o[r] + (double)a[r]);
             }
             __syncthreads();
@@ -1073,7 +1071,7 @@ __global__ void __launch_bounds__(kTreeTopThreads) k_down_top(BatchView bv) {
     for (int rg = 0; rg < k; rg += kTreeRhs) {
         const int nr = min(kTreeRhs, k - rg);
         for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {
-            const TQ* o = pot + (size_t)J.perm[J.tt_node[i]] * kp + rg;
+            const TQ* o = pot + (size_t)J.tt_rows[i].x * kp + rg;
             for (int r = 0; r < nr; ++r) tacc[i * kTreeRhs + r] = (double)o[r];
         }
         __syncthreads();
@@ -1086,7 +1084,7 @@ __global__ void __launch_bounds__(kTreeTopThreads) k_down_top(BatchView bv) {
             __syncthreads();
         }
         for (int i = (int)threadIdx.x; i < nt; i += kTreeTopThreads) {
-            TQ* o = pot + (size_t)J.perm[J.tt_node[i]] * kp + rg;
+            TQ* o = pot + (size_t)J.tt_rows[i].x * kp + rg;
             for (int r = 0; r < nr; ++r) o[r] = (TQ)tacc[i * kTreeRhs + r];
         }
         __syncthreads();
