@@ -1516,6 +1516,38 @@ __device__ void dcgs_rotate(double* __restrict__ D, int c) {
 }
 __device__ __forceinline__ double& dcgs_g(double* D, int c) { return c == 0 ? D[2] : D[dcgs_rec(c - 1) + 2 * c + 3]; }
 
+// LDS of k_dcgs_coef when it fits in 48 KB (else 0: the projection goes through
+// the step buffer and the rotations read the record in global memory): the
+// projection of one rhs (2 maxnv + 3 values), then for the rotations the (cs, sn)
+// pairs (2 maxnv + 2) and the two columns they rotate (2 maxnv + 3)
+__host__ __device__ inline size_t coef_lds_bytes(int maxnv) {
+    const size_t b = sizeof(double) * (6 * (size_t)maxnv + 8);
+    return b <= 49152 ? b : 0;
+}
+
+// dcgs_rotate with column c's entries in H (LDS) and the rotations 0..c-1 in rot
+// (LDS, cs and sn interleaved); the rotated rows and the new rotation go to the
+// record in D as dcgs_rotate writes them, the new rotation to rot too
+__device__ void dcgs_rotate_lds(double* __restrict__ D, int c, const double* H, double* rot) {
+    double* R = D + dcgs_rec(c) + c + 2;
+    double a = H[0];
+    for (int r = 0; r < c; ++r) {
+        const double cs = rot[2 * r], sn = rot[2 * r + 1];
+        const double b = H[r + 1];
+        R[r] = cs * a - sn * b;
+        a = cs * b + sn * a;
+    }
+    const double b = H[c + 1];
+    const double len = sqrt(a * a + b * b);
+    const double cs = len > 0.0 ? a / len : 1.0, sn = len > 0.0 ? -b / len : 0.0;
+    R[c] = cs * a - sn * b;
+    double* rc = D + dcgs_rec(c) + 2 * c + 3;
+    rc[0] = cs;
+    rc[1] = sn;
+    rot[2 * c] = cs;
+    rot[2 * c + 1] = sn;
+}
+
 // grid (nsegs), one wavefront per (job, rhs): the host part of a DCGS2 step, on
 // the device.  With s = Q^T v_j, z = Q^T w and the norms from the projection:
 // nu^2 = v.v - s.s gives the final q_j = (v_j - Q s) / nu.  Since v_j = nu q_j +
@@ -1532,6 +1564,10 @@ __device__ __forceinline__ double& dcgs_g(double* D, int c) { return c == 0 ? D[
 __global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, const double* __restrict__ partial,
                                                   double* __restrict__ hbuf, double* __restrict__ coef) {
     __shared__ double tj_sh;
+    // the segment's projection, in LDS when the launch gave room for 2 maxnv + 3
+    // values (else in hbuf at hoff)
+    extern __shared__ double hvl[];
+    const bool in_lds = coef_lds_bytes(kv.maxnv) > 0;
     const Seg sg = kv.segs[blockIdx.x];
     const KJob kj = kv.kjobs[sg.job];
     const int j = kj.nv, k = kj.k, r = sg.r, lane = threadIdx.x;
@@ -1552,12 +1588,15 @@ __global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, const double* _
                 for (int u = 0; u < 8; ++u) s += v[u];
             }
             for (; t < n; ++t) s += pp[(base + t) * P];
-            hbuf[(size_t)kj.hoff + (size_t)e * k + r] = s;
+            if (in_lds)
+                hvl[e] = s;
+            else
+                hbuf[(size_t)kj.hoff + (size_t)e * k + r] = s;
         }
         __syncthreads();
     }
     const double* hv = hbuf + kj.hoff;
-    auto at = [&](int e) { return hv[(size_t)e * k + r]; };
+    auto at = [&](int e) { return in_lds ? hvl[e] : hv[(size_t)e * k + r]; };
     double* cf = coef + kj.coff + (size_t)r * (2 * j + 3);
     double* out = hbuf + kj.ooff + 2 * r;
     double* D = kj.hd + (size_t)r * kj.hstride;
@@ -1601,6 +1640,7 @@ __global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, const double* _
     double tq = 0.0, tt = 0.0, hh = 0.0;
     for (int rr = lane; rr <= j; rr += 64) {
         double tr = 0.0;
+#pragma unroll 4
         for (int c = rr > 0 ? rr - 1 : 0; c < j; ++c) tr += D[dcgs_rec(c) + rr] * at(c);
         const double qw = rr < j ? at(j + rr) : (vw - sz) / nu;
         const double hr = (qw - tr) / nu;
@@ -1633,15 +1673,37 @@ __global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, const double* _
         cf[2 * j + 2] = bt > 0.0 ? cj / (nu * bt) : 0.0;
     }
     __syncthreads();
+    // the serial rotations of lane 0 read LDS: the stored rotations 0..j-2 and
+    // the columns j-1 and j, loaded by the whole wavefront first
+    double* rot_l = hvl + 2 * kv.maxnv + 3;
+    double* col_a = rot_l + 2 * kv.maxnv + 2;  // column j-1, rows 0..j
+    double* col_b = col_a + kv.maxnv + 1;  // column j, rows 0..j+1
+    if (in_lds) {
+        for (int q = lane; q < j - 1; q += 64) {
+            const double* rr = D + dcgs_rec(q) + 2 * q + 3;
+            rot_l[2 * q] = rr[0];
+            rot_l[2 * q + 1] = rr[1];
+        }
+        if (j >= 1)
+            for (int q = lane; q <= j; q += 64) col_a[q] = D[dcgs_rec(j - 1) + q];
+        for (int q = lane; q <= j + 1; q += 64) col_b[q] = D[dcgs_rec(j) + q];
+        __syncthreads();
+    }
     if (lane == 0) {
         if (j >= 1) {
-            dcgs_rotate(D, j - 1);
+            if (in_lds)
+                dcgs_rotate_lds(D, j - 1, col_a, rot_l);
+            else
+                dcgs_rotate(D, j - 1);
             const double* rc = D + dcgs_rec(j - 1) + 2 * (j - 1) + 3;
             const double gp = D[1];
             dcgs_g(D, j - 1) = rc[0] * gp;
             dcgs_g(D, j) = rc[1] * gp;
         }
-        dcgs_rotate(D, j);
+        if (in_lds)
+            dcgs_rotate_lds(D, j, col_b, rot_l);
+        else
+            dcgs_rotate(D, j);
         const double* rc = D + dcgs_rec(j) + 2 * j + 3;
         const double gp = dcgs_g(D, j);
         D[1] = gp;
@@ -1783,7 +1845,7 @@ hipError_t launch_dcgs_update(const KrylovView& kv, const double* coef, hipStrea
 }
 hipError_t launch_dcgs_coef(const KrylovView& kv, const double* partial, double* hbuf, double* coef, hipStream_t s) {
     if (kv.nsegs == 0) return hipSuccess;
-    k_dcgs_coef<<<kv.nsegs, 64, 0, s>>>(kv, partial, hbuf, coef);
+    k_dcgs_coef<<<kv.nsegs, 64, coef_lds_bytes(kv.maxnv), s>>>(kv, partial, hbuf, coef);
     return hipGetLastError();
 }
 hipError_t launch_dcgs_lsq(const KrylovView& kv, double* z, hipStream_t s) {
