@@ -1648,6 +1648,7 @@ int make_krylov(fcg_ctx* ctx, Tables& tabs, const std::vector<KEnt>& ents, Krylo
         J.last = J.flags + S.k;
         J.beta = reinterpret_cast<const double*>(bd) + ioff[e] / 2;
         maxnv = std::max(maxnv, ents[e].nv);
+        kv->maxnz = std::max(kv->maxnz, S.nz);
         for (int r = 0; r < S.k; ++r) {
             Seg sg;
             sg.job = (int)e;

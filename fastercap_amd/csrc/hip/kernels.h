@@ -198,6 +198,7 @@ struct KrylovView {
     const Seg* segs;
     const Chunk* chunks;
     int nsegs, nchunks, maxnv;
+    int maxnz;  // the most solution-update columns of a job (k_dcgs_lsq)
     int basis32;  // the basis vectors are FP32 (inner solves of the mixed path)
 };
 

@@ -1716,13 +1716,45 @@ __global__ void __launch_bounds__(64) k_dcgs_coef(KrylovView kv, const double* _
 
 // grid (nsegs), one wavefront per (job, rhs): back substitution of the rotated
 // Hessenberg system over columns 0..last (R y = g), y into z (0 beyond last)
+// LDS of k_dcgs_lsq when it fits in 48 KB (else 0: the substitution works on the
+// record and z in global memory): y (maxnz) and the rotated columns 0..maxnz-1
+// packed, column ii at ii (ii + 1) / 2
+__host__ __device__ inline size_t lsq_lds_bytes(int maxnz) {
+    const size_t m = (size_t)(maxnz > 0 ? maxnz : 1);
+    const size_t b = sizeof(double) * (m + m * (m + 1) / 2);
+    return b <= 49152 ? b : 0;
+}
+
 __global__ void __launch_bounds__(64) k_dcgs_lsq(KrylovView kv, double* __restrict__ z) {
+    extern __shared__ double lsq_l[];
     const Seg sg = kv.segs[blockIdx.x];
     const KJob kj = kv.kjobs[sg.job];
     const int k = kj.k, r = sg.r, lane = threadIdx.x;
     const int it = kj.last[r];
     double* D = kj.hd + (size_t)r * kj.hstride;
     double* y = z + kj.zoff + r;  // y[jj * k]
+    if (lsq_lds_bytes(kv.maxnz) > 0 && it < kv.maxnz) {
+        // y and the triangle in LDS (the whole wavefront loads them), then the
+        // same substitution in LDS
+        double* yl = lsq_l;
+        double* Rl = lsq_l + (kv.maxnz > 0 ? kv.maxnz : 1);
+        for (int jj = lane; jj <= it; jj += 64) yl[jj] = dcgs_g(D, jj);
+        for (int ii = 0; ii <= it; ++ii) {
+            const double* R = D + dcgs_rec(ii) + ii + 2;
+            for (int jj = lane; jj <= ii; jj += 64) Rl[ii * (ii + 1) / 2 + jj] = R[jj];
+        }
+        __syncthreads();
+        for (int ii = it; ii >= 0; --ii) {
+            const double* R = Rl + ii * (ii + 1) / 2;
+            const double yi = yl[ii] / R[ii];
+            __syncthreads();
+            if (lane == 0) yl[ii] = yi;
+            for (int jj = lane; jj < ii; jj += 64) yl[jj] -= R[jj] * yi;
+            __syncthreads();
+        }
+        for (int jj = lane; jj < kj.nz; jj += 64) y[(size_t)jj * k] = jj <= it ? yl[jj] : 0.0;
+        return;
+    }
     for (int jj = lane; jj < kj.nz; jj += 64) y[(size_t)jj * k] = jj <= it ? dcgs_g(D, jj) : 0.0;
     __syncthreads();
     for (int ii = it; ii >= 0; --ii) {
@@ -1850,7 +1882,7 @@ hipError_t launch_dcgs_coef(const KrylovView& kv, const double* partial, double*
 }
 hipError_t launch_dcgs_lsq(const KrylovView& kv, double* z, hipStream_t s) {
     if (kv.nsegs == 0) return hipSuccess;
-    k_dcgs_lsq<<<kv.nsegs, 64, 0, s>>>(kv, z);
+    k_dcgs_lsq<<<kv.nsegs, 64, lsq_lds_bytes(kv.maxnz), s>>>(kv, z);
     return hipGetLastError();
 }
 hipError_t launch_reduce_norm(const KrylovView& kv, const double* partial, double* hbuf, hipStream_t s) {
